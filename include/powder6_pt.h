@@ -1,0 +1,199 @@
+/*
+ * powder6_pt.h -- C ABI of the MI355X parallel-tempering trial evaluator.
+ *
+ * This is the drop-in boundary for the WebGPU backend `RVPTGPU = {compile, create, run}`
+ * of Powder 6 (js/pt_gpu.js:424).  A host (the Node N-API addon under powder6_amd/node,
+ * the Python binding powder6_amd/pt.py, or any C caller) flattens the PT context that
+ * RVCore.refineAsync yields (js/rietveld_core.js:2271-2273) into a `pt_problem`, creates
+ * a plan, and then drives it with the calls below.  Every call returns 0 on success and a
+ * negative status on failure; pt_last_error() then holds a message whose wording follows
+ * the reference's thrown errors (js/pt_gpu.js:119,121,373,380,402,415) so that callers can
+ * keep matching on /disagrees/, /unavailable/, /did not converge/ and the budget phrases.
+ *
+ * Ownership: every input array is read during pt_plan_create and copied; outputs are
+ * written into caller buffers; device memory belongs to the plan until pt_plan_destroy.
+ * Threading: plans are independent (one HIP stream each); a plan is not re-entrant.
+ */
+#ifndef POWDER6_PT_H
+#define POWDER6_PT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------------- */
+#define PT_OK 0
+#define PT_ERR_ARG -1         /* invalid argument or problem description          */
+#define PT_ERR_BUDGET -2      /* a setup budget is exceeded (pt_gpu.js:121,175,381)*/
+#define PT_ERR_DEVICE -3      /* no usable GPU, or a HIP call failed (pt_gpu.js:373,380) */
+#define PT_ERR_NUMERIC -4     /* a device result disagrees / is not finite (pt_gpu.js:415) */
+#define PT_ERR_STATE -5       /* call out of order (e.g. pt_batch before pt_set_states) */
+
+/* ---- enumerations ------------------------------------------------------------- */
+/* crystal systems, js/crystal.js:76-119 (rhombohedral uses hexagonal axes) */
+enum { PT_TRICLINIC = 0, PT_MONOCLINIC = 1, PT_ORTHORHOMBIC = 2, PT_TETRAGONAL = 3,
+       PT_TRIGONAL = 4, PT_HEXAGONAL = 5, PT_CUBIC = 6 };
+/* profile families, js/profile.js:603-718 (type 1 simple, 2 split, 3 TCH) */
+enum { PT_PROFILE_SIMPLE = 1, PT_PROFILE_SPLIT = 2, PT_PROFILE_TCH = 3 };
+enum { PT_XRAY = 0, PT_NEUTRON = 1 };
+
+/* ---- model-parameter layout ----------------------------------------------------
+ * The model parameters of one dataset, the names js/rietveld_core.js reads from its
+ * value map (`zero`, `disp`, `transp`, `bkg<k>`, `p1.a`, `p1.U`, `p1.C1.x`, ...), in a
+ * fixed order.  Dataset-level parameters come first; each phase then has a block of
+ * PT_PH_ATOM0 slots followed by PT_AT_STRIDE slots per atom.  pt_phase_base() gives
+ * the first index of a phase block.                                                  */
+enum { PT_P_ZERO = 0, PT_P_DISP = 1, PT_P_TRANSP = 2, PT_P_BKG0 = 3, PT_MAX_BKG = 24,
+       PT_P_PHASE0 = 3 + 24 };
+enum { PT_PH_A = 0, PT_PH_B = 1, PT_PH_C = 2, PT_PH_ALPHA = 3, PT_PH_BETA = 4, PT_PH_GAMMA = 5,
+       PT_PH_SCALE = 6, PT_PH_PO = 7, PT_PH_PROF0 = 8, PT_PH_ATOM0 = 24 };
+/* profile slots (relative to PT_PH_PROF0) -- js/rietveld_core.js:446-451 names */
+enum { PT_TCH_U = 0, PT_TCH_V, PT_TCH_W, PT_TCH_X, PT_TCH_Y, PT_TCH_SL, PT_TCH_HL };
+enum { PT_SIM_GU = 0, PT_SIM_GV, PT_SIM_GW, PT_SIM_GP, PT_SIM_LX, PT_SIM_ETA };
+enum { PT_SPL_GU_L = 0, PT_SPL_GV_L, PT_SPL_GW_L, PT_SPL_LX_L,
+       PT_SPL_GU_R, PT_SPL_GV_R, PT_SPL_GW_R, PT_SPL_LX_R, PT_SPL_ETA };
+/* atom slots: free-coordinate values, occupancy, isotropic B */
+enum { PT_AT_X = 0, PT_AT_Y = 1, PT_AT_Z = 2, PT_AT_OCC = 3, PT_AT_B = 4, PT_AT_STRIDE = 5 };
+
+#define PT_MAX_GAUSS 5
+
+/* One atom: its Wyckoff site r = r0 + sum_f u_f basis[f] over the free coordinates f
+ * (js/rietveld_core.js:305-378 siteSymmetry/sitePosition), multiplicity, and scattering
+ * factor f0(s) = c + sum_i a_i exp(-b_i (s/2)^2) (+f', f'') or a neutron length
+ * (js/rietveld_core.js:383-390).                                                      */
+typedef struct {
+    double r0[3];
+    double basis[3][3];   /* basis[f][i]: component i of the direction of free coordinate f */
+    int32_t free_mask;    /* bit f set: coordinate f (0=x,1=y,2=z) is free */
+    int32_t n_gauss;      /* 0..PT_MAX_GAUSS */
+    double mult;          /* site multiplicity, site.mult */
+    double ff_a[PT_MAX_GAUSS], ff_b[PT_MAX_GAUSS], ff_c;
+    double b_neutron;     /* neutron scattering length (PT_NEUTRON datasets) */
+    double fp, fpp;       /* anomalous f', f'' of the dataset for this atom */
+} pt_atom;
+
+/* One phase of the dataset (all Rietveld).  `op_r`/`op_t` hold EVERY operator of the
+ * setting, centring translations included (js/rietveld_core.js:139-153 expandOps):
+ * op_r[9*i + 3*row + col] integer rotation, op_t[3*i + row] translation.               */
+typedef struct {
+    int32_t system;        /* PT_CUBIC ... */
+    int32_t unique_axis;   /* monoclinic unique axis: 0=a, 1=b, 2=c */
+    int32_t profile_type;  /* PT_PROFILE_* */
+    int32_t n_ops;
+    const int32_t *op_r;
+    const double *op_t;
+    double po_dir[3];      /* March-Dollase direction, phase.poDir */
+    int32_t n_atoms;
+    const pt_atom *atoms;
+} pt_phase;
+
+/* The whole PT problem of one dataset.  Model parameters are an affine function of
+ * the refined variables x:  p[k] = param_base[k] + sum_{t: term_param[t]==k}
+ * term_coeff[t] * x[term_var[t]]  -- plain variables (coefficient 1, base 0) and the
+ * exact affine links of js/atomic_constraints.js:76-155 use the same form.              */
+typedef struct {
+    /* dataset: js/rietveld_core.js:622-627 (tth ascending, w = 0 excludes a point) */
+    int32_t n_points;
+    const double *tth, *y_obs, *weight;
+    double lambda1, lambda2, ratio, pol_k, min_fwhm;
+    int32_t radiation;     /* PT_XRAY / PT_NEUTRON */
+    int32_t n_bkg;         /* Chebyshev terms, 0..PT_MAX_BKG */
+    /* phases */
+    int32_t n_phases;
+    const pt_phase *phases;
+    /* model parameters */
+    int32_t n_params;
+    const double *param_base;
+    /* refined variables: start x0, box [lo, hi] (+-inf allowed), PT step scale */
+    int32_t n_vars;
+    const double *x0, *lo, *hi, *scale;
+    int32_t n_terms;
+    const int32_t *term_param, *term_var;
+    const double *term_coeff;
+    /* PT settings (js/rietveld_core.js:2147-2158); steps_per_sweep 0 means n_vars */
+    int32_t chains, sweeps, steps_per_sweep;
+    double max_temp;
+    uint32_t seed;
+} pt_problem;
+
+/* Plan summary: the numeric fields of pt_gpu.js compile() (js/pt_gpu.js:226).        */
+typedef struct {
+    int32_t nv, chains, stride, point_count, reflection_count, line_count;
+    int32_t steps_per_sweep, n_params;
+    int32_t uniform_grid;   /* 1 when tth is t0 + i*step to 1e-9 step */
+    int32_t device_ok;      /* 1 when a gfx950 device is attached to the plan */
+} pt_plan_info;
+
+/* State layout per chain, js/pt_gpu.js state buffer (stride 3*nv + 12):
+ *   [0,nv) current x, [nv,2nv) proposal, [2nv,3nv) best,
+ *   3nv+0 cost, +1 best cost, +2 step counter (-1 = bootstrap), +3 accepted,
+ *   +4 proposals, +5 swaps, +6 swap attempts, +7 trial valid, +8 log u,
+ *   +9 unsolved flag (always 0: no free intensities), +10 trial cost, +11 feasible.
+ * Snapshots are returned in double precision.                                         */
+enum { PT_S_COST = 0, PT_S_BEST = 1, PT_S_STEP = 2, PT_S_ACCEPTED = 3, PT_S_PROPOSALS = 4,
+       PT_S_SWAPS = 5, PT_S_ATTEMPTS = 6, PT_S_VALID = 7, PT_S_LOGU = 8, PT_S_UNSOLVED = 9,
+       PT_S_TRIAL = 10, PT_S_FEASIBLE = 11, PT_S_COUNT = 12 };
+
+/* Evaluation forms for pt_eval */
+enum { PT_FORM_PLAIN = 0,    /* one trial per launch sequence, whole-GPU kernels        */
+       PT_FORM_BATCHED = 1,  /* many trials per launch, one workgroup per trial (VALU) */
+       PT_FORM_MATRIX = 2 }; /* shared profile matrix x intensities on MFMA (atom-only trials) */
+
+typedef struct pt_plan pt_plan;
+
+/* Library version and the last error of the calling thread (js/pt_gpu.js error strings). */
+int pt_abi_version(void);
+const char *pt_last_error(void);
+
+/* First model-parameter index of phase `phase` (layout above). */
+int pt_phase_base(const pt_problem *prob, int phase);
+
+/* compile(ctx) + create(ctx): js/pt_gpu.js:20-227 and :371-406.  Enumerates the
+ * reflection envelope on the host, packs the problem, allocates the device buffers. */
+int pt_plan_create(const pt_problem *prob, pt_plan **out);
+void pt_plan_destroy(pt_plan *plan);                      /* close(), js/pt_gpu.js:376 */
+/* compile(ctx) alone (js/pt_gpu.js:20-227): the host part of pt_plan_create without a
+ * device -- envelope, counts and validation.  Only pt_plan_info_get,
+ * pt_plan_reflections and pt_plan_destroy accept such a plan. */
+int pt_plan_compile(const pt_problem *prob, pt_plan **out);
+int pt_plan_info_get(const pt_plan *plan, pt_plan_info *info);
+
+/* Reflection envelope of one phase as enumerated by the plan (h,k,l,m per row):
+ * the list js/pt_gpu.js:110-131 builds.  Returns the row count, or <0 on error; writes
+ * at most `cap` rows.                                                                */
+int pt_plan_reflections(const pt_plan *plan, int phase, int32_t *hklm, int cap);
+
+/* initial(vectors): js/pt_gpu.js:400.  x holds nvec vectors of nv doubles, cycled over
+ * the chains. */
+int pt_set_states(pt_plan *plan, const double *x, int nvec);
+
+/* costs(vectors): js/pt_gpu.js:401.  chi^2 of every chain at the vectors (cycled),
+ * written to chi2[chains]. */
+int pt_costs(pt_plan *plan, const double *x, int nvec, double *chi2);
+
+/* batch(count, bootstrap): js/pt_gpu.js:402.  `count` sweeps (or the bootstrap step),
+ * then a snapshot of chains*stride doubles into state_out (may be NULL). */
+int pt_batch(pt_plan *plan, int count, int bootstrap, double *state_out);
+
+/* Evaluate ntrials arbitrary variable vectors (ntrials*nv doubles) with one of the
+ * evaluation forms; chi2[ntrials] out.  PT_FORM_MATRIX requires that only structure
+ * (atom) and scale variables differ between the trials and the plan's start vector. */
+int pt_eval(pt_plan *plan, int form, const double *x, int ntrials, double *chi2);
+
+/* Calculated pattern (background + phases) of one vector, FP32 kernels, n_points out. */
+int pt_pattern(pt_plan *plan, const double *x, double *ycalc);
+
+/* Timing of the last pt_batch / pt_eval call on the plan's stream (HIP events):
+ * elapsed milliseconds of the whole call and of its dominant kernel, and the number of
+ * dominant-kernel launches. */
+int pt_last_timing(const pt_plan *plan, double *total_ms, double *kernel_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POWDER6_PT_H */

@@ -1,0 +1,1132 @@
+/*
+ * p6_oracle.c -- FP64 CPU restatement of the Powder 6 Rietveld trial cost and the CPU
+ * parallel-tempering loop.
+ *
+ * TEST INFRASTRUCTURE.  This file is the oracle: the parity checker for the HIP path and
+ * the CPU baseline that bench.py times.  Nothing in the product library uses it.
+ *
+ * Restated from the reference's JavaScript (file:line cited per function), keeping its
+ * floating-point operation order: compile with -ffp-contract=off.  Transcendentals are
+ * glibc's, where the reference's are V8's (fdlibm port); the two can differ in the last
+ * ulp, so cost() agrees with the reference to ~1e-12 relative, not bitwise.  Math.hypot
+ * is restated with V8's Kahan-compensated algorithm.
+ */
+#define _GNU_SOURCE
+#include "p6_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------------ */
+/*  constants (js/constants.js, js/crystal.js:38, js/rietveld_core.js:108)          */
+/* ------------------------------------------------------------------------------ */
+#define JS_PI 3.141592653589793
+#define JS_LN2 0.6931471805599453
+static double D2R, R2D, TWO_PI, PV_G0, PV_L0, PV_GAUSS_AREA, PV_LORENTZ_AREA, GCONV, LCONV;
+static const double WINDOW_G = 8.0, WINDOW_L = 25.0, WINDOW_MAX = 10.0; /* constants.js:59,70,84 */
+static const double MIN_FWHM_DEFAULT = 1e-3;                              /* constants.js:108 */
+static const double CG = 2.772588722239781;                               /* profile.js:606 */
+static const double GL8_X[4] = {0.1834346424956498, 0.5255324099163290, 0.7966664774136267, 0.9602898564975363};
+static const double GL8_W[4] = {0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
+
+static pthread_once_t const_once = PTHREAD_ONCE_INIT;
+static void init_constants(void) {
+    D2R = JS_PI / 180;                  /* crystal.js:38, rietveld_core.js:108 */
+    R2D = 180 / JS_PI;
+    TWO_PI = 2 * JS_PI;
+    PV_G0 = 2 * sqrt(JS_LN2 / JS_PI);   /* constants.js:155 */
+    PV_L0 = 2 / JS_PI;                  /* constants.js:162 */
+    PV_GAUSS_AREA = 1 / PV_G0;
+    PV_LORENTZ_AREA = 1 / PV_L0;
+    GCONV = sqrt(8 * JS_LN2) / 100.0;   /* constants.js:248 */
+    LCONV = 1.0 / 100.0;                /* constants.js:251 */
+}
+static void consts(void) { pthread_once(&const_once, init_constants); }
+
+static __thread char err_buf[256];
+const char *or_error(void) { return err_buf; }
+static void set_err(const char *m) { snprintf(err_buf, sizeof err_buf, "%s", m); }
+
+static int js_finite(double v) { return isfinite(v); }
+static double dmax(double a, double b) { return a > b ? a : b; }  /* Math.max for finite args */
+static double dmin(double a, double b) { return a < b ? a : b; }
+static double js_round(double v) { return floor(v + 0.5); }       /* Math.round */
+
+/* Math.hypot(x, y): V8 scales by the largest magnitude and sums squares with Kahan
+ * compensation, then returns sqrt(sum) * max. */
+double or_hypot2(double x, double y) {
+    double ax = fabs(x), ay = fabs(y);
+    if (isinf(ax) || isinf(ay)) return INFINITY;
+    if (isnan(ax) || isnan(ay)) return NAN;
+    double mx = ax > ay ? ax : ay;
+    if (mx == 0) return 0;
+    double v[2] = {ax, ay}, sum = 0, comp = 0;
+    for (int i = 0; i < 2; i++) {
+        double n = v[i] / mx;
+        double summand = n * n - comp;
+        double prelim = sum + summand;
+        comp = (prelim - sum) - summand;
+        sum = prelim;
+    }
+    return sqrt(sum) * mx;
+}
+
+/* constants.js:138-143 softPositive */
+double or_soft_positive(double x, double f) {
+    consts();
+    if (!js_finite(x)) return f;
+    double h = or_hypot2(x, 2 * f);
+    return x < 0 ? f * (f / (0.5 * h - 0.5 * x)) : 0.5 * x + 0.5 * h;
+}
+
+/* profile.js:182-189 softFloor */
+double or_soft_floor(double x, double f) {
+    if (!js_finite(x) || !(x > 0)) return f;
+    if (!(f > 0)) return x;
+    double r = x / f;
+    if (r > 20) return x;
+    return f * pow(pow(r, 4) + 1, 0.25);
+}
+
+/* profile.js:791-797 sharkoErfc (Numerical Recipes erfcc) */
+double or_erfc(double x) {
+    double z = fabs(x), t = 1 / (1 + 0.5 * z);
+    double ans = t * exp(-z * z - 1.26551223 + t * (1.00002368 + t * (0.37409196 +
+                 t * (0.09678418 + t * (-0.18628806 + t * (0.27886807 + t * (-1.13520398 +
+                 t * (1.48851587 + t * (-0.82215223 + t * 0.17087277)))))))));
+    return x >= 0 ? ans : 2 - ans;
+}
+
+/* crystal.js:465-471 sharkoLorentzFactor */
+double or_lorentz(double tth) {
+    consts();
+    if (!js_finite(tth) || tth <= 0 || tth >= 180) return NAN;
+    double th = 0.5 * tth * D2R, s = sin(th), c = cos(th);
+    if (s < 1e-6 || c < 1e-6) return NAN;
+    return 1 / (s * s * c);
+}
+/* crystal.js:474-478 sharkoPolarizationFactor */
+double or_polarization(double tth, double K) {
+    consts();
+    if (!js_finite(tth)) return NAN;
+    double k = (js_finite(K) && K >= 0) ? K : 1;
+    double c = cos(tth * D2R);
+    return (1 + k * (c * c)) / (1 + k);
+}
+/* crystal.js:487-497 sharkoLorentzPolarization / sharkoLp({K}) */
+double or_lp(double tth, double K) {
+    double L = or_lorentz(tth);
+    if (!js_finite(L)) return 1;
+    double P = or_polarization(tth, (js_finite(K) && K >= 0) ? K : 1);
+    return js_finite(P) ? L * P : 1;
+}
+
+/* profile.js:552-565 getPeakFWHM (Thompson, Cox & Hastings 1987) */
+double or_tch_fwhm(double gamma_g, double gamma_l) {
+    double gG = dmax(1e-9, (gamma_g != 0 && !isnan(gamma_g)) ? gamma_g : 1e-9);
+    double gL = dmax(1e-9, (gamma_l != 0 && !isnan(gamma_l)) ? gamma_l : 1e-9);
+    double g5 = pow(gG, 5), l5 = pow(gL, 5);
+    double g4l = 2.69269 * pow(gG, 4) * gL;
+    double g3l2 = 2.42843 * pow(gG, 3) * pow(gL, 2);
+    double g2l3 = 4.47163 * pow(gG, 2) * pow(gL, 3);
+    double gl4 = 0.07842 * gG * pow(gL, 4);
+    double s = g5 + g4l + g3l2 + g2l3 + gl4 + l5;
+    if (s < 0 || !js_finite(s)) return dmax(dmax(gG, gL), 1e-6);
+    return dmax(1e-6, pow(s, 0.2));
+}
+
+/* crystal.js:245-252 cellVolume */
+double or_cell_volume(const double c[6]) {
+    consts();
+    double ca = cos(c[3] * D2R), cb = cos(c[4] * D2R), cg = cos(c[5] * D2R);
+    double t = 1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg;
+    for (int i = 0; i < 3; i++) if (!(js_finite(c[i]) && c[i] > 0)) return NAN;
+    for (int i = 3; i < 6; i++) if (!(js_finite(c[i]) && c[i] > 0 && c[i] < 180)) return NAN;
+    if (!(t > 0)) return NAN;
+    return c[0] * c[1] * c[2] * sqrt(t);
+}
+
+/* crystal.js:326-366 buildInvDsq -> form {aa,bb,cc,ab,bc,ac}; returns 0 for a
+ * degenerate cell */
+int or_inv_dsq(const double cell[6], double f[6]) {
+    consts();
+    double a = cell[0], b = cell[1] > 0 ? cell[1] : a, c = cell[2] > 0 ? cell[2] : a;
+    if (!(a > 0) || !(b > 0) || !(c > 0)) return 0;
+    double al = cell[3] * D2R, be = cell[4] * D2R, ga = cell[5] * D2R;
+    double cosA = cos(al), cosB = cos(be), cosG = cos(ga);
+    double sinA = sin(al), sinB = sin(be), sinG = sin(ga);
+    double vf = 1 - cosA * cosA - cosB * cosB - cosG * cosG + 2 * cosA * cosB * cosG;
+    if (vf <= 1e-9 || fabs(sinA) < 1e-9 || fabs(sinB) < 1e-9 || fabs(sinG) < 1e-9) return 0;
+    double V = a * b * c * sqrt(vf);
+    double as = b * c * sinA / V, bs = a * c * sinB / V, cs = a * b * sinG / V;
+    double cA = (cosB * cosG - cosA) / (sinB * sinG);
+    double cB = (cosA * cosG - cosB) / (sinA * sinG);
+    double cG = (cosA * cosB - cosG) / (sinA * sinB);
+    f[0] = as * as; f[1] = bs * bs; f[2] = cs * cs;
+    f[3] = 2 * as * bs * cG; f[4] = 2 * bs * cs * cA; f[5] = 2 * as * cs * cB;
+    return 1;
+}
+/* crystal.js:374-377 invDsq */
+static double inv_dsq(const double f[6], double h, double k, double l) {
+    return h * h * f[0] + k * k * f[1] + l * l * f[2] + h * k * f[3] + k * l * f[4] + h * l * f[5];
+}
+
+/* profile.js:108-132 calculatePeakShift with profileType 'simple_pvoigt' (the sample
+ * displacement and transparency shift applied in Model.geometry and in
+ * buildPeakContributions' shifted()). */
+double or_peak_shift(double tth, double shft, double trns) {
+    consts();
+    double th = tth * (JS_PI / 180) / 2;
+    if (fabs(th - JS_PI / 2.0) < 1e-6) return 0;
+    double cosT = cos(th), sin2T = sin(2 * th);
+    double d = -(shft / 1000) * cosT * (180 / JS_PI);
+    double t = trns * sin2T * (180 / JS_PI);
+    double tot = d + t;
+    return js_finite(tot) ? tot : 0;
+}
+
+/* ------------------------------------------------------------------------------ */
+/*  symmetry and reflections (rietveld_core.js:155-276)                              */
+/* ------------------------------------------------------------------------------ */
+static int is_identity_r(const int32_t *r) {
+    return r[0] == 1 && r[4] == 1 && r[8] == 1 && !r[1] && !r[2] && !r[3] && !r[5] && !r[6] && !r[7];
+}
+
+/* rietveld_core.js:164-179 cosets */
+int or_cosets(int n_ops, const int32_t *op_r, const double *op_t, int32_t *rep_index, int *n_cent) {
+    int nC = 0;
+    double *C = (double *)malloc(sizeof(double) * 3 * (n_ops + 1));
+    for (int i = 0; i < n_ops; i++)
+        if (is_identity_r(op_r + 9 * i)) { memcpy(C + 3 * nC, op_t + 3 * i, 3 * sizeof(double)); nC++; }
+    if (!nC) { C[0] = C[1] = C[2] = 0; nC = 1; }
+    int nrep = 0;
+    for (int i = 0; i < n_ops; i++) {
+        int dup = 0;
+        for (int q = 0; q < nrep && !dup; q++) {
+            int j = rep_index[q];
+            if (memcmp(op_r + 9 * i, op_r + 9 * j, 9 * sizeof(int32_t)) != 0) continue;
+            for (int c = 0; c < nC && !dup; c++) {
+                int ok = 1;
+                for (int x = 0; x < 3; x++) {
+                    double d = op_t[3 * i + x] - op_t[3 * j + x] - C[3 * c + x];
+                    if (fabs(d - js_round(d)) > 1e-6) { ok = 0; break; }
+                }
+                if (ok) dup = 1;
+            }
+        }
+        if (!dup) rep_index[nrep++] = i;
+    }
+    free(C);
+    if (n_cent) *n_cent = nC;
+    return nrep;
+}
+
+/* rietveld_core.js:182-188 laueRotations: +R and -R of every op, first occurrence kept */
+static int laue_rotations(int n_ops, const int32_t *op_r, int32_t *rots) {
+    int n = 0;
+    for (int i = 0; i < n_ops; i++) {
+        for (int s = 0; s < 2; s++) {
+            int32_t r[9];
+            for (int x = 0; x < 9; x++) r[x] = s ? -op_r[9 * i + x] : op_r[9 * i + x];
+            int seen = 0;
+            for (int q = 0; q < n && !seen; q++) if (!memcmp(rots + 9 * q, r, sizeof r)) seen = 1;
+            if (!seen) { memcpy(rots + 9 * n, r, sizeof r); n++; }
+        }
+    }
+    return n;
+}
+
+/* rietveld_core.js:192-200 isAllowed */
+static int is_allowed(int h, int k, int l, int n_ops, const int32_t *op_r, const double *op_t) {
+    for (int i = 0; i < n_ops; i++) {
+        const int32_t *r = op_r + 9 * i;
+        const double *t = op_t + 3 * i;
+        if (h * r[0] + k * r[3] + l * r[6] != h || h * r[1] + k * r[4] + l * r[7] != k ||
+            h * r[2] + k * r[5] + l * r[8] != l) continue;
+        double p = h * t[0] + k * t[1] + l * t[2];
+        if (fabs(p - js_round(p)) > 1e-6) return 0;
+    }
+    return 1;
+}
+
+typedef struct { int h, k, l, m; double d0; int eq_off; } refl_t;
+
+static int refl_cmp(const void *pa, const void *pb) {
+    const refl_t *a = (const refl_t *)pa, *b = (const refl_t *)pb;
+    double d = b->d0 - a->d0;                      /* b.d0 - a.d0 || b.h - a.h || ... */
+    if (d != 0 && !isnan(d)) return d < 0 ? -1 : 1;
+    if (b->h != a->h) return b->h - a->h < 0 ? -1 : 1;
+    if (b->k != a->k) return b->k - a->k < 0 ? -1 : 1;
+    if (b->l != a->l) return b->l - a->l < 0 ? -1 : 1;
+    return 0;
+}
+
+typedef struct {
+    int n, cap;
+    refl_t *r;
+    int n_eq, cap_eq;
+    int32_t *eq;   /* 3 ints per equivalent */
+} refl_list;
+
+static void rl_free(refl_list *L) { free(L->r); free(L->eq); memset(L, 0, sizeof *L); }
+
+/* rietveld_core.js:247-276 generateReflections */
+static int gen_reflections(const double cell[6], int n_ops, const int32_t *op_r, const double *op_t,
+                           double dmin_, const int32_t *box, refl_list *out) {
+    double f[6];
+    out->n = 0; out->n_eq = 0;
+    if (!or_inv_dsq(cell, f) || !(dmin_ > 0)) return 0;
+    double lim = 1 / (dmin_ * dmin_) * (1 + 1e-9);
+    int32_t *rots = (int32_t *)malloc(sizeof(int32_t) * 18 * (n_ops > 0 ? n_ops : 1));
+    int nr = laue_rotations(n_ops, op_r, rots);
+    double cb = cell[1] > 0 ? cell[1] : cell[0], cc = cell[2] > 0 ? cell[2] : cell[0];
+    int hmax = box ? box[0] : (int)floor(cell[0] / dmin_);
+    int kmax = box ? box[1] : (int)floor(cb / dmin_);
+    int lmax = box ? box[2] : (int)floor(cc / dmin_);
+    int32_t *orbit = (int32_t *)malloc(sizeof(int32_t) * 3 * (nr + 1));
+    for (int h = -hmax; h <= hmax; h++)
+        for (int k = -kmax; k <= kmax; k++)
+            for (int l = -lmax; l <= lmax; l++) {
+                if (!h && !k && !l) continue;
+                double q = inv_dsq(f, h, k, l);
+                if (!(q > 0) || (!box && q > lim)) continue;
+                int ch = -1000000000, ck = 0, cl = 0, cp = -1, no = 0;
+                for (int i = 0; i < nr; i++) {
+                    const int32_t *r = rots + 9 * i;
+                    int v0 = h * r[0] + k * r[3] + l * r[6], v1 = h * r[1] + k * r[4] + l * r[7],
+                        v2 = h * r[2] + k * r[5] + l * r[8];
+                    int np = (v0 >= 0) + (v1 >= 0) + (v2 >= 0);
+                    int seen = 0;
+                    for (int o = 0; o < no && !seen; o++)
+                        if (orbit[3 * o] == v0 && orbit[3 * o + 1] == v1 && orbit[3 * o + 2] == v2) seen = 1;
+                    if (!seen) { orbit[3 * no] = v0; orbit[3 * no + 1] = v1; orbit[3 * no + 2] = v2; no++; }
+                    if (np > cp || (np == cp && (v0 > ch || (v0 == ch && (v1 > ck || (v1 == ck && v2 > cl)))))) {
+                        cp = np; ch = v0; ck = v1; cl = v2;
+                    }
+                }
+                if (ch != h || ck != k || cl != l) continue;
+                if (!is_allowed(h, k, l, n_ops, op_r, op_t)) continue;
+                if (out->n == out->cap) {
+                    out->cap = out->cap ? 2 * out->cap : 256;
+                    out->r = (refl_t *)realloc(out->r, sizeof(refl_t) * out->cap);
+                }
+                if (out->n_eq + no > out->cap_eq) {
+                    out->cap_eq = 2 * (out->n_eq + no) + 1024;
+                    out->eq = (int32_t *)realloc(out->eq, sizeof(int32_t) * 3 * out->cap_eq);
+                }
+                refl_t *R = &out->r[out->n++];
+                R->h = h; R->k = k; R->l = l; R->m = no; R->d0 = 1 / sqrt(q); R->eq_off = out->n_eq;
+                memcpy(out->eq + 3 * out->n_eq, orbit, sizeof(int32_t) * 3 * no);
+                out->n_eq += no;
+            }
+    free(orbit); free(rots);
+    qsort(out->r, out->n, sizeof(refl_t), refl_cmp);
+    return out->n;
+}
+
+int or_generate_reflections(const double cell[6], int n_ops, const int32_t *op_r, const double *op_t,
+                            double dmin_, const int32_t *box, int32_t *hklm, double *d0, int cap) {
+    refl_list L = {0};
+    int n = gen_reflections(cell, n_ops, op_r, op_t, dmin_, box, &L);
+    for (int i = 0; i < n && i < cap; i++) {
+        if (hklm) { hklm[4 * i] = L.r[i].h; hklm[4 * i + 1] = L.r[i].k; hklm[4 * i + 2] = L.r[i].l; hklm[4 * i + 3] = L.r[i].m; }
+        if (d0) d0[i] = L.r[i].d0;
+    }
+    rl_free(&L);
+    return n;
+}
+
+/* crystal.js:76-119 normaliseCell + rietveld_core.js:207-221 cellOf: cell of phase at p */
+static int cell_of(const pt_phase *ph, const double *pp, double cell[6]) {
+    double a = pp[PT_PH_A];
+    if (!(a > 0)) return 0;
+    double b = pp[PT_PH_B] > 0 ? pp[PT_PH_B] : a, c = pp[PT_PH_C] > 0 ? pp[PT_PH_C] : a;
+    cell[0] = a; cell[1] = b; cell[2] = c; cell[3] = cell[4] = cell[5] = 90;
+    switch (ph->system) {
+    case PT_CUBIC: cell[1] = cell[2] = a; break;
+    case PT_TETRAGONAL: cell[1] = a; break;
+    case PT_HEXAGONAL: case PT_TRIGONAL: cell[1] = a; cell[5] = 120; break;
+    case PT_ORTHORHOMBIC: break;
+    case PT_MONOCLINIC: {
+        int ax = ph->unique_axis;
+        double v = pp[PT_PH_ALPHA + (ax == 0 ? 0 : ax == 2 ? 2 : 1)];
+        cell[3 + (ax == 0 ? 0 : ax == 2 ? 2 : 1)] = js_finite(v) ? v : 90;
+        break;
+    }
+    default:
+        cell[3] = js_finite(pp[PT_PH_ALPHA]) ? pp[PT_PH_ALPHA] : 90;
+        cell[4] = js_finite(pp[PT_PH_BETA]) ? pp[PT_PH_BETA] : 90;
+        cell[5] = js_finite(pp[PT_PH_GAMMA]) ? pp[PT_PH_GAMMA] : 90;
+    }
+    return 1;
+}
+
+/* rietveld_core.js:238-245 reflectionDmin */
+double or_reflection_dmin(const pt_problem *prob, double zero, double disp, double transp) {
+    consts();
+    double lambda = prob->lambda1;
+    if (prob->ratio > 1e-6 && prob->lambda2 > 0) lambda = dmin(lambda, prob->lambda2);
+    double margin = WINDOW_MAX + fabs(zero) + (fabs(disp) * .001 + fabs(transp)) * R2D;
+    double upper = dmin(179.999999, prob->tth[prob->n_points - 1] + margin + .001);
+    return lambda / (2 * sin(upper * D2R / 2));
+}
+
+/* ------------------------------------------------------------------------------ */
+/*  profile (profile.js:234-759, constants.js:194-201)                              */
+/* ------------------------------------------------------------------------------ */
+/* constants.js:194-201 pvMixCoeffs */
+static void pv_mix(double eta, double hg, double hl, double *cL, double *cG) {
+    double e = dmax(0, dmin(1, eta));
+    double wL = e * PV_L0 / dmax(1e-12, hl);
+    double wG = (1 - e) * PV_G0 / dmax(1e-12, hg);
+    double s = wL + wG;
+    if (!(s > 0) || !js_finite(s)) { *cL = e; *cG = 1 - e; return; }
+    *cL = wL / s; *cG = wG / s;
+}
+/* profile.js:149-153 gaussianFromSigmaSq */
+static double gaussian_from_sigma_sq(double s2, double minF) {
+    double fwhmMin = minF > 0 ? minF : MIN_FWHM_DEFAULT;
+    double floorSq = pow(0.25 * fwhmMin / GCONV, 2);
+    return sqrt(or_soft_positive(s2, floorSq)) * GCONV;
+}
+/* profile.js:160-164 lorentzianFromGamma */
+static double lorentzian_from_gamma(double g, double minF) {
+    double fwhmMin = minF > 0 ? minF : MIN_FWHM_DEFAULT;
+    double fl = 0.25 * fwhmMin / LCONV;
+    return or_soft_positive(g, fl) * LCONV;
+}
+static double orz(double v) { return (v != 0 && !isnan(v)) ? v : 0; }  /* `x || 0` */
+
+/* profile.js:501-537 calculateProfileWidths (non-separated TCH / simple / split);
+ * side: 0 left, 1 right/centre */
+static void profile_widths(int type, const double *p, double minF, double tth, int side, double *gG, double *gL) {
+    double fwhmMin = minF > 0 ? minF : MIN_FWHM_DEFAULT;
+    double th = tth * (JS_PI / 180) / 2, MAXA = JS_PI / 2.0 - 1e-6, safe = dmin(th, MAXA);
+    if (isnan(th)) safe = NAN;
+    if (safe < 1e-6) { *gG = fwhmMin; *gL = fwhmMin; return; }
+    double tn = tan(safe), cs = cos(safe), cs_safe = dmax(cs, 1e-9), cs2_safe = dmax(cs * cs, 1e-9);
+    double g, l;
+    if (type == PT_PROFILE_SIMPLE) {            /* profile.js:234-254 */
+        double GU = orz(p[PT_SIM_GU]), GV = orz(p[PT_SIM_GV]), GW = orz(p[PT_SIM_GW]), GP = orz(p[PT_SIM_GP]), LX = orz(p[PT_SIM_LX]);
+        double s2 = GU * tn * tn + GV * tn + GW + GP / cs2_safe;
+        g = gaussian_from_sigma_sq(s2, fwhmMin);
+        l = lorentzian_from_gamma(LX / cs_safe, fwhmMin);
+    } else if (type == PT_PROFILE_SPLIT) {      /* profile.js:270-298 */
+        int o = side == 0 ? 0 : 4;
+        double GU = orz(p[PT_SPL_GU_L + o]), GV = orz(p[PT_SPL_GV_L + o]), GW = orz(p[PT_SPL_GW_L + o]), LX = orz(p[PT_SPL_LX_L + o]);
+        double s2 = GU * tn * tn + GV * tn + GW;
+        g = gaussian_from_sigma_sq(s2, fwhmMin);
+        l = lorentzian_from_gamma(LX / cs_safe, fwhmMin);
+    } else {                                    /* profile.js:426-482 (no Stephens terms) */
+        double U = orz(p[PT_TCH_U]), V = orz(p[PT_TCH_V]), W = orz(p[PT_TCH_W]), X = orz(p[PT_TCH_X]), Y = orz(p[PT_TCH_Y]);
+        double s2 = U * tn * tn + V * tn + W;
+        g = gaussian_from_sigma_sq(s2, fwhmMin);
+        l = lorentzian_from_gamma(X / cs_safe + Y * tn, fwhmMin);
+    }
+    *gG = or_soft_floor(g, fwhmMin);            /* profile.js:213-219 applyFwhmFloor */
+    *gL = or_soft_floor(l, fwhmMin);
+}
+
+/* profile.js:733-759 evalVoigt */
+double or_eval_voigt(double x, const or_voigt *p) {
+    double delta = x - p->x0;
+    if (delta > p->max_d || delta < -p->max_d) return 0.0;
+    if (p->asym != 0 && fabs(delta) >= 1e-9) {
+        double t = dmax(-0.95, dmin(0.95, p->asym * delta));
+        delta /= (1.0 - t);
+    }
+    int s = (p->type == PT_PROFILE_SPLIT && !(delta < 0)) ? 1 : 0;
+    double ig = p->inv_g[s], il = p->inv_l[s], cL = p->cl[s], cG = p->cg[s];
+    double dg = (delta * ig) * (delta * ig);
+    double dl = (delta * il) * (delta * il);
+    double g = exp(-CG * dg);
+    double l = 1.0 / (1.0 + 4.0 * dl);
+    double v = cL * l + cG * g - p->pedestal;
+    return v > 0 ? v : 0.0;
+}
+
+/* profile.js:799-831 asymmetryAreaCorrection */
+double or_asym_area(const or_voigt *prep) {
+    consts();
+    if (prep->asym == 0 || !(prep->max_d > 0) || prep->type != PT_PROFILE_TCH) return 0;
+    or_voigt as = *prep, sy;
+    as.x0 = 0; as.max_d = INFINITY; as.pedestal = 0;
+    sy = as; sy.asym = 0;
+    double a = prep->asym, H = prep->fwhm, w = dmax(1e-6, prep->window_fwhm);
+    double clampAt = 0.95 / fabs(a), D = dmax(prep->max_d, dmin(clampAt, 4096 * w));
+    double total = 0;
+    for (int si = 0; si < 2; si++) {
+        double side = si ? 1 : -1, lo = 0, e = 0.25 * w;
+        for (;;) {
+            double hi = dmin(e, D);
+            double bounds[2][2]; int np = 0;
+            if (clampAt > lo && clampAt < hi) { bounds[np][0] = lo; bounds[np][1] = clampAt; np++; lo = clampAt; }
+            bounds[np][0] = lo; bounds[np][1] = hi; np++;
+            for (int b = 0; b < np; b++) {
+                double m = 0.5 * (bounds[b][0] + bounds[b][1]), h = 0.5 * (bounds[b][1] - bounds[b][0]), s = 0;
+                for (int i = 0; i < 4; i++)
+                    for (int sg = -1; sg <= 1; sg += 2) {
+                        double d = side * (m + sg * h * GL8_X[i]);
+                        s += GL8_W[i] * (or_eval_voigt(d, &as) - or_eval_voigt(d, &sy));
+                    }
+                total += s * h;
+            }
+            lo = hi;
+            if (hi >= D) break;
+            e *= 2;
+        }
+        double k = 1 - dmax(-0.95, dmin(0.95, a * side * D));
+        double x1 = D / k, x2 = D;
+        double t1 = prep->cl[0] * 0.5 * H * (0.5 * JS_PI - atan(2 * x1 / H)) +
+                    prep->cg[0] * 0.5 * H * sqrt(JS_PI / CG) * or_erfc(sqrt(CG) * x1 / H);
+        double t2 = prep->cl[0] * 0.5 * H * (0.5 * JS_PI - atan(2 * x2 / H)) +
+                    prep->cg[0] * 0.5 * H * sqrt(JS_PI / CG) * or_erfc(sqrt(CG) * x2 / H);
+        total += k * t1 - t2;
+    }
+    return total;
+}
+
+/* profile.js:603-718 prepareVoigt (+ preparedProfileArea, profile.js:854-858) */
+int or_prepare_voigt(int type, const double *p, double minF, double tth_peak, double x0, or_voigt *o) {
+    consts();
+    memset(o, 0, sizeof *o);
+    o->type = type; o->x0 = x0;
+    double asym = 0;
+    if (type == PT_PROFILE_TCH && (orz(p[PT_TCH_SL]) != 0 || orz(p[PT_TCH_HL]) != 0) && tth_peak >= 0.1 && tth_peak < 180) {
+        double th = tth_peak * (JS_PI / 180) / 2.0;
+        double safe = dmax(1e-6, dmin(JS_PI / 2.0 - 1e-6, th));
+        double tn = tan(safe);
+        if (fabs(tn) >= 1e-9) asym = orz(p[PT_TCH_SL]) / tn + orz(p[PT_TCH_HL]);
+    }
+    o->asym = asym;
+    double fwhm_total, eta;
+    if (type == PT_PROFILE_SIMPLE) {
+        double gG, gL;
+        profile_widths(type, p, minF, tth_peak, 1, &gG, &gL);
+        double HG = dmax(1e-9, gG), HL = dmax(1e-9, gL);
+        double e = p[PT_SIM_ETA];
+        eta = dmax(0, dmin(1, js_finite(e) ? e : 0.5));   /* readEta */
+        fwhm_total = or_tch_fwhm(HG, HL);
+        double cL, cG; pv_mix(eta, HG, HL, &cL, &cG);
+        o->inv_g[0] = o->inv_g[1] = 1 / HG; o->inv_l[0] = o->inv_l[1] = 1 / HL;
+        o->cl[0] = o->cl[1] = cL; o->cg[0] = o->cg[1] = cG;
+        o->analytic_area = cL * PV_LORENTZ_AREA * HL + cG * PV_GAUSS_AREA * HG;
+        o->fwhm = fwhm_total;
+    } else if (type == PT_PROFILE_SPLIT) {
+        double gGL, gLL, gGR, gLR;
+        profile_widths(type, p, minF, tth_peak, 0, &gGL, &gLL);
+        profile_widths(type, p, minF, tth_peak, 1, &gGR, &gLR);
+        double HGL = dmax(1e-9, gGL), HLL = dmax(1e-9, gLL), HGR = dmax(1e-9, gGR), HLR = dmax(1e-9, gLR);
+        double e = p[PT_SPL_ETA];
+        eta = dmax(0, dmin(1, js_finite(e) ? e : 0.5));
+        fwhm_total = dmax(or_tch_fwhm(HGL, HLL), or_tch_fwhm(HGR, HLR));
+        double cLL, cGL, cLR, cGR;
+        pv_mix(eta, HGL, HLL, &cLL, &cGL);
+        pv_mix(eta, HGR, HLR, &cLR, &cGR);
+        o->inv_g[0] = 1 / HGL; o->inv_l[0] = 1 / HLL; o->inv_g[1] = 1 / HGR; o->inv_l[1] = 1 / HLR;
+        o->cl[0] = cLL; o->cg[0] = cGL; o->cl[1] = cLR; o->cg[1] = cGR;
+        o->analytic_area = 0.5 * (cLL * PV_LORENTZ_AREA * HLL + cGL * PV_GAUSS_AREA * HGL) +
+                           0.5 * (cLR * PV_LORENTZ_AREA * HLR + cGR * PV_GAUSS_AREA * HGR);
+        o->fwhm = fwhm_total;
+    } else {
+        double gG, gL;
+        profile_widths(PT_PROFILE_TCH, p, minF, tth_peak, 1, &gG, &gL);
+        double HG = dmax(1e-9, gG), HL = dmax(1e-9, gL);
+        double fwhm = dmax(1e-9, or_tch_fwhm(HG, HL));
+        eta = 0;
+        if (fwhm > 1e-9) {
+            double r = HL / fwhm;
+            eta = dmax(0, dmin(1, 1.36603 * r - 0.47719 * (r * r) + 0.11116 * pow(r, 3)));
+        }
+        fwhm_total = fwhm;
+        double cL, cG; pv_mix(eta, fwhm, fwhm, &cL, &cG);
+        o->inv_g[0] = o->inv_g[1] = o->inv_l[0] = o->inv_l[1] = 1 / fwhm;
+        o->cl[0] = o->cl[1] = cL; o->cg[0] = o->cg[1] = cG;
+        o->analytic_area = (cL * PV_LORENTZ_AREA + cG * PV_GAUSS_AREA) * fwhm;
+        o->fwhm = fwhm;
+    }
+    o->eta = eta;
+    o->window_fwhm = dmax(1e-6, fwhm_total);
+    double etaWin = dmax(0, dmin(1, eta));
+    double mult = WINDOW_G + (WINDOW_L - WINDOW_G) * etaWin;
+    double R = dmin(WINDOW_MAX, mult * dmax(0.01, o->window_fwhm) * (asym != 0 ? 2.0 : 1.0));
+    o->pedestal = 0;
+    o->max_d = INFINITY;
+    o->pedestal = dmax(or_eval_voigt(x0 + R, o), or_eval_voigt(x0 - R, o));
+    o->max_d = R;
+    o->area = o->analytic_area + or_asym_area(o);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------------ */
+/*  the model (rietveld_core.js:629-931, 1152-1164, 1878-1889)                      */
+/* ------------------------------------------------------------------------------ */
+typedef struct {
+    int nrep, ncent;
+    int32_t *rep;                    /* indices into the phase's ops */
+    refl_list refl;                  /* current reflection list */
+} or_phase;
+
+struct or_model {
+    pt_problem P;                    /* shallow copy; arrays owned below */
+    double *tth, *y, *w;
+    pt_phase *phases;
+    pt_atom **atoms;
+    int32_t **op_r;
+    double **op_t;
+    double *base, *lo, *hi, *scale, *x0, *coeff;
+    int32_t *tparam, *tvar;
+    or_phase *ph;
+    double **T;                      /* Chebyshev T_k(x_i) */
+    double guard_lo, guard_hi;
+    int has_key;
+    double *key;                     /* cache key: zero, disp, transp, cells */
+    /* scratch */
+    double *params, *ybuf, *col;
+};
+
+static double *dup_d(const double *s, int n) {
+    double *d = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    if (n > 0) memcpy(d, s, sizeof(double) * n);
+    return d;
+}
+
+or_model *or_create(const pt_problem *prob) {
+    consts();
+    if (!prob || prob->n_points < 2 || prob->n_phases < 1 || prob->n_bkg < 0 || prob->n_bkg > PT_MAX_BKG) { set_err("invalid problem"); return NULL; }
+    or_model *m = (or_model *)calloc(1, sizeof *m);
+    m->P = *prob;
+    int n = prob->n_points;
+    m->tth = dup_d(prob->tth, n); m->y = dup_d(prob->y_obs, n); m->w = dup_d(prob->weight, n);
+    m->P.tth = m->tth; m->P.y_obs = m->y; m->P.weight = m->w;
+    m->phases = (pt_phase *)malloc(sizeof(pt_phase) * prob->n_phases);
+    m->atoms = (pt_atom **)calloc(prob->n_phases, sizeof(pt_atom *));
+    m->op_r = (int32_t **)calloc(prob->n_phases, sizeof(int32_t *));
+    m->op_t = (double **)calloc(prob->n_phases, sizeof(double *));
+    m->ph = (or_phase *)calloc(prob->n_phases, sizeof(or_phase));
+    for (int p = 0; p < prob->n_phases; p++) {
+        const pt_phase *s = &prob->phases[p];
+        m->phases[p] = *s;
+        m->atoms[p] = (pt_atom *)malloc(sizeof(pt_atom) * (s->n_atoms > 0 ? s->n_atoms : 1));
+        memcpy(m->atoms[p], s->atoms, sizeof(pt_atom) * s->n_atoms);
+        m->op_r[p] = (int32_t *)malloc(sizeof(int32_t) * 9 * s->n_ops);
+        memcpy(m->op_r[p], s->op_r, sizeof(int32_t) * 9 * s->n_ops);
+        m->op_t[p] = dup_d(s->op_t, 3 * s->n_ops);
+        m->phases[p].atoms = m->atoms[p]; m->phases[p].op_r = m->op_r[p]; m->phases[p].op_t = m->op_t[p];
+        m->ph[p].rep = (int32_t *)malloc(sizeof(int32_t) * (s->n_ops + 1));
+        m->ph[p].nrep = or_cosets(s->n_ops, s->op_r, s->op_t, m->ph[p].rep, &m->ph[p].ncent);
+        if (!s->n_ops) { set_err("phase without operators"); }
+    }
+    m->P.phases = m->phases;
+    m->base = dup_d(prob->param_base, prob->n_params);
+    m->lo = dup_d(prob->lo, prob->n_vars); m->hi = dup_d(prob->hi, prob->n_vars);
+    m->scale = dup_d(prob->scale, prob->n_vars); m->x0 = dup_d(prob->x0, prob->n_vars);
+    m->coeff = dup_d(prob->term_coeff, prob->n_terms);
+    m->tparam = (int32_t *)malloc(sizeof(int32_t) * (prob->n_terms + 1));
+    m->tvar = (int32_t *)malloc(sizeof(int32_t) * (prob->n_terms + 1));
+    if (prob->n_terms) {
+        memcpy(m->tparam, prob->term_param, sizeof(int32_t) * prob->n_terms);
+        memcpy(m->tvar, prob->term_var, sizeof(int32_t) * prob->n_terms);
+    }
+    /* rietveld_core.js:647-653 Chebyshev polynomials over the fitted range */
+    double lo = m->tth[0], hi = m->tth[n - 1];
+    m->T = (double **)calloc(prob->n_bkg > 0 ? prob->n_bkg : 1, sizeof(double *));
+    for (int k = 0; k < prob->n_bkg; k++) m->T[k] = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; i++) {
+        double x = hi > lo ? 2 * (m->tth[i] - lo) / (hi - lo) - 1 : 0;
+        if (prob->n_bkg > 0) m->T[0][i] = 1;
+        if (prob->n_bkg > 1) m->T[1][i] = x;
+        for (int k = 2; k < prob->n_bkg; k++) m->T[k][i] = 2 * x * m->T[k - 1][i] - m->T[k - 2][i];
+    }
+    /* rietveld_core.js:1422-1424 guard range in tan(theta) */
+    m->guard_lo = tan(dmax(1e-6, dmin(179.999999, m->tth[0])) * JS_PI / 360);
+    m->guard_hi = tan(dmax(1e-6, dmin(179.999999, m->tth[n - 1])) * JS_PI / 360);
+    m->key = (double *)malloc(sizeof(double) * (3 + 6 * prob->n_phases));
+    m->params = (double *)malloc(sizeof(double) * (prob->n_params > 0 ? prob->n_params : 1));
+    m->ybuf = (double *)malloc(sizeof(double) * n);
+    m->col = (double *)malloc(sizeof(double) * n);
+    return m;
+}
+
+void or_destroy(or_model *m) {
+    if (!m) return;
+    for (int p = 0; p < m->P.n_phases; p++) {
+        free(m->atoms[p]); free(m->op_r[p]); free(m->op_t[p]); free(m->ph[p].rep); rl_free(&m->ph[p].refl);
+    }
+    for (int k = 0; k < m->P.n_bkg; k++) free(m->T[k]);
+    free(m->T); free(m->atoms); free(m->op_r); free(m->op_t); free(m->ph); free(m->phases);
+    free(m->tth); free(m->y); free(m->w); free(m->base); free(m->lo); free(m->hi); free(m->scale);
+    free(m->x0); free(m->coeff); free(m->tparam); free(m->tvar); free(m->key); free(m->params);
+    free(m->ybuf); free(m->col); free(m);
+}
+
+/* apply(x), rietveld_core.js:1855-1859: every model parameter at the variables */
+void or_params(const or_model *m, const double *x, double *p) {
+    memcpy(p, m->base, sizeof(double) * m->P.n_params);
+    for (int t = 0; t < m->P.n_terms; t++) p[m->tparam[t]] += m->coeff[t] * x[m->tvar[t]];
+}
+
+static int phase_base(const pt_problem *P, int phase) {
+    int b = PT_P_PHASE0;
+    for (int q = 0; q < phase; q++) b += PT_PH_ATOM0 + PT_AT_STRIDE * P->phases[q].n_atoms;
+    return b;
+}
+
+/* Model.refreshReflections, rietveld_core.js:680-689: re-enumerate when the cell, zero,
+ * disp or transp change */
+static void refresh_reflections(or_model *m, const double *p) {
+    int np = m->P.n_phases, changed = !m->has_key;
+    double key[3 + 6 * 8];
+    double *kk = np <= 8 ? key : (double *)malloc(sizeof(double) * (3 + 6 * np));
+    kk[0] = p[PT_P_ZERO]; kk[1] = p[PT_P_DISP]; kk[2] = p[PT_P_TRANSP];
+    for (int q = 0; q < np; q++) {
+        const double *pp = p + phase_base(&m->P, q);
+        for (int c = 0; c < 6; c++) kk[3 + 6 * q + c] = pp[c];
+    }
+    if (!changed) for (int i = 0; i < 3 + 6 * np; i++) if (memcmp(&kk[i], &m->key[i], sizeof(double))) { changed = 1; break; }
+    if (changed) {
+        double dm = or_reflection_dmin(&m->P, p[PT_P_ZERO], p[PT_P_DISP], p[PT_P_TRANSP]);
+        for (int q = 0; q < np; q++) {
+            double cell[6];
+            const double *pp = p + phase_base(&m->P, q);
+            if (cell_of(&m->phases[q], pp, cell))
+                gen_reflections(cell, m->phases[q].n_ops, m->phases[q].op_r, m->phases[q].op_t, dm, NULL, &m->ph[q].refl);
+            else m->ph[q].refl.n = 0;
+        }
+        memcpy(m->key, kk, sizeof(double) * (3 + 6 * np));
+        m->has_key = 1;
+    }
+    if (kk != key) free(kk);
+}
+
+/* rietveld_core.js:383-390 formFactor */
+static double form_factor(const pt_atom *a, int radiation, double s) {
+    if (radiation == PT_NEUTRON) return js_finite(a->b_neutron) ? a->b_neutron : 0;
+    double q = (s / 2) * (s / 2), f = a->ff_c;
+    for (int i = 0; i < a->n_gauss; i++) f += a->ff_a[i] * exp(-a->ff_b[i] * q);
+    return f;
+}
+
+/* per-reflection working data of one phase */
+typedef struct {
+    double tth, bragg, s, d, f2, po, lp, K, height;
+    int has_col, start, stop;
+    or_voigt v[2];
+    double weight[2];
+    int present[2], cstart[2], cstop[2];
+} refl_work;
+
+/* Model.structure, rietveld_core.js:749-821 (no gradients) */
+static void structure(or_model *m, int q, const double *pp, refl_work *rw) {
+    const pt_phase *ph = &m->phases[q];
+    or_phase *op = &m->ph[q];
+    int nA = ph->n_atoms, nR = op->nrep, rad = m->P.radiation;
+    double pos[64][3];
+    double *posd = nA <= 64 ? &pos[0][0] : (double *)malloc(sizeof(double) * 3 * nA);
+    for (int j = 0; j < nA; j++) {
+        const pt_atom *a = &ph->atoms[j];
+        const double *ap = pp + PT_PH_ATOM0 + PT_AT_STRIDE * j;
+        double r[3] = {a->r0[0], a->r0[1], a->r0[2]};     /* sitePosition, :372-378 */
+        for (int f = 0; f < 3; f++) if (a->free_mask & (1 << f)) {
+            double v = ap[f];
+            r[0] += v * a->basis[f][0]; r[1] += v * a->basis[f][1]; r[2] += v * a->basis[f][2];
+        }
+        posd[3 * j] = r[0]; posd[3 * j + 1] = r[1]; posd[3 * j + 2] = r[2];
+    }
+    refl_list *L = &op->refl;
+    for (int k = 0; k < L->n; k++) {
+        refl_work *R = &rw[k];
+        R->f2 = 0;
+        if (!js_finite(R->tth)) continue;
+        double s = R->s, s2 = s * s, Ar = 0, Br = 0, Cr = 0, Dr = 0;
+        int h = L->r[k].h, kk = L->r[k].k, l = L->r[k].l;
+        for (int j = 0; j < nA; j++) {
+            const pt_atom *a = &ph->atoms[j];
+            const double *ap = pp + PT_PH_ATOM0 + PT_AT_STRIDE * j;
+            double occ = ap[PT_AT_OCC], B = ap[PT_AT_B];
+            double w0 = (a->mult != 0 ? a->mult : 1) / nR;
+            double dw = exp(-B * s2 / 4), f0 = form_factor(a, rad, s);
+            int neutron = rad == PT_NEUTRON;
+            double wj = occ * (neutron ? f0 : f0 + a->fp) * dw * w0;
+            double wi = (neutron || !(a->fpp != 0)) ? 0 : occ * a->fpp * dw * w0;
+            double c = 0, sn = 0;
+            const double *p = posd + 3 * j;
+            for (int qq = 0; qq < nR; qq++) {
+                const int32_t *r = ph->op_r + 9 * op->rep[qq];
+                const double *t = ph->op_t + 3 * op->rep[qq];
+                double H0 = h * r[0] + kk * r[3] + l * r[6], H1 = h * r[1] + kk * r[4] + l * r[7], H2 = h * r[2] + kk * r[5] + l * r[8];
+                double phs = TWO_PI * (H0 * p[0] + H1 * p[1] + H2 * p[2] + h * t[0] + kk * t[1] + l * t[2]);
+                c += cos(phs); sn += sin(phs);
+            }
+            Ar += wj * c; Br += wj * sn; Cr += wi * c; Dr += wi * sn;
+        }
+        R->f2 = Ar * Ar + Br * Br + Cr * Cr + Dr * Dr;
+    }
+    if (posd != &pos[0][0]) free(posd);
+}
+
+/* Model.preferredOrientation, rietveld_core.js:824-843 */
+static void preferred_orientation(or_model *m, int q, const double *pp, const double f[6], int have_f, refl_work *rw) {
+    refl_list *L = &m->ph[q].refl;
+    double r = pp[PT_PH_PO];
+    for (int k = 0; k < L->n; k++) rw[k].po = 1;
+    if (!(r > 0) || fabs(r - 1) < 1e-12 || !have_f) return;
+    const double *H = m->phases[q].po_dir;
+#define GMET(a, b) ((a)[0] * (b)[0] * f[0] + (a)[1] * (b)[1] * f[1] + (a)[2] * (b)[2] * f[2] + \
+                    ((a)[0] * (b)[1] + (a)[1] * (b)[0]) * f[3] / 2 + ((a)[1] * (b)[2] + (a)[2] * (b)[1]) * f[4] / 2 + \
+                    ((a)[0] * (b)[2] + (a)[2] * (b)[0]) * f[5] / 2)
+    double HH = GMET(H, H), r2 = r * r, ir = 1 / r;
+    if (!(HH > 0)) return;
+    for (int k = 0; k < L->n; k++) {
+        int mm = L->r[k].m;
+        const int32_t *eq = L->eq + 3 * L->r[k].eq_off;
+        double sum = 0;
+        for (int e = 0; e < mm; e++) {
+            double hv[3] = {eq[3 * e], eq[3 * e + 1], eq[3 * e + 2]};
+            double hh = GMET(hv, hv), c = GMET(hv, H) / sqrt(hh * HH), c2 = dmin(1, c * c);
+            sum += pow(r2 * c2 + (1 - c2) * ir, -1.5);
+        }
+        rw[k].po = sum / mm;
+    }
+#undef GMET
+}
+
+/* constants.js:506-513 lowerBound */
+static int lower_bound(const double *a, int n, double target) {
+    int lo = 0, hi = n;
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (a[mid] < target) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+
+/* one phase: Model.geometry (:719-736), buildPeakContributions (profile.js:1013-1080),
+ * evalPhase (:855-897), render (:908-917).  Adds the phase pattern into y. */
+static int eval_phase(or_model *m, int q, const double *p, double *y, refl_work **rw_out, int *nr_out) {
+    const pt_phase *ph = &m->phases[q];
+    const double *pp = p + phase_base(&m->P, q);
+    refl_list *L = &m->ph[q].refl;
+    int n = m->P.n_points, nr = L->n;
+    double cell[6], f[6];
+    int have_f = cell_of(ph, pp, cell) && or_inv_dsq(cell, f);
+    double lam = m->P.lambda1, disp = p[PT_P_DISP], transp = p[PT_P_TRANSP], zero = p[PT_P_ZERO];
+    refl_work *rw = (refl_work *)calloc(nr > 0 ? nr : 1, sizeof(refl_work));
+    /* geometry */
+    for (int k = 0; k < nr; k++) {
+        double qv = have_f ? inv_dsq(f, L->r[k].h, L->r[k].k, L->r[k].l) : NAN, s = sqrt(qv), sinT = lam * s / 2, tth = NAN;
+        if (qv > 0 && sinT < 1) tth = 2 * asin(sinT) * R2D;
+        double bragg = tth;
+        if (js_finite(tth)) tth += or_peak_shift(tth, disp, transp);
+        rw[k].tth = tth; rw[k].bragg = bragg; rw[k].s = s; rw[k].d = 1 / s;
+    }
+    /* contributions: all Kalpha1, then all Kalpha2 */
+    const double *prof = pp + PT_PH_PROF0;
+    double l1 = m->P.lambda1 != 0 ? m->P.lambda1 : 1.54056, l2 = m->P.lambda2, ratio = m->P.ratio;
+    int doublet = ratio > 1e-6 && l2 > 1e-6 && fabs(l1 - l2) > 1e-6;
+    double lratio = doublet ? l2 / l1 : 1.0;
+    for (int pass = 0; pass < (doublet ? 2 : 1); pass++) {
+        for (int k = 0; k < nr; k++) {
+            refl_work *R = &rw[k];
+            if (!(js_finite(R->tth) && R->tth > 0 && R->tth < 180)) continue;   /* hasRenderablePosition */
+            double bragg = js_finite(R->bragg) ? R->bragg : R->tth, tl = bragg;
+            if (pass == 1) {
+                double st2 = sin(bragg * (JS_PI / 180) / 2.0) * lratio;
+                if (fabs(st2) >= 1) continue;
+                tl = 2 * asin(st2) / (JS_PI / 180);
+            }
+            double base = tl + or_peak_shift(tl, disp, transp) + zero;
+            double centre = base;   /* calculatePeakShift(base, params) is 0: no shft/trns in params */
+            or_voigt *v = &R->v[pass];
+            or_prepare_voigt(ph->profile_type, prof, m->P.min_fwhm, base, centre, v);
+            R->present[pass] = 1;
+            if (pass == 0) R->weight[0] = 1.0;
+            else {
+                double lpr = js_finite(m->P.pol_k) ? or_lp(base, m->P.pol_k) / or_lp(R->v[0].x0, m->P.pol_k) : 1;
+                R->weight[1] = ratio * R->v[0].area / v->area * lpr;
+            }
+            /* push(): window in the axis */
+            R->cstart[pass] = R->cstop[pass] = 0;
+            if (!js_finite(v->max_d)) { R->present[pass] = 0; continue; }
+            double lo = centre - v->max_d, hi = centre + v->max_d;
+            int start = lower_bound(m->tth, n, lo);
+            if (start >= n) { R->present[pass] = 0; continue; }
+            int stop = start;
+            while (stop < n && m->tth[stop] <= hi) stop++;
+            if (stop <= start) { R->present[pass] = 0; continue; }
+            R->cstart[pass] = start; R->cstop[pass] = stop;
+        }
+    }
+    /* evalPhase: K = area1, Lp, structure, PO, heights */
+    for (int k = 0; k < nr; k++) {
+        refl_work *R = &rw[k];
+        R->has_col = 0;
+        if (!(R->present[0] || R->present[1]) || !js_finite(R->tth)) continue;
+        double tc = R->tth + zero, a = R->v[0].area;
+        if (!(a > 0)) continue;
+        R->has_col = 1; R->K = a; R->lp = or_lp(tc, m->P.pol_k);
+        R->start = R->present[0] ? R->cstart[0] : R->cstart[1];
+        R->stop = R->present[0] ? R->cstop[0] : R->cstop[1];
+        if (R->present[1]) { if (R->cstart[1] < R->start) R->start = R->cstart[1]; if (R->cstop[1] > R->stop) R->stop = R->cstop[1]; }
+    }
+    structure(m, q, pp, rw);
+    preferred_orientation(m, q, pp, f, have_f, rw);
+    double S = pp[PT_PH_SCALE];
+    if (!(S != 0) || isnan(S)) S = 0;
+    double *col = m->col;
+    for (int k = 0; k < nr; k++) {
+        refl_work *R = &rw[k];
+        R->height = 0;
+        if (!R->has_col) continue;
+        double factor = L->r[k].m * R->lp * R->po / R->K;
+        R->height = S * (factor * R->f2);
+        /* intensityDerivativeColumn (profile.js:1131-1150), then render */
+        int len = R->stop - R->start;
+        for (int i = 0; i < len; i++) col[i] = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            if (!R->present[pass]) continue;
+            double wgt = R->weight[pass];
+            for (int i = R->cstart[pass]; i < R->cstop[pass]; i++)
+                col[i - R->start] += wgt * or_eval_voigt(m->tth[i], &R->v[pass]);
+        }
+        double hk = R->height;
+        if (!(hk != 0)) continue;
+        for (int i = 0; i < len; i++) y[R->start + i] += hk * col[i];
+    }
+    if (rw_out) { *rw_out = rw; *nr_out = nr; } else free(rw);
+    return 0;
+}
+
+static int cell_ok(or_model *m, const double *p) {
+    for (int q = 0; q < m->P.n_phases; q++) {
+        double cell[6], f[6];
+        const double *pp = p + phase_base(&m->P, q);
+        if (!cell_of(&m->phases[q], pp, cell)) return 0;
+        if (!(or_cell_volume(cell) > 0) || !or_inv_dsq(cell, f)) return 0;
+    }
+    return 1;
+}
+int or_cell_valid(or_model *m, const double *x) {
+    or_params(m, x, m->params);
+    return cell_ok(m, m->params);
+}
+
+/* Model.background (:920-931) + evalDataset (:1152-1164) */
+static int pattern_at_params(or_model *m, const double *p, double *y) {
+    int n = m->P.n_points;
+    refresh_reflections(m, p);
+    for (int i = 0; i < n; i++) y[i] = 0;
+    for (int k = 0; k < m->P.n_bkg; k++) {
+        double b = p[PT_P_BKG0 + k];
+        if (!(b != 0) || isnan(b)) continue;
+        for (int i = 0; i < n; i++) y[i] += b * m->T[k][i];
+    }
+    /* each phase renders into its own array, then addTo(y, ph.y) */
+    double *yp = (double *)malloc(sizeof(double) * n);
+    for (int q = 0; q < m->P.n_phases; q++) {
+        for (int i = 0; i < n; i++) yp[i] = 0;
+        eval_phase(m, q, p, yp, NULL, NULL);
+        for (int i = 0; i < n; i++) y[i] += yp[i];
+    }
+    free(yp);
+    return 0;
+}
+
+int or_pattern(or_model *m, const double *x, double *ycalc) {
+    or_params(m, x, m->params);
+    if (!cell_ok(m, m->params)) return -1;
+    return pattern_at_params(m, m->params, ycalc);
+}
+
+/* cost(), rietveld_core.js:1878-1889 */
+double or_cost(or_model *m, const double *x) {
+    or_params(m, x, m->params);
+    if (!cell_ok(m, m->params)) return INFINITY;
+    double *y = m->ybuf;
+    pattern_at_params(m, m->params, y);
+    double s = 0;
+    for (int i = 0; i < m->P.n_points; i++) {
+        if (!(m->w[i] > 0)) continue;
+        double r = m->y[i] - y[i];
+        s += m->w[i] * r * r;
+    }
+    if (!(s < INFINITY)) return INFINITY;
+    return s;
+}
+
+int or_reflection_table(or_model *m, const double *x, int q, int32_t *hklm, double *f2, double *height,
+                        double *centre, int cap) {
+    or_params(m, x, m->params);
+    if (!cell_ok(m, m->params)) return -1;
+    refresh_reflections(m, m->params);
+    double *yp = (double *)calloc(m->P.n_points, sizeof(double));
+    refl_work *rw = NULL; int nr = 0;
+    eval_phase(m, q, m->params, yp, &rw, &nr);
+    refl_list *L = &m->ph[q].refl;
+    for (int k = 0; k < nr && k < cap; k++) {
+        if (hklm) { hklm[4 * k] = L->r[k].h; hklm[4 * k + 1] = L->r[k].k; hklm[4 * k + 2] = L->r[k].l; hklm[4 * k + 3] = L->r[k].m; }
+        if (f2) f2[k] = rw[k].f2;
+        if (height) height[k] = rw[k].height;
+        if (centre) centre[k] = rw[k].v[0].x0;
+    }
+    free(rw); free(yp);
+    return nr;
+}
+
+/* gaussianVarianceMinimum (:1393-1398) */
+static double variance_min(double u, double v, double w, double lo, double hi) {
+    double c[3] = {lo, hi, 0}; int nc = 2;
+    if (u > 0) { double vx = -v / (2 * u); if (vx > lo && vx < hi) c[nc++] = vx; }
+    double best = (u * c[0] + v) * c[0] + w;
+    for (int i = 1; i < nc; i++) { double val = (u * c[i] + v) * c[i] + w; if (val < best) best = val; }
+    return best;
+}
+
+/* gaussianVarianceGuards + varianceViolation (:1413-1496), non-separated profiles */
+static double violation_at_params(or_model *m, const double *p) {
+    double worst = 0;
+    for (int q = 0; q < m->P.n_phases; q++) {
+        const double *pr = p + phase_base(&m->P, q) + PT_PH_PROF0;
+        int type = m->phases[q].profile_type;
+        int sets = type == PT_PROFILE_SPLIT ? 2 : 1;
+        for (int s = 0; s < sets; s++) {
+            double u, v, w;
+            if (type == PT_PROFILE_TCH) { u = pr[PT_TCH_U]; v = pr[PT_TCH_V]; w = pr[PT_TCH_W]; }
+            else if (type == PT_PROFILE_SIMPLE) { double gp = pr[PT_SIM_GP]; u = pr[PT_SIM_GU] + gp; v = pr[PT_SIM_GV]; w = pr[PT_SIM_GW] + gp; }
+            else { int o = s ? 4 : 0; u = pr[PT_SPL_GU_L + o]; v = pr[PT_SPL_GV_L + o]; w = pr[PT_SPL_GW_L + o]; }
+            double mn = variance_min(u, v, w, m->guard_lo, m->guard_hi);
+            if (mn < 0 && -mn > worst) worst = -mn;
+        }
+    }
+    return worst;
+}
+double or_violation(or_model *m, const double *x) {
+    or_params(m, x, m->params);
+    return violation_at_params(m, m->params);
+}
+
+/* temper(), rietveld_core.js:2171-2240 */
+typedef struct { uint32_t s; } xs32;
+static double xs_random(xs32 *r) {
+    uint32_t v = r->s;
+    v ^= v << 13; v ^= v >> 17; v ^= v << 5;
+    r->s = v;
+    return ((double)v + 0.5) / 4294967296.0;
+}
+static double xs_gauss(xs32 *r) {
+    double u1 = xs_random(r);
+    double a = sqrt(-2 * log(u1));
+    double u2 = xs_random(r);
+    return a * cos(2 * JS_PI * u2);
+}
+static double reflect_js(double value, double low, double high) {
+    if (!js_finite(value)) return NAN;
+    if (js_finite(low) && js_finite(high)) {
+        double span = high - low;
+        if (!(span > 0)) return low;
+        double folded = fmod(fmod(value - low, 2 * span) + 2 * span, 2 * span);
+        return low + (folded <= span ? folded : 2 * span - folded);
+    }
+    if (value < low) return low + (low - value);
+    if (value > high) return high - (value - high);
+    return value;
+}
+
+double or_temper(or_model *m, const double *x0, const double *scales, int sweeps, int chains, double max_temp,
+                 uint32_t seed, double *best_x, double *stats) {
+    int nv = m->P.n_vars;
+    xs32 rng = {seed};
+    double *xs = (double *)malloc(sizeof(double) * nv * chains), *costs = (double *)malloc(sizeof(double) * chains);
+    int *feas = (int *)malloc(sizeof(int) * chains);
+    double *temps = (double *)malloc(sizeof(double) * chains), *prop = (double *)malloc(sizeof(double) * nv);
+    double initial = or_cost(m, x0);
+    double startViolation = or_violation(m, x0);
+    double bestCost = initial;
+    memcpy(best_x, x0, sizeof(double) * nv);
+    for (int r = 0; r < chains; r++) {
+        temps[r] = pow(max_temp, (double)r / (chains - 1));
+        memcpy(xs + r * nv, x0, sizeof(double) * nv);
+        costs[r] = initial;
+        feas[r] = startViolation == 0;
+    }
+    double accepted = 0, proposals = 0, swaps = 0, attempts = 0;
+    for (int sweep = 0; sweep < sweeps; sweep++) {
+        for (int r = 0; r < chains; r++) {
+            double *x = xs + r * nv;
+            for (int move = 0; move < nv; move++) {
+                int c = (int)floor(xs_random(&rng) * nv);
+                memcpy(prop, x, sizeof(double) * nv);
+                double g = xs_gauss(&rng);
+                prop[c] = reflect_js(prop[c] + g * scales[c] * sqrt(temps[r]), m->lo[c], m->hi[c]);
+                if (!js_finite(prop[c])) continue;
+                double cc = or_cost(m, prop);
+                proposals++;
+                double viol = or_violation(m, prop);
+                if (viol > 0 && feas[r]) cc = INFINITY;
+                if (js_finite(cc) && (cc <= costs[r] || log(xs_random(&rng)) < (costs[r] - cc) / (2 * temps[r]))) {
+                    memcpy(x, prop, sizeof(double) * nv);
+                    costs[r] = cc; feas[r] = !(viol > 0); accepted++;
+                    if (cc < bestCost) { bestCost = cc; memcpy(best_x, prop, sizeof(double) * nv); }
+                }
+            }
+        }
+        for (int r = sweep % 2; r < chains - 1; r += 2) {
+            double la = 0.5 * (1 / temps[r] - 1 / temps[r + 1]) * (costs[r] - costs[r + 1]);
+            attempts++;
+            if (la >= 0 || log(xs_random(&rng)) < la) {
+                double *a = xs + r * nv, *b = xs + (r + 1) * nv;
+                for (int j = 0; j < nv; j++) { double t = a[j]; a[j] = b[j]; b[j] = t; }
+                double tc = costs[r]; costs[r] = costs[r + 1]; costs[r + 1] = tc;
+                int tf = feas[r]; feas[r] = feas[r + 1]; feas[r + 1] = tf;
+                swaps++;
+            }
+        }
+    }
+    if (stats) { stats[0] = accepted; stats[1] = proposals; stats[2] = swaps; stats[3] = attempts; }
+    free(xs); free(costs); free(feas); free(temps); free(prop);
+    return bestCost;
+}
+
+typedef struct {
+    const pt_problem *prob;
+    const double *scales;
+    int sweeps, chains;
+    double max_temp;
+    uint32_t seed;
+    double proposals;
+} temper_job;
+
+static void *temper_thread(void *arg) {
+    temper_job *j = (temper_job *)arg;
+    or_model *m = or_create(j->prob);
+    double *best = (double *)malloc(sizeof(double) * j->prob->n_vars), st[4] = {0};
+    or_temper(m, j->prob->x0, j->scales, j->sweeps, j->chains, j->max_temp, j->seed, best, st);
+    j->proposals = st[1];
+    free(best);
+    or_destroy(m);
+    return NULL;
+}
+
+double or_temper_parallel(const pt_problem *prob, const double *scales, int sweeps, int chains, double max_temp,
+                          uint32_t seed, int threads, double *seconds) {
+    if (threads < 1) threads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    temper_job *jobs = (temper_job *)calloc(threads, sizeof(temper_job));
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < threads; i++) {
+        jobs[i].prob = prob; jobs[i].scales = scales; jobs[i].sweeps = sweeps; jobs[i].chains = chains;
+        jobs[i].max_temp = max_temp; jobs[i].seed = seed + (uint32_t)i;
+        pthread_create(&th[i], NULL, temper_thread, &jobs[i]);
+    }
+    double total = 0;
+    for (int i = 0; i < threads; i++) { pthread_join(th[i], NULL); total += jobs[i].proposals; }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    if (seconds) *seconds = sec;
+    free(th); free(jobs);
+    return total / sec;
+}

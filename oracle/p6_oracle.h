@@ -1,0 +1,91 @@
+/*
+ * p6_oracle.h -- FP64 CPU restatement of the Powder 6 Rietveld cost and CPU parallel
+ * tempering (js/rietveld_core.js, js/profile.js, js/crystal.js, js/constants.js).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared against and
+ * the CPU baseline that bench.py times; the product library never links, loads or
+ * calls it.  Each function cites the reference file:line it restates.
+ */
+#ifndef P6_ORACLE_H
+#define P6_ORACLE_H
+
+#include <stdint.h>
+#include "../include/powder6_pt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_model or_model;
+
+/* ---- building blocks (pinned by closed-form tests) ---- */
+double or_soft_positive(double x, double floor);                 /* constants.js:138-143 */
+double or_soft_floor(double x, double f);                        /* profile.js:182-189 */
+double or_erfc(double x);                                        /* profile.js:791-797 */
+double or_hypot2(double x, double y);                            /* Math.hypot, 2 args */
+double or_lp(double tth, double k);                              /* crystal.js:487-497 */
+double or_lorentz(double tth);                                   /* crystal.js:465-471 */
+double or_polarization(double tth, double k);                    /* crystal.js:474-478 */
+double or_tch_fwhm(double gg, double gl);                        /* profile.js:552-565 */
+int or_inv_dsq(const double cell[6], double form[6]);            /* crystal.js:326-366 */
+double or_cell_volume(const double cell[6]);                     /* crystal.js:245-252 */
+double or_peak_shift(double tth, double disp, double transp);    /* profile.js:108-132 */
+
+/* Prepared pseudo-Voigt of one line (profile.js:603-718), flattened.  kind = profile
+ * type; p = the profile slots of the phase (PT_TCH_* / PT_SIM_* / PT_SPL_*). */
+typedef struct {
+    int type;
+    double x0, asym, pedestal, max_d, window_fwhm, fwhm, eta;
+    double inv_g[2], inv_l[2], cl[2], cg[2];   /* [0] left / centre, [1] right */
+    double analytic_area, area;                /* area = analytic + asymmetry correction */
+} or_voigt;
+int or_prepare_voigt(int type, const double *prof, double min_fwhm, double tth_peak, double x0,
+                     or_voigt *out);
+double or_eval_voigt(double x, const or_voigt *p);               /* profile.js:733-759 */
+double or_asym_area(const or_voigt *p);                          /* profile.js:799-831 */
+
+/* Reflection list (rietveld_core.js:247-276).  cell = a,b,c,alpha,beta,gamma.  box NULL
+ * for the metric cut.  Writes up to cap rows of (h,k,l,m); d0 optional.  Returns the
+ * total count. */
+int or_generate_reflections(const double cell[6], int n_ops, const int32_t *op_r,
+                            const double *op_t, double dmin, const int32_t *box,
+                            int32_t *hklm, double *d0, int cap);
+/* Coset representatives modulo centring (rietveld_core.js:164-179); returns the count
+ * and writes the representative indices; *n_cent = number of centring vectors. */
+int or_cosets(int n_ops, const int32_t *op_r, const double *op_t, int32_t *rep_index,
+              int *n_cent);
+/* reflectionDmin (rietveld_core.js:238-245) at the given zero/disp/transp magnitudes */
+double or_reflection_dmin(const pt_problem *prob, double zero, double disp, double transp);
+
+/* ---- the model ---- */
+or_model *or_create(const pt_problem *prob);
+void or_destroy(or_model *m);
+const char *or_error(void);
+/* model parameters at the variables x: p = base + M x */
+void or_params(const or_model *m, const double *x, double *p);
+/* chi^2 at variables x: rietveld_core.js:1878-1889 (+inf when invalid) */
+double or_cost(or_model *m, const double *x);
+/* calculated pattern (background + phases) at x; returns 0, or -1 for an invalid cell */
+int or_pattern(or_model *m, const double *x, double *ycalc);
+/* per-reflection diagnostics of phase 0 at x: F^2, height, Kalpha1 centre (count out) */
+int or_reflection_table(or_model *m, const double *x, int phase, int32_t *hklm, double *f2,
+                        double *height, double *centre, int cap);
+/* varianceViolation (rietveld_core.js:1489-1496) at x */
+double or_violation(or_model *m, const double *x);
+/* the profile-feasibility rule of pt_gpu.js propose (js/pt_gpu.js:361): 1 valid */
+int or_cell_valid(or_model *m, const double *x);
+
+/* CPU PT temper() (rietveld_core.js:2171-2240).  scales[nv]; best_x[nv] out;
+ * stats[4] = accepted, proposals, swaps, swap attempts.  Returns best cost. */
+double or_temper(or_model *m, const double *x0, const double *scales, int sweeps, int chains,
+                 double max_temp, uint32_t seed, double *best_x, double *stats);
+
+/* One independent temper() per thread on `threads` POSIX threads, each on its own
+ * model and seed+i; returns proposals per second over all threads (wall clock). */
+double or_temper_parallel(const pt_problem *prob, const double *scales, int sweeps, int chains,
+                          double max_temp, uint32_t seed, int threads, double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,48 @@
+"""Build the gfx950 shared library powder6_amd/_lib/libpowder6_pt.so with hipcc.
+
+Flags: -O3, gfx950 only, packed-FP32 instructions disabled (-Xclang -target-feature
+-Xclang -packed-fp32-ops), FP contraction left on for the device (FMA) and off for the
+host plan compiler's reflection enumeration (bit-for-bit with the reference ordering).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+LIBDIR = os.path.join(HERE, '_lib')
+LIB = os.path.join(LIBDIR, 'libpowder6_pt.so')
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+ARCH = '--offload-arch=gfx950'
+DEVICE_FLAGS = ['-O3', '-std=c++17', '-fPIC', ARCH, '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+                '-munsafe-fp-atomics', '-Wall', '-Wno-unused-function']
+
+
+def run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError('build failed: ' + ' '.join(cmd))
+    return r.stdout
+
+
+def build(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    kern = os.path.join(LIBDIR, 'pt_kernels.o')
+    run([HIPCC] + DEVICE_FLAGS + ['-c', os.path.join(CSRC, 'pt_kernels.hip'), '-o', kern])
+    objs.append(kern)
+    host = os.path.join(LIBDIR, 'pt_host.o')
+    run([HIPCC, '-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', ARCH, '-Wall', '-x', 'hip',
+         '-c', os.path.join(CSRC, 'pt_host.cpp'), '-o', host])
+    objs.append(host)
+    tmp = LIB + '.tmp'
+    run([HIPCC, '-shared', '-fPIC', ARCH, '-o', tmp] + objs)
+    os.replace(tmp, LIB)
+    if verbose:
+        print('built', LIB)
+    return LIB
+
+
+if __name__ == '__main__':
+    build(verbose=True)

@@ -1,0 +1,704 @@
+// pt_host.cpp -- C ABI of the MI355X PT trial evaluator (include/powder6_pt.h).
+//
+// Host side of js/pt_gpu.js: compile() (reflection envelope, packing), create()
+// (device buffers, stream), costs()/batch()/run() plumbing.  All arithmetic of a trial
+// runs in the kernels of pt_kernels.hip; this file only prepares and launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/powder6_pt.h"
+#include "pt_launch.h"
+
+namespace {
+
+thread_local std::string g_err;
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+const double kPi = 3.141592653589793;
+const double D2R = kPi / 180, R2D = 180 / kPi;
+
+// ---------------------------------------------------------------------------------
+// reflection enumeration: the envelope of js/pt_gpu.js:110-131 through
+// generateReflections (js/rietveld_core.js:182-276)
+// ---------------------------------------------------------------------------------
+struct Refl {
+    int h, k, l, m;
+    double d0;
+    std::vector<short> eq;   // 3 per equivalent
+};
+
+bool inv_dsq_form(const double cell[6], double f[6]) {   // crystal.js:326-366
+    double a = cell[0], b = cell[1] > 0 ? cell[1] : a, c = cell[2] > 0 ? cell[2] : a;
+    if (!(a > 0) || !(b > 0) || !(c > 0)) return false;
+    double al = cell[3] * D2R, be = cell[4] * D2R, ga = cell[5] * D2R;
+    double ca = std::cos(al), cb = std::cos(be), cg = std::cos(ga);
+    double sa = std::sin(al), sb = std::sin(be), sg = std::sin(ga);
+    double vf = 1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg;
+    if (vf <= 1e-9 || std::fabs(sa) < 1e-9 || std::fabs(sb) < 1e-9 || std::fabs(sg) < 1e-9) return false;
+    double V = a * b * c * std::sqrt(vf);
+    double as = b * c * sa / V, bs = a * c * sb / V, cs = a * b * sg / V;
+    double cA = (cb * cg - ca) / (sb * sg), cB = (ca * cg - cb) / (sa * sg), cG = (ca * cb - cg) / (sa * sb);
+    f[0] = as * as; f[1] = bs * bs; f[2] = cs * cs;
+    f[3] = 2 * as * bs * cG; f[4] = 2 * bs * cs * cA; f[5] = 2 * as * cs * cB;
+    return true;
+}
+
+inline double rnd(double v) { return std::floor(v + 0.5); }
+
+std::vector<Refl> enumerate_reflections(const double cell[6], int n_ops, const int32_t *R, const double *T, double dmin,
+                                        const int *box) {
+    std::vector<Refl> out;
+    double f[6];
+    if (!inv_dsq_form(cell, f) || !(dmin > 0)) return out;
+    double lim = 1 / (dmin * dmin) * (1 + 1e-9);
+    // Laue group: +R and -R of every operator, first occurrence kept (row-vector action h R)
+    std::vector<std::array<int, 9>> rots;
+    for (int i = 0; i < n_ops; ++i)
+        for (int s = 0; s < 2; ++s) {
+            std::array<int, 9> r;
+            for (int x = 0; x < 9; ++x) r[x] = s ? -R[9 * i + x] : R[9 * i + x];
+            if (std::find(rots.begin(), rots.end(), r) == rots.end()) rots.push_back(r);
+        }
+    double cb = cell[1] > 0 ? cell[1] : cell[0], cc = cell[2] > 0 ? cell[2] : cell[0];
+    int hmax = box ? box[0] : (int)std::floor(cell[0] / dmin);
+    int kmax = box ? box[1] : (int)std::floor(cb / dmin);
+    int lmax = box ? box[2] : (int)std::floor(cc / dmin);
+    std::vector<std::array<int, 3>> orbit;
+    for (int h = -hmax; h <= hmax; ++h)
+        for (int k = -kmax; k <= kmax; ++k)
+            for (int l = -lmax; l <= lmax; ++l) {
+                if (!h && !k && !l) continue;
+                double q = h * h * f[0] + k * k * f[1] + l * l * f[2] + h * k * f[3] + k * l * f[4] + h * l * f[5];
+                if (!(q > 0) || (!box && q > lim)) continue;
+                // canonical member: most non-negative indices, then the largest (h, k, l)
+                int bh = -1000000000, bk = 0, bl = 0, bp = -1;
+                orbit.clear();
+                for (const auto &r : rots) {
+                    int v0 = h * r[0] + k * r[3] + l * r[6], v1 = h * r[1] + k * r[4] + l * r[7], v2 = h * r[2] + k * r[5] + l * r[8];
+                    int np = (v0 >= 0) + (v1 >= 0) + (v2 >= 0);
+                    std::array<int, 3> v{v0, v1, v2};
+                    if (std::find(orbit.begin(), orbit.end(), v) == orbit.end()) orbit.push_back(v);
+                    if (np > bp || (np == bp && (v0 > bh || (v0 == bh && (v1 > bk || (v1 == bk && v2 > bl)))))) {
+                        bp = np; bh = v0; bk = v1; bl = v2;
+                    }
+                }
+                if (bh != h || bk != k || bl != l) continue;
+                // systematic absence: some (R, t) with h R = h and h.t not an integer
+                bool allowed = true;
+                for (int i = 0; i < n_ops && allowed; ++i) {
+                    const int32_t *r = R + 9 * i;
+                    if (h * r[0] + k * r[3] + l * r[6] != h || h * r[1] + k * r[4] + l * r[7] != k || h * r[2] + k * r[5] + l * r[8] != l) continue;
+                    double p = h * T[3 * i] + k * T[3 * i + 1] + l * T[3 * i + 2];
+                    if (std::fabs(p - rnd(p)) > 1e-6) allowed = false;
+                }
+                if (!allowed) continue;
+                Refl rf;
+                rf.h = h; rf.k = k; rf.l = l; rf.m = (int)orbit.size(); rf.d0 = 1 / std::sqrt(q);
+                for (auto &v : orbit) { rf.eq.push_back((short)v[0]); rf.eq.push_back((short)v[1]); rf.eq.push_back((short)v[2]); }
+                out.push_back(std::move(rf));
+            }
+    std::sort(out.begin(), out.end(), [](const Refl &a, const Refl &b) {
+        double d = b.d0 - a.d0;
+        if (d != 0 && !std::isnan(d)) return d < 0;
+        if (a.h != b.h) return a.h > b.h;
+        if (a.k != b.k) return a.k > b.k;
+        return a.l > b.l;
+    });
+    return out;
+}
+
+// coset representatives modulo the centring translations (rietveld_core.js:164-179)
+std::vector<int> coset_reps(int n_ops, const int32_t *R, const double *T) {
+    std::vector<std::array<double, 3>> C;
+    for (int i = 0; i < n_ops; ++i) {
+        const int32_t *r = R + 9 * i;
+        if (r[0] == 1 && r[4] == 1 && r[8] == 1 && !r[1] && !r[2] && !r[3] && !r[5] && !r[6] && !r[7])
+            C.push_back({T[3 * i], T[3 * i + 1], T[3 * i + 2]});
+    }
+    if (C.empty()) C.push_back({0, 0, 0});
+    std::vector<int> reps;
+    for (int i = 0; i < n_ops; ++i) {
+        bool dup = false;
+        for (int j : reps) {
+            if (std::memcmp(R + 9 * i, R + 9 * j, 9 * sizeof(int32_t))) continue;
+            for (auto &c : C) {
+                bool same = true;
+                for (int x = 0; x < 3; ++x) {
+                    double d = T[3 * i + x] - T[3 * j + x] - c[x];
+                    if (std::fabs(d - rnd(d)) > 1e-6) { same = false; break; }
+                }
+                if (same) { dup = true; break; }
+            }
+            if (dup) break;
+        }
+        if (!dup) reps.push_back(i);
+    }
+    return reps;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// the plan
+// ---------------------------------------------------------------------------------
+struct pt_plan {
+    int nv = 0, chains = 0, stride = 0, np = 0, nr = 0, n_params = 0, steps_per_sweep = 0, uniform = 0;
+    int device = -1;
+    int block = 512, ch = 256;
+    size_t dyn = 0, plain_dyn = 0;
+    std::vector<std::vector<int32_t>> phase_hklm;   // per phase (h,k,l,m)
+    ptk::DevPlan P{};
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    // device scratch
+    double *d_x = nullptr, *d_chi2 = nullptr, *d_partial = nullptr, *d_y = nullptr;
+    size_t cap_trials = 0;
+    void *d_plain = nullptr;
+    ptk::LineRec *d_rec = nullptr;
+    int4 *d_span = nullptr;
+    bool states_set = false;
+    unsigned long long step_counter = 0, sweep_counter = 0;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t ev_total[2] = {nullptr, nullptr};
+    double last_total_ms = 0, last_kernel_ms = 0;
+    int last_launches = 0;
+    double qlim = 0;
+};
+
+namespace {
+
+template <class T>
+int dalloc(pt_plan *pl, T **ptr, size_t count) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return fail(PT_ERR_DEVICE, "PT buffers exceed this GPU's memory (hipMalloc failed).");
+    pl->allocs.push_back(p);
+    *ptr = reinterpret_cast<T *>(p);
+    return 0;
+}
+template <class T>
+int upload(pt_plan *pl, T **ptr, const std::vector<T> &v) {
+    int rc = dalloc(pl, ptr, v.size());
+    if (rc) return rc;
+    if (!v.empty() && hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PT_ERR_DEVICE, "PT device upload failed.");
+    return 0;
+}
+
+int device_check(int *dev) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return fail(PT_ERR_DEVICE, "MI355X HIP device is unavailable in this process or context.");
+    int d = 0;
+    (void)hipGetDevice(&d);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) != hipSuccess) return fail(PT_ERR_DEVICE, "MI355X HIP device is unavailable (device query failed).");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PT_ERR_DEVICE, std::string("MI355X HIP device is unavailable: found ") + prop.gcnArchName + ", need gfx950.");
+    *dev = d;
+    return 0;
+}
+
+#define HIPCHECK(expr, what)                                                                    \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(PT_ERR_DEVICE, std::string("PT device lost during ") + what + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int phase_base_of(const pt_problem *pb, int q) {
+    int b = PT_P_PHASE0;
+    for (int i = 0; i < q; ++i) b += PT_PH_ATOM0 + PT_AT_STRIDE * pb->phases[i].n_atoms;
+    return b;
+}
+
+// range of a model parameter over the variable boxes: base + sum c [lo, hi]
+void param_range(const pt_problem *pb, int k, double &lo, double &hi) {
+    lo = hi = pb->param_base[k];
+    for (int t = 0; t < pb->n_terms; ++t) {
+        if (pb->term_param[t] != k) continue;
+        double c = pb->term_coeff[t];
+        int j = pb->term_var[t];
+        lo += c * (c > 0 ? pb->lo[j] : pb->hi[j]);
+        hi += c * (c > 0 ? pb->hi[j] : pb->lo[j]);
+    }
+}
+double param_value(const pt_problem *pb, int k) {
+    double v = pb->param_base[k];
+    for (int t = 0; t < pb->n_terms; ++t)
+        if (pb->term_param[t] == k) v += pb->term_coeff[t] * pb->x0[pb->term_var[t]];
+    return v;
+}
+
+// cellOf (rietveld_core.js:207-221 with crystal.js:76-119) from per-slot values
+void cell_from(const pt_phase &ph, const double v[6], double cell[6]) {
+    double a = v[0], b = v[1] > 0 ? v[1] : a, c = v[2] > 0 ? v[2] : a;
+    cell[0] = a; cell[1] = b; cell[2] = c; cell[3] = cell[4] = cell[5] = 90;
+    switch (ph.system) {
+    case PT_CUBIC: cell[1] = cell[2] = a; break;
+    case PT_TETRAGONAL: cell[1] = a; break;
+    case PT_HEXAGONAL: case PT_TRIGONAL: cell[1] = a; cell[5] = 120; break;
+    case PT_ORTHORHOMBIC: break;
+    case PT_MONOCLINIC: {
+        int s = ph.unique_axis == 0 ? 3 : ph.unique_axis == 2 ? 5 : 4;
+        cell[s] = std::isfinite(v[s]) ? v[s] : 90;
+        break;
+    }
+    default:
+        for (int s = 3; s < 6; ++s) cell[s] = std::isfinite(v[s]) ? v[s] : 90;
+    }
+}
+// free cell slots of a system (rietveld_core.js:207-215 cellParams)
+std::vector<int> cell_slots(const pt_phase &ph) {
+    switch (ph.system) {
+    case PT_CUBIC: return {0};
+    case PT_TETRAGONAL: case PT_HEXAGONAL: case PT_TRIGONAL: return {0, 2};
+    case PT_ORTHORHOMBIC: return {0, 1, 2};
+    case PT_MONOCLINIC: return {0, 1, 2, ph.unique_axis == 0 ? 3 : ph.unique_axis == 2 ? 5 : 4};
+    default: return {0, 1, 2, 3, 4, 5};
+    }
+}
+
+int validate(const pt_problem *pb) {
+    if (!pb) return fail(PT_ERR_ARG, "PT problem is missing.");
+    if (pb->n_points < 2 || !pb->tth || !pb->y_obs || !pb->weight) return fail(PT_ERR_ARG, "a dataset has fewer than two points in its fitted range");
+    for (int i = 0; i < pb->n_points; ++i) {
+        if (!std::isfinite(pb->tth[i]) || (i && !(pb->tth[i] > pb->tth[i - 1]))) return fail(PT_ERR_ARG, "PT 2-theta axis must be finite and strictly ascending.");
+        if (!std::isfinite(pb->y_obs[i]) || !std::isfinite(pb->weight[i])) return fail(PT_ERR_ARG, "PT observations must be finite.");
+    }
+    if (pb->n_phases < 1 || pb->n_phases > PTK_MAX_PHASES) return fail(PT_ERR_ARG, "PT supports 1 to 4 phases per dataset.");
+    if (pb->n_bkg < 0 || pb->n_bkg > PT_MAX_BKG) return fail(PT_ERR_ARG, "Choose 0 to 24 Chebyshev terms.");
+    int atoms = 0;
+    for (int q = 0; q < pb->n_phases; ++q) {
+        const pt_phase &ph = pb->phases[q];
+        if (ph.n_ops < 1 || !ph.op_r || !ph.op_t) return fail(PT_ERR_ARG, "a phase has no symmetry operators");
+        if (ph.profile_type < 1 || ph.profile_type > 3) return fail(PT_ERR_ARG, "unknown profile type");
+        if (ph.n_atoms < 0 || (ph.n_atoms && !ph.atoms)) return fail(PT_ERR_ARG, "a phase has an invalid atom list");
+        for (int j = 0; j < ph.n_atoms; ++j)
+            if (ph.atoms[j].n_gauss < 0 || ph.atoms[j].n_gauss > PT_MAX_GAUSS) return fail(PT_ERR_ARG, "a scattering factor has more than 5 Gaussians");
+        atoms += ph.n_atoms;
+    }
+    if (atoms > PTK_MAX_ATOMS) return fail(PT_ERR_BUDGET, "PT has more than 160 atoms; using the CPU path.");
+    int expect = phase_base_of(pb, pb->n_phases);
+    if (pb->n_params != expect || !pb->param_base) return fail(PT_ERR_ARG, "PT model-parameter vector has the wrong length.");
+    if (expect > PTK_MAX_PARAMS) return fail(PT_ERR_BUDGET, "PT has more than 512 model parameters.");
+    if (pb->n_vars < 1 || pb->n_vars > PTK_MAX_VARS) return fail(PT_ERR_ARG, "PT needs 1 to 256 refined variables.");
+    if (!pb->x0 || !pb->lo || !pb->hi || !pb->scale) return fail(PT_ERR_ARG, "PT variables are missing.");
+    for (int t = 0; t < pb->n_terms; ++t)
+        if (pb->term_param[t] < 0 || pb->term_param[t] >= pb->n_params || pb->term_var[t] < 0 || pb->term_var[t] >= pb->n_vars || !std::isfinite(pb->term_coeff[t]))
+            return fail(PT_ERR_ARG, "PT parameter link out of range.");
+    if (pb->chains < 1 || pb->chains > 65536) return fail(PT_ERR_ARG, "invalid parallel-tempering settings");
+    if (!(pb->max_temp >= 1) || !std::isfinite(pb->max_temp)) return fail(PT_ERR_ARG, "invalid parallel-tempering settings");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_abi_version(void) { return PT_ABI_VERSION; }
+const char *pt_last_error(void) { return g_err.c_str(); }
+int pt_phase_base(const pt_problem *prob, int phase) {
+    if (!prob || phase < 0 || phase > prob->n_phases) return fail(PT_ERR_ARG, "phase index out of range");
+    return phase_base_of(prob, phase);
+}
+
+void pt_plan_destroy(pt_plan *pl) {
+    if (!pl) return;
+    if (pl->device >= 0) {
+        (void)hipSetDevice(pl->device);
+        if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+        for (void *p : pl->allocs) (void)hipFree(p);
+        for (auto e : pl->ev) (void)hipEventDestroy(e);
+        if (pl->ev_total[0]) (void)hipEventDestroy(pl->ev_total[0]);
+        if (pl->ev_total[1]) (void)hipEventDestroy(pl->ev_total[1]);
+        if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    }
+    delete pl;
+}
+
+static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
+    if (!out) return fail(PT_ERR_ARG, "output handle is NULL");
+    *out = nullptr;
+    int rc = validate(pb);
+    if (rc) return rc;
+    pt_plan *pl = new pt_plan();
+    pl->nv = pb->n_vars;
+    pl->chains = pb->chains;
+    pl->stride = 3 * pb->n_vars + PT_S_COUNT;
+    pl->np = pb->n_points;
+    pl->n_params = pb->n_params;
+    pl->steps_per_sweep = pb->steps_per_sweep > 0 ? pb->steps_per_sweep : pb->n_vars;
+    const int n = pb->n_points;
+
+    // limits-based dmin (js/pt_gpu.js:104-106, rietveld_core.js:238-245)
+    double mag[3];
+    for (int s = 0; s < 3; ++s) {
+        double lo, hi;
+        param_range(pb, s, lo, hi);
+        mag[s] = std::max(std::fabs(lo), std::fabs(hi));
+    }
+    double lambda = pb->lambda1;
+    if (pb->ratio > 1e-6 && pb->lambda2 > 0) lambda = std::min(lambda, pb->lambda2);
+    double margin = 10.0 + mag[0] + (mag[1] * .001 + mag[2]) * R2D;
+    double upper = std::min(179.999999, pb->tth[n - 1] + margin + .001);
+    double dmin = lambda / (2 * std::sin(upper * D2R / 2));
+    pl->qlim = 1 / (dmin * dmin) * (1 + 1e-9);
+
+    // envelope per phase (js/pt_gpu.js:107-131)
+    std::vector<int4> hklm;
+    std::vector<int> eq_off{0};
+    std::vector<short> eq;
+    std::vector<int> rep_r;
+    std::vector<double> rep_t;
+    std::vector<ptk::DevAtom> atoms;
+    ptk::DevPlan &P = pl->P;
+    for (int q = 0; q < pb->n_phases; ++q) {
+        const pt_phase &ph = pb->phases[q];
+        int pbase = phase_base_of(pb, q);
+        double v[6], lo6[6], hi6[6];
+        for (int s = 0; s < 6; ++s) {
+            v[s] = param_value(pb, pbase + s);
+            param_range(pb, pbase + s, lo6[s], hi6[s]);
+        }
+        double cell[6];
+        cell_from(ph, v, cell);
+        double maxes[3] = {hi6[0], hi6[1], hi6[2]};
+        if (ph.system == PT_CUBIC) maxes[1] = maxes[2] = maxes[0];
+        else if (ph.system == PT_TETRAGONAL || ph.system == PT_HEXAGONAL || ph.system == PT_TRIGONAL) maxes[1] = maxes[0];
+        // constrained systems: an unrefined b or c follows a
+        for (int s = 0; s < 3; ++s) if (!(maxes[s] > 0)) maxes[s] = cell[s];
+        for (int s = 0; s < 3; ++s)
+            if (!(maxes[s] > 0) || !std::isfinite(maxes[s])) { delete pl; return fail(PT_ERR_ARG, "PT cell limits need finite positive upper bounds."); }
+        int box[3];
+        double prod = 1;
+        for (int s = 0; s < 3; ++s) { box[s] = (int)std::floor(maxes[s] / dmin); prod *= 2.0 * box[s] + 1; }
+        if (prod > 1e6) { delete pl; return fail(PT_ERR_BUDGET, "PT reflection envelope exceeds the setup budget."); }
+        bool cellMoves = false, anglesMove = false;
+        for (int s : cell_slots(ph)) {
+            if (lo6[s] != hi6[s]) { cellMoves = true; if (s >= 3) anglesMove = true; }
+        }
+        bool orthogonal = cell[3] == 90 && cell[4] == 90 && cell[5] == 90;
+        bool monotone = !anglesMove && (orthogonal || ph.system == PT_HEXAGONAL || ph.system == PT_TRIGONAL);
+        double env[6] = {maxes[0], maxes[1], maxes[2], cell[3], cell[4], cell[5]};
+        std::vector<Refl> list = !cellMoves ? enumerate_reflections(cell, ph.n_ops, ph.op_r, ph.op_t, dmin, nullptr)
+                                 : monotone ? enumerate_reflections(env, ph.n_ops, ph.op_r, ph.op_t, dmin, nullptr)
+                                            : enumerate_reflections(cell, ph.n_ops, ph.op_r, ph.op_t, dmin, box);
+        std::vector<int> reps = coset_reps(ph.n_ops, ph.op_r, ph.op_t);
+        ptk::DevPhase &dp = P.phase[q];
+        dp.system = ph.system;
+        dp.unique_axis = ph.unique_axis;
+        dp.profile_type = ph.profile_type;
+        dp.n_reps = (int)reps.size();
+        dp.rep_off = (int)(rep_r.size() / 9);
+        dp.n_atoms = ph.n_atoms;
+        dp.atom_off = (int)atoms.size();
+        dp.refl_off = (int)hklm.size();
+        dp.refl_count = (int)list.size();
+        dp.param_base = pbase;
+        for (int s = 0; s < 3; ++s) dp.po_dir[s] = ph.po_dir[s];
+        if (dp.rep_off + dp.n_reps > PTK_MAX_REPS_TOTAL) { delete pl; return fail(PT_ERR_BUDGET, "PT has more than 64 coset representatives."); }
+        for (int r : reps) {
+            for (int x = 0; x < 9; ++x) rep_r.push_back(ph.op_r[9 * r + x]);
+            for (int x = 0; x < 3; ++x) rep_t.push_back(ph.op_t[3 * r + x]);
+        }
+        for (int j = 0; j < ph.n_atoms; ++j) {
+            const pt_atom &a = ph.atoms[j];
+            ptk::DevAtom d{};
+            for (int x = 0; x < 3; ++x) { d.r0[x] = a.r0[x]; for (int y = 0; y < 3; ++y) d.basis[x][y] = a.basis[x][y]; }
+            d.free_mask = a.free_mask;
+            d.n_gauss = a.n_gauss;
+            for (int g = 0; g < PT_MAX_GAUSS; ++g) { d.ff_a[g] = (float)a.ff_a[g]; d.ff_b[g] = (float)a.ff_b[g]; }
+            d.ff_c = (float)a.ff_c;
+            d.w0 = (float)((a.mult != 0 ? a.mult : 1) / (double)reps.size());
+            d.b_neutron = (float)a.b_neutron;
+            d.fp = (float)a.fp;
+            d.fpp = (float)a.fpp;
+            atoms.push_back(d);
+        }
+        std::vector<int32_t> rows;
+        for (auto &r : list) {
+            hklm.push_back(make_int4(r.h, r.k, r.l, r.m));
+            for (short s : r.eq) eq.push_back(s);
+            eq_off.push_back((int)(eq.size() / 3));
+            rows.push_back(r.h); rows.push_back(r.k); rows.push_back(r.l); rows.push_back(r.m);
+        }
+        pl->phase_hklm.push_back(std::move(rows));
+    }
+    pl->nr = (int)hklm.size();
+    if (!pl->nr) { delete pl; return fail(PT_ERR_ARG, "No reflections for PT."); }
+
+    // grid
+    double t0 = pb->tth[0], step = (pb->tth[n - 1] - pb->tth[0]) / (n - 1);
+    int uniform = 1;
+    for (int i = 0; i < n && uniform; ++i)
+        if (std::fabs(pb->tth[i] - (t0 + i * step)) > 1e-9 * step) uniform = 0;
+    pl->uniform = uniform;
+    std::vector<float2> tff(n), obs(n);
+    for (int i = 0; i < n; ++i) {
+        float hi = (float)pb->tth[i];
+        tff[i] = make_float2(hi, (float)(pb->tth[i] - (double)hi));
+        obs[i] = make_float2((float)pb->y_obs[i], (float)pb->weight[i]);
+    }
+    // CSR of the links by parameter
+    std::vector<int> term_off(pb->n_params + 1, 0), term_var;
+    std::vector<double> term_coeff;
+    for (int k = 0; k < pb->n_params; ++k) {
+        for (int t = 0; t < pb->n_terms; ++t)
+            if (pb->term_param[t] == k) { term_var.push_back(pb->term_var[t]); term_coeff.push_back(pb->term_coeff[t]); }
+        term_off[k + 1] = (int)term_var.size();
+    }
+
+    if (!with_device) { *out = pl; return 0; }
+    rc = device_check(&pl->device);
+    if (rc) { delete pl; return rc; }
+    (void)hipSetDevice(pl->device);
+    if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess) { delete pl; return fail(PT_ERR_DEVICE, "PT stream creation failed."); }
+    (void)hipEventCreate(&pl->ev_total[0]);
+    (void)hipEventCreate(&pl->ev_total[1]);
+
+    P.np = n; P.nr = pl->nr; P.nv = pb->n_vars; P.n_params = pb->n_params; P.n_phases = pb->n_phases; P.n_bkg = pb->n_bkg;
+    P.chains = pb->chains; P.stride = pl->stride; P.steps_per_sweep = pl->steps_per_sweep;
+    P.uniform = uniform; P.radiation = pb->radiation;
+    P.doublet = (pb->ratio > 1e-6 && pb->lambda2 > 1e-6 && std::fabs(pb->lambda1 - pb->lambda2) > 1e-6) ? 1 : 0;
+    P.t0 = t0; P.step = step;
+    P.step_hi = (float)step; P.step_lo = (float)(step - (double)P.step_hi);
+    P.tlo = pb->tth[0]; P.thi = pb->tth[n - 1];
+    P.lambda1 = pb->lambda1 != 0 ? pb->lambda1 : 1.54056; P.lambda2 = pb->lambda2; P.ratio = pb->ratio;
+    P.pol_k = pb->pol_k; P.min_fwhm = pb->min_fwhm > 0 ? pb->min_fwhm : 1e-3;
+    P.qlim = pl->qlim;
+    P.guard_lo = std::tan(std::max(1e-6, std::min(179.999999, pb->tth[0])) * kPi / 360);
+    P.guard_hi = std::tan(std::max(1e-6, std::min(179.999999, pb->tth[n - 1])) * kPi / 360);
+    P.max_temp = pb->max_temp;
+    P.seed = pb->seed;
+
+    std::vector<double> tth(pb->tth, pb->tth + n), base(pb->param_base, pb->param_base + pb->n_params);
+    std::vector<double> vlo(pb->lo, pb->lo + pb->n_vars), vhi(pb->hi, pb->hi + pb->n_vars), vsc(pb->scale, pb->scale + pb->n_vars);
+    if (eq.empty()) eq.push_back(0);
+    double *d_tth; float2 *d_tff, *d_obs; int4 *d_hklm; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
+    short *d_eq; double *d_rep_t, *d_base, *d_coeff, *d_lo, *d_hi, *d_sc, *d_state; ptk::DevAtom *d_atoms;
+    if ((rc = upload(pl, &d_tth, tth)) || (rc = upload(pl, &d_tff, tff)) || (rc = upload(pl, &d_obs, obs)) ||
+        (rc = upload(pl, &d_hklm, hklm)) || (rc = upload(pl, &d_eq_off, eq_off)) || (rc = upload(pl, &d_eq, eq)) ||
+        (rc = upload(pl, &d_rep_r, rep_r)) || (rc = upload(pl, &d_rep_t, rep_t)) || (rc = upload(pl, &d_atoms, atoms)) ||
+        (rc = upload(pl, &d_base, base)) || (rc = upload(pl, &d_term_off, term_off)) || (rc = upload(pl, &d_term_var, term_var)) ||
+        (rc = upload(pl, &d_coeff, term_coeff)) || (rc = upload(pl, &d_lo, vlo)) || (rc = upload(pl, &d_hi, vhi)) ||
+        (rc = upload(pl, &d_sc, vsc)) || (rc = dalloc(pl, &d_state, (size_t)pl->chains * pl->stride))) {
+        pt_plan_destroy(pl);
+        return rc;
+    }
+    P.tth = d_tth; P.tff = d_tff; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
+    P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
+    P.term_var = d_term_var; P.term_coeff = d_coeff; P.vlo = d_lo; P.vhi = d_hi; P.vscale = d_sc; P.state = d_state;
+
+    // LDS: line-record chunk + y[np]; keep two workgroups per CU when the pattern allows
+    size_t stat = ptk::shared_static_bytes() + (3 * PTK_MAX_VARS + PTK_S_COUNT_K) * sizeof(double);
+    size_t ybytes = (size_t)n * sizeof(float);
+    const size_t rec_bytes = sizeof(ptk::LineRec) + sizeof(int4);
+    const size_t lds_max = 160 * 1024;
+    if (stat + ybytes + 64 * rec_bytes > lds_max) { pt_plan_destroy(pl); return fail(PT_ERR_BUDGET, "PT pattern has too many points for one workgroup (LDS)."); }
+    size_t target = 80 * 1024;
+    long want = 2L * pl->nr;
+    long room = (long)((target > stat + ybytes ? target - stat - ybytes : 0) / rec_bytes);
+    if (room < 128) room = (long)((lds_max - stat - ybytes) / rec_bytes);
+    long chl = std::min(want, room);
+    chl = std::max(64L, (chl / 64) * 64);
+    pl->ch = (int)chl;
+    pl->dyn = rec_bytes * pl->ch + ybytes;
+    pl->plain_dyn = rec_bytes * pl->ch + 256 * sizeof(float);
+    pl->block = 512;
+    HIPCHECK(ptk::set_lds_limits(std::max(pl->dyn, pl->plain_dyn)), "kernel setup");
+    *out = pl;
+    return 0;
+}
+
+int pt_plan_create(const pt_problem *pb, pt_plan **out) { return create_impl(pb, out, true); }
+int pt_plan_compile(const pt_problem *pb, pt_plan **out) { return create_impl(pb, out, false); }
+
+#define NEED_DEVICE(pl) \
+    do { if ((pl)->device < 0) return fail(PT_ERR_STATE, "PT plan was compiled without a device."); } while (0)
+
+int pt_plan_info_get(const pt_plan *pl, pt_plan_info *info) {
+    if (!pl || !info) return fail(PT_ERR_ARG, "NULL argument");
+    info->nv = pl->nv; info->chains = pl->chains; info->stride = pl->stride; info->point_count = pl->np;
+    info->reflection_count = pl->nr; info->line_count = 2 * pl->nr; info->steps_per_sweep = pl->steps_per_sweep;
+    info->n_params = pl->n_params; info->uniform_grid = pl->uniform; info->device_ok = pl->device >= 0;
+    return 0;
+}
+
+int pt_plan_reflections(const pt_plan *pl, int phase, int32_t *hklm, int cap) {
+    if (!pl || phase < 0 || phase >= (int)pl->phase_hklm.size()) return fail(PT_ERR_ARG, "phase index out of range");
+    const auto &rows = pl->phase_hklm[phase];
+    int n = (int)(rows.size() / 4);
+    if (hklm) std::memcpy(hklm, rows.data(), sizeof(int32_t) * 4 * std::min(n, std::max(cap, 0)));
+    return n;
+}
+
+static int ensure_trials(pt_plan *pl, size_t ntrials) {
+    if (ntrials <= pl->cap_trials) return 0;
+    size_t cap = std::max<size_t>(ntrials, 64);
+    int rc;
+    if ((rc = dalloc(pl, &pl->d_x, cap * pl->nv)) || (rc = dalloc(pl, &pl->d_chi2, cap))) return rc;
+    pl->cap_trials = cap;
+    return 0;
+}
+
+int pt_set_states(pt_plan *pl, const double *x, int nvec) {
+    if (!pl || !x || nvec < 1) return fail(PT_ERR_ARG, "PT initial vectors are missing.");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    std::vector<double> st((size_t)pl->chains * pl->stride, 0.0);
+    const int nv = pl->nv;
+    for (int c = 0; c < pl->chains; ++c) {
+        double *b = st.data() + (size_t)c * pl->stride;
+        const double *v = x + (size_t)(c % nvec) * nv;
+        for (int j = 0; j < nv; ++j) b[j] = b[nv + j] = b[2 * nv + j] = v[j];
+        b[3 * nv + PT_S_COST] = 3e38;
+        b[3 * nv + PT_S_BEST] = 3e38;
+        b[3 * nv + PT_S_STEP] = -1;
+        b[3 * nv + PT_S_VALID] = 1;
+    }
+    HIPCHECK(hipMemcpyAsync(pl->P.state, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, pl->stream), "state upload");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "state upload");
+    pl->states_set = true;
+    pl->step_counter = 0;
+    pl->sweep_counter = 0;
+    return 0;
+}
+
+static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2, bool time_it) {
+    int rc = ensure_trials(pl, ntrials);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * ntrials * pl->nv, hipMemcpyHostToDevice, pl->stream), "trial upload");
+    if (time_it) HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+    HIPCHECK(ptk::launch_eval_batch(pl->P, pl->d_x, pl->d_chi2, ntrials, ntrials, pl->block, pl->ch, pl->dyn, pl->stream), "batched evaluation");
+    if (time_it) HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
+    HIPCHECK(hipMemcpyAsync(chi2, pl->d_chi2, sizeof(double) * ntrials, hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "batched evaluation");
+    if (time_it) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
+        pl->last_total_ms = pl->last_kernel_ms = ms;
+        pl->last_launches = 1;
+    }
+    return 0;
+}
+
+int pt_costs(pt_plan *pl, const double *x, int nvec, double *chi2) {
+    if (!pl || !x || !chi2 || nvec < 1) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    int rc = pt_set_states(pl, x, nvec);
+    if (rc) return rc;
+    std::vector<double> X((size_t)pl->chains * pl->nv);
+    for (int c = 0; c < pl->chains; ++c) std::memcpy(&X[(size_t)c * pl->nv], x + (size_t)(c % nvec) * pl->nv, sizeof(double) * pl->nv);
+    return eval_batched(pl, X.data(), pl->chains, chi2, false);
+}
+
+int pt_batch(pt_plan *pl, int count, int bootstrap, double *state_out) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    NEED_DEVICE(pl);
+    if (!pl->states_set) return fail(PT_ERR_STATE, "PT batch before initial state.");
+    if (count < 0) return fail(PT_ERR_ARG, "negative sweep count");
+    (void)hipSetDevice(pl->device);
+    int launches = bootstrap ? 1 : count;
+    if ((int)pl->ev.size() < 2 * launches) {
+        for (auto e : pl->ev) (void)hipEventDestroy(e);
+        pl->ev.assign(2 * launches, nullptr);
+        for (auto &e : pl->ev) (void)hipEventCreate(&e);
+    }
+    HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+    if (bootstrap) {
+        HIPCHECK(hipEventRecord(pl->ev[0], pl->stream), "timing");
+        HIPCHECK(ptk::launch_pt_sweep(pl->P, 1, 1, pl->block, pl->ch, pl->dyn, pl->step_counter, pl->stream), "bootstrap");
+        HIPCHECK(hipEventRecord(pl->ev[1], pl->stream), "timing");
+    } else {
+        for (int j = 0; j < count; ++j) {
+            HIPCHECK(hipEventRecord(pl->ev[2 * j], pl->stream), "timing");
+            HIPCHECK(ptk::launch_pt_sweep(pl->P, pl->steps_per_sweep, 0, pl->block, pl->ch, pl->dyn, pl->step_counter, pl->stream), "sweep");
+            HIPCHECK(hipEventRecord(pl->ev[2 * j + 1], pl->stream), "timing");
+            pl->step_counter += pl->steps_per_sweep;
+            HIPCHECK(ptk::launch_exchange(pl->P, (int)(pl->sweep_counter % 2), pl->sweep_counter, pl->stream), "exchange");
+            pl->sweep_counter++;
+        }
+    }
+    HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
+    if (state_out)
+        HIPCHECK(hipMemcpyAsync(state_out, pl->P.state, sizeof(double) * pl->chains * pl->stride, hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "sweeps");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
+    pl->last_total_ms = ms;
+    double k = 0;
+    for (int j = 0; j < launches; ++j) {
+        float m = 0;
+        (void)hipEventElapsedTime(&m, pl->ev[2 * j], pl->ev[2 * j + 1]);
+        k += m;
+    }
+    pl->last_kernel_ms = k;
+    pl->last_launches = launches;
+    return 0;
+}
+
+int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
+    if (!pl || !x || !chi2 || ntrials < 1) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    if (form == PT_FORM_BATCHED) return eval_batched(pl, x, ntrials, chi2, true);
+    if (form == PT_FORM_PLAIN) {
+        int rc = ensure_trials(pl, ntrials);
+        if (rc) return rc;
+        if (!pl->d_plain) {
+            int nblk = (pl->np + 255) / 256;
+            if ((rc = dalloc(pl, (char **)&pl->d_plain, ptk::plain_scratch_bytes())) || (rc = dalloc(pl, &pl->d_rec, 2 * (size_t)pl->nr)) ||
+                (rc = dalloc(pl, &pl->d_span, 2 * (size_t)pl->nr)) || (rc = dalloc(pl, &pl->d_partial, (size_t)nblk)))
+                return rc;
+        }
+        HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * ntrials * pl->nv, hipMemcpyHostToDevice, pl->stream), "trial upload");
+        HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+        for (int t = 0; t < ntrials; ++t)
+            HIPCHECK(ptk::launch_plain(pl->P, pl->d_x + (size_t)t * pl->nv, pl->d_plain, pl->d_rec, pl->d_span, pl->d_partial,
+                                       pl->d_chi2 + t, pl->ch, pl->stream), "plain evaluation");
+        HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
+        HIPCHECK(hipMemcpyAsync(chi2, pl->d_chi2, sizeof(double) * ntrials, hipMemcpyDeviceToHost, pl->stream), "readback");
+        HIPCHECK(hipStreamSynchronize(pl->stream), "plain evaluation");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
+        pl->last_total_ms = pl->last_kernel_ms = ms;
+        pl->last_launches = 4 * ntrials;
+        return 0;
+    }
+    return fail(PT_ERR_STATE, "PT evaluation form not available for this plan.");
+}
+
+int pt_pattern(pt_plan *pl, const double *x, double *ycalc) {
+    if (!pl || !x || !ycalc) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    int rc = ensure_trials(pl, 1);
+    if (rc) return rc;
+    if (!pl->d_y && (rc = dalloc(pl, &pl->d_y, (size_t)pl->np))) return rc;
+    HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
+    HIPCHECK(ptk::launch_pattern(pl->P, pl->d_x, pl->d_y, pl->block, pl->ch, pl->dyn, pl->stream), "pattern");
+    HIPCHECK(hipMemcpyAsync(ycalc, pl->d_y, sizeof(double) * pl->np, hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "pattern");
+    return 0;
+}
+
+int pt_last_timing(const pt_plan *pl, double *total_ms, double *kernel_ms, int *launches) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    if (total_ms) *total_ms = pl->last_total_ms;
+    if (kernel_ms) *kernel_ms = pl->last_kernel_ms;
+    if (launches) *launches = pl->last_launches;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,587 @@
+// pt_kernels.hip -- gfx950 kernels of the PT trial evaluator.
+//
+//   k_eval_batch   one workgroup per trial vector: parameters -> cell metric -> per-line
+//                  geometry/profile/|F|^2 (LDS line records) -> banded profile sum over
+//                  64-point tiles (one wavefront per tile, lanes = points) -> compensated
+//                  weighted chi^2 (wavefront shuffles + LDS).      [PT_FORM_BATCHED]
+//   k_pt_sweep     the same evaluator inside a persistent per-replica loop: propose,
+//                  evaluate, Metropolis accept in FP64, `steps` times per launch; one
+//                  workgroup per replica (js/pt_gpu.js propose/geometry/columns/targets/
+//                  energy/accept, :361-364, fused).
+//   k_exchange     replica exchange between neighbouring temperatures (js/pt_gpu.js:365).
+//   k_plain_*      the plain single-trial path: one trial spread over the whole GPU
+//                  (setup, lines, points, reduce), one launch sequence per trial.
+//   k_pattern      background + phases of one vector (no chi^2), for pt_pattern().
+#include "pt_device.h"
+
+namespace ptk {
+
+// ---- per-workgroup shared state ----------------------------------------------------
+struct Shared {
+    double p[PTK_MAX_PARAMS];                 // model parameters of the trial
+    double metric[PTK_MAX_PHASES][6];         // reciprocal quadratic forms
+    double apos[PTK_MAX_ATOMS][3];            // atom positions (fractional)
+    float aocc[PTK_MAX_ATOMS], aB[PTK_MAX_ATOMS];
+    float prof[PTK_MAX_PHASES][16];           // profile slots (FP32)
+    float bkg[PTK_MAX_BKG_F];                 // Chebyshev coefficients (FP32)
+    int rep_r[PTK_MAX_REPS_TOTAL][9];
+    double rep_t[PTK_MAX_REPS_TOTAL][3];
+    float red_s[32], red_c[32];
+    int phase_ok[PTK_MAX_PHASES];
+    int cell_ok, prof_ok, go, accept;
+    double chi2;
+};
+
+__device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// stage the coset representatives of every phase in LDS (once per launch)
+__device__ inline void stage_reps(const DevPlan &P, Shared &sh) {
+    int total = 0;
+    for (int q = 0; q < P.n_phases; ++q) total += P.phase[q].n_reps;
+    for (int i = threadIdx.x; i < total * 9; i += blockDim.x) sh.rep_r[i / 9][i % 9] = P.rep_r[i];
+    for (int i = threadIdx.x; i < total * 3; i += blockDim.x) sh.rep_t[i / 3][i % 3] = P.rep_t[i];
+}
+
+// p = base + M x  (rietveld_core.js:1855-1859 apply, affine links of atomic_constraints.js)
+__device__ inline void materialize(const DevPlan &P, const double *x, Shared &sh) {
+    for (int k = threadIdx.x; k < P.n_params; k += blockDim.x) {
+        double v = P.base[k];
+        for (int t = P.term_off[k]; t < P.term_off[k + 1]; ++t) v += P.term_coeff[t] * x[P.term_var[t]];
+        sh.p[k] = v;
+    }
+}
+
+// per-trial setup after materialize (+barrier): metrics, atoms, profile slots, guards.
+// Ends with a barrier; sets sh.cell_ok and sh.prof_ok.
+__device__ inline void trial_setup(const DevPlan &P, Shared &sh) {
+    int tid = threadIdx.x, nt = blockDim.x;
+    if (tid < P.n_phases) {
+        const DevPhase &ph = P.phase[tid];
+        sh.phase_ok[tid] = phase_metric(ph, sh.p + ph.param_base, sh.metric[tid]) ? 1 : 0;
+    }
+    if (tid == 63 || (nt < 64 && tid == 0)) sh.prof_ok = profile_valid(P, sh.p) ? 1 : 0;
+    for (int i = tid; i < P.n_phases * 16; i += nt) {
+        int q = i >> 4, s = i & 15;
+        sh.prof[q][s] = (float)sh.p[P.phase[q].param_base + 8 + s];
+    }
+    for (int i = tid; i < P.n_bkg; i += nt) sh.bkg[i] = (float)sh.p[3 + i];
+    for (int q = 0; q < P.n_phases; ++q) {
+        const DevPhase &ph = P.phase[q];
+        for (int j = tid; j < ph.n_atoms; j += nt) {
+            int ga = ph.atom_off + j;
+            const DevAtom &A = P.atoms[ga];
+            const double *ap = sh.p + ph.param_base + 24 + 5 * j;
+            double r[3] = {A.r0[0], A.r0[1], A.r0[2]};      // sitePosition, rietveld_core.js:372-378
+            for (int f = 0; f < 3; ++f)
+                if (A.free_mask & (1 << f)) {
+                    double v = ap[f];
+                    r[0] += v * A.basis[f][0]; r[1] += v * A.basis[f][1]; r[2] += v * A.basis[f][2];
+                }
+            sh.apos[ga][0] = r[0]; sh.apos[ga][1] = r[1]; sh.apos[ga][2] = r[2];
+            sh.aocc[ga] = (float)ap[3];
+            sh.aB[ga] = (float)ap[4];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int ok = 1;
+        for (int q = 0; q < P.n_phases; ++q) ok &= sh.phase_ok[q];
+        sh.cell_ok = ok;
+    }
+    __syncthreads();
+}
+
+// |F|^2 of one reflection (rietveld_core.js:749-821): A^2 + B^2 + C^2 + D^2 over the atoms
+// and coset representatives; phases h.(R r + t) in FP64 reduced to turns, sincospi in FP32.
+__device__ inline float structure_f2(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k, int l,
+                                     float s2) {
+    float Ar = 0.f, Br = 0.f, Cr = 0.f, Dr = 0.f;
+    float q4 = 0.25f * s2;
+    for (int j = 0; j < ph.n_atoms; ++j) {
+        int ga = ph.atom_off + j;
+        const DevAtom &A = P.atoms[ga];
+        float f0;
+        if (P.radiation == 1) f0 = A.b_neutron;
+        else {
+            f0 = A.ff_c;
+            for (int i = 0; i < A.n_gauss; ++i) f0 += A.ff_a[i] * expf(-A.ff_b[i] * q4);
+        }
+        float dw = expf(-sh.aB[ga] * q4);
+        float wj = sh.aocc[ga] * (P.radiation == 1 ? f0 : f0 + A.fp) * dw * A.w0;
+        float wi = (P.radiation == 1 || !(A.fpp != 0.f)) ? 0.f : sh.aocc[ga] * A.fpp * dw * A.w0;
+        double x = sh.apos[ga][0], y = sh.apos[ga][1], z = sh.apos[ga][2];
+        float c = 0.f, s = 0.f;
+        for (int r = 0; r < ph.n_reps; ++r) {
+            const int *R = sh.rep_r[ph.rep_off + r];
+            const double *t = sh.rep_t[ph.rep_off + r];
+            int H0 = h * R[0] + k * R[3] + l * R[6], H1 = h * R[1] + k * R[4] + l * R[7], H2 = h * R[2] + k * R[5] + l * R[8];
+            double phs = (double)H0 * x + (double)H1 * y + (double)H2 * z + ((double)h * t[0] + (double)k * t[1] + (double)l * t[2]);
+            float fr = (float)(phs - floor(phs));
+            float sn, cs;
+            sincospif(2.0f * fr, &sn, &cs);
+            c += cs; s += sn;
+        }
+        Ar += wj * c; Br += wj * s; Cr += wi * c; Dr += wi * s;
+    }
+    return Ar * Ar + Br * Br + Cr * Cr + Dr * Dr;
+}
+
+// March-Dollase factor averaged over the equivalents (rietveld_core.js:824-843)
+__device__ inline float march_dollase(const DevPlan &P, const DevPhase &ph, const double *f, double r, int kref) {
+    if (!(r > 0.0) || fabs(r - 1.0) < 1e-12) return 1.0f;
+    const double *H = ph.po_dir;
+#define PTK_G(a0, a1, a2, b0, b1, b2) ((a0) * (b0) * f[0] + (a1) * (b1) * f[1] + (a2) * (b2) * f[2] + \
+    ((a0) * (b1) + (a1) * (b0)) * f[3] * 0.5 + ((a1) * (b2) + (a2) * (b1)) * f[4] * 0.5 + ((a0) * (b2) + (a2) * (b0)) * f[5] * 0.5)
+    double HH = PTK_G(H[0], H[1], H[2], H[0], H[1], H[2]);
+    if (!(HH > 0.0)) return 1.0f;
+    float r2 = (float)(r * r), ir = (float)(1.0 / r);
+    int e0 = P.eq_off[kref], e1 = P.eq_off[kref + 1];
+    float sum = 0.f;
+    for (int e = e0; e < e1; ++e) {
+        double a0 = P.eq[3 * e], a1 = P.eq[3 * e + 1], a2 = P.eq[3 * e + 2];
+        double hh = PTK_G(a0, a1, a2, a0, a1, a2);
+        double c = PTK_G(a0, a1, a2, H[0], H[1], H[2]) / sqrt(hh * HH);
+        float c2 = (float)fmin(1.0, c * c);
+        sum += powf(r2 * c2 + (1.0f - c2) * ir, -1.5f);
+    }
+#undef PTK_G
+    return sum / (float)(e1 - e0);
+}
+
+// write one line record: window [start, stop) of the axis, float-float centre, flanks
+__device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, float amp, LineRec &rec, int4 &span) {
+    span = make_int4(0, 0, 0, 0);
+    double lo = x0 - (double)pk.max_d, hi = x0 + (double)pk.max_d;
+    int start = lower_bound_t(P, lo);
+    if (start >= P.np) return;
+    int stop = upper_stop(P, start, hi);
+    if (stop <= start) return;
+    float xh = (float)x0;
+    rec.a = make_float4(xh, (float)(x0 - (double)xh), amp, pk.pedestal);
+    const float kG = -kCg * kLog2e;
+    rec.l = make_float4(kG * pk.ig[0] * pk.ig[0], 4.0f * pk.il[0] * pk.il[0], pk.cl[0], pk.cg[0]);
+    rec.r = make_float4(kG * pk.ig[1] * pk.ig[1], 4.0f * pk.il[1] * pk.il[1], pk.cl[1], pk.cg[1]);
+    span = make_int4(start, stop, __float_as_int(pk.asym), 0);
+}
+
+// Both lines (Kalpha1, Kalpha2) of envelope reflection k: Model.geometry (:719-736),
+// buildPeakContributions (profile.js:1013-1080), evalPhase heights (:855-897).
+__device__ inline void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span) {
+    span[0] = make_int4(0, 0, 0, 0);
+    span[1] = make_int4(0, 0, 0, 0);
+    int q = 0;
+    while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
+    const DevPhase &ph = P.phase[q];
+    if (!sh.phase_ok[q]) return;
+    int4 hk = P.hklm[k];
+    const double *f = sh.metric[q];
+    double h = hk.x, kk = hk.y, l = hk.z;
+    double qv = h * h * f[0] + kk * kk * f[1] + l * l * f[2] + h * kk * f[3] + kk * l * f[4] + h * l * f[5];
+    if (!(qv > 0.0) || qv > P.qlim) return;
+    double s = sqrt(qv), sinT = P.lambda1 * s * 0.5;
+    if (!(sinT < 1.0)) return;
+    double bragg = 2.0 * asin(sinT) * kR2D;
+    float disp = (float)sh.p[1], transp = (float)sh.p[2];
+    double zero = sh.p[0];
+    double tth = bragg + (double)peak_shift(bragg, disp, transp);
+    if (!(tth > 0.0 && tth < 180.0)) return;                 // hasRenderablePosition
+    double base1 = tth + zero;
+    Prep pk1;
+    prepare_line(ph.profile_type, sh.prof[q], (float)P.min_fwhm, (float)base1, pk1);
+    if (!(pk1.area > 0.0f)) return;
+    float lp1 = lp_factor((float)base1, (float)P.pol_k);
+    float f2 = structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv);
+    const double *pp = sh.p + ph.param_base;
+    float po = march_dollase(P, ph, f, pp[7], k);
+    double S = pp[6];
+    if (!(S != 0.0) || isnan(S)) S = 0.0;
+    float height = (float)S * ((float)hk.w * lp1 * po / pk1.area * f2);
+    emit_line(P, pk1, base1, height, rec[0], span[0]);
+    if (P.doublet) {
+        double st2 = sin(bragg * kD2R * 0.5) * (P.lambda2 / P.lambda1);
+        if (fabs(st2) < 1.0) {
+            double b2 = 2.0 * asin(st2) * kR2D;
+            double base2 = b2 + (double)peak_shift(b2, disp, transp) + zero;
+            Prep pk2;
+            prepare_line(ph.profile_type, sh.prof[q], (float)P.min_fwhm, (float)base2, pk2);
+            float w2 = (float)P.ratio * pk1.area / pk2.area * (lp_factor((float)base2, (float)P.pol_k) / lp1);
+            emit_line(P, pk2, base2, height * w2, rec[1], span[1]);
+        }
+    }
+}
+
+// value of one line at one point (delta = t - centre), profile.js:733-759 inside the window
+__device__ __forceinline__ float line_value(const LineRec &L, float asym, float delta) {
+    if (asym != 0.0f && fabsf(delta) >= 1e-9f) {
+        float t = fmaxf_js(-0.95f, fminf_js(0.95f, asym * delta));
+        delta = delta / (1.0f - t);
+    }
+    float4 fl = delta < 0.0f ? L.l : L.r;
+    float d2 = delta * delta;
+    float g = __builtin_amdgcn_exp2f(fl.x * d2);
+    float lo = __builtin_amdgcn_rcpf(fmaf(fl.y, d2, 1.0f));
+    float v = fmaf(fl.z, lo, fmaf(fl.w, g, -L.a.w));
+    return v > 0.0f ? v : 0.0f;
+}
+
+// Banded profile sum of `nlines` LDS line records into y[] (LDS or global) for the 64-point
+// tiles tile0, tile0 + tstep, ...  One wavefront per tile; every lane owns one point and
+// walks the lines whose window meets the tile (ballot over the line spans), in line
+// order, so the sum is deterministic.
+__device__ inline void accumulate_tiles(const DevPlan &P, const LineRec *rec, const int4 *span, int nlines, float *y,
+                                        int tile0, int tstep) {
+    const int lane = lane_id();
+    const int ntiles = (P.np + 63) >> 6;
+    for (int tile = tile0; tile < ntiles; tile += tstep) {
+        const int t_lo = tile << 6, t_hi = t_lo + 64;
+        const int i = t_lo + lane;
+        const bool live = i < P.np;
+        float2 ti = live ? P.tff[i] : make_float2(0.f, 0.f);
+        float acc = 0.0f;
+        for (int lb = 0; lb < nlines; lb += 64) {
+            int L = lb + lane;
+            bool ov = false;
+            if (L < nlines) {
+                int4 sp = span[L];
+                ov = sp.x < t_hi && sp.y > t_lo;
+            }
+            unsigned long long mask = __ballot(ov);
+            while (mask) {
+                int b = __ffsll((long long)mask) - 1;
+                mask &= mask - 1ull;
+                int line = lb + b;
+                int4 sp = span[line];
+                const LineRec &R = rec[line];
+                if (i >= sp.x && i < sp.y) {
+                    float delta = (ti.x - R.a.x) + (ti.y - R.a.y);
+                    acc = fmaf(R.a.z, line_value(R, __int_as_float(sp.z), delta), acc);
+                }
+            }
+        }
+        if (live) y[i] += acc;
+    }
+}
+
+// Neumaier two-sum accumulate
+__device__ __forceinline__ void neumaier(float &s, float &c, float v) {
+    float t = s + v;
+    if (fabsf(s) >= fabsf(v)) c += (s - t) + v; else c += (v - t) + s;
+    s = t;
+}
+
+// Chebyshev background at point i (rietveld_core.js:647-653, 920-931), FP32 recurrence
+__device__ __forceinline__ float background_at(const DevPlan &P, const Shared &sh, int i) {
+    if (P.n_bkg <= 0) return 0.0f;
+    float z = (float)(P.thi > P.tlo ? 2.0 * (P.tth[i] - P.tlo) / (P.thi - P.tlo) - 1.0 : 0.0);
+    float a = 1.0f, b = z, y = sh.bkg[0];
+    if (P.n_bkg > 1) y = fmaf(sh.bkg[1], z, y);
+    for (int k = 2; k < P.n_bkg; ++k) {
+        float nx = 2.0f * z * b - a;
+        a = b; b = nx;
+        y = fmaf(sh.bkg[k], nx, y);
+    }
+    return y;
+}
+
+// chi^2 = sum_{w>0} w (y_obs - y_calc)^2 (rietveld_core.js:1885) over points [i0, i1)
+// with stride, Neumaier per lane, two-sum tree over the wavefront and the block.
+// Result valid in thread 0 (and written to sh.chi2).  ypk indexed by point.
+__device__ inline double chi2_block(const DevPlan &P, Shared &sh, const float *ypk, int i0, int i1) {
+    float s = 0.f, c = 0.f;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        float2 ow = P.obs[i];
+        if (!(ow.y > 0.0f)) continue;
+        float yc = background_at(P, sh, i) + ypk[i];
+        float r = ow.x - yc;
+        neumaier(s, c, ow.y * r * r);
+    }
+    // wavefront: combine (s, c) pairs with two-sum
+    for (int off = 32; off > 0; off >>= 1) {
+        float so = __shfl_xor(s, off), co = __shfl_xor(c, off);
+        float t = s + so;
+        float e = fabsf(s) >= fabsf(so) ? (s - t) + so : (so - t) + s;
+        s = t; c = c + co + e;
+    }
+    int w = wave_id(), nw = (blockDim.x + 63) >> 6;
+    if (lane_id() == 0) { sh.red_s[w] = s; sh.red_c[w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int k = 0; k < nw; ++k) tot += (double)sh.red_s[k] + (double)sh.red_c[k];
+        sh.chi2 = tot;
+    }
+    __syncthreads();
+    return sh.chi2;
+}
+
+// Full evaluation of the trial held in sh (after trial_setup): lines in chunks of `ch`
+// records, banded sum into ylds[np], then chi^2.  Returns chi^2 (all threads).
+__device__ inline double eval_trial(const DevPlan &P, Shared &sh, float *ylds, LineRec *rec, int4 *span, int ch) {
+    for (int i = threadIdx.x; i < P.np; i += blockDim.x) ylds[i] = 0.0f;
+    const int nwaves = (blockDim.x + 63) >> 6;
+    const int per = ch >> 1;   // reflections per chunk
+    for (int k0 = 0; k0 < P.nr; k0 += per) {
+        int kn = min(per, P.nr - k0);
+        __syncthreads();
+        for (int kk = threadIdx.x; kk < kn; kk += blockDim.x) build_reflection(P, sh, k0 + kk, rec + 2 * kk, span + 2 * kk);
+        __syncthreads();
+        accumulate_tiles(P, rec, span, 2 * kn, ylds, wave_id(), nwaves);
+    }
+    __syncthreads();
+    return chi2_block(P, sh, ylds, 0, P.np);
+}
+
+__device__ __forceinline__ void dyn_layout(int np, int ch, float *&ylds, LineRec *&rec, int4 *&span) {
+    extern __shared__ __align__(16) unsigned char dyn[];
+    rec = reinterpret_cast<LineRec *>(dyn);
+    span = reinterpret_cast<int4 *>(dyn + sizeof(LineRec) * ch);
+    ylds = reinterpret_cast<float *>(dyn + (sizeof(LineRec) + sizeof(int4)) * ch);
+}
+
+// ---- batched evaluation: one workgroup per trial ------------------------------------
+__global__ void __launch_bounds__(1024) k_eval_batch(DevPlan P, const double *__restrict__ X, double *__restrict__ chi2,
+                                                    int ntrials, int ch) {
+    __shared__ Shared sh;
+    float *ylds; LineRec *rec; int4 *span;
+    dyn_layout(P.np, ch, ylds, rec, span);
+    stage_reps(P, sh);
+    for (int t = blockIdx.x; t < ntrials; t += gridDim.x) {
+        __syncthreads();
+        materialize(P, X + (size_t)t * P.nv, sh);
+        __syncthreads();
+        trial_setup(P, sh);
+        double c = 3e38;
+        if (sh.cell_ok) c = eval_trial(P, sh, ylds, rec, span, ch);
+        if (threadIdx.x == 0) chi2[t] = (isfinite(c) && c < 3e38) ? c : 3e38;
+    }
+}
+
+// ---- calculated pattern of one vector -----------------------------------------------
+__global__ void __launch_bounds__(1024) k_pattern(DevPlan P, const double *__restrict__ X, double *__restrict__ yout,
+                                                 int ch) {
+    __shared__ Shared sh;
+    float *ylds; LineRec *rec; int4 *span;
+    dyn_layout(P.np, ch, ylds, rec, span);
+    stage_reps(P, sh);
+    __syncthreads();
+    materialize(P, X, sh);
+    __syncthreads();
+    trial_setup(P, sh);
+    if (!sh.cell_ok) {
+        for (int i = threadIdx.x; i < P.np; i += blockDim.x) yout[i] = NAN;
+        return;
+    }
+    eval_trial(P, sh, ylds, rec, span, ch);
+    for (int i = threadIdx.x; i < P.np; i += blockDim.x) yout[i] = (double)(background_at(P, sh, i) + ylds[i]);
+}
+
+// ---- persistent PT sweep: one workgroup per replica ----------------------------------
+// State layout per chain (stride 3nv+12) as js/pt_gpu.js.  `counter0` numbers the steps
+// for the random streams.  bootstrap: one step, no move, unconditional accept.
+__global__ void __launch_bounds__(1024) k_pt_sweep(DevPlan P, int steps, int bootstrap, int ch, unsigned long long counter0) {
+    __shared__ Shared sh;
+    __shared__ double xs[3 * PTK_MAX_VARS];      // current, proposal, best
+    __shared__ double sc[PTK_S_COUNT_K];
+    float *ylds; LineRec *rec; int4 *span;
+    dyn_layout(P.np, ch, ylds, rec, span);
+    const int c = blockIdx.x, nv = P.nv;
+    if (c >= P.chains) return;
+    double *st = P.state + (size_t)c * P.stride;
+    stage_reps(P, sh);
+    for (int j = threadIdx.x; j < 3 * nv; j += blockDim.x) xs[j] = st[j];
+    for (int j = threadIdx.x; j < PTK_S_COUNT_K; j += blockDim.x) sc[j] = st[3 * nv + j];
+    const double temp = pow(P.max_temp, (double)c / (double)(P.chains > 1 ? P.chains - 1 : 1));
+    double *x = xs, *prop = xs + nv, *best = xs + 2 * nv;
+    for (int step = 0; step < steps; ++step) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < nv; j += blockDim.x) prop[j] = x[j];
+        __syncthreads();
+        const double sweep = sc[2];
+        const bool boot = bootstrap || sweep < 0.0;
+        if (threadIdx.x == 0 && !boot) {
+            unsigned long long ctr = counter0 + (unsigned long long)step;
+            double u0 = uniform01(P.seed, c, ctr, 0), u1 = uniform01(P.seed, c, ctr, 1), u2 = uniform01(P.seed, c, ctr, 2);
+            double u3 = uniform01(P.seed, c, ctr, 3);
+            int j = (int)floor(u0 * nv);
+            if (j > nv - 1) j = nv - 1;
+            double z = sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            prop[j] = reflect_box(x[j] + z * P.vscale[j] * sqrt(temp), P.vlo[j], P.vhi[j]);
+            sc[8] = log(u3);
+        }
+        __syncthreads();
+        materialize(P, prop, sh);
+        __syncthreads();
+        trial_setup(P, sh);
+        if (threadIdx.x == 0) sh.go = (sh.cell_ok && (boot || sc[11] < 0.5 || sh.prof_ok)) ? 1 : 0;
+        __syncthreads();
+        const bool valid = sh.go != 0;
+        double cost = 3e38;
+        if (valid) cost = eval_trial(P, sh, ylds, rec, span, ch);
+        if (threadIdx.x == 0) {
+            if (!(isfinite(cost)) || cost > 3e38) cost = 3e38;
+            sc[7] = valid ? 1.0 : 0.0;
+            sc[10] = cost;
+            double old = sc[0];
+            bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
+            int take_best = 0;
+            if (acc) {
+                sc[0] = cost;
+                sc[11] = sh.prof_ok ? 1.0 : 0.0;
+                if (!boot) sc[3] += 1.0;
+                if (sh.prof_ok && (boot || cost < sc[1])) { sc[1] = cost; take_best = 1; }
+            }
+            if (!boot) sc[4] += 1.0;
+            sc[2] = sweep + 1.0;
+            sh.accept = acc ? 1 + take_best : 0;
+        }
+        __syncthreads();
+        if (sh.accept) {
+            for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+                x[j] = prop[j];
+                if (sh.accept == 2) best[j] = prop[j];
+            }
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 3 * nv; j += blockDim.x) st[j] = xs[j];
+    for (int j = threadIdx.x; j < PTK_S_COUNT_K; j += blockDim.x) st[3 * nv + j] = sc[j];
+}
+
+// ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
+__global__ void k_exchange(DevPlan P, int parity, unsigned long long counter) {
+    int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 2 * pair + parity;
+    if (c + 1 >= P.chains) return;
+    const int nv = P.nv;
+    double *a = P.state + (size_t)c * P.stride, *b = P.state + (size_t)(c + 1) * P.stride;
+    double den = (double)(P.chains > 1 ? P.chains - 1 : 1);
+    double t0 = pow(P.max_temp, (double)c / den), t1 = pow(P.max_temp, (double)(c + 1) / den);
+    double la = 0.5 * (1.0 / t0 - 1.0 / t1) * (a[3 * nv] - b[3 * nv]);
+    a[3 * nv + 6] += 1.0;
+    bool sw = la >= 0.0;
+    if (!sw) sw = log(uniform01(P.seed ^ 0x5bd1e995u, c, counter, 7)) < la;
+    if (sw) {
+        for (int j = 0; j < nv; ++j) { double v = a[j]; a[j] = b[j]; b[j] = v; }
+        double v = a[3 * nv]; a[3 * nv] = b[3 * nv]; b[3 * nv] = v;
+        v = a[3 * nv + 11]; a[3 * nv + 11] = b[3 * nv + 11]; b[3 * nv + 11] = v;
+        a[3 * nv + 5] += 1.0;
+    }
+}
+
+// ---- plain single-trial path: setup, lines, points, reduce ---------------------------
+// scratch layout (global): [0] Shared image of the trial (params, metrics, atoms, profile)
+struct PlainScratch {
+    Shared sh;
+};
+
+__global__ void __launch_bounds__(256) k_plain_setup(DevPlan P, const double *__restrict__ X, PlainScratch *scr) {
+    __shared__ Shared sh;
+    stage_reps(P, sh);
+    __syncthreads();
+    materialize(P, X, sh);
+    __syncthreads();
+    trial_setup(P, sh);
+    // copy the shared image out (byte-wise, all threads)
+    const int n = (int)(sizeof(Shared) / 4);
+    const int *src = reinterpret_cast<const int *>(&sh);
+    int *dst = reinterpret_cast<int *>(&scr->sh);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_plain_lines(DevPlan P, const PlainScratch *__restrict__ scr, LineRec *rec, int4 *span) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.nr) return;
+    if (!scr->sh.cell_ok) { span[2 * k] = make_int4(0, 0, 0, 0); span[2 * k + 1] = make_int4(0, 0, 0, 0); return; }
+    build_reflection(P, scr->sh, k, rec + 2 * k, span + 2 * k);
+}
+
+// each workgroup: 4 wavefronts = 4 tiles of 64 points; line records streamed through LDS
+__global__ void __launch_bounds__(256) k_plain_points(DevPlan P, const PlainScratch *__restrict__ scr,
+                                                     const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
+                                                     double *partial, int ch) {
+    __shared__ Shared sh;
+    extern __shared__ __align__(16) unsigned char dyn[];
+    LineRec *rec = reinterpret_cast<LineRec *>(dyn);
+    int4 *span = reinterpret_cast<int4 *>(dyn + sizeof(LineRec) * ch);
+    float *ytile = reinterpret_cast<float *>(dyn + (sizeof(LineRec) + sizeof(int4)) * ch);
+    // bring the trial image (bkg coefficients) in
+    for (int i = threadIdx.x; i < PTK_MAX_BKG_F; i += blockDim.x) sh.bkg[i] = scr->sh.bkg[i];
+    const int tile = blockIdx.x * 4 + wave_id();
+    const int p0 = blockIdx.x * 256;
+    ytile[threadIdx.x] = 0.0f;
+    const int nl = 2 * P.nr;
+    for (int l0 = 0; l0 < nl; l0 += ch) {
+        int n = min(ch, nl - l0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) { rec[i] = grec[l0 + i]; span[i] = gspan[l0 + i]; }
+        __syncthreads();
+        // accumulate into ytile, indexed relative to p0
+        accumulate_tiles(P, rec, span, n, ytile - p0, tile, 1 << 30);
+    }
+    __syncthreads();
+    double c = 3e38;
+    if (scr->sh.cell_ok) c = chi2_block(P, sh, ytile - p0, p0, min(P.np, p0 + 256));
+    if (threadIdx.x == 0) partial[blockIdx.x] = c;
+}
+
+__global__ void k_plain_reduce(const double *partial, int n, double *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += partial[i];
+        *out = (isfinite(s) && s < 3e38) ? s : 3e38;
+    }
+}
+
+}  // namespace ptk
+
+// ---- host-side launchers (used by pt_host.cpp) ----------------------------------------
+namespace ptk {
+
+size_t shared_static_bytes() { return sizeof(Shared); }
+size_t plain_scratch_bytes() { return sizeof(PlainScratch); }
+
+hipError_t set_lds_limits(size_t bytes) {
+    hipError_t e;
+    if ((e = hipFuncSetAttribute((const void *)k_eval_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
+    if ((e = hipFuncSetAttribute((const void *)k_pattern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
+    if ((e = hipFuncSetAttribute((const void *)k_pt_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_plain_points, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_eval_batch(const DevPlan &P, const double *X, double *chi2, int ntrials, int grid, int block, int ch,
+                             size_t dyn, hipStream_t s) {
+    hipLaunchKernelGGL(k_eval_batch, dim3(grid), dim3(block), dyn, s, P, X, chi2, ntrials, ch);
+    return hipGetLastError();
+}
+
+hipError_t launch_pattern(const DevPlan &P, const double *X, double *y, int block, int ch, size_t dyn, hipStream_t s) {
+    hipLaunchKernelGGL(k_pattern, dim3(1), dim3(block), dyn, s, P, X, y, ch);
+    return hipGetLastError();
+}
+
+hipError_t launch_pt_sweep(const DevPlan &P, int steps, int bootstrap, int block, int ch, size_t dyn,
+                           unsigned long long counter0, hipStream_t s) {
+    hipLaunchKernelGGL(k_pt_sweep, dim3(P.chains), dim3(block), dyn, s, P, steps, bootstrap, ch, counter0);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange(const DevPlan &P, int parity, unsigned long long counter, hipStream_t s) {
+    int pairs = (P.chains + 1) / 2;
+    hipLaunchKernelGGL(k_exchange, dim3((pairs + 63) / 64), dim3(64), 0, s, P, parity, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain(const DevPlan &P, const double *X, void *scratch, LineRec *rec, int4 *span, double *partial,
+                        double *out, int ch, hipStream_t s) {
+    PlainScratch *scr = reinterpret_cast<PlainScratch *>(scratch);
+    hipLaunchKernelGGL(k_plain_setup, dim3(1), dim3(256), 0, s, P, X, scr);
+    hipLaunchKernelGGL(k_plain_lines, dim3((P.nr + 255) / 256), dim3(256), 0, s, P, scr, rec, span);
+    int nblk = (P.np + 255) / 256;
+    size_t dyn = (sizeof(LineRec) + sizeof(int4)) * (size_t)ch + 256 * sizeof(float);
+    hipLaunchKernelGGL(k_plain_points, dim3(nblk), dim3(256), dyn, s, P, scr, rec, span, partial, ch);
+    hipLaunchKernelGGL(k_plain_reduce, dim3(1), dim3(64), 0, s, partial, nblk, out);
+    return hipGetLastError();
+}
+
+}  // namespace ptk

@@ -1,0 +1,94 @@
+// pt_plan.h -- device-side layout of a compiled PT plan (shared by host and kernels).
+//
+// HBM layout (one allocation per array, all read-only during a launch except state):
+//   tth[np] f64             2-theta axis (window searches)
+//   tff[np] float2          the same as a float-float pair (per-point delta = t - centre)
+//   obs[np] float2          {y_obs, w} per point
+//   hklm[nr] int4           envelope reflections of every phase, phase blocks contiguous
+//   eq_off[nr+1], eq[] short3   Laue-orbit equivalents (March-Dollase average)
+//   rep_r[nrep*9] int, rep_t[nrep*3] f64   coset representatives of every phase
+//   atoms[natoms] DevAtom   Wyckoff sites, scattering factors, weights
+//   base[n_params] f64, term_off[n_params+1], term_var[], term_coeff[] f64
+//                           model parameters p = base + M x (CSR by parameter)
+//   vlo/vhi/vscale[nv] f64  variable boxes and PT step scales
+//   state[chains*stride] f64  PT chain state (js/pt_gpu.js state layout)
+#pragma once
+#include <stdint.h>
+
+#define PTK_MAX_PHASES 4
+#define PTK_MAX_PARAMS 512
+#define PTK_MAX_ATOMS 160
+#define PTK_MAX_REPS 48
+#define PTK_MAX_REPS_TOTAL 64
+#define PTK_MAX_VARS 256
+#define PTK_MAX_BKG_F 24
+#define PTK_S_COUNT_K 12
+
+namespace ptk {
+
+struct DevAtom {
+    double r0[3];
+    double basis[3][3];
+    int free_mask;
+    int n_gauss;
+    float ff_a[5], ff_b[5], ff_c;
+    float w0;            // site multiplicity / number of coset representatives
+    float b_neutron, fp, fpp;
+    float pad;
+};
+
+struct DevPhase {
+    int system, unique_axis, profile_type;
+    int n_reps, rep_off;       // coset representatives [rep_off, rep_off + n_reps)
+    int n_atoms, atom_off;     // atoms [atom_off, atom_off + n_atoms)
+    int refl_off, refl_count;  // envelope reflections [refl_off, refl_off + refl_count)
+    int param_base;            // first model parameter of the phase block
+    double po_dir[3];
+};
+
+struct DevPlan {
+    // sizes
+    int np, nr, nv, n_params, n_phases, n_bkg;
+    int chains, stride, steps_per_sweep;
+    int uniform, radiation, doublet;
+    int max_lines_lds;         // line-record chunk held in LDS
+    int pad0;
+    // dataset
+    double t0, step;           // uniform grid: t_i = t0 + i*step
+    float step_hi, step_lo;    // float-float split of step
+    double tlo, thi;           // Chebyshev range (first and last point)
+    double lambda1, lambda2, ratio, pol_k, min_fwhm;
+    double qlim;               // envelope cut 1/dmin^2 (1 + 1e-9) at the limits
+    double guard_lo, guard_hi; // tan(theta) range of the variance guard
+    double max_temp;
+    uint32_t seed;
+    int pad1;
+    DevPhase phase[PTK_MAX_PHASES];
+    // device arrays
+    const double *tth;
+    const float2 *tff;         // float-float split of tth
+    const float2 *obs;
+    const int4 *hklm;
+    const int *eq_off;
+    const short *eq;
+    const int *rep_r;
+    const double *rep_t;
+    const DevAtom *atoms;
+    const double *base;
+    const int *term_off;
+    const int *term_var;
+    const double *term_coeff;
+    const double *vlo, *vhi, *vscale;
+    double *state;
+};
+
+// line record (one Kalpha line of one reflection), as held in LDS / global memory:
+//   a = {centre_hi, centre_lo (float-float centre, deg), amplitude, pedestal}
+//   l = {kg, kl, cl, cg} left flank (delta < 0), r = the same for the right flank, where
+//       g = exp2(kg delta^2), l = 1/(1 + kl delta^2), value = cl l + cg g - pedestal
+//   span = {start, stop, asymmetry (float bits), 0}: points in [start, stop)
+struct LineRec {
+    float4 a, l, r;
+};
+
+}  // namespace ptk

@@ -1,0 +1,563 @@
+"""Problem descriptions for the PT trial evaluator, and the seeded synthetic workloads.
+
+A `Problem` is the flat form of the PT context js/rietveld_core.js:2271-2273 hands to
+the GPU backend: one dataset (2-theta axis, counts, weights, wavelengths, polarisation,
+width floor, Chebyshev background), its Rietveld phases (operators, atoms on Wyckoff
+sites, profile family), the model parameters as an affine function of the refined
+variables, the variables' boxes and PT step scales, and the PT settings.  pt.py turns it
+into the C struct of include/powder6_pt.h.
+
+Workloads (named by what they are):
+  nacl_fm3m()        NaCl Fm-3m a=5.64, Cu Ka1 only, 10-120 deg step 0.02, TCH, 6-term
+                     Chebyshev background, 16 replicas, 12 refined parameters
+  pbso4_pnma()       PbSO4-type Pnma a=8.48 b=5.40 c=6.96, 5 sites (24 atoms), 10-150 deg
+                     step 0.02 with the Ka1/Ka2 doublet, 256 replicas x 64 trials/sweep
+  uneven_batch()     48 independent cubic / tetragonal / monoclinic patterns
+  molecular_p21c()   P2_1/c a=10.4 b=18.8 c=6.8 beta=106, 17 C/N/S atoms, 512 replicas
+  triclinic_probe()  the P-1 one-carbon problem of the reference's WebGPU test page
+                     (tests/webgpu-pt.js:11-36), for every profile family
+
+The observed pattern is the model at seeded true parameters plus seeded Poisson noise,
+w = 1/max(y, 1) (js/rietveld/rv_state.js:1122-1129); the width floor is two median
+steps (rv_state.js:1167-1172).  `simulate(problem, x) -> ycalc` computes the model; the
+caller passes the GPU pattern function of the library (or any other evaluator).
+"""
+
+import math
+
+import numpy as np
+
+from . import symmetry as S
+from .scatter_wk1995 import WK1995
+
+# model-parameter layout of include/powder6_pt.h
+P_ZERO, P_DISP, P_TRANSP, P_BKG0, MAX_BKG, P_PHASE0 = 0, 1, 2, 3, 24, 27
+PH = {'a': 0, 'b': 1, 'c': 2, 'alpha': 3, 'beta': 4, 'gamma': 5, 'scale': 6, 'po': 7}
+PH_PROF0, PH_ATOM0, AT_STRIDE = 8, 24, 5
+AT = {'x': 0, 'y': 1, 'z': 2, 'occ': 3, 'B': 4}
+PROFILE = {'simple_pvoigt': 1, 'split_pvoigt': 2, 'tch_aniso': 3}
+PROFILE_SLOTS = {
+    'tch_aniso': ['U', 'V', 'W', 'X', 'Y', 'SL', 'HL'],
+    'simple_pvoigt': ['GU', 'GV', 'GW', 'GP', 'LX', 'eta'],
+    'split_pvoigt': ['GU_L', 'GV_L', 'GW_L', 'LX_L', 'GU_R', 'GV_R', 'GW_R', 'LX_R', 'eta_split'],
+}
+# start values of the profile families, js/rietveld_core.js:446-451
+PROFILE_DEFAULTS = {
+    'tch_aniso': {'U': 2, 'V': -2, 'W': 5, 'X': 1, 'Y': 0, 'SL': 0, 'HL': 0},
+    'simple_pvoigt': {'GU': 2, 'GV': -2, 'GW': 5, 'GP': 0, 'LX': 1, 'eta': 0.5},
+    'split_pvoigt': {'GU_L': 2, 'GV_L': -2, 'GW_L': 5, 'LX_L': 1, 'GU_R': 2, 'GV_R': -2, 'GW_R': 5, 'LX_R': 1, 'eta_split': 0.5},
+}
+SYSTEMS = {'triclinic': 0, 'monoclinic': 1, 'orthorhombic': 2, 'tetragonal': 3, 'trigonal': 4, 'hexagonal': 5, 'cubic': 6}
+CU_KA1, CU_KA2, CU_RATIO = 1.540598, 1.544426, 0.5        # rv_state.js:32-37
+
+
+class Atom:
+    def __init__(self, label, element, site, occ=1.0, B=1.0, ff=None, b_neutron=0.0, fp=0.0, fpp=0.0):
+        self.label, self.element, self.site = label, element, site
+        self.occ, self.B = occ, B
+        if ff is None:
+            ff = WK1995[element]
+        self.ff_a, self.ff_b, self.ff_c = list(ff[0]), list(ff[1]), ff[2]
+        self.n_gauss = len(self.ff_a)
+        self.b_neutron, self.fp, self.fpp = b_neutron, fp, fpp
+
+
+class Phase:
+    def __init__(self, key, group, cell, atoms, profile='tch_aniso', po_dir=(0, 0, 1), unique_axis=1, ops=None):
+        self.key = key
+        if ops is None:
+            ops = S.group_ops(group)
+        self.op_r, self.op_t = ops
+        self.system = S.group_system(group) if group is not None else 'triclinic'
+        self.unique_axis = unique_axis
+        self.cell = list(cell)
+        self.atoms = atoms
+        self.profile = profile
+        self.po_dir = list(po_dir)
+
+
+class Problem:
+    """One dataset, its phases, model parameters p = base + M x, variables, PT settings."""
+
+    def __init__(self, tth, phases, lambda1=CU_KA1, lambda2=0.0, ratio=0.0, pol_k=1.0, min_fwhm=None,
+                 n_bkg=6, radiation=0):
+        self.tth = np.ascontiguousarray(tth, dtype=np.float64)
+        self.y = np.zeros_like(self.tth)
+        self.w = np.ones_like(self.tth)
+        self.lambda1, self.lambda2, self.ratio, self.pol_k = lambda1, lambda2, ratio, pol_k
+        if min_fwhm is None:
+            min_fwhm = max(1e-4, 2 * float(np.median(np.diff(self.tth))))
+        self.min_fwhm = min_fwhm
+        self.n_bkg, self.radiation = n_bkg, radiation
+        self.phases = phases
+        self.names = []
+        self.index = {}
+        self._layout()
+        self.base = np.zeros(len(self.names))
+        self.vars, self.x0, self.lo, self.hi, self.scale = [], [], [], [], []
+        self.terms = []       # (param, var, coeff)
+        self.chains, self.sweeps, self.steps_per_sweep, self.max_temp, self.seed = 6, 50, 0, 100.0, 12345
+
+    # -- layout ------------------------------------------------------------------
+    def _layout(self):
+        names = ['zero', 'disp', 'transp'] + ['bkg%d' % k for k in range(MAX_BKG)]
+        for ph in self.phases:
+            pre = ph.key + '.'
+            names += [pre + n for n in ('a', 'b', 'c', 'alpha', 'beta', 'gamma', 'scale', 'po')]
+            slots = PROFILE_SLOTS[ph.profile]
+            names += [pre + (slots[i] if i < len(slots) else '_prof%d' % i) for i in range(16)]
+            for at in ph.atoms:
+                names += [pre + at.label + '.' + n for n in ('x', 'y', 'z', 'occ', 'B')]
+        self.names = names
+        self.index = {n: i for i, n in enumerate(names)}
+
+    def phase_base(self, q):
+        b = P_PHASE0
+        for ph in self.phases[:q]:
+            b += PH_ATOM0 + AT_STRIDE * len(ph.atoms)
+        return b
+
+    def set(self, name, value):
+        self.base[self.index[name]] = value
+
+    def values(self, x=None):
+        """All model parameters at the variables x (default x0): base + M x."""
+        p = self.base.copy()
+        xx = np.asarray(self.x0 if x is None else x, dtype=np.float64)
+        for (k, j, c) in self.terms:
+            p[k] += c * xx[j]
+        return p
+
+    def refine(self, name, lo=-math.inf, hi=math.inf, scale=None):
+        """Make model parameter `name` a variable, started at its current base value."""
+        k = self.index[name]
+        j = len(self.vars)
+        v = float(self.base[k])
+        self.base[k] = 0.0
+        self.vars.append(name)
+        self.x0.append(min(hi, max(lo, v)))
+        self.lo.append(lo)
+        self.hi.append(hi)
+        self.scale.append(pt_scale(name, v) if scale is None else scale)
+        self.terms.append((k, j, 1.0))
+        return j
+
+    @property
+    def nv(self):
+        return len(self.vars)
+
+    def default_values(self):
+        """Fill the parameters the reference would hold: cell, unit occupancies, phase
+        scale and po = 1, profile defaults."""
+        for q, ph in enumerate(self.phases):
+            pre = ph.key + '.'
+            for n, v in zip(('a', 'b', 'c', 'alpha', 'beta', 'gamma'), ph.cell):
+                self.set(pre + n, v)
+            self.set(pre + 'scale', 1.0)
+            self.set(pre + 'po', 1.0)
+            for n, v in PROFILE_DEFAULTS[ph.profile].items():
+                self.set(pre + n, v)
+            for at in ph.atoms:
+                pos = at.site['pos']
+                for i, n in enumerate('xyz'):
+                    self.set(pre + at.label + '.' + n, float(pos[i]))
+                self.set(pre + at.label + '.occ', at.occ)
+                self.set(pre + at.label + '.B', at.B)
+
+    def observe(self, simulate, x_true, seed, peak=1e4):
+        """y = Poisson(model at x_true scaled so its tallest net peak is `peak` counts)."""
+        ycalc = np.asarray(simulate(self, x_true), dtype=np.float64)
+        bk = self.background(x_true)
+        net = float(np.max(ycalc - bk))
+        if net > 0 and peak:
+            f = peak / net
+            for q, ph in enumerate(self.phases):
+                k = self.index[ph.key + '.scale']
+                self._scale_param(k, f, x_true)
+            ycalc = np.asarray(simulate(self, x_true), dtype=np.float64)
+        rng = np.random.default_rng(seed)
+        y = rng.poisson(np.maximum(ycalc, 0.0)).astype(np.float64)
+        self.y = y
+        self.w = 1.0 / np.maximum(y, 1.0)
+        return ycalc
+
+    def _scale_param(self, k, f, x_true):
+        hit = False
+        for (kk, j, c) in self.terms:
+            if kk == k:
+                x_true[j] *= f
+                self.x0[j] *= f
+                hit = True
+        if not hit:
+            self.base[k] *= f
+
+    def background(self, x):
+        p = self.values(x)
+        t = self.tth
+        lo, hi = t[0], t[-1]
+        z = 2 * (t - lo) / (hi - lo) - 1 if hi > lo else np.zeros_like(t)
+        T = [np.ones_like(t), z]
+        y = np.zeros_like(t)
+        for k in range(self.n_bkg):
+            if k >= 2:
+                T.append(2 * z * T[k - 1] - T[k - 2])
+            y += p[P_BKG0 + k] * T[k]
+        return y
+
+    def start_vector(self):
+        return np.array(self.x0, dtype=np.float64)
+
+    def var_index(self, name):
+        return self.vars.index(name)
+
+
+def classify(name):
+    """Kind of a parameter name, js/rietveld_core.js:470-484 classify."""
+    if name in ('zero', 'disp', 'transp'):
+        return name, None
+    if name.startswith('bkg'):
+        return 'bkg', None
+    parts = name.split('.')
+    if len(parts) == 2:
+        if parts[1] == 'scale':
+            return 'scale', None
+        if parts[1] == 'po':
+            return 'po', None
+        if parts[1] in ('a', 'b', 'c', 'alpha', 'beta', 'gamma'):
+            return 'cell', parts[1]
+        return 'profile', parts[1]
+    if len(parts) == 3:
+        return 'atom', parts[2]
+    return 'unknown', None
+
+
+def step_cap(name, v):
+    """js/rietveld_core.js:511-523 stepCap."""
+    kind, comp = classify(name)
+    if kind == 'cell':
+        return 0.02 * abs(v) + 1e-6 if comp in ('a', 'b', 'c') else 1.0
+    if kind == 'atom':
+        return 0.2 if comp == 'occ' else 2.0 if comp == 'B' else 0.05
+    if kind == 'zero':
+        return 0.1
+    if kind == 'disp':
+        return 2.0
+    if kind == 'transp':
+        return 0.01
+    if kind == 'po':
+        return 0.25
+    if kind == 'profile':
+        if comp in ('sizeMix', 'strainMix'):
+            return 0.2
+        if comp.startswith('eta'):
+            return 0.25
+        if comp in ('SL', 'HL'):
+            return 0.05
+        return 0.5 * max(abs(v), 5)
+    return math.inf
+
+
+def pt_scale(name, v):
+    """The PT step scale of js/rietveld_core.js:2161-2167 without the curvature term:
+    natural = 0.15 * stepCap, or 0.1 * max(1, |x|) for an uncapped kind."""
+    cap = step_cap(name, v)
+    natural = 0.15 * cap if math.isfinite(cap) else 0.1 * max(1.0, abs(v))
+    return max(natural, 1e-8)
+
+
+def _grid(lo, hi, step):
+    n = int(round((hi - lo) / step)) + 1
+    return lo + step * np.arange(n, dtype=np.float64)
+
+
+def _cheb(problem, coeffs):
+    for k, v in enumerate(coeffs):
+        problem.set('bkg%d' % k, v)
+
+
+# ---------------------------------------------------------------------------
+#  workloads
+# ---------------------------------------------------------------------------
+def nacl_fm3m(simulate=None, seed=20260101, chains=16, sweeps=1000):
+    """NaCl Fm-3m a=5.64, Na 4a, Cl 4b; Cu Ka1 only; 2theta 10-120 step 0.02 (5501 pts);
+    TCH widths U,V,W,X,Y = 18.03, -9.02, 5.41, 5, 2 (GSAS cdeg units: FullProf 0.01,
+    -0.005, 0.003 deg^2 x 1803.4 and 0.05, 0.02 deg x 100); 6-term Chebyshev; 12 refined
+    parameters: a, zero, scale, U, V, W, X, Y, B_Na, B_Cl, bkg0, bkg1."""
+    r, t = S.group_ops(225)
+    cell = [5.64, 5.64, 5.64, 90, 90, 90]
+    atoms = [Atom('Na1', 'Na', S.site_symmetry(r, t, [0, 0, 0], cell), B=1.1),
+             Atom('Cl1', 'Cl', S.site_symmetry(r, t, [0.5, 0.5, 0.5], cell), B=0.9)]
+    ph = Phase('p1', 225, cell, atoms, 'tch_aniso')
+    pb = Problem(_grid(10.0, 120.0, 0.02), [ph], lambda1=1.5406, lambda2=0.0, ratio=0.0, pol_k=1.0)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    pb.set('zero', 0.012)
+    _cheb(pb, [220.0, -60.0, 25.0, -12.0, 6.0, -3.0])
+    pb.set('p1.scale', 1.0)
+    names = ['p1.a', 'zero', 'p1.scale', 'p1.U', 'p1.V', 'p1.W', 'p1.X', 'p1.Y', 'p1.Na1.B', 'p1.Cl1.B', 'bkg0', 'bkg1']
+    limits = {'p1.a': (5.64 * 0.96, 5.64 * 1.04), 'zero': (-2, 2), 'p1.scale': (0, math.inf), 'p1.X': (0, math.inf),
+              'p1.Y': (0, math.inf), 'p1.Na1.B': (-2, 30), 'p1.Cl1.B': (-2, 30)}
+    for n in names:
+        lo, hi = limits.get(n, (-math.inf, math.inf))
+        pb.refine(n, lo, hi)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, sweeps, 0, 100.0, 137
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed)
+        x0 = _perturb(pb, x_true, seed + 1)
+        pb.x0 = list(x0)
+    return pb
+
+
+PBSO4_SITES = [  # anglesite-like PbSO4, Pnma (ITA standard setting), 4c x 4 + 8d
+    ('Pb1', 'Pb', (0.188, 0.25, 0.167), 1.2),
+    ('S1', 'S', (0.063, 0.25, 0.684), 0.8),
+    ('O1', 'O', (0.908, 0.25, 0.595), 1.6),
+    ('O2', 'O', (0.193, 0.25, 0.543), 1.5),
+    ('O3', 'O', (0.081, 0.027, 0.809), 1.4),
+]
+
+
+def pbso4_pnma(simulate=None, seed=20260202, chains=256, steps_per_sweep=64, sweeps=10, jitter=0.05,
+               atoms_only=False):
+    """Pnma phase a~8.48 b~5.40 c~6.96, 5 independent sites (24 atoms), coordinates drawn
+    within +-jitter of the seeded sites, B in U[0.5, 2]; 2theta 10-150 step 0.02 (7001
+    points) with Cu Ka1/Ka2 (ratio 0.5); TCH profile; 6-term Chebyshev.  Refined: cell,
+    zero, scale, U..Y, 11 free coordinates, 5 B, 6 background terms (32 variables), or
+    only the coordinates, B and scale when atoms_only (the shared-profile workload)."""
+    rng = np.random.default_rng(seed)
+    r, t = S.group_ops(62)
+    cell = [8.48, 5.40, 6.96, 90, 90, 90]
+    atoms = []
+    for (lab, el, xyz, B) in PBSO4_SITES:
+        site = S.site_symmetry(r, t, xyz, cell)
+        pos = list(site['pos'])
+        for f in site['free']:
+            pos[f] += rng.uniform(-jitter, jitter)
+        site['pos'] = pos
+        atoms.append(Atom(lab, el, site, B=float(rng.uniform(0.5, 2.0))))
+    ph = Phase('p1', 62, cell, atoms, 'tch_aniso')
+    pb = Problem(_grid(10.0, 150.0, 0.02), [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    pb.set('zero', -0.008)
+    _cheb(pb, [180.0, -40.0, 18.0, -9.0, 4.0, -2.0])
+    names = []
+    if not atoms_only:
+        names += ['p1.a', 'p1.b', 'p1.c', 'zero', 'p1.U', 'p1.V', 'p1.W', 'p1.X', 'p1.Y']
+    names += ['p1.scale']
+    for at in atoms:
+        names += ['p1.%s.%s' % (at.label, 'xyz'[f]) for f in at.site['free']]
+    for at in atoms:
+        names.append('p1.%s.B' % at.label)
+    if not atoms_only:
+        names += ['bkg%d' % k for k in range(6)]
+    for n in names:
+        kind, comp = classify(n)
+        if kind == 'cell':
+            v = pb.base[pb.index[n]]
+            pb.refine(n, 0.96 * v, 1.04 * v)
+        elif n == 'zero':
+            pb.refine(n, -2, 2)
+        elif n == 'p1.scale':
+            pb.refine(n, 0, math.inf)
+        elif kind == 'profile' and comp in ('X', 'Y'):
+            pb.refine(n, 0, math.inf)
+        elif kind == 'atom' and comp == 'B':
+            pb.refine(n, -2, 30)
+        elif kind == 'atom':
+            v = pb.base[pb.index[n]]
+            pb.refine(n, v - 0.1, v + 0.1)
+        else:
+            pb.refine(n)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, sweeps, steps_per_sweep, 100.0, 4242
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed + 7)
+        pb.x0 = list(_perturb(pb, x_true, seed + 11))
+    return pb
+
+
+def _perturb(pb, x_true, seed, frac=0.3):
+    """A start vector away from the truth: each variable moved by +-frac of its PT scale
+    times a few, clipped to its box."""
+    rng = np.random.default_rng(seed)
+    x = np.array(x_true, dtype=np.float64)
+    for j, n in enumerate(pb.vars):
+        kind, comp = classify(n)
+        if kind == 'bkg' or n.endswith('.scale'):
+            continue
+        x[j] += rng.uniform(-1, 1) * frac * 4 * pb.scale[j]
+        x[j] = min(pb.hi[j], max(pb.lo[j], x[j]))
+    return x
+
+
+MOLECULE_ATOMS = [  # 17 non-H atoms, general positions (fractional, seeded layout)
+    ('S1', 'S'), ('N1', 'N'), ('N2', 'N'), ('N3', 'N'), ('N4', 'N'), ('N5', 'N'), ('N6', 'N'),
+    ('C1', 'C'), ('C2', 'C'), ('C3', 'C'), ('C4', 'C'), ('C5', 'C'), ('C6', 'C'), ('C7', 'C'),
+    ('C8', 'C'), ('C9', 'C'), ('C10', 'C'),
+]
+
+
+def molecular_p21c(simulate=None, seed=20260404, chains=512, steps_per_sweep=0, sweeps=10):
+    """P2_1/c (unique axis b, -P 2ybc) a=10.4 b=18.8 c=6.8 beta=106; 17 C/N/S atoms in
+    general positions (a seeded random molecule-like cluster); 2theta 5-145 step 0.02
+    (7001 points), Cu Ka1/Ka2; refined: 51 coordinates, 17 B, scale (69 variables)."""
+    rng = np.random.default_rng(seed)
+    r, t = S.group_ops(14)
+    cell = [10.4, 18.8, 6.8, 90, 106.0, 90]
+    centre = np.array([0.25, 0.12, 0.3])
+    atoms = []
+    for (lab, el) in MOLECULE_ATOMS:
+        xyz = centre + rng.uniform(-0.18, 0.18, size=3) * np.array([1.0, 0.55, 1.4])
+        site = S.site_symmetry(r, t, list(xyz), cell)
+        atoms.append(Atom(lab, el, site, B=float(rng.uniform(2.0, 4.0))))
+    ph = Phase('p1', 14, cell, atoms, 'tch_aniso', unique_axis=1)
+    pb = Problem(_grid(5.0, 145.0, 0.02), [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    _cheb(pb, [150.0, -30.0, 12.0, -6.0, 3.0, -1.5])
+    pb.refine('p1.scale', 0, math.inf)
+    for at in atoms:
+        for f in at.site['free']:
+            n = 'p1.%s.%s' % (at.label, 'xyz'[f])
+            v = pb.base[pb.index[n]]
+            pb.refine(n, v - 0.15, v + 0.15)
+    for at in atoms:
+        pb.refine('p1.%s.B' % at.label, -2, 30)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, sweeps, steps_per_sweep, 100.0, 77
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed + 3)
+        pb.x0 = list(_perturb(pb, x_true, seed + 5))
+    return pb
+
+
+UNEVEN_GROUPS = [(221, 'cubic'), (225, 'cubic'), (229, 'cubic'), (123, 'tetragonal'), (139, 'tetragonal'),
+                 (136, 'tetragonal'), (14, 'monoclinic'), (15, 'monoclinic'), (10, 'monoclinic')]
+UNEVEN_ELEMENTS = ['O', 'Mg', 'Si', 'S', 'K', 'Ca', 'Ti', 'Fe', 'Cu', 'Zn', 'Ba', 'Pb', 'C', 'N', 'Na', 'Cl']
+
+
+def uneven_pattern(index, simulate=None, seed=20260303, chains=32):
+    """Pattern `index` of the uneven batch: a cubic, tetragonal or monoclinic cell with
+    edges in [4, 20] A, 1-12 sites, 2000-12000 points over a seeded 2theta range."""
+    rng = np.random.default_rng(seed + 1000 * index)
+    grp, system = UNEVEN_GROUPS[index % len(UNEVEN_GROUPS)]
+    r, t = S.group_ops(grp)
+    if system == 'cubic':
+        a = rng.uniform(4.0, 14.0)
+        cell = [a, a, a, 90, 90, 90]
+    elif system == 'tetragonal':
+        a = rng.uniform(4.0, 14.0)
+        cell = [a, a, rng.uniform(4.0, 20.0), 90, 90, 90]
+    else:
+        cell = [rng.uniform(4.0, 16.0), rng.uniform(4.0, 20.0), rng.uniform(4.0, 14.0), 90, rng.uniform(95, 115), 90]
+    n_sites = int(rng.integers(1, 13))
+    atoms = []
+    for s_i in range(n_sites):
+        xyz = list(rng.uniform(0.02, 0.98, size=3))
+        site = S.site_symmetry(r, t, xyz, cell)
+        el = UNEVEN_ELEMENTS[int(rng.integers(0, len(UNEVEN_ELEMENTS)))]
+        atoms.append(Atom('A%d' % (s_i + 1), el, site, B=float(rng.uniform(0.5, 2.0))))
+    n_pts = int(rng.integers(2000, 12001))
+    lo = float(rng.uniform(5.0, 20.0))
+    step = float(rng.choice([0.01, 0.015, 0.02]))
+    tth = lo + step * np.arange(n_pts, dtype=np.float64)
+    tth = tth[tth < 160.0]
+    ph = Phase('p1', grp, cell, atoms, 'tch_aniso', unique_axis=1)
+    pb = Problem(tth, [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    _cheb(pb, [150.0, -30.0, 10.0, -5.0, 2.0, -1.0])
+    for n in ('a',) + (('c',) if system == 'tetragonal' else ('b', 'c') if system == 'monoclinic' else ()):
+        v = pb.base[pb.index['p1.' + n]]
+        pb.refine('p1.' + n, 0.98 * v, 1.02 * v)
+    pb.refine('zero', -2, 2)
+    pb.refine('p1.scale', 0, math.inf)
+    for at in atoms:
+        for f in at.site['free']:
+            n = 'p1.%s.%s' % (at.label, 'xyz'[f])
+            v = pb.base[pb.index[n]]
+            pb.refine(n, v - 0.1, v + 0.1)
+        pb.refine('p1.%s.B' % at.label, -2, 30)
+    for k in range(2):
+        pb.refine('bkg%d' % k)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 10, 0, 100.0, 900 + index
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed + 17 * index + 1)
+        pb.x0 = list(_perturb(pb, x_true, seed + 19 * index + 2))
+    return pb
+
+
+def uneven_batch(simulate=None, seed=20260303, count=48, chains=32):
+    return [uneven_pattern(i, simulate, seed, chains) for i in range(count)]
+
+
+TRICLINIC_FF = ((2.31, 1.02, 1.5886, .865), (20.8439, 10.2075, .5687, 51.6512), .2156)
+
+
+def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, hl=0.0, po=0.87, n_points=160):
+    """The one-atom P-1 problem of tests/webgpu-pt.js:11-36: a=3.7 b=4.1 c=4.3 alpha=87
+    beta=94 gamma=102, C at (.13,.21,.32) with the 4-Gaussian f0 given there; 160 points
+    8-63.65 deg (step 0.35), w 0.01 (point 5 excluded), lambda 1.540598/1.544 ratio .5,
+    polK .83, minFwhm .07; zero .03, disp 1.5, transp 1e-5, bkg 13 + 2 T1, scale .012,
+    po .87.  Refined: a, b, c (+-4%), zero, disp, transp, bkg0, scale, po, C1 x,y,z,B,occ
+    and the family's width parameters."""
+    ops = S.group_ops(2)
+    cell = [3.7, 4.1, 4.3, 87, 94, 102]
+    at = Atom('C1', 'C', S.site_symmetry(ops[0], ops[1], [.13, .21, .32], cell), B=0.5, ff=TRICLINIC_FF,
+              b_neutron=6.646)
+    ph = Phase('p1', 2, cell, [at], profile, po_dir=(0, 0, 1))
+    tth = 8 + np.arange(n_points, dtype=np.float64) * .35
+    pb = Problem(tth, [ph], lambda1=1.540598, lambda2=1.544, ratio=.5, pol_k=.83, min_fwhm=.07, n_bkg=2,
+                 radiation=1 if neutron else 0)
+    pb.default_values()
+    pb.set('zero', .03)
+    pb.set('disp', 1.5)
+    pb.set('transp', .00001)
+    pb.set('bkg0', 13)
+    pb.set('bkg1', 2)
+    pb.set('p1.scale', .012)
+    pb.set('p1.po', po)
+    if profile == 'tch_aniso':
+        pb.set('p1.SL', sl)
+        pb.set('p1.HL', hl)
+    names = ['p1.a', 'p1.b', 'p1.c', 'zero', 'disp', 'transp', 'bkg0', 'p1.scale', 'p1.po', 'p1.C1.x', 'p1.C1.y',
+             'p1.C1.z', 'p1.C1.B', 'p1.C1.occ']
+    for n in ['GU', 'LX', 'eta', 'GU_L', 'GU_R', 'LX_L', 'LX_R', 'eta_split', 'U', 'W', 'X', 'Y', 'SL', 'HL']:
+        if 'p1.' + n in pb.index and n in PROFILE_SLOTS[profile]:
+            names.append('p1.' + n)
+    for n in names:
+        kind, comp = classify(n)
+        if kind == 'cell':
+            v = pb.base[pb.index[n]]
+            pb.refine(n, .96 * v, 1.04 * v)
+        elif n == 'zero':
+            pb.refine(n, -2, 2)
+        elif n == 'p1.scale':
+            pb.refine(n, 0, math.inf)
+        elif n == 'p1.po':
+            pb.refine(n, 0.1, 5)
+        elif kind == 'atom' and comp == 'occ':
+            pb.refine(n, 0, 2)
+        elif kind == 'atom' and comp == 'B':
+            pb.refine(n, -2, 30)
+        elif kind == 'profile' and comp.startswith('eta'):
+            pb.refine(n, 0, 1)
+        elif kind == 'profile' and comp in ('LX', 'LX_L', 'LX_R', 'X', 'Y', 'GP'):
+            pb.refine(n, 0, math.inf)
+        else:
+            pb.refine(n)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = 4, 10, 0, 5.0, 137
+    if simulate is not None:
+        ycalc = np.asarray(simulate(pb, pb.start_vector()), dtype=np.float64)
+        i = np.arange(n_points)
+        pb.y = ycalc + 2 * np.sin(i * .13) + 1
+        pb.w = np.full(n_points, .01)
+        pb.w[5] = 0
+    return pb

@@ -1,0 +1,152 @@
+"""PT trial-pattern throughput on one MI355X.
+
+Workload (the batched FP32 VALU configuration): Pnma PbSO4-type phase a=8.48 b=5.40 c=6.96,
+5 sites / 24 atoms, 2theta 10-150 deg step 0.02 (7001 points), Cu Ka1/Ka2, TCH profile,
+6-term Chebyshev background, 32 refined variables; 256 replicas x 64 trials per sweep.
+A "step" is one PT sweep: 256 x 64 = 16384 trial patterns, each a full calculated pattern
+and its chi^2, proposed, evaluated and accepted on the GPU, then one replica exchange.
+
+Prints one JSON line (value = trial patterns/s of the chosen form, the persistent PT
+sweep kernel), with the plain single-trial form, the roofline of the dominant kernel
+(k_pt_sweep, FP32 VALU) and the FP64 CPU PT baseline (the oracle, timed on this host) in
+the same line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'PT trial patterns/s on 1 MI355X, plain vs chosen form, χ² rel err ≤1e-5'
+PEAK_FP32 = 157.3e12      # MI355X FP32 vector peak, FLOP/s
+
+
+def flop_per_trial(w):
+    """Algorithmic FLOP of one trial: 8 N_hkl S + 10 W_tot + 4 NP + 2 N_T NP."""
+    return 8 * w['n_hkl'] * w['s_terms'] + 10 * w['w_tot'] + 4 * w['np'] + 2 * w['n_bkg'] * w['np']
+
+
+def cpu_baseline(pb, x_states, gpu_costs, seconds_target=12.0):
+    """The FP64 CPU PT path (oracle restatement of temper() + cost()) on this host, on a
+    bounded sample; also checks chi^2 of the GPU chain states against the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import oracle_lib as O
+    m = O.Model(pb)
+    errs = []
+    for x, g in zip(x_states, gpu_costs):
+        c = m.cost(x)
+        errs.append(abs(g - c) / max(1.0, c))
+    # single thread: one temper() run of 4 replicas until ~seconds_target
+    x0 = pb.start_vector()
+    t = time.perf_counter()
+    _, _, st = m.temper(x0, pb.scale, 1, 4, pb.max_temp, 99)
+    dt = time.perf_counter() - t
+    single = st[1] / dt
+    # all host threads: one independent temper() per thread (the reference runs one PT fit
+    # per worker, js/rietveld/rv_commands.js:11-100)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    from powder6_amd.pt import CProblem
+    cp = CProblem(pb)
+    sc = np.ascontiguousarray(pb.scale, dtype=np.float64)
+    secs = O.ctypes.c_double()
+    rate = O.lib().or_temper_parallel(O.ctypes.addressof(cp.struct), sc.ctypes.data_as(O.P_D), 1, 2, pb.max_temp, 7,
+                                      threads, O.ctypes.byref(secs))
+    m.close()
+    return {
+        'value': rate, 'unit': 'trial patterns/s', 'cores': threads, 'kind': 'port',
+        'sample': ('oracle FP64 temper(): %d threads x (2 replicas x 1 sweep x %d trials), %.1f s wall; '
+                   'single thread %.1f trials/s (4 replicas x 1 sweep)' % (threads, pb.nv, secs.value, single)),
+        'single_thread': single,
+        'chi2_rel_err_vs_fp64': max(errs) if errs else None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--plain-trials', type=int, default=512)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one MI355X: this build runs on a single GPU (replica sharding across '
+              'GPUs is a design note only); run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    from powder6_amd import problems as PR
+    from powder6_amd import pt
+
+    pb = PR.pbso4_pnma(simulate=pt.gpu_simulate)          # 256 replicas x 64 trials/sweep
+    x0 = pb.start_vector()
+    with pt.Plan(pb) as plan:
+        chains, spw = plan.chains, plan.info['steps_per_sweep']
+        work = plan.workload(x0)
+        plan.initial([x0])
+        plan.batch(1, bootstrap=True)
+        for _ in range(args.warmup):
+            plan.sweeps(1)
+        t0 = time.perf_counter()
+        plan.sweeps(args.steps)                               # synchronises its stream
+        t1 = time.perf_counter()
+        total_ms, kernel_ms, launches = plan.timing()
+        state = plan.batch(0)
+        trials = chains * spw * args.steps
+        value = trials / (t1 - t0)
+        ms_per_step = 1e3 * (t1 - t0) / args.steps
+        # plain single-trial form on a bounded sample of trial vectors
+        rng = np.random.default_rng(1)
+        X = x0 + rng.uniform(-1, 1, size=(args.plain_trials, pb.nv)) * np.array(pb.scale)
+        X = np.minimum(np.array(pb.hi), np.maximum(np.array(pb.lo), X))
+        plan.eval(X[:8], pt.FORM_PLAIN)
+        t = time.perf_counter()
+        plan.eval(X, pt.FORM_PLAIN)
+        plain = len(X) / (time.perf_counter() - t)
+        # batched (one launch, one workgroup per trial) on 16384 independent vectors
+        XB = np.repeat(X, (16384 + len(X) - 1) // len(X), axis=0)[:16384]
+        plan.eval(XB, pt.FORM_BATCHED)
+        t = time.perf_counter()
+        gb = plan.eval(XB, pt.FORM_BATCHED)
+        batched = len(XB) / (time.perf_counter() - t)
+
+    S = state.reshape(chains, -1)
+    nv = pb.nv
+    pick = list(range(0, chains, max(1, chains // 16)))
+    xs = [S[c, :nv].copy() for c in pick]
+    gc = [S[c, 3 * nv + pt.S_COST] for c in pick]
+
+    f_trial = flop_per_trial(work)
+    launch_s = kernel_ms / max(1, launches) / 1e3
+    achieved = f_trial * chains * spw / launch_s
+    roof = {'bound': 'valu', 'kernel': 'k_pt_sweep', 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
+            'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': None,
+            'flop_per_trial': f_trial, 'avg_launch_ms': launch_s * 1e3, 'workload': work}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pb, xs, gc)
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'trial patterns/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'fp32', 'data': 'synthetic (seeded Poisson pattern of the model at seeded true parameters)',
+        'config': {'workload': 'Pnma PbSO4-type, 24 atoms, 7001 pts Ka1/Ka2, TCH, 256 replicas x 64 trials/sweep',
+                   'replicas': chains, 'trials_per_sweep': spw, 'n_vars': nv, 'points': work['np'],
+                   'reflections': work['n_hkl'], 'lines': work['lines'], 'form': 'persistent PT sweep (k_pt_sweep)'},
+        'plain': {'value': plain, 'unit': 'trial patterns/s', 'form': 'one trial per launch sequence'},
+        'batched_eval': {'value': batched, 'unit': 'trial patterns/s', 'form': 'k_eval_batch, 16384 vectors'},
+        'chosen_over_plain': value / plain,
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    if cpu:
+        out['chosen_over_cpu'] = value / cpu['value']
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

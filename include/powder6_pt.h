@@ -187,6 +187,12 @@ int pt_eval(pt_plan *plan, int form, const double *x, int ntrials, double *chi2)
 /* Calculated pattern (background + phases) of one vector, FP32 kernels, n_points out. */
 int pt_pattern(pt_plan *plan, const double *x, double *ycalc);
 
+/* Work of one trial at vector x, for the roofline: out[0] lines with a window, out[1]
+ * window points summed over the lines (W_tot), out[2] reflections with a Kalpha1 window
+ * (N_hkl), out[3] structure-factor terms per reflection S = sum_atoms |coset reps|,
+ * out[4] points, out[5] Chebyshev terms.  No counterpart in the reference (measurement). */
+int pt_workload(pt_plan *plan, const double *x, double *out);
+
 /* Timing of the last pt_batch / pt_eval call on the plan's stream (HIP events):
  * elapsed milliseconds of the whole call and of its dominant kernel, and the number of
  * dominant-kernel launches. */

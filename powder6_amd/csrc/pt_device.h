@@ -318,8 +318,16 @@ __device__ inline bool phase_metric(const DevPhase &ph, const double *pp, double
         be = isfinite(pp[4]) ? pp[4] : 90.0;
         ga = isfinite(pp[5]) ? pp[5] : 90.0;
     }
-    double ca = cos(al * kD2R), cb = cos(be * kD2R), cg = cos(ga * kD2R);
-    double sa = sin(al * kD2R), sb = sin(be * kD2R), sg = sin(ga * kD2R);
+    double ca, cb, cg, sa, sb, sg;
+    if (ph.fixed_angles) {
+        // angles that do not move: their cosines and sines are computed once on the host
+        al = ph.ang[0]; be = ph.ang[1]; ga = ph.ang[2];
+        ca = ph.ang[3]; cb = ph.ang[4]; cg = ph.ang[5];
+        sa = ph.ang_sin[0]; sb = ph.ang_sin[1]; sg = ph.ang_sin[2];
+    } else {
+        ca = cos(al * kD2R); cb = cos(be * kD2R); cg = cos(ga * kD2R);
+        sa = sin(al * kD2R); sb = sin(be * kD2R); sg = sin(ga * kD2R);
+    }
     double vf = 1.0 - ca * ca - cb * cb - cg * cg + 2.0 * ca * cb * cg;
     // cellVolume (crystal.js:245-252) validity + buildInvDsq degeneracy
     if (!(vf > 0.0) || !(al > 0.0 && al < 180.0 && be > 0.0 && be < 180.0 && ga > 0.0 && ga < 180.0)) return false;

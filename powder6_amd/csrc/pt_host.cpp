@@ -407,6 +407,12 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         dp.refl_count = (int)list.size();
         dp.param_base = pbase;
         for (int s = 0; s < 3; ++s) dp.po_dir[s] = ph.po_dir[s];
+        dp.fixed_angles = anglesMove ? 0 : 1;
+        for (int s = 0; s < 3; ++s) {
+            dp.ang[s] = cell[3 + s];
+            dp.ang[3 + s] = std::cos(cell[3 + s] * D2R);
+            dp.ang_sin[s] = std::sin(cell[3 + s] * D2R);
+        }
         if (dp.rep_off + dp.n_reps > PTK_MAX_REPS_TOTAL) { delete pl; return fail(PT_ERR_BUDGET, "PT has more than 64 coset representatives."); }
         for (int r : reps) {
             for (int x = 0; x < 9; ++x) rep_r.push_back(ph.op_r[9 * r + x]);
@@ -690,6 +696,26 @@ int pt_pattern(pt_plan *pl, const double *x, double *ycalc) {
     HIPCHECK(ptk::launch_pattern(pl->P, pl->d_x, pl->d_y, pl->block, pl->ch, pl->dyn, pl->stream), "pattern");
     HIPCHECK(hipMemcpyAsync(ycalc, pl->d_y, sizeof(double) * pl->np, hipMemcpyDeviceToHost, pl->stream), "readback");
     HIPCHECK(hipStreamSynchronize(pl->stream), "pattern");
+    return 0;
+}
+
+int pt_workload(pt_plan *pl, const double *x, double *out) {
+    if (!pl || !x || !out) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    int rc = ensure_trials(pl, 1);
+    if (rc) return rc;
+    long long *d_stats = nullptr;
+    if ((rc = dalloc(pl, &d_stats, 3))) return rc;
+    HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
+    HIPCHECK(ptk::launch_workload(pl->P, pl->d_x, d_stats, pl->block, pl->ch, pl->dyn, pl->stream), "workload");
+    long long h[3] = {0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "workload");
+    int S = 0;
+    for (int q = 0; q < pl->P.n_phases; ++q) S += pl->P.phase[q].n_atoms * pl->P.phase[q].n_reps;
+    out[0] = (double)h[0]; out[1] = (double)h[1]; out[2] = (double)h[2]; out[3] = (double)S;
+    out[4] = (double)pl->np; out[5] = (double)pl->P.n_bkg;
     return 0;
 }
 

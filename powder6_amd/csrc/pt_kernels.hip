@@ -376,6 +376,42 @@ __global__ void __launch_bounds__(1024) k_pattern(DevPlan P, const double *__res
     for (int i = threadIdx.x; i < P.np; i += blockDim.x) yout[i] = (double)(background_at(P, sh, i) + ylds[i]);
 }
 
+// ---- workload of one vector: active lines, window points, active reflections --------
+// stats[0] = lines with a window, stats[1] = sum of window lengths (points),
+// stats[2] = reflections with a Kalpha1 window
+__global__ void __launch_bounds__(1024) k_workload(DevPlan P, const double *__restrict__ X, long long *stats, int ch) {
+    __shared__ Shared sh;
+    __shared__ unsigned long long cnt[3];
+    float *ylds; LineRec *rec; int4 *span;
+    dyn_layout(P.np, ch, ylds, rec, span);
+    stage_reps(P, sh);
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
+    __syncthreads();
+    materialize(P, X, sh);
+    __syncthreads();
+    trial_setup(P, sh);
+    if (sh.cell_ok) {
+        const int per = ch >> 1;
+        for (int k0 = 0; k0 < P.nr; k0 += per) {
+            int kn = min(per, P.nr - k0);
+            __syncthreads();
+            for (int kk = threadIdx.x; kk < kn; kk += blockDim.x) {
+                build_reflection(P, sh, k0 + kk, rec + 2 * kk, span + 2 * kk);
+                for (int s = 0; s < 2; ++s) {
+                    int4 sp = span[2 * kk + s];
+                    if (sp.y > sp.x) {
+                        atomicAdd(&cnt[0], 1ull);
+                        atomicAdd(&cnt[1], (unsigned long long)(sp.y - sp.x));
+                        if (s == 0) atomicAdd(&cnt[2], 1ull);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) stats[threadIdx.x] = (long long)cnt[threadIdx.x];
+}
+
 // ---- persistent PT sweep: one workgroup per replica ----------------------------------
 // State layout per chain (stride 3nv+12) as js/pt_gpu.js.  `counter0` numbers the steps
 // for the random streams.  bootstrap: one step, no move, unconditional accept.
@@ -544,6 +580,7 @@ size_t plain_scratch_bytes() { return sizeof(PlainScratch); }
 hipError_t set_lds_limits(size_t bytes) {
     hipError_t e;
     if ((e = hipFuncSetAttribute((const void *)k_eval_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
+    if ((e = hipFuncSetAttribute((const void *)k_workload, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
     if ((e = hipFuncSetAttribute((const void *)k_pattern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
     if ((e = hipFuncSetAttribute((const void *)k_pt_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_plain_points, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -557,6 +594,12 @@ hipError_t launch_eval_batch(const DevPlan &P, const double *X, double *chi2, in
 
 hipError_t launch_pattern(const DevPlan &P, const double *X, double *y, int block, int ch, size_t dyn, hipStream_t s) {
     hipLaunchKernelGGL(k_pattern, dim3(1), dim3(block), dyn, s, P, X, y, ch);
+    return hipGetLastError();
+}
+
+hipError_t launch_workload(const DevPlan &P, const double *X, long long *stats, int block, int ch, size_t dyn,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_workload, dim3(1), dim3(block), dyn, s, P, X, stats, ch);
     return hipGetLastError();
 }
 

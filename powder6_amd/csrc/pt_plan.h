@@ -43,7 +43,10 @@ struct DevPhase {
     int n_atoms, atom_off;     // atoms [atom_off, atom_off + n_atoms)
     int refl_off, refl_count;  // envelope reflections [refl_off, refl_off + refl_count)
     int param_base;            // first model parameter of the phase block
+    int fixed_angles;          // 1: the cell angles do not move; trig below is used
     double po_dir[3];
+    double ang[6];             // alpha, beta, gamma (deg), cos alpha, cos beta, cos gamma
+    double ang_sin[3];
 };
 
 struct DevPlan {

@@ -158,6 +158,7 @@ def lib():
         L.pt_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_last_timing.argtypes = [ctypes.c_void_p, P_D, P_D, P_I]
         L.pt_phase_base.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pt_workload.argtypes = [ctypes.c_void_p, P_D, P_D]
         _lib = L
     return _lib
 
@@ -242,6 +243,14 @@ class Plan:
         out = np.zeros(self.info['point_count'])
         _check(lib().pt_pattern(self.h, _dptr(x[0]), _dptr(out)))
         return out
+
+    def workload(self, x):
+        """Per-trial work at x: lines, W_tot, N_hkl, S, NP, N_T."""
+        x = self._vectors(x)
+        out = np.zeros(6)
+        _check(lib().pt_workload(self.h, _dptr(x[0]), _dptr(out)))
+        return dict(lines=int(out[0]), w_tot=int(out[1]), n_hkl=int(out[2]), s_terms=int(out[3]), np=int(out[4]),
+                    n_bkg=int(out[5]))
 
     def timing(self):
         t, k, n = c_double(), c_double(), c_int32()

@@ -1,0 +1,77 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares; host-only
+calls work without a device, and device calls fail loudly (no CPU fallback)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from powder6_amd import pt
+from powder6_amd import problems as PR
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, 'include', '*.h')):
+        text = open(h).read()
+        text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+        for m in re.finditer(r'^[A-Za-z_][\w \*]*?\b(pt_\w+)\s*\(', text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ('pt_plan_create', 'pt_plan_destroy', 'pt_costs', 'pt_batch', 'pt_eval', 'pt_last_error'):
+        assert n in names
+
+
+@pytest.mark.parametrize('name', declared_functions())
+def test_library_exports(name):
+    lib = ctypes.CDLL(pt.LIB_PATH)
+    assert hasattr(lib, name), name
+
+
+def test_abi_version():
+    assert pt.lib().pt_abi_version() == 1
+
+
+def test_phase_base_layout():
+    pb = PR.pbso4_pnma()
+    pb.y = np.ones_like(pb.tth)
+    cp = pt.CProblem(pb)
+    lib = pt.lib()
+    assert lib.pt_phase_base(cp.ref(), 0) == 27
+    assert lib.pt_phase_base(cp.ref(), 1) == len(pb.base) == 27 + 24 + 5 * len(pb.phases[0].atoms)
+
+
+def test_compile_without_device_reports_counts():
+    pb = PR.nacl_fm3m()
+    pb.y = np.ones_like(pb.tth)
+    info = pt.compile({'problem': pb})
+    assert info['point_count'] == 5501 and info['nv'] == 12 and info['chains'] == 16
+    assert info['stride'] == 3 * 12 + 12 and info['uniform_grid'] == 1
+    assert info['reflection_count'] > 0
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_has_gpu(), reason='checks the behaviour without a device')
+def test_device_calls_fail_loudly_without_gpu():
+    pb = PR.nacl_fm3m()
+    pb.y = np.ones_like(pb.tth)
+    with pytest.raises(pt.PTError, match='unavailable'):
+        pt.Plan(pb)
+    with pt.Plan(pb, device=False) as pl:
+        with pytest.raises(pt.PTError, match='without a device'):
+            pl.costs(pb.start_vector())

@@ -193,9 +193,14 @@ int pt_pattern(pt_plan *plan, const double *x, double *ycalc);
  * out[4] points, out[5] Chebyshev terms.  No counterpart in the reference (measurement). */
 int pt_workload(pt_plan *plan, const double *x, double *out);
 
+/* Kernel timing mode (0 off, default): when on, pt_batch launches every kernel directly
+ * instead of replaying the captured sweep graph and brackets each launch of the dominant
+ * kernel (the banded profile sum) with HIP events.  No counterpart in the reference. */
+int pt_set_kernel_timing(pt_plan *plan, int on);
+
 /* Timing of the last pt_batch / pt_eval call on the plan's stream (HIP events):
- * elapsed milliseconds of the whole call and of its dominant kernel, and the number of
- * dominant-kernel launches. */
+ * elapsed milliseconds of the whole call, summed milliseconds of the dominant kernel's
+ * timed launches (kernel timing mode, or batched pt_eval), and that launch count. */
 int pt_last_timing(const pt_plan *plan, double *total_ms, double *kernel_ms, int *launches);
 
 #ifdef __cplusplus

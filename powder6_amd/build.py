@@ -41,7 +41,28 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print('built', LIB)
+    build_addon(verbose)
     return LIB
+
+
+NODE_INCLUDE = os.environ.get('NODE_INCLUDE', '/usr/include/node')
+ADDON = os.path.join(LIBDIR, 'pt_addon.node')
+
+
+def build_addon(verbose=False):
+    """The Node N-API addon powder6_amd/_lib/pt_addon.node (links libpowder6_pt.so, found
+    next to it through an $ORIGIN rpath).  Skipped when the Node headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, 'node_api.h')):
+        if verbose:
+            print('node headers not found; addon not built')
+        return None
+    tmp = ADDON + '.tmp'
+    run(['gcc', '-O2', '-fPIC', '-shared', '-std=c11', '-Wall', '-DNODE_GYP_MODULE_NAME=pt_addon', '-I' + NODE_INCLUDE,
+         os.path.join(HERE, 'node', 'pt_addon.c'), '-L' + LIBDIR, '-lpowder6_pt', '-Wl,-rpath,$ORIGIN', '-o', tmp])
+    os.replace(tmp, ADDON)
+    if verbose:
+        print('built', ADDON)
+    return ADDON
 
 
 if __name__ == '__main__':

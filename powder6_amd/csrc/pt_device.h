@@ -119,11 +119,13 @@ __device__ __forceinline__ float raw_profile(float delta, const Prep &p) {
         float t = fmaxf_js(-0.95f, fminf_js(0.95f, p.asym * delta));
         delta /= (1.0f - t);
     }
-    int s = delta < 0.0f ? 0 : 1;
-    float u = delta * p.ig[s], v = delta * p.il[s];
+    const bool left = delta < 0.0f;           // selects, not an indexed load (no scratch)
+    const float ig = left ? p.ig[0] : p.ig[1], il = left ? p.il[0] : p.il[1];
+    const float cl = left ? p.cl[0] : p.cl[1], cg = left ? p.cg[0] : p.cg[1];
+    float u = delta * ig, v = delta * il;
     float g = __expf(-kCg * u * u);
     float l = 1.0f / (1.0f + 4.0f * v * v);
-    return p.cl[s] * l + p.cg[s] * g;
+    return cl * l + cg * g;
 }
 
 // NR erfcc, profile.js:791-797
@@ -149,12 +151,16 @@ __device__ inline float asym_area(const Prep &pk) {
         float side = si ? 1.0f : -1.0f, lo = 0.0f, e = 0.25f * w;
         for (int guard = 0; guard < 64; ++guard) {
             float hi = fminf_js(e, D);
-            float b0[2], b1[2];
-            int np = 0;
-            if (clampAt > lo && clampAt < hi) { b0[np] = lo; b1[np] = clampAt; ++np; lo = clampAt; }
-            b0[np] = lo; b1[np] = hi; ++np;
-            for (int b = 0; b < np; ++b) {
-                float m = 0.5f * (b0[b] + b1[b]), h = 0.5f * (b1[b] - b0[b]), s = 0.0f;
+            // a panel that straddles the clamp point is split there (profile.js:821)
+            for (int part = 0; part < 2; ++part) {
+                float pa, pb;
+                if (part == 0) {
+                    if (!(clampAt > lo && clampAt < hi)) continue;
+                    pa = lo; pb = clampAt; lo = clampAt;
+                } else {
+                    pa = lo; pb = hi;
+                }
+                float m = 0.5f * (pa + pb), h = 0.5f * (pb - pa), s = 0.0f;
                 for (int i = 0; i < 4; ++i) {
                     float d1 = side * (m - h * X[i]), d2 = side * (m + h * X[i]);
                     s += W[i] * (raw_profile(d1, pk) - raw_profile(d1, sym) + raw_profile(d2, pk) - raw_profile(d2, sym));

@@ -169,7 +169,17 @@ struct pt_plan {
     ptk::LineRec *d_rec = nullptr;
     int4 *d_span = nullptr;
     bool states_set = false;
-    unsigned long long step_counter = 0, sweep_counter = 0;
+    // staged pipeline buffers (per trial slot)
+    int slots = 0, nsplit = 1;
+    void *d_img = nullptr;
+    ptk::LineRec *d_srec = nullptr;
+    int4 *d_sspan = nullptr;
+    double2 *d_part = nullptr;
+    double *d_schi2 = nullptr;
+    // one PT sweep captured as a graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool kernel_timing = false;
     std::vector<hipEvent_t> ev;
     hipEvent_t ev_total[2] = {nullptr, nullptr};
     double last_total_ms = 0, last_kernel_ms = 0;
@@ -320,6 +330,8 @@ void pt_plan_destroy(pt_plan *pl) {
         if (pl->stream) (void)hipStreamSynchronize(pl->stream);
         for (void *p : pl->allocs) (void)hipFree(p);
         for (auto e : pl->ev) (void)hipEventDestroy(e);
+        if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
+        if (pl->graph) (void)hipGraphDestroy(pl->graph);
         if (pl->ev_total[0]) (void)hipEventDestroy(pl->ev_total[0]);
         if (pl->ev_total[1]) (void)hipEventDestroy(pl->ev_total[1]);
         if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -522,6 +534,8 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     pl->dyn = rec_bytes * pl->ch + ybytes;
     pl->plain_dyn = rec_bytes * pl->ch + 256 * sizeof(float);
     pl->block = 512;
+    int ppg = ptk::accum_points_per_group();
+    pl->nsplit = (n + ppg - 1) / ppg;
     HIPCHECK(ptk::set_lds_limits(std::max(pl->dyn, pl->plain_dyn)), "kernel setup");
     *out = pl;
     return 0;
@@ -576,26 +590,76 @@ int pt_set_states(pt_plan *pl, const double *x, int nvec) {
     HIPCHECK(hipMemcpyAsync(pl->P.state, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, pl->stream), "state upload");
     HIPCHECK(hipStreamSynchronize(pl->stream), "state upload");
     pl->states_set = true;
-    pl->step_counter = 0;
-    pl->sweep_counter = 0;
     return 0;
 }
+
+// staged-pipeline buffers for `n` trial slots
+static int ensure_slots(pt_plan *pl, int n) {
+    if (n <= pl->slots) return 0;
+    int rc;
+    size_t nl = 2 * (size_t)pl->nr;
+    if ((rc = dalloc(pl, (char **)&pl->d_img, (size_t)n * ptk::trial_image_bytes())) ||
+        (rc = dalloc(pl, &pl->d_srec, (size_t)n * nl)) || (rc = dalloc(pl, &pl->d_sspan, (size_t)n * nl)) ||
+        (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->nsplit)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
+        return rc;
+    pl->slots = n;
+    return 0;
+}
+
+// one pass of the staged pipeline over `ntrials` slots; accum timed when ev_pair != null
+static int run_stages(pt_plan *pl, const double *dX, int ntrials, int pt_mode, int bootstrap, double *d_chi2,
+                      hipEvent_t *ev_pair) {
+    HIPCHECK(ptk::launch_stage_setup(pl->P, dX, pl->d_img, ntrials, pt_mode, bootstrap, pl->stream), "setup");
+    HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, ntrials, pl->stream), "lines");
+    if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[0], pl->stream), "timing");
+    HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->nsplit, ntrials, pl->stream), "accumulate");
+    if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[1], pl->stream), "timing");
+    HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->nsplit, ntrials, d_chi2, pt_mode, pl->stream), "finish");
+    return 0;
+}
+
+static int ensure_events(pt_plan *pl, int pairs) {
+    if ((int)pl->ev.size() >= 2 * pairs) return 0;
+    for (auto e : pl->ev) (void)hipEventDestroy(e);
+    pl->ev.assign(2 * pairs, nullptr);
+    for (auto &e : pl->ev) HIPCHECK(hipEventCreate(&e), "event");
+    return 0;
+}
+
+static double sum_pairs(pt_plan *pl, int pairs) {
+    double k = 0;
+    for (int j = 0; j < pairs; ++j) {
+        float m = 0;
+        (void)hipEventElapsedTime(&m, pl->ev[2 * j], pl->ev[2 * j + 1]);
+        k += m;
+    }
+    return k;
+}
+
+static const int kEvalChunk = 2048;   // trial slots per pass of pt_eval
 
 static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2, bool time_it) {
     int rc = ensure_trials(pl, ntrials);
     if (rc) return rc;
+    int chunk = std::min(ntrials, kEvalChunk);
+    if ((rc = ensure_slots(pl, std::max(chunk, pl->chains)))) return rc;
+    int passes = (ntrials + chunk - 1) / chunk;
+    if (time_it && (rc = ensure_events(pl, passes))) return rc;
     HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * ntrials * pl->nv, hipMemcpyHostToDevice, pl->stream), "trial upload");
-    if (time_it) HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
-    HIPCHECK(ptk::launch_eval_batch(pl->P, pl->d_x, pl->d_chi2, ntrials, ntrials, pl->block, pl->ch, pl->dyn, pl->stream), "batched evaluation");
-    if (time_it) HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
+    HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+    for (int p = 0; p < passes; ++p) {
+        int t0 = p * chunk, n = std::min(chunk, ntrials - t0);
+        if ((rc = run_stages(pl, pl->d_x + (size_t)t0 * pl->nv, n, 0, 0, pl->d_chi2 + t0, time_it ? &pl->ev[2 * p] : nullptr)))
+            return rc;
+    }
+    HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
     HIPCHECK(hipMemcpyAsync(chi2, pl->d_chi2, sizeof(double) * ntrials, hipMemcpyDeviceToHost, pl->stream), "readback");
     HIPCHECK(hipStreamSynchronize(pl->stream), "batched evaluation");
-    if (time_it) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
-        pl->last_total_ms = pl->last_kernel_ms = ms;
-        pl->last_launches = 1;
-    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
+    pl->last_total_ms = ms;
+    pl->last_kernel_ms = time_it ? sum_pairs(pl, passes) : 0;
+    pl->last_launches = time_it ? passes : 0;
     return 0;
 }
 
@@ -609,32 +673,52 @@ int pt_costs(pt_plan *pl, const double *x, int nvec, double *chi2) {
     return eval_batched(pl, X.data(), pl->chains, chi2, false);
 }
 
+// one sweep: steps_per_sweep x (setup/propose, lines, accumulate, finish/accept), then the
+// replica exchange; accum launches bracketed by events when ev != null
+static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
+    for (int s = 0; s < pl->steps_per_sweep; ++s) {
+        int rc = run_stages(pl, nullptr, pl->chains, 1, 0, pl->d_schi2, ev ? ev + 2 * s : nullptr);
+        if (rc) return rc;
+    }
+    HIPCHECK(ptk::launch_exchange(pl->P, pl->stream), "exchange");
+    return 0;
+}
+
+int pt_set_kernel_timing(pt_plan *pl, int on) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    pl->kernel_timing = on != 0;
+    return 0;
+}
+
 int pt_batch(pt_plan *pl, int count, int bootstrap, double *state_out) {
     if (!pl) return fail(PT_ERR_ARG, "NULL plan");
     NEED_DEVICE(pl);
     if (!pl->states_set) return fail(PT_ERR_STATE, "PT batch before initial state.");
     if (count < 0) return fail(PT_ERR_ARG, "negative sweep count");
     (void)hipSetDevice(pl->device);
-    int launches = bootstrap ? 1 : count;
-    if ((int)pl->ev.size() < 2 * launches) {
-        for (auto e : pl->ev) (void)hipEventDestroy(e);
-        pl->ev.assign(2 * launches, nullptr);
-        for (auto &e : pl->ev) (void)hipEventCreate(&e);
-    }
+    int rc = ensure_slots(pl, pl->chains);
+    if (rc) return rc;
     HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+    int timed = 0;
     if (bootstrap) {
-        HIPCHECK(hipEventRecord(pl->ev[0], pl->stream), "timing");
-        HIPCHECK(ptk::launch_pt_sweep(pl->P, 1, 1, pl->block, pl->ch, pl->dyn, pl->step_counter, pl->stream), "bootstrap");
-        HIPCHECK(hipEventRecord(pl->ev[1], pl->stream), "timing");
-    } else {
-        for (int j = 0; j < count; ++j) {
-            HIPCHECK(hipEventRecord(pl->ev[2 * j], pl->stream), "timing");
-            HIPCHECK(ptk::launch_pt_sweep(pl->P, pl->steps_per_sweep, 0, pl->block, pl->ch, pl->dyn, pl->step_counter, pl->stream), "sweep");
-            HIPCHECK(hipEventRecord(pl->ev[2 * j + 1], pl->stream), "timing");
-            pl->step_counter += pl->steps_per_sweep;
-            HIPCHECK(ptk::launch_exchange(pl->P, (int)(pl->sweep_counter % 2), pl->sweep_counter, pl->stream), "exchange");
-            pl->sweep_counter++;
+        if ((rc = run_stages(pl, nullptr, pl->chains, 1, 1, pl->d_schi2, nullptr))) return rc;
+    } else if (pl->kernel_timing) {
+        timed = count * pl->steps_per_sweep;
+        if ((rc = ensure_events(pl, timed))) return rc;
+        for (int j = 0; j < count; ++j)
+            if ((rc = enqueue_sweep(pl, &pl->ev[2 * (size_t)j * pl->steps_per_sweep]))) return rc;
+    } else if (count > 0) {
+        if (!pl->gexec) {
+            HIPCHECK(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            rc = enqueue_sweep(pl, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(pl->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(PT_ERR_DEVICE, std::string("PT graph capture failed: ") + hipGetErrorString(e));
+            pl->graph = g;
+            HIPCHECK(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0), "graph instantiation");
         }
+        for (int j = 0; j < count; ++j) HIPCHECK(hipGraphLaunch(pl->gexec, pl->stream), "sweep");
     }
     HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
     if (state_out)
@@ -643,14 +727,8 @@ int pt_batch(pt_plan *pl, int count, int bootstrap, double *state_out) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
     pl->last_total_ms = ms;
-    double k = 0;
-    for (int j = 0; j < launches; ++j) {
-        float m = 0;
-        (void)hipEventElapsedTime(&m, pl->ev[2 * j], pl->ev[2 * j + 1]);
-        k += m;
-    }
-    pl->last_kernel_ms = k;
-    pl->last_launches = launches;
+    pl->last_kernel_ms = timed ? sum_pairs(pl, timed) : 0;
+    pl->last_launches = timed;
     return 0;
 }
 
@@ -678,8 +756,9 @@ int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
         HIPCHECK(hipStreamSynchronize(pl->stream), "plain evaluation");
         float ms = 0;
         (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
-        pl->last_total_ms = pl->last_kernel_ms = ms;
-        pl->last_launches = 4 * ntrials;
+        pl->last_total_ms = ms;
+        pl->last_kernel_ms = 0;
+        pl->last_launches = 0;
         return 0;
     }
     return fail(PT_ERR_STATE, "PT evaluation form not available for this plan.");

@@ -162,7 +162,8 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
     const float kG = -kCg * kLog2e;
     rec.l = make_float4(kG * pk.ig[0] * pk.ig[0], 4.0f * pk.il[0] * pk.il[0], pk.cl[0], pk.cg[0]);
     rec.r = make_float4(kG * pk.ig[1] * pk.ig[1], 4.0f * pk.il[1] * pk.il[1], pk.cl[1], pk.cg[1]);
-    span = make_int4(start, stop, __float_as_int(pk.asym), 0);
+    const int split = (pk.ig[0] != pk.ig[1] || pk.il[0] != pk.il[1] || pk.cl[0] != pk.cl[1] || pk.cg[0] != pk.cg[1]) ? 1 : 0;
+    span = make_int4(start, stop, __float_as_int(pk.asym), split);
 }
 
 // Both lines (Kalpha1, Kalpha2) of envelope reflection k: Model.geometry (:719-736),
@@ -339,26 +340,8 @@ __device__ __forceinline__ void dyn_layout(int np, int ch, float *&ylds, LineRec
     ylds = reinterpret_cast<float *>(dyn + (sizeof(LineRec) + sizeof(int4)) * ch);
 }
 
-// ---- batched evaluation: one workgroup per trial ------------------------------------
-__global__ void __launch_bounds__(1024) k_eval_batch(DevPlan P, const double *__restrict__ X, double *__restrict__ chi2,
-                                                    int ntrials, int ch) {
-    __shared__ Shared sh;
-    float *ylds; LineRec *rec; int4 *span;
-    dyn_layout(P.np, ch, ylds, rec, span);
-    stage_reps(P, sh);
-    for (int t = blockIdx.x; t < ntrials; t += gridDim.x) {
-        __syncthreads();
-        materialize(P, X + (size_t)t * P.nv, sh);
-        __syncthreads();
-        trial_setup(P, sh);
-        double c = 3e38;
-        if (sh.cell_ok) c = eval_trial(P, sh, ylds, rec, span, ch);
-        if (threadIdx.x == 0) chi2[t] = (isfinite(c) && c < 3e38) ? c : 3e38;
-    }
-}
-
 // ---- calculated pattern of one vector -----------------------------------------------
-__global__ void __launch_bounds__(1024) k_pattern(DevPlan P, const double *__restrict__ X, double *__restrict__ yout,
+__global__ void __launch_bounds__(512) k_pattern(DevPlan P, const double *__restrict__ X, double *__restrict__ yout,
                                                  int ch) {
     __shared__ Shared sh;
     float *ylds; LineRec *rec; int4 *span;
@@ -379,7 +362,7 @@ __global__ void __launch_bounds__(1024) k_pattern(DevPlan P, const double *__res
 // ---- workload of one vector: active lines, window points, active reflections --------
 // stats[0] = lines with a window, stats[1] = sum of window lengths (points),
 // stats[2] = reflections with a Kalpha1 window
-__global__ void __launch_bounds__(1024) k_workload(DevPlan P, const double *__restrict__ X, long long *stats, int ch) {
+__global__ void __launch_bounds__(512) k_workload(DevPlan P, const double *__restrict__ X, long long *stats, int ch) {
     __shared__ Shared sh;
     __shared__ unsigned long long cnt[3];
     float *ylds; LineRec *rec; int4 *span;
@@ -412,92 +395,338 @@ __global__ void __launch_bounds__(1024) k_workload(DevPlan P, const double *__re
     if (threadIdx.x < 3) stats[threadIdx.x] = (long long)cnt[threadIdx.x];
 }
 
-// ---- persistent PT sweep: one workgroup per replica ----------------------------------
-// State layout per chain (stride 3nv+12) as js/pt_gpu.js.  `counter0` numbers the steps
-// for the random streams.  bootstrap: one step, no move, unconditional accept.
-__global__ void __launch_bounds__(1024) k_pt_sweep(DevPlan P, int steps, int bootstrap, int ch, unsigned long long counter0) {
+// ---- staged pipeline: setup -> lines -> accumulate -> finish -------------------------
+// One PT step (or one batch of independent trial vectors) runs as four launches over the
+// whole GPU: per-trial setup (and the proposal), per-reflection line records written to
+// HBM (they stay in the 256 MB Infinity Cache between launches), the banded profile sum
+// split over several workgroups per trial, and the ordered reduction with the FP64
+// Metropolis decision.  Every kernel is small in registers, so each CU holds many waves.
+
+// per-trial image of the setup: what the line and accumulate kernels need
+struct TrialImage {
+    double p[PTK_MAX_PARAMS];
+    double metric[PTK_MAX_PHASES][6];
+    double apos[PTK_MAX_ATOMS][3];
+    float aocc[PTK_MAX_ATOMS], aB[PTK_MAX_ATOMS];
+    float prof[PTK_MAX_PHASES][16];
+    float bkg[PTK_MAX_BKG_F];
+    int phase_ok[PTK_MAX_PHASES];
+    int cell_ok, prof_ok, go, boot;
+};
+
+__device__ inline int total_atoms(const DevPlan &P) {
+    const DevPhase &l = P.phase[P.n_phases - 1];
+    return l.atom_off + l.n_atoms;
+}
+
+__device__ inline void image_store(const DevPlan &P, const Shared &sh, TrialImage *img) {
+    const int tid = threadIdx.x, nt = blockDim.x, na = total_atoms(P);
+    for (int i = tid; i < P.n_params; i += nt) img->p[i] = sh.p[i];
+    for (int i = tid; i < P.n_phases * 6; i += nt) img->metric[i / 6][i % 6] = sh.metric[i / 6][i % 6];
+    for (int i = tid; i < na * 3; i += nt) img->apos[i / 3][i % 3] = sh.apos[i / 3][i % 3];
+    for (int i = tid; i < na; i += nt) { img->aocc[i] = sh.aocc[i]; img->aB[i] = sh.aB[i]; }
+    for (int i = tid; i < P.n_phases * 16; i += nt) img->prof[i >> 4][i & 15] = sh.prof[i >> 4][i & 15];
+    for (int i = tid; i < PTK_MAX_BKG_F; i += nt) img->bkg[i] = sh.bkg[i];
+    if (tid < P.n_phases) img->phase_ok[tid] = sh.phase_ok[tid];
+    if (tid == 0) { img->cell_ok = sh.cell_ok; img->prof_ok = sh.prof_ok; img->go = sh.go; img->boot = sh.accept; }
+}
+
+__device__ inline void image_load(const DevPlan &P, const TrialImage *img, Shared &sh) {
+    const int tid = threadIdx.x, nt = blockDim.x, na = total_atoms(P);
+    for (int i = tid; i < P.n_params; i += nt) sh.p[i] = img->p[i];
+    for (int i = tid; i < P.n_phases * 6; i += nt) sh.metric[i / 6][i % 6] = img->metric[i / 6][i % 6];
+    for (int i = tid; i < na * 3; i += nt) sh.apos[i / 3][i % 3] = img->apos[i / 3][i % 3];
+    for (int i = tid; i < na; i += nt) { sh.aocc[i] = img->aocc[i]; sh.aB[i] = img->aB[i]; }
+    for (int i = tid; i < P.n_phases * 16; i += nt) sh.prof[i >> 4][i & 15] = img->prof[i >> 4][i & 15];
+    for (int i = tid; i < PTK_MAX_BKG_F; i += nt) sh.bkg[i] = img->bkg[i];
+    if (tid < P.n_phases) sh.phase_ok[tid] = img->phase_ok[tid];
+    if (tid == 0) { sh.cell_ok = img->cell_ok; sh.prof_ok = img->prof_ok; sh.go = img->go; }
+}
+
+// Setup of trial t: a given vector (X != null), or the PT proposal of chain t
+// (js/pt_gpu.js:361 propose; rietveld_core.js:2196-2212): one coordinate j drawn
+// uniformly, x_j + z scale_j sqrt(T_c) reflected into its box, log u for the Metropolis
+// test.  Random numbers are keyed on (seed, chain, the chain's step counter).
+__global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__restrict__ X, TrialImage *img,
+                                                    int pt_mode, int bootstrap) {
     __shared__ Shared sh;
-    __shared__ double xs[3 * PTK_MAX_VARS];      // current, proposal, best
-    __shared__ double sc[PTK_S_COUNT_K];
-    float *ylds; LineRec *rec; int4 *span;
-    dyn_layout(P.np, ch, ylds, rec, span);
-    const int c = blockIdx.x, nv = P.nv;
-    if (c >= P.chains) return;
-    double *st = P.state + (size_t)c * P.stride;
-    stage_reps(P, sh);
-    for (int j = threadIdx.x; j < 3 * nv; j += blockDim.x) xs[j] = st[j];
-    for (int j = threadIdx.x; j < PTK_S_COUNT_K; j += blockDim.x) sc[j] = st[3 * nv + j];
-    const double temp = pow(P.max_temp, (double)c / (double)(P.chains > 1 ? P.chains - 1 : 1));
-    double *x = xs, *prop = xs + nv, *best = xs + 2 * nv;
-    for (int step = 0; step < steps; ++step) {
+    __shared__ double prop[PTK_MAX_VARS];
+    const int t = blockIdx.x, nv = P.nv;
+    bool boot = false;
+    double *st = nullptr;
+    if (pt_mode) {
+        st = P.state + (size_t)t * P.stride;
+        const double step = st[3 * nv + 2];
+        boot = bootstrap || step < 0.0;
+        for (int j = threadIdx.x; j < nv; j += blockDim.x) prop[j] = st[j];
         __syncthreads();
-        for (int j = threadIdx.x; j < nv; j += blockDim.x) prop[j] = x[j];
-        __syncthreads();
-        const double sweep = sc[2];
-        const bool boot = bootstrap || sweep < 0.0;
         if (threadIdx.x == 0 && !boot) {
-            unsigned long long ctr = counter0 + (unsigned long long)step;
-            double u0 = uniform01(P.seed, c, ctr, 0), u1 = uniform01(P.seed, c, ctr, 1), u2 = uniform01(P.seed, c, ctr, 2);
-            double u3 = uniform01(P.seed, c, ctr, 3);
+            const unsigned long long ctr = (unsigned long long)step;
+            const double temp = pow(P.max_temp, (double)t / (double)(P.chains > 1 ? P.chains - 1 : 1));
+            double u0 = uniform01(P.seed, t, ctr, 0), u1 = uniform01(P.seed, t, ctr, 1);
+            double u2 = uniform01(P.seed, t, ctr, 2), u3 = uniform01(P.seed, t, ctr, 3);
             int j = (int)floor(u0 * nv);
             if (j > nv - 1) j = nv - 1;
             double z = sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-            prop[j] = reflect_box(x[j] + z * P.vscale[j] * sqrt(temp), P.vlo[j], P.vhi[j]);
-            sc[8] = log(u3);
+            prop[j] = reflect_box(prop[j] + z * P.vscale[j] * sqrt(temp), P.vlo[j], P.vhi[j]);
+            st[3 * nv + 8] = log(u3);
         }
         __syncthreads();
+        for (int j = threadIdx.x; j < nv; j += blockDim.x) st[nv + j] = prop[j];
         materialize(P, prop, sh);
-        __syncthreads();
-        trial_setup(P, sh);
-        if (threadIdx.x == 0) sh.go = (sh.cell_ok && (boot || sc[11] < 0.5 || sh.prof_ok)) ? 1 : 0;
-        __syncthreads();
-        const bool valid = sh.go != 0;
-        double cost = 3e38;
-        if (valid) cost = eval_trial(P, sh, ylds, rec, span, ch);
-        if (threadIdx.x == 0) {
-            if (!(isfinite(cost)) || cost > 3e38) cost = 3e38;
-            sc[7] = valid ? 1.0 : 0.0;
-            sc[10] = cost;
-            double old = sc[0];
-            bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
-            int take_best = 0;
-            if (acc) {
-                sc[0] = cost;
-                sc[11] = sh.prof_ok ? 1.0 : 0.0;
-                if (!boot) sc[3] += 1.0;
-                if (sh.prof_ok && (boot || cost < sc[1])) { sc[1] = cost; take_best = 1; }
-            }
-            if (!boot) sc[4] += 1.0;
-            sc[2] = sweep + 1.0;
-            sh.accept = acc ? 1 + take_best : 0;
+    } else {
+        materialize(P, X + (size_t)t * nv, sh);
+    }
+    __syncthreads();
+    trial_setup(P, sh);
+    if (threadIdx.x == 0) {
+        bool go = sh.cell_ok;
+        if (pt_mode) {
+            // a feasible replica may not step outside the variance region (pt_gpu.js:361)
+            go = go && (boot || st[3 * nv + 11] < 0.5 || sh.prof_ok);
+            st[3 * nv + 7] = go ? 1.0 : 0.0;
         }
+        sh.go = go ? 1 : 0;
+        sh.accept = boot ? 1 : 0;
+    }
+    __syncthreads();
+    image_store(P, sh, img + t);
+}
+
+// line records of reflections [blockIdx.x * blockDim.x, ...) of trial blockIdx.y
+__global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, const TrialImage *__restrict__ img, LineRec *rec,
+                                                     int4 *span) {
+    __shared__ Shared sh;
+    const int t = blockIdx.y;
+    stage_reps(P, sh);
+    image_load(P, img + t, sh);
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.nr) return;
+    const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
+    if (sh.go) {
+        build_reflection(P, sh, k, rec + base, span + base);
+    } else {
+        span[base] = make_int4(0, 0, 0, 0);
+        span[base + 1] = make_int4(0, 0, 0, 0);
+    }
+}
+
+constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
+constexpr int kAccTilesPerWave = 2;    // 128-point tiles per wavefront (2 points per lane)
+constexpr int kAccPoints = kAccWaves * kAccTilesPerWave * 128;   // points per workgroup
+constexpr int kStage = 384;            // line records staged in LDS per pass
+
+// one line at the two points (i, i + 64) of a lane; `split` and `asym` are uniform over
+// the wavefront (scalar branches), the window test is a select
+__device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, float asym, float2 t0, float2 t1,
+                                          int i0, float &a0, float &a1) {
+    float d0 = (t0.x - R.a.x) + (t0.y - R.a.y);
+    float d1 = (t1.x - R.a.x) + (t1.y - R.a.y);
+    if (asym != 0.0f) {
+        if (fabsf(d0) >= 1e-9f) d0 = d0 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d0)));
+        if (fabsf(d1) >= 1e-9f) d1 = d1 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d1)));
+    }
+    float4 f0 = R.r, f1 = R.r;
+    if (split) {
+        if (d0 < 0.0f) f0 = R.l;
+        if (d1 < 0.0f) f1 = R.l;
+    }
+    const float q0 = d0 * d0, q1 = d1 * d1;
+    const float g0 = __builtin_amdgcn_exp2f(f0.x * q0), g1 = __builtin_amdgcn_exp2f(f1.x * q1);
+    const float l0 = __builtin_amdgcn_rcpf(fmaf(f0.y, q0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(f1.y, q1, 1.0f));
+    const float v0 = fmaxf(fmaf(f0.z, l0, fmaf(f0.w, g0, -R.a.w)), 0.0f);
+    const float v1 = fmaxf(fmaf(f1.z, l1, fmaf(f1.w, g1, -R.a.w)), 0.0f);
+    const float amp0 = (i0 >= v.x && i0 < v.y) ? R.a.z : 0.0f;
+    const float amp1 = (i0 + 64 >= v.x && i0 + 64 < v.y) ? R.a.z : 0.0f;
+    a0 = fmaf(amp0, v0, a0);
+    a1 = fmaf(amp1, v1, a1);
+}
+
+// Banded profile sum and partial chi^2 of points [1024 s, 1024 s + 1024) of trial t
+// (s = blockIdx.x, t = blockIdx.y): the lines whose windows meet those points are
+// compacted, in line order, into LDS; every wavefront then sums them for its own two
+// 128-point tiles (two points per lane) and the chi^2 of its points is reduced with
+// two-sum.
+__global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
+                                                     const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
+                                                     double2 *partial, int nsplit) {
+    __shared__ LineRec srec[kStage];
+    __shared__ int4 sspan[kStage];
+    __shared__ float bkg[PTK_MAX_BKG_F];
+    __shared__ int wcnt[kAccWaves];
+    __shared__ int s_count;
+    __shared__ float red_s[kAccWaves], red_c[kAccWaves];
+    const int t = blockIdx.y, split = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const TrialImage *im = img + t;
+    if (!im->go) {
+        if (tid == 0) partial[(size_t)t * nsplit + split] = make_double2(0.0, 0.0);
+        return;
+    }
+    if (tid < PTK_MAX_BKG_F) bkg[tid] = im->bkg[tid];
+    const int lo_pt = split * kAccPoints, hi_pt = min(P.np, lo_pt + kAccPoints);
+    const int nl = 2 * P.nr;
+    const LineRec *rec = grec + (size_t)t * nl;
+    const int4 *sp = gspan + (size_t)t * nl;
+    float acc[2 * kAccTilesPerWave];
+    float2 tf[2 * kAccTilesPerWave];
+#pragma unroll
+    for (int j = 0; j < kAccTilesPerWave; ++j) {
+        const int base = lo_pt + (wave + kAccWaves * j) * 128 + lane;
+        acc[2 * j] = acc[2 * j + 1] = 0.0f;
+        tf[2 * j] = base < P.np ? P.tff[base] : make_float2(0.f, 0.f);
+        tf[2 * j + 1] = base + 64 < P.np ? P.tff[base + 64] : make_float2(0.f, 0.f);
+    }
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nl; c0 += 256) {
+        // compact the overlapping lines of this chunk, in order, into the stage
+        const int L = c0 + tid;
+        int4 sv = make_int4(0, 0, 0, 0);
+        bool ov = false;
+        if (L < nl) { sv = sp[L]; ov = sv.x < hi_pt && sv.y > lo_pt; }
+        const unsigned long long m = __ballot(ov);
+        if (lane == 0) wcnt[wave] = __popcll(m);
         __syncthreads();
-        if (sh.accept) {
-            for (int j = threadIdx.x; j < nv; j += blockDim.x) {
-                x[j] = prop[j];
-                if (sh.accept == 2) best[j] = prop[j];
+        int off = s_count;
+        for (int w = 0; w < wave; ++w) off += wcnt[w];
+        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (ov) { sspan[off + pre] = sv; srec[off + pre] = rec[L]; }
+        __syncthreads();
+        if (tid == 0) { int n = s_count; for (int w = 0; w < kAccWaves; ++w) n += wcnt[w]; s_count = n; }
+        __syncthreads();
+        const int count = s_count;
+        if (count > kStage - 256 || (c0 + 256 >= nl && count > 0)) {
+#pragma unroll
+            for (int j = 0; j < kAccTilesPerWave; ++j) {
+                const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
+                if (t_lo < hi_pt) {
+                    const int t_hi = t_lo + 128, i0 = t_lo + lane;
+                    float a0 = acc[2 * j], a1 = acc[2 * j + 1];
+                    for (int lb = 0; lb < count; lb += 64) {
+                        const int q = lb + lane;
+                        bool o = false;
+                        if (q < count) { int4 v = sspan[q]; o = v.x < t_hi && v.y > t_lo; }
+                        unsigned long long mask = __ballot(o);
+                        while (mask) {
+                            const int b = __ffsll((long long)mask) - 1;
+                            mask &= mask - 1ull;
+                            const int4 v = sspan[lb + b];
+                            const LineRec &R = srec[lb + b];
+                            // line flags are the same for every lane: keep them scalar
+                            const int flags = __builtin_amdgcn_readfirstlane(v.w);
+                            const float asym = __int_as_float(__builtin_amdgcn_readfirstlane(v.z));
+                            line_pair(R, v, (flags & 1) != 0, asym, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                        }
+                    }
+                    acc[2 * j] = a0;
+                    acc[2 * j + 1] = a1;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_count = 0;
+            __syncthreads();
+        }
+    }
+    // chi^2 of this workgroup's points
+    float s = 0.f, c = 0.f;
+#pragma unroll
+    for (int j = 0; j < 2 * kAccTilesPerWave; ++j) {
+        const int i = lo_pt + (wave + kAccWaves * (j >> 1)) * 128 + (j & 1) * 64 + lane;
+        if (i < hi_pt) {
+            const float2 ow = P.obs[i];
+            if (ow.y > 0.0f) {
+                float yb = 0.0f;
+                if (P.n_bkg > 0) {
+                    float z = (float)(P.thi > P.tlo ? 2.0 * (P.tth[i] - P.tlo) / (P.thi - P.tlo) - 1.0 : 0.0);
+                    float ta = 1.0f, tb = z;
+                    yb = bkg[0];
+                    if (P.n_bkg > 1) yb = fmaf(bkg[1], z, yb);
+                    for (int k = 2; k < P.n_bkg; ++k) {
+                        float nx = 2.0f * z * tb - ta;
+                        ta = tb; tb = nx;
+                        yb = fmaf(bkg[k], nx, yb);
+                    }
+                }
+                const float r = ow.x - (yb + acc[j]);
+                neumaier(s, c, ow.y * r * r);
             }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        float so = __shfl_xor(s, o), co = __shfl_xor(c, o);
+        float tt = s + so;
+        float e = fabsf(s) >= fabsf(so) ? (s - tt) + so : (so - tt) + s;
+        s = tt; c = c + co + e;
+    }
+    if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
     __syncthreads();
-    for (int j = threadIdx.x; j < 3 * nv; j += blockDim.x) st[j] = xs[j];
-    for (int j = threadIdx.x; j < PTK_S_COUNT_K; j += blockDim.x) st[3 * nv + j] = sc[j];
+    if (tid == 0) {
+        double hi = 0.0, lo = 0.0;
+        for (int w = 0; w < kAccWaves; ++w) { hi += (double)red_s[w]; lo += (double)red_c[w]; }
+        partial[(size_t)t * nsplit + split] = make_double2(hi, lo);
+    }
+}
+
+// chi^2 of trial t (ordered sum of its partials) and, in PT mode, the Metropolis step of
+// chain t in FP64 (js/pt_gpu.js:363-364, rietveld_core.js:2213-2222).
+__global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage *__restrict__ img,
+                                                     const double2 *__restrict__ partial, int nsplit, int ntrials,
+                                                     double *chi2, int pt_mode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntrials) return;
+    const TrialImage *im = img + t;
+    double cost = 3e38;
+    if (im->go) {
+        double hi = 0.0, lo = 0.0;
+        for (int k = 0; k < nsplit; ++k) { double2 v = partial[(size_t)t * nsplit + k]; hi += v.x; lo += v.y; }
+        cost = hi + lo;
+        if (!isfinite(cost) || cost > 3e38) cost = 3e38;
+    }
+    if (!pt_mode) { chi2[t] = cost; return; }
+    const int nv = P.nv;
+    double *st = P.state + (size_t)t * P.stride;
+    double *sc = st + 3 * nv;
+    const bool boot = im->boot != 0;
+    const double step = sc[2];
+    const double temp = pow(P.max_temp, (double)t / (double)(P.chains > 1 ? P.chains - 1 : 1));
+    sc[10] = cost;
+    const double old = sc[0];
+    const bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
+    if (acc) {
+        const bool take_best = im->prof_ok && (boot || cost < sc[1]);
+        for (int j = 0; j < nv; ++j) {
+            st[j] = st[nv + j];
+            if (take_best) st[2 * nv + j] = st[nv + j];
+        }
+        sc[0] = cost;
+        sc[11] = im->prof_ok ? 1.0 : 0.0;
+        if (!boot) sc[3] += 1.0;
+        if (take_best) sc[1] = cost;
+    }
+    if (!boot) sc[4] += 1.0;
+    sc[2] = step + 1.0;
 }
 
 // ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
-__global__ void k_exchange(DevPlan P, int parity, unsigned long long counter) {
+__global__ void k_exchange(DevPlan P) {
+    const int nv = P.nv;
+    // sweeps completed, from chain 0's step counter (js/pt_gpu.js:365 parity rule)
+    const double done = P.state[3 * nv + 2] / (double)P.steps_per_sweep;
+    const long long sw = (long long)done;
+    const int parity = (int)(((sw - 1) % 2 + 2) % 2);
     int pair = blockIdx.x * blockDim.x + threadIdx.x;
     int c = 2 * pair + parity;
     if (c + 1 >= P.chains) return;
-    const int nv = P.nv;
     double *a = P.state + (size_t)c * P.stride, *b = P.state + (size_t)(c + 1) * P.stride;
     double den = (double)(P.chains > 1 ? P.chains - 1 : 1);
     double t0 = pow(P.max_temp, (double)c / den), t1 = pow(P.max_temp, (double)(c + 1) / den);
     double la = 0.5 * (1.0 / t0 - 1.0 / t1) * (a[3 * nv] - b[3 * nv]);
     a[3 * nv + 6] += 1.0;
-    bool sw = la >= 0.0;
-    if (!sw) sw = log(uniform01(P.seed ^ 0x5bd1e995u, c, counter, 7)) < la;
-    if (sw) {
+    bool swp = la >= 0.0;
+    if (!swp) swp = log(uniform01(P.seed ^ 0x5bd1e995u, c, (unsigned long long)sw, 7)) < la;
+    if (swp) {
         for (int j = 0; j < nv; ++j) { double v = a[j]; a[j] = b[j]; b[j] = v; }
         double v = a[3 * nv]; a[3 * nv] = b[3 * nv]; b[3 * nv] = v;
         v = a[3 * nv + 11]; a[3 * nv + 11] = b[3 * nv + 11]; b[3 * nv + 11] = v;
@@ -576,19 +805,34 @@ namespace ptk {
 
 size_t shared_static_bytes() { return sizeof(Shared); }
 size_t plain_scratch_bytes() { return sizeof(PlainScratch); }
+size_t trial_image_bytes() { return sizeof(TrialImage); }
+int accum_points_per_group() { return kAccPoints; }
 
 hipError_t set_lds_limits(size_t bytes) {
     hipError_t e;
-    if ((e = hipFuncSetAttribute((const void *)k_eval_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
     if ((e = hipFuncSetAttribute((const void *)k_workload, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
     if ((e = hipFuncSetAttribute((const void *)k_pattern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
-    if ((e = hipFuncSetAttribute((const void *)k_pt_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_plain_points, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_eval_batch(const DevPlan &P, const double *X, double *chi2, int ntrials, int grid, int block, int ch,
-                             size_t dyn, hipStream_t s) {
-    hipLaunchKernelGGL(k_eval_batch, dim3(grid), dim3(block), dyn, s, P, X, chi2, ntrials, ch);
+hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap);
+    return hipGetLastError();
+}
+hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_lines, dim3((P.nr + 127) / 128, ntrials), dim3(128), 0, s, P, (const TrialImage *)img, rec, span);
+    return hipGetLastError();
+}
+hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
+                              int nsplit, int ntrials, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_accum, dim3(nsplit, ntrials), dim3(256), 0, s, P, (const TrialImage *)img, rec, span, partial, nsplit);
+    return hipGetLastError();
+}
+hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
+                               double *chi2, int pt_mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_finish, dim3((ntrials + 63) / 64), dim3(64), 0, s, P, (const TrialImage *)img, partial, nsplit,
+                       ntrials, chi2, pt_mode);
     return hipGetLastError();
 }
 
@@ -603,15 +847,9 @@ hipError_t launch_workload(const DevPlan &P, const double *X, long long *stats, 
     return hipGetLastError();
 }
 
-hipError_t launch_pt_sweep(const DevPlan &P, int steps, int bootstrap, int block, int ch, size_t dyn,
-                           unsigned long long counter0, hipStream_t s) {
-    hipLaunchKernelGGL(k_pt_sweep, dim3(P.chains), dim3(block), dyn, s, P, steps, bootstrap, ch, counter0);
-    return hipGetLastError();
-}
-
-hipError_t launch_exchange(const DevPlan &P, int parity, unsigned long long counter, hipStream_t s) {
+hipError_t launch_exchange(const DevPlan &P, hipStream_t s) {
     int pairs = (P.chains + 1) / 2;
-    hipLaunchKernelGGL(k_exchange, dim3((pairs + 63) / 64), dim3(64), 0, s, P, parity, counter);
+    hipLaunchKernelGGL(k_exchange, dim3((pairs + 63) / 64), dim3(64), 0, s, P);
     return hipGetLastError();
 }
 

@@ -89,7 +89,7 @@ struct DevPlan {
 //   a = {centre_hi, centre_lo (float-float centre, deg), amplitude, pedestal}
 //   l = {kg, kl, cl, cg} left flank (delta < 0), r = the same for the right flank, where
 //       g = exp2(kg delta^2), l = 1/(1 + kl delta^2), value = cl l + cg g - pedestal
-//   span = {start, stop, asymmetry (float bits), 0}: points in [start, stop)
+//   span = {start, stop, asymmetry (float bits), 1 when the flanks differ}: points in [start, stop)
 struct LineRec {
     float4 a, l, r;
 };

@@ -1,0 +1,230 @@
+/* pt_hip.js -- MI355X parallel-tempering backend with the RVPTGPU surface of
+ * js/pt_gpu.js:424 ({compile, create, run}), over the N-API addon pt_addon.node.
+ *
+ * Written in Node 12 syntax (no ?? or ?.), loadable with require() or importScripts-like
+ * evaluation.  rietveld_worker.js would load it where it loads pt_gpu.js
+ * (js/rietveld_worker.js:15-17) and RVCore.refineAsync would call run(ctx)
+ * (js/rietveld_core.js:2455-2459); any throw there makes the caller fall back to CPU PT.
+ *
+ * ctx is the context RVCore yields (js/rietveld_core.js:2271-2273): {problem, model, x,
+ * info, setup, constraints, cost, feasible, shouldStop, onIter, maxIter}.  fromContext()
+ * flattens it into the C-ABI problem (include/powder6_pt.h); a ctx may instead carry
+ * that flat problem directly as ctx.flat.
+ */
+'use strict';
+var path = require('path');
+
+var addon = null;
+function native() {
+    if (!addon) {
+        try {
+            addon = require(path.join(__dirname, '..', '_lib', 'pt_addon.node'));
+        } catch (e) {
+            throw new Error('MI355X PT addon is unavailable in this process or context: ' + e.message);
+        }
+    }
+    return addon;
+}
+
+var SYSTEM = { triclinic: 0, monoclinic: 1, orthorhombic: 2, tetragonal: 3, trigonal: 4, hexagonal: 5, rhombohedral: 5, cubic: 6 };
+var PROFILE = { simple_pvoigt: 1, split_pvoigt: 2, tch_aniso: 3 };
+var SLOTS = {
+    tch_aniso: ['U', 'V', 'W', 'X', 'Y', 'SL', 'HL'],
+    simple_pvoigt: ['GU', 'GV', 'GW', 'GP', 'LX', 'eta'],
+    split_pvoigt: ['GU_L', 'GV_L', 'GW_L', 'LX_L', 'GU_R', 'GV_R', 'GW_R', 'LX_R', 'eta_split']
+};
+var CELL = ['a', 'b', 'c', 'alpha', 'beta', 'gamma'];
+var P_PHASE0 = 27, PH_ATOM0 = 24, AT_STRIDE = 5;
+
+/* Parameter index of a reference value name in the C-ABI layout, or -1. */
+function paramIndex(layout, name) {
+    if (name === 'zero') return 0;
+    if (name === 'disp') return 1;
+    if (name === 'transp') return 2;
+    var m = /^bkg(\d+)$/.exec(name);
+    if (m) return +m[1] < 24 ? 3 + (+m[1]) : -1;
+    for (var q = 0; q < layout.length; q++) {
+        var L = layout[q];
+        if (name.indexOf(L.prefix) !== 0) continue;
+        var rest = name.slice(L.prefix.length);
+        var c = CELL.indexOf(rest);
+        if (c >= 0) return L.base + c;
+        if (rest === 'scale') return L.base + 6;
+        if (rest === 'po') return L.base + 7;
+        var s = L.slots.indexOf(rest);
+        if (s >= 0) return L.base + 8 + s;
+        var am = /^(.+)\.(x|y|z|occ|B)$/.exec(rest);
+        if (am) {
+            var j = L.labels.indexOf(am[1]);
+            if (j >= 0) return L.base + PH_ATOM0 + AT_STRIDE * j + ['x', 'y', 'z', 'occ', 'B'].indexOf(am[2]);
+        }
+    }
+    return -1;
+}
+
+/* Flatten a reference PT context (one dataset, Rietveld phases, Chebyshev background,
+ * non-separated TCH / simple / split pseudo-Voigt) into the C-ABI problem. */
+function fromContext(ctx) {
+    if (ctx.flat) return ctx.flat;
+    var model = ctx.model, problem = ctx.problem;
+    if (!model || model.datasets.length !== 1) throw new Error('MI355X PT backend handles one dataset per fit; using the CPU path.');
+    var D = model.datasets[0], def = D.def;
+    if (D.spline) throw new Error('MI355X PT backend has no spline background yet; using the CPU path.');
+    var values = problem.values[0], layout = [], base = P_PHASE0, phases = [];
+    model.phases.forEach(function (P, q) {
+        if (!D.has[q]) throw new Error('MI355X PT backend needs every phase in the dataset.');
+        if (P.method !== 'rietveld') throw new Error('MI355X PT backend evaluates Rietveld phases only; using the CPU path.');
+        var type = P.def.profileType;
+        if (P.def.profileType === 'gsas_cw' || P.def.separatedProfile) throw new Error('MI355X PT backend has no separated (GSAS) profile yet; using the CPU path.');
+        if (!PROFILE[type]) throw new Error('unknown profile type ' + type);
+        var atoms = P.def.atoms || [];
+        layout.push({ prefix: P.prefix, base: base, slots: SLOTS[type], labels: atoms.map(function (a) { return a.label; }) });
+        var opR = [], opT = [];
+        P.def.ops.forEach(function (o) { for (var i = 0; i < 9; i++) opR.push(o.r[i]); for (var k = 0; k < 3; k++) opT.push(o.t[k]); });
+        var anom = (D.anom && D.anom[q]) || [];
+        phases.push({
+            system: SYSTEM[String(P.def.system || 'triclinic').toLowerCase()] || 0,
+            uniqueAxis: { a: 0, b: 1, c: 2 }[P.def.uniqueAxis || 'b'],
+            profileType: PROFILE[type], opR: Int32Array.from(opR), opT: Float64Array.from(opT),
+            poDir: P.def.poDir || [0, 0, 1],
+            atoms: atoms.map(function (at, j) {
+                var site = at.site, basis = [0, 0, 0, 0, 0, 0, 0, 0, 0], mask = 0;
+                (site.free || []).forEach(function (f) { mask |= 1 << f; for (var i = 0; i < 3; i++) basis[3 * f + i] = site.basis[f][i]; });
+                var xr = (at.ff && at.ff.xray) || { a: [], b: [], c: 0 }, an = anom[j] || [0, 0];
+                return { r0: site.r0, basis: basis, freeMask: mask, mult: site.mult || 1, ffA: xr.a, ffB: xr.b, ffC: xr.c || 0,
+                    bNeutron: (at.ff && at.ff.neutron) || 0, fp: an[0], fpp: an[1] };
+            })
+        });
+        base += PH_ATOM0 + AT_STRIDE * atoms.length;
+    });
+    var nParams = base, paramBase = new Float64Array(nParams);
+    Object.keys(values).forEach(function (name) {
+        var k = paramIndex(layout, name);
+        if (k >= 0 && Number.isFinite(values[name])) paramBase[k] = values[name];
+    });
+    // a monoclinic or triclinic cell reads all six slots; default the unused angles to 90
+    layout.forEach(function (L) { for (var c = 3; c < 6; c++) if (!paramBase[L.base + c]) paramBase[L.base + c] = 90; });
+    var termParam = [], termVar = [], termCoeff = [];
+    problem.vars.forEach(function (vr, j) {
+        var k = paramIndex(layout, vr.name);
+        if (k < 0) throw new Error('MI355X PT backend cannot refine ' + vr.name + '; using the CPU path.');
+        paramBase[k] = 0; termParam.push(k); termVar.push(j); termCoeff.push(1);
+    });
+    if (ctx.constraints) ctx.constraints.links.forEach(function (link) {
+        var k = paramIndex(layout, link.name);
+        if (k < 0) throw new Error('MI355X PT backend cannot link ' + link.name + '; using the CPU path.');
+        paramBase[k] = link.offset;
+        link.terms.forEach(function (t) { termParam.push(k); termVar.push(t[0]); termCoeff.push(t[1]); });
+    });
+    return {
+        tth: Float64Array.from(def.tth), y: Float64Array.from(def.y), w: Float64Array.from(def.w),
+        lambda1: def.lambda1, lambda2: def.lambda2 || 0, ratio: def.ratio || 0, polK: def.polK, minFwhm: def.minFwhm,
+        radiation: def.radiation === 'neutron' ? 1 : 0, nBkg: D.T.length, phases: phases, paramBase: paramBase,
+        x0: Float64Array.from(ctx.x), lo: Float64Array.from(ctx.info.map(function (i) { return i.lo; })),
+        hi: Float64Array.from(ctx.info.map(function (i) { return i.hi; })), scale: Float64Array.from(ctx.setup.scales),
+        termParam: Int32Array.from(termParam), termVar: Int32Array.from(termVar), termCoeff: Float64Array.from(termCoeff),
+        chains: ctx.setup.chains, sweeps: ctx.setup.sweeps, stepsPerSweep: 0, maxTemp: ctx.setup.maxTemp, seed: ctx.setup.seed
+    };
+}
+
+/* compile(ctx): the plan summary without a device (js/pt_gpu.js:226 fields) */
+function compile(ctx) {
+    var r = native().planCreate(fromContext(ctx), false), i = r.info;
+    native().planDestroy(r.handle);
+    return { nv: i.nv, chains: i.chains, stride: i.stride, pointCount: i.pointCount, reflectionCount: i.reflectionCount,
+             columnCount: i.lineCount, workCount: i.pointCount, datasets: 1, paramCount: i.paramCount };
+}
+
+function flatten(vectors, nv) {
+    var out = new Float64Array(vectors.length * nv);
+    vectors.forEach(function (v, c) { for (var j = 0; j < nv; j++) out[c * nv + j] = v[j]; });
+    return out;
+}
+
+/* create(ctx) -> Promise<session> (js/pt_gpu.js:371-406) */
+function create(ctx) {
+    return new Promise(function (resolve) {
+        var A = native(), r = A.planCreate(fromContext(ctx), true), h = r.handle, plan = r.info, closed = false;
+        resolve({
+            plan: plan,
+            initial: function (vectors) { return A.initial(h, flatten(vectors, plan.nv)); },
+            costs: function (vectors) { return A.costs(h, flatten(vectors, plan.nv)).then(function (c) { return Array.from(c); }); },
+            batch: function (count, bootstrap) { return A.batch(h, count, !!bootstrap); },
+            evalTrials: function (form, vectors) { return A.evalTrials(h, form, flatten(vectors, plan.nv)); },
+            pattern: function (x) { return A.pattern(h, Float64Array.from(x)); },
+            reflections: function (phase) { return A.reflections(h, phase || 0); },
+            close: function () { if (!closed) { closed = true; A.planDestroy(h); } }
+        });
+    });
+}
+
+function reflected(value, lo, hi) {
+    if (Number.isFinite(lo) && Number.isFinite(hi)) {
+        var span = hi - lo;
+        if (span <= 0) return lo;
+        var folded = ((value - lo) % (2 * span) + 2 * span) % (2 * span);
+        return lo + (folded <= span ? folded : 2 * span - folded);
+    }
+    return value < lo ? 2 * lo - value : value > hi ? 2 * hi - value : value;
+}
+
+/* run(ctx) (js/pt_gpu.js:409-423): validate against ctx.cost (FP64), PT sweeps on the GPU,
+ * FP64 re-scoring of every new best, best-so-far kept; resolves {x, stopped, pt}. */
+function run(ctx, options) {
+    options = options || {};
+    var tolerance = options.tolerance || 1e-3, gpu = null;
+    return create(ctx).then(function (g) {
+        gpu = g;
+        var nv = ctx.x.length, chains = ctx.setup.chains;
+        var probes = [Array.from(ctx.x), Array.from(ctx.x, function (v, j) { return reflected(v + ctx.setup.scales[j] * 0.25, ctx.info[j].lo, ctx.info[j].hi); })];
+        return gpu.costs(probes).then(function (gc) {
+            var validationError = 0;
+            for (var c = 0; c < chains; c++) {
+                var cpu = ctx.cost(probes[c % probes.length]);
+                if (!Number.isFinite(cpu)) continue;
+                var error = Math.abs(gc[c] - cpu) / Math.max(1, cpu);
+                validationError = Math.max(validationError, error);
+                if (!Number.isFinite(gc[c]) || error > tolerance)
+                    throw new Error('MI355X PT cost disagrees with the double-precision model (relative error ' + error.toPrecision(3) + ').');
+            }
+            var best = Float64Array.from(ctx.x), bestCost = ctx.setup.initialCost, completed = 0, stopped = false, seen = new Set(), state = null;
+            var stride = gpu.plan.stride;
+            function inspect() {
+                for (var c = 0; c < chains; c++) {
+                    var b = c * stride, candidate = Array.from(state.subarray(b + 2 * nv, b + 3 * nv)), key = candidate.join(',');
+                    if (seen.has(key)) continue;
+                    seen.add(key);
+                    if (!candidate.every(function (v, j) { return Number.isFinite(v) && v >= ctx.info[j].lo && v <= ctx.info[j].hi; })) continue;
+                    if (ctx.feasible && !ctx.feasible(candidate)) continue;
+                    var value = ctx.cost(candidate);
+                    if (value < bestCost) { best = Float64Array.from(candidate); bestCost = value; }
+                }
+            }
+            function loop() {
+                if (completed >= ctx.setup.sweeps) return Promise.resolve();
+                if (ctx.shouldStop && ctx.shouldStop()) { stopped = true; return Promise.resolve(); }
+                var count = Math.min(Math.max(1, Math.ceil(8 / nv)), ctx.setup.sweeps - completed);
+                return gpu.batch(count, false).then(function (s) {
+                    state = s; completed += count; inspect();
+                    if (ctx.onIter) ctx.onIter({ it: completed, chi2: bestCost, maxIter: ctx.maxIter, stage: 'pt', backend: 'mi355x' });
+                    return loop();
+                });
+            }
+            return gpu.initial([ctx.x]).then(function () { return gpu.batch(1, true); }).then(function (s) { state = s; return loop(); }).then(function () {
+                var stats = { accepted: 0, proposals: 0, swaps: 0, swapAttempts: 0 };
+                for (var c = 0; c < chains; c++) {
+                    var b = c * stride + 3 * nv;
+                    stats.accepted += state[b + 3]; stats.proposals += state[b + 4]; stats.swaps += state[b + 5]; stats.swapAttempts += state[b + 6];
+                }
+                var pt = { backend: 'mi355x', chains: chains, sweeps: completed, requestedSweeps: ctx.setup.sweeps, maxTemp: ctx.setup.maxTemp,
+                           seed: ctx.setup.seed, initialChi2: ctx.setup.initialCost, bestChi2: bestCost, validationError: validationError };
+                Object.keys(stats).forEach(function (k) { pt[k] = stats[k]; });
+                return { x: best, stopped: stopped, pt: pt };
+            });
+        });
+    }).then(function (result) { if (gpu) gpu.close(); return result; }, function (e) { if (gpu) gpu.close(); throw e; });
+}
+
+var RVPTGPU = { compile: compile, create: create, run: run, fromContext: fromContext, FORM_PLAIN: 0, FORM_BATCHED: 1 };
+if (typeof module !== 'undefined' && module.exports) module.exports = RVPTGPU;
+if (typeof self !== 'undefined') self.RVPTGPU = RVPTGPU;

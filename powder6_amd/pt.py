@@ -159,6 +159,7 @@ def lib():
         L.pt_last_timing.argtypes = [ctypes.c_void_p, P_D, P_D, P_I]
         L.pt_phase_base.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_workload.argtypes = [ctypes.c_void_p, P_D, P_D]
+        L.pt_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -251,6 +252,10 @@ class Plan:
         _check(lib().pt_workload(self.h, _dptr(x[0]), _dptr(out)))
         return dict(lines=int(out[0]), w_tot=int(out[1]), n_hkl=int(out[2]), s_terms=int(out[3]), np=int(out[4]),
                     n_bkg=int(out[5]))
+
+    def set_kernel_timing(self, on):
+        """Direct launches with HIP events around the dominant kernel (no sweep graph)."""
+        _check(lib().pt_set_kernel_timing(self.h, 1 if on else 0))
 
     def timing(self):
         t, k, n = c_double(), c_double(), c_int32()

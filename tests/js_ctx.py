@@ -1,0 +1,56 @@
+"""A PT context shaped like the one RVCore yields (js/rietveld_core.js:2271-2273), built
+from a problems.Problem, for driving powder6_amd/node/pt_hip.js from tests.  JSON has no
+infinities: +-inf limits are written as null and restored by tests/js/pt_hip_probe.js."""
+import json
+import math
+
+import numpy as np
+
+
+def _num(v):
+    v = float(v)
+    return None if not math.isfinite(v) else v
+
+
+def reference_ctx(pb, initial_cost=None):
+    values = {}
+    p = pb.values()
+    for name, k in pb.index.items():
+        if name.startswith('p1._prof') or '._prof' in name:
+            continue
+        values[name] = float(p[k])
+    phases = []
+    for q, ph in enumerate(pb.phases):
+        atoms = []
+        for at in ph.atoms:
+            basis = {str(f): list(map(float, at.site['basis'][f])) for f in at.site['free']}
+            atoms.append({'label': at.label, 'ff': {'xray': {'a': list(at.ff_a), 'b': list(at.ff_b), 'c': at.ff_c},
+                                                    'neutron': at.b_neutron},
+                          'site': {'r0': list(map(float, at.site['r0'])), 'free': list(at.site['free']),
+                                   'basis': basis, 'mult': at.site['mult']}})
+        ops = [{'r': [int(v) for v in r], 't': [float(v) for v in t]} for r, t in zip(ph.op_r, ph.op_t)]
+        phases.append({'def': {'key': ph.key, 'system': ph.system, 'uniqueAxis': 'abc'[ph.unique_axis],
+                               'profileType': ph.profile, 'ops': ops, 'poDir': list(ph.po_dir), 'atoms': atoms},
+                       'method': 'rietveld', 'prefix': ph.key + '.'})
+    anom = [[[at.fp, at.fpp] for at in ph.atoms] for ph in pb.phases]
+    ds = {'def': {'tth': pb.tth.tolist(), 'y': pb.y.tolist(), 'w': pb.w.tolist(), 'lambda1': pb.lambda1,
+                  'lambda2': pb.lambda2, 'ratio': pb.ratio, 'polK': pb.pol_k, 'minFwhm': pb.min_fwhm,
+                  'radiation': 'neutron' if pb.radiation else 'xray'},
+          'spline': False, 'T': [0] * pb.n_bkg, 'has': [True] * len(pb.phases), 'anom': anom}
+    return {
+        'problem': {'values': [values], 'vars': [{'name': n} for n in pb.vars]},
+        'model': {'datasets': [ds], 'phases': phases},
+        'x': [float(v) for v in pb.x0],
+        'info': [{'lo': _num(lo), 'hi': _num(hi)} for lo, hi in zip(pb.lo, pb.hi)],
+        'setup': {'scales': [float(s) for s in pb.scale], 'chains': pb.chains, 'sweeps': pb.sweeps,
+                  'maxTemp': pb.max_temp, 'seed': pb.seed, 'initialCost': initial_cost},
+        'constraints': None, 'maxIter': pb.sweeps,
+    }
+
+
+def write_ctx(pb, path, **extra):
+    ctx = reference_ctx(pb, extra.pop('initial_cost', None))
+    ctx.update(extra)
+    with open(path, 'w') as f:
+        json.dump(ctx, f)
+    return path

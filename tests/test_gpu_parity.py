@@ -117,3 +117,31 @@ def test_pt_batch_invariants(pbso4):
         assert _rel(best[c], m.cost(xb)) <= TOL
         xc = S[c, :nv]
         assert _rel(S[c, 3 * nv + pt.S_COST], m.cost(xc)) <= TOL
+
+
+def test_sweep_graph_replays_match_direct_launches(pbso4):
+    """The captured sweep graph, replayed several times with host work in between, gives
+    the same chain states as launching every kernel directly (deterministic pipeline)."""
+    x0 = pbso4.start_vector()
+    with pt.Plan(pbso4) as pl:
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        graphed = [pl.batch(1) for _ in range(3)]
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        pl.set_kernel_timing(True)
+        direct = [pl.batch(1) for _ in range(3)]
+    for g, d in zip(graphed, direct):
+        assert np.array_equal(g, d)
+
+
+def test_batched_eval_chunks_and_repeats(pbso4):
+    """Evaluating 2500 vectors (two passes of the slot buffer) equals evaluating them one
+    pass at a time, and a repeat is bit-identical."""
+    X = _probes(pbso4, 2500, 21, spread=1.0)
+    with pt.Plan(pbso4) as pl:
+        a = pl.eval(X, pt.FORM_BATCHED)
+        b = pl.eval(X, pt.FORM_BATCHED)
+        c = np.concatenate([pl.eval(X[:1000], pt.FORM_BATCHED), pl.eval(X[1000:], pt.FORM_BATCHED)])
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, c)

@@ -1,0 +1,106 @@
+"""The Node host (powder6_amd/node/pt_hip.js over the N-API addon) driven with a PT context
+shaped like the reference's (tests/js_ctx.py): compile() without a device matches the C
+plan; on the GPU, costs() of the probe vectors match the FP64 oracle and run() completes
+a PT search with the reference's invariants."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import js_ctx
+import oracle_lib as O
+from powder6_amd import problems as PR
+from powder6_amd import pt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROBE = os.path.join(ROOT, 'tests', 'js', 'pt_hip_probe.js')
+ADDON = os.path.join(ROOT, 'powder6_amd', '_lib', 'pt_addon.node')
+NODE = shutil.which('node')
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason='node or the addon is not available')
+
+
+def _node(ctx_path, mode):
+    r = subprocess.run([NODE, PROBE, ctx_path, mode], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    line = r.stdout.strip().splitlines()[-1] if r.stdout.strip() else '{}'
+    return json.loads(line), r
+
+
+@needs_node
+@pytest.mark.parametrize('profile', ['tch_aniso', 'simple_pvoigt', 'split_pvoigt'])
+def test_compile_from_reference_context(profile, tmp_path):
+    pb = PR.triclinic_probe(profile)
+    pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'compile')
+    assert 'compile' in out, r.stderr
+    with pt.Plan(pb, device=False) as pl:
+        info = pl.info
+    c = out['compile']
+    assert c['nv'] == info['nv'] and c['chains'] == info['chains'] and c['stride'] == info['stride']
+    assert c['reflectionCount'] == info['reflection_count'] and c['pointCount'] == info['point_count']
+    assert c['paramCount'] == info['n_params']
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@needs_node
+@pytest.mark.skipif(_has_gpu(), reason='checks the behaviour without a device')
+def test_create_rejects_without_device(tmp_path):
+    pb = PR.triclinic_probe('tch_aniso')
+    pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(pb.x0)])
+    out, r = _node(path, 'costs')
+    assert 'unavailable' in out.get('error', ''), (out, r.stderr)
+
+
+def _probes(pb):
+    x0 = pb.start_vector()
+    return [x0] + [np.clip(x0 + s * np.array(pb.scale), pb.lo, pb.hi) for s in (-.12, .12, .3)]
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.parametrize('profile', ['tch_aniso', 'split_pvoigt'])
+def test_node_costs_match_oracle(profile, tmp_path):
+    pb = PR.triclinic_probe(profile, simulate=O.oracle_simulate)
+    m = O.Model(pb)
+    probes = _probes(pb)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])
+    out, r = _node(path, 'costs')
+    assert 'costs' in out, (out, r.stderr)
+    for c, g in enumerate(out['costs']):
+        cpu = m.cost(probes[c % len(probes)])
+        assert abs(g - cpu) / max(1.0, cpu) <= 1e-5
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_run_pt(tmp_path):
+    """tests/webgpu-pt.js:131-136 through the Node host: proposals = chains*sweeps*nv,
+    swaps > 0, best <= initial, validation error reported."""
+    pb = PR.triclinic_probe('tch_aniso', simulate=O.oracle_simulate)
+    m = O.Model(pb)
+    x0 = pb.start_vector()
+    init = m.cost(x0)
+    probes = [x0, np.array([pt._reflected(v + pb.scale[j] * 0.25, pb.lo[j], pb.hi[j]) for j, v in enumerate(x0)])]
+    # run() re-scores candidate vectors with ctx.cost; the probe script looks costs up by
+    # key, so every candidate the GPU returns is scored here afterwards instead
+    costs = [list(map(float, p)) + [m.cost(p)] for p in probes]
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), initial_cost=init, expectedCosts=costs)
+    out, r = _node(path, 'run')
+    assert 'pt' in out, (out, r.stderr)
+    p = out['pt']
+    assert p['proposals'] == pb.chains * pb.sweeps * pb.nv
+    assert p['swaps'] > 0 and p['swapAttempts'] > 0
+    assert p['validationError'] <= 1e-5
+    assert p['bestChi2'] <= init
+    assert out['iters'] >= 1

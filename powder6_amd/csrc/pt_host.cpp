@@ -674,11 +674,19 @@ int pt_costs(pt_plan *pl, const double *x, int nvec, double *chi2) {
 }
 
 // one sweep: steps_per_sweep x (setup/propose, lines, accumulate, finish/accept), then the
-// replica exchange; accum launches bracketed by events when ev != null
+// replica exchange; the finish of each step but the last is fused with the next setup.
+// Accumulate launches are bracketed by events when ev != null.
 static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
+    HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, pl->chains, 1, 0, pl->stream), "setup");
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
-        int rc = run_stages(pl, nullptr, pl->chains, 1, 0, pl->d_schi2, ev ? ev + 2 * s : nullptr);
-        if (rc) return rc;
+        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
+        if (ev) HIPCHECK(hipEventRecord(ev[2 * s], pl->stream), "timing");
+        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->nsplit, pl->chains, pl->stream), "accumulate");
+        if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], pl->stream), "timing");
+        if (s + 1 < pl->steps_per_sweep)
+            HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->nsplit, pl->stream), "finish");
+        else
+            HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->nsplit, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
     }
     HIPCHECK(ptk::launch_exchange(pl->P, pl->stream), "exchange");
     return 0;

@@ -26,6 +26,8 @@ struct Shared {
     float bkg[PTK_MAX_BKG_F];                 // Chebyshev coefficients (FP32)
     int rep_r[PTK_MAX_REPS_TOTAL][9];
     double rep_t[PTK_MAX_REPS_TOTAL][3];
+    float aff[PTK_MAX_ATOMS][12];             // atom constants: a1..a5, b1..b5, c, w0
+    float afp[PTK_MAX_ATOMS][2];              // f', f'' (or the neutron length in [0])
     float red_s[32], red_c[32];
     int phase_ok[PTK_MAX_PHASES];
     int cell_ok, prof_ok, go, accept;
@@ -35,12 +37,24 @@ struct Shared {
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-// stage the coset representatives of every phase in LDS (once per launch)
+// stage the coset representatives and the atoms' scattering constants of every phase in
+// LDS (once per launch)
 __device__ inline void stage_reps(const DevPlan &P, Shared &sh) {
-    int total = 0;
-    for (int q = 0; q < P.n_phases; ++q) total += P.phase[q].n_reps;
+    int total = 0, na = 0;
+    for (int q = 0; q < P.n_phases; ++q) { total += P.phase[q].n_reps; na += P.phase[q].n_atoms; }
     for (int i = threadIdx.x; i < total * 9; i += blockDim.x) sh.rep_r[i / 9][i % 9] = P.rep_r[i];
     for (int i = threadIdx.x; i < total * 3; i += blockDim.x) sh.rep_t[i / 3][i % 3] = P.rep_t[i];
+    for (int j = threadIdx.x; j < na; j += blockDim.x) {
+        const DevAtom &A = P.atoms[j];
+        for (int g = 0; g < 5; ++g) {
+            sh.aff[j][g] = g < A.n_gauss ? A.ff_a[g] : 0.0f;
+            sh.aff[j][5 + g] = g < A.n_gauss ? A.ff_b[g] : 0.0f;
+        }
+        sh.aff[j][10] = A.ff_c;
+        sh.aff[j][11] = A.w0;
+        sh.afp[j][0] = P.radiation == 1 ? A.b_neutron : A.fp;
+        sh.afp[j][1] = P.radiation == 1 ? 0.0f : A.fpp;
+    }
 }
 
 // p = base + M x  (rietveld_core.js:1855-1859 apply, affine links of atomic_constraints.js)
@@ -93,36 +107,56 @@ __device__ inline void trial_setup(const DevPlan &P, Shared &sh) {
 }
 
 // |F|^2 of one reflection (rietveld_core.js:749-821): A^2 + B^2 + C^2 + D^2 over the atoms
-// and coset representatives; phases h.(R r + t) in FP64 reduced to turns, sincospi in FP32.
+// and coset representatives.  Phases h.(R r + t) are formed in FP64 and reduced to turns
+// in [0, 1); sine and cosine of the turn come from the hardware (v_sin_f32 / v_cos_f32
+// take revolutions).  Loop order: representatives outside, up to four atoms inside, so
+// h R and h.t are formed once per representative.
 __device__ inline float structure_f2(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k, int l,
                                      float s2) {
     float Ar = 0.f, Br = 0.f, Cr = 0.f, Dr = 0.f;
-    float q4 = 0.25f * s2;
-    for (int j = 0; j < ph.n_atoms; ++j) {
-        int ga = ph.atom_off + j;
-        const DevAtom &A = P.atoms[ga];
-        float f0;
-        if (P.radiation == 1) f0 = A.b_neutron;
-        else {
-            f0 = A.ff_c;
-            for (int i = 0; i < A.n_gauss; ++i) f0 += A.ff_a[i] * expf(-A.ff_b[i] * q4);
+    const float q4 = 0.25f * s2;
+    for (int j0 = 0; j0 < ph.n_atoms; j0 += 4) {
+        const int nj = min(4, ph.n_atoms - j0);
+        float ca[4] = {0.f, 0.f, 0.f, 0.f}, sa[4] = {0.f, 0.f, 0.f, 0.f};
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int ga = ph.atom_off + j0 + (a < nj ? a : 0);
+            px[a] = sh.apos[ga][0]; py[a] = sh.apos[ga][1]; pz[a] = sh.apos[ga][2];
         }
-        float dw = expf(-sh.aB[ga] * q4);
-        float wj = sh.aocc[ga] * (P.radiation == 1 ? f0 : f0 + A.fp) * dw * A.w0;
-        float wi = (P.radiation == 1 || !(A.fpp != 0.f)) ? 0.f : sh.aocc[ga] * A.fpp * dw * A.w0;
-        double x = sh.apos[ga][0], y = sh.apos[ga][1], z = sh.apos[ga][2];
-        float c = 0.f, s = 0.f;
         for (int r = 0; r < ph.n_reps; ++r) {
             const int *R = sh.rep_r[ph.rep_off + r];
             const double *t = sh.rep_t[ph.rep_off + r];
-            int H0 = h * R[0] + k * R[3] + l * R[6], H1 = h * R[1] + k * R[4] + l * R[7], H2 = h * R[2] + k * R[5] + l * R[8];
-            double phs = (double)H0 * x + (double)H1 * y + (double)H2 * z + ((double)h * t[0] + (double)k * t[1] + (double)l * t[2]);
-            float fr = (float)(phs - floor(phs));
-            float sn, cs;
-            sincospif(2.0f * fr, &sn, &cs);
-            c += cs; s += sn;
+            const double H0 = h * R[0] + k * R[3] + l * R[6], H1 = h * R[1] + k * R[4] + l * R[7], H2 = h * R[2] + k * R[5] + l * R[8];
+            const double ht = (double)h * t[0] + (double)k * t[1] + (double)l * t[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                if (a < nj) {
+                    const double phs = H0 * px[a] + H1 * py[a] + H2 * pz[a] + ht;
+                    const float fr = (float)(phs - floor(phs));
+                    ca[a] += __builtin_amdgcn_cosf(fr);
+                    sa[a] += __builtin_amdgcn_sinf(fr);
+                }
+            }
         }
-        Ar += wj * c; Br += wj * s; Cr += wi * c; Dr += wi * s;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            if (a < nj) {
+                const int ga = ph.atom_off + j0 + a;
+                const float *F = sh.aff[ga];
+                float f0;
+                if (P.radiation == 1) f0 = sh.afp[ga][0];
+                else {
+                    f0 = F[10];
+                    for (int g = 0; g < 5; ++g) f0 += F[g] * __expf(-F[5 + g] * q4);
+                    f0 += sh.afp[ga][0];
+                }
+                const float dw = __expf(-sh.aB[ga] * q4);
+                const float wj = sh.aocc[ga] * f0 * dw * F[11];
+                const float wi = (P.radiation == 1 || !(sh.afp[ga][1] != 0.f)) ? 0.f : sh.aocc[ga] * sh.afp[ga][1] * dw * F[11];
+                Ar += wj * ca[a]; Br += wj * sa[a]; Cr += wi * ca[a]; Dr += wi * sa[a];
+            }
+        }
     }
     return Ar * Ar + Br * Br + Cr * Cr + Dr * Dr;
 }
@@ -200,7 +234,7 @@ __device__ inline void build_reflection(const DevPlan &P, const Shared &sh, int 
     float height = (float)S * ((float)hk.w * lp1 * po / pk1.area * f2);
     emit_line(P, pk1, base1, height, rec[0], span[0]);
     if (P.doublet) {
-        double st2 = sin(bragg * kD2R * 0.5) * (P.lambda2 / P.lambda1);
+        double st2 = sinT * (P.lambda2 / P.lambda1);   // sin(theta1) lambda2/lambda1
         if (fabs(st2) < 1.0) {
             double b2 = 2.0 * asin(st2) * kR2D;
             double base2 = b2 + (double)peak_shift(b2, disp, transp) + zero;
@@ -447,11 +481,9 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Share
 // (js/pt_gpu.js:361 propose; rietveld_core.js:2196-2212): one coordinate j drawn
 // uniformly, x_j + z scale_j sqrt(T_c) reflected into its box, log u for the Metropolis
 // test.  Random numbers are keyed on (seed, chain, the chain's step counter).
-__global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__restrict__ X, TrialImage *img,
-                                                    int pt_mode, int bootstrap) {
-    __shared__ Shared sh;
-    __shared__ double prop[PTK_MAX_VARS];
-    const int t = blockIdx.x, nv = P.nv;
+__device__ inline void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
+                                  Shared &sh, double *prop) {
+    const int nv = P.nv;
     bool boot = false;
     double *st = nullptr;
     if (pt_mode) {
@@ -491,6 +523,13 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
     }
     __syncthreads();
     image_store(P, sh, img + t);
+}
+
+__global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__restrict__ X, TrialImage *img,
+                                                    int pt_mode, int bootstrap) {
+    __shared__ Shared sh;
+    __shared__ double prop[PTK_MAX_VARS];
+    setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop);
 }
 
 // line records of reflections [blockIdx.x * blockDim.x, ...) of trial blockIdx.y
@@ -669,14 +708,8 @@ __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage
     }
 }
 
-// chi^2 of trial t (ordered sum of its partials) and, in PT mode, the Metropolis step of
-// chain t in FP64 (js/pt_gpu.js:363-364, rietveld_core.js:2213-2222).
-__global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage *__restrict__ img,
-                                                     const double2 *__restrict__ partial, int nsplit, int ntrials,
-                                                     double *chi2, int pt_mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntrials) return;
-    const TrialImage *im = img + t;
+// chi^2 of trial t: the ordered sum of its partials (3e38 for an invalid trial)
+__device__ inline double trial_cost(const TrialImage *im, const double2 *partial, int nsplit, int t) {
     double cost = 3e38;
     if (im->go) {
         double hi = 0.0, lo = 0.0;
@@ -684,29 +717,74 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
         cost = hi + lo;
         if (!isfinite(cost) || cost > 3e38) cost = 3e38;
     }
-    if (!pt_mode) { chi2[t] = cost; return; }
+    return cost;
+}
+
+// the Metropolis step of chain t in FP64 (js/pt_gpu.js:363-364, rietveld_core.js:2213-2222),
+// run by one thread; returns 0 (rejected), 1 (accepted) or 2 (accepted, new best)
+__device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, double cost, int t) {
     const int nv = P.nv;
-    double *st = P.state + (size_t)t * P.stride;
-    double *sc = st + 3 * nv;
+    double *sc = P.state + (size_t)t * P.stride + 3 * nv;
     const bool boot = im->boot != 0;
     const double step = sc[2];
     const double temp = pow(P.max_temp, (double)t / (double)(P.chains > 1 ? P.chains - 1 : 1));
     sc[10] = cost;
     const double old = sc[0];
     const bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
+    int r = 0;
     if (acc) {
         const bool take_best = im->prof_ok && (boot || cost < sc[1]);
-        for (int j = 0; j < nv; ++j) {
-            st[j] = st[nv + j];
-            if (take_best) st[2 * nv + j] = st[nv + j];
-        }
         sc[0] = cost;
         sc[11] = im->prof_ok ? 1.0 : 0.0;
         if (!boot) sc[3] += 1.0;
         if (take_best) sc[1] = cost;
+        r = take_best ? 2 : 1;
     }
     if (!boot) sc[4] += 1.0;
     sc[2] = step + 1.0;
+    return r;
+}
+
+// chi^2 of every trial (batch mode) or the Metropolis step of every chain (PT mode)
+__global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage *__restrict__ img,
+                                                     const double2 *__restrict__ partial, int nsplit, int ntrials,
+                                                     double *chi2, int pt_mode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntrials) return;
+    const double cost = trial_cost(img + t, partial, nsplit, t);
+    if (!pt_mode) { chi2[t] = cost; return; }
+    const int r = accept_chain(P, img + t, cost, t);
+    if (r) {
+        double *st = P.state + (size_t)t * P.stride;
+        const int nv = P.nv;
+        for (int j = 0; j < nv; ++j) {
+            st[j] = st[nv + j];
+            if (r == 2) st[2 * nv + j] = st[nv + j];
+        }
+    }
+}
+
+// the Metropolis step of chain blockIdx.x, then its next proposal and setup: the finish
+// of one PT step fused with the setup of the next (one launch less per step)
+__global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
+                                                           int nsplit) {
+    __shared__ Shared sh;
+    __shared__ double prop[PTK_MAX_VARS];
+    __shared__ int s_acc;
+    const int t = blockIdx.x, nv = P.nv;
+    if (threadIdx.x == 0) s_acc = accept_chain(P, img + t, trial_cost(img + t, partial, nsplit, t), t);
+    __syncthreads();
+    const int r = s_acc;
+    if (r) {
+        double *st = P.state + (size_t)t * P.stride;
+        for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+            const double v = st[nv + j];
+            st[j] = v;
+            if (r == 2) st[2 * nv + j] = v;
+        }
+    }
+    __syncthreads();
+    setup_body(P, nullptr, img, t, 1, 0, sh, prop);
 }
 
 // ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
@@ -833,6 +911,11 @@ hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 
                                double *chi2, int pt_mode, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_finish, dim3((ntrials + 63) / 64), dim3(64), 0, s, P, (const TrialImage *)img, partial, nsplit,
                        ntrials, chi2, pt_mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_finish_setup, dim3(P.chains), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit);
     return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@ hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *
                               int nsplit, int ntrials, hipStream_t s);
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
                                double *chi2, int pt_mode, hipStream_t s);
+hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s);
 hipError_t launch_pattern(const DevPlan &P, const double *X, double *y, int block, int ch, size_t dyn, hipStream_t s);
 hipError_t launch_workload(const DevPlan &P, const double *X, long long *stats, int block, int ch, size_t dyn,
                            hipStream_t s);

@@ -67,6 +67,32 @@ def cpu_baseline(pb, x_states, gpu_costs):
     }
 
 
+PEAK_FP16_MFMA = 2.5e15   # MI355X dense FP16/BF16 MFMA, FLOP/s
+
+
+def matrix_form(pt, PR, trials=32768):
+    """The shared-profile MFMA form on the atom-only variant of the same Pnma workload:
+    P (7001 x N_hkl) fixed, I[hkl, trial] per trial, Y = P I as FP16 hi/lo on MFMA."""
+    pb = PR.pbso4_pnma(simulate=pt.gpu_simulate, atoms_only=True)
+    x0 = pb.start_vector()
+    rng = np.random.default_rng(5)
+    X = x0 + rng.uniform(-1, 1, size=(trials, pb.nv)) * np.array(pb.scale)
+    X = np.minimum(np.array(pb.hi), np.maximum(np.array(pb.lo), X))
+    with pt.Plan(pb) as plan:
+        plan.eval(X[:4096], pt.FORM_MATRIX)                 # builds P, warms up
+        t = time.perf_counter()
+        chi2 = plan.eval(X, pt.FORM_MATRIX)
+        dt = time.perf_counter() - t
+        ti_ms, tp_ms = plan.matrix_timing()
+        n_hkl = plan.workload(x0)['n_hkl']
+    dense = 2.0 * len(pb.tth) * n_hkl                      # FLOP per trial of the dense product
+    return {'value': trials / dt, 'unit': 'trial patterns/s', 'trials': trials, 'n_vars': pb.nv,
+            'form': 'shared profile matrix, Y = P I on MFMA (FP16 hi/lo, 3 products), fused chi^2',
+            'intensity_ms': ti_ms, 'product_ms': tp_ms,
+            'product_dense_tflops': dense * trials / (tp_ms / 1e3) / 1e12,
+            'product_frac_of_fp16_peak': dense * trials / (tp_ms / 1e3) / PEAK_FP16_MFMA}, pb, X[:8], chi2[:8]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -119,6 +145,8 @@ def main():
         gb = plan.eval(XB, pt.FORM_BATCHED)
         batched = len(XB) / (time.perf_counter() - t)
 
+    mat, mpb, mX, mchi = matrix_form(pt, PR)
+
     S = state.reshape(chains, -1)
     nv = pb.nv
     pick = list(range(0, chains, max(1, chains // 16)))
@@ -134,6 +162,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pb, xs, gc)
+        # the matrix form's chi^2 against the FP64 oracle on a sample of its trials
+        sys.path.insert(0, os.path.join(ROOT, 'tests'))
+        import oracle_lib as O
+        mm = O.Model(mpb)
+        mat['chi2_rel_err_vs_fp64'] = max(abs(g - mm.cost(x)) / max(1.0, mm.cost(x)) for x, g in zip(mX, mchi))
+        mm.close()
     out = {
         'metric': METRIC, 'value': value, 'unit': 'trial patterns/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
@@ -143,6 +177,7 @@ def main():
                    'reflections': work['n_hkl'], 'lines': work['lines'], 'form': 'staged PT step pipeline, one graph per sweep'},
         'plain': {'value': plain, 'unit': 'trial patterns/s', 'form': 'one trial per launch sequence'},
         'batched_eval': {'value': batched, 'unit': 'trial patterns/s', 'form': 'staged pipeline, 16384 independent vectors'},
+        'matrix_form': mat,
         'chosen_over_plain': value / plain,
         'roofline': roof,
         'cpu_baseline': cpu,

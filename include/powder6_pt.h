@@ -184,6 +184,13 @@ int pt_batch(pt_plan *plan, int count, int bootstrap, double *state_out);
  * (atom) and scale variables differ between the trials and the plan's start vector. */
 int pt_eval(pt_plan *plan, int form, const double *x, int ntrials, double *chi2);
 
+/* Pattern of one structure-only vector through the shared profile matrix (PT_FORM_MATRIX
+ * path): background of the start vector + P I.  No counterpart in the reference. */
+int pt_matrix_pattern(pt_plan *plan, const double *x, double *ycalc);
+/* Milliseconds of the intensity and matrix-product kernels of the last PT_FORM_MATRIX
+ * evaluation (HIP events).  No counterpart in the reference (measurement). */
+int pt_matrix_timing(const pt_plan *plan, double *intensity_ms, double *product_ms);
+
 /* Calculated pattern (background + phases) of one vector, FP32 kernels, n_points out. */
 int pt_pattern(pt_plan *plan, const double *x, double *ycalc);
 

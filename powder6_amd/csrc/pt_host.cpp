@@ -181,8 +181,20 @@ struct pt_plan {
     hipGraphExec_t gexec = nullptr;
     bool kernel_timing = false;
     std::vector<hipEvent_t> ev;
+    // shared-profile matrix form
+    bool matrix_ready = false;
+    int kpad = 0, np_pad = 0, row_tiles = 0, cap_mtrials = 0;
+    unsigned short *d_phi = nullptr, *d_plo = nullptr, *d_ihi = nullptr, *d_ilo = nullptr;
+    float *d_factor = nullptr, *d_iscale = nullptr, *d_mpart = nullptr;
+    int2 *d_band = nullptr;
+    float2 *d_yw = nullptr;
+    double *d_myout = nullptr;
+    std::vector<char> struct_var;        // variables that may differ from x0 in the matrix form
+    std::vector<double> x0, base_params; // start vector and its model parameters
+    std::vector<double> tth_host;
+    int n_bkg = 0;
     hipEvent_t ev_total[2] = {nullptr, nullptr};
-    double last_total_ms = 0, last_kernel_ms = 0;
+    double last_total_ms = 0, last_kernel_ms = 0, last_matrix_ms[2] = {0, 0};
     int last_launches = 0;
     double qlim = 0;
 };
@@ -501,6 +513,21 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     P.seed = pb->seed;
 
     std::vector<double> tth(pb->tth, pb->tth + n), base(pb->param_base, pb->param_base + pb->n_params);
+    pl->x0.assign(pb->x0, pb->x0 + pb->n_vars);
+    pl->base_params.resize(pb->n_params);
+    for (int k = 0; k < pb->n_params; ++k) pl->base_params[k] = param_value(pb, k);
+    pl->tth_host = tth;
+    pl->n_bkg = pb->n_bkg;
+    pl->struct_var.assign(pb->n_vars, 1);
+    for (int t = 0; t < pb->n_terms; ++t) {
+        // structure parameters: atom slots and the phase scale of some phase
+        int k = pb->term_param[t], ok = 0;
+        for (int q = 0; q < pb->n_phases; ++q) {
+            int b = phase_base_of(pb, q);
+            if (k == b + PT_PH_SCALE || (k >= b + PT_PH_ATOM0 && k < b + PT_PH_ATOM0 + PT_AT_STRIDE * pb->phases[q].n_atoms)) ok = 1;
+        }
+        if (!ok) pl->struct_var[pb->term_var[t]] = 0;
+    }
     std::vector<double> vlo(pb->lo, pb->lo + pb->n_vars), vhi(pb->hi, pb->hi + pb->n_vars), vsc(pb->scale, pb->scale + pb->n_vars);
     if (eq.empty()) eq.push_back(0);
     double *d_tth; float2 *d_tff, *d_obs; int4 *d_hklm; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
@@ -638,6 +665,8 @@ static double sum_pairs(pt_plan *pl, int pairs) {
 
 static const int kEvalChunk = 2048;   // trial slots per pass of pt_eval
 
+static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout);
+
 static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2, bool time_it) {
     int rc = ensure_trials(pl, ntrials);
     if (rc) return rc;
@@ -769,7 +798,162 @@ int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
         pl->last_launches = 0;
         return 0;
     }
+    if (form == PT_FORM_MATRIX) return eval_matrix(pl, x, ntrials, chi2, nullptr);
     return fail(PT_ERR_STATE, "PT evaluation form not available for this plan.");
+}
+
+// ---- shared-profile matrix form ----------------------------------------------------
+static int matrix_prepare(pt_plan *pl) {
+    if (pl->matrix_ready) return 0;
+    if (pl->nr > PTK_MATRIX_MAX_REFL) return fail(PT_ERR_BUDGET, "PT shared-profile form supports at most 4096 reflections.");
+    int rc = ensure_trials(pl, 1);
+    if (rc || (rc = ensure_slots(pl, std::max(1, pl->chains)))) return rc;
+    pl->kpad = (pl->nr + 15) / 16 * 16;
+    pl->np_pad = (pl->np + 63) / 64 * 64;
+    pl->row_tiles = pl->np_pad / 64;
+    if ((rc = dalloc(pl, &pl->d_factor, (size_t)pl->nr)) || (rc = dalloc(pl, &pl->d_phi, (size_t)pl->np_pad * pl->kpad)) ||
+        (rc = dalloc(pl, &pl->d_plo, (size_t)pl->np_pad * pl->kpad)) || (rc = dalloc(pl, &pl->d_band, (size_t)pl->row_tiles)) ||
+        (rc = dalloc(pl, &pl->d_yw, (size_t)pl->np_pad)))
+        return rc;
+    // unit-height lines of the start vector
+    HIPCHECK(hipMemcpyAsync(pl->d_x, pl->x0.data(), sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
+    HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, 1, 0, 0, pl->stream), "setup");
+    HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_factor, pl->stream), "matrix columns");
+    std::vector<int4> spans(2 * (size_t)pl->nr);
+    HIPCHECK(hipMemcpyAsync(spans.data(), pl->d_sspan, sizeof(int4) * spans.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipMemsetAsync(pl->d_phi, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
+    HIPCHECK(hipMemsetAsync(pl->d_plo, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
+    HIPCHECK(ptk::launch_matrix_fill(pl->P, pl->d_srec, pl->d_sspan, pl->d_phi, pl->d_plo, pl->kpad, pl->stream), "matrix fill");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "matrix build");
+    // band of reflections meeting each 64-point row tile
+    std::vector<int2> band(pl->row_tiles);
+    for (int rt = 0; rt < pl->row_tiles; ++rt) {
+        int lo_pt = 64 * rt, hi_pt = lo_pt + 64, kmin = -1, kmax = -1;
+        for (int k = 0; k < pl->nr; ++k) {
+            const int4 &a = spans[2 * k], &b = spans[2 * k + 1];
+            bool hit = (a.y > a.x && a.x < hi_pt && a.y > lo_pt) || (b.y > b.x && b.x < hi_pt && b.y > lo_pt);
+            if (hit) { if (kmin < 0) kmin = k; kmax = k; }
+        }
+        band[rt] = kmin < 0 ? make_int2(0, 0) : make_int2(kmin / 16 * 16, std::min(pl->kpad, (kmax + 16) / 16 * 16));
+    }
+    // y_obs - background of the start vector (rietveld_core.js:647-653, 920-931), FP64 on the host
+    std::vector<float2> yw(pl->np_pad, make_float2(0.f, 0.f));
+    std::vector<double> yobs(pl->np), w(pl->np);
+    {
+        std::vector<float2> obs(pl->np);
+        HIPCHECK(hipMemcpy(obs.data(), pl->P.obs, sizeof(float2) * pl->np, hipMemcpyDeviceToHost), "readback");
+        double lo = pl->tth_host[0], hi = pl->tth_host[pl->np - 1];
+        for (int i = 0; i < pl->np; ++i) {
+            double z = hi > lo ? 2 * (pl->tth_host[i] - lo) / (hi - lo) - 1 : 0, ta = 1, tb = z, bk = 0;
+            for (int k = 0; k < pl->n_bkg; ++k) {
+                double T = k == 0 ? 1 : k == 1 ? z : 2 * z * tb - ta;
+                if (k >= 2) { ta = tb; tb = T; }
+                bk += pl->base_params[PT_P_BKG0 + k] * T;
+            }
+            yw[i] = make_float2((float)((double)obs[i].x - bk), obs[i].y);
+        }
+    }
+    HIPCHECK(hipMemcpy(pl->d_band, band.data(), sizeof(int2) * band.size(), hipMemcpyHostToDevice), "upload");
+    HIPCHECK(hipMemcpy(pl->d_yw, yw.data(), sizeof(float2) * yw.size(), hipMemcpyHostToDevice), "upload");
+    pl->matrix_ready = true;
+    return 0;
+}
+
+static int ensure_matrix_trials(pt_plan *pl, int tpad) {
+    if (tpad <= pl->cap_mtrials) return 0;
+    int rc;
+    if ((rc = dalloc(pl, &pl->d_ihi, (size_t)tpad * pl->kpad)) || (rc = dalloc(pl, &pl->d_ilo, (size_t)tpad * pl->kpad)) ||
+        (rc = dalloc(pl, &pl->d_iscale, (size_t)tpad)) || (rc = dalloc(pl, &pl->d_mpart, (size_t)tpad * pl->row_tiles)))
+        return rc;
+    pl->cap_mtrials = tpad;
+    return 0;
+}
+
+static const int kMatrixChunk = 4096;
+
+// chi^2 of ntrials structure-only vectors through Y = P I; yout (optional, ntrials <= 64)
+// receives Y (without background) per trial and point
+static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout) {
+    int rc = matrix_prepare(pl);
+    if (rc) return rc;
+    for (int t = 0; t < ntrials; ++t)
+        for (int j = 0; j < pl->nv; ++j)
+            if (!pl->struct_var[j] && x[(size_t)t * pl->nv + j] != pl->x0[j])
+                return fail(PT_ERR_ARG, "PT shared-profile form: only structure variables (coordinates, occupancies, B, scale) "
+                                        "may differ from the start vector.");
+    if ((rc = ensure_trials(pl, ntrials))) return rc;
+    int chunk = std::min(ntrials, kMatrixChunk);
+    int tpad_max = (chunk + 63) / 64 * 64;
+    if ((rc = ensure_slots(pl, chunk)) || (rc = ensure_matrix_trials(pl, tpad_max))) return rc;
+    if (yout && !pl->d_myout && (rc = dalloc(pl, &pl->d_myout, (size_t)64 * pl->np))) return rc;
+    int passes = (ntrials + chunk - 1) / chunk;
+    if ((rc = ensure_events(pl, 2 * passes))) return rc;
+    HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * ntrials * pl->nv, hipMemcpyHostToDevice, pl->stream), "trial upload");
+    HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+    for (int p = 0; p < passes; ++p) {
+        int t0 = p * chunk, n = std::min(chunk, ntrials - t0), tpad = (n + 63) / 64 * 64;
+        HIPCHECK(hipMemsetAsync(pl->d_ihi, 0, sizeof(unsigned short) * (size_t)tpad * pl->kpad, pl->stream), "clear");
+        HIPCHECK(hipMemsetAsync(pl->d_ilo, 0, sizeof(unsigned short) * (size_t)tpad * pl->kpad, pl->stream), "clear");
+        HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x + (size_t)t0 * pl->nv, pl->d_img, n, 0, 0, pl->stream), "setup");
+        HIPCHECK(hipEventRecord(pl->ev[4 * p], pl->stream), "timing");
+        HIPCHECK(ptk::launch_matrix_intensity(pl->P, pl->d_img, pl->d_factor, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->kpad, 0, n,
+                                              pl->stream), "intensities");
+        HIPCHECK(hipEventRecord(pl->ev[4 * p + 1], pl->stream), "timing");
+        HIPCHECK(hipEventRecord(pl->ev[4 * p + 2], pl->stream), "timing");
+        HIPCHECK(ptk::launch_matrix_chi2(pl->P, pl->d_phi, pl->d_plo, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->d_band, pl->d_yw,
+                                         pl->d_mpart, pl->kpad, tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, pl->stream),
+                 "matrix product");
+        HIPCHECK(hipEventRecord(pl->ev[4 * p + 3], pl->stream), "timing");
+        HIPCHECK(ptk::launch_matrix_reduce(pl->d_mpart, pl->row_tiles, tpad, n, pl->d_img, pl->d_chi2 + t0, pl->stream), "reduce");
+    }
+    HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
+    HIPCHECK(hipMemcpyAsync(chi2, pl->d_chi2, sizeof(double) * ntrials, hipMemcpyDeviceToHost, pl->stream), "readback");
+    if (yout)
+        HIPCHECK(hipMemcpyAsync(yout, pl->d_myout, sizeof(double) * (size_t)ntrials * pl->np, hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "matrix evaluation");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
+    pl->last_total_ms = ms;
+    double ki = 0, kg = 0;
+    for (int p = 0; p < passes; ++p) {
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, pl->ev[4 * p], pl->ev[4 * p + 1]);
+        (void)hipEventElapsedTime(&b, pl->ev[4 * p + 2], pl->ev[4 * p + 3]);
+        ki += a; kg += b;
+    }
+    pl->last_kernel_ms = ki + kg;
+    pl->last_launches = passes;
+    pl->last_matrix_ms[0] = ki;
+    pl->last_matrix_ms[1] = kg;
+    return 0;
+}
+
+int pt_matrix_pattern(pt_plan *pl, const double *x, double *ycalc) {
+    if (!pl || !x || !ycalc) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    double chi2 = 0;
+    int rc = eval_matrix(pl, x, 1, &chi2, ycalc);
+    if (rc) return rc;
+    // add the background of the start vector back
+    double lo = pl->tth_host[0], hi = pl->tth_host[pl->np - 1];
+    for (int i = 0; i < pl->np; ++i) {
+        double z = hi > lo ? 2 * (pl->tth_host[i] - lo) / (hi - lo) - 1 : 0, ta = 1, tb = z, bk = 0;
+        for (int k = 0; k < pl->n_bkg; ++k) {
+            double T = k == 0 ? 1 : k == 1 ? z : 2 * z * tb - ta;
+            if (k >= 2) { ta = tb; tb = T; }
+            bk += pl->base_params[PT_P_BKG0 + k] * T;
+        }
+        ycalc[i] += bk;
+    }
+    return 0;
+}
+
+int pt_matrix_timing(const pt_plan *pl, double *intensity_ms, double *product_ms) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    if (intensity_ms) *intensity_ms = pl->last_matrix_ms[0];
+    if (product_ms) *product_ms = pl->last_matrix_ms[1];
+    return 0;
 }
 
 int pt_pattern(pt_plan *pl, const double *x, double *ycalc) {

@@ -202,7 +202,12 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
 
 // Both lines (Kalpha1, Kalpha2) of envelope reflection k: Model.geometry (:719-736),
 // buildPeakContributions (profile.js:1013-1080), evalPhase heights (:855-897).
-__device__ inline void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span) {
+// With unit_factor != null the lines are written at unit height (the Kalpha2 line at its
+// doublet weight) and the reflection's height factor m Lp PO / area goes to
+// unit_factor[k] instead: the columns of the shared profile matrix.
+__device__ inline void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
+                                        float *unit_factor = nullptr) {
+    if (unit_factor) unit_factor[k] = 0.0f;
     span[0] = make_int4(0, 0, 0, 0);
     span[1] = make_int4(0, 0, 0, 0);
     int q = 0;
@@ -226,12 +231,18 @@ __device__ inline void build_reflection(const DevPlan &P, const Shared &sh, int 
     prepare_line(ph.profile_type, sh.prof[q], (float)P.min_fwhm, (float)base1, pk1);
     if (!(pk1.area > 0.0f)) return;
     float lp1 = lp_factor((float)base1, (float)P.pol_k);
-    float f2 = structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv);
     const double *pp = sh.p + ph.param_base;
     float po = march_dollase(P, ph, f, pp[7], k);
-    double S = pp[6];
-    if (!(S != 0.0) || isnan(S)) S = 0.0;
-    float height = (float)S * ((float)hk.w * lp1 * po / pk1.area * f2);
+    float height;
+    if (unit_factor) {
+        unit_factor[k] = (float)hk.w * lp1 * po / pk1.area;
+        height = 1.0f;
+    } else {
+        float f2 = structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv);
+        double S = pp[6];
+        if (!(S != 0.0) || isnan(S)) S = 0.0;
+        height = (float)S * ((float)hk.w * lp1 * po / pk1.area * f2);
+    }
     emit_line(P, pk1, base1, height, rec[0], span[0]);
     if (P.doublet) {
         double st2 = sinT * (P.lambda2 / P.lambda1);   // sin(theta1) lambda2/lambda1
@@ -556,6 +567,20 @@ constexpr int kAccTilesPerWave = 2;    // 128-point tiles per wavefront (2 point
 constexpr int kAccPoints = kAccWaves * kAccTilesPerWave * 128;   // points per workgroup
 constexpr int kStage = 384;            // line records staged in LDS per pass
 
+// one line at one point of a lane (delta = t - centre); `split` and `asym` are uniform
+// over the wavefront (scalar branches), the window test is a select
+__device__ __forceinline__ float line_point(const LineRec &R, bool split, float asym, float2 t, bool inwin) {
+    float d = (t.x - R.a.x) + (t.y - R.a.y);
+    if (asym != 0.0f && fabsf(d) >= 1e-9f) d = d / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d)));
+    float4 f = R.r;
+    if (split && d < 0.0f) f = R.l;
+    const float q = d * d;
+    const float g = __builtin_amdgcn_exp2f(f.x * q);
+    const float l = __builtin_amdgcn_rcpf(fmaf(f.y, q, 1.0f));
+    const float v = fmaxf(fmaf(f.z, l, fmaf(f.w, g, -R.a.w)), 0.0f);
+    return inwin ? R.a.z * v : 0.0f;
+}
+
 // one line at the two points (i, i + 64) of a lane; `split` and `asym` are uniform over
 // the wavefront (scalar branches), the window test is a select
 __device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, float asym, float2 t0, float2 t1,
@@ -876,6 +901,194 @@ __global__ void k_plain_reduce(const double *partial, int n, double *out) {
     }
 }
 
+
+// ---- shared-profile matrix form: Y = P I on MFMA ---------------------------------------
+// When only structure parameters (coordinates, occupancies, B, the phase scale) differ
+// between trials, the profile of every reflection is fixed and the pattern is
+//   y(t_i) = bkg(t_i) + sum_k P[i, k] I[k, trial],
+// P[i, k] the unit-height column of reflection k (Kalpha1 + weighted Kalpha2) and
+// I[k, trial] = S m Lp PO |F|^2 / area.  P and I are held as FP16 hi/lo pairs (P scaled
+// by 2^12, each trial's I by a power of two), the products hi*hi + hi*lo + lo*hi are
+// accumulated in FP32 by v_mfma_f32_16x16x16f16, and the weighted chi^2 is reduced in
+// the epilogue -- Y is never stored.  Only the band of reflections whose windows meet a
+// 64-point row tile is multiplied.
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+constexpr float kPScale = 4096.0f;   // 2^12
+
+// unit-height lines of the start vector and the per-reflection height factors
+__global__ void __launch_bounds__(128) k_matrix_columns(DevPlan P, const TrialImage *__restrict__ img, LineRec *rec,
+                                                        int4 *span, float *factor) {
+    __shared__ Shared sh;
+    stage_reps(P, sh);
+    image_load(P, img, sh);
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.nr) return;
+    build_reflection(P, sh, k, rec + 2 * k, span + 2 * k, factor);
+}
+
+// column k of P: Kalpha1 then Kalpha2 line values over the union of their windows
+__global__ void __launch_bounds__(256) k_matrix_fill(DevPlan P, const LineRec *__restrict__ rec,
+                                                     const int4 *__restrict__ span, unsigned short *phi,
+                                                     unsigned short *plo, int kpad) {
+    const int k = blockIdx.x;
+    const int4 s1 = span[2 * k], s2 = span[2 * k + 1];
+    int lo = P.np, hi = 0;
+    if (s1.y > s1.x) { lo = min(lo, s1.x); hi = max(hi, s1.y); }
+    if (s2.y > s2.x) { lo = min(lo, s2.x); hi = max(hi, s2.y); }
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const float2 t = P.tff[i];
+        float v = 0.0f;
+        if (i >= s1.x && i < s1.y)
+            v += line_point(rec[2 * k], s1.w & 1, __int_as_float(s1.z), t, true);
+        if (i >= s2.x && i < s2.y)
+            v += line_point(rec[2 * k + 1], s2.w & 1, __int_as_float(s2.z), t, true);
+        const float sv = v * kPScale;
+        const _Float16 h = (_Float16)sv;
+        const _Float16 l = (_Float16)(sv - (float)h);
+        phi[(size_t)i * kpad + k] = __builtin_bit_cast(unsigned short, h);
+        plo[(size_t)i * kpad + k] = __builtin_bit_cast(unsigned short, l);
+    }
+}
+
+// I[k, t] = S m Lp PO |F|^2 / area of every reflection of trial t, as an FP16 hi/lo pair
+// scaled by 2^-e_t with max_k |I| 2^-e_t <= 2^14; iscale[t] = 2^e_t.
+__global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const TrialImage *__restrict__ img,
+                                                          const float *__restrict__ factor, unsigned short *ihi,
+                                                          unsigned short *ilo, float *iscale, int kpad, int t0) {
+    __shared__ Shared sh;
+    __shared__ float ival[PTK_MATRIX_MAX_REFL];
+    __shared__ float wmax[16];
+    const int tl = blockIdx.x, t = t0 + tl;
+    stage_reps(P, sh);
+    image_load(P, img + tl, sh);
+    __syncthreads();
+    float mx = 0.0f;
+    for (int k = threadIdx.x; k < P.nr; k += blockDim.x) {
+        float v = 0.0f;
+        const float fk = factor[k];
+        if (sh.go && fk != 0.0f) {
+            int q = 0;
+            while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
+            const DevPhase &ph = P.phase[q];
+            const int4 hk = P.hklm[k];
+            const double *f = sh.metric[q];
+            const double h = hk.x, kk = hk.y, l = hk.z;
+            const double qv = h * h * f[0] + kk * kk * f[1] + l * l * f[2] + h * kk * f[3] + kk * l * f[4] + h * l * f[5];
+            double S = sh.p[ph.param_base + 6];
+            if (!(S != 0.0) || isnan(S)) S = 0.0;
+            v = (float)S * (fk * structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv));
+        }
+        ival[k] = v;
+        mx = fmaxf(mx, fabsf(v));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wmax[w]);
+        int e = 0;
+        if (m > 0.0f && isfinite(m)) { frexpf(m, &e); e -= 14; }
+        wmax[0] = ldexpf(1.0f, e);
+    }
+    __syncthreads();
+    const float sc = wmax[0], inv = 1.0f / sc;
+    for (int k = threadIdx.x; k < kpad; k += blockDim.x) {
+        const float v = k < P.nr ? ival[k] * inv : 0.0f;
+        const _Float16 h = (_Float16)v;
+        const _Float16 l = (_Float16)(v - (float)h);
+        ihi[(size_t)t * kpad + k] = __builtin_bit_cast(unsigned short, h);
+        ilo[(size_t)t * kpad + k] = __builtin_bit_cast(unsigned short, l);
+    }
+    if (threadIdx.x == 0) iscale[t] = sc;
+}
+
+__device__ __forceinline__ half4_t load_h4(const unsigned short *p) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    half4_t h;
+    h[0] = __builtin_bit_cast(_Float16, (unsigned short)(u.x & 0xffff));
+    h[1] = __builtin_bit_cast(_Float16, (unsigned short)(u.x >> 16));
+    h[2] = __builtin_bit_cast(_Float16, (unsigned short)(u.y & 0xffff));
+    h[3] = __builtin_bit_cast(_Float16, (unsigned short)(u.y >> 16));
+    return h;
+}
+
+// chi^2 partials of a 64-point x 64-trial tile: wavefront w multiplies rows
+// [i0 + 16 w, i0 + 16 w + 16) by the four 16-trial column blocks over the band of the row
+// tile; lane l holds D[4 (l/16) + e][l % 16] of each 16x16 block.
+__global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned short *__restrict__ phi,
+                                                     const unsigned short *__restrict__ plo,
+                                                     const unsigned short *__restrict__ ihi,
+                                                     const unsigned short *__restrict__ ilo,
+                                                     const float *__restrict__ iscale, const int2 *__restrict__ band,
+                                                     const float2 *__restrict__ yw, float *partial, int kpad, int tpad,
+                                                     double *yout) {
+    __shared__ float red[4][64];
+    const int rt = blockIdx.x, j0 = blockIdx.y * 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i0 = rt * 64 + wave * 16;
+    const int2 b = band[rt];
+    floatx4_t acc[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) acc[nb] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+    const int arow = i0 + (lane & 15), kq = 4 * (lane >> 4);
+    for (int k = b.x; k < b.y; k += 16) {
+        const half4_t ah = load_h4(phi + (size_t)arow * kpad + k + kq);
+        const half4_t al = load_h4(plo + (size_t)arow * kpad + k + kq);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int col = j0 + 16 * nb + (lane & 15);
+            const half4_t bh = load_h4(ihi + (size_t)col * kpad + k + kq);
+            const half4_t bl = load_h4(ilo + (size_t)col * kpad + k + kq);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[nb], 0, 0, 0);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[nb], 0, 0, 0);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[nb], 0, 0, 0);
+        }
+    }
+    // epilogue: weighted residuals, summed over the 16 rows of this wavefront
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const int col = j0 + 16 * nb + (lane & 15);
+        const float sc = iscale[col] * (1.0f / kPScale);
+        float sum = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = i0 + kq + e;
+            if (row < P.np) {
+                const float Y = acc[nb][e] * sc;
+                const float2 o = yw[row];
+                if (yout && col < tpad) yout[(size_t)col * P.np + row] = (double)Y;
+                if (o.y > 0.0f) {
+                    const float r = o.x - Y;
+                    sum = fmaf(o.y * r, r, sum);
+                }
+            }
+        }
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        if (lane < 16) red[wave][16 * nb + lane] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const float s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        partial[(size_t)rt * tpad + j0 + threadIdx.x] = s;
+    }
+}
+
+// chi^2 of each trial: ordered FP64 sum of its row-tile partials
+__global__ void k_matrix_reduce(const float *__restrict__ partial, int row_tiles, int tpad, int ntrials,
+                                const TrialImage *__restrict__ img, double *chi2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntrials) return;
+    double s = 0.0;
+    for (int r = 0; r < row_tiles; ++r) s += (double)partial[(size_t)r * tpad + t];
+    if (!img[t].go || !isfinite(s) || s > 3e38) s = 3e38;
+    chi2[t] = s;
+}
+
 }  // namespace ptk
 
 // ---- host-side launchers (used by pt_host.cpp) ----------------------------------------
@@ -916,6 +1129,34 @@ hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 
 
 hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_finish_setup, dim3(P.chains), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit);
+    return hipGetLastError();
+}
+
+hipError_t launch_matrix_columns(const DevPlan &P, const void *img, LineRec *rec, int4 *span, float *factor, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_columns, dim3((P.nr + 127) / 128), dim3(128), 0, s, P, (const TrialImage *)img, rec, span, factor);
+    return hipGetLastError();
+}
+hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *span, unsigned short *phi, unsigned short *plo,
+                              int kpad, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_fill, dim3(P.nr), dim3(256), 0, s, P, rec, span, phi, plo, kpad);
+    return hipGetLastError();
+}
+hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
+                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_intensity, dim3(n), dim3(256), 0, s, P, (const TrialImage *)img, factor, ihi, ilo, iscale, kpad, t0);
+    return hipGetLastError();
+}
+hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
+                              const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
+                              float *partial, int kpad, int tpad, int row_tiles, double *yout, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_chi2, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
+                       partial, kpad, tpad, yout);
+    return hipGetLastError();
+}
+hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_reduce, dim3((ntrials + 63) / 64), dim3(64), 0, s, partial, row_tiles, tpad, ntrials,
+                       (const TrialImage *)img, chi2);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@
 #define PTK_MAX_VARS 256
 #define PTK_MAX_BKG_F 24
 #define PTK_S_COUNT_K 12
+#define PTK_MATRIX_MAX_REFL 4096
 
 namespace ptk {
 

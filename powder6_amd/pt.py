@@ -160,6 +160,8 @@ def lib():
         L.pt_phase_base.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_workload.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
+        L.pt_matrix_timing.argtypes = [ctypes.c_void_p, P_D, P_D]
         _lib = L
     return _lib
 
@@ -252,6 +254,18 @@ class Plan:
         _check(lib().pt_workload(self.h, _dptr(x[0]), _dptr(out)))
         return dict(lines=int(out[0]), w_tot=int(out[1]), n_hkl=int(out[2]), s_terms=int(out[3]), np=int(out[4]),
                     n_bkg=int(out[5]))
+
+    def matrix_pattern(self, x):
+        """Pattern of one structure-only vector through the shared profile matrix."""
+        x = self._vectors(x)
+        out = np.zeros(self.info['point_count'])
+        _check(lib().pt_matrix_pattern(self.h, _dptr(x[0]), _dptr(out)))
+        return out
+
+    def matrix_timing(self):
+        a, b = c_double(), c_double()
+        _check(lib().pt_matrix_timing(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_kernel_timing(self, on):
         """Direct launches with HIP events around the dominant kernel (no sweep graph)."""

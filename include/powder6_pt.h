@@ -210,6 +210,41 @@ int pt_set_kernel_timing(pt_plan *plan, int on);
  * timed launches (kernel timing mode, or batched pt_eval), and that launch count. */
 int pt_last_timing(const pt_plan *plan, double *total_ms, double *kernel_ms, int *launches);
 
+/* Tile shape of the banded profile sum: 128-point tiles per wavefront (1, 2 or 4; a
+ * workgroup of four wavefronts covers 512 * tiles points).  pt_batch tunes it on its
+ * first sweep call by timing each shape on the chains' current vectors.  request: 0
+ * reports, -1 runs the tuner now, 1 / 2 / 4 fixes that shape (no tuning afterwards).
+ * Reports the shape in use and the tuner's three timings (ms, 0 before tuning).
+ * No counterpart in the reference (the WGSL path has fixed 64-wide dispatches). */
+int pt_tiles(pt_plan *plan, int request, int *tiles_per_wave, double *candidate_ms);
+
+/* ---- groups: many independent problems in the same launches ------------------------
+ * The reference fits several problems side by side as independent workers
+ * (js/rietveld/rv_commands.js:11-100, up to min(8, cores - 1) at once), each with its own
+ * PT run.  A group runs the PT runs of `count` problems -- each with its own chains,
+ * temperatures, seed, reflections and points -- through shared launches: every stage of
+ * a PT step is one launch over all members, and one sweep of all members is one graph
+ * launch.  All members take `steps_per_sweep` steps per sweep (0: the largest member
+ * value; a member's own 0 means its nv), then exchange replicas.  Member m's chains
+ * occupy state_out[offset_m, offset_m + chains_m * stride_m), offsets from
+ * pt_group_member.  No counterpart in the reference. */
+typedef struct pt_group pt_group;
+int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, pt_group **out);
+void pt_group_destroy(pt_group *group);
+int pt_group_member(const pt_group *group, int member, pt_plan_info *info, int64_t *state_offset);
+/* initial(vectors) of one member (pt_set_states) */
+int pt_group_set_states(pt_group *group, int member, const double *x, int nvec);
+/* chi^2 of every member's chains at their current vectors, concatenated member by member
+ * (sum of chains doubles); the costs() step of js/pt_gpu.js:401 for all members at once */
+int pt_group_costs(pt_group *group, double *chi2);
+/* batch(count, bootstrap) for every member (pt_batch); state_out holds the concatenated
+ * snapshots (may be NULL) */
+int pt_group_batch(pt_group *group, int count, int bootstrap, double *state_out);
+/* the group's tile shape and tuner timings (request as for pt_tiles) */
+int pt_group_tiles(pt_group *group, int request, int *tiles_per_wave, double *candidate_ms);
+/* milliseconds of the last pt_group_batch / pt_group_costs call (HIP events) */
+int pt_group_last_timing(const pt_group *group, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

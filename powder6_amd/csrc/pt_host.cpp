@@ -170,7 +170,10 @@ struct pt_plan {
     int4 *d_span = nullptr;
     bool states_set = false;
     // staged pipeline buffers (per trial slot)
-    int slots = 0, nsplit = 1;
+    int slots = 0, ngroups = 1;           // 512-point groups per trial (one partial chi^2 each)
+    int tpw = 2;                          // accumulate tile shape: 128-point tiles per wavefront
+    bool tuned = false;
+    double tune_ms[3] = {0, 0, 0};        // accumulate time per step at 1, 2, 4 tiles per wavefront
     void *d_img = nullptr;
     ptk::LineRec *d_srec = nullptr;
     int4 *d_sspan = nullptr;
@@ -201,16 +204,16 @@ struct pt_plan {
 
 namespace {
 
-template <class T>
-int dalloc(pt_plan *pl, T **ptr, size_t count) {
+template <class Owner, class T>
+int dalloc(Owner *pl, T **ptr, size_t count) {
     void *p = nullptr;
     if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return fail(PT_ERR_DEVICE, "PT buffers exceed this GPU's memory (hipMalloc failed).");
     pl->allocs.push_back(p);
     *ptr = reinterpret_cast<T *>(p);
     return 0;
 }
-template <class T>
-int upload(pt_plan *pl, T **ptr, const std::vector<T> &v) {
+template <class Owner, class T>
+int upload(Owner *pl, T **ptr, const std::vector<T> &v) {
     int rc = dalloc(pl, ptr, v.size());
     if (rc) return rc;
     if (!v.empty() && hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
@@ -561,8 +564,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     pl->dyn = rec_bytes * pl->ch + ybytes;
     pl->plain_dyn = rec_bytes * pl->ch + 256 * sizeof(float);
     pl->block = 512;
-    int ppg = ptk::accum_points_per_group();
-    pl->nsplit = (n + ppg - 1) / ppg;
+    pl->ngroups = (n + 511) / 512;
     HIPCHECK(ptk::set_lds_limits(std::max(pl->dyn, pl->plain_dyn)), "kernel setup");
     *out = pl;
     return 0;
@@ -627,7 +629,7 @@ static int ensure_slots(pt_plan *pl, int n) {
     size_t nl = 2 * (size_t)pl->nr;
     if ((rc = dalloc(pl, (char **)&pl->d_img, (size_t)n * ptk::trial_image_bytes())) ||
         (rc = dalloc(pl, &pl->d_srec, (size_t)n * nl)) || (rc = dalloc(pl, &pl->d_sspan, (size_t)n * nl)) ||
-        (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->nsplit)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
+        (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->ngroups)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
         return rc;
     pl->slots = n;
     return 0;
@@ -639,9 +641,9 @@ static int run_stages(pt_plan *pl, const double *dX, int ntrials, int pt_mode, i
     HIPCHECK(ptk::launch_stage_setup(pl->P, dX, pl->d_img, ntrials, pt_mode, bootstrap, pl->stream), "setup");
     HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, ntrials, pl->stream), "lines");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[0], pl->stream), "timing");
-    HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->nsplit, ntrials, pl->stream), "accumulate");
+    HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[1], pl->stream), "timing");
-    HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->nsplit, ntrials, d_chi2, pt_mode, pl->stream), "finish");
+    HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, ntrials, d_chi2, pt_mode, pl->stream), "finish");
     return 0;
 }
 
@@ -702,6 +704,92 @@ int pt_costs(pt_plan *pl, const double *x, int nvec, double *chi2) {
     return eval_batched(pl, X.data(), pl->chains, chi2, false);
 }
 
+// ---- tile-shape autotuner ------------------------------------------------------------
+// Times `launch(tpw)` (one accumulate stage over the live line records) at 1, 2 and 4
+// 128-point tiles per wavefront: one untimed launch, then the best of three timed ones.
+// Writes the times to ms[3] and the fastest shape to *best.
+extern "C++" {
+template <class F>
+static int time_tiles(hipStream_t s, F launch, double ms[3], int *best) {
+    static const int cand[3] = {1, 2, 4};
+    hipEvent_t e[2];
+    HIPCHECK(hipEventCreate(&e[0]), "event");
+    HIPCHECK(hipEventCreate(&e[1]), "event");
+    int rc = 0;
+    for (int i = 0; i < 3 && !rc; ++i) {
+        double m = 1e30;
+        for (int r = 0; r < 4 && !rc; ++r) {
+            if (hipEventRecord(e[0], s) != hipSuccess || launch(cand[i]) != hipSuccess || hipEventRecord(e[1], s) != hipSuccess ||
+                hipEventSynchronize(e[1]) != hipSuccess) {
+                rc = fail(PT_ERR_DEVICE, "PT device lost during tile tuning.");
+                break;
+            }
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e[0], e[1]);
+            if (r > 0) m = std::min(m, (double)t);
+        }
+        ms[i] = m;
+    }
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    if (rc) return rc;
+    int b = 1;
+    for (int i = 0; i < 3; ++i)
+        if (ms[i] < ms[b]) b = i;
+    *best = cand[b];
+    return 0;
+}
+}  // extern "C++"
+
+static void drop_graph(hipGraphExec_t &ge, hipGraph_t &g) {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    ge = nullptr;
+    g = nullptr;
+}
+
+// tile shape of the accumulate stage for this plan: the chains' current vectors go
+// through setup and lines once (batch mode, the PT state is not touched), then the
+// accumulate stage is timed at each shape
+static int tune_plan(pt_plan *pl) {
+    int rc = ensure_trials(pl, pl->chains);
+    if (rc || (rc = ensure_slots(pl, pl->chains))) return rc;
+    const size_t nvb = sizeof(double) * pl->nv;
+    HIPCHECK(hipMemcpy2DAsync(pl->d_x, nvb, pl->P.state, sizeof(double) * pl->stride, nvb, pl->chains, hipMemcpyDeviceToDevice,
+                              pl->stream), "tile tuning");
+    HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, pl->chains, 0, 0, pl->stream), "setup");
+    HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
+    int best = 2;
+    rc = time_tiles(pl->stream, [&](int tpw) {
+        return ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, pl->chains, tpw, pl->stream);
+    }, pl->tune_ms, &best);
+    if (rc) return rc;
+    if (best != pl->tpw) drop_graph(pl->gexec, pl->graph);
+    pl->tpw = best;
+    pl->tuned = true;
+    return 0;
+}
+
+int pt_tiles(pt_plan *pl, int request, int *tiles_per_wave, double *candidate_ms) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    NEED_DEVICE(pl);
+    if (request == -1) {
+        if (!pl->states_set) return fail(PT_ERR_STATE, "PT tile tuning before initial state.");
+        (void)hipSetDevice(pl->device);
+        int rc = tune_plan(pl);
+        if (rc) return rc;
+    } else if (request == 1 || request == 2 || request == 4) {
+        if (request != pl->tpw) drop_graph(pl->gexec, pl->graph);
+        pl->tpw = request;
+        pl->tuned = true;
+    } else if (request != 0) {
+        return fail(PT_ERR_ARG, "tile request must be 0, -1, 1, 2 or 4");
+    }
+    if (tiles_per_wave) *tiles_per_wave = pl->tpw;
+    if (candidate_ms) for (int i = 0; i < 3; ++i) candidate_ms[i] = pl->tune_ms[i];
+    return 0;
+}
+
 // one sweep: steps_per_sweep x (setup/propose, lines, accumulate, finish/accept), then the
 // replica exchange; the finish of each step but the last is fused with the next setup.
 // Accumulate launches are bracketed by events when ev != null.
@@ -710,12 +798,12 @@ static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
         HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], pl->stream), "timing");
-        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->nsplit, pl->chains, pl->stream), "accumulate");
+        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, pl->chains, pl->tpw, pl->stream), "accumulate");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], pl->stream), "timing");
         if (s + 1 < pl->steps_per_sweep)
-            HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->nsplit, pl->stream), "finish");
+            HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->stream), "finish");
         else
-            HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->nsplit, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
+            HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
     }
     HIPCHECK(ptk::launch_exchange(pl->P, pl->stream), "exchange");
     return 0;
@@ -740,11 +828,13 @@ int pt_batch(pt_plan *pl, int count, int bootstrap, double *state_out) {
     if (bootstrap) {
         if ((rc = run_stages(pl, nullptr, pl->chains, 1, 1, pl->d_schi2, nullptr))) return rc;
     } else if (pl->kernel_timing) {
+        if (!pl->tuned && (rc = tune_plan(pl))) return rc;
         timed = count * pl->steps_per_sweep;
         if ((rc = ensure_events(pl, timed))) return rc;
         for (int j = 0; j < count; ++j)
             if ((rc = enqueue_sweep(pl, &pl->ev[2 * (size_t)j * pl->steps_per_sweep]))) return rc;
     } else if (count > 0) {
+        if (!pl->tuned && (rc = tune_plan(pl))) return rc;
         if (!pl->gexec) {
             HIPCHECK(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal), "graph capture");
             rc = enqueue_sweep(pl, nullptr);
@@ -995,6 +1085,299 @@ int pt_last_timing(const pt_plan *pl, double *total_ms, double *kernel_ms, int *
     if (total_ms) *total_ms = pl->last_total_ms;
     if (kernel_ms) *kernel_ms = pl->last_kernel_ms;
     if (launches) *launches = pl->last_launches;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------
+// groups: independent problems sharing every launch
+// ---------------------------------------------------------------------------------
+struct pt_group {
+    std::vector<pt_plan *> members;
+    std::vector<int64_t> offsets;          // state offsets of the members (doubles)
+    std::vector<int> chain_base;           // first entry of each member in the chain table
+    int device = -1, steps = 0, tpw = 2, total_chains = 0, total_pairs = 0, n_lines = 0;
+    int n_accum[3] = {0, 0, 0};
+    bool tuned = false;
+    double tune_ms[3] = {0, 0, 0};
+    double last_ms = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    std::vector<ptk::DevMember> hmem;
+    ptk::DevPlan *d_plans = nullptr;
+    ptk::DevMember *d_mem = nullptr;
+    int2 *d_chain_tab = nullptr, *d_pair_tab = nullptr;
+    int4 *d_lines_tab = nullptr;
+    int4 *d_accum_tab[3] = {nullptr, nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+void pt_group_destroy(pt_group *g) {
+    if (!g) return;
+    if (g->device >= 0) {
+        (void)hipSetDevice(g->device);
+        if (g->stream) (void)hipStreamSynchronize(g->stream);
+        drop_graph(g->gexec, g->graph);
+        for (void *p : g->allocs) (void)hipFree(p);
+        for (auto e : g->ev) if (e) (void)hipEventDestroy(e);
+        if (g->stream) (void)hipStreamDestroy(g->stream);
+    }
+    for (pt_plan *m : g->members) pt_plan_destroy(m);
+    delete g;
+}
+
+static inline int tiles_of(int np, int tpw) {
+    return (np + ptk::accum_points_per_group(tpw) - 1) / ptk::accum_points_per_group(tpw);
+}
+
+// accumulate work table for one tile shape: the tiles of each (member, chain) trial are
+// kept on one XCD (table slot i runs on XCD i mod 8), trials are placed heaviest first on
+// the least loaded XCD, and each XCD's list starts with its heaviest trials.  Unequal
+// lists are padded with empty slots (member -1).
+static std::vector<int4> accum_table(const pt_group *g, int tpw) {
+    struct Trial { double cost; int m, c, n; };
+    std::vector<Trial> trials;
+    for (int m = 0; m < (int)g->members.size(); ++m) {
+        const pt_plan *pl = g->members[m];
+        const int n = tiles_of(pl->np, tpw);
+        const double nl = 2.0 * pl->nr;
+        // per tile: scan of every span + evaluation of the lines that meet the tile
+        const double tile = nl + nl * (ptk::accum_points_per_group(tpw) + 100.0) / std::max(1, pl->np) * 4.0 * tpw;
+        for (int c = 0; c < pl->chains; ++c) trials.push_back({tile * n, m, c, n});
+    }
+    std::stable_sort(trials.begin(), trials.end(), [](const Trial &a, const Trial &b) { return a.cost > b.cost; });
+    const int kX = 8;
+    std::vector<std::vector<int4>> lists(kX);
+    double load[kX] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (const Trial &t : trials) {
+        int x = 0;
+        for (int k = 1; k < kX; ++k)
+            if (load[k] < load[x]) x = k;
+        load[x] += t.cost;
+        for (int s = 0; s < t.n; ++s) lists[x].push_back(make_int4(t.m, t.c, s, 0));
+    }
+    size_t len = 0;
+    for (auto &l : lists) len = std::max(len, l.size());
+    std::vector<int4> tab(len * kX, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < kX; ++x)
+        for (size_t j = 0; j < lists[x].size(); ++j) tab[j * kX + x] = lists[x][j];
+    return tab;
+}
+
+int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, pt_group **out) {
+    if (!out) return fail(PT_ERR_ARG, "output handle is NULL");
+    *out = nullptr;
+    if (!problems || count < 1) return fail(PT_ERR_ARG, "PT group needs at least one problem.");
+    pt_group *g = new pt_group();
+    int rc = 0;
+    for (int m = 0; m < count && !rc; ++m) {
+        pt_plan *pl = nullptr;
+        rc = create_impl(&problems[m], &pl, true);
+        if (rc) {
+            g_err = "PT group member " + std::to_string(m) + ": " + g_err;
+            break;
+        }
+        g->members.push_back(pl);
+    }
+    if (rc) { pt_group_destroy(g); return rc; }
+    g->device = g->members[0]->device;
+    (void)hipSetDevice(g->device);
+    int steps = steps_per_sweep;
+    if (steps <= 0)
+        for (pt_plan *pl : g->members) steps = std::max(steps, pl->steps_per_sweep);
+    g->steps = steps;
+    auto bail = [&](int code) { pt_group_destroy(g); return code; };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
+        return bail(fail(PT_ERR_DEVICE, "PT device lost during group setup."));
+
+    // per-member device copies of the plans (common sweep length) and staged buffers
+    std::vector<ptk::DevPlan> hp;
+    std::vector<int2> chain_tab, pair_tab;
+    std::vector<int4> lines_tab;
+    int64_t off = 0;
+    for (int m = 0; m < count; ++m) {
+        pt_plan *pl = g->members[m];
+        ptk::DevPlan P = pl->P;
+        P.steps_per_sweep = steps;
+        pl->steps_per_sweep = steps;
+        hp.push_back(P);
+        ptk::DevMember M{};
+        const size_t C = (size_t)pl->chains, nl = 2 * (size_t)pl->nr;
+        for (int v = 0; v < 3; ++v) M.nsplit[v] = tiles_of(pl->np, 1 << v);
+        if ((rc = dalloc(g, (char **)&M.img, C * ptk::trial_image_bytes())) || (rc = dalloc(g, &M.rec, C * nl)) ||
+            (rc = dalloc(g, &M.span, C * nl)) || (rc = dalloc(g, &M.partial, C * M.nsplit[0])) || (rc = dalloc(g, &M.chi2, C)))
+            return bail(rc);
+        g->hmem.push_back(M);
+        g->offsets.push_back(off);
+        off += (int64_t)pl->chains * pl->stride;
+        g->chain_base.push_back((int)chain_tab.size());
+        for (int c = 0; c < pl->chains; ++c) chain_tab.push_back(make_int2(m, c));
+        for (int q = 0; q < (pl->chains + 1) / 2; ++q) pair_tab.push_back(make_int2(m, q));
+        const int kb = (pl->nr + 127) / 128;
+        for (int c = 0; c < pl->chains; ++c)
+            for (int b = 0; b < kb; ++b) lines_tab.push_back(make_int4(m, c, b, 0));
+    }
+    g->total_chains = (int)chain_tab.size();
+    g->total_pairs = (int)pair_tab.size();
+    g->n_lines = (int)lines_tab.size();
+    if ((rc = upload(g, &g->d_plans, hp)) || (rc = upload(g, &g->d_mem, g->hmem)) || (rc = upload(g, &g->d_chain_tab, chain_tab)) ||
+        (rc = upload(g, &g->d_pair_tab, pair_tab)) || (rc = upload(g, &g->d_lines_tab, lines_tab)))
+        return bail(rc);
+    for (int v = 0; v < 3; ++v) {
+        std::vector<int4> tab = accum_table(g, 1 << v);
+        g->n_accum[v] = (int)tab.size();
+        if ((rc = upload(g, &g->d_accum_tab[v], tab))) return bail(rc);
+    }
+    *out = g;
+    return 0;
+}
+
+#define NEED_GROUP(g) \
+    do { if (!(g)) return fail(PT_ERR_ARG, "NULL group"); } while (0)
+
+int pt_group_member(const pt_group *g, int member, pt_plan_info *info, int64_t *state_offset) {
+    NEED_GROUP(g);
+    if (member < 0 || member >= (int)g->members.size()) return fail(PT_ERR_ARG, "group member out of range");
+    if (info) { int rc = pt_plan_info_get(g->members[member], info); if (rc) return rc; }
+    if (state_offset) *state_offset = g->offsets[member];
+    return 0;
+}
+
+int pt_group_set_states(pt_group *g, int member, const double *x, int nvec) {
+    NEED_GROUP(g);
+    if (member < 0 || member >= (int)g->members.size()) return fail(PT_ERR_ARG, "group member out of range");
+    return pt_set_states(g->members[member], x, nvec);
+}
+
+static int group_ready(pt_group *g) {
+    for (size_t m = 0; m < g->members.size(); ++m)
+        if (!g->members[m]->states_set)
+            return fail(PT_ERR_STATE, "PT group batch before initial state of member " + std::to_string(m) + ".");
+    (void)hipSetDevice(g->device);
+    return 0;
+}
+
+static inline int vidx(int tpw) { return tpw == 1 ? 0 : tpw == 4 ? 2 : 1; }
+
+// one bootstrap-style pass over every chain's current vector: setup (no move), lines,
+// accumulate, then the Metropolis bootstrap (pt_mode) or the chi^2 into the members' chi2
+static int group_pass(pt_group *g, int pt_mode) {
+    hipStream_t s = g->stream;
+    HIPCHECK(ptk::launch_group_setup(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, 1, s), "setup");
+    HIPCHECK(ptk::launch_group_lines(g->d_plans, g->d_mem, g->d_lines_tab, g->n_lines, s), "lines");
+    HIPCHECK(ptk::launch_group_accum(g->d_plans, g->d_mem, g->d_accum_tab[vidx(g->tpw)], g->n_accum[vidx(g->tpw)], g->tpw, s),
+             "accumulate");
+    HIPCHECK(ptk::launch_group_finish(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, pt_mode, s), "finish");
+    return 0;
+}
+
+static int group_sweep(pt_group *g) {
+    hipStream_t s = g->stream;
+    const int v = vidx(g->tpw);
+    HIPCHECK(ptk::launch_group_setup(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, 0, s), "setup");
+    for (int k = 0; k < g->steps; ++k) {
+        HIPCHECK(ptk::launch_group_lines(g->d_plans, g->d_mem, g->d_lines_tab, g->n_lines, s), "lines");
+        HIPCHECK(ptk::launch_group_accum(g->d_plans, g->d_mem, g->d_accum_tab[v], g->n_accum[v], g->tpw, s), "accumulate");
+        if (k + 1 < g->steps)
+            HIPCHECK(ptk::launch_group_finish_setup(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, s), "finish");
+        else
+            HIPCHECK(ptk::launch_group_finish(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, 1, s), "finish");
+    }
+    HIPCHECK(ptk::launch_group_exchange(g->d_plans, g->d_pair_tab, g->total_pairs, s), "exchange");
+    return 0;
+}
+
+static int tune_group(pt_group *g) {
+    hipStream_t s = g->stream;
+    HIPCHECK(ptk::launch_group_setup(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, 1, s), "setup");
+    HIPCHECK(ptk::launch_group_lines(g->d_plans, g->d_mem, g->d_lines_tab, g->n_lines, s), "lines");
+    int best = 2;
+    int rc = time_tiles(s, [&](int tpw) {
+        return ptk::launch_group_accum(g->d_plans, g->d_mem, g->d_accum_tab[vidx(tpw)], g->n_accum[vidx(tpw)], tpw, s);
+    }, g->tune_ms, &best);
+    if (rc) return rc;
+    if (best != g->tpw) drop_graph(g->gexec, g->graph);
+    g->tpw = best;
+    g->tuned = true;
+    return 0;
+}
+
+int pt_group_tiles(pt_group *g, int request, int *tiles_per_wave, double *candidate_ms) {
+    NEED_GROUP(g);
+    if (request == -1) {
+        int rc = group_ready(g);
+        if (rc || (rc = tune_group(g))) return rc;
+    } else if (request == 1 || request == 2 || request == 4) {
+        if (request != g->tpw) drop_graph(g->gexec, g->graph);
+        g->tpw = request;
+        g->tuned = true;
+    } else if (request != 0) {
+        return fail(PT_ERR_ARG, "tile request must be 0, -1, 1, 2 or 4");
+    }
+    if (tiles_per_wave) *tiles_per_wave = g->tpw;
+    if (candidate_ms) for (int i = 0; i < 3; ++i) candidate_ms[i] = g->tune_ms[i];
+    return 0;
+}
+
+int pt_group_costs(pt_group *g, double *chi2) {
+    NEED_GROUP(g);
+    if (!chi2) return fail(PT_ERR_ARG, "NULL argument");
+    int rc = group_ready(g);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(g->ev[0], g->stream), "timing");
+    if ((rc = group_pass(g, 0))) return rc;
+    HIPCHECK(hipEventRecord(g->ev[1], g->stream), "timing");
+    for (size_t m = 0; m < g->members.size(); ++m)
+        HIPCHECK(hipMemcpyAsync(chi2 + g->chain_base[m], g->hmem[m].chi2, sizeof(double) * g->members[m]->chains,
+                                hipMemcpyDeviceToHost, g->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(g->stream), "group costs");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
+    g->last_ms = ms;
+    return 0;
+}
+
+int pt_group_batch(pt_group *g, int count, int bootstrap, double *state_out) {
+    NEED_GROUP(g);
+    if (count < 0) return fail(PT_ERR_ARG, "negative sweep count");
+    int rc = group_ready(g);
+    if (rc) return rc;
+    if (!bootstrap && count > 0 && !g->tuned && (rc = tune_group(g))) return rc;
+    HIPCHECK(hipEventRecord(g->ev[0], g->stream), "timing");
+    if (bootstrap) {
+        if ((rc = group_pass(g, 1))) return rc;
+    } else if (count > 0) {
+        if (!g->gexec) {
+            HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            rc = group_sweep(g);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(g->stream, &gr);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(PT_ERR_DEVICE, std::string("PT graph capture failed: ") + hipGetErrorString(e));
+            g->graph = gr;
+            HIPCHECK(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0), "graph instantiation");
+        }
+        for (int j = 0; j < count; ++j) HIPCHECK(hipGraphLaunch(g->gexec, g->stream), "sweep");
+    }
+    HIPCHECK(hipEventRecord(g->ev[1], g->stream), "timing");
+    if (state_out)
+        for (size_t m = 0; m < g->members.size(); ++m) {
+            const pt_plan *pl = g->members[m];
+            HIPCHECK(hipMemcpyAsync(state_out + g->offsets[m], pl->P.state, sizeof(double) * pl->chains * pl->stride,
+                                    hipMemcpyDeviceToHost, g->stream), "readback");
+        }
+    HIPCHECK(hipStreamSynchronize(g->stream), "sweeps");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
+    g->last_ms = ms;
+    return 0;
+}
+
+int pt_group_last_timing(const pt_group *g, double *total_ms) {
+    NEED_GROUP(g);
+    if (total_ms) *total_ms = g->last_ms;
     return 0;
 }
 

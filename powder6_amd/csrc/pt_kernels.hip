@@ -205,7 +205,7 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
 // With unit_factor != null the lines are written at unit height (the Kalpha2 line at its
 // doublet weight) and the reflection's height factor m Lp PO / area goes to
 // unit_factor[k] instead: the columns of the shared profile matrix.
-__device__ inline void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
+__device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
                                         float *unit_factor = nullptr) {
     if (unit_factor) unit_factor[k] = 0.0f;
     span[0] = make_int4(0, 0, 0, 0);
@@ -492,7 +492,7 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Share
 // (js/pt_gpu.js:361 propose; rietveld_core.js:2196-2212): one coordinate j drawn
 // uniformly, x_j + z scale_j sqrt(T_c) reflected into its box, log u for the Metropolis
 // test.  Random numbers are keyed on (seed, chain, the chain's step counter).
-__device__ inline void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
+__device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
                                   Shared &sh, double *prop) {
     const int nv = P.nv;
     bool boot = false;
@@ -543,15 +543,13 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
     setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop);
 }
 
-// line records of reflections [blockIdx.x * blockDim.x, ...) of trial blockIdx.y
-__global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, const TrialImage *__restrict__ img, LineRec *rec,
-                                                     int4 *span) {
-    __shared__ Shared sh;
-    const int t = blockIdx.y;
+// line records of reflections [kb * blockDim.x, ...) of trial t
+__device__ __forceinline__ void lines_body(const DevPlan &P, const TrialImage *__restrict__ img, LineRec *rec, int4 *span, int t,
+                                  int kb, Shared &sh) {
     stage_reps(P, sh);
     image_load(P, img + t, sh);
     __syncthreads();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = kb * blockDim.x + threadIdx.x;
     if (k >= P.nr) return;
     const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
     if (sh.go) {
@@ -562,10 +560,18 @@ __global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, const TrialImage
     }
 }
 
+__global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, const TrialImage *__restrict__ img, LineRec *rec,
+                                                     int4 *span) {
+    __shared__ Shared sh;
+    lines_body(P, img, rec, span, blockIdx.y, blockIdx.x, sh);
+}
+
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
-constexpr int kAccTilesPerWave = 2;    // 128-point tiles per wavefront (2 points per lane)
-constexpr int kAccPoints = kAccWaves * kAccTilesPerWave * 128;   // points per workgroup
 constexpr int kStage = 384;            // line records staged in LDS per pass
+
+// points per accumulate workgroup for TPW 128-point tiles per wavefront (two points per lane
+// and tile); TPW is the tile shape the autotuner picks (1, 2 or 4)
+__host__ __device__ constexpr int acc_points(int tpw) { return kAccWaves * tpw * 128; }
 
 // one line at one point of a lane (delta = t - centre); `split` and `asym` are uniform
 // over the wavefront (scalar branches), the window test is a select
@@ -607,42 +613,46 @@ __device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, 
     a1 = fmaf(amp1, v1, a1);
 }
 
-// Banded profile sum and partial chi^2 of points [1024 s, 1024 s + 1024) of trial t
-// (s = blockIdx.x, t = blockIdx.y): the lines whose windows meet those points are
-// compacted, in line order, into LDS; every wavefront then sums them for its own two
-// 128-point tiles (two points per lane) and the chi^2 of its points is reduced with
-// two-sum.
-__global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
-                                                     const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
-                                                     double2 *partial, int nsplit) {
-    __shared__ LineRec srec[kStage];
-    __shared__ int4 sspan[kStage];
-    __shared__ float bkg[PTK_MAX_BKG_F];
-    __shared__ int wcnt[kAccWaves];
-    __shared__ int s_count;
-    __shared__ float red_s[kAccWaves], red_c[kAccWaves];
-    const int t = blockIdx.y, split = blockIdx.x;
+// LDS of one accumulate workgroup
+struct AccShared {
+    LineRec rec[kStage];
+    int4 span[kStage];
+    float bkg[PTK_MAX_BKG_F];
+    int wcnt[kAccWaves];
+    int count;
+    float red_s[4][kAccWaves], red_c[4][kAccWaves];
+};
+
+// Banded profile sum and partial chi^2 of points [P s, P s + P) of trial t, P =
+// acc_points(TPW): the lines whose windows meet those points are compacted, in line
+// order, into LDS; every wavefront then sums them for its own TPW 128-point tiles (two
+// points per lane) and the chi^2 of every 512-point group is reduced with two-sum into
+// partial[t * nsplit + group], nsplit = the trial's 512-point group count.
+template <int TPW>
+__device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict__ img, const LineRec *__restrict__ grec,
+                                  const int4 *__restrict__ gspan, double2 *partial, int nsplit, int t, int split,
+                                  AccShared &S) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const TrialImage *im = img + t;
     if (!im->go) {
-        if (tid == 0) partial[(size_t)t * nsplit + split] = make_double2(0.0, 0.0);
+        if (tid < TPW && split * TPW + tid < nsplit) partial[(size_t)t * nsplit + split * TPW + tid] = make_double2(0.0, 0.0);
         return;
     }
-    if (tid < PTK_MAX_BKG_F) bkg[tid] = im->bkg[tid];
-    const int lo_pt = split * kAccPoints, hi_pt = min(P.np, lo_pt + kAccPoints);
+    if (tid < PTK_MAX_BKG_F) S.bkg[tid] = im->bkg[tid];
+    const int lo_pt = split * acc_points(TPW), hi_pt = min(P.np, lo_pt + acc_points(TPW));
     const int nl = 2 * P.nr;
     const LineRec *rec = grec + (size_t)t * nl;
     const int4 *sp = gspan + (size_t)t * nl;
-    float acc[2 * kAccTilesPerWave];
-    float2 tf[2 * kAccTilesPerWave];
+    float acc[2 * TPW];
+    float2 tf[2 * TPW];
 #pragma unroll
-    for (int j = 0; j < kAccTilesPerWave; ++j) {
+    for (int j = 0; j < TPW; ++j) {
         const int base = lo_pt + (wave + kAccWaves * j) * 128 + lane;
         acc[2 * j] = acc[2 * j + 1] = 0.0f;
         tf[2 * j] = base < P.np ? P.tff[base] : make_float2(0.f, 0.f);
         tf[2 * j + 1] = base + 64 < P.np ? P.tff[base + 64] : make_float2(0.f, 0.f);
     }
-    if (tid == 0) s_count = 0;
+    if (tid == 0) S.count = 0;
     __syncthreads();
     for (int c0 = 0; c0 < nl; c0 += 256) {
         // compact the overlapping lines of this chunk, in order, into the stage
@@ -651,19 +661,19 @@ __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage
         bool ov = false;
         if (L < nl) { sv = sp[L]; ov = sv.x < hi_pt && sv.y > lo_pt; }
         const unsigned long long m = __ballot(ov);
-        if (lane == 0) wcnt[wave] = __popcll(m);
+        if (lane == 0) S.wcnt[wave] = __popcll(m);
         __syncthreads();
-        int off = s_count;
-        for (int w = 0; w < wave; ++w) off += wcnt[w];
+        int off = S.count;
+        for (int w = 0; w < wave; ++w) off += S.wcnt[w];
         const int pre = __popcll(m & ((1ull << lane) - 1ull));
-        if (ov) { sspan[off + pre] = sv; srec[off + pre] = rec[L]; }
+        if (ov) { S.span[off + pre] = sv; S.rec[off + pre] = rec[L]; }
         __syncthreads();
-        if (tid == 0) { int n = s_count; for (int w = 0; w < kAccWaves; ++w) n += wcnt[w]; s_count = n; }
+        if (tid == 0) { int n = S.count; for (int w = 0; w < kAccWaves; ++w) n += S.wcnt[w]; S.count = n; }
         __syncthreads();
-        const int count = s_count;
+        const int count = S.count;
         if (count > kStage - 256 || (c0 + 256 >= nl && count > 0)) {
 #pragma unroll
-            for (int j = 0; j < kAccTilesPerWave; ++j) {
+            for (int j = 0; j < TPW; ++j) {
                 const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
                 if (t_lo < hi_pt) {
                     const int t_hi = t_lo + 128, i0 = t_lo + lane;
@@ -671,13 +681,13 @@ __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage
                     for (int lb = 0; lb < count; lb += 64) {
                         const int q = lb + lane;
                         bool o = false;
-                        if (q < count) { int4 v = sspan[q]; o = v.x < t_hi && v.y > t_lo; }
+                        if (q < count) { int4 v = S.span[q]; o = v.x < t_hi && v.y > t_lo; }
                         unsigned long long mask = __ballot(o);
                         while (mask) {
                             const int b = __ffsll((long long)mask) - 1;
                             mask &= mask - 1ull;
-                            const int4 v = sspan[lb + b];
-                            const LineRec &R = srec[lb + b];
+                            const int4 v = S.span[lb + b];
+                            const LineRec &R = S.rec[lb + b];
                             // line flags are the same for every lane: keep them scalar
                             const int flags = __builtin_amdgcn_readfirstlane(v.w);
                             const float asym = __int_as_float(__builtin_amdgcn_readfirstlane(v.z));
@@ -689,48 +699,65 @@ __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage
                 }
             }
             __syncthreads();
-            if (tid == 0) s_count = 0;
+            if (tid == 0) S.count = 0;
             __syncthreads();
         }
     }
-    // chi^2 of this workgroup's points
-    float s = 0.f, c = 0.f;
+    // chi^2 of this workgroup's points, one partial per 512-point group (the 128-point
+    // tiles j of the four wavefronts), so that the partials -- and the ordered sum the
+    // finish takes over them -- do not depend on the tile shape
+    const int ngroups = (P.np + 511) >> 9;
 #pragma unroll
-    for (int j = 0; j < 2 * kAccTilesPerWave; ++j) {
-        const int i = lo_pt + (wave + kAccWaves * (j >> 1)) * 128 + (j & 1) * 64 + lane;
-        if (i < hi_pt) {
-            const float2 ow = P.obs[i];
-            if (ow.y > 0.0f) {
-                float yb = 0.0f;
-                if (P.n_bkg > 0) {
-                    float z = (float)(P.thi > P.tlo ? 2.0 * (P.tth[i] - P.tlo) / (P.thi - P.tlo) - 1.0 : 0.0);
-                    float ta = 1.0f, tb = z;
-                    yb = bkg[0];
-                    if (P.n_bkg > 1) yb = fmaf(bkg[1], z, yb);
-                    for (int k = 2; k < P.n_bkg; ++k) {
-                        float nx = 2.0f * z * tb - ta;
-                        ta = tb; tb = nx;
-                        yb = fmaf(bkg[k], nx, yb);
+    for (int j = 0; j < TPW; ++j) {
+        float s = 0.f, c = 0.f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = lo_pt + (wave + kAccWaves * j) * 128 + h * 64 + lane;
+            if (i < hi_pt) {
+                const float2 ow = P.obs[i];
+                if (ow.y > 0.0f) {
+                    float yb = 0.0f;
+                    if (P.n_bkg > 0) {
+                        float z = (float)(P.thi > P.tlo ? 2.0 * (P.tth[i] - P.tlo) / (P.thi - P.tlo) - 1.0 : 0.0);
+                        float ta = 1.0f, tb = z;
+                        yb = S.bkg[0];
+                        if (P.n_bkg > 1) yb = fmaf(S.bkg[1], z, yb);
+                        for (int k = 2; k < P.n_bkg; ++k) {
+                            float nx = 2.0f * z * tb - ta;
+                            ta = tb; tb = nx;
+                            yb = fmaf(S.bkg[k], nx, yb);
+                        }
                     }
+                    const float r = ow.x - (yb + acc[2 * j + h]);
+                    neumaier(s, c, ow.y * r * r);
                 }
-                const float r = ow.x - (yb + acc[j]);
-                neumaier(s, c, ow.y * r * r);
             }
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            float so = __shfl_xor(s, o), co = __shfl_xor(c, o);
+            float tt = s + so;
+            float e = fabsf(s) >= fabsf(so) ? (s - tt) + so : (so - tt) + s;
+            s = tt; c = c + co + e;
+        }
+        if (lane == 0) { S.red_s[j][wave] = s; S.red_c[j][wave] = c; }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        float so = __shfl_xor(s, o), co = __shfl_xor(c, o);
-        float tt = s + so;
-        float e = fabsf(s) >= fabsf(so) ? (s - tt) + so : (so - tt) + s;
-        s = tt; c = c + co + e;
-    }
-    if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
     __syncthreads();
-    if (tid == 0) {
-        double hi = 0.0, lo = 0.0;
-        for (int w = 0; w < kAccWaves; ++w) { hi += (double)red_s[w]; lo += (double)red_c[w]; }
-        partial[(size_t)t * nsplit + split] = make_double2(hi, lo);
+    if (tid < TPW) {
+        const int g = split * TPW + tid;
+        if (g < ngroups) {
+            double hi = 0.0, lo = 0.0;
+            for (int w = 0; w < kAccWaves; ++w) { hi += (double)S.red_s[tid][w]; lo += (double)S.red_c[tid][w]; }
+            partial[(size_t)t * nsplit + g] = make_double2(hi, lo);
+        }
     }
+}
+
+template <int TPW>
+__global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
+                                                     const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
+                                                     double2 *partial, int nsplit) {
+    __shared__ AccShared S;
+    accum_body<TPW>(P, img, grec, gspan, partial, nsplit, blockIdx.y, blockIdx.x, S);
 }
 
 // chi^2 of trial t: the ordered sum of its partials (3e38 for an invalid trial)
@@ -770,15 +797,10 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
     return r;
 }
 
-// chi^2 of every trial (batch mode) or the Metropolis step of every chain (PT mode)
-__global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage *__restrict__ img,
-                                                     const double2 *__restrict__ partial, int nsplit, int ntrials,
-                                                     double *chi2, int pt_mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntrials) return;
-    const double cost = trial_cost(img + t, partial, nsplit, t);
-    if (!pt_mode) { chi2[t] = cost; return; }
-    const int r = accept_chain(P, img + t, cost, t);
+// the end of one PT step for chain t: Metropolis accept, and on acceptance the proposal
+// becomes the chain's state (and its best when it is a new feasible best)
+__device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, double cost, int t) {
+    const int r = accept_chain(P, im, cost, t);
     if (r) {
         double *st = P.state + (size_t)t * P.stride;
         const int nv = P.nv;
@@ -789,14 +811,22 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
     }
 }
 
-// the Metropolis step of chain blockIdx.x, then its next proposal and setup: the finish
-// of one PT step fused with the setup of the next (one launch less per step)
-__global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
-                                                           int nsplit) {
-    __shared__ Shared sh;
-    __shared__ double prop[PTK_MAX_VARS];
-    __shared__ int s_acc;
-    const int t = blockIdx.x, nv = P.nv;
+// chi^2 of every trial (batch mode) or the Metropolis step of every chain (PT mode)
+__global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage *__restrict__ img,
+                                                     const double2 *__restrict__ partial, int nsplit, int ntrials,
+                                                     double *chi2, int pt_mode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntrials) return;
+    const double cost = trial_cost(img + t, partial, nsplit, t);
+    if (!pt_mode) { chi2[t] = cost; return; }
+    finish_chain(P, img + t, cost, t);
+}
+
+// the Metropolis step of chain t, then its next proposal and setup: the finish of one PT
+// step fused with the setup of the next (one launch less per step)
+__device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *img, const double2 *__restrict__ partial,
+                                         int nsplit, int t, Shared &sh, double *prop, int &s_acc) {
+    const int nv = P.nv;
     if (threadIdx.x == 0) s_acc = accept_chain(P, img + t, trial_cost(img + t, partial, nsplit, t), t);
     __syncthreads();
     const int r = s_acc;
@@ -812,20 +842,28 @@ __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage
     setup_body(P, nullptr, img, t, 1, 0, sh, prop);
 }
 
+__global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
+                                                           int nsplit) {
+    __shared__ Shared sh;
+    __shared__ double prop[PTK_MAX_VARS];
+    __shared__ int s_acc;
+    finish_setup_body(P, img, partial, nsplit, blockIdx.x, sh, prop, s_acc);
+}
+
 // ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
-__global__ void k_exchange(DevPlan P) {
+// pair `pair` of the sweep's parity: chains c = 2 pair + parity and c + 1
+__device__ inline void exchange_body(const DevPlan &P, int pair) {
     const int nv = P.nv;
     // sweeps completed, from chain 0's step counter (js/pt_gpu.js:365 parity rule)
     const double done = P.state[3 * nv + 2] / (double)P.steps_per_sweep;
     const long long sw = (long long)done;
     const int parity = (int)(((sw - 1) % 2 + 2) % 2);
-    int pair = blockIdx.x * blockDim.x + threadIdx.x;
-    int c = 2 * pair + parity;
+    const int c = 2 * pair + parity;
     if (c + 1 >= P.chains) return;
     double *a = P.state + (size_t)c * P.stride, *b = P.state + (size_t)(c + 1) * P.stride;
-    double den = (double)(P.chains > 1 ? P.chains - 1 : 1);
-    double t0 = pow(P.max_temp, (double)c / den), t1 = pow(P.max_temp, (double)(c + 1) / den);
-    double la = 0.5 * (1.0 / t0 - 1.0 / t1) * (a[3 * nv] - b[3 * nv]);
+    const double den = (double)(P.chains > 1 ? P.chains - 1 : 1);
+    const double t0 = pow(P.max_temp, (double)c / den), t1 = pow(P.max_temp, (double)(c + 1) / den);
+    const double la = 0.5 * (1.0 / t0 - 1.0 / t1) * (a[3 * nv] - b[3 * nv]);
     a[3 * nv + 6] += 1.0;
     bool swp = la >= 0.0;
     if (!swp) swp = log(uniform01(P.seed ^ 0x5bd1e995u, c, (unsigned long long)sw, 7)) < la;
@@ -835,6 +873,78 @@ __global__ void k_exchange(DevPlan P) {
         v = a[3 * nv + 11]; a[3 * nv + 11] = b[3 * nv + 11]; b[3 * nv + 11] = v;
         a[3 * nv + 5] += 1.0;
     }
+}
+
+__global__ void k_exchange(DevPlan P) { exchange_body(P, blockIdx.x * blockDim.x + threadIdx.x); }
+
+// ---- groups: independent problems sharing every launch --------------------------------
+// Each stage of a group runs as one launch over all members.  Workgroup b takes its
+// (member, chain, tile) from a table built on the host: member-major for the per-chain
+// stages; for the accumulate stage grouped so that every tile of one trial lands on the
+// same XCD (workgroup b runs on XCD b mod 8, so the trial's line records are fetched into
+// one L2) with the heaviest trials first.  A member entry of -1 is an empty slot.
+__global__ void __launch_bounds__(64) k_group_setup(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
+                                                    const int2 *__restrict__ tab, int bootstrap) {
+    __shared__ Shared sh;
+    __shared__ double prop[PTK_MAX_VARS];
+    const int2 w = tab[blockIdx.x];
+    const DevPlan &P = plans[w.x];
+    setup_body(P, nullptr, (TrialImage *)mem[w.x].img, w.y, 1, bootstrap, sh, prop);
+}
+
+__global__ void __launch_bounds__(128) k_group_lines(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
+                                                     const int4 *__restrict__ tab) {
+    __shared__ Shared sh;
+    const int4 w = tab[blockIdx.x];
+    const DevPlan &P = plans[w.x];
+    const DevMember &M = mem[w.x];
+    lines_body(P, (const TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh);
+}
+
+template <int TPW>
+__global__ void __launch_bounds__(256) k_group_accum(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
+                                                     const int4 *__restrict__ tab) {
+    __shared__ AccShared S;
+    const int4 w = tab[blockIdx.x];
+    if (w.x < 0) return;
+    const DevPlan &P = plans[w.x];
+    const DevMember &M = mem[w.x];
+    accum_body<TPW>(P, (const TrialImage *)M.img, M.rec, M.span, M.partial, M.nsplit[0], w.y, w.z, S);
+}
+
+// one thread per chain: the Metropolis step (pt_mode) or the chi^2 of the chain's vector
+__global__ void __launch_bounds__(64) k_group_finish(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
+                                                     const int2 *__restrict__ tab, int nchains, int pt_mode) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nchains) return;
+    const int2 w = tab[b];
+    const DevPlan &P = plans[w.x];
+    const DevMember &M = mem[w.x];
+    const TrialImage *img = (const TrialImage *)M.img;
+    const double cost = trial_cost(img + w.y, M.partial, M.nsplit[0], w.y);
+    if (pt_mode) finish_chain(P, img + w.y, cost, w.y);
+    else M.chi2[w.y] = cost;
+}
+
+__global__ void __launch_bounds__(64) k_group_finish_setup(const DevPlan *__restrict__ plans,
+                                                           const DevMember *__restrict__ mem,
+                                                           const int2 *__restrict__ tab) {
+    __shared__ Shared sh;
+    __shared__ double prop[PTK_MAX_VARS];
+    __shared__ int s_acc;
+    const int2 w = tab[blockIdx.x];
+    const DevPlan &P = plans[w.x];
+    const DevMember &M = mem[w.x];
+    finish_setup_body(P, (TrialImage *)M.img, M.partial, M.nsplit[0], w.y, sh, prop, s_acc);
+}
+
+// one thread per (member, pair)
+__global__ void __launch_bounds__(64) k_group_exchange(const DevPlan *__restrict__ plans, const int2 *__restrict__ tab,
+                                                       int npairs) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= npairs) return;
+    const int2 w = tab[b];
+    exchange_body(plans[w.x], w.y);
 }
 
 // ---- plain single-trial path: setup, lines, points, reduce ---------------------------
@@ -1097,7 +1207,7 @@ namespace ptk {
 size_t shared_static_bytes() { return sizeof(Shared); }
 size_t plain_scratch_bytes() { return sizeof(PlainScratch); }
 size_t trial_image_bytes() { return sizeof(TrialImage); }
-int accum_points_per_group() { return kAccPoints; }
+int accum_points_per_group(int tpw) { return acc_points(tpw); }
 
 hipError_t set_lds_limits(size_t bytes) {
     hipError_t e;
@@ -1116,8 +1226,12 @@ hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, i
     return hipGetLastError();
 }
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
-                              int nsplit, int ntrials, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_accum, dim3(nsplit, ntrials), dim3(256), 0, s, P, (const TrialImage *)img, rec, span, partial, nsplit);
+                              int ngroups, int ntrials, int tpw, hipStream_t s) {
+    const dim3 grid((ngroups + tpw - 1) / tpw, ntrials);
+    const TrialImage *im = (const TrialImage *)img;
+    if (tpw == 1) hipLaunchKernelGGL(k_stage_accum<1>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups);
+    else if (tpw == 4) hipLaunchKernelGGL(k_stage_accum<4>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups);
+    else hipLaunchKernelGGL(k_stage_accum<2>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups);
     return hipGetLastError();
 }
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
@@ -1174,6 +1288,35 @@ hipError_t launch_workload(const DevPlan &P, const double *X, long long *stats, 
 hipError_t launch_exchange(const DevPlan &P, hipStream_t s) {
     int pairs = (P.chains + 1) / 2;
     hipLaunchKernelGGL(k_exchange, dim3((pairs + 63) / 64), dim3(64), 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_setup(const DevPlan *plans, const DevMember *mem, const int2 *tab, int n, int bootstrap,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_group_setup, dim3(n), dim3(64), 0, s, plans, mem, tab, bootstrap);
+    return hipGetLastError();
+}
+hipError_t launch_group_lines(const DevPlan *plans, const DevMember *mem, const int4 *tab, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_group_lines, dim3(n), dim3(128), 0, s, plans, mem, tab);
+    return hipGetLastError();
+}
+hipError_t launch_group_accum(const DevPlan *plans, const DevMember *mem, const int4 *tab, int n, int tpw, hipStream_t s) {
+    if (tpw == 1) hipLaunchKernelGGL(k_group_accum<1>, dim3(n), dim3(256), 0, s, plans, mem, tab);
+    else if (tpw == 4) hipLaunchKernelGGL(k_group_accum<4>, dim3(n), dim3(256), 0, s, plans, mem, tab);
+    else hipLaunchKernelGGL(k_group_accum<2>, dim3(n), dim3(256), 0, s, plans, mem, tab);
+    return hipGetLastError();
+}
+hipError_t launch_group_finish(const DevPlan *plans, const DevMember *mem, const int2 *tab, int nchains, int pt_mode,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_group_finish, dim3((nchains + 63) / 64), dim3(64), 0, s, plans, mem, tab, nchains, pt_mode);
+    return hipGetLastError();
+}
+hipError_t launch_group_finish_setup(const DevPlan *plans, const DevMember *mem, const int2 *tab, int nchains, hipStream_t s) {
+    hipLaunchKernelGGL(k_group_finish_setup, dim3(nchains), dim3(64), 0, s, plans, mem, tab);
+    return hipGetLastError();
+}
+hipError_t launch_group_exchange(const DevPlan *plans, const int2 *tab, int npairs, hipStream_t s) {
+    if (npairs > 0) hipLaunchKernelGGL(k_group_exchange, dim3((npairs + 63) / 64), dim3(64), 0, s, plans, tab, npairs);
     return hipGetLastError();
 }
 

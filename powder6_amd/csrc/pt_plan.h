@@ -95,4 +95,17 @@ struct LineRec {
     float4 a, l, r;
 };
 
+// staged-pipeline buffers of one member of a group (per chain: a trial image, 2 nr line
+// records and spans, one partial chi^2 per 512-point group, the chi^2)
+struct DevMember {
+    void *img;
+    LineRec *rec;
+    int4 *span;
+    double2 *partial;
+    double *chi2;
+    int nsplit[3];             // accumulate workgroups per trial for 1, 2, 4 tiles per wavefront;
+                               // nsplit[0] is also the trial's 512-point group count
+    int pad;
+};
+
 }  // namespace ptk

@@ -443,18 +443,41 @@ UNEVEN_ELEMENTS = ['O', 'Mg', 'Si', 'S', 'K', 'Ca', 'Ti', 'Fe', 'Cu', 'Zn', 'Ba'
 
 def uneven_pattern(index, simulate=None, seed=20260303, chains=32):
     """Pattern `index` of the uneven batch: a cubic, tetragonal or monoclinic cell with
-    edges in [4, 20] A, 1-12 sites, 2000-12000 points over a seeded 2theta range."""
+    edges in [4, 20] A, 1-12 sites, 2000-12000 points.  The reflection count is aimed at a
+    log-uniform target in [50, 3000]: the cell is scaled so that the lattice points in the
+    envelope sphere, (4 pi / 3) q^3 V, divided by the operator count (Laue orbit x
+    centring) meet the target, edges clipped to [4, 20] A."""
     rng = np.random.default_rng(seed + 1000 * index)
     grp, system = UNEVEN_GROUPS[index % len(UNEVEN_GROUPS)]
     r, t = S.group_ops(grp)
+    n_pts = int(rng.integers(2000, 12001))
+    lo = float(rng.uniform(5.0, 20.0))
+    step = float(rng.choice([0.01, 0.015, 0.02]))
+    step = min(step, (160.0 - lo) / n_pts)
+    tth = lo + step * np.arange(n_pts, dtype=np.float64)
+    target = math.exp(rng.uniform(math.log(50.0), math.log(3000.0)))
+    # envelope sphere: 2theta_max + 10 deg + the zero-shift limit (2 deg), lambda = Ka1
+    th = math.radians(min(179.0, tth[-1] + 12.0) / 2.0)
+    q = 2.0 * math.sin(th) / CU_KA1
     if system == 'cubic':
-        a = rng.uniform(4.0, 14.0)
-        cell = [a, a, a, 90, 90, 90]
+        shape = [1.0, 1.0, 1.0]
+        ang = [90, 90, 90]
     elif system == 'tetragonal':
-        a = rng.uniform(4.0, 14.0)
-        cell = [a, a, rng.uniform(4.0, 20.0), 90, 90, 90]
+        shape = [1.0, 1.0, rng.uniform(0.4, 2.5)]
+        ang = [90, 90, 90]
     else:
-        cell = [rng.uniform(4.0, 16.0), rng.uniform(4.0, 20.0), rng.uniform(4.0, 14.0), 90, rng.uniform(95, 115), 90]
+        shape = [1.0, rng.uniform(0.6, 1.8), rng.uniform(0.5, 1.5)]
+        ang = [90, rng.uniform(95, 115), 90]
+    sin_b = math.sin(math.radians(ang[1]))
+    v_shape = shape[0] * shape[1] * shape[2] * sin_b
+    v_target = target * len(r) / (4.0 * math.pi / 3.0 * q ** 3)
+    f = (v_target / v_shape) ** (1.0 / 3.0)
+    edges = [min(20.0, max(4.0, f * e)) for e in shape]
+    if system == 'cubic':
+        edges = [edges[0]] * 3
+    elif system == 'tetragonal':
+        edges[1] = edges[0]
+    cell = edges + ang
     n_sites = int(rng.integers(1, 13))
     atoms = []
     for s_i in range(n_sites):
@@ -462,11 +485,6 @@ def uneven_pattern(index, simulate=None, seed=20260303, chains=32):
         site = S.site_symmetry(r, t, xyz, cell)
         el = UNEVEN_ELEMENTS[int(rng.integers(0, len(UNEVEN_ELEMENTS)))]
         atoms.append(Atom('A%d' % (s_i + 1), el, site, B=float(rng.uniform(0.5, 2.0))))
-    n_pts = int(rng.integers(2000, 12001))
-    lo = float(rng.uniform(5.0, 20.0))
-    step = float(rng.choice([0.01, 0.015, 0.02]))
-    tth = lo + step * np.arange(n_pts, dtype=np.float64)
-    tth = tth[tth < 160.0]
     ph = Phase('p1', grp, cell, atoms, 'tch_aniso', unique_axis=1)
     pb = Problem(tth, [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
     pb.default_values()

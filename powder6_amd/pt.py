@@ -162,6 +162,17 @@ def lib():
         L.pt_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_matrix_timing.argtypes = [ctypes.c_void_p, P_D, P_D]
+        L.pt_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, P_I, P_D]
+        L.pt_group_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.pt_group_destroy.argtypes = [ctypes.c_void_p]
+        L.pt_group_destroy.restype = None
+        L.pt_group_member.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PtPlanInfo),
+                                      ctypes.POINTER(ctypes.c_int64)]
+        L.pt_group_set_states.argtypes = [ctypes.c_void_p, ctypes.c_int, P_D, ctypes.c_int]
+        L.pt_group_costs.argtypes = [ctypes.c_void_p, P_D]
+        L.pt_group_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P_D]
+        L.pt_group_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, P_I, P_D]
+        L.pt_group_last_timing.argtypes = [ctypes.c_void_p, P_D]
         _lib = L
     return _lib
 
@@ -275,6 +286,86 @@ class Plan:
         t, k, n = c_double(), c_double(), c_int32()
         _check(lib().pt_last_timing(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(n)))
         return t.value, k.value, n.value
+
+    def tiles(self, request=0):
+        """Tile shape of the profile sum: request 0 reports, -1 retunes, 1/2/4 fixes it.
+        Returns (128-point tiles per wavefront in use, tuner times in ms for 1, 2, 4)."""
+        t, ms = c_int32(), np.zeros(3)
+        _check(lib().pt_tiles(self.h, int(request), ctypes.byref(t), _dptr(ms)))
+        return t.value, ms
+
+
+class Group:
+    """Independent PT runs of several problems sharing every launch (pt_group_*)."""
+
+    def __init__(self, problems, steps_per_sweep=0):
+        self.problems = list(problems)
+        self.cps = [CProblem(pb) for pb in self.problems]
+        arr = (PtProblem * len(self.cps))(*[cp.struct for cp in self.cps])
+        self._arr = arr
+        h = ctypes.c_void_p()
+        _check(lib().pt_group_create(ctypes.cast(arr, ctypes.c_void_p), len(self.cps), int(steps_per_sweep),
+                                     ctypes.byref(h)))
+        self.h = h
+        self.members = []
+        for m in range(len(self.cps)):
+            info, off = PtPlanInfo(), ctypes.c_int64()
+            _check(lib().pt_group_member(self.h, m, ctypes.byref(info), ctypes.byref(off)))
+            self.members.append(dict({f: getattr(info, f) for f, _ in PtPlanInfo._fields_}, offset=off.value))
+        last = self.members[-1]
+        self.state_size = last['offset'] + last['chains'] * last['stride']
+        self.total_chains = sum(m['chains'] for m in self.members)
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().pt_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def initial(self, member, vectors):
+        x = np.ascontiguousarray(np.atleast_2d(np.asarray(vectors, dtype=np.float64)))
+        if x.shape[1] != self.members[member]['nv']:
+            raise PTError('trial vectors must have %d entries' % self.members[member]['nv'])
+        _check(lib().pt_group_set_states(self.h, member, _dptr(x), x.shape[0]))
+
+    def costs(self):
+        """chi^2 of every member's chains at their current vectors, per member."""
+        out = np.zeros(self.total_chains)
+        _check(lib().pt_group_costs(self.h, _dptr(out)))
+        res, k = [], 0
+        for m in self.members:
+            res.append(out[k:k + m['chains']])
+            k += m['chains']
+        return res
+
+    def batch(self, count, bootstrap=False, snapshot=True):
+        out = np.zeros(self.state_size) if snapshot else None
+        _check(lib().pt_group_batch(self.h, int(count), 1 if bootstrap else 0, _dptr(out) if snapshot else None))
+        return self.split(out) if snapshot else None
+
+    def split(self, state):
+        return [state[m['offset']:m['offset'] + m['chains'] * m['stride']] for m in self.members]
+
+    def tiles(self, request=0):
+        t, ms = c_int32(), np.zeros(3)
+        _check(lib().pt_group_tiles(self.h, int(request), ctypes.byref(t), _dptr(ms)))
+        return t.value, ms
+
+    def timing(self):
+        t = c_double()
+        _check(lib().pt_group_last_timing(self.h, ctypes.byref(t)))
+        return t.value
 
 
 def gpu_simulate(pb, x):

@@ -135,6 +135,21 @@ def test_sweep_graph_replays_match_direct_launches(pbso4):
         assert np.array_equal(g, d)
 
 
+def test_tile_shapes_give_identical_states(pbso4):
+    """The chi^2 partials are per 512-point group whatever the tile shape, so the PT
+    trajectory does not depend on the shape the tuner picks: bit-identical states."""
+    x0 = pbso4.start_vector()
+    runs = []
+    for tiles in (1, 2, 4):
+        with pt.Plan(pbso4) as pl:
+            pl.tiles(tiles)
+            pl.initial([x0])
+            pl.batch(1, bootstrap=True)
+            runs.append(pl.batch(2))
+            assert pl.tiles()[0] == tiles
+    assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[1], runs[2])
+
+
 def test_batched_eval_chunks_and_repeats(pbso4):
     """Evaluating 2500 vectors (two passes of the slot buffer) equals evaluating them one
     pass at a time, and a repeat is bit-identical."""

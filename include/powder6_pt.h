@@ -218,6 +218,55 @@ int pt_last_timing(const pt_plan *plan, double *total_ms, double *kernel_ms, int
  * No counterpart in the reference (the WGSL path has fixed 64-wide dispatches). */
 int pt_tiles(pt_plan *plan, int request, int *tiles_per_wave, double *candidate_ms);
 
+/* ---- surrogate network and screened PT ----------------------------------------------
+ * A 64 -> 256 -> 256 -> 256 -> 1 SiLU network (BF16 products with FP32 accumulation on
+ * MFMA, FP32 master weights, Adam) that predicts log Rwp of a trial vector.  No
+ * counterpart in the reference.  With pt_screen on, every PT step of pt_batch draws
+ * `oversample` candidate moves per chain (candidate 0 is exactly the plain proposal) and
+ * predicts their log Rwp and that of the chain's current vector; a screened chain (the
+ * coldest cold_fraction of the ladder) evaluates the best candidate exactly when its
+ * predicted gain exceeds `margin`, else the plain proposal, and runs the usual Metropolis
+ * step on the exact chi^2.  Every screened chain's trial (vector, exact log Rwp) goes
+ * into a replay ring on which the network trains `train_iters` Adam steps per PT step.
+ * Until the network has trained `warmup_iters` steps the pick is the plain proposal, and
+ * with the screen off (cfg NULL, the default) pt_batch is plain PT. */
+typedef struct {
+    int oversample;      /* candidates per chain and step, 2..16                       */
+    int train_iters;     /* Adam steps per PT step                                      */
+    int batch;           /* minibatch rows, a multiple of 64 in 64..4096                */
+    int warmup_iters;    /* Adam steps before the network's picks are used             */
+    int capacity;        /* replay ring entries (>= batch and >= chains)                */
+    float lr;            /* Adam learning rate                                          */
+    uint32_t seed;       /* network initialisation and minibatch sampling               */
+    float margin;        /* predicted log Rwp gain over the current vector to take a pick */
+    float cold_fraction; /* screened share of the ladder, coldest chains first (0, 1]   */
+} pt_screen_config;
+int pt_screen(pt_plan *plan, const pt_screen_config *cfg);
+/* out[0] Adam steps taken, out[1] ring entries filled, out[2] running mean of the
+ * minibatch loss (squared error in log Rwp, masked rows counted as zero), out[3] picks of
+ * a candidate other than the plain proposal */
+int pt_screen_stats(pt_plan *plan, double *out);
+/* log Rwp predicted by the plan's network for n vectors of nv doubles */
+int pt_screen_predict(pt_plan *plan, const double *x, int n, float *pred);
+
+/* The network alone: parameters are one flat FP32 vector (W1 [64][256] row major by input,
+ * b1, W2 [256][256], b2, W3, b3, w4 [256], b4: pt_mlp_param_count() values).  Inputs are
+ * n rows of n_in <= 64 values (zero padded to 64). */
+typedef struct pt_mlp pt_mlp;
+int pt_mlp_create(uint32_t seed, pt_mlp **out);
+void pt_mlp_destroy(pt_mlp *net);
+int pt_mlp_param_count(void);
+int pt_mlp_get_params(pt_mlp *net, float *params);
+int pt_mlp_set_params(pt_mlp *net, const float *params);     /* also resets Adam's state */
+int pt_mlp_forward(pt_mlp *net, const float *x, int n, int n_in, float *y);
+/* loss = (1/n) sum mask (y - t)^2 (mask NULL: ones) and its gradient (param layout) */
+int pt_mlp_gradient(pt_mlp *net, const float *x, const float *t, const float *mask, int n, int n_in, float *loss,
+                    float *grad);
+/* `iters` Adam steps on minibatches of `batch` rows drawn from the n given pairs; *loss
+ * receives the last minibatch loss */
+int pt_mlp_train(pt_mlp *net, const float *x, const float *t, int n, int n_in, int iters, int batch, float lr,
+                 uint32_t seed, float *loss);
+
 /* ---- groups: many independent problems in the same launches ------------------------
  * The reference fits several problems side by side as independent workers
  * (js/rietveld/rv_commands.js:11-100, up to min(8, cores - 1) at once), each with its own

@@ -29,13 +29,15 @@ def run(cmd):
 def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
-    kern = os.path.join(LIBDIR, 'pt_kernels.o')
-    run([HIPCC] + DEVICE_FLAGS + ['-c', os.path.join(CSRC, 'pt_kernels.hip'), '-o', kern])
-    objs.append(kern)
-    host = os.path.join(LIBDIR, 'pt_host.o')
-    run([HIPCC, '-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', ARCH, '-Wall', '-x', 'hip',
-         '-c', os.path.join(CSRC, 'pt_host.cpp'), '-o', host])
-    objs.append(host)
+    for src in ('pt_kernels.hip', 'pt_mlp.hip'):
+        obj = os.path.join(LIBDIR, src.replace('.hip', '.o'))
+        run([HIPCC] + DEVICE_FLAGS + ['-c', os.path.join(CSRC, src), '-o', obj])
+        objs.append(obj)
+    for src in ('pt_host.cpp', 'pt_mlp_host.cpp'):
+        obj = os.path.join(LIBDIR, src.replace('.cpp', '.o'))
+        run([HIPCC, '-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', ARCH, '-Wall', '-x', 'hip',
+             '-c', os.path.join(CSRC, src), '-o', obj])
+        objs.append(obj)
     tmp = LIB + '.tmp'
     run([HIPCC, '-shared', '-fPIC', ARCH, '-o', tmp] + objs)
     os.replace(tmp, LIB)

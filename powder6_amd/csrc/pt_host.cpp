@@ -16,7 +16,9 @@
 #include <vector>
 
 #include "../../include/powder6_pt.h"
+#include "pt_common.h"
 #include "pt_launch.h"
+#include "pt_mlp_host.h"
 
 namespace {
 
@@ -28,6 +30,12 @@ int fail(int code, const std::string &msg) {
 
 const double kPi = 3.141592653589793;
 const double D2R = kPi / 180, R2D = 180 / kPi;
+
+}  // namespace
+
+int ptk_host::fail(int code, const std::string &msg) { return ::fail(code, msg); }
+
+namespace {
 
 // ---------------------------------------------------------------------------------
 // reflection enumeration: the envelope of js/pt_gpu.js:110-131 through
@@ -194,6 +202,14 @@ struct pt_plan {
     double *d_myout = nullptr;
     std::vector<char> struct_var;        // variables that may differ from x0 in the matrix form
     std::vector<double> x0, base_params; // start vector and its model parameters
+    std::vector<double> vlo_h, vhi_h, vscale_h;
+    double log_wss = 0;                  // log sum w y_obs^2 (the Rwp denominator)
+    // surrogate-screened PT
+    pt_mlp *mlp = nullptr;
+    bool screen_on = false;
+    pt_screen_config scfg{};
+    ptk::ScreenDev S{};
+    int screen_cap = 0;
     std::vector<double> tth_host;
     int n_bkg = 0;
     hipEvent_t ev_total[2] = {nullptr, nullptr};
@@ -350,6 +366,7 @@ void pt_plan_destroy(pt_plan *pl) {
         if (pl->ev_total[0]) (void)hipEventDestroy(pl->ev_total[0]);
         if (pl->ev_total[1]) (void)hipEventDestroy(pl->ev_total[1]);
         if (pl->stream) (void)hipStreamDestroy(pl->stream);
+        mlpx::destroy(pl->mlp);
     }
     delete pl;
 }
@@ -517,6 +534,15 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
 
     std::vector<double> tth(pb->tth, pb->tth + n), base(pb->param_base, pb->param_base + pb->n_params);
     pl->x0.assign(pb->x0, pb->x0 + pb->n_vars);
+    pl->vlo_h.assign(pb->lo, pb->lo + pb->n_vars);
+    pl->vhi_h.assign(pb->hi, pb->hi + pb->n_vars);
+    pl->vscale_h.assign(pb->scale, pb->scale + pb->n_vars);
+    {
+        double wss = 0;
+        for (int i = 0; i < pb->n_points; ++i)
+            if (pb->weight[i] > 0) wss += pb->weight[i] * pb->y_obs[i] * pb->y_obs[i];
+        pl->log_wss = std::log(std::max(wss, 1e-300));
+    }
     pl->base_params.resize(pb->n_params);
     for (int k = 0; k < pb->n_params; ++k) pl->base_params[k] = param_value(pb, k);
     pl->tth_host = tth;
@@ -790,10 +816,40 @@ int pt_tiles(pt_plan *pl, int request, int *tiles_per_wave, double *candidate_ms
     return 0;
 }
 
+// one screened sweep: every step trains the surrogate on the replay ring, draws K
+// candidate moves per chain, predicts their log Rwp, takes the lowest, then evaluates it
+// exactly (setup, lines, accumulate) and runs the Metropolis step (finish); the trial is
+// recorded into the ring.  Then the replica exchange.
+static int enqueue_screened_sweep(pt_plan *pl, hipEvent_t *ev) {
+    const pt_screen_config &c = pl->scfg;
+    const double *step_ptr = pl->P.state + 3 * pl->nv + PT_S_STEP;
+    for (int s = 0; s < pl->steps_per_sweep; ++s) {
+        for (int it = 0; it < c.train_iters; ++it) {
+            int rc = mlpx::train_iter(pl->mlp, pl->S.ring_x, pl->S.ring_t, pl->screen_cap, step_ptr, pl->chains, 0, c.batch,
+                                      c.seed, c.batch, pl->stream);
+            if (rc) return rc;
+        }
+        HIPCHECK(ptk::screen_candidates(pl->P, pl->S, pl->stream), "screen");
+        int rc = mlpx::infer(pl->mlp, pl->S.xs, pl->chains * (c.oversample + 1), pl->S.pred, pl->stream);
+        if (rc) return rc;
+        HIPCHECK(ptk::screen_pick(pl->S, pl->chains, pl->mlp->iter, pl->stream), "screen");
+        HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, pl->chains, 1, 0, pl->stream, pl->S.pick), "setup");
+        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
+        if (ev) HIPCHECK(hipEventRecord(ev[2 * s], pl->stream), "timing");
+        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, pl->chains, pl->tpw, pl->stream), "accumulate");
+        if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], pl->stream), "timing");
+        HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
+        HIPCHECK(ptk::screen_record(pl->P, pl->S, pl->stream), "screen");
+    }
+    HIPCHECK(ptk::launch_exchange(pl->P, pl->stream), "exchange");
+    return 0;
+}
+
 // one sweep: steps_per_sweep x (setup/propose, lines, accumulate, finish/accept), then the
 // replica exchange; the finish of each step but the last is fused with the next setup.
 // Accumulate launches are bracketed by events when ev != null.
 static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
+    if (pl->screen_on) return enqueue_screened_sweep(pl, ev);
     HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, pl->chains, 1, 0, pl->stream), "setup");
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
         HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
@@ -806,6 +862,106 @@ static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
             HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
     }
     HIPCHECK(ptk::launch_exchange(pl->P, pl->stream), "exchange");
+    return 0;
+}
+
+int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    drop_graph(pl->gexec, pl->graph);
+    if (!cfg) {
+        pl->screen_on = false;
+        return 0;
+    }
+    const pt_screen_config c = *cfg;
+    if (pl->nv > ptk::MLP_IN) return fail(PT_ERR_ARG, "PT surrogate screen supports at most 64 variables.");
+    if (c.oversample < 2 || c.oversample > 16) return fail(PT_ERR_ARG, "PT surrogate oversampling must be 2..16.");
+    if (c.batch < 64 || c.batch > 4096 || c.batch % 64) return fail(PT_ERR_ARG, "PT surrogate batch must be a multiple of 64 in 64..4096.");
+    if (c.train_iters < 0 || c.warmup_iters < 0 || !(c.lr > 0.0f) || !(c.margin >= 0.0f) || !(c.cold_fraction > 0.0f) ||
+        c.cold_fraction > 1.0f)
+        return fail(PT_ERR_ARG, "invalid surrogate training settings.");
+    if (c.capacity < c.batch || c.capacity < pl->chains) return fail(PT_ERR_ARG, "PT surrogate ring must hold a batch and one step of every chain.");
+    int rc = 0;
+    if (!pl->mlp && (rc = mlpx::create(c.seed ^ 0x6a09e667u, &pl->mlp))) return rc;
+    const int K = c.oversample;
+    if ((rc = mlpx::ensure_rows(pl->mlp, std::max(c.batch, pl->chains * (K + 1))))) return rc;
+    pl->mlp->lr = c.lr;
+    if (c.capacity != pl->screen_cap || K != pl->S.K) {
+        ptk::ScreenDev &S = pl->S;
+        if ((rc = dalloc(pl, &S.ring_x, (size_t)c.capacity * ptk::MLP_IN)) || (rc = dalloc(pl, &S.ring_t, (size_t)c.capacity)) ||
+            (rc = dalloc(pl, &S.cand, (size_t)pl->chains * K)) || (rc = dalloc(pl, &S.xs, (size_t)pl->chains * (K + 1) * ptk::MLP_IN)) ||
+            (rc = dalloc(pl, &S.pred, (size_t)pl->chains * (K + 1))) || (rc = dalloc(pl, &S.pick, (size_t)pl->chains)))
+            return rc;
+        if (!S.picked && (rc = dalloc(pl, &S.picked, 1))) return rc;
+        if (!S.center) {
+            std::vector<double> ctr(pl->nv), ih(pl->nv);
+            for (int j = 0; j < pl->nv; ++j) {
+                const double lo = pl->vlo_h[j], hi = pl->vhi_h[j];
+                if (std::isfinite(lo) && std::isfinite(hi) && hi > lo) {
+                    ctr[j] = 0.5 * (lo + hi);
+                    ih[j] = 2.0 / (hi - lo);
+                } else {
+                    const double sc = pl->vscale_h[j] > 0 ? pl->vscale_h[j] : 1.0;
+                    ctr[j] = pl->x0[j];
+                    ih[j] = 1.0 / (8.0 * sc);
+                }
+            }
+            double *dc = nullptr, *dh = nullptr;
+            if ((rc = upload(pl, &dc, ctr)) || (rc = upload(pl, &dh, ih))) return rc;
+            S.center = dc;
+            S.inv_half = dh;
+        }
+        HIPCHECK(hipMemset(S.ring_x, 0, sizeof(uint16_t) * (size_t)c.capacity * ptk::MLP_IN), "screen setup");
+        HIPCHECK(hipMemset(S.ring_t, 0xff, sizeof(float) * (size_t)c.capacity), "screen setup");   // NaN: empty
+        pl->screen_cap = c.capacity;
+        S.K = K;
+        S.cap = c.capacity;
+    }
+    HIPCHECK(hipMemset(pl->S.picked, 0, sizeof(unsigned)), "screen setup");
+    pl->S.warmup_iters = c.warmup_iters;
+    pl->S.log_wss = pl->log_wss;
+    pl->S.margin = c.margin;
+    pl->S.cold_chains = std::max(1, (int)std::lround(c.cold_fraction * pl->chains));
+    pl->scfg = c;
+    pl->screen_on = true;
+    return 0;
+}
+
+int pt_screen_stats(pt_plan *pl, double *out) {
+    if (!pl || !out) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    for (int i = 0; i < 4; ++i) out[i] = 0;
+    if (!pl->mlp) return 0;
+    (void)hipSetDevice(pl->device);
+    int it = 0;
+    float loss[2] = {0, 0};
+    unsigned picked = 0;
+    double step = 0;
+    HIPCHECK(hipStreamSynchronize(pl->stream), "screen statistics");
+    HIPCHECK(hipMemcpy(&it, pl->mlp->iter, sizeof(int), hipMemcpyDeviceToHost), "screen statistics");
+    HIPCHECK(hipMemcpy(loss, pl->mlp->loss, 2 * sizeof(float), hipMemcpyDeviceToHost), "screen statistics");
+    if (pl->S.picked) HIPCHECK(hipMemcpy(&picked, pl->S.picked, sizeof(unsigned), hipMemcpyDeviceToHost), "screen statistics");
+    HIPCHECK(hipMemcpy(&step, pl->P.state + 3 * pl->nv + PT_S_STEP, sizeof(double), hipMemcpyDeviceToHost), "screen statistics");
+    out[0] = it;
+    out[1] = std::min((double)pl->screen_cap, std::max(0.0, step) * pl->chains);
+    out[2] = loss[1];
+    out[3] = picked;
+    return 0;
+}
+
+int pt_screen_predict(pt_plan *pl, const double *x, int n, float *pred) {
+    if (!pl || !x || !pred || n < 1) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    if (!pl->mlp || !pl->S.center) return fail(PT_ERR_STATE, "PT surrogate screen is not set up on this plan.");
+    (void)hipSetDevice(pl->device);
+    int rc = ensure_trials(pl, n);
+    if (rc || (rc = mlpx::ensure_rows(pl->mlp, n))) return rc;
+    HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * n * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
+    HIPCHECK(ptk::screen_inputs(pl->d_x, n, pl->nv, pl->S, pl->mlp->x16, pl->stream), "screen");
+    if ((rc = mlpx::infer(pl->mlp, pl->mlp->x16, n, pl->mlp->y, pl->stream))) return rc;
+    HIPCHECK(hipMemcpyAsync(pred, pl->mlp->y, sizeof(float) * n, hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "screen prediction");
     return 0;
 }
 

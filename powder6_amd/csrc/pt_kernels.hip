@@ -493,7 +493,7 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Share
 // uniformly, x_j + z scale_j sqrt(T_c) reflected into its box, log u for the Metropolis
 // test.  Random numbers are keyed on (seed, chain, the chain's step counter).
 __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
-                                  Shared &sh, double *prop) {
+                                  Shared &sh, double *prop, const double2 *pick = nullptr) {
     const int nv = P.nv;
     bool boot = false;
     double *st = nullptr;
@@ -506,12 +506,20 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
         if (threadIdx.x == 0 && !boot) {
             const unsigned long long ctr = (unsigned long long)step;
             const double temp = pow(P.max_temp, (double)t / (double)(P.chains > 1 ? P.chains - 1 : 1));
-            double u0 = uniform01(P.seed, t, ctr, 0), u1 = uniform01(P.seed, t, ctr, 1);
-            double u2 = uniform01(P.seed, t, ctr, 2), u3 = uniform01(P.seed, t, ctr, 3);
-            int j = (int)floor(u0 * nv);
-            if (j > nv - 1) j = nv - 1;
-            double z = sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-            prop[j] = reflect_box(prop[j] + z * P.vscale[j] * sqrt(temp), P.vlo[j], P.vhi[j]);
+            const double u3 = uniform01(P.seed, t, ctr, 3);
+            if (pick) {
+                // the move chosen by the surrogate screen among candidates drawn from
+                // the same stream (candidate 0 is the move below)
+                const double2 mv = pick[t];
+                prop[(int)mv.x] = mv.y;
+            } else {
+                double u0 = uniform01(P.seed, t, ctr, 0), u1 = uniform01(P.seed, t, ctr, 1);
+                double u2 = uniform01(P.seed, t, ctr, 2);
+                int j = (int)floor(u0 * nv);
+                if (j > nv - 1) j = nv - 1;
+                double z = sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                prop[j] = reflect_box(prop[j] + z * P.vscale[j] * sqrt(temp), P.vlo[j], P.vhi[j]);
+            }
             st[3 * nv + 8] = log(u3);
         }
         __syncthreads();
@@ -537,10 +545,10 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
 }
 
 __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__restrict__ X, TrialImage *img,
-                                                    int pt_mode, int bootstrap) {
+                                                    int pt_mode, int bootstrap, const double2 *__restrict__ pick) {
     __shared__ Shared sh;
     __shared__ double prop[PTK_MAX_VARS];
-    setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop);
+    setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
 }
 
 // line records of reflections [kb * blockDim.x, ...) of trial t
@@ -1217,8 +1225,8 @@ hipError_t set_lds_limits(size_t bytes) {
 }
 
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap);
+                              hipStream_t s, const double2 *pick) {
+    hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick);
     return hipGetLastError();
 }
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s) {

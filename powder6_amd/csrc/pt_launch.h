@@ -9,8 +9,10 @@ size_t plain_scratch_bytes();
 size_t trial_image_bytes();
 int accum_points_per_group(int tpw);   // points per accumulate workgroup at tpw tiles per wavefront
 hipError_t set_lds_limits(size_t bytes);
+// pick (PT mode): the move of each chain chosen by the surrogate screen, else the
+// setup draws the plain proposal itself
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
-                              hipStream_t s);
+                              hipStream_t s, const double2 *pick = nullptr);
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s);
 // partial chi^2 per 512-point group: ngroups = ceil(np / 512) per trial, whatever tpw
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,

@@ -142,6 +142,22 @@ class Problem:
         self.terms.append((k, j, 1.0))
         return j
 
+    def link(self, label, names, lo=-math.inf, hi=math.inf, scale=None):
+        """One variable driving several model parameters with coefficient 1 (the affine
+        links of js/atomic_constraints.js), started at the mean of their base values."""
+        ks = [self.index[n] for n in names]
+        v = float(np.mean([self.base[k] for k in ks]))
+        j = len(self.vars)
+        for k in ks:
+            self.base[k] = 0.0
+            self.terms.append((k, j, 1.0))
+        self.vars.append(label)
+        self.x0.append(min(hi, max(lo, v)))
+        self.lo.append(lo)
+        self.hi.append(hi)
+        self.scale.append(pt_scale(names[0], v) if scale is None else scale)
+        return j
+
     @property
     def nv(self):
         return len(self.vars)
@@ -401,19 +417,23 @@ MOLECULE_ATOMS = [  # 17 non-H atoms, general positions (fractional, seeded layo
 ]
 
 
-def molecular_p21c(simulate=None, seed=20260404, chains=512, steps_per_sweep=0, sweeps=10):
+def molecular_p21c(simulate=None, seed=20260404, chains=512, steps_per_sweep=0, sweeps=10, shared_b=False):
     """P2_1/c (unique axis b, -P 2ybc) a=10.4 b=18.8 c=6.8 beta=106; 17 C/N/S atoms in
     general positions (a seeded random molecule-like cluster); 2theta 5-145 step 0.02
-    (7001 points), Cu Ka1/Ka2; refined: 51 coordinates, 17 B, scale (69 variables)."""
+    (7001 points), Cu Ka1/Ka2; refined: 51 coordinates, scale, and either 17 atomic B
+    (69 variables) or, with shared_b, one Biso linked to all 17 atoms (53 variables: the
+    input width of the 64-wide surrogate network)."""
     rng = np.random.default_rng(seed)
     r, t = S.group_ops(14)
     cell = [10.4, 18.8, 6.8, 90, 106.0, 90]
     centre = np.array([0.25, 0.12, 0.3])
     atoms = []
+    b_common = float(rng.uniform(2.0, 4.0))
     for (lab, el) in MOLECULE_ATOMS:
         xyz = centre + rng.uniform(-0.18, 0.18, size=3) * np.array([1.0, 0.55, 1.4])
         site = S.site_symmetry(r, t, list(xyz), cell)
-        atoms.append(Atom(lab, el, site, B=float(rng.uniform(2.0, 4.0))))
+        b = float(rng.uniform(2.0, 4.0))
+        atoms.append(Atom(lab, el, site, B=b_common if shared_b else b))
     ph = Phase('p1', 14, cell, atoms, 'tch_aniso', unique_axis=1)
     pb = Problem(_grid(5.0, 145.0, 0.02), [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
     pb.default_values()
@@ -426,8 +446,11 @@ def molecular_p21c(simulate=None, seed=20260404, chains=512, steps_per_sweep=0, 
             n = 'p1.%s.%s' % (at.label, 'xyz'[f])
             v = pb.base[pb.index[n]]
             pb.refine(n, v - 0.15, v + 0.15)
-    for at in atoms:
-        pb.refine('p1.%s.B' % at.label, -2, 30)
+    if shared_b:
+        pb.link('p1.Biso', ['p1.%s.B' % at.label for at in atoms], -2, 30)
+    else:
+        for at in atoms:
+            pb.refine('p1.%s.B' % at.label, -2, 30)
     pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, sweeps, steps_per_sweep, 100.0, 77
     x_true = pb.start_vector()
     if simulate is not None:

@@ -163,6 +163,20 @@ def lib():
         L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_matrix_timing.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, P_I, P_D]
+        P_F = ctypes.POINTER(ctypes.c_float)
+        L.pt_screen.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.pt_screen_stats.argtypes = [ctypes.c_void_p, P_D]
+        L.pt_screen_predict.argtypes = [ctypes.c_void_p, P_D, ctypes.c_int, P_F]
+        L.pt_mlp_create.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+        L.pt_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.pt_mlp_destroy.restype = None
+        L.pt_mlp_param_count.argtypes = []
+        L.pt_mlp_get_params.argtypes = [ctypes.c_void_p, P_F]
+        L.pt_mlp_set_params.argtypes = [ctypes.c_void_p, P_F]
+        L.pt_mlp_forward.argtypes = [ctypes.c_void_p, P_F, ctypes.c_int, ctypes.c_int, P_F]
+        L.pt_mlp_gradient.argtypes = [ctypes.c_void_p, P_F, P_F, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P_F, P_F]
+        L.pt_mlp_train.argtypes = [ctypes.c_void_p, P_F, P_F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, ctypes.c_uint32, P_F]
         L.pt_group_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.pt_group_destroy.argtypes = [ctypes.c_void_p]
         L.pt_group_destroy.restype = None
@@ -287,12 +301,116 @@ class Plan:
         _check(lib().pt_last_timing(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(n)))
         return t.value, k.value, n.value
 
+    def screen(self, **cfg):
+        """Surrogate-screened PT (pt_screen) with SCREEN_DEFAULTS overridden by cfg;
+        screen(off=True) returns to plain PT."""
+        if cfg.pop('off', False):
+            _check(lib().pt_screen(self.h, None))
+            return
+        c = dict(SCREEN_DEFAULTS, **cfg)
+        self._screen_cfg = PtScreenConfig(**c)
+        _check(lib().pt_screen(self.h, ctypes.byref(self._screen_cfg)))
+
+    def screen_predict(self, x):
+        """log Rwp of trial vectors as the plan's surrogate network predicts it."""
+        x = self._vectors(x)
+        out = np.zeros(x.shape[0], dtype=np.float32)
+        _check(lib().pt_screen_predict(self.h, _dptr(x), x.shape[0], _fptr(out)))
+        return out
+
+    def screen_stats(self):
+        out = np.zeros(4)
+        _check(lib().pt_screen_stats(self.h, _dptr(out)))
+        return dict(iterations=int(out[0]), ring=int(out[1]), loss=float(out[2]), picked=int(out[3]))
+
     def tiles(self, request=0):
         """Tile shape of the profile sum: request 0 reports, -1 retunes, 1/2/4 fixes it.
         Returns (128-point tiles per wavefront in use, tuner times in ms for 1, 2, 4)."""
         t, ms = c_int32(), np.zeros(3)
         _check(lib().pt_tiles(self.h, int(request), ctypes.byref(t), _dptr(ms)))
         return t.value, ms
+
+
+class PtScreenConfig(ctypes.Structure):
+    _fields_ = [('oversample', ctypes.c_int), ('train_iters', ctypes.c_int), ('batch', ctypes.c_int),
+                ('warmup_iters', ctypes.c_int), ('capacity', ctypes.c_int), ('lr', ctypes.c_float),
+                ('seed', ctypes.c_uint32), ('margin', ctypes.c_float), ('cold_fraction', ctypes.c_float)]
+
+
+SCREEN_DEFAULTS = dict(oversample=4, train_iters=1, batch=512, warmup_iters=200, capacity=65536, lr=1e-3, seed=7,
+                       margin=0.0, cold_fraction=1.0)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The surrogate network alone (pt_mlp_*): 64 -> 256 -> 256 -> 256 -> 1 SiLU,
+    BF16 products, FP32 master weights."""
+
+    INPUTS, HIDDEN = 64, 256
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().pt_mlp_create(seed, ctypes.byref(h)))
+        self.h = h
+        self.n_params = lib().pt_mlp_param_count()
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().pt_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().pt_mlp_get_params(self.h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().pt_mlp_set_params(self.h, _fptr(p)))
+
+    @staticmethod
+    def _rows(x):
+        x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float32)))
+        return x, x.shape[0], x.shape[1]
+
+    def forward(self, x):
+        x, n, k = self._rows(x)
+        y = np.zeros(n, dtype=np.float32)
+        _check(lib().pt_mlp_forward(self.h, _fptr(x), n, k, _fptr(y)))
+        return y
+
+    def gradient(self, x, t, mask=None):
+        x, n, k = self._rows(x)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().pt_mlp_gradient(self.h, _fptr(x), _fptr(t), None if m is None else m.ctypes.data_as(ctypes.c_void_p),
+                                     n, k, ctypes.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train(self, x, t, iters, batch=256, lr=1e-3, seed=3):
+        x, n, k = self._rows(x)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().pt_mlp_train(self.h, _fptr(x), _fptr(t), n, k, int(iters), int(batch), float(lr), seed,
+                                  ctypes.byref(loss)))
+        return loss.value
 
 
 class Group:

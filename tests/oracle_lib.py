@@ -23,7 +23,7 @@ def _stale():
     if not os.path.exists(LIB):
         return True
     t = os.path.getmtime(LIB)
-    for f in ('p6_oracle.c', 'p6_oracle.h'):
+    for f in ('p6_oracle.c', 'p6_oracle.h', 'mlp_oracle.c'):
         if os.path.getmtime(os.path.join(ORACLE_DIR, f)) > t:
             return True
     return False
@@ -50,6 +50,11 @@ def lib():
         L.or_cosets.argtypes = [ctypes.c_int, P_I, P_D, P_I, ctypes.POINTER(ctypes.c_int)]
         L.or_reflection_dmin.restype = ctypes.c_double
         L.or_reflection_dmin.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.om_param_count.restype = ctypes.c_long
+        L.om_param_count.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.om_forward.argtypes = [P_D, ctypes.c_int, ctypes.c_int, P_D, ctypes.c_int, P_D]
+        L.om_loss_grad.restype = ctypes.c_double
+        L.om_loss_grad.argtypes = [P_D, ctypes.c_int, ctypes.c_int, P_D, P_D, ctypes.c_void_p, ctypes.c_int, P_D]
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.c_void_p]
         L.or_destroy.argtypes = [ctypes.c_void_p]
@@ -176,3 +181,27 @@ def oracle_simulate(pb, x):
         return m.pattern(x)
     finally:
         m.close()
+
+
+# ---- surrogate network (oracle/mlp_oracle.c) ----------------------------------------
+def mlp_param_count(n_in, h):
+    return int(lib().om_param_count(n_in, h))
+
+
+def mlp_forward(params, n_in, h, x):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, n_in)
+    y = np.zeros(x.shape[0])
+    lib().om_forward(p.ctypes.data_as(P_D), n_in, h, x.ctypes.data_as(P_D), x.shape[0], y.ctypes.data_as(P_D))
+    return y
+
+
+def mlp_loss_grad(params, n_in, h, x, t, mask=None):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, n_in)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+    g = np.zeros(p.size)
+    loss = lib().om_loss_grad(p.ctypes.data_as(P_D), n_in, h, x.ctypes.data_as(P_D), t.ctypes.data_as(P_D),
+                              None if m is None else m.ctypes.data_as(P_D), x.shape[0], g.ctypes.data_as(P_D))
+    return loss, g

@@ -119,6 +119,52 @@ def uneven_batch(pt, PR, sweeps=2):
             'form': 'one launch per stage over all 48 PT runs, one graph per sweep'}, pbs, costs
 
 
+MLP_FLOP_FWD = 2 * (64 * 256 + 2 * 256 * 256 + 256)   # one row through 64-256-256-256-1
+PEAK_BF16_MFMA = 2.5e15
+
+
+def learned_screening(pt, PR, sweeps=2):
+    """The P2_1/c molecular problem (17 atoms, 53 variables with one Biso), 512 replicas,
+    7001 points: plain PT against surrogate-screened PT (4 candidates per chain and step,
+    one Adam step of 512 rows per PT step), and the network's training throughput."""
+    pb = PR.molecular_p21c(simulate=pt.gpu_simulate, chains=512, shared_b=True)
+    x0 = pb.start_vector()
+    out = {'workload': 'P2_1/c, 17 C/N/S atoms, 53 variables, 7001 pts Ka1/Ka2, 512 replicas',
+           'network': '64-256-256-256-1 SiLU, BF16 MFMA, FP32 accumulation, Adam'}
+    for label, screen in (('plain', None), ('screened', dict(oversample=4, train_iters=1, batch=512,
+                                                             warmup_iters=100, margin=0.002, cold_fraction=0.5))):
+        with pt.Plan(pb) as pl:
+            if screen:
+                pl.screen(**screen)
+            pl.initial([x0])
+            pl.batch(1, bootstrap=True)
+            pl.sweeps(2)                                     # warm: graph capture, network warm-up
+            t = time.perf_counter()
+            pl.sweeps(sweeps)
+            dt = time.perf_counter() - t
+            spw = pl.info['steps_per_sweep']
+            out[label] = {'value': pb.chains * spw * sweeps / dt, 'unit': 'exact trial patterns/s',
+                          'ms_per_sweep': 1e3 * dt / sweeps}
+            if screen:
+                out[label]['proposals_screened_per_s'] = 4 * pb.chains * spw * sweeps / dt
+                out[label]['stats'] = pl.screen_stats()
+    out['screen_overhead'] = out['screened']['ms_per_sweep'] / out['plain']['ms_per_sweep'] - 1.0
+    # training throughput of the network alone (minibatches of 512 rows)
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-1, 1, size=(65536, 53)).astype(np.float32)
+    T = rng.normal(size=65536).astype(np.float32)
+    with pt.Mlp(seed=2) as net:
+        net.train(X, T, iters=20, batch=512)
+        t = time.perf_counter()
+        net.train(X, T, iters=400, batch=512)
+        dt = time.perf_counter() - t
+    flop = 3 * MLP_FLOP_FWD * 512 * 400
+    out['training'] = {'rows_per_s': 512 * 400 / dt, 'tflops': flop / dt / 1e12,
+                       'frac_of_bf16_peak': flop / dt / PEAK_BF16_MFMA, 'batch': 512,
+                       'note': 'wall time of 400 Adam steps (13 launches each) incl. launch gaps'}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -126,6 +172,12 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--plain-trials', type=int, default=512)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--main-only', action='store_true',
+                    help='only the chosen PT form (for counter passes under a profiler)')
+    ap.add_argument('--tiles', type=int, default=0, choices=(0, 1, 2, 4),
+                    help='accumulate tile shape (128-point tiles per wavefront); 0 lets the autotuner pick')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='also run a plain 1 GiB device copy (HBM counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one MI355X: this build runs on a single GPU (replica sharding across '
@@ -140,6 +192,8 @@ def main():
     with pt.Plan(pb) as plan:
         chains, spw = plan.chains, plan.info['steps_per_sweep']
         work = plan.workload(x0)
+        if args.tiles:
+            plan.tiles(args.tiles)
         plan.initial([x0])
         plan.batch(1, bootstrap=True)
         for _ in range(args.warmup):
@@ -172,8 +226,17 @@ def main():
         gb = plan.eval(XB, pt.FORM_BATCHED)
         batched = len(XB) / (time.perf_counter() - t)
 
+    copy_gbps = None
+    if args.copy_probe:
+        copy_gbps = pt.hbm_copy_probe(1 << 30, 3)       # 1 + 3 launches of k_copy_probe, 1 GiB each
+        print('copy probe: k_copy_probe 1 GiB, %.0f GB/s' % copy_gbps, file=sys.stderr)
+    if args.main_only:
+        print(json.dumps({'metric': METRIC, 'value': value, 'unit': 'trial patterns/s', 'main_only': True,
+                          'plain': plain, 'batched': batched, 'hbm_copy_gbps': copy_gbps}))
+        return
     mat, mpb, mX, mchi = matrix_form(pt, PR)
     uneven, upbs, ucosts = uneven_batch(pt, PR)
+    learned = learned_screening(pt, PR)
 
     S = state.reshape(chains, -1)
     nv = pb.nv
@@ -184,8 +247,19 @@ def main():
     f_trial = flop_per_trial(work)
     launch_s = kernel_ms / max(1, launches) / 1e3
     achieved = f_trial * chains / launch_s           # one launch = one PT step of every chain
+    # HBM bytes per launch of the tuned tile shape, from the committed FETCH_SIZE / WRITE_SIZE
+    # passes (scaled by the k_copy_probe calibration)
+    traffic, traffic_src = None, None
+    tfile = os.path.join(ROOT, 'profiles', 'k_stage_accum_traffic.json')
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        ent = tr.get('per_tiles_per_wave', {}).get(str(tiles))
+        if ent:
+            traffic, traffic_src = ent['bytes_per_launch'], 'profiles/k_stage_accum_traffic.json (tiles per wave %d)' % tiles
     roof = {'bound': 'valu', 'kernel': 'k_stage_accum', 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
-            'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': None,
+            'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': traffic, 'traffic_source': traffic_src,
+            'hbm_gbps_at_traffic': (traffic / launch_s / 1e9) if traffic else None,
             'flop_per_trial': f_trial, 'avg_launch_ms': launch_s * 1e3, 'workload': work}
     cpu = None
     if not args.no_cpu:
@@ -215,6 +289,7 @@ def main():
         'batched_eval': {'value': batched, 'unit': 'trial patterns/s', 'form': 'staged pipeline, 16384 independent vectors'},
         'matrix_form': mat,
         'uneven_batch': uneven,
+        'learned_screening': learned,
         'chosen_over_plain': value / plain,
         'roofline': roof,
         'cpu_baseline': cpu,

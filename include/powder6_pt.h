@@ -17,6 +17,7 @@
 #ifndef POWDER6_PT_H
 #define POWDER6_PT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -266,6 +267,11 @@ int pt_mlp_gradient(pt_mlp *net, const float *x, const float *t, const float *ma
  * receives the last minibatch loss */
 int pt_mlp_train(pt_mlp *net, const float *x, const float *t, int n, int n_in, int iters, int batch, float lr,
                  uint32_t seed, float *loss);
+
+/* HBM calibration: `repeats` launches of a plain streaming copy kernel (k_copy_probe) of
+ * `bytes` bytes, after one untimed launch; *gbps = bytes read + written per second.  Used
+ * to scale the FETCH_SIZE / WRITE_SIZE counters.  No counterpart in the reference. */
+int pt_hbm_copy_probe(size_t bytes, int repeats, double *gbps);
 
 /* ---- groups: many independent problems in the same launches ------------------------
  * The reference fits several problems side by side as independent workers

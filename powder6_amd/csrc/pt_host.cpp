@@ -579,7 +579,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     size_t ybytes = (size_t)n * sizeof(float);
     const size_t rec_bytes = sizeof(ptk::LineRec) + sizeof(int4);
     const size_t lds_max = 160 * 1024;
-    if (stat + ybytes + 64 * rec_bytes > lds_max) { pt_plan_destroy(pl); return fail(PT_ERR_BUDGET, "PT pattern has too many points for one workgroup (LDS)."); }
+    if (stat + ybytes + 64 * rec_bytes > lds_max || n > PTK_MAX_GROUPS * 512) { pt_plan_destroy(pl); return fail(PT_ERR_BUDGET, "PT pattern has too many points for one workgroup (LDS)."); }
     size_t target = 80 * 1024;
     long want = 2L * pl->nr;
     long room = (long)((target > stat + ybytes ? target - stat - ybytes : 0) / rec_bytes);
@@ -1287,39 +1287,47 @@ static inline int tiles_of(int np, int tpw) {
     return (np + ptk::accum_points_per_group(tpw) - 1) / ptk::accum_points_per_group(tpw);
 }
 
-// accumulate work table for one tile shape: the tiles of each (member, chain) trial are
-// kept on one XCD (table slot i runs on XCD i mod 8), trials are placed heaviest first on
-// the least loaded XCD, and each XCD's list starts with its heaviest trials.  Unequal
-// lists are padded with empty slots (member -1).
-static std::vector<int4> accum_table(const pt_group *g, int tpw) {
-    struct Trial { double cost; int m, c, n; };
-    std::vector<Trial> trials;
+// XCD placement of the group's trials: heaviest first (estimated lines x points of the
+// banded sum) onto the least-loaded XCD.  Returns the trials in placement order with
+// their XCD.
+struct GroupTrial { double cost; int m, c, x; };
+static std::vector<GroupTrial> place_trials(const pt_group *g) {
+    std::vector<GroupTrial> trials;
     for (int m = 0; m < (int)g->members.size(); ++m) {
         const pt_plan *pl = g->members[m];
-        const int n = tiles_of(pl->np, tpw);
         const double nl = 2.0 * pl->nr;
-        // per tile: scan of every span + evaluation of the lines that meet the tile
-        const double tile = nl + nl * (ptk::accum_points_per_group(tpw) + 100.0) / std::max(1, pl->np) * 4.0 * tpw;
-        for (int c = 0; c < pl->chains; ++c) trials.push_back({tile * n, m, c, n});
+        const double cost = nl * tiles_of(pl->np, 2) + nl * (pl->np + 100.0 * tiles_of(pl->np, 2)) / std::max(1, pl->np) * 8.0;
+        for (int c = 0; c < pl->chains; ++c) trials.push_back({cost, m, c, 0});
     }
-    std::stable_sort(trials.begin(), trials.end(), [](const Trial &a, const Trial &b) { return a.cost > b.cost; });
-    const int kX = 8;
-    std::vector<std::vector<int4>> lists(kX);
-    double load[kX] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (const Trial &t : trials) {
+    std::stable_sort(trials.begin(), trials.end(), [](const GroupTrial &a, const GroupTrial &b) { return a.cost > b.cost; });
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (GroupTrial &t : trials) {
         int x = 0;
-        for (int k = 1; k < kX; ++k)
+        for (int k = 1; k < 8; ++k)
             if (load[k] < load[x]) x = k;
         load[x] += t.cost;
-        for (int s = 0; s < t.n; ++s) lists[x].push_back(make_int4(t.m, t.c, s, 0));
+        t.x = x;
     }
+    return trials;
+}
+
+// work table with `units(m)` units per trial: every unit of a trial in slots of its XCD
+// (table slot i runs on XCD i mod 8), each XCD's list in placement order (heaviest trials
+// first); unequal lists are padded with empty slots (member -1)
+extern "C++" {
+template <class F>
+static std::vector<int4> xcd_table(const std::vector<GroupTrial> &trials, F units) {
+    std::vector<std::vector<int4>> lists(8);
+    for (const GroupTrial &t : trials)
+        for (int u = 0; u < units(t.m); ++u) lists[t.x].push_back(make_int4(t.m, t.c, u, 0));
     size_t len = 0;
     for (auto &l : lists) len = std::max(len, l.size());
-    std::vector<int4> tab(len * kX, make_int4(-1, 0, 0, 0));
-    for (int x = 0; x < kX; ++x)
-        for (size_t j = 0; j < lists[x].size(); ++j) tab[j * kX + x] = lists[x][j];
+    std::vector<int4> tab(len * 8, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < 8; ++x)
+        for (size_t j = 0; j < lists[x].size(); ++j) tab[j * 8 + x] = lists[x][j];
     return tab;
 }
+}  // extern "C++"
 
 int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, pt_group **out) {
     if (!out) return fail(PT_ERR_ARG, "output handle is NULL");
@@ -1371,10 +1379,9 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
         g->chain_base.push_back((int)chain_tab.size());
         for (int c = 0; c < pl->chains; ++c) chain_tab.push_back(make_int2(m, c));
         for (int q = 0; q < (pl->chains + 1) / 2; ++q) pair_tab.push_back(make_int2(m, q));
-        const int kb = (pl->nr + 127) / 128;
-        for (int c = 0; c < pl->chains; ++c)
-            for (int b = 0; b < kb; ++b) lines_tab.push_back(make_int4(m, c, b, 0));
     }
+    const std::vector<GroupTrial> placed = place_trials(g);
+    lines_tab = xcd_table(placed, [&](int m) { return (g->members[m]->nr + 127) / 128; });
     g->total_chains = (int)chain_tab.size();
     g->total_pairs = (int)pair_tab.size();
     g->n_lines = (int)lines_tab.size();
@@ -1382,7 +1389,7 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
         (rc = upload(g, &g->d_pair_tab, pair_tab)) || (rc = upload(g, &g->d_lines_tab, lines_tab)))
         return bail(rc);
     for (int v = 0; v < 3; ++v) {
-        std::vector<int4> tab = accum_table(g, 1 << v);
+        std::vector<int4> tab = xcd_table(placed, [&](int m) { return tiles_of(g->members[m]->np, 1 << v); });
         g->n_accum[v] = (int)tab.size();
         if ((rc = upload(g, &g->d_accum_tab[v], tab))) return bail(rc);
     }
@@ -1535,6 +1542,35 @@ int pt_group_last_timing(const pt_group *g, double *total_ms) {
     NEED_GROUP(g);
     if (total_ms) *total_ms = g->last_ms;
     return 0;
+}
+
+int pt_hbm_copy_probe(size_t bytes, int repeats, double *gbps) {
+    if (bytes < 16 || repeats < 1) return fail(PT_ERR_ARG, "copy probe needs bytes >= 16 and repeats >= 1");
+    int dev = -1, rc = device_check(&dev);
+    if (rc) return rc;
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    rc = 0;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e[0]) != hipSuccess || hipEventCreate(&e[1]) != hipSuccess || hipMemsetAsync(a, 0, bytes, st) != hipSuccess)
+        rc = fail(PT_ERR_DEVICE, "copy probe buffers could not be set up.");
+    float ms = 0;
+    if (!rc && (ptk::launch_copy_probe(a, b, bytes, st) != hipSuccess || hipEventRecord(e[0], st) != hipSuccess))
+        rc = fail(PT_ERR_DEVICE, "copy probe launch failed.");
+    for (int r = 0; r < repeats && !rc; ++r)
+        if (ptk::launch_copy_probe(a, b, bytes, st) != hipSuccess) rc = fail(PT_ERR_DEVICE, "copy probe launch failed.");
+    if (!rc && (hipEventRecord(e[1], st) != hipSuccess || hipEventSynchronize(e[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, e[0], e[1]) != hipSuccess))
+        rc = fail(PT_ERR_DEVICE, "copy probe timing failed.");
+    if (!rc && gbps) *gbps = 2.0 * (double)bytes * repeats / (ms * 1e-3) / 1e9;
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto ev : e) if (ev) (void)hipEventDestroy(ev);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 }  // extern "C"

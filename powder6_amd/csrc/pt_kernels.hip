@@ -457,6 +457,9 @@ struct TrialImage {
     float bkg[PTK_MAX_BKG_F];
     int phase_ok[PTK_MAX_PHASES];
     int cell_ok, prof_ok, go, boot;
+    // per 512-point group: the first and last line index whose window meets the group,
+    // reset by the setup and gathered by the lines stage (atomic min / max)
+    int grange[PTK_MAX_GROUPS][2];
 };
 
 __device__ inline int total_atoms(const DevPlan &P) {
@@ -474,6 +477,7 @@ __device__ inline void image_store(const DevPlan &P, const Shared &sh, TrialImag
     for (int i = tid; i < PTK_MAX_BKG_F; i += nt) img->bkg[i] = sh.bkg[i];
     if (tid < P.n_phases) img->phase_ok[tid] = sh.phase_ok[tid];
     if (tid == 0) { img->cell_ok = sh.cell_ok; img->prof_ok = sh.prof_ok; img->go = sh.go; img->boot = sh.accept; }
+    for (int g = tid; g < (P.np + 511) >> 9; g += nt) { img->grange[g][0] = 0x7fffffff; img->grange[g][1] = -1; }
 }
 
 __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Shared &sh) {
@@ -551,9 +555,10 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
     setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
 }
 
-// line records of reflections [kb * blockDim.x, ...) of trial t
-__device__ __forceinline__ void lines_body(const DevPlan &P, const TrialImage *__restrict__ img, LineRec *rec, int4 *span, int t,
-                                  int kb, Shared &sh) {
+// line records of reflections [kb * blockDim.x, ...) of trial t; every line with a window
+// widens the line range of the 512-point groups it meets
+__device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, LineRec *rec, int4 *span, int t, int kb,
+                                           Shared &sh) {
     stage_reps(P, sh);
     image_load(P, img + t, sh);
     __syncthreads();
@@ -562,16 +567,37 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, const TrialImage *_
     const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
     if (sh.go) {
         build_reflection(P, sh, k, rec + base, span + base);
+        for (int q = 0; q < 2; ++q) {
+            const int4 sp = span[base + q];
+            if (sp.y > sp.x)
+                for (int g = sp.x >> 9; g <= (sp.y - 1) >> 9; ++g) {
+                    atomicMin(&img[t].grange[g][0], 2 * k + q);
+                    atomicMax(&img[t].grange[g][1], 2 * k + q);
+                }
+        }
     } else {
         span[base] = make_int4(0, 0, 0, 0);
         span[base + 1] = make_int4(0, 0, 0, 0);
     }
 }
 
-__global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, const TrialImage *__restrict__ img, LineRec *rec,
-                                                     int4 *span) {
+// Workgroup b of a 1-D launch over ntrials x `per` units: unit u of trial t, with every
+// unit of trial t on XCD t mod 8 (workgroups go to the XCDs round robin), so a trial's line
+// records are written and read through one L2.  The grid is 8 ceil(ntrials / 8) per;
+// returns false for the padding slots.
+__device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, int &u) {
+    const int x = b & 7, j = b >> 3;
+    t = (j / per) * 8 + x;
+    u = j % per;
+    return t < ntrials;
+}
+
+__global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, TrialImage *img, LineRec *rec,
+                                                     int4 *span, int per, int ntrials) {
     __shared__ Shared sh;
-    lines_body(P, img, rec, span, blockIdx.y, blockIdx.x, sh);
+    int t, u;
+    if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
+    lines_body(P, img, rec, span, t, u, sh);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
@@ -651,6 +677,13 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     const int nl = 2 * P.nr;
     const LineRec *rec = grec + (size_t)t * nl;
     const int4 *sp = gspan + (size_t)t * nl;
+    // lines that can meet this workgroup's points: the union of its groups' ranges
+    int first = 0x7fffffff, last = -1;
+    for (int g = split * TPW; g < min(nsplit, split * TPW + TPW); ++g) {
+        first = min(first, im->grange[g][0]);
+        last = max(last, im->grange[g][1]);
+    }
+    const int lend = min(nl, last + 1);
     float acc[2 * TPW];
     float2 tf[2 * TPW];
 #pragma unroll
@@ -662,12 +695,12 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     }
     if (tid == 0) S.count = 0;
     __syncthreads();
-    for (int c0 = 0; c0 < nl; c0 += 256) {
+    for (int c0 = first; c0 < lend; c0 += 256) {
         // compact the overlapping lines of this chunk, in order, into the stage
         const int L = c0 + tid;
         int4 sv = make_int4(0, 0, 0, 0);
         bool ov = false;
-        if (L < nl) { sv = sp[L]; ov = sv.x < hi_pt && sv.y > lo_pt; }
+        if (L < lend) { sv = sp[L]; ov = sv.x < hi_pt && sv.y > lo_pt; }
         const unsigned long long m = __ballot(ov);
         if (lane == 0) S.wcnt[wave] = __popcll(m);
         __syncthreads();
@@ -679,7 +712,7 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
         if (tid == 0) { int n = S.count; for (int w = 0; w < kAccWaves; ++w) n += S.wcnt[w]; S.count = n; }
         __syncthreads();
         const int count = S.count;
-        if (count > kStage - 256 || (c0 + 256 >= nl && count > 0)) {
+        if (count > kStage - 256 || (c0 + 256 >= lend && count > 0)) {
 #pragma unroll
             for (int j = 0; j < TPW; ++j) {
                 const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
@@ -763,9 +796,11 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
 template <int TPW>
 __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
                                                      const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
-                                                     double2 *partial, int nsplit) {
+                                                     double2 *partial, int nsplit, int per, int ntrials) {
     __shared__ AccShared S;
-    accum_body<TPW>(P, img, grec, gspan, partial, nsplit, blockIdx.y, blockIdx.x, S);
+    int t, u;
+    if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
+    accum_body<TPW>(P, img, grec, gspan, partial, nsplit, t, u, S);
 }
 
 // chi^2 of trial t: the ordered sum of its partials (3e38 for an invalid trial)
@@ -887,10 +922,11 @@ __global__ void k_exchange(DevPlan P) { exchange_body(P, blockIdx.x * blockDim.x
 
 // ---- groups: independent problems sharing every launch --------------------------------
 // Each stage of a group runs as one launch over all members.  Workgroup b takes its
-// (member, chain, tile) from a table built on the host: member-major for the per-chain
-// stages; for the accumulate stage grouped so that every tile of one trial lands on the
-// same XCD (workgroup b runs on XCD b mod 8, so the trial's line records are fetched into
-// one L2) with the heaviest trials first.  A member entry of -1 is an empty slot.
+// (member, chain, unit) from a table built on the host: member-major for the per-chain
+// stages; for the lines and accumulate stages grouped so that every unit of one trial
+// lands on the same XCD (workgroup b runs on XCD b mod 8, so the trial's line records are
+// written and read through one L2), heaviest trials first.  A member entry of -1 is an
+// empty slot.
 __global__ void __launch_bounds__(64) k_group_setup(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
                                                     const int2 *__restrict__ tab, int bootstrap) {
     __shared__ Shared sh;
@@ -904,9 +940,10 @@ __global__ void __launch_bounds__(128) k_group_lines(const DevPlan *__restrict__
                                                      const int4 *__restrict__ tab) {
     __shared__ Shared sh;
     const int4 w = tab[blockIdx.x];
+    if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    lines_body(P, (const TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh);
+    lines_body(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh);
 }
 
 template <int TPW>
@@ -1207,6 +1244,11 @@ __global__ void k_matrix_reduce(const float *__restrict__ partial, int row_tiles
     chi2[t] = s;
 }
 
+// ---- HBM calibration: a plain streaming copy of known size ------------------------------
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
 }  // namespace ptk
 
 // ---- host-side launchers (used by pt_host.cpp) ----------------------------------------
@@ -1229,17 +1271,22 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
     hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick);
     return hipGetLastError();
 }
+static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }
+
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_lines, dim3((P.nr + 127) / 128, ntrials), dim3(128), 0, s, P, (const TrialImage *)img, rec, span);
+    const int per = (P.nr + 127) / 128;
+    hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(128), 0, s, P, (TrialImage *)img, rec, span, per,
+                       ntrials);
     return hipGetLastError();
 }
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
                               int ngroups, int ntrials, int tpw, hipStream_t s) {
-    const dim3 grid((ngroups + tpw - 1) / tpw, ntrials);
+    const int per = (ngroups + tpw - 1) / tpw;
+    const dim3 grid(xcd_grid(per, ntrials));
     const TrialImage *im = (const TrialImage *)img;
-    if (tpw == 1) hipLaunchKernelGGL(k_stage_accum<1>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups);
-    else if (tpw == 4) hipLaunchKernelGGL(k_stage_accum<4>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups);
-    else hipLaunchKernelGGL(k_stage_accum<2>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups);
+    if (tpw == 1) hipLaunchKernelGGL(k_stage_accum<1>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials);
+    else if (tpw == 4) hipLaunchKernelGGL(k_stage_accum<4>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials);
+    else hipLaunchKernelGGL(k_stage_accum<2>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials);
     return hipGetLastError();
 }
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
@@ -1325,6 +1372,11 @@ hipError_t launch_group_finish_setup(const DevPlan *plans, const DevMember *mem,
 }
 hipError_t launch_group_exchange(const DevPlan *plans, const int2 *tab, int npairs, hipStream_t s) {
     if (npairs > 0) hipLaunchKernelGGL(k_group_exchange, dim3((npairs + 63) / 64), dim3(64), 0, s, plans, tab, npairs);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_probe(const void *a, void *b, size_t bytes, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const float4 *)a, (float4 *)b, bytes / sizeof(float4));
     return hipGetLastError();
 }
 

@@ -42,6 +42,7 @@ hipError_t launch_group_finish(const DevPlan *plans, const DevMember *mem, const
                                hipStream_t s);
 hipError_t launch_group_finish_setup(const DevPlan *plans, const DevMember *mem, const int2 *tab, int nchains, hipStream_t s);
 hipError_t launch_group_exchange(const DevPlan *plans, const int2 *tab, int npairs, hipStream_t s);
+hipError_t launch_copy_probe(const void *a, void *b, size_t bytes, hipStream_t s);
 hipError_t launch_plain(const DevPlan &P, const double *X, void *scratch, LineRec *rec, int4 *span, double *partial,
                         double *out, int ch, hipStream_t s);
 }  // namespace ptk

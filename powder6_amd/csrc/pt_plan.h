@@ -24,6 +24,7 @@
 #define PTK_MAX_BKG_F 24
 #define PTK_S_COUNT_K 12
 #define PTK_MATRIX_MAX_REFL 4096
+#define PTK_MAX_GROUPS 64          // 512-point groups per pattern (32768 points)
 
 namespace ptk {
 

@@ -163,6 +163,7 @@ def lib():
         L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_matrix_timing.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, P_I, P_D]
+        L.pt_hbm_copy_probe.argtypes = [ctypes.c_size_t, ctypes.c_int, P_D]
         P_F = ctypes.POINTER(ctypes.c_float)
         L.pt_screen.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.pt_screen_stats.argtypes = [ctypes.c_void_p, P_D]
@@ -329,6 +330,13 @@ class Plan:
         t, ms = c_int32(), np.zeros(3)
         _check(lib().pt_tiles(self.h, int(request), ctypes.byref(t), _dptr(ms)))
         return t.value, ms
+
+
+def hbm_copy_probe(nbytes=1 << 30, repeats=3):
+    """GB/s (read + write) of a plain streaming copy of nbytes (pt_hbm_copy_probe)."""
+    g = c_double()
+    _check(lib().pt_hbm_copy_probe(int(nbytes), int(repeats), ctypes.byref(g)))
+    return g.value
 
 
 class PtScreenConfig(ctypes.Structure):

@@ -556,29 +556,39 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
 }
 
 // line records of reflections [kb * blockDim.x, ...) of trial t; every line with a window
-// widens the line range of the 512-point groups it meets
+// widens the line range of the 512-point groups it meets (reduced in LDS, then one global
+// atomic min / max per group and workgroup)
 __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, LineRec *rec, int4 *span, int t, int kb,
-                                           Shared &sh) {
+                                           Shared &sh, int (*gr)[2]) {
     stage_reps(P, sh);
     image_load(P, img + t, sh);
+    const int ng = (P.np + 511) >> 9;
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
     const int k = kb * blockDim.x + threadIdx.x;
-    if (k >= P.nr) return;
-    const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
-    if (sh.go) {
-        build_reflection(P, sh, k, rec + base, span + base);
-        for (int q = 0; q < 2; ++q) {
-            const int4 sp = span[base + q];
-            if (sp.y > sp.x)
-                for (int g = sp.x >> 9; g <= (sp.y - 1) >> 9; ++g) {
-                    atomicMin(&img[t].grange[g][0], 2 * k + q);
-                    atomicMax(&img[t].grange[g][1], 2 * k + q);
-                }
+    if (k < P.nr) {
+        const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
+        if (sh.go) {
+            build_reflection(P, sh, k, rec + base, span + base);
+            for (int q = 0; q < 2; ++q) {
+                const int4 sp = span[base + q];
+                if (sp.y > sp.x)
+                    for (int g = sp.x >> 9; g <= (sp.y - 1) >> 9; ++g) {
+                        atomicMin(&gr[g][0], 2 * k + q);
+                        atomicMax(&gr[g][1], 2 * k + q);
+                    }
+            }
+        } else {
+            span[base] = make_int4(0, 0, 0, 0);
+            span[base + 1] = make_int4(0, 0, 0, 0);
         }
-    } else {
-        span[base] = make_int4(0, 0, 0, 0);
-        span[base + 1] = make_int4(0, 0, 0, 0);
     }
+    __syncthreads();
+    for (int g = threadIdx.x; g < ng; g += blockDim.x)
+        if (gr[g][1] >= 0) {
+            atomicMin(&img[t].grange[g][0], gr[g][0]);
+            atomicMax(&img[t].grange[g][1], gr[g][1]);
+        }
 }
 
 // Workgroup b of a 1-D launch over ntrials x `per` units: unit u of trial t, with every
@@ -595,9 +605,10 @@ __device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, in
 __global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, TrialImage *img, LineRec *rec,
                                                      int4 *span, int per, int ntrials) {
     __shared__ Shared sh;
+    __shared__ int gr[PTK_MAX_GROUPS][2];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    lines_body(P, img, rec, span, t, u, sh);
+    lines_body(P, img, rec, span, t, u, sh, gr);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
@@ -622,7 +633,9 @@ __device__ __forceinline__ float line_point(const LineRec &R, bool split, float 
 }
 
 // one line at the two points (i, i + 64) of a lane; `split` and `asym` are uniform over
-// the wavefront (scalar branches), the window test is a select
+// the wavefront (scalar branches), the window test is a select -- skipped (INSIDE) when
+// the window covers the lane's whole tile
+template <bool INSIDE>
 __device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, float asym, float2 t0, float2 t1,
                                           int i0, float &a0, float &a1) {
     float d0 = (t0.x - R.a.x) + (t0.y - R.a.y);
@@ -641,8 +654,8 @@ __device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, 
     const float l0 = __builtin_amdgcn_rcpf(fmaf(f0.y, q0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(f1.y, q1, 1.0f));
     const float v0 = fmaxf(fmaf(f0.z, l0, fmaf(f0.w, g0, -R.a.w)), 0.0f);
     const float v1 = fmaxf(fmaf(f1.z, l1, fmaf(f1.w, g1, -R.a.w)), 0.0f);
-    const float amp0 = (i0 >= v.x && i0 < v.y) ? R.a.z : 0.0f;
-    const float amp1 = (i0 + 64 >= v.x && i0 + 64 < v.y) ? R.a.z : 0.0f;
+    const float amp0 = (INSIDE || (i0 >= v.x && i0 < v.y)) ? R.a.z : 0.0f;
+    const float amp1 = (INSIDE || (i0 + 64 >= v.x && i0 + 64 < v.y)) ? R.a.z : 0.0f;
     a0 = fmaf(amp0, v0, a0);
     a1 = fmaf(amp1, v1, a1);
 }
@@ -732,7 +745,11 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                             // line flags are the same for every lane: keep them scalar
                             const int flags = __builtin_amdgcn_readfirstlane(v.w);
                             const float asym = __int_as_float(__builtin_amdgcn_readfirstlane(v.z));
-                            line_pair(R, v, (flags & 1) != 0, asym, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                            const int ws = __builtin_amdgcn_readfirstlane(v.x), we = __builtin_amdgcn_readfirstlane(v.y);
+                            if (ws <= t_lo && we >= t_hi)    // the window covers the whole tile
+                                line_pair<true>(R, v, (flags & 1) != 0, asym, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                            else
+                                line_pair<false>(R, v, (flags & 1) != 0, asym, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
                         }
                     }
                     acc[2 * j] = a0;
@@ -939,11 +956,12 @@ __global__ void __launch_bounds__(64) k_group_setup(const DevPlan *__restrict__ 
 __global__ void __launch_bounds__(128) k_group_lines(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
                                                      const int4 *__restrict__ tab) {
     __shared__ Shared sh;
+    __shared__ int gr[PTK_MAX_GROUPS][2];
     const int4 w = tab[blockIdx.x];
     if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    lines_body(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh);
+    lines_body(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh, gr);
 }
 
 template <int TPW>

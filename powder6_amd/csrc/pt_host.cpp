@@ -128,6 +128,50 @@ std::vector<Refl> enumerate_reflections(const double cell[6], int n_ops, const i
 }
 
 // coset representatives modulo the centring translations (rietveld_core.js:164-179)
+// centring translations of the operator list (identity rotation)
+std::vector<std::array<double, 3>> centrings(int n_ops, const int32_t *R, const double *T) {
+    std::vector<std::array<double, 3>> C;
+    for (int i = 0; i < n_ops; ++i) {
+        const int32_t *r = R + 9 * i;
+        if (r[0] == 1 && r[4] == 1 && r[8] == 1 && !r[1] && !r[2] && !r[3] && !r[5] && !r[6] && !r[7])
+            C.push_back({T[3 * i], T[3 * i + 1], T[3 * i + 2]});
+    }
+    if (C.empty()) C.push_back({0, 0, 0});
+    return C;
+}
+
+// One of each (R, t) / (-R, -t) pair of representatives, when every representative has
+// its partner (-R, -t + centring) in the list: the inversion lies at the origin, so
+// cos h.(R r + t) is the same for both members and the sines cancel -- the structure
+// factor sums cosines over half the list and doubles them.  Empty when some
+// representative has no partner.
+std::vector<int> centro_half(const std::vector<int> &reps, int n_ops, const int32_t *R, const double *T) {
+    const auto C = centrings(n_ops, R, T);
+    std::vector<int> partner(reps.size(), -1), half;
+    for (size_t a = 0; a < reps.size(); ++a) {
+        if (partner[a] >= 0) continue;
+        const int i = reps[a];
+        for (size_t b = 0; b < reps.size() && partner[a] < 0; ++b) {
+            if (b == a || partner[b] >= 0) continue;
+            const int j = reps[b];
+            bool neg = true;
+            for (int x = 0; x < 9; ++x) neg = neg && R[9 * j + x] == -R[9 * i + x];
+            if (!neg) continue;
+            for (auto &c : C) {
+                bool ok = true;
+                for (int x = 0; x < 3; ++x) {
+                    const double d = T[3 * i + x] + T[3 * j + x] - c[x];
+                    if (std::fabs(d - rnd(d)) > 1e-6) { ok = false; break; }
+                }
+                if (ok) { partner[a] = (int)b; partner[b] = (int)a; break; }
+            }
+        }
+        if (partner[a] < 0) return {};
+        half.push_back(i);
+    }
+    return half;
+}
+
 std::vector<int> coset_reps(int n_ops, const int32_t *R, const double *T) {
     std::vector<std::array<double, 3>> C;
     for (int i = 0; i < n_ops; ++i) {
@@ -438,8 +482,12 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         std::vector<Refl> list = !cellMoves ? enumerate_reflections(cell, ph.n_ops, ph.op_r, ph.op_t, dmin, nullptr)
                                  : monotone ? enumerate_reflections(env, ph.n_ops, ph.op_r, ph.op_t, dmin, nullptr)
                                             : enumerate_reflections(cell, ph.n_ops, ph.op_r, ph.op_t, dmin, box);
-        std::vector<int> reps = coset_reps(ph.n_ops, ph.op_r, ph.op_t);
+        const std::vector<int> all_reps = coset_reps(ph.n_ops, ph.op_r, ph.op_t);
+        const std::vector<int> half = centro_half(all_reps, ph.n_ops, ph.op_r, ph.op_t);
+        const std::vector<int> &reps = half.empty() ? all_reps : half;
         ptk::DevPhase &dp = P.phase[q];
+        dp.centro = half.empty() ? 0 : 1;
+        dp.n_reps_full = (int)all_reps.size();
         dp.system = ph.system;
         dp.unique_axis = ph.unique_axis;
         dp.profile_type = ph.profile_type;
@@ -470,7 +518,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
             d.n_gauss = a.n_gauss;
             for (int g = 0; g < PT_MAX_GAUSS; ++g) { d.ff_a[g] = (float)a.ff_a[g]; d.ff_b[g] = (float)a.ff_b[g]; }
             d.ff_c = (float)a.ff_c;
-            d.w0 = (float)((a.mult != 0 ? a.mult : 1) / (double)reps.size());
+            d.w0 = (float)((a.mult != 0 ? a.mult : 1) / (double)all_reps.size());
             d.b_neutron = (float)a.b_neutron;
             d.fp = (float)a.fp;
             d.fpp = (float)a.fpp;
@@ -1230,7 +1278,7 @@ int pt_workload(pt_plan *pl, const double *x, double *out) {
     HIPCHECK(hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, pl->stream), "readback");
     HIPCHECK(hipStreamSynchronize(pl->stream), "workload");
     int S = 0;
-    for (int q = 0; q < pl->P.n_phases; ++q) S += pl->P.phase[q].n_atoms * pl->P.phase[q].n_reps;
+    for (int q = 0; q < pl->P.n_phases; ++q) S += pl->P.phase[q].n_atoms * pl->P.phase[q].n_reps_full;
     out[0] = (double)h[0]; out[1] = (double)h[1]; out[2] = (double)h[2]; out[3] = (double)S;
     out[4] = (double)pl->np; out[5] = (double)pl->P.n_bkg;
     return 0;

@@ -46,6 +46,8 @@ struct DevPhase {
     int refl_off, refl_count;  // envelope reflections [refl_off, refl_off + refl_count)
     int param_base;            // first model parameter of the phase block
     int fixed_angles;          // 1: the cell angles do not move; trig below is used
+    int centro;                // 1: the representatives are one of each (R, t) / (-R, -t) pair
+    int n_reps_full;           // representatives before that halving
     double po_dir[3];
     double ang[6];             // alpha, beta, gamma (deg), cos alpha, cos beta, cos gamma
     double ang_sin[3];

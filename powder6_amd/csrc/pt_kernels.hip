@@ -110,9 +110,11 @@ __device__ inline void trial_setup(const DevPlan &P, Shared &sh) {
 // and coset representatives.  Phases h.(R r + t) are formed in FP64 and reduced to turns
 // in [0, 1); sine and cosine of the turn come from the hardware (v_sin_f32 / v_cos_f32
 // take revolutions).  Loop order: representatives outside, up to four atoms inside, so
-// h R and h.t are formed once per representative.
-__device__ inline float structure_f2(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k, int l,
-                                     float s2) {
+// h R and h.t are formed once per representative.  CENTRO: the representatives are one
+// of each (R, t) / (-R, -t) pair, so A = 2 sum cos, B = 0 (and C, D alike).
+template <bool CENTRO>
+__device__ __forceinline__ float structure_f2_impl(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k,
+                                                   int l, float s2) {
     float Ar = 0.f, Br = 0.f, Cr = 0.f, Dr = 0.f;
     const float q4 = 0.25f * s2;
     for (int j0 = 0; j0 < ph.n_atoms; j0 += 4) {
@@ -135,7 +137,7 @@ __device__ inline float structure_f2(const DevPlan &P, const Shared &sh, const D
                     const double phs = H0 * px[a] + H1 * py[a] + H2 * pz[a] + ht;
                     const float fr = (float)(phs - floor(phs));
                     ca[a] += __builtin_amdgcn_cosf(fr);
-                    sa[a] += __builtin_amdgcn_sinf(fr);
+                    if (!CENTRO) sa[a] += __builtin_amdgcn_sinf(fr);
                 }
             }
         }
@@ -154,11 +156,18 @@ __device__ inline float structure_f2(const DevPlan &P, const Shared &sh, const D
                 const float dw = __expf(-sh.aB[ga] * q4);
                 const float wj = sh.aocc[ga] * f0 * dw * F[11];
                 const float wi = (P.radiation == 1 || !(sh.afp[ga][1] != 0.f)) ? 0.f : sh.aocc[ga] * sh.afp[ga][1] * dw * F[11];
-                Ar += wj * ca[a]; Br += wj * sa[a]; Cr += wi * ca[a]; Dr += wi * sa[a];
+                Ar += wj * ca[a]; Cr += wi * ca[a];
+                if (!CENTRO) { Br += wj * sa[a]; Dr += wi * sa[a]; }
             }
         }
     }
+    if (CENTRO) return 4.0f * (Ar * Ar + Cr * Cr);
     return Ar * Ar + Br * Br + Cr * Cr + Dr * Dr;
+}
+
+__device__ __forceinline__ float structure_f2(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k, int l,
+                                              float s2) {
+    return ph.centro ? structure_f2_impl<true>(P, sh, ph, h, k, l, s2) : structure_f2_impl<false>(P, sh, ph, h, k, l, s2);
 }
 
 // March-Dollase factor averaged over the equivalents (rietveld_core.js:824-843)
@@ -197,7 +206,7 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
     rec.l = make_float4(kG * pk.ig[0] * pk.ig[0], 4.0f * pk.il[0] * pk.il[0], pk.cl[0], pk.cg[0]);
     rec.r = make_float4(kG * pk.ig[1] * pk.ig[1], 4.0f * pk.il[1] * pk.il[1], pk.cl[1], pk.cg[1]);
     const int split = (pk.ig[0] != pk.ig[1] || pk.il[0] != pk.il[1] || pk.cl[0] != pk.cl[1] || pk.cg[0] != pk.cg[1]) ? 1 : 0;
-    span = make_int4(start, stop, __float_as_int(pk.asym), split);
+    span = make_int4(start, stop, __float_as_int(pk.asym), split | (pk.asym != 0.0f ? 2 : 0));
 }
 
 // Both lines (Kalpha1, Kalpha2) of envelope reflection k: Model.geometry (:719-736),
@@ -632,20 +641,20 @@ __device__ __forceinline__ float line_point(const LineRec &R, bool split, float 
     return inwin ? R.a.z * v : 0.0f;
 }
 
-// one line at the two points (i, i + 64) of a lane; `split` and `asym` are uniform over
-// the wavefront (scalar branches), the window test is a select -- skipped (INSIDE) when
-// the window covers the lane's whole tile
-template <bool INSIDE>
-__device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, float asym, float2 t0, float2 t1,
-                                          int i0, float &a0, float &a1) {
+// one line at the two points (i, i + 64) of a lane.  INSIDE: the window covers the lane's
+// whole tile (no window test); PLAIN: equal flanks and no asymmetry (the common case, no
+// flag tests).  Otherwise `split`, `has_asym` and `asym` are uniform over the wavefront.
+template <bool INSIDE, bool PLAIN>
+__device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, bool has_asym, float asym, float2 t0,
+                                          float2 t1, int i0, float &a0, float &a1) {
     float d0 = (t0.x - R.a.x) + (t0.y - R.a.y);
     float d1 = (t1.x - R.a.x) + (t1.y - R.a.y);
-    if (asym != 0.0f) {
+    if (!PLAIN && has_asym) {
         if (fabsf(d0) >= 1e-9f) d0 = d0 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d0)));
         if (fabsf(d1) >= 1e-9f) d1 = d1 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d1)));
     }
     float4 f0 = R.r, f1 = R.r;
-    if (split) {
+    if (!PLAIN && split) {
         if (d0 < 0.0f) f0 = R.l;
         if (d1 < 0.0f) f1 = R.l;
     }
@@ -679,7 +688,8 @@ template <int TPW>
 __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict__ img, const LineRec *__restrict__ grec,
                                   const int4 *__restrict__ gspan, double2 *partial, int nsplit, int t, int split,
                                   AccShared &S) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wavefront index is uniform: keep it (and every tile bound) in scalar registers
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const TrialImage *im = img + t;
     if (!im->go) {
         if (tid < TPW && split * TPW + tid < nsplit) partial[(size_t)t * nsplit + split * TPW + tid] = make_double2(0.0, 0.0);
@@ -737,19 +747,39 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                         bool o = false;
                         if (q < count) { int4 v = S.span[q]; o = v.x < t_hi && v.y > t_lo; }
                         unsigned long long mask = __ballot(o);
-                        while (mask) {
-                            const int b = __ffsll((long long)mask) - 1;
-                            mask &= mask - 1ull;
-                            const int4 v = S.span[lb + b];
-                            const LineRec &R = S.rec[lb + b];
-                            // line flags are the same for every lane: keep them scalar
+                        // one staged line at this wavefront's tile; flags and window are the
+                        // same for every lane, so they live in scalar registers
+                        auto line = [&](const int4 &v, const LineRec &R) {
                             const int flags = __builtin_amdgcn_readfirstlane(v.w);
-                            const float asym = __int_as_float(__builtin_amdgcn_readfirstlane(v.z));
                             const int ws = __builtin_amdgcn_readfirstlane(v.x), we = __builtin_amdgcn_readfirstlane(v.y);
-                            if (ws <= t_lo && we >= t_hi)    // the window covers the whole tile
-                                line_pair<true>(R, v, (flags & 1) != 0, asym, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
-                            else
-                                line_pair<false>(R, v, (flags & 1) != 0, asym, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                            const bool inside = ws <= t_lo && we >= t_hi;   // the window covers the tile
+                            if (flags == 0) {
+                                if (inside) line_pair<true, true>(R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                                else line_pair<false, true>(R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                            } else {
+                                const bool has_asym = (flags & 2) != 0;
+                                const float asym = has_asym ? __int_as_float(__builtin_amdgcn_readfirstlane(v.z)) : 0.0f;
+                                line_pair<false, false>(R, v, (flags & 1) != 0, has_asym, asym, tf[2 * j], tf[2 * j + 1], i0, a0,
+                                                        a1);
+                            }
+                        };
+                        // two lines per iteration, in line order: the second line's LDS reads
+                        // are issued before the first line is computed
+                        while (mask) {
+                            const int b1 = __ffsll((long long)mask) - 1;
+                            mask &= mask - 1ull;
+                            const int4 v1 = S.span[lb + b1];
+                            const LineRec R1 = S.rec[lb + b1];
+                            if (mask) {
+                                const int b2 = __ffsll((long long)mask) - 1;
+                                mask &= mask - 1ull;
+                                const int4 v2 = S.span[lb + b2];
+                                const LineRec R2 = S.rec[lb + b2];
+                                line(v1, R1);
+                                line(v2, R2);
+                            } else {
+                                line(v1, R1);
+                            }
                         }
                     }
                     acc[2 * j] = a0;

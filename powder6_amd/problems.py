@@ -326,6 +326,101 @@ def nacl_fm3m(simulate=None, seed=20260101, chains=16, sweeps=1000):
     return pb
 
 
+def zincblende_f43m(simulate=None, seed=20260505, chains=16, anomalous=False, neutron=False):
+    """ZnS F-43m (non-centrosymmetric) a=5.41, Zn 4a (0,0,0), S 4c (1/4,1/4,1/4); Cu
+    Ka1/Ka2; 2theta 20-130 step 0.02; optional anomalous f', f'' on Zn (-1.6, 0.68 at Cu Ka)
+    or neutron lengths (Zn 5.68 fm, S 2.847 fm).  Refined: a, zero, scale, U, V, W, B_Zn,
+    B_S, bkg0, bkg1."""
+    r, t = S.group_ops(216)
+    cell = [5.41, 5.41, 5.41, 90, 90, 90]
+    fp, fpp = (-1.6, 0.68) if anomalous else (0.0, 0.0)
+    atoms = [Atom('Zn1', 'Zn', S.site_symmetry(r, t, [0, 0, 0], cell), B=0.7, fp=fp, fpp=fpp, b_neutron=5.68),
+             Atom('S1', 'S', S.site_symmetry(r, t, [0.25, 0.25, 0.25], cell), B=0.9, b_neutron=2.847)]
+    ph = Phase('p1', 216, cell, atoms, 'tch_aniso')
+    pb = Problem(_grid(20.0, 130.0, 0.02), [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0,
+                 radiation=1 if neutron else 0)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    _cheb(pb, [120.0, -20.0, 8.0, -3.0, 1.0, -0.5])
+    limits = {'p1.a': (5.41 * 0.97, 5.41 * 1.03), 'zero': (-1, 1), 'p1.scale': (0, math.inf), 'p1.Zn1.B': (-2, 30),
+              'p1.S1.B': (-2, 30)}
+    for n in ('p1.a', 'zero', 'p1.scale', 'p1.U', 'p1.V', 'p1.W', 'p1.Zn1.B', 'p1.S1.B', 'bkg0', 'bkg1'):
+        lo, hi = limits.get(n, (-math.inf, math.inf))
+        pb.refine(n, lo, hi)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 10, 0, 100.0, 211
+    if simulate is not None:
+        x_true = pb.start_vector()
+        pb.observe(simulate, x_true, seed)
+        pb.x0 = list(_perturb(pb, x_true, seed + 1))
+    return pb
+
+
+def general_noncentro(group, simulate=None, seed=20260606, chains=16):
+    """A non-centrosymmetric phase in general positions: P2_1 (group 4, a=5.2 b=6.3 c=7.1
+    beta=101, 3 atoms) or P2_12_12_1 (group 19, a=5.2 b=6.3 c=7.1, 3 atoms); Cu Ka1/Ka2,
+    2theta 10-110 step 0.02; every coordinate and B refined with the cell, zero, scale."""
+    rng = np.random.default_rng(seed + group)
+    r, t = S.group_ops(group)
+    cell = [5.2, 6.3, 7.1, 90, 101.0 if group == 4 else 90, 90]
+    atoms = []
+    for j, el in enumerate(('Fe', 'O', 'C')):
+        xyz = list(rng.uniform(0.05, 0.95, size=3))
+        atoms.append(Atom('%s%d' % (el, j + 1), el, S.site_symmetry(r, t, xyz, cell), B=float(rng.uniform(0.5, 2.0))))
+    ph = Phase('p1', group, cell, atoms, 'tch_aniso', unique_axis=1)
+    pb = Problem(_grid(10.0, 110.0, 0.02), [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    _cheb(pb, [100.0, -10.0, 5.0])
+    for n in ('a', 'b', 'c'):
+        v = pb.base[pb.index['p1.' + n]]
+        pb.refine('p1.' + n, 0.98 * v, 1.02 * v)
+    pb.refine('zero', -1, 1)
+    pb.refine('p1.scale', 0, math.inf)
+    for at in atoms:
+        for f in at.site['free']:
+            n = 'p1.%s.%s' % (at.label, 'xyz'[f])
+            v = pb.base[pb.index[n]]
+            pb.refine(n, v - 0.1, v + 0.1)
+        pb.refine('p1.%s.B' % at.label, -2, 30)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 10, 0, 100.0, 300 + group
+    if simulate is not None:
+        x_true = pb.start_vector()
+        pb.observe(simulate, x_true, seed)
+        pb.x0 = list(_perturb(pb, x_true, seed + 1))
+    return pb
+
+
+def two_phase(simulate=None, seed=20260707, chains=16):
+    """NaCl (Fm-3m, a=5.64) + ZnS (F-43m, a=5.41) in one pattern, Cu Ka1/Ka2, 2theta 20-120
+    step 0.02; each phase with its own cell, scale, widths and B; shared zero and background."""
+    r1, t1 = S.group_ops(225)
+    r2, t2 = S.group_ops(216)
+    c1, c2 = [5.64, 5.64, 5.64, 90, 90, 90], [5.41, 5.41, 5.41, 90, 90, 90]
+    ph1 = Phase('p1', 225, c1, [Atom('Na1', 'Na', S.site_symmetry(r1, t1, [0, 0, 0], c1), B=1.1),
+                                Atom('Cl1', 'Cl', S.site_symmetry(r1, t1, [0.5, 0.5, 0.5], c1), B=0.9)], 'tch_aniso')
+    ph2 = Phase('p2', 216, c2, [Atom('Zn1', 'Zn', S.site_symmetry(r2, t2, [0, 0, 0], c2), B=0.7),
+                                Atom('S1', 'S', S.site_symmetry(r2, t2, [0.25, 0.25, 0.25], c2), B=0.9)], 'tch_aniso')
+    pb = Problem(_grid(20.0, 120.0, 0.02), [ph1, ph2], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0)
+    pb.default_values()
+    for q in ('p1', 'p2'):
+        for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+            pb.set(q + '.' + n, v)
+    pb.set('p2.scale', 0.6)
+    _cheb(pb, [150.0, -30.0, 10.0])
+    for n, lo, hi in (('p1.a', 5.5, 5.8), ('p2.a', 5.3, 5.5), ('zero', -1, 1), ('p1.scale', 0, math.inf),
+                      ('p2.scale', 0, math.inf), ('p1.U', -math.inf, math.inf), ('p2.W', -math.inf, math.inf),
+                      ('p1.Na1.B', -2, 30), ('p2.S1.B', -2, 30), ('bkg0', -math.inf, math.inf)):
+        pb.refine(n, lo, hi)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 10, 0, 100.0, 401
+    if simulate is not None:
+        x_true = pb.start_vector()
+        pb.observe(simulate, x_true, seed)
+        pb.x0 = list(_perturb(pb, x_true, seed + 1))
+    return pb
+
+
 PBSO4_SITES = [  # anglesite-like PbSO4, Pnma (ITA standard setting), 4c x 4 + 8d
     ('Pb1', 'Pb', (0.188, 0.25, 0.167), 1.2),
     ('S1', 'S', (0.063, 0.25, 0.684), 0.8),

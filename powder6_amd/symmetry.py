@@ -94,15 +94,19 @@ CENTRING = {
 # ITA Vol. A generators (standard settings; origin choice 1 where there are two)
 GROUPS = {
     # number: (symbol, system, centring, generators)
+    1: ('P1', 'triclinic', 'P', []),
     2: ('P-1', 'triclinic', 'P', ['-x,-y,-z']),
+    4: ('P2_1', 'monoclinic', 'P', ['-x,y+1/2,-z']),
     10: ('P2/m', 'monoclinic', 'P', ['-x,y,-z', '-x,-y,-z']),
     14: ('P2_1/c', 'monoclinic', 'P', ['-x,y+1/2,-z+1/2', '-x,-y,-z']),
     15: ('C2/c', 'monoclinic', 'C', ['-x,y,-z+1/2', '-x,-y,-z']),
+    19: ('P2_12_12_1', 'orthorhombic', 'P', ['-x+1/2,-y,z+1/2', '-x,y+1/2,-z+1/2']),
     62: ('Pnma', 'orthorhombic', 'P', ['-x+1/2,-y,z+1/2', '-x,y+1/2,-z', '-x,-y,-z']),
     123: ('P4/mmm', 'tetragonal', 'P', ['-y,x,z', 'x,-y,-z', '-x,-y,-z']),
     136: ('P4_2/mnm', 'tetragonal', 'P', ['-y+1/2,x+1/2,z+1/2', '-x+1/2,y+1/2,-z+1/2', '-x,-y,-z']),
     139: ('I4/mmm', 'tetragonal', 'I', ['-y,x,z', 'x,-y,-z', '-x,-y,-z']),
     221: ('Pm-3m', 'cubic', 'P', ['-y,x,z', 'z,x,y', '-x,-y,-z']),
+    216: ('F-43m', 'cubic', 'F', ['y,-x,-z', 'z,x,y']),
     225: ('Fm-3m', 'cubic', 'F', ['-y,x,z', 'z,x,y', '-x,-y,-z']),
     229: ('Im-3m', 'cubic', 'I', ['-y,x,z', 'z,x,y', '-x,-y,-z']),
 }

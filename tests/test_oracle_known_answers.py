@@ -239,10 +239,27 @@ def _orbit_size(op_r, h):
     return len(seen)
 
 
+def _allowed_p21(h, k, l):          # P2_1 (unique b): 0k0 k = 2n
+    return not (h == 0 and l == 0 and k % 2)
+
+
+def _allowed_p212121(h, k, l):      # h00 h = 2n, 0k0 k = 2n, 00l l = 2n
+    if k == 0 and l == 0 and h % 2:
+        return False
+    if h == 0 and l == 0 and k % 2:
+        return False
+    if h == 0 and k == 0 and l % 2:
+        return False
+    return True
+
+
 @pytest.mark.parametrize('group,cell,rule', [
     (225, (5.64, 5.64, 5.64, 90, 90, 90), _allowed_fm3m),
+    (216, (5.41, 5.41, 5.41, 90, 90, 90), _allowed_fm3m),
     (62, (8.48, 5.40, 6.96, 90, 90, 90), _allowed_pnma),
     (14, (10.4, 18.8, 6.8, 90, 106, 90), _allowed_p21c),
+    (4, (5.2, 6.3, 7.1, 90, 101, 90), _allowed_p21),
+    (19, (5.2, 6.3, 7.1, 90, 90, 90), _allowed_p212121),
 ])
 def test_reflection_list_brute_force(group, cell, rule):
     """Every canonical hkl is allowed by the ITA conditions, the multiplicities are Laue
@@ -290,6 +307,33 @@ def test_nacl_structure_factor_closed_form():
         sign = 1 if hh % 2 == 0 else -1
         F = 4 * (fna * math.exp(-Bn * q) + sign * fcl * math.exp(-Bc * q))
         near(F2, F * F, 1e-9 * max(1, F * F))
+
+
+@pytest.mark.parametrize('anomalous', [False, True])
+def test_zincblende_structure_factor_closed_form(anomalous):
+    """F-43m ZnS (no inversion centre): Zn at 0, S at 1/4 1/4 1/4, unmixed hkl, phi =
+    pi (h + k + l) / 2.  The reference's |F|^2 = A^2 + B^2 + C^2 + D^2 (f'' in C, D,
+    Friedel-averaged) is 16 [(f_Zn + f_S cos phi)^2 + (f_S sin phi)^2 + f''_Zn^2], with
+    f_Zn = (f0 + f') DW and f''_Zn scaled by the same DW."""
+    pb = PR.zincblende_f43m(anomalous=anomalous)
+    pb.y = np.ones_like(pb.tth)
+    m = O.Model(pb)
+    x = pb.start_vector()
+    hk, f2, h, cen = m.reflections(x)
+    p = pb.values(x)
+    a = p[pb.index['p1.a']]
+    zn, su = pb.phases[0].atoms
+    Bz, Bs = p[pb.index['p1.Zn1.B']], p[pb.index['p1.S1.B']]
+    assert len(hk) > 5
+    for (hh, kk, ll, mm), F2 in zip(hk, f2):
+        q = (hh * hh + kk * kk + ll * ll) / a ** 2 / 4
+        f0z = zn.ff_c + sum(ai * math.exp(-bi * q) for ai, bi in zip(zn.ff_a, zn.ff_b))
+        f0s = su.ff_c + sum(ai * math.exp(-bi * q) for ai, bi in zip(su.ff_a, su.ff_b))
+        dz, ds = math.exp(-Bz * q), math.exp(-Bs * q)
+        fz, fs, fzz = (f0z + zn.fp) * dz, (f0s + su.fp) * ds, zn.fpp * dz
+        phi = math.pi * (hh + kk + ll) / 2
+        F2ref = 16 * ((fz + fs * math.cos(phi)) ** 2 + (fs * math.sin(phi)) ** 2 + fzz ** 2)
+        near(F2, F2ref, 1e-9 * max(1, F2ref))
 
 
 def test_cosets_centring():

@@ -85,12 +85,13 @@ def matrix_form(pt, PR, trials=32768):
         dt = time.perf_counter() - t
         ti_ms, tp_ms = plan.matrix_timing()
         n_hkl = plan.workload(x0)['n_hkl']
+        chi_bf16 = plan.eval(X[:8], pt.FORM_MATRIX_BF16)     # the BF16-pair precision comparison
     dense = 2.0 * len(pb.tth) * n_hkl                      # FLOP per trial of the dense product
     return {'value': trials / dt, 'unit': 'trial patterns/s', 'trials': trials, 'n_vars': pb.nv,
             'form': 'shared profile matrix, Y = P I on MFMA (FP16 hi/lo, 3 products), fused chi^2',
             'intensity_ms': ti_ms, 'product_ms': tp_ms,
             'product_dense_tflops': dense * trials / (tp_ms / 1e3) / 1e12,
-            'product_frac_of_fp16_peak': dense * trials / (tp_ms / 1e3) / PEAK_FP16_MFMA}, pb, X[:8], chi2[:8]
+            'product_frac_of_fp16_peak': dense * trials / (tp_ms / 1e3) / PEAK_FP16_MFMA}, pb, X[:8], chi2[:8], chi_bf16
 
 
 def uneven_batch(pt, PR, sweeps=2):
@@ -234,7 +235,7 @@ def main():
         print(json.dumps({'metric': METRIC, 'value': value, 'unit': 'trial patterns/s', 'main_only': True,
                           'plain': plain, 'batched': batched, 'hbm_copy_gbps': copy_gbps}))
         return
-    mat, mpb, mX, mchi = matrix_form(pt, PR)
+    mat, mpb, mX, mchi, mchi_bf16 = matrix_form(pt, PR)
     uneven, upbs, ucosts = uneven_batch(pt, PR)
     learned = learned_screening(pt, PR)
 
@@ -269,6 +270,8 @@ def main():
         import oracle_lib as O
         mm = O.Model(mpb)
         mat['chi2_rel_err_vs_fp64'] = max(abs(g - mm.cost(x)) / max(1.0, mm.cost(x)) for x, g in zip(mX, mchi))
+        mat['chi2_rel_err_vs_fp64_with_bf16_pairs'] = max(abs(g - mm.cost(x)) / max(1.0, mm.cost(x))
+                                                          for x, g in zip(mX, mchi_bf16))
         mm.close()
         err = 0.0
         for upb, g in zip(upbs, ucosts):

@@ -142,7 +142,9 @@ enum { PT_S_COST = 0, PT_S_BEST = 1, PT_S_STEP = 2, PT_S_ACCEPTED = 3, PT_S_PROP
 /* Evaluation forms for pt_eval */
 enum { PT_FORM_PLAIN = 0,    /* one trial per launch sequence, whole-GPU kernels        */
        PT_FORM_BATCHED = 1,  /* many trials per launch, one workgroup per trial (VALU) */
-       PT_FORM_MATRIX = 2 }; /* shared profile matrix x intensities on MFMA (atom-only trials) */
+       PT_FORM_MATRIX = 2,   /* shared profile matrix x intensities on MFMA (atom-only trials),
+                                FP16 hi/lo pairs                                           */
+       PT_FORM_MATRIX_BF16 = 3 }; /* the same with BF16 hi/lo pairs (precision comparison) */
 
 typedef struct pt_plan pt_plan;
 

@@ -238,6 +238,7 @@ struct pt_plan {
     std::vector<hipEvent_t> ev;
     // shared-profile matrix form
     bool matrix_ready = false;
+    int matrix_bf16 = 0;                 // element type P is filled in (0 FP16, 1 BF16)
     int kpad = 0, np_pad = 0, row_tiles = 0, cap_mtrials = 0;
     unsigned short *d_phi = nullptr, *d_plo = nullptr, *d_ihi = nullptr, *d_ilo = nullptr;
     float *d_factor = nullptr, *d_iscale = nullptr, *d_mpart = nullptr;
@@ -741,7 +742,7 @@ static double sum_pairs(pt_plan *pl, int pairs) {
 
 static const int kEvalChunk = 2048;   // trial slots per pass of pt_eval
 
-static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout);
+static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout, int bf16 = 0);
 
 static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2, bool time_it) {
     int rc = ensure_trials(pl, ntrials);
@@ -1092,13 +1093,32 @@ int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
         pl->last_launches = 0;
         return 0;
     }
-    if (form == PT_FORM_MATRIX) return eval_matrix(pl, x, ntrials, chi2, nullptr);
+    if (form == PT_FORM_MATRIX) return eval_matrix(pl, x, ntrials, chi2, nullptr, 0);
+    if (form == PT_FORM_MATRIX_BF16) return eval_matrix(pl, x, ntrials, chi2, nullptr, 1);
     return fail(PT_ERR_STATE, "PT evaluation form not available for this plan.");
 }
 
 // ---- shared-profile matrix form ----------------------------------------------------
-static int matrix_prepare(pt_plan *pl) {
-    if (pl->matrix_ready) return 0;
+static int matrix_fill(pt_plan *pl, int bf16) {
+    HIPCHECK(hipMemsetAsync(pl->d_phi, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
+    HIPCHECK(hipMemsetAsync(pl->d_plo, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
+    HIPCHECK(ptk::launch_matrix_fill(pl->P, pl->d_srec, pl->d_sspan, pl->d_phi, pl->d_plo, pl->kpad, bf16, pl->stream), "matrix fill");
+    pl->matrix_bf16 = bf16;
+    return 0;
+}
+
+static int matrix_prepare(pt_plan *pl, int bf16) {
+    if (pl->matrix_ready && pl->matrix_bf16 == bf16) return 0;
+    if (pl->matrix_ready) {
+        // same columns, other element type: rebuild the unit-height lines and refill P
+        HIPCHECK(hipMemcpyAsync(pl->d_x, pl->x0.data(), sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
+        HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, 1, 0, 0, pl->stream), "setup");
+        HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_factor, pl->stream), "matrix columns");
+        int rc = matrix_fill(pl, bf16);
+        if (rc) return rc;
+        HIPCHECK(hipStreamSynchronize(pl->stream), "matrix build");
+        return 0;
+    }
     if (pl->nr > PTK_MATRIX_MAX_REFL) return fail(PT_ERR_BUDGET, "PT shared-profile form supports at most 4096 reflections.");
     int rc = ensure_trials(pl, 1);
     if (rc || (rc = ensure_slots(pl, std::max(1, pl->chains)))) return rc;
@@ -1115,9 +1135,7 @@ static int matrix_prepare(pt_plan *pl) {
     HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_factor, pl->stream), "matrix columns");
     std::vector<int4> spans(2 * (size_t)pl->nr);
     HIPCHECK(hipMemcpyAsync(spans.data(), pl->d_sspan, sizeof(int4) * spans.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
-    HIPCHECK(hipMemsetAsync(pl->d_phi, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
-    HIPCHECK(hipMemsetAsync(pl->d_plo, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
-    HIPCHECK(ptk::launch_matrix_fill(pl->P, pl->d_srec, pl->d_sspan, pl->d_phi, pl->d_plo, pl->kpad, pl->stream), "matrix fill");
+    if ((rc = matrix_fill(pl, bf16))) return rc;
     HIPCHECK(hipStreamSynchronize(pl->stream), "matrix build");
     // band of reflections meeting each 64-point row tile
     std::vector<int2> band(pl->row_tiles);
@@ -1167,8 +1185,8 @@ static const int kMatrixChunk = 4096;
 
 // chi^2 of ntrials structure-only vectors through Y = P I; yout (optional, ntrials <= 64)
 // receives Y (without background) per trial and point
-static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout) {
-    int rc = matrix_prepare(pl);
+static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout, int bf16) {
+    int rc = matrix_prepare(pl, bf16);
     if (rc) return rc;
     for (int t = 0; t < ntrials; ++t)
         for (int j = 0; j < pl->nv; ++j)
@@ -1191,11 +1209,11 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
         HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x + (size_t)t0 * pl->nv, pl->d_img, n, 0, 0, pl->stream), "setup");
         HIPCHECK(hipEventRecord(pl->ev[4 * p], pl->stream), "timing");
         HIPCHECK(ptk::launch_matrix_intensity(pl->P, pl->d_img, pl->d_factor, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->kpad, 0, n,
-                                              pl->stream), "intensities");
+                                              bf16, pl->stream), "intensities");
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 1], pl->stream), "timing");
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 2], pl->stream), "timing");
         HIPCHECK(ptk::launch_matrix_chi2(pl->P, pl->d_phi, pl->d_plo, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->d_band, pl->d_yw,
-                                         pl->d_mpart, pl->kpad, tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, pl->stream),
+                                         pl->d_mpart, pl->kpad, tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, bf16, pl->stream),
                  "matrix product");
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 3], pl->stream), "timing");
         HIPCHECK(ptk::launch_matrix_reduce(pl->d_mpart, pl->row_tiles, tpad, n, pl->d_img, pl->d_chi2 + t0, pl->stream), "reduce");

@@ -643,28 +643,31 @@ __device__ __forceinline__ float line_point(const LineRec &R, bool split, float 
 
 // one line at the two points (i, i + 64) of a lane.  INSIDE: the window covers the lane's
 // whole tile (no window test); PLAIN: equal flanks and no asymmetry (the common case, no
-// flag tests).  Otherwise `split`, `has_asym` and `asym` are uniform over the wavefront.
+// flag tests).  a and r are the record's first 32 bytes, already in registers; the left
+// flank is read from the staged record L only for a line whose flanks differ.  Otherwise
+// `split`, `has_asym` and `asym` are uniform over the wavefront.
 template <bool INSIDE, bool PLAIN>
-__device__ __forceinline__ void line_pair(const LineRec &R, int4 v, bool split, bool has_asym, float asym, float2 t0,
-                                          float2 t1, int i0, float &a0, float &a1) {
-    float d0 = (t0.x - R.a.x) + (t0.y - R.a.y);
-    float d1 = (t1.x - R.a.x) + (t1.y - R.a.y);
+__device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, const LineRec &L, int4 v, bool split,
+                                          bool has_asym, float asym, float2 t0, float2 t1, int i0, float &a0, float &a1) {
+    float d0 = (t0.x - ra.x) + (t0.y - ra.y);
+    float d1 = (t1.x - ra.x) + (t1.y - ra.y);
     if (!PLAIN && has_asym) {
         if (fabsf(d0) >= 1e-9f) d0 = d0 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d0)));
         if (fabsf(d1) >= 1e-9f) d1 = d1 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d1)));
     }
-    float4 f0 = R.r, f1 = R.r;
+    float4 f0 = rr, f1 = rr;
     if (!PLAIN && split) {
-        if (d0 < 0.0f) f0 = R.l;
-        if (d1 < 0.0f) f1 = R.l;
+        const float4 fl = L.l;
+        if (d0 < 0.0f) f0 = fl;
+        if (d1 < 0.0f) f1 = fl;
     }
     const float q0 = d0 * d0, q1 = d1 * d1;
     const float g0 = __builtin_amdgcn_exp2f(f0.x * q0), g1 = __builtin_amdgcn_exp2f(f1.x * q1);
     const float l0 = __builtin_amdgcn_rcpf(fmaf(f0.y, q0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(f1.y, q1, 1.0f));
-    const float v0 = fmaxf(fmaf(f0.z, l0, fmaf(f0.w, g0, -R.a.w)), 0.0f);
-    const float v1 = fmaxf(fmaf(f1.z, l1, fmaf(f1.w, g1, -R.a.w)), 0.0f);
-    const float amp0 = (INSIDE || (i0 >= v.x && i0 < v.y)) ? R.a.z : 0.0f;
-    const float amp1 = (INSIDE || (i0 + 64 >= v.x && i0 + 64 < v.y)) ? R.a.z : 0.0f;
+    const float v0 = fmaxf(fmaf(f0.z, l0, fmaf(f0.w, g0, -ra.w)), 0.0f);
+    const float v1 = fmaxf(fmaf(f1.z, l1, fmaf(f1.w, g1, -ra.w)), 0.0f);
+    const float amp0 = (INSIDE || (i0 >= v.x && i0 < v.y)) ? ra.z : 0.0f;
+    const float amp1 = (INSIDE || (i0 + 64 >= v.x && i0 + 64 < v.y)) ? ra.z : 0.0f;
     a0 = fmaf(amp0, v0, a0);
     a1 = fmaf(amp1, v1, a1);
 }
@@ -749,18 +752,20 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                         unsigned long long mask = __ballot(o);
                         // one staged line at this wavefront's tile; flags and window are the
                         // same for every lane, so they live in scalar registers
-                        auto line = [&](const int4 &v, const LineRec &R) {
+                        auto line = [&](const int4 &v, const float4 &ra, const float4 &rr, const LineRec &R) {
                             const int flags = __builtin_amdgcn_readfirstlane(v.w);
                             const int ws = __builtin_amdgcn_readfirstlane(v.x), we = __builtin_amdgcn_readfirstlane(v.y);
                             const bool inside = ws <= t_lo && we >= t_hi;   // the window covers the tile
                             if (flags == 0) {
-                                if (inside) line_pair<true, true>(R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
-                                else line_pair<false, true>(R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                                if (inside)
+                                    line_pair<true, true>(ra, rr, R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                                else
+                                    line_pair<false, true>(ra, rr, R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
                             } else {
                                 const bool has_asym = (flags & 2) != 0;
                                 const float asym = has_asym ? __int_as_float(__builtin_amdgcn_readfirstlane(v.z)) : 0.0f;
-                                line_pair<false, false>(R, v, (flags & 1) != 0, has_asym, asym, tf[2 * j], tf[2 * j + 1], i0, a0,
-                                                        a1);
+                                line_pair<false, false>(ra, rr, R, v, (flags & 1) != 0, has_asym, asym, tf[2 * j], tf[2 * j + 1], i0,
+                                                        a0, a1);
                             }
                         };
                         // two lines per iteration, in line order: the second line's LDS reads
@@ -769,16 +774,16 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                             const int b1 = __ffsll((long long)mask) - 1;
                             mask &= mask - 1ull;
                             const int4 v1 = S.span[lb + b1];
-                            const LineRec R1 = S.rec[lb + b1];
+                            const float4 a1r = S.rec[lb + b1].a, r1r = S.rec[lb + b1].r;
                             if (mask) {
                                 const int b2 = __ffsll((long long)mask) - 1;
                                 mask &= mask - 1ull;
                                 const int4 v2 = S.span[lb + b2];
-                                const LineRec R2 = S.rec[lb + b2];
-                                line(v1, R1);
-                                line(v2, R2);
+                                const float4 a2r = S.rec[lb + b2].a, r2r = S.rec[lb + b2].r;
+                                line(v1, a1r, r1r, S.rec[lb + b1]);
+                                line(v2, a2r, r2r, S.rec[lb + b2]);
                             } else {
-                                line(v1, R1);
+                                line(v1, a1r, r1r, S.rec[lb + b1]);
                             }
                         }
                     }
@@ -1114,11 +1119,31 @@ __global__ void k_plain_reduce(const double *partial, int n, double *out) {
 // by 2^12, each trial's I by a power of two), the products hi*hi + hi*lo + lo*hi are
 // accumulated in FP32 by v_mfma_f32_16x16x16f16, and the weighted chi^2 is reduced in
 // the epilogue -- Y is never stored.  Only the band of reflections whose windows meet a
-// 64-point row tile is multiplied.
+// 64-point row tile is multiplied.  The same kernels also run the pairs as BF16
+// (v_mfma_f32_16x16x16_bf16), the precision comparison of PT_FORM_MATRIX_BF16.
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 constexpr float kPScale = 4096.0f;   // 2^12
+
+// v as a hi/lo pair of 16-bit floats, FP16 (22 significant bits together) or BF16 (16):
+// hi = round(v), lo = round(v - hi), round to nearest even
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void split_pair(float v, bool bf16, unsigned short &hi, unsigned short &lo) {
+    if (bf16) {
+        hi = bf16_bits(v);
+        lo = bf16_bits(v - __uint_as_float((uint32_t)hi << 16));
+    } else {
+        const _Float16 h = (_Float16)v;
+        hi = __builtin_bit_cast(unsigned short, h);
+        lo = __builtin_bit_cast(unsigned short, (_Float16)(v - (float)h));
+    }
+}
 
 // unit-height lines of the start vector and the per-reflection height factors
 __global__ void __launch_bounds__(128) k_matrix_columns(DevPlan P, const TrialImage *__restrict__ img, LineRec *rec,
@@ -1135,7 +1160,7 @@ __global__ void __launch_bounds__(128) k_matrix_columns(DevPlan P, const TrialIm
 // column k of P: Kalpha1 then Kalpha2 line values over the union of their windows
 __global__ void __launch_bounds__(256) k_matrix_fill(DevPlan P, const LineRec *__restrict__ rec,
                                                      const int4 *__restrict__ span, unsigned short *phi,
-                                                     unsigned short *plo, int kpad) {
+                                                     unsigned short *plo, int kpad, int bf16) {
     const int k = blockIdx.x;
     const int4 s1 = span[2 * k], s2 = span[2 * k + 1];
     int lo = P.np, hi = 0;
@@ -1148,11 +1173,10 @@ __global__ void __launch_bounds__(256) k_matrix_fill(DevPlan P, const LineRec *_
             v += line_point(rec[2 * k], s1.w & 1, __int_as_float(s1.z), t, true);
         if (i >= s2.x && i < s2.y)
             v += line_point(rec[2 * k + 1], s2.w & 1, __int_as_float(s2.z), t, true);
-        const float sv = v * kPScale;
-        const _Float16 h = (_Float16)sv;
-        const _Float16 l = (_Float16)(sv - (float)h);
-        phi[(size_t)i * kpad + k] = __builtin_bit_cast(unsigned short, h);
-        plo[(size_t)i * kpad + k] = __builtin_bit_cast(unsigned short, l);
+        unsigned short h, l;
+        split_pair(v * kPScale, bf16 != 0, h, l);
+        phi[(size_t)i * kpad + k] = h;
+        plo[(size_t)i * kpad + k] = l;
     }
 }
 
@@ -1160,7 +1184,7 @@ __global__ void __launch_bounds__(256) k_matrix_fill(DevPlan P, const LineRec *_
 // scaled by 2^-e_t with max_k |I| 2^-e_t <= 2^14; iscale[t] = 2^e_t.
 __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const TrialImage *__restrict__ img,
                                                           const float *__restrict__ factor, unsigned short *ihi,
-                                                          unsigned short *ilo, float *iscale, int kpad, int t0) {
+                                                          unsigned short *ilo, float *iscale, int kpad, int t0, int bf16) {
     __shared__ Shared sh;
     __shared__ float ival[PTK_MATRIX_MAX_REFL];
     __shared__ float wmax[16];
@@ -1200,13 +1224,22 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const Trial
     __syncthreads();
     const float sc = wmax[0], inv = 1.0f / sc;
     for (int k = threadIdx.x; k < kpad; k += blockDim.x) {
-        const float v = k < P.nr ? ival[k] * inv : 0.0f;
-        const _Float16 h = (_Float16)v;
-        const _Float16 l = (_Float16)(v - (float)h);
-        ihi[(size_t)t * kpad + k] = __builtin_bit_cast(unsigned short, h);
-        ilo[(size_t)t * kpad + k] = __builtin_bit_cast(unsigned short, l);
+        unsigned short h, l;
+        split_pair(k < P.nr ? ival[k] * inv : 0.0f, bf16 != 0, h, l);
+        ihi[(size_t)t * kpad + k] = h;
+        ilo[(size_t)t * kpad + k] = l;
     }
     if (threadIdx.x == 0) iscale[t] = sc;
+}
+
+__device__ __forceinline__ bf16x4_t load_b4(const unsigned short *p) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    bf16x4_t h;
+    h[0] = (short)(u.x & 0xffff);
+    h[1] = (short)(u.x >> 16);
+    h[2] = (short)(u.y & 0xffff);
+    h[3] = (short)(u.y >> 16);
+    return h;
 }
 
 __device__ __forceinline__ half4_t load_h4(const unsigned short *p) {
@@ -1222,6 +1255,7 @@ __device__ __forceinline__ half4_t load_h4(const unsigned short *p) {
 // chi^2 partials of a 64-point x 64-trial tile: wavefront w multiplies rows
 // [i0 + 16 w, i0 + 16 w + 16) by the four 16-trial column blocks over the band of the row
 // tile; lane l holds D[4 (l/16) + e][l % 16] of each 16x16 block.
+template <bool BF16>
 __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned short *__restrict__ phi,
                                                      const unsigned short *__restrict__ plo,
                                                      const unsigned short *__restrict__ ihi,
@@ -1239,16 +1273,30 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     for (int nb = 0; nb < 4; ++nb) acc[nb] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
     const int arow = i0 + (lane & 15), kq = 4 * (lane >> 4);
     for (int k = b.x; k < b.y; k += 16) {
-        const half4_t ah = load_h4(phi + (size_t)arow * kpad + k + kq);
-        const half4_t al = load_h4(plo + (size_t)arow * kpad + k + kq);
+        if (BF16) {
+            const bf16x4_t ah = load_b4(phi + (size_t)arow * kpad + k + kq);
+            const bf16x4_t al = load_b4(plo + (size_t)arow * kpad + k + kq);
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int col = j0 + 16 * nb + (lane & 15);
-            const half4_t bh = load_h4(ihi + (size_t)col * kpad + k + kq);
-            const half4_t bl = load_h4(ilo + (size_t)col * kpad + k + kq);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[nb], 0, 0, 0);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[nb], 0, 0, 0);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[nb], 0, 0, 0);
+            for (int nb = 0; nb < 4; ++nb) {
+                const int col = j0 + 16 * nb + (lane & 15);
+                const bf16x4_t bh = load_b4(ihi + (size_t)col * kpad + k + kq);
+                const bf16x4_t bl = load_b4(ilo + (size_t)col * kpad + k + kq);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[nb], 0, 0, 0);
+            }
+        } else {
+            const half4_t ah = load_h4(phi + (size_t)arow * kpad + k + kq);
+            const half4_t al = load_h4(plo + (size_t)arow * kpad + k + kq);
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int col = j0 + 16 * nb + (lane & 15);
+                const half4_t bh = load_h4(ihi + (size_t)col * kpad + k + kq);
+                const half4_t bl = load_h4(ilo + (size_t)col * kpad + k + kq);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[nb], 0, 0, 0);
+            }
         }
     }
     // epilogue: weighted residuals, summed over the 16 rows of this wavefront
@@ -1354,20 +1402,25 @@ hipError_t launch_matrix_columns(const DevPlan &P, const void *img, LineRec *rec
     return hipGetLastError();
 }
 hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *span, unsigned short *phi, unsigned short *plo,
-                              int kpad, hipStream_t s) {
-    hipLaunchKernelGGL(k_matrix_fill, dim3(P.nr), dim3(256), 0, s, P, rec, span, phi, plo, kpad);
+                              int kpad, int bf16, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_fill, dim3(P.nr), dim3(256), 0, s, P, rec, span, phi, plo, kpad, bf16);
     return hipGetLastError();
 }
 hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
-                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_matrix_intensity, dim3(n), dim3(256), 0, s, P, (const TrialImage *)img, factor, ihi, ilo, iscale, kpad, t0);
+                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_intensity, dim3(n), dim3(256), 0, s, P, (const TrialImage *)img, factor, ihi, ilo, iscale, kpad, t0,
+                       bf16);
     return hipGetLastError();
 }
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
-                              float *partial, int kpad, int tpad, int row_tiles, double *yout, hipStream_t s) {
-    hipLaunchKernelGGL(k_matrix_chi2, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
-                       partial, kpad, tpad, yout);
+                              float *partial, int kpad, int tpad, int row_tiles, double *yout, int bf16, hipStream_t s) {
+    if (bf16)
+        hipLaunchKernelGGL(k_matrix_chi2<true>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
+                           partial, kpad, tpad, yout);
+    else
+        hipLaunchKernelGGL(k_matrix_chi2<false>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
+                           partial, kpad, tpad, yout);
     return hipGetLastError();
 }
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,

@@ -21,13 +21,14 @@ hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 
                                double *chi2, int pt_mode, hipStream_t s);
 hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s);
 hipError_t launch_matrix_columns(const DevPlan &P, const void *img, LineRec *rec, int4 *span, float *factor, hipStream_t s);
+// bf16 != 0: the hi/lo pairs as BF16 instead of FP16
 hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *span, unsigned short *phi, unsigned short *plo,
-                              int kpad, hipStream_t s);
+                              int kpad, int bf16, hipStream_t s);
 hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
-                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, hipStream_t s);
+                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16, hipStream_t s);
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
-                              float *partial, int kpad, int tpad, int row_tiles, double *yout, hipStream_t s);
+                              float *partial, int kpad, int tpad, int row_tiles, double *yout, int bf16, hipStream_t s);
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,
                                 hipStream_t s);
 hipError_t launch_pattern(const DevPlan &P, const double *X, double *y, int block, int ch, size_t dyn, hipStream_t s);

@@ -91,11 +91,11 @@ struct DevPlan {
 
 // line record (one Kalpha line of one reflection), as held in LDS / global memory:
 //   a = {centre_hi, centre_lo (float-float centre, deg), amplitude, pedestal}
-//   l = {kg, kl, cl, cg} left flank (delta < 0), r = the same for the right flank, where
+//   r = {kg, kl, cl, cg} right flank (delta >= 0), l = the same for the left flank, where
 //       g = exp2(kg delta^2), l = 1/(1 + kl delta^2), value = cl l + cg g - pedestal
 //   span = {start, stop, asymmetry (float bits), 1 when the flanks differ}: points in [start, stop)
 struct LineRec {
-    float4 a, l, r;
+    float4 a, r, l;            // r first: a line with equal flanks needs only a and r
 };
 
 // staged-pipeline buffers of one member of a group (per chain: a trial image, 2 nr line

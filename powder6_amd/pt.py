@@ -21,7 +21,7 @@ c_int32, c_double, c_uint32 = ctypes.c_int32, ctypes.c_double, ctypes.c_uint32
 P_D = ctypes.POINTER(c_double)
 P_I = ctypes.POINTER(c_int32)
 
-FORM_PLAIN, FORM_BATCHED, FORM_MATRIX = 0, 1, 2
+FORM_PLAIN, FORM_BATCHED, FORM_MATRIX, FORM_MATRIX_BF16 = 0, 1, 2, 3
 S_COST, S_BEST, S_STEP, S_ACCEPTED, S_PROPOSALS, S_SWAPS, S_ATTEMPTS, S_VALID, S_LOGU, S_UNSOLVED, S_TRIAL, S_FEASIBLE = range(12)
 
 

@@ -70,3 +70,23 @@ def test_matrix_refuses_profile_moves():
         pl.eval(X[:1], pt.FORM_MATRIX)
         with pytest.raises(pt.PTError, match='only structure variables'):
             pl.eval(X, pt.FORM_MATRIX)
+
+
+def test_bf16_pairs_against_fp16_pairs(atoms_only):
+    """The precision choice of the matrix form, measured: the same product with BF16
+    hi/lo pairs (16 significant bits) against FP16 pairs (22 bits), both against the FP64
+    oracle.  FP16 meets the 1e-5 bar; BF16's error is larger, and both are printed."""
+    pb = atoms_only
+    m = O.Model(pb)
+    X = _struct_probes(pb, 48, 5)
+    with pt.Plan(pb) as pl:
+        g16 = pl.eval(X, pt.FORM_MATRIX)
+        gb = pl.eval(X, pt.FORM_MATRIX_BF16)
+        g16b = pl.eval(X, pt.FORM_MATRIX)          # switching back refills P in FP16
+    cpu = np.array([m.cost(x) for x in X])
+    e16 = np.max(np.abs(g16 - cpu) / np.maximum(1.0, cpu))
+    eb = np.max(np.abs(gb - cpu) / np.maximum(1.0, cpu))
+    print('matrix form chi2 max rel err vs FP64: FP16 pairs %.2e, BF16 pairs %.2e' % (e16, eb))
+    assert e16 <= TOL
+    assert eb > e16
+    assert np.array_equal(g16, g16b)

@@ -1122,9 +1122,9 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
     if (pl->nr > PTK_MATRIX_MAX_REFL) return fail(PT_ERR_BUDGET, "PT shared-profile form supports at most 4096 reflections.");
     int rc = ensure_trials(pl, 1);
     if (rc || (rc = ensure_slots(pl, std::max(1, pl->chains)))) return rc;
-    pl->kpad = (pl->nr + 15) / 16 * 16;
-    pl->np_pad = (pl->np + 63) / 64 * 64;
-    pl->row_tiles = pl->np_pad / 64;
+    pl->kpad = (pl->nr + 31) / 32 * 32;           // 32-reflection slices
+    pl->np_pad = (pl->np + 127) / 128 * 128;      // 128-point row tiles
+    pl->row_tiles = pl->np_pad / 128;
     if ((rc = dalloc(pl, &pl->d_factor, (size_t)pl->nr)) || (rc = dalloc(pl, &pl->d_phi, (size_t)pl->np_pad * pl->kpad)) ||
         (rc = dalloc(pl, &pl->d_plo, (size_t)pl->np_pad * pl->kpad)) || (rc = dalloc(pl, &pl->d_band, (size_t)pl->row_tiles)) ||
         (rc = dalloc(pl, &pl->d_yw, (size_t)pl->np_pad)))
@@ -1137,16 +1137,16 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
     HIPCHECK(hipMemcpyAsync(spans.data(), pl->d_sspan, sizeof(int4) * spans.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
     if ((rc = matrix_fill(pl, bf16))) return rc;
     HIPCHECK(hipStreamSynchronize(pl->stream), "matrix build");
-    // band of reflections meeting each 64-point row tile
+    // band of reflections meeting each 128-point row tile, in whole 32-reflection slices
     std::vector<int2> band(pl->row_tiles);
     for (int rt = 0; rt < pl->row_tiles; ++rt) {
-        int lo_pt = 64 * rt, hi_pt = lo_pt + 64, kmin = -1, kmax = -1;
+        int lo_pt = 128 * rt, hi_pt = lo_pt + 128, kmin = -1, kmax = -1;
         for (int k = 0; k < pl->nr; ++k) {
             const int4 &a = spans[2 * k], &b = spans[2 * k + 1];
             bool hit = (a.y > a.x && a.x < hi_pt && a.y > lo_pt) || (b.y > b.x && b.x < hi_pt && b.y > lo_pt);
             if (hit) { if (kmin < 0) kmin = k; kmax = k; }
         }
-        band[rt] = kmin < 0 ? make_int2(0, 0) : make_int2(kmin / 16 * 16, std::min(pl->kpad, (kmax + 16) / 16 * 16));
+        band[rt] = kmin < 0 ? make_int2(0, 0) : make_int2(kmin / 32 * 32, std::min(pl->kpad, (kmax + 32) / 32 * 32));
     }
     // y_obs - background of the start vector (rietveld_core.js:647-653, 920-931), FP64 on the host
     std::vector<float2> yw(pl->np_pad, make_float2(0.f, 0.f));

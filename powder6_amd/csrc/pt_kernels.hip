@@ -1232,29 +1232,35 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const Trial
     if (threadIdx.x == 0) iscale[t] = sc;
 }
 
-__device__ __forceinline__ bf16x4_t load_b4(const unsigned short *p) {
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
-    bf16x4_t h;
-    h[0] = (short)(u.x & 0xffff);
-    h[1] = (short)(u.x >> 16);
-    h[2] = (short)(u.y & 0xffff);
-    h[3] = (short)(u.y >> 16);
-    return h;
-}
+// 16-bit operand quad as the MFMA input type
+template <bool BF16>
+struct Quad;
+template <>
+struct Quad<false> {
+    typedef half4_t T;
+    static __device__ __forceinline__ T from(uint2 u) { return __builtin_bit_cast(half4_t, u); }
+    static __device__ __forceinline__ floatx4_t mfma(T a, T b, floatx4_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Quad<true> {
+    typedef bf16x4_t T;
+    static __device__ __forceinline__ T from(uint2 u) { return __builtin_bit_cast(bf16x4_t, u); }
+    static __device__ __forceinline__ floatx4_t mfma(T a, T b, floatx4_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+};
 
-__device__ __forceinline__ half4_t load_h4(const unsigned short *p) {
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
-    half4_t h;
-    h[0] = __builtin_bit_cast(_Float16, (unsigned short)(u.x & 0xffff));
-    h[1] = __builtin_bit_cast(_Float16, (unsigned short)(u.x >> 16));
-    h[2] = __builtin_bit_cast(_Float16, (unsigned short)(u.y & 0xffff));
-    h[3] = __builtin_bit_cast(_Float16, (unsigned short)(u.y >> 16));
-    return h;
-}
+constexpr int kMxK = 32;                 // reflections per staged slice
+constexpr int kMxLd = kMxK + 4;          // LDS row stride (16-bit elements), padded
 
-// chi^2 partials of a 64-point x 64-trial tile: wavefront w multiplies rows
-// [i0 + 16 w, i0 + 16 w + 16) by the four 16-trial column blocks over the band of the row
-// tile; lane l holds D[4 (l/16) + e][l % 16] of each 16x16 block.
+// chi^2 partials of a 128-point x 64-trial tile.  The trial operand (I hi/lo, 64 trials x
+// 32 reflections per slice) is staged through LDS, double buffered, and shared by the four
+// wavefronts; wavefront w multiplies its 32 rows [i0 + 32 w, i0 + 32 w + 32) (two 16-row
+// blocks, P read from global) by the four 16-trial column blocks over the tile's band.
+// MFMA operand layout (16x16x16): lane l supplies A[l % 16][4 (l / 16) + e] and
+// B[4 (l / 16) + e][l % 16] and holds D[4 (l / 16) + e][l % 16].
 template <bool BF16>
 __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned short *__restrict__ phi,
                                                      const unsigned short *__restrict__ plo,
@@ -1263,69 +1269,93 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
                                                      const float *__restrict__ iscale, const int2 *__restrict__ band,
                                                      const float2 *__restrict__ yw, float *partial, int kpad, int tpad,
                                                      double *yout) {
+    typedef Quad<BF16> Q;
+    __shared__ __align__(16) unsigned short sB[2][2][64][kMxLd];   // [buffer][hi, lo][trial][k]
     __shared__ float red[4][64];
     const int rt = blockIdx.x, j0 = blockIdx.y * 64;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i0 = rt * 64 + wave * 16;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = rt * 128 + wave * 32;
     const int2 b = band[rt];
-    floatx4_t acc[4];
+    const int li = lane & 15, kq = 4 * (lane >> 4);
+    floatx4_t acc[2][4];
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) acc[nb] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
-    const int arow = i0 + (lane & 15), kq = 4 * (lane >> 4);
-    for (int k = b.x; k < b.y; k += 16) {
-        if (BF16) {
-            const bf16x4_t ah = load_b4(phi + (size_t)arow * kpad + k + kq);
-            const bf16x4_t al = load_b4(plo + (size_t)arow * kpad + k + kq);
+    for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                const int col = j0 + 16 * nb + (lane & 15);
-                const bf16x4_t bh = load_b4(ihi + (size_t)col * kpad + k + kq);
-                const bf16x4_t bl = load_b4(ilo + (size_t)col * kpad + k + kq);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[nb], 0, 0, 0);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[nb], 0, 0, 0);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[nb], 0, 0, 0);
-            }
-        } else {
-            const half4_t ah = load_h4(phi + (size_t)arow * kpad + k + kq);
-            const half4_t al = load_h4(plo + (size_t)arow * kpad + k + kq);
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                const int col = j0 + 16 * nb + (lane & 15);
-                const half4_t bh = load_h4(ihi + (size_t)col * kpad + k + kq);
-                const half4_t bl = load_h4(ilo + (size_t)col * kpad + k + kq);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[nb], 0, 0, 0);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[nb], 0, 0, 0);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[nb], 0, 0, 0);
-            }
-        }
+        for (int nb = 0; nb < 4; ++nb) acc[rb][nb] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+    // staging: thread t moves trial j0 + t / 4, reflections [8 (t % 4), 8 (t % 4) + 8) of
+    // the slice, hi and lo (16 bytes each)
+    const int sc_col = tid >> 2, sc_k = 8 * (tid & 3);
+    const size_t src = (size_t)(j0 + sc_col) * kpad + sc_k;
+    uint4 nh = make_uint4(0u, 0u, 0u, 0u), nl = nh;
+    if (b.x < b.y) {
+        nh = *reinterpret_cast<const uint4 *>(ihi + src + b.x);
+        nl = *reinterpret_cast<const uint4 *>(ilo + src + b.x);
     }
-    // epilogue: weighted residuals, summed over the 16 rows of this wavefront
+    int buf = 0;
+    for (int k0 = b.x; k0 < b.y; k0 += kMxK) {
+        // the slice k0 into LDS, then the next slice's loads in flight while computing
+        *reinterpret_cast<uint2 *>(&sB[buf][0][sc_col][sc_k]) = make_uint2(nh.x, nh.y);
+        *reinterpret_cast<uint2 *>(&sB[buf][0][sc_col][sc_k + 4]) = make_uint2(nh.z, nh.w);
+        *reinterpret_cast<uint2 *>(&sB[buf][1][sc_col][sc_k]) = make_uint2(nl.x, nl.y);
+        *reinterpret_cast<uint2 *>(&sB[buf][1][sc_col][sc_k + 4]) = make_uint2(nl.z, nl.w);
+        __syncthreads();
+        if (k0 + kMxK < b.y) {
+            nh = *reinterpret_cast<const uint4 *>(ihi + src + k0 + kMxK);
+            nl = *reinterpret_cast<const uint4 *>(ilo + src + k0 + kMxK);
+        }
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-        const int col = j0 + 16 * nb + (lane & 15);
-        const float sc = iscale[col] * (1.0f / kPScale);
-        float sum = 0.0f;
+        for (int kk = 0; kk < kMxK; kk += 16) {
+            if (k0 + kk >= b.y) break;
+            typename Q::T ah[2], al[2];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int row = i0 + kq + e;
-            if (row < P.np) {
-                const float Y = acc[nb][e] * sc;
-                const float2 o = yw[row];
-                if (yout && col < tpad) yout[(size_t)col * P.np + row] = (double)Y;
-                if (o.y > 0.0f) {
-                    const float r = o.x - Y;
-                    sum = fmaf(o.y * r, r, sum);
+            for (int rb = 0; rb < 2; ++rb) {
+                const size_t a = (size_t)(i0 + 16 * rb + li) * kpad + k0 + kk + kq;
+                ah[rb] = Q::from(*reinterpret_cast<const uint2 *>(phi + a));
+                al[rb] = Q::from(*reinterpret_cast<const uint2 *>(plo + a));
+            }
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const typename Q::T bh = Q::from(*reinterpret_cast<const uint2 *>(&sB[buf][0][16 * nb + li][kk + kq]));
+                const typename Q::T bl = Q::from(*reinterpret_cast<const uint2 *>(&sB[buf][1][16 * nb + li][kk + kq]));
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    acc[rb][nb] = Q::mfma(ah[rb], bh, acc[rb][nb]);
+                    acc[rb][nb] = Q::mfma(ah[rb], bl, acc[rb][nb]);
+                    acc[rb][nb] = Q::mfma(al[rb], bh, acc[rb][nb]);
                 }
             }
         }
+        buf ^= 1;
+    }
+    // epilogue: weighted residuals, summed over the 32 rows of this wavefront
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const int col = j0 + 16 * nb + li;
+        const float sc = iscale[col] * (1.0f / kPScale);
+        float sum = 0.0f;
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = i0 + 16 * rb + kq + e;
+                if (row < P.np) {
+                    const float Y = acc[rb][nb][e] * sc;
+                    const float2 o = yw[row];
+                    if (yout && col < tpad) yout[(size_t)col * P.np + row] = (double)Y;
+                    if (o.y > 0.0f) {
+                        const float r = o.x - Y;
+                        sum = fmaf(o.y * r, r, sum);
+                    }
+                }
+            }
         sum += __shfl_xor(sum, 16);
         sum += __shfl_xor(sum, 32);
         if (lane < 16) red[wave][16 * nb + lane] = sum;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const float s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-        partial[(size_t)rt * tpad + j0 + threadIdx.x] = s;
+    if (tid < 64) {
+        const float s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        partial[(size_t)rt * tpad + j0 + tid] = s;
     }
 }
 
@@ -1415,6 +1445,7 @@ hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const floa
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
                               float *partial, int kpad, int tpad, int row_tiles, double *yout, int bf16, hipStream_t s) {
+    // row_tiles: 128-point tiles
     if (bf16)
         hipLaunchKernelGGL(k_matrix_chi2<true>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
                            partial, kpad, tpad, yout);

@@ -243,11 +243,14 @@ typedef struct {
     uint32_t seed;       /* network initialisation and minibatch sampling               */
     float margin;        /* predicted log Rwp gain over the current vector to take a pick */
     float cold_fraction; /* screened share of the ladder, coldest chains first (0, 1]   */
+    float skip_margin;   /* log Rwp allowance taken off a prediction before a skip      */
+    float skip_eps;      /* skip the exact evaluation of a screened chain's pick when its
+                            predicted Metropolis acceptance is below skip_eps (0: never) */
 } pt_screen_config;
 int pt_screen(pt_plan *plan, const pt_screen_config *cfg);
 /* out[0] Adam steps taken, out[1] ring entries filled, out[2] running mean of the
  * minibatch loss (squared error in log Rwp, masked rows counted as zero), out[3] picks of
- * a candidate other than the plain proposal */
+ * a candidate other than the plain proposal, out[4] proposals skipped (not evaluated) */
 int pt_screen_stats(pt_plan *plan, double *out);
 /* log Rwp predicted by the plan's network for n vectors of nv doubles */
 int pt_screen_predict(pt_plan *plan, const double *x, int n, float *pred);

@@ -881,7 +881,7 @@ static int enqueue_screened_sweep(pt_plan *pl, hipEvent_t *ev) {
         HIPCHECK(ptk::screen_candidates(pl->P, pl->S, pl->stream), "screen");
         int rc = mlpx::infer(pl->mlp, pl->S.xs, pl->chains * (c.oversample + 1), pl->S.pred, pl->stream);
         if (rc) return rc;
-        HIPCHECK(ptk::screen_pick(pl->S, pl->chains, pl->mlp->iter, pl->stream), "screen");
+        HIPCHECK(ptk::screen_pick(pl->P, pl->S, pl->mlp->iter, pl->stream), "screen");
         HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, pl->chains, 1, 0, pl->stream, pl->S.pick), "setup");
         HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], pl->stream), "timing");
@@ -925,10 +925,10 @@ int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
     }
     const pt_screen_config c = *cfg;
     if (pl->nv > ptk::MLP_IN) return fail(PT_ERR_ARG, "PT surrogate screen supports at most 64 variables.");
-    if (c.oversample < 2 || c.oversample > 16) return fail(PT_ERR_ARG, "PT surrogate oversampling must be 2..16.");
+    if (c.oversample < 1 || c.oversample > 16) return fail(PT_ERR_ARG, "PT surrogate oversampling must be 1..16.");
     if (c.batch < 64 || c.batch > 4096 || c.batch % 64) return fail(PT_ERR_ARG, "PT surrogate batch must be a multiple of 64 in 64..4096.");
     if (c.train_iters < 0 || c.warmup_iters < 0 || !(c.lr > 0.0f) || !(c.margin >= 0.0f) || !(c.cold_fraction > 0.0f) ||
-        c.cold_fraction > 1.0f)
+        c.cold_fraction > 1.0f || !(c.skip_margin >= 0.0f) || c.skip_eps < 0.0f || c.skip_eps >= 1.0f)
         return fail(PT_ERR_ARG, "invalid surrogate training settings.");
     if (c.capacity < c.batch || c.capacity < pl->chains) return fail(PT_ERR_ARG, "PT surrogate ring must hold a batch and one step of every chain.");
     int rc = 0;
@@ -942,7 +942,7 @@ int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
             (rc = dalloc(pl, &S.cand, (size_t)pl->chains * K)) || (rc = dalloc(pl, &S.xs, (size_t)pl->chains * (K + 1) * ptk::MLP_IN)) ||
             (rc = dalloc(pl, &S.pred, (size_t)pl->chains * (K + 1))) || (rc = dalloc(pl, &S.pick, (size_t)pl->chains)))
             return rc;
-        if (!S.picked && (rc = dalloc(pl, &S.picked, 1))) return rc;
+        if (!S.picked && ((rc = dalloc(pl, &S.picked, 1)) || (rc = dalloc(pl, &S.skipped, 1)))) return rc;
         if (!S.center) {
             std::vector<double> ctr(pl->nv), ih(pl->nv);
             for (int j = 0; j < pl->nv; ++j) {
@@ -968,6 +968,9 @@ int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
         S.cap = c.capacity;
     }
     HIPCHECK(hipMemset(pl->S.picked, 0, sizeof(unsigned)), "screen setup");
+    HIPCHECK(hipMemset(pl->S.skipped, 0, sizeof(unsigned)), "screen setup");
+    pl->S.skip_margin = c.skip_margin;
+    pl->S.skip_log = c.skip_eps > 0.0f ? (float)-std::log((double)c.skip_eps) : 0.0f;
     pl->S.warmup_iters = c.warmup_iters;
     pl->S.log_wss = pl->log_wss;
     pl->S.margin = c.margin;
@@ -980,22 +983,24 @@ int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
 int pt_screen_stats(pt_plan *pl, double *out) {
     if (!pl || !out) return fail(PT_ERR_ARG, "NULL argument");
     NEED_DEVICE(pl);
-    for (int i = 0; i < 4; ++i) out[i] = 0;
+    for (int i = 0; i < 5; ++i) out[i] = 0;
     if (!pl->mlp) return 0;
     (void)hipSetDevice(pl->device);
     int it = 0;
     float loss[2] = {0, 0};
-    unsigned picked = 0;
+    unsigned picked = 0, skipped = 0;
     double step = 0;
     HIPCHECK(hipStreamSynchronize(pl->stream), "screen statistics");
     HIPCHECK(hipMemcpy(&it, pl->mlp->iter, sizeof(int), hipMemcpyDeviceToHost), "screen statistics");
     HIPCHECK(hipMemcpy(loss, pl->mlp->loss, 2 * sizeof(float), hipMemcpyDeviceToHost), "screen statistics");
     if (pl->S.picked) HIPCHECK(hipMemcpy(&picked, pl->S.picked, sizeof(unsigned), hipMemcpyDeviceToHost), "screen statistics");
+    if (pl->S.skipped) HIPCHECK(hipMemcpy(&skipped, pl->S.skipped, sizeof(unsigned), hipMemcpyDeviceToHost), "screen statistics");
     HIPCHECK(hipMemcpy(&step, pl->P.state + 3 * pl->nv + PT_S_STEP, sizeof(double), hipMemcpyDeviceToHost), "screen statistics");
     out[0] = it;
     out[1] = std::min((double)pl->screen_cap, std::max(0.0, step) * pl->chains);
     out[2] = loss[1];
     out[3] = picked;
+    out[4] = skipped;
     return 0;
 }
 

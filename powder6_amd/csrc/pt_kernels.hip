@@ -508,7 +508,7 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Share
 __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
                                   Shared &sh, double *prop, const double2 *pick = nullptr) {
     const int nv = P.nv;
-    bool boot = false;
+    bool boot = false, skip = false;
     double *st = nullptr;
     if (pt_mode) {
         st = P.state + (size_t)t * P.stride;
@@ -524,7 +524,8 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
                 // the move chosen by the surrogate screen among candidates drawn from
                 // the same stream (candidate 0 is the move below)
                 const double2 mv = pick[t];
-                prop[(int)mv.x] = mv.y;
+                if (mv.x < 0.0) skip = true;     // predicted hopeless: not evaluated, rejected
+                else prop[(int)mv.x] = mv.y;
             } else {
                 double u0 = uniform01(P.seed, t, ctr, 0), u1 = uniform01(P.seed, t, ctr, 1);
                 double u2 = uniform01(P.seed, t, ctr, 2);
@@ -547,7 +548,7 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
         bool go = sh.cell_ok;
         if (pt_mode) {
             // a feasible replica may not step outside the variance region (pt_gpu.js:361)
-            go = go && (boot || st[3 * nv + 11] < 0.5 || sh.prof_ok);
+            go = go && (boot || st[3 * nv + 11] < 0.5 || sh.prof_ok) && !skip;
             st[3 * nv + 7] = go ? 1.0 : 0.0;
         }
         sh.go = go ? 1 : 0;

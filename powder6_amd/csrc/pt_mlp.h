@@ -45,8 +45,11 @@ struct ScreenDev {
     float *pred;                       // chains * (K + 1) predictions
     double2 *pick;                     // chosen move per chain
     unsigned *picked;                  // count of picks other than the plain proposal
+    unsigned *skipped;                 // count of proposals not evaluated (predicted hopeless)
     double log_wss;                    // log sum w y_obs^2
     float margin;                      // predicted log Rwp gain a pick must show
+    float skip_margin;                 // log Rwp allowance taken off a prediction before a skip
+    float skip_log;                    // skip when the predicted acceptance < exp(-skip_log); 0 off
     int cold_chains;                   // chains [0, cold_chains) are screened and recorded
     int cap, K, warmup_iters, pad;
 };
@@ -69,7 +72,7 @@ hipError_t mlp_gather(const uint16_t *ring_x, const float *ring_t, int cap, cons
                       hipStream_t s);
 hipError_t screen_candidates(const DevPlan &P, const ScreenDev &S, hipStream_t s);
 hipError_t screen_inputs(const double *x, int n, int nv, const ScreenDev &S, uint16_t *xs, hipStream_t s);
-hipError_t screen_pick(const ScreenDev &S, int chains, const int *iter, hipStream_t s);
+hipError_t screen_pick(const DevPlan &P, const ScreenDev &S, const int *iter, hipStream_t s);
 hipError_t screen_record(const DevPlan &P, const ScreenDev &S, hipStream_t s);
 
 }  // namespace ptk

@@ -361,21 +361,33 @@ __global__ void __launch_bounds__(64) k_screen_inputs(const double *__restrict__
 // per chain: once the network has trained warmup_iters iterations, a screened (cold)
 // chain takes the candidate with the lowest predicted log Rwp if that beats the
 // prediction at its current vector by the margin; otherwise, and for every other chain,
-// candidate 0 (the plain PT proposal)
-__global__ void __launch_bounds__(64) k_screen_pick(ScreenDev S, int chains, const int *iter) {
+// candidate 0 (the plain PT proposal).  With skipping on, a screened chain whose pick is
+// predicted hopeless -- even after skip_margin is taken off its predicted log Rwp rise d,
+// the Metropolis acceptance exp(-(chi2 (e^{2d} - 1)) / 2T) (rietveld_core.js:2213-2222)
+// stays below exp(-skip_log) -- is marked (move -1) and not evaluated.
+__global__ void __launch_bounds__(64) k_screen_pick(DevPlan P, ScreenDev S, const int *iter) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= chains) return;
+    if (c >= P.chains) return;
     const int R = S.K + 1;
     int best = 0;
+    bool skip = false;
     if (*iter >= S.warmup_iters && c < S.cold_chains) {
         const float *pr = S.pred + (size_t)c * R;
         int b = 0;
         for (int k = 1; k < S.K; ++k)
             if (pr[k] < pr[b]) b = k;
         if (pr[b] < pr[S.K] - S.margin) best = b;
+        const float d = pr[best] - pr[S.K] - S.skip_margin;
+        if (S.skip_log > 0.0f && d > 0.0f) {
+            const double *st = P.state + (size_t)c * P.stride;
+            const double cur = st[3 * P.nv];
+            const double temp = pow(P.max_temp, (double)c / (double)(P.chains > 1 ? P.chains - 1 : 1));
+            if (cur > 0.0 && cur < 3e38 && cur * expm1(2.0 * (double)d) > 2.0 * temp * (double)S.skip_log) skip = true;
+        }
     }
-    S.pick[c] = S.cand[(size_t)c * S.K + best];
-    if (best != 0) atomicAdd(S.picked, 1u);
+    S.pick[c] = skip ? make_double2(-1.0, 0.0) : S.cand[(size_t)c * S.K + best];
+    if (best != 0 && !skip) atomicAdd(S.picked, 1u);
+    if (skip) atomicAdd(S.skipped, 1u);
 }
 
 // after the Metropolis step: the trial just evaluated (the proposal, its exact chi^2) goes
@@ -386,7 +398,7 @@ __global__ void __launch_bounds__(64) k_screen_record(DevPlan P, ScreenDev S) {
     const int c = blockIdx.x, i = threadIdx.x, nv = P.nv;
     const double *st = P.state + (size_t)c * P.stride;
     const long long step = (long long)st[3 * nv + 2];
-    if (step < 1) return;
+    if (step < 1 || S.pick[c].x < 0.0) return;     // a skipped proposal has no evaluation
     const size_t pos = (size_t)(((step - 1) * P.chains + c) % S.cap);
     uint16_t u = 0;
     if (i < nv) u = screen_input(S, i, st[nv + i]);
@@ -462,8 +474,8 @@ hipError_t screen_inputs(const double *x, int n, int nv, const ScreenDev &S, uin
     if (n > 0) hipLaunchKernelGGL(k_screen_inputs, dim3(n), dim3(64), 0, s, x, nv, S, xs);
     return hipGetLastError();
 }
-hipError_t screen_pick(const ScreenDev &S, int chains, const int *iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_screen_pick, dim3((chains + 63) / 64), dim3(64), 0, s, S, chains, iter);
+hipError_t screen_pick(const DevPlan &P, const ScreenDev &S, const int *iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_screen_pick, dim3((P.chains + 63) / 64), dim3(64), 0, s, P, S, iter);
     return hipGetLastError();
 }
 hipError_t screen_record(const DevPlan &P, const ScreenDev &S, hipStream_t s) {

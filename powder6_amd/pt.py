@@ -320,9 +320,10 @@ class Plan:
         return out
 
     def screen_stats(self):
-        out = np.zeros(4)
+        out = np.zeros(5)
         _check(lib().pt_screen_stats(self.h, _dptr(out)))
-        return dict(iterations=int(out[0]), ring=int(out[1]), loss=float(out[2]), picked=int(out[3]))
+        return dict(iterations=int(out[0]), ring=int(out[1]), loss=float(out[2]), picked=int(out[3]),
+                    skipped=int(out[4]))
 
     def tiles(self, request=0):
         """Tile shape of the profile sum: request 0 reports, -1 retunes, 1/2/4 fixes it.
@@ -342,11 +343,12 @@ def hbm_copy_probe(nbytes=1 << 30, repeats=3):
 class PtScreenConfig(ctypes.Structure):
     _fields_ = [('oversample', ctypes.c_int), ('train_iters', ctypes.c_int), ('batch', ctypes.c_int),
                 ('warmup_iters', ctypes.c_int), ('capacity', ctypes.c_int), ('lr', ctypes.c_float),
-                ('seed', ctypes.c_uint32), ('margin', ctypes.c_float), ('cold_fraction', ctypes.c_float)]
+                ('seed', ctypes.c_uint32), ('margin', ctypes.c_float), ('cold_fraction', ctypes.c_float),
+                ('skip_margin', ctypes.c_float), ('skip_eps', ctypes.c_float)]
 
 
 SCREEN_DEFAULTS = dict(oversample=4, train_iters=1, batch=512, warmup_iters=200, capacity=65536, lr=1e-3, seed=7,
-                       margin=0.0, cold_fraction=1.0)
+                       margin=0.0, cold_fraction=1.0, skip_margin=0.0, skip_eps=0.0)
 
 
 def _fptr(a):

@@ -106,3 +106,40 @@ def test_trained_network_ranks_trials_and_pt_keeps_its_quality(mol):
           (sweeps, rwp(plain_best), rwp(screened_best)))
     assert trained > 0.8 and trained > untrained + 0.3
     assert rwp(screened_best) <= 1.02 * rwp(plain_best)
+
+
+def test_skip_screening_saves_exact_evaluations_at_equal_quality(mol):
+    """Skip mode: a chain's proposal whose predicted Metropolis acceptance (after a margin
+    in log Rwp) is below skip_eps is rejected without an exact evaluation.  From a far start
+    the skipping run reaches a best Rwp within 2% of plain PT's with fewer exact
+    evaluations, and every recorded best is still the exact chi^2 of its vector."""
+    x0 = _far_start(mol, 5)
+    wss = float(np.sum(mol.w * mol.y ** 2))
+    sweeps = 4
+    nv, C = mol.nv, mol.chains
+    cfg = dict(oversample=1, warmup_iters=50, batch=256, capacity=16384, cold_fraction=1.0,
+               skip_eps=1e-2, skip_margin=0.02)
+    with pt.Plan(mol) as pl:
+        pl.screen(**cfg)
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        st = pl.batch(sweeps).reshape(C, -1)
+        stats = pl.screen_stats()
+    with pt.Plan(mol) as pl:
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        plain = pl.batch(sweeps).reshape(C, -1)
+    proposals = float(np.sum(st[:, 3 * nv + 4]))
+    rwp = lambda c: 100 * np.sqrt(c / wss)
+    skip_best, plain_best = float(np.min(st[:, 3 * nv + 1])), float(np.min(plain[:, 3 * nv + 1]))
+    print('skip stats %s; exact evaluations %.1f%% of proposals; best Rwp plain %.3f%%, skipping %.3f%%' %
+          (stats, 100 * (1 - stats['skipped'] / proposals), rwp(plain_best), rwp(skip_best)))
+    assert stats['picked'] == 0 and 0 < stats['skipped'] < proposals
+    assert np.all(st[:, 3 * nv + 4] == sweeps * nv)               # a skip still counts as a step
+    assert rwp(skip_best) <= 1.02 * rwp(plain_best)
+    m = O.Model(mol)
+    for c in range(0, C, 16):
+        best = st[c, 3 * nv + 1]
+        if best < 1e38:
+            ref = m.cost(st[c, 2 * nv:3 * nv])
+            assert abs(best - ref) <= 1e-5 * max(1.0, ref), (c, best, ref)

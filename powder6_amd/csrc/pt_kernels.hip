@@ -622,7 +622,7 @@ __global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, TrialImage *img,
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
-constexpr int kStage = 384;            // line records staged in LDS per pass
+constexpr int kStage = 256;            // line records staged in LDS per pass (16 KB: 8 waves per SIMD)
 
 // points per accumulate workgroup for TPW 128-point tiles per wavefront (two points per lane
 // and tile); TPW is the tile shape the autotuner picks (1, 2 or 4)
@@ -647,9 +647,20 @@ __device__ __forceinline__ float line_point(const LineRec &R, bool split, float 
 // flag tests).  a and r are the record's first 32 bytes, already in registers; the left
 // flank is read from the staged record L only for a line whose flanks differ.  Otherwise
 // `split`, `has_asym` and `asym` are uniform over the wavefront.
-template <bool INSIDE, bool PLAIN>
-__device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, const LineRec &L, int4 v, bool split,
-                                          bool has_asym, float asym, float2 t0, float2 t1, int i0, float &a0, float &a1) {
+// a line record staged in LDS together with its window
+struct StagedLine {
+    float4 a, r, l;
+    int4 span;
+};
+
+// One staged line at a lane's two points t0, t1 (float-float 2theta).  No window test:
+// every profile here decreases away from its centre on each side (the asymmetry map
+// d / (1 - clamp(a d)) is increasing), and the pedestal is the larger of its two values
+// at the window edges, so at a point outside the window max(0, profile - pedestal) is
+// zero -- up to the rounding of the pedestal itself, as at the window edge.
+template <bool PLAIN>
+__device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, const StagedLine &L, bool split, bool has_asym,
+                                          float asym, float2 t0, float2 t1, float &a0, float &a1) {
     float d0 = (t0.x - ra.x) + (t0.y - ra.y);
     float d1 = (t1.x - ra.x) + (t1.y - ra.y);
     if (!PLAIN && has_asym) {
@@ -667,16 +678,13 @@ __device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, co
     const float l0 = __builtin_amdgcn_rcpf(fmaf(f0.y, q0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(f1.y, q1, 1.0f));
     const float v0 = fmaxf(fmaf(f0.z, l0, fmaf(f0.w, g0, -ra.w)), 0.0f);
     const float v1 = fmaxf(fmaf(f1.z, l1, fmaf(f1.w, g1, -ra.w)), 0.0f);
-    const float amp0 = (INSIDE || (i0 >= v.x && i0 < v.y)) ? ra.z : 0.0f;
-    const float amp1 = (INSIDE || (i0 + 64 >= v.x && i0 + 64 < v.y)) ? ra.z : 0.0f;
-    a0 = fmaf(amp0, v0, a0);
-    a1 = fmaf(amp1, v1, a1);
+    a0 = fmaf(ra.z, v0, a0);
+    a1 = fmaf(ra.z, v1, a1);
 }
 
 // LDS of one accumulate workgroup
 struct AccShared {
-    LineRec rec[kStage];
-    int4 span[kStage];
+    StagedLine line[kStage];
     float bkg[PTK_MAX_BKG_F];
     int wcnt[kAccWaves];
     int count;
@@ -734,7 +742,10 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
         int off = S.count;
         for (int w = 0; w < wave; ++w) off += S.wcnt[w];
         const int pre = __popcll(m & ((1ull << lane) - 1ull));
-        if (ov) { S.span[off + pre] = sv; S.rec[off + pre] = rec[L]; }
+        if (ov) {
+            const LineRec R = rec[L];
+            S.line[off + pre] = StagedLine{R.a, R.r, R.l, sv};
+        }
         __syncthreads();
         if (tid == 0) { int n = S.count; for (int w = 0; w < kAccWaves; ++w) n += S.wcnt[w]; S.count = n; }
         __syncthreads();
@@ -744,29 +755,31 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
             for (int j = 0; j < TPW; ++j) {
                 const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
                 if (t_lo < hi_pt) {
-                    const int t_hi = t_lo + 128, i0 = t_lo + lane;
+                    const int t_hi = t_lo + 128;
                     float a0 = acc[2 * j], a1 = acc[2 * j + 1];
                     for (int lb = 0; lb < count; lb += 64) {
+                        // per staged line, as bit masks over the 64 lines of this block (scalar
+                        // registers): meets the tile, plain (equal flanks, no asymmetry), split
+                        // flanks, asymmetric
                         const int q = lb + lane;
-                        bool o = false;
-                        if (q < count) { int4 v = S.span[q]; o = v.x < t_hi && v.y > t_lo; }
+                        bool o = false, pl = false, sf = false, af = false;
+                        if (q < count) {
+                            const int4 v = S.line[q].span;
+                            o = v.x < t_hi && v.y > t_lo;
+                            pl = v.w == 0;
+                            sf = (v.w & 1) != 0;
+                            af = (v.w & 2) != 0;
+                        }
                         unsigned long long mask = __ballot(o);
-                        // one staged line at this wavefront's tile; flags and window are the
-                        // same for every lane, so they live in scalar registers
-                        auto line = [&](const int4 &v, const float4 &ra, const float4 &rr, const LineRec &R) {
-                            const int flags = __builtin_amdgcn_readfirstlane(v.w);
-                            const int ws = __builtin_amdgcn_readfirstlane(v.x), we = __builtin_amdgcn_readfirstlane(v.y);
-                            const bool inside = ws <= t_lo && we >= t_hi;   // the window covers the tile
-                            if (flags == 0) {
-                                if (inside)
-                                    line_pair<true, true>(ra, rr, R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
-                                else
-                                    line_pair<false, true>(ra, rr, R, v, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], i0, a0, a1);
+                        const unsigned long long m_plain = __ballot(pl), m_split = __ballot(sf), m_asym = __ballot(af);
+                        auto line = [&](int b, const int4 &v, const float4 &ra, const float4 &rr) {
+                            const StagedLine &L = S.line[lb + b];
+                            if ((m_plain >> b) & 1ull) {
+                                line_pair<true>(ra, rr, L, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], a0, a1);
                             } else {
-                                const bool has_asym = (flags & 2) != 0;
+                                const bool has_asym = (m_asym >> b) & 1ull;
                                 const float asym = has_asym ? __int_as_float(__builtin_amdgcn_readfirstlane(v.z)) : 0.0f;
-                                line_pair<false, false>(ra, rr, R, v, (flags & 1) != 0, has_asym, asym, tf[2 * j], tf[2 * j + 1], i0,
-                                                        a0, a1);
+                                line_pair<false>(ra, rr, L, (m_split >> b) & 1ull, has_asym, asym, tf[2 * j], tf[2 * j + 1], a0, a1);
                             }
                         };
                         // two lines per iteration, in line order: the second line's LDS reads
@@ -774,17 +787,17 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                         while (mask) {
                             const int b1 = __ffsll((long long)mask) - 1;
                             mask &= mask - 1ull;
-                            const int4 v1 = S.span[lb + b1];
-                            const float4 a1r = S.rec[lb + b1].a, r1r = S.rec[lb + b1].r;
+                            const int4 v1 = S.line[lb + b1].span;
+                            const float4 a1r = S.line[lb + b1].a, r1r = S.line[lb + b1].r;
                             if (mask) {
                                 const int b2 = __ffsll((long long)mask) - 1;
                                 mask &= mask - 1ull;
-                                const int4 v2 = S.span[lb + b2];
-                                const float4 a2r = S.rec[lb + b2].a, r2r = S.rec[lb + b2].r;
-                                line(v1, a1r, r1r, S.rec[lb + b1]);
-                                line(v2, a2r, r2r, S.rec[lb + b2]);
+                                const int4 v2 = S.line[lb + b2].span;
+                                const float4 a2r = S.line[lb + b2].a, r2r = S.line[lb + b2].r;
+                                line(b1, v1, a1r, r1r);
+                                line(b2, v2, a2r, r2r);
                             } else {
-                                line(v1, a1r, r1r, S.rec[lb + b1]);
+                                line(b1, v1, a1r, r1r);
                             }
                         }
                     }

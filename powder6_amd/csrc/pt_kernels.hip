@@ -642,27 +642,67 @@ __device__ __forceinline__ float line_point(const LineRec &R, bool split, float 
     return inwin ? R.a.z * v : 0.0f;
 }
 
-// one line at the two points (i, i + 64) of a lane.  INSIDE: the window covers the lane's
-// whole tile (no window test); PLAIN: equal flanks and no asymmetry (the common case, no
-// flag tests).  a and r are the record's first 32 bytes, already in registers; the left
-// flank is read from the staged record L only for a line whose flanks differ.  Otherwise
-// `split`, `has_asym` and `asym` are uniform over the wavefront.
-// a line record staged in LDS together with its window
+// A line record staged in LDS together with its window.  On a uniform grid the staging
+// rewrites the centre relative to the nearest grid point k: a = (k, c - t_k, height,
+// pedestal), and a plain line (equal flanks, no asymmetry) gets its profile in scaled
+// form, l = (step s, (c - t_k) s, -B, 0) with s = sqrt(-r.x) and B = r.y / s^2, so that
+// u = (i - k) step s - (c - t_k) s is the Gaussian's argument and -u^2 serves both terms.
 struct StagedLine {
     float4 a, r, l;
     int4 span;
 };
 
-// One staged line at a lane's two points t0, t1 (float-float 2theta).  No window test:
-// every profile here decreases away from its centre on each side (the asymmetry map
-// d / (1 - clamp(a d)) is increasing), and the pedestal is the larger of its two values
-// at the window edges, so at a point outside the window max(0, profile - pedestal) is
-// zero -- up to the rounding of the pedestal itself, as at the window edge.
-template <bool PLAIN>
+// span.w bit of a staged line that must take the general path
+constexpr int kGeneralLine = 4;
+
+__device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec &R, int4 v, bool uni) {
+    if (!uni) return StagedLine{R.a, R.r, R.l, v};
+    const double c = (double)R.a.x + (double)R.a.y;
+    const double k = rint((c - P.t0) / P.step);
+    const double off = c - (P.t0 + k * P.step);
+    StagedLine o{make_float4((float)k, (float)off, R.a.z, R.a.w), R.r, R.l, v};
+    if (v.w == 0) {
+        if (R.r.x < 0.0f) {
+            const double sc = sqrt(-(double)R.r.x);
+            o.l = make_float4((float)(P.step * sc), (float)(off * sc), R.r.y / R.r.x, 0.0f);
+        } else {
+            o.span.w = kGeneralLine;
+        }
+    }
+    return o;
+}
+
+// One staged line at a lane's two points (indices i0 = fi0, i1 = fi1; float-float 2theta
+// t0, t1).  No window test: every profile here decreases away from its centre on each
+// side (the asymmetry map d / (1 - clamp(a d)) is increasing), and the pedestal is the
+// larger of its two values at the window edges, so at a point outside the window
+// max(0, profile - pedestal) is zero -- up to the rounding of the pedestal itself, as at
+// the window edge.  The profile is at most 1 at the centre (its two weights sum to 1), so
+// the clip is a clamp to [0, 1].
+template <bool UNI, bool PLAIN>
 __device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, const StagedLine &L, bool split, bool has_asym,
-                                          float asym, float2 t0, float2 t1, float &a0, float &a1) {
-    float d0 = (t0.x - ra.x) + (t0.y - ra.y);
-    float d1 = (t1.x - ra.x) + (t1.y - ra.y);
+                                          float asym, float2 t0, float2 t1, float fi0, float fi1, float step, float &a0,
+                                          float &a1) {
+    if (UNI && PLAIN) {
+        const float4 sl = L.l;
+        const float u0 = fmaf(fi0 - ra.x, sl.x, -sl.y), u1 = fmaf(fi1 - ra.x, sl.x, -sl.y);
+        const float n0 = u0 * -u0, n1 = u1 * -u1;
+        const float g0 = __builtin_amdgcn_exp2f(n0), g1 = __builtin_amdgcn_exp2f(n1);
+        const float l0 = __builtin_amdgcn_rcpf(fmaf(sl.z, n0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(sl.z, n1, 1.0f));
+        const float v0 = __builtin_amdgcn_fmed3f(fmaf(rr.z, l0, fmaf(rr.w, g0, -ra.w)), 0.0f, 1.0f);
+        const float v1 = __builtin_amdgcn_fmed3f(fmaf(rr.z, l1, fmaf(rr.w, g1, -ra.w)), 0.0f, 1.0f);
+        a0 = fmaf(ra.z, v0, a0);
+        a1 = fmaf(ra.z, v1, a1);
+        return;
+    }
+    float d0, d1;
+    if (UNI) {
+        d0 = fmaf(fi0 - ra.x, step, -ra.y);
+        d1 = fmaf(fi1 - ra.x, step, -ra.y);
+    } else {
+        d0 = (t0.x - ra.x) + (t0.y - ra.y);
+        d1 = (t1.x - ra.x) + (t1.y - ra.y);
+    }
     if (!PLAIN && has_asym) {
         if (fabsf(d0) >= 1e-9f) d0 = d0 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d0)));
         if (fabsf(d1) >= 1e-9f) d1 = d1 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d1)));
@@ -676,10 +716,43 @@ __device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, co
     const float q0 = d0 * d0, q1 = d1 * d1;
     const float g0 = __builtin_amdgcn_exp2f(f0.x * q0), g1 = __builtin_amdgcn_exp2f(f1.x * q1);
     const float l0 = __builtin_amdgcn_rcpf(fmaf(f0.y, q0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(f1.y, q1, 1.0f));
-    const float v0 = fmaxf(fmaf(f0.z, l0, fmaf(f0.w, g0, -ra.w)), 0.0f);
-    const float v1 = fmaxf(fmaf(f1.z, l1, fmaf(f1.w, g1, -ra.w)), 0.0f);
+    const float v0 = __builtin_amdgcn_fmed3f(fmaf(f0.z, l0, fmaf(f0.w, g0, -ra.w)), 0.0f, 1.0f);
+    const float v1 = __builtin_amdgcn_fmed3f(fmaf(f1.z, l1, fmaf(f1.w, g1, -ra.w)), 0.0f, 1.0f);
     a0 = fmaf(ra.z, v0, a0);
     a1 = fmaf(ra.z, v1, a1);
+}
+
+// Two plain staged lines at a lane's two points, written out together so that the two
+// lines' work interleaves (line 1 is added before line 2 at each point).
+template <bool UNI>
+__device__ __forceinline__ void plain_two(const float4 &a1, const float4 &r1, const float4 &l1, const float4 &a2, const float4 &r2,
+                                          const float4 &l2, float2 t0, float2 t1, float fi0, float fi1, float &acc0, float &acc1) {
+    float n[4], c[4];
+    if (UNI) {
+        const float u0 = fmaf(fi0 - a1.x, l1.x, -l1.y), u1 = fmaf(fi1 - a1.x, l1.x, -l1.y);
+        const float u2 = fmaf(fi0 - a2.x, l2.x, -l2.y), u3 = fmaf(fi1 - a2.x, l2.x, -l2.y);
+        n[0] = u0 * -u0; n[1] = u1 * -u1; n[2] = u2 * -u2; n[3] = u3 * -u3;
+        c[0] = c[1] = l1.z; c[2] = c[3] = l2.z;
+    } else {
+        const float d0 = (t0.x - a1.x) + (t0.y - a1.y), d1 = (t1.x - a1.x) + (t1.y - a1.y);
+        const float d2 = (t0.x - a2.x) + (t0.y - a2.y), d3 = (t1.x - a2.x) + (t1.y - a2.y);
+        // in the float-float form the Gaussian and Lorentzian factors stay separate
+        n[0] = d0 * d0; n[1] = d1 * d1; n[2] = d2 * d2; n[3] = d3 * d3;
+        c[0] = c[1] = r1.y; c[2] = c[3] = r2.y;
+    }
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 &a = k < 2 ? a1 : a2;
+        const float4 &r = k < 2 ? r1 : r2;
+        const float g = __builtin_amdgcn_exp2f(UNI ? n[k] : r.x * n[k]);
+        const float l = __builtin_amdgcn_rcpf(fmaf(c[k], n[k], 1.0f));
+        v[k] = __builtin_amdgcn_fmed3f(fmaf(r.z, l, fmaf(r.w, g, -a.w)), 0.0f, 1.0f);
+    }
+    acc0 = fmaf(a1.z, v[0], acc0);
+    acc1 = fmaf(a1.z, v[1], acc1);
+    acc0 = fmaf(a2.z, v[2], acc0);
+    acc1 = fmaf(a2.z, v[3], acc1);
 }
 
 // LDS of one accumulate workgroup
@@ -690,6 +763,59 @@ struct AccShared {
     int count;
     float red_s[4][kAccWaves], red_c[4][kAccWaves];
 };
+
+// The staged lines that meet one wavefront's 128-point tile [t_lo, t_lo + 128), summed
+// into the lane's two points (t_lo + lane, t_lo + lane + 64): per block of 64 staged
+// lines, first the plain ones (equal flanks, no asymmetry) in line order, then the others
+// in line order -- a fixed order, the same for every tile shape.
+template <bool UNI>
+__device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_lo, float2 t0, float2 t1, float step, float &a0,
+                                           float &a1) {
+    const int lane = threadIdx.x & 63, t_hi = t_lo + 128;
+    const float fi0 = (float)(t_lo + lane), fi1 = fi0 + 64.0f;
+    for (int lb = 0; lb < count; lb += 64) {
+        // per staged line, as bit masks over the 64 lines of this block (scalar registers):
+        // meets the tile and is plain; meets the tile and is not; split flanks; asymmetric
+        const int q = lb + lane;
+        bool o = false, pl = false, sf = false, af = false;
+        if (q < count) {
+            const int4 v = S.line[q].span;
+            o = v.x < t_hi && v.y > t_lo;
+            pl = v.w == 0;
+            sf = (v.w & 1) != 0;
+            af = (v.w & 2) != 0;
+        }
+        unsigned long long m_plain = __ballot(o && pl), m_gen = __ballot(o && !pl);
+        const unsigned long long m_split = __ballot(sf), m_asym = __ballot(af);
+        // plain lines, two per iteration (both lines' LDS reads are issued before either
+        // is computed)
+        int np = __popcll(m_plain);
+        for (; np >= 2; np -= 2) {
+            const int b1 = __ffsll((long long)m_plain) - 1;
+            m_plain &= m_plain - 1ull;
+            const int b2 = __ffsll((long long)m_plain) - 1;
+            m_plain &= m_plain - 1ull;
+            const StagedLine &L1 = S.line[lb + b1], &L2 = S.line[lb + b2];
+            const float4 a1r = L1.a, r1r = L1.r, l1r = UNI ? L1.l : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 a2r = L2.a, r2r = L2.r, l2r = UNI ? L2.l : make_float4(0.f, 0.f, 0.f, 0.f);
+            plain_two<UNI>(a1r, r1r, l1r, a2r, r2r, l2r, t0, t1, fi0, fi1, a0, a1);
+        }
+        if (np) {
+            const int b1 = __ffsll((long long)m_plain) - 1;
+            const StagedLine &L1 = S.line[lb + b1];
+            line_pair<UNI, true>(L1.a, L1.r, L1, false, false, 0.0f, t0, t1, fi0, fi1, step, a0, a1);
+        }
+        // the others (split flanks, asymmetry, or a profile that needs the general form)
+        while (m_gen) {
+            const int b = __ffsll((long long)m_gen) - 1;
+            m_gen &= m_gen - 1ull;
+            const StagedLine &L = S.line[lb + b];
+            const bool has_asym = (m_asym >> b) & 1ull;
+            const float asym = has_asym ? __int_as_float(__builtin_amdgcn_readfirstlane(L.span.z)) : 0.0f;
+            line_pair<UNI, false>(L.a, L.r, L, (m_split >> b) & 1ull, has_asym, asym, t0, t1, fi0, fi1, step, a0, a1);
+        }
+    }
+}
 
 // Banded profile sum and partial chi^2 of points [P s, P s + P) of trial t, P =
 // acc_points(TPW): the lines whose windows meet those points are compacted, in line
@@ -712,6 +838,7 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     const int nl = 2 * P.nr;
     const LineRec *rec = grec + (size_t)t * nl;
     const int4 *sp = gspan + (size_t)t * nl;
+    const bool uni = P.uniform != 0;
     // lines that can meet this workgroup's points: the union of its groups' ranges
     int first = 0x7fffffff, last = -1;
     for (int g = split * TPW; g < min(nsplit, split * TPW + TPW); ++g) {
@@ -725,8 +852,8 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     for (int j = 0; j < TPW; ++j) {
         const int base = lo_pt + (wave + kAccWaves * j) * 128 + lane;
         acc[2 * j] = acc[2 * j + 1] = 0.0f;
-        tf[2 * j] = base < P.np ? P.tff[base] : make_float2(0.f, 0.f);
-        tf[2 * j + 1] = base + 64 < P.np ? P.tff[base + 64] : make_float2(0.f, 0.f);
+        tf[2 * j] = !uni && base < P.np ? P.tff[base] : make_float2(0.f, 0.f);
+        tf[2 * j + 1] = !uni && base + 64 < P.np ? P.tff[base + 64] : make_float2(0.f, 0.f);
     }
     if (tid == 0) S.count = 0;
     __syncthreads();
@@ -742,10 +869,7 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
         int off = S.count;
         for (int w = 0; w < wave; ++w) off += S.wcnt[w];
         const int pre = __popcll(m & ((1ull << lane) - 1ull));
-        if (ov) {
-            const LineRec R = rec[L];
-            S.line[off + pre] = StagedLine{R.a, R.r, R.l, sv};
-        }
+        if (ov) S.line[off + pre] = stage_line(P, rec[L], sv, uni);
         __syncthreads();
         if (tid == 0) { int n = S.count; for (int w = 0; w < kAccWaves; ++w) n += S.wcnt[w]; S.count = n; }
         __syncthreads();
@@ -755,52 +879,9 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
             for (int j = 0; j < TPW; ++j) {
                 const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
                 if (t_lo < hi_pt) {
-                    const int t_hi = t_lo + 128;
                     float a0 = acc[2 * j], a1 = acc[2 * j + 1];
-                    for (int lb = 0; lb < count; lb += 64) {
-                        // per staged line, as bit masks over the 64 lines of this block (scalar
-                        // registers): meets the tile, plain (equal flanks, no asymmetry), split
-                        // flanks, asymmetric
-                        const int q = lb + lane;
-                        bool o = false, pl = false, sf = false, af = false;
-                        if (q < count) {
-                            const int4 v = S.line[q].span;
-                            o = v.x < t_hi && v.y > t_lo;
-                            pl = v.w == 0;
-                            sf = (v.w & 1) != 0;
-                            af = (v.w & 2) != 0;
-                        }
-                        unsigned long long mask = __ballot(o);
-                        const unsigned long long m_plain = __ballot(pl), m_split = __ballot(sf), m_asym = __ballot(af);
-                        auto line = [&](int b, const int4 &v, const float4 &ra, const float4 &rr) {
-                            const StagedLine &L = S.line[lb + b];
-                            if ((m_plain >> b) & 1ull) {
-                                line_pair<true>(ra, rr, L, false, false, 0.0f, tf[2 * j], tf[2 * j + 1], a0, a1);
-                            } else {
-                                const bool has_asym = (m_asym >> b) & 1ull;
-                                const float asym = has_asym ? __int_as_float(__builtin_amdgcn_readfirstlane(v.z)) : 0.0f;
-                                line_pair<false>(ra, rr, L, (m_split >> b) & 1ull, has_asym, asym, tf[2 * j], tf[2 * j + 1], a0, a1);
-                            }
-                        };
-                        // two lines per iteration, in line order: the second line's LDS reads
-                        // are issued before the first line is computed
-                        while (mask) {
-                            const int b1 = __ffsll((long long)mask) - 1;
-                            mask &= mask - 1ull;
-                            const int4 v1 = S.line[lb + b1].span;
-                            const float4 a1r = S.line[lb + b1].a, r1r = S.line[lb + b1].r;
-                            if (mask) {
-                                const int b2 = __ffsll((long long)mask) - 1;
-                                mask &= mask - 1ull;
-                                const int4 v2 = S.line[lb + b2].span;
-                                const float4 a2r = S.line[lb + b2].a, r2r = S.line[lb + b2].r;
-                                line(b1, v1, a1r, r1r);
-                                line(b2, v2, a2r, r2r);
-                            } else {
-                                line(b1, v1, a1r, r1r);
-                            }
-                        }
-                    }
+                    if (uni) tile_lines<true>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, a0, a1);
+                    else tile_lines<false>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, a0, a1);
                     acc[2 * j] = a0;
                     acc[2 * j + 1] = a1;
                 }

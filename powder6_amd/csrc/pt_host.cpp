@@ -544,10 +544,13 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         if (std::fabs(pb->tth[i] - (t0 + i * step)) > 1e-9 * step) uniform = 0;
     pl->uniform = uniform;
     std::vector<float2> tff(n), obs(n);
+    std::vector<float> zcheb(n);
+    const double tlo = pb->tth[0], thi = pb->tth[n - 1];
     for (int i = 0; i < n; ++i) {
         float hi = (float)pb->tth[i];
         tff[i] = make_float2(hi, (float)(pb->tth[i] - (double)hi));
         obs[i] = make_float2((float)pb->y_obs[i], (float)pb->weight[i]);
+        zcheb[i] = (float)(thi > tlo ? 2.0 * (pb->tth[i] - tlo) / (thi - tlo) - 1.0 : 0.0);
     }
     // CSR of the links by parameter
     std::vector<int> term_off(pb->n_params + 1, 0), term_var;
@@ -608,9 +611,10 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     }
     std::vector<double> vlo(pb->lo, pb->lo + pb->n_vars), vhi(pb->hi, pb->hi + pb->n_vars), vsc(pb->scale, pb->scale + pb->n_vars);
     if (eq.empty()) eq.push_back(0);
-    double *d_tth; float2 *d_tff, *d_obs; int4 *d_hklm; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
+    double *d_tth; float2 *d_tff, *d_obs; float *d_zcheb; int4 *d_hklm; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
     short *d_eq; double *d_rep_t, *d_base, *d_coeff, *d_lo, *d_hi, *d_sc, *d_state; ptk::DevAtom *d_atoms;
     if ((rc = upload(pl, &d_tth, tth)) || (rc = upload(pl, &d_tff, tff)) || (rc = upload(pl, &d_obs, obs)) ||
+        (rc = upload(pl, &d_zcheb, zcheb)) ||
         (rc = upload(pl, &d_hklm, hklm)) || (rc = upload(pl, &d_eq_off, eq_off)) || (rc = upload(pl, &d_eq, eq)) ||
         (rc = upload(pl, &d_rep_r, rep_r)) || (rc = upload(pl, &d_rep_t, rep_t)) || (rc = upload(pl, &d_atoms, atoms)) ||
         (rc = upload(pl, &d_base, base)) || (rc = upload(pl, &d_term_off, term_off)) || (rc = upload(pl, &d_term_var, term_var)) ||
@@ -619,7 +623,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         pt_plan_destroy(pl);
         return rc;
     }
-    P.tth = d_tth; P.tff = d_tff; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
+    P.tth = d_tth; P.tff = d_tff; P.zcheb = d_zcheb; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
     P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
     P.term_var = d_term_var; P.term_coeff = d_coeff; P.vlo = d_lo; P.vhi = d_hi; P.vscale = d_sc; P.state = d_state;
 

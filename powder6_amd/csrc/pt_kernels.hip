@@ -328,7 +328,7 @@ __device__ __forceinline__ void neumaier(float &s, float &c, float v) {
 // Chebyshev background at point i (rietveld_core.js:647-653, 920-931), FP32 recurrence
 __device__ __forceinline__ float background_at(const DevPlan &P, const Shared &sh, int i) {
     if (P.n_bkg <= 0) return 0.0f;
-    float z = (float)(P.thi > P.tlo ? 2.0 * (P.tth[i] - P.tlo) / (P.thi - P.tlo) - 1.0 : 0.0);
+    const float z = P.zcheb[i];
     float a = 1.0f, b = z, y = sh.bkg[0];
     if (P.n_bkg > 1) y = fmaf(sh.bkg[1], z, y);
     for (int k = 2; k < P.n_bkg; ++k) {
@@ -657,13 +657,14 @@ constexpr int kGeneralLine = 4;
 
 __device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec &R, int4 v, bool uni) {
     if (!uni) return StagedLine{R.a, R.r, R.l, v};
+    // k need not be the nearest grid point: any k within a few steps keeps c - t_k small
     const double c = (double)R.a.x + (double)R.a.y;
-    const double k = rint((c - P.t0) / P.step);
+    const double k = (double)rintf((float)(c - P.t0) * (1.0f / P.step_hi));
     const double off = c - (P.t0 + k * P.step);
     StagedLine o{make_float4((float)k, (float)off, R.a.z, R.a.w), R.r, R.l, v};
     if (v.w == 0) {
         if (R.r.x < 0.0f) {
-            const double sc = sqrt(-(double)R.r.x);
+            const double sc = (double)sqrtf(-R.r.x);
             o.l = make_float4((float)(P.step * sc), (float)(off * sc), R.r.y / R.r.x, 0.0f);
         } else {
             o.span.w = kGeneralLine;
@@ -672,29 +673,17 @@ __device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec
     return o;
 }
 
-// One staged line at a lane's two points (indices i0 = fi0, i1 = fi1; float-float 2theta
-// t0, t1).  No window test: every profile here decreases away from its centre on each
-// side (the asymmetry map d / (1 - clamp(a d)) is increasing), and the pedestal is the
-// larger of its two values at the window edges, so at a point outside the window
-// max(0, profile - pedestal) is zero -- up to the rounding of the pedestal itself, as at
-// the window edge.  The profile is at most 1 at the centre (its two weights sum to 1), so
-// the clip is a clamp to [0, 1].
-template <bool UNI, bool PLAIN>
-__device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, const StagedLine &L, bool split, bool has_asym,
-                                          float asym, float2 t0, float2 t1, float fi0, float fi1, float step, float &a0,
-                                          float &a1) {
-    if (UNI && PLAIN) {
-        const float4 sl = L.l;
-        const float u0 = fmaf(fi0 - ra.x, sl.x, -sl.y), u1 = fmaf(fi1 - ra.x, sl.x, -sl.y);
-        const float n0 = u0 * -u0, n1 = u1 * -u1;
-        const float g0 = __builtin_amdgcn_exp2f(n0), g1 = __builtin_amdgcn_exp2f(n1);
-        const float l0 = __builtin_amdgcn_rcpf(fmaf(sl.z, n0, 1.0f)), l1 = __builtin_amdgcn_rcpf(fmaf(sl.z, n1, 1.0f));
-        const float v0 = __builtin_amdgcn_fmed3f(fmaf(rr.z, l0, fmaf(rr.w, g0, -ra.w)), 0.0f, 1.0f);
-        const float v1 = __builtin_amdgcn_fmed3f(fmaf(rr.z, l1, fmaf(rr.w, g1, -ra.w)), 0.0f, 1.0f);
-        a0 = fmaf(ra.z, v0, a0);
-        a1 = fmaf(ra.z, v1, a1);
-        return;
-    }
+// One staged line of the general form (split flanks, asymmetry) at a lane's two points
+// (indices fi0, fi1; float-float 2theta t0, t1).  No window test: every profile here
+// decreases away from its centre on each side (the asymmetry map d / (1 - clamp(a d)) is
+// increasing), and the pedestal is the larger of its two values at the window edges, so
+// at a point outside the window max(0, profile - pedestal) is zero -- up to the rounding
+// of the pedestal itself, as at the window edge.  The profile is at most 1 at the centre
+// (its two weights sum to 1 on each flank), so the clip is a clamp to [0, 1].
+template <bool UNI>
+__device__ __forceinline__ void line_general(const float4 &ra, const float4 &rr, const StagedLine &L, bool split, bool has_asym,
+                                             float asym, float2 t0, float2 t1, float fi0, float fi1, float step, float &a0,
+                                             float &a1) {
     float d0, d1;
     if (UNI) {
         d0 = fmaf(fi0 - ra.x, step, -ra.y);
@@ -703,12 +692,12 @@ __device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, co
         d0 = (t0.x - ra.x) + (t0.y - ra.y);
         d1 = (t1.x - ra.x) + (t1.y - ra.y);
     }
-    if (!PLAIN && has_asym) {
+    if (has_asym) {
         if (fabsf(d0) >= 1e-9f) d0 = d0 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d0)));
         if (fabsf(d1) >= 1e-9f) d1 = d1 / (1.0f - fmaxf_js(-0.95f, fminf_js(0.95f, asym * d1)));
     }
     float4 f0 = rr, f1 = rr;
-    if (!PLAIN && split) {
+    if (split) {
         const float4 fl = L.l;
         if (d0 < 0.0f) f0 = fl;
         if (d1 < 0.0f) f1 = fl;
@@ -722,37 +711,23 @@ __device__ __forceinline__ void line_pair(const float4 &ra, const float4 &rr, co
     a1 = fmaf(ra.z, v1, a1);
 }
 
-// Two plain staged lines at a lane's two points, written out together so that the two
-// lines' work interleaves (line 1 is added before line 2 at each point).
+// A plain staged line (equal flanks, no asymmetry) at one point of a lane (index fi,
+// float-float 2theta t), before the line's height: clamp(cl L + cg G - pedestal, 0, 1).
 template <bool UNI>
-__device__ __forceinline__ void plain_two(const float4 &a1, const float4 &r1, const float4 &l1, const float4 &a2, const float4 &r2,
-                                          const float4 &l2, float2 t0, float2 t1, float fi0, float fi1, float &acc0, float &acc1) {
-    float n[4], c[4];
+__device__ __forceinline__ float plain_value(const float4 &a, const float4 &r, const float4 &l, float2 t, float fi) {
+    float n, c;
     if (UNI) {
-        const float u0 = fmaf(fi0 - a1.x, l1.x, -l1.y), u1 = fmaf(fi1 - a1.x, l1.x, -l1.y);
-        const float u2 = fmaf(fi0 - a2.x, l2.x, -l2.y), u3 = fmaf(fi1 - a2.x, l2.x, -l2.y);
-        n[0] = u0 * -u0; n[1] = u1 * -u1; n[2] = u2 * -u2; n[3] = u3 * -u3;
-        c[0] = c[1] = l1.z; c[2] = c[3] = l2.z;
+        const float u = fmaf(fi - a.x, l.x, -l.y);
+        n = u * -u;                     // exp2 argument of the Gaussian
+        c = l.z;
     } else {
-        const float d0 = (t0.x - a1.x) + (t0.y - a1.y), d1 = (t1.x - a1.x) + (t1.y - a1.y);
-        const float d2 = (t0.x - a2.x) + (t0.y - a2.y), d3 = (t1.x - a2.x) + (t1.y - a2.y);
-        // in the float-float form the Gaussian and Lorentzian factors stay separate
-        n[0] = d0 * d0; n[1] = d1 * d1; n[2] = d2 * d2; n[3] = d3 * d3;
-        c[0] = c[1] = r1.y; c[2] = c[3] = r2.y;
+        const float d = (t.x - a.x) + (t.y - a.y);
+        n = d * d;
+        c = r.y;
     }
-    float v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float4 &a = k < 2 ? a1 : a2;
-        const float4 &r = k < 2 ? r1 : r2;
-        const float g = __builtin_amdgcn_exp2f(UNI ? n[k] : r.x * n[k]);
-        const float l = __builtin_amdgcn_rcpf(fmaf(c[k], n[k], 1.0f));
-        v[k] = __builtin_amdgcn_fmed3f(fmaf(r.z, l, fmaf(r.w, g, -a.w)), 0.0f, 1.0f);
-    }
-    acc0 = fmaf(a1.z, v[0], acc0);
-    acc1 = fmaf(a1.z, v[1], acc1);
-    acc0 = fmaf(a2.z, v[2], acc0);
-    acc1 = fmaf(a2.z, v[3], acc1);
+    const float g = __builtin_amdgcn_exp2f(UNI ? n : r.x * n);
+    const float lz = __builtin_amdgcn_rcpf(fmaf(c, n, 1.0f));
+    return __builtin_amdgcn_fmed3f(fmaf(r.z, lz, fmaf(r.w, g, -a.w)), 0.0f, 1.0f);
 }
 
 // LDS of one accumulate workgroup
@@ -764,46 +739,84 @@ struct AccShared {
     float red_s[4][kAccWaves], red_c[4][kAccWaves];
 };
 
+// Per-point sums of one wavefront tile, one per class of (line, tile) pair: plain lines
+// that meet both 64-point halves of the tile, plain lines that meet one half, other lines.
+// Each class is summed in line order and the three are added at the end, so the order of
+// the additions does not depend on how lines fall into LDS passes or tile shapes.
+struct TileAcc {
+    float two0, two1, one0, one1, gen0, gen1;
+};
+
 // The staged lines that meet one wavefront's 128-point tile [t_lo, t_lo + 128), summed
-// into the lane's two points (t_lo + lane, t_lo + lane + 64): per block of 64 staged
-// lines, first the plain ones (equal flanks, no asymmetry) in line order, then the others
-// in line order -- a fixed order, the same for every tile shape.
+// into the lane's two points (t_lo + lane, t_lo + lane + 64).
 template <bool UNI>
-__device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_lo, float2 t0, float2 t1, float step, float &a0,
-                                           float &a1) {
-    const int lane = threadIdx.x & 63, t_hi = t_lo + 128;
+__device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_lo, float2 t0, float2 t1, float step, TileAcc &A) {
+    const int lane = threadIdx.x & 63, t_mid = t_lo + 64, t_hi = t_lo + 128;
     const float fi0 = (float)(t_lo + lane), fi1 = fi0 + 64.0f;
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int lb = 0; lb < count; lb += 64) {
-        // per staged line, as bit masks over the 64 lines of this block (scalar registers):
-        // meets the tile and is plain; meets the tile and is not; split flanks; asymmetric
+        // per staged line, as bit masks over the 64 lines of this block (scalar registers)
         const int q = lb + lane;
-        bool o = false, pl = false, sf = false, af = false;
+        bool oa = false, ob = false, pl = false, sf = false, af = false;
         if (q < count) {
             const int4 v = S.line[q].span;
-            o = v.x < t_hi && v.y > t_lo;
+            oa = v.x < t_mid && v.y > t_lo;
+            ob = v.x < t_hi && v.y > t_mid;
             pl = v.w == 0;
             sf = (v.w & 1) != 0;
             af = (v.w & 2) != 0;
         }
-        unsigned long long m_plain = __ballot(o && pl), m_gen = __ballot(o && !pl);
+        unsigned long long m_two = __ballot(pl && oa && ob), m_a = __ballot(pl && oa && !ob), m_b = __ballot(pl && ob && !oa);
+        unsigned long long m_gen = __ballot(!pl && (oa || ob));
         const unsigned long long m_split = __ballot(sf), m_asym = __ballot(af);
-        // plain lines, two per iteration (both lines' LDS reads are issued before either
-        // is computed)
-        int np = __popcll(m_plain);
-        for (; np >= 2; np -= 2) {
-            const int b1 = __ffsll((long long)m_plain) - 1;
-            m_plain &= m_plain - 1ull;
-            const int b2 = __ffsll((long long)m_plain) - 1;
-            m_plain &= m_plain - 1ull;
+        // plain lines over the whole tile, two per iteration (all their LDS reads first)
+        int n = __popcll(m_two);
+        for (; n >= 2; n -= 2) {
+            const int b1 = __ffsll((long long)m_two) - 1;
+            m_two &= m_two - 1ull;
+            const int b2 = __ffsll((long long)m_two) - 1;
+            m_two &= m_two - 1ull;
             const StagedLine &L1 = S.line[lb + b1], &L2 = S.line[lb + b2];
-            const float4 a1r = L1.a, r1r = L1.r, l1r = UNI ? L1.l : make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 a2r = L2.a, r2r = L2.r, l2r = UNI ? L2.l : make_float4(0.f, 0.f, 0.f, 0.f);
-            plain_two<UNI>(a1r, r1r, l1r, a2r, r2r, l2r, t0, t1, fi0, fi1, a0, a1);
+            const float4 a1 = L1.a, r1 = L1.r, l1 = UNI ? L1.l : zero4, a2 = L2.a, r2 = L2.r, l2 = UNI ? L2.l : zero4;
+            const float v10 = plain_value<UNI>(a1, r1, l1, t0, fi0), v11 = plain_value<UNI>(a1, r1, l1, t1, fi1);
+            const float v20 = plain_value<UNI>(a2, r2, l2, t0, fi0), v21 = plain_value<UNI>(a2, r2, l2, t1, fi1);
+            A.two0 = fmaf(a1.z, v10, A.two0);
+            A.two1 = fmaf(a1.z, v11, A.two1);
+            A.two0 = fmaf(a2.z, v20, A.two0);
+            A.two1 = fmaf(a2.z, v21, A.two1);
         }
-        if (np) {
-            const int b1 = __ffsll((long long)m_plain) - 1;
-            const StagedLine &L1 = S.line[lb + b1];
-            line_pair<UNI, true>(L1.a, L1.r, L1, false, false, 0.0f, t0, t1, fi0, fi1, step, a0, a1);
+        if (n) {
+            const StagedLine &L1 = S.line[lb + __ffsll((long long)m_two) - 1];
+            const float4 a1 = L1.a, r1 = L1.r, l1 = UNI ? L1.l : zero4;
+            const float v10 = plain_value<UNI>(a1, r1, l1, t0, fi0), v11 = plain_value<UNI>(a1, r1, l1, t1, fi1);
+            A.two0 = fmaf(a1.z, v10, A.two0);
+            A.two1 = fmaf(a1.z, v11, A.two1);
+        }
+        // plain lines over one half: two lines at the lane's point of that half
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            unsigned long long m = h ? m_b : m_a;
+            const float2 t = h ? t1 : t0;
+            const float fi = h ? fi1 : fi0;
+            float acc = h ? A.one1 : A.one0;
+            int k = __popcll(m);
+            for (; k >= 2; k -= 2) {
+                const int b1 = __ffsll((long long)m) - 1;
+                m &= m - 1ull;
+                const int b2 = __ffsll((long long)m) - 1;
+                m &= m - 1ull;
+                const StagedLine &L1 = S.line[lb + b1], &L2 = S.line[lb + b2];
+                const float4 a1 = L1.a, r1 = L1.r, l1 = UNI ? L1.l : zero4, a2 = L2.a, r2 = L2.r, l2 = UNI ? L2.l : zero4;
+                const float v1 = plain_value<UNI>(a1, r1, l1, t, fi), v2 = plain_value<UNI>(a2, r2, l2, t, fi);
+                acc = fmaf(a1.z, v1, acc);
+                acc = fmaf(a2.z, v2, acc);
+            }
+            if (k) {
+                const StagedLine &L1 = S.line[lb + __ffsll((long long)m) - 1];
+                const float4 a1 = L1.a;
+                acc = fmaf(a1.z, plain_value<UNI>(a1, L1.r, UNI ? L1.l : zero4, t, fi), acc);
+            }
+            if (h) A.one1 = acc; else A.one0 = acc;
         }
         // the others (split flanks, asymmetry, or a profile that needs the general form)
         while (m_gen) {
@@ -812,7 +825,7 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
             const StagedLine &L = S.line[lb + b];
             const bool has_asym = (m_asym >> b) & 1ull;
             const float asym = has_asym ? __int_as_float(__builtin_amdgcn_readfirstlane(L.span.z)) : 0.0f;
-            line_pair<UNI, false>(L.a, L.r, L, (m_split >> b) & 1ull, has_asym, asym, t0, t1, fi0, fi1, step, a0, a1);
+            line_general<UNI>(L.a, L.r, L, (m_split >> b) & 1ull, has_asym, asym, t0, t1, fi0, fi1, step, A.gen0, A.gen1);
         }
     }
 }
@@ -846,12 +859,12 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
         last = max(last, im->grange[g][1]);
     }
     const int lend = min(nl, last + 1);
-    float acc[2 * TPW];
+    TileAcc ta[TPW];
     float2 tf[2 * TPW];
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
         const int base = lo_pt + (wave + kAccWaves * j) * 128 + lane;
-        acc[2 * j] = acc[2 * j + 1] = 0.0f;
+        ta[j] = TileAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         tf[2 * j] = !uni && base < P.np ? P.tff[base] : make_float2(0.f, 0.f);
         tf[2 * j + 1] = !uni && base + 64 < P.np ? P.tff[base + 64] : make_float2(0.f, 0.f);
     }
@@ -879,11 +892,8 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
             for (int j = 0; j < TPW; ++j) {
                 const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
                 if (t_lo < hi_pt) {
-                    float a0 = acc[2 * j], a1 = acc[2 * j + 1];
-                    if (uni) tile_lines<true>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, a0, a1);
-                    else tile_lines<false>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, a0, a1);
-                    acc[2 * j] = a0;
-                    acc[2 * j + 1] = a1;
+                    if (uni) tile_lines<true>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, ta[j]);
+                    else tile_lines<false>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, ta[j]);
                 }
             }
             __syncthreads();
@@ -906,7 +916,7 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                 if (ow.y > 0.0f) {
                     float yb = 0.0f;
                     if (P.n_bkg > 0) {
-                        float z = (float)(P.thi > P.tlo ? 2.0 * (P.tth[i] - P.tlo) / (P.thi - P.tlo) - 1.0 : 0.0);
+                        const float z = P.zcheb[i];
                         float ta = 1.0f, tb = z;
                         yb = S.bkg[0];
                         if (P.n_bkg > 1) yb = fmaf(S.bkg[1], z, yb);
@@ -916,7 +926,8 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                             yb = fmaf(S.bkg[k], nx, yb);
                         }
                     }
-                    const float r = ow.x - (yb + acc[2 * j + h]);
+                    const float yp = h ? (ta[j].two1 + ta[j].one1) + ta[j].gen1 : (ta[j].two0 + ta[j].one0) + ta[j].gen0;
+                    const float r = ow.x - (yb + yp);
                     neumaier(s, c, ow.y * r * r);
                 }
             }

@@ -74,6 +74,7 @@ struct DevPlan {
     // device arrays
     const double *tth;
     const float2 *tff;         // float-float split of tth
+    const float *zcheb;        // Chebyshev abscissa of every point, 2 (tth - tlo) / (thi - tlo) - 1
     const float2 *obs;
     const int4 *hklm;
     const int *eq_off;

@@ -1456,7 +1456,7 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
         for (int q = 0; q < (pl->chains + 1) / 2; ++q) pair_tab.push_back(make_int2(m, q));
     }
     const std::vector<GroupTrial> placed = place_trials(g);
-    lines_tab = xcd_table(placed, [&](int m) { return (g->members[m]->nr + 127) / 128; });
+    lines_tab = xcd_table(placed, [&](int m) { return (g->members[m]->nr + 255) / 256; });
     g->total_chains = (int)chain_tab.size();
     g->total_pairs = (int)pair_tab.size();
     g->n_lines = (int)lines_tab.size();

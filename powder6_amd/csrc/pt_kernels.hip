@@ -612,7 +612,7 @@ __device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, in
     return t < ntrials;
 }
 
-__global__ void __launch_bounds__(128) k_stage_lines(DevPlan P, TrialImage *img, LineRec *rec,
+__global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img, LineRec *rec,
                                                      int4 *span, int per, int ntrials) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
@@ -1094,7 +1094,7 @@ __global__ void __launch_bounds__(64) k_group_setup(const DevPlan *__restrict__ 
     setup_body(P, nullptr, (TrialImage *)mem[w.x].img, w.y, 1, bootstrap, sh, prop);
 }
 
-__global__ void __launch_bounds__(128) k_group_lines(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
+__global__ void __launch_bounds__(256) k_group_lines(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
                                                      const int4 *__restrict__ tab) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
@@ -1506,8 +1506,8 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
 static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }
 
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s) {
-    const int per = (P.nr + 127) / 128;
-    hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(128), 0, s, P, (TrialImage *)img, rec, span, per,
+    const int per = (P.nr + 255) / 256;
+    hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, span, per,
                        ntrials);
     return hipGetLastError();
 }
@@ -1590,7 +1590,7 @@ hipError_t launch_group_setup(const DevPlan *plans, const DevMember *mem, const 
     return hipGetLastError();
 }
 hipError_t launch_group_lines(const DevPlan *plans, const DevMember *mem, const int4 *tab, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_group_lines, dim3(n), dim3(128), 0, s, plans, mem, tab);
+    hipLaunchKernelGGL(k_group_lines, dim3(n), dim3(256), 0, s, plans, mem, tab);
     return hipGetLastError();
 }
 hipError_t launch_group_accum(const DevPlan *plans, const DevMember *mem, const int4 *tab, int n, int tpw, hipStream_t s) {

@@ -271,10 +271,12 @@ __device__ __forceinline__ float peak_shift(double tth, float disp, float transp
 __device__ __forceinline__ int lower_bound_t(const DevPlan &P, double target) {
     int lo = 0, hi = P.np;
     if (P.uniform) {
-        double g = ceil((target - P.t0) / P.step);
+        // on a uniform grid the positions are computed, not loaded: t_i = t0 + i step (the
+        // plan accepts a grid as uniform within 1e-9 step of that)
+        double g = ceil((target - P.t0) * P.inv_step);
         int i = g < 0.0 ? 0 : g > (double)P.np ? P.np : (int)g;
-        while (i > 0 && P.tth[i - 1] >= target) --i;
-        while (i < P.np && P.tth[i] < target) ++i;
+        while (i > 0 && P.t0 + (i - 1) * P.step >= target) --i;
+        while (i < P.np && P.t0 + i * P.step < target) ++i;
         return i;
     }
     while (lo < hi) {
@@ -286,10 +288,10 @@ __device__ __forceinline__ int lower_bound_t(const DevPlan &P, double target) {
 // first index >= start with t > target
 __device__ __forceinline__ int upper_stop(const DevPlan &P, int start, double target) {
     if (P.uniform) {
-        double g = floor((target - P.t0) / P.step) + 1.0;
+        double g = floor((target - P.t0) * P.inv_step) + 1.0;
         int i = g < (double)start ? start : g > (double)P.np ? P.np : (int)g;
-        while (i > start && P.tth[i - 1] > target) --i;
-        while (i < P.np && P.tth[i] <= target) ++i;
+        while (i > start && P.t0 + (i - 1) * P.step > target) --i;
+        while (i < P.np && P.t0 + i * P.step <= target) ++i;
         return i;
     }
     int lo = start, hi = P.np;

@@ -573,7 +573,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     P.chains = pb->chains; P.stride = pl->stride; P.steps_per_sweep = pl->steps_per_sweep;
     P.uniform = uniform; P.radiation = pb->radiation;
     P.doublet = (pb->ratio > 1e-6 && pb->lambda2 > 1e-6 && std::fabs(pb->lambda1 - pb->lambda2) > 1e-6) ? 1 : 0;
-    P.t0 = t0; P.step = step;
+    P.t0 = t0; P.step = step; P.inv_step = 1.0 / step;
     P.step_hi = (float)step; P.step_lo = (float)(step - (double)P.step_hi);
     P.tlo = pb->tth[0]; P.thi = pb->tth[n - 1];
     P.lambda1 = pb->lambda1 != 0 ? pb->lambda1 : 1.54056; P.lambda2 = pb->lambda2; P.ratio = pb->ratio;

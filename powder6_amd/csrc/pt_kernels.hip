@@ -215,7 +215,7 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
 // doublet weight) and the reflection's height factor m Lp PO / area goes to
 // unit_factor[k] instead: the columns of the shared profile matrix.
 __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
-                                        float *unit_factor = nullptr) {
+                                        float *unit_factor = nullptr, const int4 *hk_given = nullptr) {
     if (unit_factor) unit_factor[k] = 0.0f;
     span[0] = make_int4(0, 0, 0, 0);
     span[1] = make_int4(0, 0, 0, 0);
@@ -223,7 +223,7 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
     while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
     const DevPhase &ph = P.phase[q];
     if (!sh.phase_ok[q]) return;
-    int4 hk = P.hklm[k];
+    int4 hk = hk_given ? *hk_given : P.hklm[k];
     const double *f = sh.metric[q];
     double h = hk.x, kk = hk.y, l = hk.z;
     double qv = h * h * f[0] + kk * kk * f[1] + l * l * f[2] + h * kk * f[3] + kk * l * f[4] + h * l * f[5];
@@ -570,16 +570,17 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
 // atomic min / max per group and workgroup)
 __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, LineRec *rec, int4 *span, int t, int kb,
                                            Shared &sh, int (*gr)[2]) {
+    const int k = kb * blockDim.x + threadIdx.x;
+    const int4 hk = k < P.nr ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
     stage_reps(P, sh);
     image_load(P, img + t, sh);
     const int ng = (P.np + 511) >> 9;
     for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
-    const int k = kb * blockDim.x + threadIdx.x;
     if (k < P.nr) {
         const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
         if (sh.go) {
-            build_reflection(P, sh, k, rec + base, span + base);
+            build_reflection(P, sh, k, rec + base, span + base, nullptr, &hk);
             for (int q = 0; q < 2; ++q) {
                 const int4 sp = span[base + q];
                 if (sp.y > sp.x)

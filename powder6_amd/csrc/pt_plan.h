@@ -62,6 +62,7 @@ struct DevPlan {
     int pad0;
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
+    double inv_step;           // 1 / step
     float step_hi, step_lo;    // float-float split of step
     double tlo, thi;           // Chebyshev range (first and last point)
     double lambda1, lambda2, ratio, pol_k, min_fwhm;

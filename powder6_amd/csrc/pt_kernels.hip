@@ -17,21 +17,26 @@
 namespace ptk {
 
 // ---- per-workgroup shared state ----------------------------------------------------
-struct Shared {
+// what a trial's setup forms (parameters, metrics, atoms, profile and background slots)
+struct SetupShared {
     double p[PTK_MAX_PARAMS];                 // model parameters of the trial
     double metric[PTK_MAX_PHASES][6];         // reciprocal quadratic forms
     double apos[PTK_MAX_ATOMS][3];            // atom positions (fractional)
     float aocc[PTK_MAX_ATOMS], aB[PTK_MAX_ATOMS];
     float prof[PTK_MAX_PHASES][16];           // profile slots (FP32)
     float bkg[PTK_MAX_BKG_F];                 // Chebyshev coefficients (FP32)
-    int rep_r[PTK_MAX_REPS_TOTAL][9];
-    double rep_t[PTK_MAX_REPS_TOTAL][3];
-    float aff[PTK_MAX_ATOMS][12];             // atom constants: a1..a5, b1..b5, c, w0
-    float afp[PTK_MAX_ATOMS][2];              // f', f'' (or the neutron length in [0])
     float red_s[32], red_c[32];
     int phase_ok[PTK_MAX_PHASES];
     int cell_ok, prof_ok, go, accept;
     double chi2;
+};
+
+// ... and the symmetry and scattering constants the structure factors need
+struct Shared : SetupShared {
+    int rep_r[PTK_MAX_REPS_TOTAL][9];
+    double rep_t[PTK_MAX_REPS_TOTAL][3];
+    float aff[PTK_MAX_ATOMS][12];             // atom constants: a1..a5, b1..b5, c, w0
+    float afp[PTK_MAX_ATOMS][2];              // f', f'' (or the neutron length in [0])
 };
 
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
@@ -58,7 +63,7 @@ __device__ inline void stage_reps(const DevPlan &P, Shared &sh) {
 }
 
 // p = base + M x  (rietveld_core.js:1855-1859 apply, affine links of atomic_constraints.js)
-__device__ inline void materialize(const DevPlan &P, const double *x, Shared &sh) {
+__device__ inline void materialize(const DevPlan &P, const double *x, SetupShared &sh) {
     for (int k = threadIdx.x; k < P.n_params; k += blockDim.x) {
         double v = P.base[k];
         for (int t = P.term_off[k]; t < P.term_off[k + 1]; ++t) v += P.term_coeff[t] * x[P.term_var[t]];
@@ -68,7 +73,7 @@ __device__ inline void materialize(const DevPlan &P, const double *x, Shared &sh
 
 // per-trial setup after materialize (+barrier): metrics, atoms, profile slots, guards.
 // Ends with a barrier; sets sh.cell_ok and sh.prof_ok.
-__device__ inline void trial_setup(const DevPlan &P, Shared &sh) {
+__device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
     int tid = threadIdx.x, nt = blockDim.x;
     if (tid < P.n_phases) {
         const DevPhase &ph = P.phase[tid];
@@ -476,7 +481,7 @@ __device__ inline int total_atoms(const DevPlan &P) {
     return l.atom_off + l.n_atoms;
 }
 
-__device__ inline void image_store(const DevPlan &P, const Shared &sh, TrialImage *img) {
+__device__ inline void image_store(const DevPlan &P, const SetupShared &sh, TrialImage *img) {
     const int tid = threadIdx.x, nt = blockDim.x, na = total_atoms(P);
     for (int i = tid; i < P.n_params; i += nt) img->p[i] = sh.p[i];
     for (int i = tid; i < P.n_phases * 6; i += nt) img->metric[i / 6][i % 6] = sh.metric[i / 6][i % 6];
@@ -489,7 +494,7 @@ __device__ inline void image_store(const DevPlan &P, const Shared &sh, TrialImag
     for (int g = tid; g < (P.np + 511) >> 9; g += nt) { img->grange[g][0] = 0x7fffffff; img->grange[g][1] = -1; }
 }
 
-__device__ inline void image_load(const DevPlan &P, const TrialImage *img, Shared &sh) {
+__device__ inline void image_load(const DevPlan &P, const TrialImage *img, SetupShared &sh) {
     const int tid = threadIdx.x, nt = blockDim.x, na = total_atoms(P);
     for (int i = tid; i < P.n_params; i += nt) sh.p[i] = img->p[i];
     for (int i = tid; i < P.n_phases * 6; i += nt) sh.metric[i / 6][i % 6] = img->metric[i / 6][i % 6];
@@ -506,7 +511,7 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Share
 // uniformly, x_j + z scale_j sqrt(T_c) reflected into its box, log u for the Metropolis
 // test.  Random numbers are keyed on (seed, chain, the chain's step counter).
 __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
-                                  Shared &sh, double *prop, const double2 *pick = nullptr) {
+                                  SetupShared &sh, double *prop, const double2 *pick = nullptr) {
     const int nv = P.nv;
     bool boot = false, skip = false;
     double *st = nullptr;
@@ -560,7 +565,7 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
 
 __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__restrict__ X, TrialImage *img,
                                                     int pt_mode, int bootstrap, const double2 *__restrict__ pick) {
-    __shared__ Shared sh;
+    __shared__ SetupShared sh;
     __shared__ double prop[PTK_MAX_VARS];
     setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
 }
@@ -1027,7 +1032,7 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
 // the Metropolis step of chain t, then its next proposal and setup: the finish of one PT
 // step fused with the setup of the next (one launch less per step)
 __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *img, const double2 *__restrict__ partial,
-                                         int nsplit, int t, Shared &sh, double *prop, int &s_acc) {
+                                         int nsplit, int t, SetupShared &sh, double *prop, int &s_acc) {
     const int nv = P.nv;
     if (threadIdx.x == 0) s_acc = accept_chain(P, img + t, trial_cost(img + t, partial, nsplit, t), t);
     __syncthreads();
@@ -1046,7 +1051,7 @@ __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *
 
 __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
                                                            int nsplit) {
-    __shared__ Shared sh;
+    __shared__ SetupShared sh;
     __shared__ double prop[PTK_MAX_VARS];
     __shared__ int s_acc;
     finish_setup_body(P, img, partial, nsplit, blockIdx.x, sh, prop, s_acc);

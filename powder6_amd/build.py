@@ -1,8 +1,11 @@
 """Build the gfx950 shared library powder6_amd/_lib/libpowder6_pt.so with hipcc.
 
 Flags: -O3, gfx950 only, packed-FP32 instructions disabled (-Xclang -target-feature
--Xclang -packed-fp32-ops), FP contraction left on for the device (FMA) and off for the
-host plan compiler's reflection enumeration (bit-for-bit with the reference ordering).
+-Xclang -packed-fp32-ops), FP32 division and square root by faster sequences that are not
+correctly rounded (-fno-hip-fp32-correctly-rounded-divide-sqrt; the profile constants they
+feed are FP32 anyway), FP contraction left on for the device (FMA)
+and off for the host plan compiler's reflection enumeration (bit-for-bit with the reference
+ordering).
 """
 import os
 import subprocess
@@ -15,7 +18,7 @@ LIB = os.path.join(LIBDIR, 'libpowder6_pt.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = '--offload-arch=gfx950'
 DEVICE_FLAGS = ['-O3', '-std=c++17', '-fPIC', ARCH, '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-                '-munsafe-fp-atomics', '-Wall', '-Wno-unused-function']
+                '-munsafe-fp-atomics', '-fno-hip-fp32-correctly-rounded-divide-sqrt', '-Wall', '-Wno-unused-function']
 
 
 def run(cmd):

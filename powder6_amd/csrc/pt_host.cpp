@@ -213,6 +213,12 @@ struct pt_plan {
     std::vector<std::vector<int32_t>> phase_hklm;   // per phase (h,k,l,m)
     ptk::DevPlan P{};
     hipStream_t stream = nullptr;
+    // a sweep's chains run as `parts` contiguous parts, part p > 0 on side[p - 1], so that
+    // one part's latency-bound stages overlap another part's accumulation
+    static constexpr int kMaxParts = 4;
+    int parts = 1;
+    hipStream_t side[kMaxParts - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxParts - 1] = {};
     std::vector<void *> allocs;
     // device scratch
     double *d_x = nullptr, *d_chi2 = nullptr, *d_partial = nullptr, *d_y = nullptr;
@@ -410,6 +416,11 @@ void pt_plan_destroy(pt_plan *pl) {
         if (pl->graph) (void)hipGraphDestroy(pl->graph);
         if (pl->ev_total[0]) (void)hipEventDestroy(pl->ev_total[0]);
         if (pl->ev_total[1]) (void)hipEventDestroy(pl->ev_total[1]);
+        for (int p = 0; p + 1 < pt_plan::kMaxParts; ++p) {
+            if (pl->side[p]) { (void)hipStreamSynchronize(pl->side[p]); (void)hipStreamDestroy(pl->side[p]); }
+            if (pl->join[p]) (void)hipEventDestroy(pl->join[p]);
+        }
+        if (pl->fork) (void)hipEventDestroy(pl->fork);
         if (pl->stream) (void)hipStreamDestroy(pl->stream);
         mlpx::destroy(pl->mlp);
     }
@@ -568,6 +579,15 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess) { delete pl; return fail(PT_ERR_DEVICE, "PT stream creation failed."); }
     (void)hipEventCreate(&pl->ev_total[0]);
     (void)hipEventCreate(&pl->ev_total[1]);
+    // sweep parts: as many as keep at least 64 chains each, up to four
+    pl->parts = std::max(1, std::min(pt_plan::kMaxParts, pb->chains / 64));
+    if (pl->parts > 1) {
+        bool ok = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming) == hipSuccess;
+        for (int p = 0; ok && p + 1 < pl->parts; ++p)
+            ok = hipStreamCreateWithFlags(&pl->side[p], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&pl->join[p], hipEventDisableTiming) == hipSuccess;
+        if (!ok) { pt_plan_destroy(pl); return fail(PT_ERR_DEVICE, "PT side stream creation failed."); }
+    }
 
     P.np = n; P.nr = pl->nr; P.nv = pb->n_vars; P.n_params = pb->n_params; P.n_phases = pb->n_phases; P.n_bkg = pb->n_bkg;
     P.chains = pb->chains; P.stride = pl->stride; P.steps_per_sweep = pl->steps_per_sweep;
@@ -901,18 +921,44 @@ static int enqueue_screened_sweep(pt_plan *pl, hipEvent_t *ev) {
 // one sweep: steps_per_sweep x (setup/propose, lines, accumulate, finish/accept), then the
 // replica exchange; the finish of each step but the last is fused with the next setup.
 // Accumulate launches are bracketed by events when ev != null.
+static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *ev) {
+    HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, n, 1, 0, st, nullptr, t0), "setup");
+    for (int s = 0; s < pl->steps_per_sweep; ++s) {
+        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, n, st, t0), "lines");
+        if (ev) HIPCHECK(hipEventRecord(ev[2 * s], st), "timing");
+        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, n, pl->tpw, st, t0),
+                 "accumulate");
+        if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], st), "timing");
+        if (s + 1 < pl->steps_per_sweep)
+            HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, st, n, t0), "finish");
+        else
+            HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, n, pl->d_schi2, 1, st, t0), "finish");
+    }
+    return 0;
+}
+
 static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
     if (pl->screen_on) return enqueue_screened_sweep(pl, ev);
-    HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, pl->chains, 1, 0, pl->stream), "setup");
-    for (int s = 0; s < pl->steps_per_sweep; ++s) {
-        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
-        if (ev) HIPCHECK(hipEventRecord(ev[2 * s], pl->stream), "timing");
-        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, pl->chains, pl->tpw, pl->stream), "accumulate");
-        if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], pl->stream), "timing");
-        if (s + 1 < pl->steps_per_sweep)
-            HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->stream), "finish");
-        else
-            HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
+    // chains take no part in each other's steps until the exchange, so the parts run
+    // independently between a fork and a join (the timed form runs as one part)
+    const int parts = ev ? 1 : pl->parts;
+    if (parts <= 1) {
+        int rc = enqueue_part(pl, pl->stream, 0, pl->chains, ev);
+        if (rc) return rc;
+    } else {
+        const int chunk = ((pl->chains + parts - 1) / parts + 7) / 8 * 8;
+        HIPCHECK(hipEventRecord(pl->fork, pl->stream), "fork");
+        for (int p = 1; p < parts; ++p) HIPCHECK(hipStreamWaitEvent(pl->side[p - 1], pl->fork, 0), "fork");
+        for (int p = 0; p < parts; ++p) {
+            const int t0 = p * chunk, n = std::min(chunk, pl->chains - t0);
+            if (n <= 0) continue;
+            int rc = enqueue_part(pl, p ? pl->side[p - 1] : pl->stream, t0, n, nullptr);
+            if (rc) return rc;
+        }
+        for (int p = 1; p < parts; ++p) {
+            HIPCHECK(hipEventRecord(pl->join[p - 1], pl->side[p - 1]), "join");
+            HIPCHECK(hipStreamWaitEvent(pl->stream, pl->join[p - 1], 0), "join");
+        }
     }
     HIPCHECK(ptk::launch_exchange(pl->P, pl->stream), "exchange");
     return 0;

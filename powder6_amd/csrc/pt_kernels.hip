@@ -563,11 +563,12 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
     image_store(P, sh, img + t);
 }
 
+// (t_off: the first trial of a launch over a contiguous part of the trials)
 __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__restrict__ X, TrialImage *img,
-                                                    int pt_mode, int bootstrap, const double2 *__restrict__ pick) {
+                                                    int pt_mode, int bootstrap, const double2 *__restrict__ pick, int t_off) {
     __shared__ SetupShared sh;
     __shared__ double prop[PTK_MAX_VARS];
-    setup_body(P, X, img, blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
+    setup_body(P, X, img, t_off + blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
 }
 
 // line records of reflections [kb * blockDim.x, ...) of trial t; every line with a window
@@ -619,12 +620,12 @@ __device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, in
 }
 
 __global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img, LineRec *rec,
-                                                     int4 *span, int per, int ntrials) {
+                                                     int4 *span, int per, int ntrials, int t_off) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    lines_body(P, img, rec, span, t, u, sh, gr);
+    lines_body(P, img, rec, span, t_off + t, u, sh, gr);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
@@ -960,11 +961,11 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
 template <int TPW>
 __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
                                                      const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
-                                                     double2 *partial, int nsplit, int per, int ntrials) {
+                                                     double2 *partial, int nsplit, int per, int ntrials, int t_off) {
     __shared__ AccShared S;
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    accum_body<TPW>(P, img, grec, gspan, partial, nsplit, t, u, S);
+    accum_body<TPW>(P, img, grec, gspan, partial, nsplit, t_off + t, u, S);
 }
 
 // chi^2 of trial t: the ordered sum of its partials (3e38 for an invalid trial)
@@ -1021,9 +1022,10 @@ __device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, doub
 // chi^2 of every trial (batch mode) or the Metropolis step of every chain (PT mode)
 __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage *__restrict__ img,
                                                      const double2 *__restrict__ partial, int nsplit, int ntrials,
-                                                     double *chi2, int pt_mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntrials) return;
+                                                     double *chi2, int pt_mode, int t_off) {
+    const int tl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tl >= ntrials) return;
+    const int t = t_off + tl;
     const double cost = trial_cost(img + t, partial, nsplit, t);
     if (!pt_mode) { chi2[t] = cost; return; }
     finish_chain(P, img + t, cost, t);
@@ -1050,11 +1052,11 @@ __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *
 }
 
 __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
-                                                           int nsplit) {
+                                                           int nsplit, int t_off) {
     __shared__ SetupShared sh;
     __shared__ double prop[PTK_MAX_VARS];
     __shared__ int s_acc;
-    finish_setup_body(P, img, partial, nsplit, blockIdx.x, sh, prop, s_acc);
+    finish_setup_body(P, img, partial, nsplit, t_off + blockIdx.x, sh, prop, s_acc);
 }
 
 // ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
@@ -1505,37 +1507,41 @@ hipError_t set_lds_limits(size_t bytes) {
 }
 
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
-                              hipStream_t s, const double2 *pick) {
-    hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick);
+                              hipStream_t s, const double2 *pick, int t_off) {
+    hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick, t_off);
     return hipGetLastError();
 }
 static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }
 
-hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s) {
+hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s, int t_off) {
     const int per = (P.nr + 255) / 256;
     hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, span, per,
-                       ntrials);
+                       ntrials, t_off);
     return hipGetLastError();
 }
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
-                              int ngroups, int ntrials, int tpw, hipStream_t s) {
+                              int ngroups, int ntrials, int tpw, hipStream_t s, int t_off) {
     const int per = (ngroups + tpw - 1) / tpw;
     const dim3 grid(xcd_grid(per, ntrials));
     const TrialImage *im = (const TrialImage *)img;
-    if (tpw == 1) hipLaunchKernelGGL(k_stage_accum<1>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials);
-    else if (tpw == 4) hipLaunchKernelGGL(k_stage_accum<4>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials);
-    else hipLaunchKernelGGL(k_stage_accum<2>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials);
+    if (tpw == 1)
+        hipLaunchKernelGGL(k_stage_accum<1>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off);
+    else if (tpw == 4)
+        hipLaunchKernelGGL(k_stage_accum<4>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off);
+    else
+        hipLaunchKernelGGL(k_stage_accum<2>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off);
     return hipGetLastError();
 }
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
-                               double *chi2, int pt_mode, hipStream_t s) {
+                               double *chi2, int pt_mode, hipStream_t s, int t_off) {
     hipLaunchKernelGGL(k_stage_finish, dim3((ntrials + 63) / 64), dim3(64), 0, s, P, (const TrialImage *)img, partial, nsplit,
-                       ntrials, chi2, pt_mode);
+                       ntrials, chi2, pt_mode, t_off);
     return hipGetLastError();
 }
 
-hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_finish_setup, dim3(P.chains), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit);
+hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s, int ntrials,
+                                     int t_off) {
+    hipLaunchKernelGGL(k_stage_finish_setup, dim3(ntrials), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit, t_off);
     return hipGetLastError();
 }
 

@@ -9,17 +9,21 @@ size_t plain_scratch_bytes();
 size_t trial_image_bytes();
 int accum_points_per_group(int tpw);   // points per accumulate workgroup at tpw tiles per wavefront
 hipError_t set_lds_limits(size_t bytes);
+// Every stage launch covers trials [t_off, t_off + ntrials) (t_off a multiple of 8, so
+// that trial t keeps XCD t mod 8).
 // pick (PT mode): the move of each chain chosen by the surrogate screen, else the
 // setup draws the plain proposal itself
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
-                              hipStream_t s, const double2 *pick = nullptr);
-hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s);
+                              hipStream_t s, const double2 *pick = nullptr, int t_off = 0);
+hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s,
+                              int t_off = 0);
 // partial chi^2 per 512-point group: ngroups = ceil(np / 512) per trial, whatever tpw
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
-                              int ngroups, int ntrials, int tpw, hipStream_t s);
+                              int ngroups, int ntrials, int tpw, hipStream_t s, int t_off = 0);
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
-                               double *chi2, int pt_mode, hipStream_t s);
-hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s);
+                               double *chi2, int pt_mode, hipStream_t s, int t_off = 0);
+hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s, int ntrials,
+                                     int t_off = 0);
 hipError_t launch_matrix_columns(const DevPlan &P, const void *img, LineRec *rec, int4 *span, float *factor, hipStream_t s);
 // bf16 != 0: the hi/lo pairs as BF16 instead of FP16
 hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *span, unsigned short *phi, unsigned short *plo,

@@ -135,6 +135,28 @@ def test_sweep_graph_replays_match_direct_launches(pbso4):
         assert np.array_equal(g, d)
 
 
+@pytest.mark.parametrize('chains', [200, 256])
+def test_sweep_parts_on_streams_match_one_stream(chains):
+    """With 128 chains or more the sweep graph runs the chains as parts on their own streams
+    (200 chains: parts of 72, 72 and 56; 256: four of 64); the timed direct form runs them as
+    one part.  Chains are independent between exchanges, so the states are bit-identical."""
+    pb = PR.pbso4_pnma(simulate=O.oracle_simulate, chains=chains, steps_per_sweep=8)
+    x0 = pb.start_vector()
+    with pt.Plan(pb) as pl:
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        graphed = [pl.batch(1) for _ in range(3)]
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        pl.set_kernel_timing(True)
+        direct = [pl.batch(1) for _ in range(3)]
+    for g, d in zip(graphed, direct):
+        assert np.array_equal(g, d)
+    S = graphed[-1].reshape(chains, -1)
+    nv = pb.nv
+    assert np.all(S[:, 3 * nv + pt.S_PROPOSALS] == 3 * 8)
+
+
 def test_tile_shapes_give_identical_states(pbso4):
     """The chi^2 partials are per 512-point group whatever the tile shape, so the PT
     trajectory does not depend on the shape the tuner picks: bit-identical states."""

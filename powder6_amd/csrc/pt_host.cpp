@@ -646,6 +646,14 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     P.tth = d_tth; P.tff = d_tff; P.zcheb = d_zcheb; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
     P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
     P.term_var = d_term_var; P.term_coeff = d_coeff; P.vlo = d_lo; P.vhi = d_hi; P.vscale = d_sc; P.state = d_state;
+    {
+        // the temperature ladder once, as the reference forms it (rietveld_core.js:2195)
+        std::vector<double> temps(pl->chains);
+        for (int c = 0; c < pl->chains; ++c) temps[c] = std::pow(pb->max_temp, (double)c / (double)(pl->chains > 1 ? pl->chains - 1 : 1));
+        double *d_temp = nullptr;
+        if ((rc = upload(pl, &d_temp, temps))) { pt_plan_destroy(pl); return rc; }
+        P.temp = d_temp;
+    }
 
     // LDS: line-record chunk + y[np]; keep two workgroups per CU when the pattern allows
     size_t stat = ptk::shared_static_bytes() + (3 * PTK_MAX_VARS + PTK_S_COUNT_K) * sizeof(double);

@@ -523,7 +523,7 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
         __syncthreads();
         if (threadIdx.x == 0 && !boot) {
             const unsigned long long ctr = (unsigned long long)step;
-            const double temp = pow(P.max_temp, (double)t / (double)(P.chains > 1 ? P.chains - 1 : 1));
+            const double temp = P.temp[t];
             const double u3 = uniform01(P.seed, t, ctr, 3);
             if (pick) {
                 // the move chosen by the surrogate screen among candidates drawn from
@@ -987,7 +987,7 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
     double *sc = P.state + (size_t)t * P.stride + 3 * nv;
     const bool boot = im->boot != 0;
     const double step = sc[2];
-    const double temp = pow(P.max_temp, (double)t / (double)(P.chains > 1 ? P.chains - 1 : 1));
+    const double temp = P.temp[t];
     sc[10] = cost;
     const double old = sc[0];
     const bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
@@ -1070,8 +1070,7 @@ __device__ inline void exchange_body(const DevPlan &P, int pair) {
     const int c = 2 * pair + parity;
     if (c + 1 >= P.chains) return;
     double *a = P.state + (size_t)c * P.stride, *b = P.state + (size_t)(c + 1) * P.stride;
-    const double den = (double)(P.chains > 1 ? P.chains - 1 : 1);
-    const double t0 = pow(P.max_temp, (double)c / den), t1 = pow(P.max_temp, (double)(c + 1) / den);
+    const double t0 = P.temp[c], t1 = P.temp[c + 1];
     const double la = 0.5 * (1.0 / t0 - 1.0 / t1) * (a[3 * nv] - b[3 * nv]);
     a[3 * nv + 6] += 1.0;
     bool swp = la >= 0.0;

@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(64) k_screen_candidates(DevPlan P, ScreenDev S
         double v = 0.0;
         if (k < S.K) {
             const unsigned long long ctr = (unsigned long long)fmax(0.0, st[3 * nv + 2]);
-            const double temp = pow(P.max_temp, (double)c / (double)(P.chains > 1 ? P.chains - 1 : 1));
+            const double temp = P.temp[c];
             const int base = k == 0 ? 0 : 16 + 3 * k;
             const double u0 = uniform01(P.seed, c, ctr, base), u1 = uniform01(P.seed, c, ctr, base + 1);
             const double u2 = uniform01(P.seed, c, ctr, base + 2);
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(64) k_screen_pick(DevPlan P, ScreenDev S, cons
         if (S.skip_log > 0.0f && d > 0.0f) {
             const double *st = P.state + (size_t)c * P.stride;
             const double cur = st[3 * P.nv];
-            const double temp = pow(P.max_temp, (double)c / (double)(P.chains > 1 ? P.chains - 1 : 1));
+            const double temp = P.temp[c];
             if (cur > 0.0 && cur < 3e38 && cur * expm1(2.0 * (double)d) > 2.0 * temp * (double)S.skip_log) skip = true;
         }
     }

@@ -88,6 +88,7 @@ struct DevPlan {
     const int *term_var;
     const double *term_coeff;
     const double *vlo, *vhi, *vscale;
+    const double *temp;        // temperature of chain c: max_temp^(c / (chains - 1))
     double *state;
 };
 

@@ -1345,6 +1345,163 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const Trial
     if (threadIdx.x == 0) iscale[t] = sc;
 }
 
+// Intensities of kMxTrials trials per workgroup (patterns with at most 4 x 256
+// reflections): only structure variables differ between the trials of the shared-profile
+// form, so the metric, the form factors and h R, h.t of every representative are formed
+// once per reflection and used for all the workgroup's trials.  Per trial and reflection
+// the sums run in the order structure_f2 takes (atoms in order, representatives inside).
+constexpr int kMxTrials = 4;
+constexpr int kMxRefPerThread = 4;
+struct MxShared {
+    int rep_r[PTK_MAX_REPS_TOTAL][9];
+    double rep_t[PTK_MAX_REPS_TOTAL][3];
+    float aff[PTK_MAX_ATOMS][12];
+    float afp[PTK_MAX_ATOMS][2];
+    double metric[PTK_MAX_PHASES][6];
+    int phase_ok[PTK_MAX_PHASES];
+    double apos[kMxTrials][PTK_MAX_ATOMS][3];
+    float aocc[kMxTrials][PTK_MAX_ATOMS], aB[kMxTrials][PTK_MAX_ATOMS];
+    float S[kMxTrials][PTK_MAX_PHASES];
+    int go[kMxTrials];
+    float wmax[kMxTrials][4];
+};
+
+__global__ void __launch_bounds__(256) k_matrix_intensity4(DevPlan P, const TrialImage *__restrict__ img,
+                                                           const float *__restrict__ factor, unsigned short *ihi,
+                                                           unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16) {
+    __shared__ MxShared M;
+    const int tid = threadIdx.x, tb = blockIdx.x * kMxTrials;
+    const int na = total_atoms(P);
+    int nrep = 0;
+    for (int q = 0; q < P.n_phases; ++q) nrep += P.phase[q].n_reps;
+    for (int i = tid; i < nrep * 9; i += blockDim.x) M.rep_r[i / 9][i % 9] = P.rep_r[i];
+    for (int i = tid; i < nrep * 3; i += blockDim.x) M.rep_t[i / 3][i % 3] = P.rep_t[i];
+    for (int j = tid; j < na; j += blockDim.x) {
+        const DevAtom &A = P.atoms[j];
+        for (int g = 0; g < 5; ++g) {
+            M.aff[j][g] = g < A.n_gauss ? A.ff_a[g] : 0.0f;
+            M.aff[j][5 + g] = g < A.n_gauss ? A.ff_b[g] : 0.0f;
+        }
+        M.aff[j][10] = A.ff_c;
+        M.aff[j][11] = A.w0;
+        M.afp[j][0] = P.radiation == 1 ? A.b_neutron : A.fp;
+        M.afp[j][1] = P.radiation == 1 ? 0.0f : A.fpp;
+    }
+    // the shared metric from the first trial (every trial has the start vector's cell)
+    for (int i = tid; i < P.n_phases * 6; i += blockDim.x) M.metric[i / 6][i % 6] = img[tb].metric[i / 6][i % 6];
+    if (tid < P.n_phases) M.phase_ok[tid] = img[tb].phase_ok[tid];
+    for (int tt = 0; tt < kMxTrials; ++tt) {
+        const bool live = tb + tt < n;
+        const TrialImage *im = img + (live ? tb + tt : tb);
+        for (int i = tid; i < na * 3; i += blockDim.x) M.apos[tt][i / 3][i % 3] = im->apos[i / 3][i % 3];
+        for (int i = tid; i < na; i += blockDim.x) { M.aocc[tt][i] = im->aocc[i]; M.aB[tt][i] = im->aB[i]; }
+        if (tid < P.n_phases) M.S[tt][tid] = (float)im->p[P.phase[tid].param_base + 6];
+        if (tid == 0) M.go[tt] = live && im->go;
+    }
+    __syncthreads();
+    float v[kMxRefPerThread][kMxTrials];
+    float mx[kMxTrials] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < kMxRefPerThread; ++kk) {
+        const int k = tid + kk * (int)blockDim.x;
+#pragma unroll
+        for (int tt = 0; tt < kMxTrials; ++tt) v[kk][tt] = 0.0f;
+        if (k >= P.nr) continue;
+        const float fk = factor[k];
+        int q = 0;
+        while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
+        const DevPhase &ph = P.phase[q];
+        if (fk == 0.0f || !M.phase_ok[q]) continue;
+        const int4 hk = P.hklm[k];
+        const double *f = M.metric[q];
+        const double h = hk.x, kx = hk.y, l = hk.z;
+        const double qv = h * h * f[0] + kx * kx * f[1] + l * l * f[2] + h * kx * f[3] + kx * l * f[4] + h * l * f[5];
+        const float q4 = 0.25f * (float)qv;
+        float Ar[kMxTrials], Br[kMxTrials], Cr[kMxTrials], Dr[kMxTrials];
+#pragma unroll
+        for (int tt = 0; tt < kMxTrials; ++tt) Ar[tt] = Br[tt] = Cr[tt] = Dr[tt] = 0.f;
+        const bool centro = ph.centro != 0;
+        for (int j = 0; j < ph.n_atoms; ++j) {
+            const int ga = ph.atom_off + j;
+            float ca[kMxTrials], sa[kMxTrials];
+#pragma unroll
+            for (int tt = 0; tt < kMxTrials; ++tt) ca[tt] = sa[tt] = 0.f;
+            for (int r = 0; r < ph.n_reps; ++r) {
+                const int *R = M.rep_r[ph.rep_off + r];
+                const double *t = M.rep_t[ph.rep_off + r];
+                const double H0 = hk.x * R[0] + hk.y * R[3] + hk.z * R[6], H1 = hk.x * R[1] + hk.y * R[4] + hk.z * R[7],
+                             H2 = hk.x * R[2] + hk.y * R[5] + hk.z * R[8];
+                const double ht = (double)hk.x * t[0] + (double)hk.y * t[1] + (double)hk.z * t[2];
+#pragma unroll
+                for (int tt = 0; tt < kMxTrials; ++tt) {
+                    const double phs = H0 * M.apos[tt][ga][0] + H1 * M.apos[tt][ga][1] + H2 * M.apos[tt][ga][2] + ht;
+                    const float fr = (float)(phs - floor(phs));
+                    ca[tt] += __builtin_amdgcn_cosf(fr);
+                    if (!centro) sa[tt] += __builtin_amdgcn_sinf(fr);
+                }
+            }
+            const float *F = M.aff[ga];
+            float f0;
+            if (P.radiation == 1) f0 = M.afp[ga][0];
+            else {
+                f0 = F[10];
+                for (int g = 0; g < 5; ++g) f0 += F[g] * __expf(-F[5 + g] * q4);
+                f0 += M.afp[ga][0];
+            }
+#pragma unroll
+            for (int tt = 0; tt < kMxTrials; ++tt) {
+                const float dw = __expf(-M.aB[tt][ga] * q4);
+                const float wj = M.aocc[tt][ga] * f0 * dw * F[11];
+                const float wi = (P.radiation == 1 || !(M.afp[ga][1] != 0.f)) ? 0.f : M.aocc[tt][ga] * M.afp[ga][1] * dw * F[11];
+                Ar[tt] += wj * ca[tt]; Cr[tt] += wi * ca[tt];
+                if (!centro) { Br[tt] += wj * sa[tt]; Dr[tt] += wi * sa[tt]; }
+            }
+        }
+#pragma unroll
+        for (int tt = 0; tt < kMxTrials; ++tt) {
+            if (!M.go[tt]) continue;
+            float S = M.S[tt][q];
+            if (!(S != 0.0f) || isnan(S)) S = 0.0f;
+            const float f2 = centro ? 4.0f * (Ar[tt] * Ar[tt] + Cr[tt] * Cr[tt])
+                                    : Ar[tt] * Ar[tt] + Br[tt] * Br[tt] + Cr[tt] * Cr[tt] + Dr[tt] * Dr[tt];
+            v[kk][tt] = S * (fk * f2);
+            mx[tt] = fmaxf(mx[tt], fabsf(v[kk][tt]));
+        }
+    }
+    // per trial: a power-of-two scale that keeps the largest intensity near 2^14
+#pragma unroll
+    for (int tt = 0; tt < kMxTrials; ++tt) {
+        float m = mx[tt];
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((tid & 63) == 0) M.wmax[tt][tid >> 6] = m;
+    }
+    __syncthreads();
+    if (tid < kMxTrials) {
+        float m = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, M.wmax[tid][w]);
+        int e = 0;
+        if (m > 0.0f && isfinite(m)) { frexpf(m, &e); e -= 14; }
+        M.wmax[tid][0] = ldexpf(1.0f, e);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tt = 0; tt < kMxTrials; ++tt) {
+        const int t = t0 + tb + tt;
+        if (tb + tt >= n) break;
+        const float sc = M.wmax[tt][0], inv = 1.0f / sc;
+#pragma unroll
+        for (int kk = 0; kk < kMxRefPerThread; ++kk) {
+            const int k = tid + kk * (int)blockDim.x;
+            if (k >= kpad) break;
+            unsigned short hv, lv;
+            split_pair(k < P.nr ? v[kk][tt] * inv : 0.0f, bf16 != 0, hv, lv);
+            ihi[(size_t)t * kpad + k] = hv;
+            ilo[(size_t)t * kpad + k] = lv;
+        }
+        if (tid == 0) iscale[t] = sc;
+    }
+}
+
 // 16-bit operand quad as the MFMA input type
 template <bool BF16>
 struct Quad;
@@ -1555,8 +1712,12 @@ hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *
 }
 hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
                                    unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16, hipStream_t s) {
-    hipLaunchKernelGGL(k_matrix_intensity, dim3(n), dim3(256), 0, s, P, (const TrialImage *)img, factor, ihi, ilo, iscale, kpad, t0,
-                       bf16);
+    if (kpad <= kMxRefPerThread * 256)
+        hipLaunchKernelGGL(k_matrix_intensity4, dim3((n + kMxTrials - 1) / kMxTrials), dim3(256), 0, s, P, (const TrialImage *)img,
+                           factor, ihi, ilo, iscale, kpad, t0, n, bf16);
+    else
+        hipLaunchKernelGGL(k_matrix_intensity, dim3(n), dim3(256), 0, s, P, (const TrialImage *)img, factor, ihi, ilo, iscale, kpad,
+                           t0, bf16);
     return hipGetLastError();
 }
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,

@@ -1396,6 +1396,17 @@ struct pt_group {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // sweep parts: disjoint sets of members, each with its own work tables, part p > 0 on
+    // side[p - 1] between a fork and a join (members are independent PT runs)
+    struct Part {
+        int2 *chain_tab = nullptr, *pair_tab = nullptr;
+        int4 *lines_tab = nullptr, *accum_tab[3] = {nullptr, nullptr, nullptr};
+        int n_chains = 0, n_pairs = 0, n_lines = 0, n_accum[3] = {0, 0, 0};
+    };
+    static constexpr int kMaxParts = 4;
+    std::vector<Part> parts;
+    hipStream_t side[kMaxParts - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxParts - 1] = {};
 };
 
 void pt_group_destroy(pt_group *g) {
@@ -1406,6 +1417,11 @@ void pt_group_destroy(pt_group *g) {
         drop_graph(g->gexec, g->graph);
         for (void *p : g->allocs) (void)hipFree(p);
         for (auto e : g->ev) if (e) (void)hipEventDestroy(e);
+        for (int p = 0; p + 1 < pt_group::kMaxParts; ++p) {
+            if (g->side[p]) { (void)hipStreamSynchronize(g->side[p]); (void)hipStreamDestroy(g->side[p]); }
+            if (g->join[p]) (void)hipEventDestroy(g->join[p]);
+        }
+        if (g->fork) (void)hipEventDestroy(g->fork);
         if (g->stream) (void)hipStreamDestroy(g->stream);
     }
     for (pt_plan *m : g->members) pt_plan_destroy(m);
@@ -1416,16 +1432,21 @@ static inline int tiles_of(int np, int tpw) {
     return (np + ptk::accum_points_per_group(tpw) - 1) / ptk::accum_points_per_group(tpw);
 }
 
-// XCD placement of the group's trials: heaviest first (estimated lines x points of the
-// banded sum) onto the least-loaded XCD.  Returns the trials in placement order with
-// their XCD.
+// estimated cost of one trial of member m (lines x points of the banded sum)
+static double trial_cost_estimate(const pt_plan *pl) {
+    const double nl = 2.0 * pl->nr;
+    return nl * tiles_of(pl->np, 2) + nl * (pl->np + 100.0 * tiles_of(pl->np, 2)) / std::max(1, pl->np) * 8.0;
+}
+
+// XCD placement of the trials of the members flagged in `use` (all when empty): heaviest
+// first onto the least-loaded XCD.  Returns the trials in placement order with their XCD.
 struct GroupTrial { double cost; int m, c, x; };
-static std::vector<GroupTrial> place_trials(const pt_group *g) {
+static std::vector<GroupTrial> place_trials(const pt_group *g, const std::vector<char> &use = {}) {
     std::vector<GroupTrial> trials;
     for (int m = 0; m < (int)g->members.size(); ++m) {
+        if (!use.empty() && !use[m]) continue;
         const pt_plan *pl = g->members[m];
-        const double nl = 2.0 * pl->nr;
-        const double cost = nl * tiles_of(pl->np, 2) + nl * (pl->np + 100.0 * tiles_of(pl->np, 2)) / std::max(1, pl->np) * 8.0;
+        const double cost = trial_cost_estimate(pl);
         for (int c = 0; c < pl->chains; ++c) trials.push_back({cost, m, c, 0});
     }
     std::stable_sort(trials.begin(), trials.end(), [](const GroupTrial &a, const GroupTrial &b) { return a.cost > b.cost; });
@@ -1522,6 +1543,53 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
         g->n_accum[v] = (int)tab.size();
         if ((rc = upload(g, &g->d_accum_tab[v], tab))) return bail(rc);
     }
+    // sweep parts: members heaviest first onto the lightest part, as many parts as keep at
+    // least 64 chains each (up to four, never more than the members)
+    const int nparts = std::max(1, std::min({pt_group::kMaxParts, count, g->total_chains / 64}));
+    if (nparts > 1) {
+        std::vector<int> order(count);
+        for (int m = 0; m < count; ++m) order[m] = m;
+        auto mcost = [&](int m) { return trial_cost_estimate(g->members[m]) * g->members[m]->chains; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return mcost(a) > mcost(b); });
+        std::vector<double> load(nparts, 0.0);
+        std::vector<int> part_of(count, 0);
+        for (int m : order) {
+            int p = 0;
+            for (int k = 1; k < nparts; ++k)
+                if (load[k] < load[p]) p = k;
+            load[p] += mcost(m);
+            part_of[m] = p;
+        }
+        bool ok = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
+        for (int p = 0; ok && p + 1 < nparts; ++p)
+            ok = hipStreamCreateWithFlags(&g->side[p], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&g->join[p], hipEventDisableTiming) == hipSuccess;
+        if (!ok) return bail(fail(PT_ERR_DEVICE, "PT group side stream creation failed."));
+        g->parts.resize(nparts);
+        for (int p = 0; p < nparts; ++p) {
+            pt_group::Part &P = g->parts[p];
+            std::vector<char> use(count, 0);
+            std::vector<int2> ct, pt;
+            for (int m = 0; m < count; ++m) {
+                if (part_of[m] != p) continue;
+                use[m] = 1;
+                for (int c = 0; c < g->members[m]->chains; ++c) ct.push_back(make_int2(m, c));
+                for (int q = 0; q < (g->members[m]->chains + 1) / 2; ++q) pt.push_back(make_int2(m, q));
+            }
+            const std::vector<GroupTrial> pl_p = place_trials(g, use);
+            std::vector<int4> lt = xcd_table(pl_p, [&](int m) { return (g->members[m]->nr + 255) / 256; });
+            P.n_chains = (int)ct.size();
+            P.n_pairs = (int)pt.size();
+            P.n_lines = (int)lt.size();
+            if ((rc = upload(g, &P.chain_tab, ct)) || (rc = upload(g, &P.pair_tab, pt)) || (rc = upload(g, &P.lines_tab, lt)))
+                return bail(rc);
+            for (int v = 0; v < 3; ++v) {
+                std::vector<int4> tab = xcd_table(pl_p, [&](int m) { return tiles_of(g->members[m]->np, 1 << v); });
+                P.n_accum[v] = (int)tab.size();
+                if ((rc = upload(g, &P.accum_tab[v], tab))) return bail(rc);
+            }
+        }
+    }
     *out = g;
     return 0;
 }
@@ -1565,19 +1633,40 @@ static int group_pass(pt_group *g, int pt_mode) {
     return 0;
 }
 
-static int group_sweep(pt_group *g) {
-    hipStream_t s = g->stream;
-    const int v = vidx(g->tpw);
-    HIPCHECK(ptk::launch_group_setup(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, 0, s), "setup");
+// one sweep of the members behind the given work tables, on stream s
+static int group_sweep_tables(pt_group *g, hipStream_t s, const int2 *chain_tab, int n_chains, const int2 *pair_tab, int n_pairs,
+                              const int4 *lines_tab, int n_lines, const int4 *accum_tab, int n_accum) {
+    HIPCHECK(ptk::launch_group_setup(g->d_plans, g->d_mem, chain_tab, n_chains, 0, s), "setup");
     for (int k = 0; k < g->steps; ++k) {
-        HIPCHECK(ptk::launch_group_lines(g->d_plans, g->d_mem, g->d_lines_tab, g->n_lines, s), "lines");
-        HIPCHECK(ptk::launch_group_accum(g->d_plans, g->d_mem, g->d_accum_tab[v], g->n_accum[v], g->tpw, s), "accumulate");
+        HIPCHECK(ptk::launch_group_lines(g->d_plans, g->d_mem, lines_tab, n_lines, s), "lines");
+        HIPCHECK(ptk::launch_group_accum(g->d_plans, g->d_mem, accum_tab, n_accum, g->tpw, s), "accumulate");
         if (k + 1 < g->steps)
-            HIPCHECK(ptk::launch_group_finish_setup(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, s), "finish");
+            HIPCHECK(ptk::launch_group_finish_setup(g->d_plans, g->d_mem, chain_tab, n_chains, s), "finish");
         else
-            HIPCHECK(ptk::launch_group_finish(g->d_plans, g->d_mem, g->d_chain_tab, g->total_chains, 1, s), "finish");
+            HIPCHECK(ptk::launch_group_finish(g->d_plans, g->d_mem, chain_tab, n_chains, 1, s), "finish");
     }
-    HIPCHECK(ptk::launch_group_exchange(g->d_plans, g->d_pair_tab, g->total_pairs, s), "exchange");
+    HIPCHECK(ptk::launch_group_exchange(g->d_plans, pair_tab, n_pairs, s), "exchange");
+    return 0;
+}
+
+static int group_sweep(pt_group *g) {
+    const int v = vidx(g->tpw);
+    if (g->parts.size() < 2)
+        return group_sweep_tables(g, g->stream, g->d_chain_tab, g->total_chains, g->d_pair_tab, g->total_pairs, g->d_lines_tab,
+                                  g->n_lines, g->d_accum_tab[v], g->n_accum[v]);
+    const int np = (int)g->parts.size();
+    HIPCHECK(hipEventRecord(g->fork, g->stream), "fork");
+    for (int p = 1; p < np; ++p) HIPCHECK(hipStreamWaitEvent(g->side[p - 1], g->fork, 0), "fork");
+    for (int p = 0; p < np; ++p) {
+        const pt_group::Part &P = g->parts[p];
+        int rc = group_sweep_tables(g, p ? g->side[p - 1] : g->stream, P.chain_tab, P.n_chains, P.pair_tab, P.n_pairs, P.lines_tab,
+                                    P.n_lines, P.accum_tab[v], P.n_accum[v]);
+        if (rc) return rc;
+    }
+    for (int p = 1; p < np; ++p) {
+        HIPCHECK(hipEventRecord(g->join[p - 1], g->side[p - 1]), "join");
+        HIPCHECK(hipStreamWaitEvent(g->stream, g->join[p - 1], 0), "join");
+    }
     return 0;
 }
 

@@ -74,6 +74,27 @@ def test_group_matches_single_plans(members):
         assert np.array_equal(got[m], want), (MEMBERS[m], np.max(np.abs(got[m] - want)))
 
 
+def test_group_parts_match_single_plans():
+    """With 256 chains (eight members of 32) the group sweep runs the members as four parts
+    on their own streams; the states still equal the members run as separate plans."""
+    pbs = [PR.uneven_pattern(i, simulate=O.oracle_simulate, chains=32) for i in MEMBERS]
+    steps = 4
+    with pt.Group(pbs, steps_per_sweep=steps) as g:
+        g.tiles(1)
+        for m, pb in enumerate(pbs):
+            g.initial(m, [pb.start_vector()])
+        g.batch(1, bootstrap=True)
+        got = g.batch(2)
+    for m, pb in enumerate(pbs):
+        pb.steps_per_sweep = steps
+        with pt.Plan(pb) as pl:
+            pl.tiles(1)
+            pl.initial([pb.start_vector()])
+            pl.batch(1, bootstrap=True)
+            want = pl.batch(2)
+        assert np.array_equal(got[m], want), (MEMBERS[m], np.max(np.abs(got[m] - want)))
+
+
 def test_group_tuner_and_invariants(members):
     with pt.Group(members) as g:
         for m, pb in enumerate(members):

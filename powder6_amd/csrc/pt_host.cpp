@@ -840,9 +840,12 @@ static int time_tiles(hipStream_t s, F launch, double ms[3], int *best) {
     (void)hipEventDestroy(e[0]);
     (void)hipEventDestroy(e[1]);
     if (rc) return rc;
-    int b = 1;
-    for (int i = 0; i < 3; ++i)
-        if (ms[i] < ms[b]) b = i;
+    // the smallest shape unless a larger one is clearly faster: smaller tiles give more
+    // workgroups per launch, which the sweep's parts overlap better than the isolated
+    // timing shows, and a near tie is timing noise
+    int b = 0;
+    for (int i = 1; i < 3; ++i)
+        if (ms[i] < 0.95 * ms[b]) b = i;
     *best = cand[b];
     return 0;
 }

@@ -880,16 +880,23 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     for (int c0 = first; c0 < lend; c0 += 256) {
         // compact the overlapping lines of this chunk, in order, into the stage
         const int L = c0 + tid;
+        // the record is loaded with its window (one round trip), used only if the window
+        // meets this workgroup's points
         int4 sv = make_int4(0, 0, 0, 0);
+        LineRec R{};
         bool ov = false;
-        if (L < lend) { sv = sp[L]; ov = sv.x < hi_pt && sv.y > lo_pt; }
+        if (L < lend) {
+            sv = sp[L];
+            R = rec[L];
+            ov = sv.x < hi_pt && sv.y > lo_pt;
+        }
         const unsigned long long m = __ballot(ov);
         if (lane == 0) S.wcnt[wave] = __popcll(m);
         __syncthreads();
         int off = S.count;
         for (int w = 0; w < wave; ++w) off += S.wcnt[w];
         const int pre = __popcll(m & ((1ull << lane) - 1ull));
-        if (ov) S.line[off + pre] = stage_line(P, rec[L], sv, uni);
+        if (ov) S.line[off + pre] = stage_line(P, R, sv, uni);
         __syncthreads();
         if (tid == 0) { int n = S.count; for (int w = 0; w < kAccWaves; ++w) n += S.wcnt[w]; S.count = n; }
         __syncthreads();

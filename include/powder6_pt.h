@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 1
+#define PT_ABI_VERSION 2
 
 /* ---- status codes -------------------------------------------------------------- */
 #define PT_OK 0
@@ -41,6 +41,8 @@ enum { PT_TRICLINIC = 0, PT_MONOCLINIC = 1, PT_ORTHORHOMBIC = 2, PT_TETRAGONAL =
 /* profile families, js/profile.js:603-718 (type 1 simple, 2 split, 3 TCH) */
 enum { PT_PROFILE_SIMPLE = 1, PT_PROFILE_SPLIT = 2, PT_PROFILE_TCH = 3 };
 enum { PT_XRAY = 0, PT_NEUTRON = 1 };
+/* background forms, js/rietveld_core.js:622-627 bkgType ('cheb' | 'spline') */
+enum { PT_BKG_CHEBYSHEV = 0, PT_BKG_SPLINE = 1 };
 
 /* ---- model-parameter layout ----------------------------------------------------
  * The model parameters of one dataset, the names js/rietveld_core.js reads from its
@@ -102,7 +104,7 @@ typedef struct {
     const double *tth, *y_obs, *weight;
     double lambda1, lambda2, ratio, pol_k, min_fwhm;
     int32_t radiation;     /* PT_XRAY / PT_NEUTRON */
-    int32_t n_bkg;         /* Chebyshev terms, 0..PT_MAX_BKG */
+    int32_t n_bkg;         /* Chebyshev terms, or spline anchors (bkg_type), 0..PT_MAX_BKG */
     /* phases */
     int32_t n_phases;
     const pt_phase *phases;
@@ -119,6 +121,13 @@ typedef struct {
     int32_t chains, sweeps, steps_per_sweep;
     double max_temp;
     uint32_t seed;
+    /* background form.  PT_BKG_SPLINE: a monotone cubic Hermite spline through n_bkg
+     * anchors at bkg_x (ascending 2theta), anchor j at the value of the model parameter
+     * in slot PT_P_BKG0 + j (the reference's bga<j>), flat outside the anchors
+     * (js/rietveld_core.js:549-612 splineSlopes / splineValues).  bkg_x is unused for
+     * PT_BKG_CHEBYSHEV. */
+    int32_t bkg_type;
+    const double *bkg_x;
 } pt_problem;
 
 /* Plan summary: the numeric fields of pt_gpu.js compile() (js/pt_gpu.js:226).        */

@@ -584,6 +584,7 @@ struct or_model {
     int32_t *tparam, *tvar;
     or_phase *ph;
     double **T;                      /* Chebyshev T_k(x_i) */
+    double *bx;                      /* spline anchors (PT_BKG_SPLINE) */
     double guard_lo, guard_hi;
     int has_key;
     double *key;                     /* cache key: zero, disp, transp, cells */
@@ -634,15 +635,24 @@ or_model *or_create(const pt_problem *prob) {
         memcpy(m->tparam, prob->term_param, sizeof(int32_t) * prob->n_terms);
         memcpy(m->tvar, prob->term_var, sizeof(int32_t) * prob->n_terms);
     }
-    /* rietveld_core.js:647-653 Chebyshev polynomials over the fitted range */
+    if (prob->bkg_type == PT_BKG_SPLINE) {
+        m->bx = dup_d(prob->bkg_x, prob->n_bkg);
+        m->P.bkg_x = m->bx;
+        for (int j = 1; j < prob->n_bkg; j++)
+            if (!(m->bx[j] >= m->bx[j - 1])) { set_err("spline anchors must ascend"); }
+    }
+    /* rietveld_core.js:647-653 Chebyshev polynomials over the fitted range (none for a
+     * spline background, :639-640) */
     double lo = m->tth[0], hi = m->tth[n - 1];
+    const int n_cheb = prob->bkg_type == PT_BKG_SPLINE ? 0 : prob->n_bkg;
+    m->P.n_bkg = prob->n_bkg;
     m->T = (double **)calloc(prob->n_bkg > 0 ? prob->n_bkg : 1, sizeof(double *));
-    for (int k = 0; k < prob->n_bkg; k++) m->T[k] = (double *)malloc(sizeof(double) * n);
+    for (int k = 0; k < n_cheb; k++) m->T[k] = (double *)malloc(sizeof(double) * n);
     for (int i = 0; i < n; i++) {
         double x = hi > lo ? 2 * (m->tth[i] - lo) / (hi - lo) - 1 : 0;
-        if (prob->n_bkg > 0) m->T[0][i] = 1;
-        if (prob->n_bkg > 1) m->T[1][i] = x;
-        for (int k = 2; k < prob->n_bkg; k++) m->T[k][i] = 2 * x * m->T[k - 1][i] - m->T[k - 2][i];
+        if (n_cheb > 0) m->T[0][i] = 1;
+        if (n_cheb > 1) m->T[1][i] = x;
+        for (int k = 2; k < n_cheb; k++) m->T[k][i] = 2 * x * m->T[k - 1][i] - m->T[k - 2][i];
     }
     /* rietveld_core.js:1422-1424 guard range in tan(theta) */
     m->guard_lo = tan(dmax(1e-6, dmin(179.999999, m->tth[0])) * JS_PI / 360);
@@ -659,8 +669,9 @@ void or_destroy(or_model *m) {
     for (int p = 0; p < m->P.n_phases; p++) {
         free(m->atoms[p]); free(m->op_r[p]); free(m->op_t[p]); free(m->ph[p].rep); rl_free(&m->ph[p].refl);
     }
-    for (int k = 0; k < m->P.n_bkg; k++) free(m->T[k]);
-    free(m->T); free(m->atoms); free(m->op_r); free(m->op_t); free(m->ph); free(m->phases);
+    if (m->P.bkg_type != PT_BKG_SPLINE)
+        for (int k = 0; k < m->P.n_bkg; k++) free(m->T[k]);
+    free(m->T); free(m->bx); free(m->atoms); free(m->op_r); free(m->op_t); free(m->ph); free(m->phases);
     free(m->tth); free(m->y); free(m->w); free(m->base); free(m->lo); free(m->hi); free(m->scale);
     free(m->x0); free(m->coeff); free(m->tparam); free(m->tvar); free(m->key); free(m->params);
     free(m->ybuf); free(m->col); free(m);
@@ -914,15 +925,60 @@ int or_cell_valid(or_model *m, const double *x) {
     return cell_ok(m, m->params);
 }
 
+/* rietveld_core.js:549-561 splineSlopes: Fritsch-Carlson tangents of the monotone cubic
+ * Hermite spline through (xs, ys), limited to tau = a^2 + b^2 <= 9 per interval */
+void or_spline_slopes(const double *xs, const double *ys, int na, double *h, double *mm) {
+    if (na < 2) { if (na == 1) mm[0] = 0; return; }
+    double *d = (double *)malloc(sizeof(double) * (na - 1));
+    for (int i = 0; i < na - 1; i++) { h[i] = xs[i + 1] - xs[i]; d[i] = (ys[i + 1] - ys[i]) / h[i]; }
+    mm[0] = d[0]; mm[na - 1] = d[na - 2];
+    for (int i = 1; i < na - 1; i++) mm[i] = d[i - 1] * d[i] <= 0 ? 0 : (h[i] * d[i - 1] + h[i - 1] * d[i]) / (h[i - 1] + h[i]);
+    for (int i = 0; i < na - 1; i++) {
+        if (d[i] == 0) { mm[i] = 0; mm[i + 1] = 0; continue; }
+        double a = mm[i] / d[i], b = mm[i + 1] / d[i], tau = a * a + b * b;
+        if (tau > 9) { double f = 3 / sqrt(tau); mm[i] = a * d[i] * f; mm[i + 1] = b * d[i] * f; }
+    }
+    free(d);
+}
+
+/* rietveld_core.js:597-612 splineValues: the spline at every point of the sorted axis,
+ * flat outside the anchors; one anchor gives a constant, none zero */
+void or_spline_values(const double *tth, int n, const double *xs, const double *ys, int na, double *out) {
+    for (int i = 0; i < n; i++) out[i] = 0;
+    if (na < 1) return;
+    if (na == 1) { for (int i = 0; i < n; i++) out[i] = ys[0]; return; }
+    double *h = (double *)malloc(sizeof(double) * na), *mm = (double *)malloc(sizeof(double) * na);
+    or_spline_slopes(xs, ys, na, h, mm);
+    int seg = 0;
+    for (int i = 0; i < n; i++) {
+        double t = tth[i];
+        if (t <= xs[0]) { out[i] = ys[0]; continue; }
+        if (t >= xs[na - 1]) { out[i] = ys[na - 1]; continue; }
+        while (seg < na - 2 && xs[seg + 1] <= t) seg++;
+        while (seg > 0 && xs[seg] > t) seg--;
+        double hi = h[seg], u = hi > 1e-12 ? (t - xs[seg]) / hi : 0, u2 = u * u, u3 = u2 * u;
+        out[i] = (2 * u3 - 3 * u2 + 1) * ys[seg] + (u3 - 2 * u2 + u) * hi * mm[seg] + (-2 * u3 + 3 * u2) * ys[seg + 1] +
+                 (u3 - u2) * hi * mm[seg + 1];
+    }
+    free(h); free(mm);
+}
+
 /* Model.background (:920-931) + evalDataset (:1152-1164) */
 static int pattern_at_params(or_model *m, const double *p, double *y) {
     int n = m->P.n_points;
     refresh_reflections(m, p);
     for (int i = 0; i < n; i++) y[i] = 0;
-    for (int k = 0; k < m->P.n_bkg; k++) {
-        double b = p[PT_P_BKG0 + k];
-        if (!(b != 0) || isnan(b)) continue;
-        for (int i = 0; i < n; i++) y[i] += b * m->T[k][i];
+    if (m->P.bkg_type == PT_BKG_SPLINE) {
+        /* Model.splineY (:932-936): anchor j takes bga<j> (0 when unset) */
+        double ys[PT_MAX_BKG];
+        for (int j = 0; j < m->P.n_bkg; j++) ys[j] = isnan(p[PT_P_BKG0 + j]) ? 0 : p[PT_P_BKG0 + j];
+        or_spline_values(m->tth, n, m->bx, ys, m->P.n_bkg, y);
+    } else {
+        for (int k = 0; k < m->P.n_bkg; k++) {
+            double b = p[PT_P_BKG0 + k];
+            if (!(b != 0) || isnan(b)) continue;
+            for (int i = 0; i < n; i++) y[i] += b * m->T[k][i];
+        }
     }
     /* each phase renders into its own array, then addTo(y, ph.y) */
     double *yp = (double *)malloc(sizeof(double) * n);

@@ -65,6 +65,10 @@ const char *or_error(void);
 void or_params(const or_model *m, const double *x, double *p);
 /* chi^2 at variables x: rietveld_core.js:1878-1889 (+inf when invalid) */
 double or_cost(or_model *m, const double *x);
+/* the spline background (rietveld_core.js:549-612): tangents mm[na] and interval widths
+ * h[na - 1] of the anchors (xs, ys); values at the n sorted points tth */
+void or_spline_slopes(const double *xs, const double *ys, int na, double *h, double *mm);
+void or_spline_values(const double *tth, int n, const double *xs, const double *ys, int na, double *out);
 /* calculated pattern (background + phases) at x; returns 0, or -1 for an invalid cell */
 int or_pattern(or_model *m, const double *x, double *ycalc);
 /* per-reflection diagnostics of phase 0 at x: F^2, height, Kalpha1 centre (count out) */

@@ -263,6 +263,8 @@ struct pt_plan {
     int screen_cap = 0;
     std::vector<double> tth_host;
     int n_bkg = 0;
+    int bkg_type = PT_BKG_CHEBYSHEV;
+    std::vector<double> bkg_x;         // spline anchors
     hipEvent_t ev_total[2] = {nullptr, nullptr};
     double last_total_ms = 0, last_kernel_ms = 0, last_matrix_ms[2] = {0, 0};
     int last_launches = 0;
@@ -370,6 +372,13 @@ int validate(const pt_problem *pb) {
     }
     if (pb->n_phases < 1 || pb->n_phases > PTK_MAX_PHASES) return fail(PT_ERR_ARG, "PT supports 1 to 4 phases per dataset.");
     if (pb->n_bkg < 0 || pb->n_bkg > PT_MAX_BKG) return fail(PT_ERR_ARG, "Choose 0 to 24 Chebyshev terms.");
+    if (pb->bkg_type != PT_BKG_CHEBYSHEV && pb->bkg_type != PT_BKG_SPLINE) return fail(PT_ERR_ARG, "unknown background form");
+    if (pb->bkg_type == PT_BKG_SPLINE) {
+        if (pb->n_bkg > 0 && !pb->bkg_x) return fail(PT_ERR_ARG, "spline background anchors are missing");
+        for (int j = 0; j < pb->n_bkg; ++j)
+            if (!std::isfinite(pb->bkg_x[j]) || (j && !(pb->bkg_x[j] > pb->bkg_x[j - 1])))
+                return fail(PT_ERR_ARG, "spline background anchors must be finite, distinct and ascending");
+    }
     int atoms = 0;
     for (int q = 0; q < pb->n_phases; ++q) {
         const pt_phase &ph = pb->phases[q];
@@ -425,6 +434,81 @@ void pt_plan_destroy(pt_plan *pl) {
         mlpx::destroy(pl->mlp);
     }
     delete pl;
+}
+
+// Spline background point table: for every point the anchor interval s and the four
+// Hermite weights, so that the device forms bkg(t) = c0 y_s + c1 m_s + c2 y_s+1 + c3 m_s+1
+// from a trial's anchor values y and tangents m (js/rietveld_core.js:597-612: flat
+// outside the anchors, one anchor a constant, none zero; c1, c3 carry the interval width).
+static void spline_point_table(const double *tth, int n, const double *xs, int na, std::vector<int> &seg,
+                               std::vector<float4> &coef) {
+    int s = 0;
+    for (int i = 0; i < n; ++i) {
+        const double t = tth[i];
+        if (na < 1) { seg[i] = 0; coef[i] = make_float4(0.f, 0.f, 0.f, 0.f); continue; }
+        if (na == 1 || t <= xs[0]) { seg[i] = 0; coef[i] = make_float4(1.f, 0.f, 0.f, 0.f); continue; }
+        if (t >= xs[na - 1]) { seg[i] = na - 2; coef[i] = make_float4(0.f, 0.f, 1.f, 0.f); continue; }
+        while (s < na - 2 && xs[s + 1] <= t) ++s;
+        while (s > 0 && xs[s] > t) --s;
+        const double h = xs[s + 1] - xs[s], u = h > 1e-12 ? (t - xs[s]) / h : 0.0, u2 = u * u, u3 = u2 * u;
+        seg[i] = s;
+        coef[i] = make_float4((float)(2 * u3 - 3 * u2 + 1), (float)((u3 - 2 * u2 + u) * h), (float)(-2 * u3 + 3 * u2),
+                              (float)((u3 - u2) * h));
+    }
+}
+
+// Fritsch-Carlson tangents of the anchors (js/rietveld_core.js:549-561)
+static void spline_tangents(const double *xs, const double *ys, int na, double *m) {
+    if (na < 2) { if (na == 1) m[0] = 0; return; }
+    std::vector<double> h(na - 1), d(na - 1);
+    for (int i = 0; i < na - 1; ++i) { h[i] = xs[i + 1] - xs[i]; d[i] = (ys[i + 1] - ys[i]) / h[i]; }
+    m[0] = d[0]; m[na - 1] = d[na - 2];
+    for (int i = 1; i < na - 1; ++i) m[i] = d[i - 1] * d[i] <= 0 ? 0 : (h[i] * d[i - 1] + h[i - 1] * d[i]) / (h[i - 1] + h[i]);
+    for (int i = 0; i < na - 1; ++i) {
+        if (d[i] == 0) { m[i] = 0; m[i + 1] = 0; continue; }
+        const double a = m[i] / d[i], b = m[i + 1] / d[i], tau = a * a + b * b;
+        if (tau > 9) { const double f = 3 / std::sqrt(tau); m[i] = a * d[i] * f; m[i + 1] = b * d[i] * f; }
+    }
+}
+
+// the background of the start vector at every point, FP64 (Chebyshev terms over the fitted
+// range, js/rietveld_core.js:647-653 and 920-931, or the spline)
+static std::vector<double> start_background(const pt_plan *pl) {
+    const int n = pl->np;
+    std::vector<double> bk(n, 0.0);
+    if (pl->bkg_type == PT_BKG_SPLINE) {
+        const int na = pl->n_bkg;
+        if (na < 1) return bk;
+        std::vector<double> ys(na), m(na);
+        for (int j = 0; j < na; ++j) { const double v = pl->base_params[PT_P_BKG0 + j]; ys[j] = std::isnan(v) ? 0.0 : v; }
+        spline_tangents(pl->bkg_x.data(), ys.data(), na, m.data());
+        std::vector<int> seg(n);
+        std::vector<float4> c(n);
+        const double *xs = pl->bkg_x.data();
+        int s = 0;
+        for (int i = 0; i < n; ++i) {
+            const double t = pl->tth_host[i];
+            if (na == 1 || t <= xs[0]) { bk[i] = ys[0]; continue; }
+            if (t >= xs[na - 1]) { bk[i] = ys[na - 1]; continue; }
+            while (s < na - 2 && xs[s + 1] <= t) ++s;
+            while (s > 0 && xs[s] > t) --s;
+            const double h = xs[s + 1] - xs[s], u = h > 1e-12 ? (t - xs[s]) / h : 0.0, u2 = u * u, u3 = u2 * u;
+            bk[i] = (2 * u3 - 3 * u2 + 1) * ys[s] + (u3 - 2 * u2 + u) * h * m[s] + (-2 * u3 + 3 * u2) * ys[s + 1] +
+                    (u3 - u2) * h * m[s + 1];
+        }
+        return bk;
+    }
+    const double lo = pl->tth_host[0], hi = pl->tth_host[n - 1];
+    for (int i = 0; i < n; ++i) {
+        double z = hi > lo ? 2 * (pl->tth_host[i] - lo) / (hi - lo) - 1 : 0, ta = 1, tb = z, b = 0;
+        for (int k = 0; k < pl->n_bkg; ++k) {
+            double T = k == 0 ? 1 : k == 1 ? z : 2 * z * tb - ta;
+            if (k >= 2) { ta = tb; tb = T; }
+            b += pl->base_params[PT_P_BKG0 + k] * T;
+        }
+        bk[i] = b;
+    }
+    return bk;
 }
 
 static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
@@ -556,6 +640,9 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     pl->uniform = uniform;
     std::vector<float2> tff(n), obs(n);
     std::vector<float> zcheb(n);
+    std::vector<int> bseg(n, 0);
+    std::vector<float4> bcoef(n, make_float4(0.f, 0.f, 0.f, 0.f));
+    if (pb->bkg_type == PT_BKG_SPLINE) spline_point_table(pb->tth, n, pb->bkg_x, pb->n_bkg, bseg, bcoef);
     const double tlo = pb->tth[0], thi = pb->tth[n - 1];
     for (int i = 0; i < n; ++i) {
         float hi = (float)pb->tth[i];
@@ -619,6 +706,8 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     for (int k = 0; k < pb->n_params; ++k) pl->base_params[k] = param_value(pb, k);
     pl->tth_host = tth;
     pl->n_bkg = pb->n_bkg;
+    pl->bkg_type = pb->bkg_type;
+    if (pb->bkg_type == PT_BKG_SPLINE) pl->bkg_x.assign(pb->bkg_x, pb->bkg_x + pb->n_bkg);
     pl->struct_var.assign(pb->n_vars, 1);
     for (int t = 0; t < pb->n_terms; ++t) {
         // structure parameters: atom slots and the phase scale of some phase
@@ -631,10 +720,14 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     }
     std::vector<double> vlo(pb->lo, pb->lo + pb->n_vars), vhi(pb->hi, pb->hi + pb->n_vars), vsc(pb->scale, pb->scale + pb->n_vars);
     if (eq.empty()) eq.push_back(0);
-    double *d_tth; float2 *d_tff, *d_obs; float *d_zcheb; int4 *d_hklm; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
+    double *d_tth, *d_bknots = nullptr; float2 *d_tff, *d_obs; float *d_zcheb; int4 *d_hklm; int *d_bseg = nullptr;
+    float4 *d_bcoef = nullptr; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
     short *d_eq; double *d_rep_t, *d_base, *d_coeff, *d_lo, *d_hi, *d_sc, *d_state; ptk::DevAtom *d_atoms;
     if ((rc = upload(pl, &d_tth, tth)) || (rc = upload(pl, &d_tff, tff)) || (rc = upload(pl, &d_obs, obs)) ||
         (rc = upload(pl, &d_zcheb, zcheb)) ||
+        (pb->bkg_type == PT_BKG_SPLINE &&
+         ((rc = upload(pl, &d_bseg, bseg)) || (rc = upload(pl, &d_bcoef, bcoef)) ||
+          (rc = upload(pl, &d_bknots, std::vector<double>(pb->bkg_x, pb->bkg_x + std::max(1, pb->n_bkg)))))) ||
         (rc = upload(pl, &d_hklm, hklm)) || (rc = upload(pl, &d_eq_off, eq_off)) || (rc = upload(pl, &d_eq, eq)) ||
         (rc = upload(pl, &d_rep_r, rep_r)) || (rc = upload(pl, &d_rep_t, rep_t)) || (rc = upload(pl, &d_atoms, atoms)) ||
         (rc = upload(pl, &d_base, base)) || (rc = upload(pl, &d_term_off, term_off)) || (rc = upload(pl, &d_term_var, term_var)) ||
@@ -643,6 +736,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         pt_plan_destroy(pl);
         return rc;
     }
+    P.bkg_spline = pb->bkg_type == PT_BKG_SPLINE ? 1 : 0; P.bseg = d_bseg; P.bcoef = d_bcoef; P.bknots = d_bknots;
     P.tth = d_tth; P.tff = d_tff; P.zcheb = d_zcheb; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
     P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
     P.term_var = d_term_var; P.term_coeff = d_coeff; P.vlo = d_lo; P.vhi = d_hi; P.vscale = d_sc; P.state = d_state;
@@ -1214,22 +1308,13 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
         }
         band[rt] = kmin < 0 ? make_int2(0, 0) : make_int2(kmin / 32 * 32, std::min(pl->kpad, (kmax + 32) / 32 * 32));
     }
-    // y_obs - background of the start vector (rietveld_core.js:647-653, 920-931), FP64 on the host
+    // y_obs - background of the start vector, FP64 on the host
     std::vector<float2> yw(pl->np_pad, make_float2(0.f, 0.f));
-    std::vector<double> yobs(pl->np), w(pl->np);
     {
         std::vector<float2> obs(pl->np);
         HIPCHECK(hipMemcpy(obs.data(), pl->P.obs, sizeof(float2) * pl->np, hipMemcpyDeviceToHost), "readback");
-        double lo = pl->tth_host[0], hi = pl->tth_host[pl->np - 1];
-        for (int i = 0; i < pl->np; ++i) {
-            double z = hi > lo ? 2 * (pl->tth_host[i] - lo) / (hi - lo) - 1 : 0, ta = 1, tb = z, bk = 0;
-            for (int k = 0; k < pl->n_bkg; ++k) {
-                double T = k == 0 ? 1 : k == 1 ? z : 2 * z * tb - ta;
-                if (k >= 2) { ta = tb; tb = T; }
-                bk += pl->base_params[PT_P_BKG0 + k] * T;
-            }
-            yw[i] = make_float2((float)((double)obs[i].x - bk), obs[i].y);
-        }
+        const std::vector<double> bk = start_background(pl);
+        for (int i = 0; i < pl->np; ++i) yw[i] = make_float2((float)((double)obs[i].x - bk[i]), obs[i].y);
     }
     HIPCHECK(hipMemcpy(pl->d_band, band.data(), sizeof(int2) * band.size(), hipMemcpyHostToDevice), "upload");
     HIPCHECK(hipMemcpy(pl->d_yw, yw.data(), sizeof(float2) * yw.size(), hipMemcpyHostToDevice), "upload");
@@ -1314,16 +1399,8 @@ int pt_matrix_pattern(pt_plan *pl, const double *x, double *ycalc) {
     int rc = eval_matrix(pl, x, 1, &chi2, ycalc);
     if (rc) return rc;
     // add the background of the start vector back
-    double lo = pl->tth_host[0], hi = pl->tth_host[pl->np - 1];
-    for (int i = 0; i < pl->np; ++i) {
-        double z = hi > lo ? 2 * (pl->tth_host[i] - lo) / (hi - lo) - 1 : 0, ta = 1, tb = z, bk = 0;
-        for (int k = 0; k < pl->n_bkg; ++k) {
-            double T = k == 0 ? 1 : k == 1 ? z : 2 * z * tb - ta;
-            if (k >= 2) { ta = tb; tb = T; }
-            bk += pl->base_params[PT_P_BKG0 + k] * T;
-        }
-        ycalc[i] += bk;
-    }
+    const std::vector<double> bk = start_background(pl);
+    for (int i = 0; i < pl->np; ++i) ycalc[i] += bk[i];
     return 0;
 }
 

@@ -24,7 +24,7 @@ struct SetupShared {
     double apos[PTK_MAX_ATOMS][3];            // atom positions (fractional)
     float aocc[PTK_MAX_ATOMS], aB[PTK_MAX_ATOMS];
     float prof[PTK_MAX_PHASES][16];           // profile slots (FP32)
-    float bkg[PTK_MAX_BKG_F];                 // Chebyshev coefficients (FP32)
+    float bkg[PTK_MAX_BKG_F];                 // Chebyshev coefficients, or spline values [0, 24) and tangents [24, 48)
     float red_s[32], red_c[32];
     int phase_ok[PTK_MAX_PHASES];
     int cell_ok, prof_ok, go, accept;
@@ -62,6 +62,32 @@ __device__ inline void stage_reps(const DevPlan &P, Shared &sh) {
     }
 }
 
+// Spline background of one trial: anchor values (NaN read as 0) into out[0, n) and their
+// Fritsch-Carlson tangents into out[24, 24 + n), slopes in FP64 from the FP64 parameters
+// (rietveld_core.js:549-561, 597-612).  One thread; n <= 24; out is LDS, so no private array.
+__device__ inline double spline_y(const double *pv, int j) { const double v = pv[j]; return isnan(v) ? 0.0 : v; }
+__device__ inline void spline_setup(const DevPlan &P, const double *pv, float *out) {
+    const int na = P.n_bkg;
+    const double *xs = P.bknots;
+    float *m = out + PTK_MAX_BKG;
+    for (int j = 0; j < na; ++j) { out[j] = (float)spline_y(pv, j); m[j] = 0.0f; }
+    if (na < 2) return;
+    double dprev = 0.0, hprev = 0.0;
+    for (int j = 0; j + 1 < na; ++j) {
+        const double h = xs[j + 1] - xs[j], d = (spline_y(pv, j + 1) - spline_y(pv, j)) / h;
+        if (j == 0) m[0] = (float)d;
+        else m[j] = dprev * d <= 0.0 ? 0.0f : (float)((h * dprev + hprev * d) / (hprev + h));
+        dprev = d; hprev = h;
+    }
+    m[na - 1] = (float)dprev;
+    for (int j = 0; j + 1 < na; ++j) {
+        const double d = (spline_y(pv, j + 1) - spline_y(pv, j)) / (xs[j + 1] - xs[j]);
+        if (d == 0.0) { m[j] = 0.0f; m[j + 1] = 0.0f; continue; }
+        const double a = m[j] / d, b = m[j + 1] / d, tau = a * a + b * b;
+        if (tau > 9.0) { const double f = 3.0 / sqrt(tau); m[j] = (float)(a * d * f); m[j + 1] = (float)(b * d * f); }
+    }
+}
+
 // p = base + M x  (rietveld_core.js:1855-1859 apply, affine links of atomic_constraints.js)
 __device__ inline void materialize(const DevPlan &P, const double *x, SetupShared &sh) {
     for (int k = threadIdx.x; k < P.n_params; k += blockDim.x) {
@@ -84,7 +110,11 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
         int q = i >> 4, s = i & 15;
         sh.prof[q][s] = (float)sh.p[P.phase[q].param_base + 8 + s];
     }
-    for (int i = tid; i < P.n_bkg; i += nt) sh.bkg[i] = (float)sh.p[3 + i];
+    if (!P.bkg_spline) {
+        for (int i = tid; i < P.n_bkg; i += nt) sh.bkg[i] = (float)sh.p[3 + i];
+    } else if (tid == nt - 1) {
+        spline_setup(P, sh.p + 3, sh.bkg);
+    }
     for (int q = 0; q < P.n_phases; ++q) {
         const DevPhase &ph = P.phase[q];
         for (int j = tid; j < ph.n_atoms; j += nt) {
@@ -330,19 +360,30 @@ __device__ __forceinline__ void neumaier(float &s, float &c, float v) {
     s = t;
 }
 
-// Chebyshev background at point i (rietveld_core.js:647-653, 920-931), FP32 recurrence
-__device__ __forceinline__ float background_at(const DevPlan &P, const Shared &sh, int i) {
+// Background at point i from a trial's coefficients: the Chebyshev sum over the fitted range
+// (rietveld_core.js:647-653, 920-931) as an FP32 recurrence, or the spline's cubic Hermite
+// form on the point's anchor interval with weights computed once per plan
+// (rietveld_core.js:597-612)
+__device__ __forceinline__ float bkg_value(const DevPlan &P, const float *bkg, int i) {
     if (P.n_bkg <= 0) return 0.0f;
+    if (P.bkg_spline) {
+        const int s = P.bseg[i];
+        const float4 c = P.bcoef[i];
+        const float y1 = P.n_bkg > 1 ? bkg[s + 1] : 0.0f, m1 = P.n_bkg > 1 ? bkg[PTK_MAX_BKG + s + 1] : 0.0f;
+        return fmaf(c.x, bkg[s], fmaf(c.y, bkg[PTK_MAX_BKG + s], fmaf(c.z, y1, c.w * m1)));
+    }
     const float z = P.zcheb[i];
-    float a = 1.0f, b = z, y = sh.bkg[0];
-    if (P.n_bkg > 1) y = fmaf(sh.bkg[1], z, y);
+    float a = 1.0f, b = z, y = bkg[0];
+    if (P.n_bkg > 1) y = fmaf(bkg[1], z, y);
     for (int k = 2; k < P.n_bkg; ++k) {
         float nx = 2.0f * z * b - a;
         a = b; b = nx;
-        y = fmaf(sh.bkg[k], nx, y);
+        y = fmaf(bkg[k], nx, y);
     }
     return y;
 }
+
+__device__ __forceinline__ float background_at(const DevPlan &P, const Shared &sh, int i) { return bkg_value(P, sh.bkg, i); }
 
 // chi^2 = sum_{w>0} w (y_obs - y_calc)^2 (rietveld_core.js:1885) over points [i0, i1)
 // with stride, Neumaier per lane, two-sum tree over the wavefront and the block.
@@ -928,18 +969,7 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
             if (i < hi_pt) {
                 const float2 ow = P.obs[i];
                 if (ow.y > 0.0f) {
-                    float yb = 0.0f;
-                    if (P.n_bkg > 0) {
-                        const float z = P.zcheb[i];
-                        float ta = 1.0f, tb = z;
-                        yb = S.bkg[0];
-                        if (P.n_bkg > 1) yb = fmaf(S.bkg[1], z, yb);
-                        for (int k = 2; k < P.n_bkg; ++k) {
-                            float nx = 2.0f * z * tb - ta;
-                            ta = tb; tb = nx;
-                            yb = fmaf(S.bkg[k], nx, yb);
-                        }
-                    }
+                    const float yb = bkg_value(P, S.bkg, i);
                     const float yp = h ? (ta[j].two1 + ta[j].one1) + ta[j].gen1 : (ta[j].two0 + ta[j].one0) + ta[j].gen0;
                     const float r = ow.x - (yb + yp);
                     neumaier(s, c, ow.y * r * r);

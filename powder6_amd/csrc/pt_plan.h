@@ -21,7 +21,8 @@
 #define PTK_MAX_REPS 48
 #define PTK_MAX_REPS_TOTAL 64
 #define PTK_MAX_VARS 256
-#define PTK_MAX_BKG_F 24
+#define PTK_MAX_BKG 24       // background coefficients or spline anchors
+#define PTK_MAX_BKG_F 48     // a trial's background slots: values, then spline tangents
 #define PTK_S_COUNT_K 12
 #define PTK_MATRIX_MAX_REFL 4096
 #define PTK_MAX_GROUPS 64          // 512-point groups per pattern (32768 points)
@@ -70,12 +71,15 @@ struct DevPlan {
     double guard_lo, guard_hi; // tan(theta) range of the variance guard
     double max_temp;
     uint32_t seed;
-    int pad1;
+    int bkg_spline;            // 1: background is the spline through the anchors, 0: Chebyshev
     DevPhase phase[PTK_MAX_PHASES];
     // device arrays
     const double *tth;
     const float2 *tff;         // float-float split of tth
     const float *zcheb;        // Chebyshev abscissa of every point, 2 (tth - tlo) / (thi - tlo) - 1
+    const int *bseg;           // spline: anchor interval of every point
+    const float4 *bcoef;       // spline: Hermite weights of every point (y_s, m_s, y_s+1, m_s+1)
+    const double *bknots;      // spline: anchor positions
     const float2 *obs;
     const int4 *hklm;
     const int *eq_off;

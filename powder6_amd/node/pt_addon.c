@@ -139,6 +139,11 @@ static const char *convert(napi_env env, napi_value o, holder *h) {
     p->min_fwhm = num(env, o, "minFwhm", 1e-3);
     p->radiation = (int32_t)num(env, o, "radiation", 0);
     p->n_bkg = (int32_t)num(env, o, "nBkg", 0);
+    p->bkg_type = (int32_t)num(env, o, "bkgType", PT_BKG_CHEBYSHEV);
+    if (p->bkg_type == PT_BKG_SPLINE) {
+        if (field_doubles(env, o, "bkgX", &d, h) != p->n_bkg) return "problem.bkgX must hold nBkg anchors";
+        p->bkg_x = d;
+    }
     napi_value phases;
     if (!get_prop(env, o, "phases", &phases)) return "problem.phases is missing";
     uint32_t nph = 0;

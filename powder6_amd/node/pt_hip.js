@@ -36,12 +36,14 @@ var SLOTS = {
 var CELL = ['a', 'b', 'c', 'alpha', 'beta', 'gamma'];
 var P_PHASE0 = 27, PH_ATOM0 = 24, AT_STRIDE = 5;
 
-/* Parameter index of a reference value name in the C-ABI layout, or -1. */
-function paramIndex(layout, name) {
+/* Parameter index of a reference value name in the C-ABI layout, or -1.  The 24
+ * background slots hold bkg<k> (Chebyshev) or bga<j> (spline anchor values,
+ * js/rietveld_core.js:932-936), as the dataset's background form says. */
+function paramIndex(layout, name, spline) {
     if (name === 'zero') return 0;
     if (name === 'disp') return 1;
     if (name === 'transp') return 2;
-    var m = /^bkg(\d+)$/.exec(name);
+    var m = (spline ? /^bga(\d+)$/ : /^bkg(\d+)$/).exec(name);
     if (m) return +m[1] < 24 ? 3 + (+m[1]) : -1;
     for (var q = 0; q < layout.length; q++) {
         var L = layout[q];
@@ -62,14 +64,15 @@ function paramIndex(layout, name) {
     return -1;
 }
 
-/* Flatten a reference PT context (one dataset, Rietveld phases, Chebyshev background,
+/* Flatten a reference PT context (one dataset, Rietveld phases, Chebyshev or spline background,
  * non-separated TCH / simple / split pseudo-Voigt) into the C-ABI problem. */
 function fromContext(ctx) {
     if (ctx.flat) return ctx.flat;
     var model = ctx.model, problem = ctx.problem;
     if (!model || model.datasets.length !== 1) throw new Error('MI355X PT backend handles one dataset per fit; using the CPU path.');
     var D = model.datasets[0], def = D.def;
-    if (D.spline) throw new Error('MI355X PT backend has no spline background yet; using the CPU path.');
+    var spline = !!D.spline;
+    if (spline && D.bkgX.length > 24) throw new Error('MI355X PT backend takes at most 24 spline anchors; using the CPU path.');
     var values = problem.values[0], layout = [], base = P_PHASE0, phases = [];
     model.phases.forEach(function (P, q) {
         if (!D.has[q]) throw new Error('MI355X PT backend needs every phase in the dataset.');
@@ -99,19 +102,19 @@ function fromContext(ctx) {
     });
     var nParams = base, paramBase = new Float64Array(nParams);
     Object.keys(values).forEach(function (name) {
-        var k = paramIndex(layout, name);
+        var k = paramIndex(layout, name, spline);
         if (k >= 0 && Number.isFinite(values[name])) paramBase[k] = values[name];
     });
     // a monoclinic or triclinic cell reads all six slots; default the unused angles to 90
     layout.forEach(function (L) { for (var c = 3; c < 6; c++) if (!paramBase[L.base + c]) paramBase[L.base + c] = 90; });
     var termParam = [], termVar = [], termCoeff = [];
     problem.vars.forEach(function (vr, j) {
-        var k = paramIndex(layout, vr.name);
+        var k = paramIndex(layout, vr.name, spline);
         if (k < 0) throw new Error('MI355X PT backend cannot refine ' + vr.name + '; using the CPU path.');
         paramBase[k] = 0; termParam.push(k); termVar.push(j); termCoeff.push(1);
     });
     if (ctx.constraints) ctx.constraints.links.forEach(function (link) {
-        var k = paramIndex(layout, link.name);
+        var k = paramIndex(layout, link.name, spline);
         if (k < 0) throw new Error('MI355X PT backend cannot link ' + link.name + '; using the CPU path.');
         paramBase[k] = link.offset;
         link.terms.forEach(function (t) { termParam.push(k); termVar.push(t[0]); termCoeff.push(t[1]); });
@@ -119,7 +122,8 @@ function fromContext(ctx) {
     return {
         tth: Float64Array.from(def.tth), y: Float64Array.from(def.y), w: Float64Array.from(def.w),
         lambda1: def.lambda1, lambda2: def.lambda2 || 0, ratio: def.ratio || 0, polK: def.polK, minFwhm: def.minFwhm,
-        radiation: def.radiation === 'neutron' ? 1 : 0, nBkg: D.T.length, phases: phases, paramBase: paramBase,
+        radiation: def.radiation === 'neutron' ? 1 : 0, nBkg: spline ? D.bkgX.length : D.T.length,
+        bkgType: spline ? 1 : 0, bkgX: Float64Array.from(spline ? D.bkgX : []), phases: phases, paramBase: paramBase,
         x0: Float64Array.from(ctx.x), lo: Float64Array.from(ctx.info.map(function (i) { return i.lo; })),
         hi: Float64Array.from(ctx.info.map(function (i) { return i.hi; })), scale: Float64Array.from(ctx.setup.scales),
         termParam: Int32Array.from(termParam), termVar: Int32Array.from(termVar), termCoeff: Float64Array.from(termCoeff),

@@ -2,7 +2,7 @@
 
 A `Problem` is the flat form of the PT context js/rietveld_core.js:2271-2273 hands to
 the GPU backend: one dataset (2-theta axis, counts, weights, wavelengths, polarisation,
-width floor, Chebyshev background), its Rietveld phases (operators, atoms on Wyckoff
+width floor, Chebyshev or spline background), its Rietveld phases (operators, atoms on Wyckoff
 sites, profile family), the model parameters as an affine function of the refined
 variables, the variables' boxes and PT step scales, and the PT settings.  pt.py turns it
 into the C struct of include/powder6_pt.h.
@@ -10,6 +10,7 @@ into the C struct of include/powder6_pt.h.
 Workloads (named by what they are):
   nacl_fm3m()        NaCl Fm-3m a=5.64, Cu Ka1 only, 10-120 deg step 0.02, TCH, 6-term
                      Chebyshev background, 16 replicas, 12 refined parameters
+  nacl_spline()      the same pattern over a monotone-spline background (8 refined anchors)
   pbso4_pnma()       PbSO4-type Pnma a=8.48 b=5.40 c=6.96, 5 sites (24 atoms), 10-150 deg
                      step 0.02 with the Ka1/Ka2 doublet, 256 replicas x 64 trials/sweep
   uneven_batch()     48 independent cubic / tetragonal / monoclinic patterns
@@ -32,6 +33,7 @@ from .scatter_wk1995 import WK1995
 
 # model-parameter layout of include/powder6_pt.h
 P_ZERO, P_DISP, P_TRANSP, P_BKG0, MAX_BKG, P_PHASE0 = 0, 1, 2, 3, 24, 27
+BKG_CHEBYSHEV, BKG_SPLINE = 0, 1
 PH = {'a': 0, 'b': 1, 'c': 2, 'alpha': 3, 'beta': 4, 'gamma': 5, 'scale': 6, 'po': 7}
 PH_PROF0, PH_ATOM0, AT_STRIDE = 8, 24, 5
 AT = {'x': 0, 'y': 1, 'z': 2, 'occ': 3, 'B': 4}
@@ -80,7 +82,7 @@ class Problem:
     """One dataset, its phases, model parameters p = base + M x, variables, PT settings."""
 
     def __init__(self, tth, phases, lambda1=CU_KA1, lambda2=0.0, ratio=0.0, pol_k=1.0, min_fwhm=None,
-                 n_bkg=6, radiation=0):
+                 n_bkg=6, radiation=0, bkg_x=None):
         self.tth = np.ascontiguousarray(tth, dtype=np.float64)
         self.y = np.zeros_like(self.tth)
         self.w = np.ones_like(self.tth)
@@ -89,6 +91,14 @@ class Problem:
             min_fwhm = max(1e-4, 2 * float(np.median(np.diff(self.tth))))
         self.min_fwhm = min_fwhm
         self.n_bkg, self.radiation = n_bkg, radiation
+        # spline background: anchors at these 2-theta (finite, sorted, as the reference's
+        # dataset build keeps them, js/rietveld_core.js:654), values bga0..bga<n-1>
+        self.bkg_type, self.bkg_x = BKG_CHEBYSHEV, None
+        if bkg_x is not None:
+            xs = np.sort(np.asarray([v for v in map(float, bkg_x) if math.isfinite(v)], dtype=np.float64))
+            if len(xs) > MAX_BKG:
+                raise ValueError('at most %d spline anchors' % MAX_BKG)
+            self.bkg_type, self.bkg_x, self.n_bkg = BKG_SPLINE, xs, len(xs)
         self.phases = phases
         self.names = []
         self.index = {}
@@ -100,7 +110,8 @@ class Problem:
 
     # -- layout ------------------------------------------------------------------
     def _layout(self):
-        names = ['zero', 'disp', 'transp'] + ['bkg%d' % k for k in range(MAX_BKG)]
+        stem = 'bga' if self.bkg_type == BKG_SPLINE else 'bkg'
+        names = ['zero', 'disp', 'transp'] + [stem + '%d' % k for k in range(MAX_BKG)]
         for ph in self.phases:
             pre = ph.key + '.'
             names += [pre + n for n in ('a', 'b', 'c', 'alpha', 'beta', 'gamma', 'scale', 'po')]
@@ -210,6 +221,8 @@ class Problem:
     def background(self, x):
         p = self.values(x)
         t = self.tth
+        if self.bkg_type == BKG_SPLINE:
+            return spline_background(t, self.bkg_x, p[P_BKG0:P_BKG0 + self.n_bkg])
         lo, hi = t[0], t[-1]
         z = 2 * (t - lo) / (hi - lo) - 1 if hi > lo else np.zeros_like(t)
         T = [np.ones_like(t), z]
@@ -233,6 +246,8 @@ def classify(name):
         return name, None
     if name.startswith('bkg'):
         return 'bkg', None
+    if name.startswith('bga'):
+        return 'bga', None
     parts = name.split('.')
     if len(parts) == 2:
         if parts[1] == 'scale':
@@ -286,6 +301,42 @@ def _grid(lo, hi, step):
     return lo + step * np.arange(n, dtype=np.float64)
 
 
+def spline_background(tth, xs, ys):
+    """Monotone cubic Hermite background through the anchors (xs, ys) with Fritsch-Carlson
+    tangents, flat outside the anchors (js/rietveld_core.js:549-612); NaN values read as
+    0 (Model.splineY, :932-936).  Host-side, for synthetic observations."""
+    t = np.asarray(tth, dtype=np.float64)
+    xs = np.asarray(xs, dtype=np.float64)
+    ys = np.nan_to_num(np.asarray(ys, dtype=np.float64), nan=0.0)
+    na = len(xs)
+    if na == 0:
+        return np.zeros_like(t)
+    if na == 1:
+        return np.full_like(t, ys[0])
+    h = np.diff(xs)
+    d = np.diff(ys) / h
+    m = np.empty(na)
+    m[0], m[-1] = d[0], d[-1]
+    for i in range(1, na - 1):
+        m[i] = 0.0 if d[i - 1] * d[i] <= 0 else (h[i] * d[i - 1] + h[i - 1] * d[i]) / (h[i - 1] + h[i])
+    for i in range(na - 1):
+        if d[i] == 0:
+            m[i] = m[i + 1] = 0.0
+            continue
+        a, b = m[i] / d[i], m[i + 1] / d[i]
+        tau = a * a + b * b
+        if tau > 9:
+            f = 3 / math.sqrt(tau)
+            m[i], m[i + 1] = a * d[i] * f, b * d[i] * f
+    s = np.clip(np.searchsorted(xs, t, side='right') - 1, 0, na - 2)
+    u = (t - xs[s]) / h[s]
+    u2, u3 = u * u, u * u * u
+    y = (2 * u3 - 3 * u2 + 1) * ys[s] + (u3 - 2 * u2 + u) * h[s] * m[s] + (-2 * u3 + 3 * u2) * ys[s + 1] \
+        + (u3 - u2) * h[s] * m[s + 1]
+    y = np.where(t <= xs[0], ys[0], y)
+    return np.where(t >= xs[-1], ys[-1], y)
+
+
 def _cheb(problem, coeffs):
     for k, v in enumerate(coeffs):
         problem.set('bkg%d' % k, v)
@@ -318,6 +369,44 @@ def nacl_fm3m(simulate=None, seed=20260101, chains=16, sweeps=1000):
         lo, hi = limits.get(n, (-math.inf, math.inf))
         pb.refine(n, lo, hi)
     pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, sweeps, 0, 100.0, 137
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed)
+        x0 = _perturb(pb, x_true, seed + 1)
+        pb.x0 = list(x0)
+    return pb
+
+
+def nacl_spline(simulate=None, seed=20260111, chains=16, sweeps=1000, n_anchors=8, atoms_only=False):
+    """The NaCl Fm-3m pattern of nacl_fm3m with a spline background instead of the
+    Chebyshev series: `n_anchors` anchors spread over 10-120 deg (a falling air-scatter
+    hump, 260 -> 90 counts, with a bump near 30 deg), every anchor value bga<j> refined
+    with a, zero, scale, U, V, W, X, Y; with atoms_only the two B factors are the only
+    variables (the structure moves of the shared-profile form)."""
+    r, t = S.group_ops(225)
+    cell = [5.64, 5.64, 5.64, 90, 90, 90]
+    atoms = [Atom('Na1', 'Na', S.site_symmetry(r, t, [0, 0, 0], cell), B=1.1),
+             Atom('Cl1', 'Cl', S.site_symmetry(r, t, [0.5, 0.5, 0.5], cell), B=0.9)]
+    ph = Phase('p1', 225, cell, atoms, 'tch_aniso')
+    xs = np.linspace(10.0, 120.0, n_anchors)
+    pb = Problem(_grid(10.0, 120.0, 0.02), [ph], lambda1=1.5406, lambda2=0.0, ratio=0.0, pol_k=1.0, bkg_x=xs)
+    pb.default_values()
+    for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+        pb.set('p1.' + n, v)
+    pb.set('zero', 0.012)
+    ys = 90.0 + 170.0 * np.exp(-(xs - 10.0) / 45.0) + 40.0 * np.exp(-0.5 * ((xs - 30.0) / 8.0) ** 2)
+    for j, v in enumerate(ys):
+        pb.set('bga%d' % j, float(v))
+    pb.set('p1.scale', 1.0)
+    names = ['p1.a', 'zero', 'p1.scale', 'p1.U', 'p1.V', 'p1.W', 'p1.X', 'p1.Y'] + ['bga%d' % j for j in range(n_anchors)]
+    if atoms_only:
+        names = ['p1.Na1.B', 'p1.Cl1.B']
+    limits = {'p1.a': (5.64 * 0.96, 5.64 * 1.04), 'zero': (-2, 2), 'p1.scale': (0, math.inf), 'p1.X': (0, math.inf),
+              'p1.Y': (0, math.inf), 'p1.Na1.B': (-2, 30), 'p1.Cl1.B': (-2, 30)}
+    for n in names:
+        lo, hi = limits.get(n, (-math.inf, math.inf))
+        pb.refine(n, lo, hi)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, sweeps, 0, 100.0, 139
     x_true = pb.start_vector()
     if simulate is not None:
         pb.observe(simulate, x_true, seed)
@@ -498,7 +587,7 @@ def _perturb(pb, x_true, seed, frac=0.3):
     x = np.array(x_true, dtype=np.float64)
     for j, n in enumerate(pb.vars):
         kind, comp = classify(n)
-        if kind == 'bkg' or n.endswith('.scale'):
+        if kind in ('bkg', 'bga') or n.endswith('.scale'):
             continue
         x[j] += rng.uniform(-1, 1) * frac * 4 * pb.scale[j]
         x[j] = min(pb.hi[j], max(pb.lo[j], x[j]))

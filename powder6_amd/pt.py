@@ -50,7 +50,7 @@ class PtProblem(ctypes.Structure):
                 ('n_vars', c_int32), ('x0', P_D), ('lo', P_D), ('hi', P_D), ('scale', P_D),
                 ('n_terms', c_int32), ('term_param', P_I), ('term_var', P_I), ('term_coeff', P_D),
                 ('chains', c_int32), ('sweeps', c_int32), ('steps_per_sweep', c_int32),
-                ('max_temp', c_double), ('seed', c_uint32)]
+                ('max_temp', c_double), ('seed', c_uint32), ('bkg_type', c_int32), ('bkg_x', P_D)]
 
 
 class PtPlanInfo(ctypes.Structure):
@@ -129,6 +129,9 @@ class CProblem:
         s.term_coeff = d([t[2] for t in pb.terms] or [0.0])
         s.chains, s.sweeps, s.steps_per_sweep = pb.chains, pb.sweeps, pb.steps_per_sweep
         s.max_temp, s.seed = pb.max_temp, pb.seed
+        s.bkg_type = getattr(pb, 'bkg_type', 0)
+        bx = getattr(pb, 'bkg_x', None)
+        s.bkg_x = d(np.asarray(bx if bx is not None else [0.0], dtype=np.float64))
 
     def ref(self):
         return ctypes.byref(self.struct)

@@ -6,6 +6,8 @@ import math
 
 import numpy as np
 
+from powder6_amd import problems as PR
+
 
 def _num(v):
     v = float(v)
@@ -36,7 +38,11 @@ def reference_ctx(pb, initial_cost=None):
     ds = {'def': {'tth': pb.tth.tolist(), 'y': pb.y.tolist(), 'w': pb.w.tolist(), 'lambda1': pb.lambda1,
                   'lambda2': pb.lambda2, 'ratio': pb.ratio, 'polK': pb.pol_k, 'minFwhm': pb.min_fwhm,
                   'radiation': 'neutron' if pb.radiation else 'xray'},
-          'spline': False, 'T': [0] * pb.n_bkg, 'has': [True] * len(pb.phases), 'anom': anom}
+          'has': [True] * len(pb.phases), 'anom': anom}
+    if getattr(pb, 'bkg_type', PR.BKG_CHEBYSHEV) == PR.BKG_SPLINE:
+        ds.update(spline=True, T=[], bkgX=[float(v) for v in pb.bkg_x])
+    else:
+        ds.update(spline=False, T=[0] * pb.n_bkg, bkgX=[])
     return {
         'problem': {'values': [values], 'vars': [{'name': n} for n in pb.vars]},
         'model': {'datasets': [ds], 'phases': phases},

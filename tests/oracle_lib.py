@@ -57,6 +57,8 @@ def lib():
         L.om_loss_grad.argtypes = [P_D, ctypes.c_int, ctypes.c_int, P_D, P_D, ctypes.c_void_p, ctypes.c_int, P_D]
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.c_void_p]
+        L.or_spline_slopes.argtypes = [P_D, P_D, ctypes.c_int, P_D, P_D]
+        L.or_spline_values.argtypes = [P_D, ctypes.c_int, P_D, P_D, ctypes.c_int, P_D]
         L.or_destroy.argtypes = [ctypes.c_void_p]
         L.or_destroy.restype = None
         L.or_cost.restype = ctypes.c_double
@@ -205,3 +207,14 @@ def mlp_loss_grad(params, n_in, h, x, t, mask=None):
     loss = lib().om_loss_grad(p.ctypes.data_as(P_D), n_in, h, x.ctypes.data_as(P_D), t.ctypes.data_as(P_D),
                               None if m is None else m.ctypes.data_as(P_D), x.shape[0], g.ctypes.data_as(P_D))
     return loss, g
+
+
+def spline_values(tth, xs, ys):
+    """The oracle's spline background at the points tth (anchors xs, values ys)."""
+    tth = np.ascontiguousarray(tth, dtype=np.float64)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    out = np.zeros(len(tth))
+    pd = lambda a: a.ctypes.data_as(P_D)
+    lib().or_spline_values(pd(tth), len(tth), pd(xs), pd(ys), len(xs), pd(out))
+    return out

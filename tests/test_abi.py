@@ -37,7 +37,7 @@ def test_library_exports(name):
 
 
 def test_abi_version():
-    assert pt.lib().pt_abi_version() == 1
+    assert pt.lib().pt_abi_version() == 2
 
 
 def test_phase_base_layout():

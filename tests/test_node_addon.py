@@ -44,6 +44,21 @@ def test_compile_from_reference_context(profile, tmp_path):
     assert c['paramCount'] == info['n_params']
 
 
+@needs_node
+def test_compile_spline_background_context(tmp_path):
+    """A dataset with a spline background (js/rietveld_core.js:639-673 bkgX) compiles:
+    the anchors become the plan's background terms."""
+    pb = PR.nacl_spline()
+    pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'compile')
+    assert 'compile' in out, r.stderr
+    with pt.Plan(pb, device=False) as pl:
+        info = pl.info
+    c = out['compile']
+    assert c['nv'] == info['nv'] == 16 and c['pointCount'] == info['point_count'] and c['paramCount'] == info['n_params']
+
+
 def _has_gpu():
     try:
         import torch
@@ -72,6 +87,20 @@ def _probes(pb):
 @pytest.mark.parametrize('profile', ['tch_aniso', 'split_pvoigt'])
 def test_node_costs_match_oracle(profile, tmp_path):
     pb = PR.triclinic_probe(profile, simulate=O.oracle_simulate)
+    m = O.Model(pb)
+    probes = _probes(pb)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])
+    out, r = _node(path, 'costs')
+    assert 'costs' in out, (out, r.stderr)
+    for c, g in enumerate(out['costs']):
+        cpu = m.cost(probes[c % len(probes)])
+        assert abs(g - cpu) / max(1.0, cpu) <= 1e-5
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_costs_spline_background_match_oracle(tmp_path):
+    pb = PR.nacl_spline(simulate=O.oracle_simulate)
     m = O.Model(pb)
     probes = _probes(pb)
     path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])

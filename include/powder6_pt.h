@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 2
+#define PT_ABI_VERSION 3
 
 /* ---- status codes -------------------------------------------------------------- */
 #define PT_OK 0
@@ -38,8 +38,14 @@ extern "C" {
 /* crystal systems, js/crystal.js:76-119 (rhombohedral uses hexagonal axes) */
 enum { PT_TRICLINIC = 0, PT_MONOCLINIC = 1, PT_ORTHORHOMBIC = 2, PT_TETRAGONAL = 3,
        PT_TRIGONAL = 4, PT_HEXAGONAL = 5, PT_CUBIC = 6 };
-/* profile families, js/profile.js:603-718 (type 1 simple, 2 split, 3 TCH) */
-enum { PT_PROFILE_SIMPLE = 1, PT_PROFILE_SPLIT = 2, PT_PROFILE_TCH = 3 };
+/* profile families, js/profile.js:603-718 (type 1 simple, 2 split, 3 TCH); 4 is the
+ * reference's 'gsas_cw', a separated profile with a TCH instrument function
+ * (js/rietveld_core.js:443-460) */
+enum { PT_PROFILE_SIMPLE = 1, PT_PROFILE_SPLIT = 2, PT_PROFILE_TCH = 3, PT_PROFILE_GSAS_CW = 4 };
+/* sample broadening models of a separated profile, js/rietveld_core.js:416-421
+ * broadeningConfig and js/profile.js:384-422 gsasCwComponents */
+enum { PT_SIZE_ISOTROPIC = 0, PT_SIZE_UNIAXIAL = 1, PT_SIZE_ELLIPSOID = 2 };
+enum { PT_STRAIN_ISOTROPIC = 0, PT_STRAIN_UNIAXIAL = 1, PT_STRAIN_STEPHENS = 2 };
 enum { PT_XRAY = 0, PT_NEUTRON = 1 };
 /* background forms, js/rietveld_core.js:622-627 bkgType ('cheb' | 'spline') */
 enum { PT_BKG_CHEBYSHEV = 0, PT_BKG_SPLINE = 1 };
@@ -51,14 +57,26 @@ enum { PT_BKG_CHEBYSHEV = 0, PT_BKG_SPLINE = 1 };
  * PT_PH_ATOM0 slots followed by PT_AT_STRIDE slots per atom.  pt_phase_base() gives
  * the first index of a phase block.                                                  */
 enum { PT_P_ZERO = 0, PT_P_DISP = 1, PT_P_TRANSP = 2, PT_P_BKG0 = 3, PT_MAX_BKG = 24,
-       PT_P_PHASE0 = 3 + 24 };
+       PT_P_INST0 = 3 + 24, PT_N_INST = 18, PT_P_PHASE0 = 3 + 24 + 18 };
+/* instrument slots (relative to PT_P_INST0): the dataset's `inst.<name>` values that a
+ * separated profile reads (js/rietveld_core.js:413, 454-459 ALL_INSTRUMENT_PROFILE);
+ * a value that is not finite reads as the listed default (:700-701) */
+enum { PT_INST_U = 0, PT_INST_V, PT_INST_W, PT_INST_X, PT_INST_Y, PT_INST_Z, PT_INST_SL, PT_INST_HL,
+       PT_INST_ETA, PT_INST_U_L, PT_INST_V_L, PT_INST_W_L, PT_INST_X_L, PT_INST_U_R, PT_INST_V_R,
+       PT_INST_W_R, PT_INST_X_R, PT_INST_ETA_SPLIT };
 enum { PT_PH_A = 0, PT_PH_B = 1, PT_PH_C = 2, PT_PH_ALPHA = 3, PT_PH_BETA = 4, PT_PH_GAMMA = 5,
-       PT_PH_SCALE = 6, PT_PH_PO = 7, PT_PH_PROF0 = 8, PT_PH_ATOM0 = 24 };
+       PT_PH_SCALE = 6, PT_PH_PO = 7, PT_PH_PROF0 = 8, PT_PH_NPROF = 32, PT_PH_ATOM0 = 8 + 32 };
 /* profile slots (relative to PT_PH_PROF0) -- js/rietveld_core.js:446-451 names */
 enum { PT_TCH_U = 0, PT_TCH_V, PT_TCH_W, PT_TCH_X, PT_TCH_Y, PT_TCH_SL, PT_TCH_HL };
 enum { PT_SIM_GU = 0, PT_SIM_GV, PT_SIM_GW, PT_SIM_GP, PT_SIM_LX, PT_SIM_ETA };
 enum { PT_SPL_GU_L = 0, PT_SPL_GV_L, PT_SPL_GW_L, PT_SPL_LX_L,
        PT_SPL_GU_R, PT_SPL_GV_R, PT_SPL_GW_R, PT_SPL_LX_R, PT_SPL_ETA };
+/* sample slots of a separated profile, js/rietveld_core.js:414-415 SAMPLE_PROFILE order:
+ * size, sizeAxial, sizeB, sizeC, rotA..rotC, strain, strainAxial, sizeMix, strainMix,
+ * then the 15 Stephens coefficients S400 S040 S004 S220 S202 S022 S310 S301 S130 S103
+ * S031 S013 S211 S121 S112 (js/profile.js:332 GSAS_QUARTIC) */
+enum { PT_GS_SIZE = 0, PT_GS_SIZE_AXIAL, PT_GS_SIZE_B, PT_GS_SIZE_C, PT_GS_ROT_A, PT_GS_ROT_B, PT_GS_ROT_C,
+       PT_GS_STRAIN, PT_GS_STRAIN_AXIAL, PT_GS_SIZE_MIX, PT_GS_STRAIN_MIX, PT_GS_S0, PT_GS_COUNT = 26 };
 /* atom slots: free-coordinate values, occupancy, isotropic B */
 enum { PT_AT_X = 0, PT_AT_Y = 1, PT_AT_Z = 2, PT_AT_OCC = 3, PT_AT_B = 4, PT_AT_STRIDE = 5 };
 
@@ -92,6 +110,15 @@ typedef struct {
     double po_dir[3];      /* March-Dollase direction, phase.poDir */
     int32_t n_atoms;
     const pt_atom *atoms;
+    /* separated (GSAS) profile: instrument function from the dataset's inst.* slots,
+     * sample broadening from the phase's PT_GS_* slots (js/rietveld_core.js:460
+     * usesSeparatedProfile: profile_type PT_PROFILE_GSAS_CW, or separated = 1 with any
+     * family).  size_axis / strain_axis: the uniaxial axes (Miller indices; an axis that
+     * is not three finite numbers with one non-zero reads as 0 0 1, :416-421). */
+    int32_t separated;
+    int32_t size_model;    /* PT_SIZE_* */
+    int32_t strain_model;  /* PT_STRAIN_* */
+    double size_axis[3], strain_axis[3];
 } pt_phase;
 
 /* The whole PT problem of one dataset.  Model parameters are an affine function of
@@ -128,6 +155,10 @@ typedef struct {
      * PT_BKG_CHEBYSHEV. */
     int32_t bkg_type;
     const double *bkg_x;
+    /* instrument function family of every separated phase (the dataset's
+     * resolutionType, js/rietveld_core.js:700): PT_PROFILE_*, or 0 for each phase's own
+     * profile_type.  PT_PROFILE_GSAS_CW means TCH. */
+    int32_t resolution_type;
 } pt_problem;
 
 /* Plan summary: the numeric fields of pt_gpu.js compile() (js/pt_gpu.js:226).        */

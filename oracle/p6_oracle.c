@@ -77,6 +77,25 @@ double or_hypot2(double x, double y) {
     return sqrt(sum) * mx;
 }
 
+/* Math.hypot with n arguments, the same compensated sum */
+static double hypot_n(const double *v, int n) {
+    double mx = 0;
+    for (int i = 0; i < n; i++) {
+        double a = fabs(v[i]);
+        if (isinf(a)) return INFINITY;
+        if (isnan(a)) return NAN;
+        if (a > mx) mx = a;
+    }
+    if (mx == 0) return 0;
+    double sum = 0, comp = 0;
+    for (int i = 0; i < n; i++) {
+        double q = fabs(v[i]) / mx, summand = q * q - comp, prelim = sum + summand;
+        comp = (prelim - sum) - summand;
+        sum = prelim;
+    }
+    return sqrt(sum) * mx;
+}
+
 /* constants.js:138-143 softPositive */
 double or_soft_positive(double x, double f) {
     consts();
@@ -403,16 +422,200 @@ static double lorentzian_from_gamma(double g, double minF) {
 }
 static double orz(double v) { return (v != 0 && !isnan(v)) ? v : 0; }  /* `x || 0` */
 
+/* ---- separated (GSAS) profile ------------------------------------------------ */
+/* profile.js:332 GSAS_QUARTIC exponents (h, k, l) */
+static const int QUARTIC[15][3] = {{4,0,0},{0,4,0},{0,0,4},{2,2,0},{2,0,2},{0,2,2},{3,1,0},{3,0,1},
+                                   {1,3,0},{1,0,3},{0,3,1},{0,1,3},{2,1,1},{1,2,1},{1,1,2}};
+/* rietveld_core.js:413, 454-459: instrument defaults, in PT_INST_* order */
+static const double INST_DEFAULT[PT_N_INST] = {2, -2, 5, 1, 0, 0, 0, 0, 0.5, 2, -2, 5, 1, 2, -2, 5, 1, 0.5};
+
+/* profile.js:334-363 gsasStephensBasis: the quartic forms in h invariant under the
+ * setting's rotations.  For every GSAS_QUARTIC key j (weight 1, 2, 3 or 4 as its
+ * exponents hold a 4, a 3, a 0 or none), the weighted monomial of h' = R^T h averaged
+ * over the distinct rotations; kept when independent of the terms before it (Gram-Schmidt,
+ * norm >= 1e-8), scaled so that its leading coefficient is the key's weight.  Writes the
+ * key index and the 15 coefficients of every term; returns the term count. */
+int or_stephens_basis(int n_ops, const int32_t *op_r, int32_t *keys, double *coeff) {
+    int nrot = 0;
+    int32_t *rots = (int32_t *)malloc(sizeof(int32_t) * 9 * (n_ops > 0 ? n_ops : 1));
+    for (int o = 0; o < n_ops; o++) {
+        int dup = 0;
+        for (int r = 0; r < nrot && !dup; r++) dup = !memcmp(rots + 9 * r, op_r + 9 * o, sizeof(int32_t) * 9);
+        if (!dup) memcpy(rots + 9 * nrot++, op_r + 9 * o, sizeof(int32_t) * 9);
+    }
+    if (!nrot) { free(rots); return 0; }
+    double orth[15][15];
+    int north = 0, nb = 0;
+    for (int j = 0; j < 15; j++) {
+        const int *ex = QUARTIC[j];
+        int has4 = ex[0] == 4 || ex[1] == 4 || ex[2] == 4, has3 = ex[0] == 3 || ex[1] == 3 || ex[2] == 3;
+        int has0 = ex[0] == 0 || ex[1] == 0 || ex[2] == 0;
+        double wj = has4 ? 1 : has3 ? 2 : has0 ? 3 : 4;
+        double c[15] = {0};
+        for (int r = 0; r < nrot; r++) {
+            const int32_t *R = rots + 9 * r;
+            /* polynomial in h as a dense 5x5x5 table of exponents, built factor by factor:
+             * h'_axis = sum_k R[3k + axis] h_k */
+            double poly[5][5][5], next[5][5][5];
+            memset(poly, 0, sizeof poly);
+            poly[0][0][0] = wj;
+            for (int axis = 0; axis < 3; axis++)
+                for (int rep = 0; rep < ex[axis]; rep++) {
+                    memset(next, 0, sizeof next);
+                    for (int a = 0; a < 5; a++) for (int b = 0; b < 5; b++) for (int d = 0; d < 5; d++) {
+                        double v = poly[a][b][d];
+                        if (v == 0) continue;
+                        for (int k = 0; k < 3; k++) {
+                            int cc = R[3 * k + axis];
+                            if (!cc) continue;
+                            int e0 = a + (k == 0), e1 = b + (k == 1), e2 = d + (k == 2);
+                            next[e0][e1][e2] += v * cc;
+                        }
+                    }
+                    memcpy(poly, next, sizeof poly);
+                }
+            for (int i = 0; i < 15; i++) c[i] += poly[QUARTIC[i][0]][QUARTIC[i][1]][QUARTIC[i][2]] / nrot;
+        }
+        double v[15];
+        memcpy(v, c, sizeof v);
+        for (int q = 0; q < north; q++) {
+            double dot = 0;
+            for (int i = 0; i < 15; i++) dot += v[i] * orth[q][i];
+            for (int i = 0; i < 15; i++) v[i] -= dot * orth[q][i];
+        }
+        double norm = hypot_n(v, 15);
+        if (norm < 1e-8) continue;
+        for (int i = 0; i < 15; i++) orth[north][i] = v[i] / norm;
+        north++;
+        int lead = -1;
+        for (int i = 0; i < 15 && lead < 0; i++) if (fabs(c[i]) > 1e-8) lead = i;
+        const int *el = QUARTIC[lead];
+        int l4 = el[0] == 4 || el[1] == 4 || el[2] == 4, l3 = el[0] == 3 || el[1] == 3 || el[2] == 3;
+        int l0 = el[0] == 0 || el[1] == 0 || el[2] == 0;
+        double factor = (l4 ? 1 : l3 ? 2 : l0 ? 3 : 4) / c[lead];
+        keys[nb] = j;
+        for (int i = 0; i < 15; i++) coeff[15 * nb + i] = fabs(c[i]) < 1e-9 ? 0 : c[i] * factor;
+        nb++;
+    }
+    free(rots);
+    return nb;
+}
+
+/* crystal.js:263-275 cellBasis and profile.js:364-369 gsasReciprocalVector */
+static void reciprocal_vector(const double cell[6], const double h[3], double out[3]) {
+    double a = cell[0], b = cell[1], c = cell[2];
+    double al = cell[3] * D2R, be = cell[4] * D2R, ga = cell[5] * D2R;
+    double ca = cos(al), cb = cos(be), cg = cos(ga), sg = sin(ga);
+    double vol = or_cell_volume(cell) / (a * b * c);
+    double av[3] = {a, 0, 0}, bv[3] = {b * cg, b * sg, 0}, cv[3] = {c * cb, c * (ca - cb * cg) / sg, c * vol / sg};
+    double bc[3] = {bv[1] * cv[2] - bv[2] * cv[1], bv[2] * cv[0] - bv[0] * cv[2], bv[0] * cv[1] - bv[1] * cv[0]};
+    double cav[3] = {cv[1] * av[2] - cv[2] * av[1], cv[2] * av[0] - cv[0] * av[2], cv[0] * av[1] - cv[1] * av[0]};
+    double ab[3] = {av[1] * bv[2] - av[2] * bv[1], av[2] * bv[0] - av[0] * bv[2], av[0] * bv[1] - av[1] * bv[0]};
+    double v = av[0] * bc[0] + av[1] * bc[1] + av[2] * bc[2];
+    for (int i = 0; i < 3; i++) out[i] = (h[0] * bc[i] + h[1] * cav[i] + h[2] * ab[i]) / v;
+}
+
+static double sval(const double *sample, int k, double d) { return js_finite(sample[k]) ? sample[k] : d; }
+
+/* profile.js:390 angle(): squared cosine between the unit reciprocal direction n of a
+ * reflection and the reciprocal vector of `axis`, clamped to [0, 1] */
+double or_axis_angle(const double cell[6], const double n[3], const double axis[3]) {
+    double a[3];
+    reciprocal_vector(cell, axis, a);
+    double na = hypot_n(a, 3), dot = 0;
+    if (na == 0) na = 1;
+    for (int i = 0; i < 3; i++) dot = dot + n[i] * a[i];
+    return dmin(1, dmax(0, pow(dot / na, 2)));
+}
+
+/* profile.js:375-378 gsasStephensVariance with a basis: sum over the terms of
+ * S_term (coefficients . monomials of h); S holds the 15 coefficients in GSAS_QUARTIC
+ * order (a value that is not finite reads as 0) */
+double or_stephens_sum(const double h[3], const double *S, int nb, const int32_t *keys, const double *coeff) {
+    double mono[15], total = 0;
+    for (int i = 0; i < 15; i++) mono[i] = pow(h[0], QUARTIC[i][0]) * pow(h[1], QUARTIC[i][1]) * pow(h[2], QUARTIC[i][2]);
+    for (int t = 0; t < nb; t++) {
+        double inner = 0;
+        for (int i = 0; i < 15; i++) inner = inner + coeff[15 * t + i] * mono[i];
+        double v = js_finite(S[keys[t]]) ? S[keys[t]] : 0;
+        total = total + v * inner;
+    }
+    return total;
+}
+
+/* profile.js:384-422 gsasCwComponents, returned as (Gaussian, Lorentzian) FWHM before the
+ * floor.  flank: -1 the family's own U..Z; 0 / 1 the split flank (Y = Z = 0). */
+static void gsas_components(double tth, const or_sep *sp, double lambda, int flank, double minF, double *gG, double *gL) {
+    double theta = tth * JS_PI / 360, tn = tan(theta), cs = dmax(1e-9, cos(theta));
+    const double *S = sp->sample;
+    int directional = sp->size_model == PT_SIZE_UNIAXIAL || sp->size_model == PT_SIZE_ELLIPSOID ||
+                      sp->strain_model == PT_STRAIN_UNIAXIAL;
+    double n[3] = {0, 0, 0};
+    if (directional) {
+        double q[3];
+        reciprocal_vector(sp->cell, sp->h, q);
+        double norm = hypot_n(q, 3), dv = norm != 0 ? norm : 1;
+        for (int i = 0; i < 3; i++) n[i] = q[i] / dv;
+    }
+    double invSize = 1 / dmax(0.001, sval(S, PT_GS_SIZE, 1)), strain = dmax(0, sval(S, PT_GS_STRAIN, 0));
+    if (sp->size_model == PT_SIZE_UNIAXIAL) {
+        double c2 = or_axis_angle(sp->cell, n, sp->size_axis);
+        double i = dmax(0.001, sval(S, PT_GS_SIZE, 1)), a = dmax(0.001, sval(S, PT_GS_SIZE_AXIAL, 1));
+        invSize = sqrt((1 - c2) / (i * i) + c2 / (a * a));
+    } else if (sp->size_model == PT_SIZE_ELLIPSOID) {
+        double v[3] = {n[0], n[1], n[2]};
+        const int axes[3] = {2, 1, 0}, keys[3] = {PT_GS_ROT_C, PT_GS_ROT_B, PT_GS_ROT_A};
+        for (int r = 0; r < 3; r++) {
+            double t = -sval(S, keys[r], 0) * JS_PI / 180, c = cos(t), sn = sin(t);
+            int i = (axes[r] + 1) % 3, j = (axes[r] + 2) % 3;
+            double x = v[i], y = v[j];
+            v[i] = c * x - sn * y; v[j] = sn * x + c * y;
+        }
+        double L[3] = {sval(S, PT_GS_SIZE, 1), sval(S, PT_GS_SIZE_B, 1), sval(S, PT_GS_SIZE_C, 1)}, acc = 0;
+        for (int i = 0; i < 3; i++) acc = acc + v[i] * v[i] / pow(dmax(0.001, L[i]), 2);
+        invSize = sqrt(acc);
+    }
+    if (sp->strain_model == PT_STRAIN_UNIAXIAL) {
+        double c2 = or_axis_angle(sp->cell, n, sp->strain_axis);
+        double i = dmax(0, sval(S, PT_GS_STRAIN, 0)), a = dmax(0, sval(S, PT_GS_STRAIN_AXIAL, 0));
+        strain = c2 <= 1e-14 ? i : c2 >= 1 - 1e-14 ? a : i * a / dmax(1e-30, sqrt(i * i * c2 + a * a * (1 - c2)));
+    } else if (sp->strain_model == PT_STRAIN_STEPHENS) {
+        double sum = or_stephens_sum(sp->h, S + PT_GS_S0, sp->n_basis, sp->basis_key, sp->basis_coeff);
+        strain = (sp->d * sp->d) * sqrt(dmax(0, sum));
+    }
+    double sizeGamma = 1.8 * (js_finite(lambda) ? lambda : 1.540598) * invSize / (JS_PI * cs);
+    double strainGamma = 0.018 * strain * tn / JS_PI;
+    double sm = dmax(0, dmin(1, sval(S, PT_GS_SIZE_MIX, 1))), em = dmax(0, dmin(1, sval(S, PT_GS_STRAIN_MIX, 1)));
+    const double *I = sp->inst;
+    double U, V, W, X, Y, Z;
+    if (flank < 0) { U = I[PT_INST_U]; V = I[PT_INST_V]; W = I[PT_INST_W]; X = I[PT_INST_X]; Y = I[PT_INST_Y]; Z = I[PT_INST_Z]; }
+    else {
+        int o = flank == 0 ? 0 : PT_INST_U_R - PT_INST_U_L;
+        U = I[PT_INST_U_L + o]; V = I[PT_INST_V_L + o]; W = I[PT_INST_W_L + o]; X = I[PT_INST_X_L + o]; Y = 0; Z = 0;
+    }
+    double instrumentSigma = U * tn * tn + V * tn + W;
+    double instrumentGamma = X / cs + Y * tn + Z;
+    double sampleSigma = (pow(sizeGamma * (1 - sm), 2) + pow(strainGamma * (1 - em), 2)) / (8 * log(2));
+    double sampleGamma = sizeGamma * sm + strainGamma * em;
+    double fwhmMin = minF > 0 ? minF : MIN_FWHM_DEFAULT;
+    *gG = gaussian_from_sigma_sq(instrumentSigma + sampleSigma, fwhmMin);
+    *gL = lorentzian_from_gamma(instrumentGamma + sampleGamma, fwhmMin);
+}
+
 /* profile.js:501-537 calculateProfileWidths (non-separated TCH / simple / split);
- * side: 0 left, 1 right/centre */
-static void profile_widths(int type, const double *p, double minF, double tth, int side, double *gG, double *gL) {
+ * side: 0 left, 1 right/centre.  sep != NULL: the separated profile (profile.js:235, 271,
+ * 427), the family's instrument function plus the sample broadening. */
+static void profile_widths(int type, const double *p, const or_sep *sep, double lambda, double minF, double tth, int side,
+                           double *gG, double *gL) {
     double fwhmMin = minF > 0 ? minF : MIN_FWHM_DEFAULT;
     double th = tth * (JS_PI / 180) / 2, MAXA = JS_PI / 2.0 - 1e-6, safe = dmin(th, MAXA);
     if (isnan(th)) safe = NAN;
     if (safe < 1e-6) { *gG = fwhmMin; *gL = fwhmMin; return; }
     double tn = tan(safe), cs = cos(safe), cs_safe = dmax(cs, 1e-9), cs2_safe = dmax(cs * cs, 1e-9);
     double g, l;
-    if (type == PT_PROFILE_SIMPLE) {            /* profile.js:234-254 */
+    if (sep) {
+        gsas_components(tth, sep, lambda, type == PT_PROFILE_SPLIT ? side : -1, minF, &g, &l);
+    } else if (type == PT_PROFILE_SIMPLE) {     /* profile.js:234-254 */
         double GU = orz(p[PT_SIM_GU]), GV = orz(p[PT_SIM_GV]), GW = orz(p[PT_SIM_GW]), GP = orz(p[PT_SIM_GP]), LX = orz(p[PT_SIM_LX]);
         double s2 = GU * tn * tn + GV * tn + GW + GP / cs2_safe;
         g = gaussian_from_sigma_sq(s2, fwhmMin);
@@ -493,7 +696,8 @@ double or_asym_area(const or_voigt *prep) {
 }
 
 /* profile.js:603-718 prepareVoigt (+ preparedProfileArea, profile.js:854-858) */
-int or_prepare_voigt(int type, const double *p, double minF, double tth_peak, double x0, or_voigt *o) {
+int or_prepare_voigt_sep(int type, const double *p, const or_sep *sep, double lambda, double minF, double tth_peak,
+                         double x0, or_voigt *o) {
     consts();
     memset(o, 0, sizeof *o);
     o->type = type; o->x0 = x0;
@@ -508,7 +712,7 @@ int or_prepare_voigt(int type, const double *p, double minF, double tth_peak, do
     double fwhm_total, eta;
     if (type == PT_PROFILE_SIMPLE) {
         double gG, gL;
-        profile_widths(type, p, minF, tth_peak, 1, &gG, &gL);
+        profile_widths(type, p, sep, lambda, minF, tth_peak, 1, &gG, &gL);
         double HG = dmax(1e-9, gG), HL = dmax(1e-9, gL);
         double e = p[PT_SIM_ETA];
         eta = dmax(0, dmin(1, js_finite(e) ? e : 0.5));   /* readEta */
@@ -520,8 +724,8 @@ int or_prepare_voigt(int type, const double *p, double minF, double tth_peak, do
         o->fwhm = fwhm_total;
     } else if (type == PT_PROFILE_SPLIT) {
         double gGL, gLL, gGR, gLR;
-        profile_widths(type, p, minF, tth_peak, 0, &gGL, &gLL);
-        profile_widths(type, p, minF, tth_peak, 1, &gGR, &gLR);
+        profile_widths(type, p, sep, lambda, minF, tth_peak, 0, &gGL, &gLL);
+        profile_widths(type, p, sep, lambda, minF, tth_peak, 1, &gGR, &gLR);
         double HGL = dmax(1e-9, gGL), HLL = dmax(1e-9, gLL), HGR = dmax(1e-9, gGR), HLR = dmax(1e-9, gLR);
         double e = p[PT_SPL_ETA];
         eta = dmax(0, dmin(1, js_finite(e) ? e : 0.5));
@@ -536,7 +740,7 @@ int or_prepare_voigt(int type, const double *p, double minF, double tth_peak, do
         o->fwhm = fwhm_total;
     } else {
         double gG, gL;
-        profile_widths(PT_PROFILE_TCH, p, minF, tth_peak, 1, &gG, &gL);
+        profile_widths(PT_PROFILE_TCH, p, sep, lambda, minF, tth_peak, 1, &gG, &gL);
         double HG = dmax(1e-9, gG), HL = dmax(1e-9, gL);
         double fwhm = dmax(1e-9, or_tch_fwhm(HG, HL));
         eta = 0;
@@ -571,7 +775,21 @@ typedef struct {
     int nrep, ncent;
     int32_t *rep;                    /* indices into the phase's ops */
     refl_list refl;                  /* current reflection list */
+    int sep, fam;                    /* separated profile; line family (PT_PROFILE_SIMPLE..TCH) */
+    double size_axis[3], strain_axis[3];
+    int nb;                          /* Stephens basis (separated, PT_STRAIN_STEPHENS) */
+    int32_t bkey[15];
+    double bcoef[15 * 15];
 } or_phase;
+
+/* rietveld_core.js:416-421 broadeningConfig axis(): three finite numbers, one non-zero,
+ * else 0 0 1 */
+static void broadening_axis(const double *a, double out[3]) {
+    int ok = 1, nz = 0;
+    for (int i = 0; i < 3; i++) { if (!js_finite(a[i])) ok = 0; if (a[i] != 0) nz = 1; }
+    if (ok && nz) { out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; }
+    else { out[0] = 0; out[1] = 0; out[2] = 1; }
+}
 
 struct or_model {
     pt_problem P;                    /* shallow copy; arrays owned below */
@@ -623,6 +841,14 @@ or_model *or_create(const pt_problem *prob) {
         m->ph[p].rep = (int32_t *)malloc(sizeof(int32_t) * (s->n_ops + 1));
         m->ph[p].nrep = or_cosets(s->n_ops, s->op_r, s->op_t, m->ph[p].rep, &m->ph[p].ncent);
         if (!s->n_ops) { set_err("phase without operators"); }
+        /* rietveld_core.js:460 usesSeparatedProfile; :700-702 the instrument family */
+        or_phase *op = &m->ph[p];
+        op->sep = s->separated || s->profile_type == PT_PROFILE_GSAS_CW;
+        op->fam = op->sep && prob->resolution_type ? prob->resolution_type : s->profile_type;
+        if (op->fam == PT_PROFILE_GSAS_CW) op->fam = PT_PROFILE_TCH;
+        broadening_axis(s->size_axis, op->size_axis);
+        broadening_axis(s->strain_axis, op->strain_axis);
+        op->nb = op->sep && s->strain_model == PT_STRAIN_STEPHENS ? or_stephens_basis(s->n_ops, s->op_r, op->bkey, op->bcoef) : 0;
     }
     m->P.phases = m->phases;
     m->base = dup_d(prob->param_base, prob->n_params);
@@ -815,6 +1041,15 @@ static int lower_bound(const double *a, int n, double target) {
     return lo;
 }
 
+int or_prepare_voigt(int type, const double *p, double minF, double tth_peak, double x0, or_voigt *o) {
+    return or_prepare_voigt_sep(type, p, NULL, 0, minF, tth_peak, x0, o);
+}
+
+void or_gsas_widths(int type, const or_sep *sep, double lambda, double minF, double tth, int side, double *gG, double *gL) {
+    consts();
+    profile_widths(type, NULL, sep, lambda, minF, tth, side, gG, gL);
+}
+
 /* one phase: Model.geometry (:719-736), buildPeakContributions (profile.js:1013-1080),
  * evalPhase (:855-897), render (:908-917).  Adds the phase pattern into y. */
 static int eval_phase(or_model *m, int q, const double *p, double *y, refl_work **rw_out, int *nr_out) {
@@ -836,6 +1071,29 @@ static int eval_phase(or_model *m, int q, const double *p, double *y, refl_work 
     }
     /* contributions: all Kalpha1, then all Kalpha2 */
     const double *prof = pp + PT_PH_PROF0;
+    /* separated profile (Model.profileParams, rietveld_core.js:694-708): instrument values
+     * with their defaults, eta / SL / HL of the family from them, sample slots as given */
+    const or_phase *oph = &m->ph[q];
+    int fam = oph->fam;
+    or_sep sp;
+    double sslots[PT_PH_NPROF];
+    if (oph->sep) {
+        memset(&sp, 0, sizeof sp);
+        for (int i = 0; i < PT_N_INST; i++) {
+            double v = p[PT_P_INST0 + i];
+            sp.inst[i] = js_finite(v) ? v : INST_DEFAULT[i];
+        }
+        sp.sample = prof;
+        sp.size_model = ph->size_model; sp.strain_model = ph->strain_model;
+        memcpy(sp.size_axis, oph->size_axis, sizeof sp.size_axis);
+        memcpy(sp.strain_axis, oph->strain_axis, sizeof sp.strain_axis);
+        memcpy(sp.cell, cell, sizeof cell);
+        sp.n_basis = oph->nb; sp.basis_key = oph->bkey; sp.basis_coeff = oph->bcoef;
+        for (int i = 0; i < PT_PH_NPROF; i++) sslots[i] = 0;
+        sslots[PT_TCH_SL] = sp.inst[PT_INST_SL]; sslots[PT_TCH_HL] = sp.inst[PT_INST_HL];
+        if (fam == PT_PROFILE_SIMPLE) sslots[PT_SIM_ETA] = sp.inst[PT_INST_ETA];
+        if (fam == PT_PROFILE_SPLIT) sslots[PT_SPL_ETA] = sp.inst[PT_INST_ETA_SPLIT];
+    }
     double l1 = m->P.lambda1 != 0 ? m->P.lambda1 : 1.54056, l2 = m->P.lambda2, ratio = m->P.ratio;
     int doublet = ratio > 1e-6 && l2 > 1e-6 && fabs(l1 - l2) > 1e-6;
     double lratio = doublet ? l2 / l1 : 1.0;
@@ -852,7 +1110,12 @@ static int eval_phase(or_model *m, int q, const double *p, double *y, refl_work 
             double base = tl + or_peak_shift(tl, disp, transp) + zero;
             double centre = base;   /* calculatePeakShift(base, params) is 0: no shft/trns in params */
             or_voigt *v = &R->v[pass];
-            or_prepare_voigt(ph->profile_type, prof, m->P.min_fwhm, base, centre, v);
+            if (oph->sep) {
+                sp.h[0] = L->r[k].h; sp.h[1] = L->r[k].k; sp.h[2] = L->r[k].l; sp.d = R->d;
+                or_prepare_voigt_sep(fam, sslots, &sp, pass ? l2 : m->P.lambda1, m->P.min_fwhm, base, centre, v);
+            } else {
+                or_prepare_voigt(fam, prof, m->P.min_fwhm, base, centre, v);
+            }
             R->present[pass] = 1;
             if (pass == 0) R->weight[0] = 1.0;
             else {
@@ -1041,19 +1304,69 @@ static double variance_min(double u, double v, double w, double lo, double hi) {
     return best;
 }
 
-/* gaussianVarianceGuards + varianceViolation (:1413-1496), non-separated profiles */
+/* The Stephens guard of a separated phase (profileVarianceGuards + guardVarianceMinimum,
+ * rietveld_core.js:1456-1478): the strain quartic at the unit direction of every probe
+ * h k l in [-2, 2] (one of each +-pair) and of every reflection of the current list; the
+ * least value.  Every direction's coefficient of term n is the quartic with S_n = 1. */
+static double stephens_guard_min(or_model *m, int q, const double *S) {
+    const or_phase *oph = &m->ph[q];
+    const refl_list *L = &oph->refl;
+    int cap = 62 + L->n, nd = 0;
+    int32_t *dirs = (int32_t *)malloc(sizeof(int32_t) * 3 * cap);
+    for (int h = -2; h <= 2; h++) for (int k = -2; k <= 2; k++) for (int l = -2; l <= 2; l++)
+        if (h > 0 || (h == 0 && (k > 0 || (k == 0 && l > 0)))) { dirs[3 * nd] = h; dirs[3 * nd + 1] = k; dirs[3 * nd + 2] = l; nd++; }
+    for (int r = 0; r < L->n; r++) {
+        int h = L->r[r].h, k = L->r[r].k, l = L->r[r].l, dup = 0;
+        if (!h && !k && !l) continue;
+        for (int e = 0; e < nd && !dup; e++) dup = dirs[3 * e] == h && dirs[3 * e + 1] == k && dirs[3 * e + 2] == l;
+        if (!dup) { dirs[3 * nd] = h; dirs[3 * nd + 1] = k; dirs[3 * nd + 2] = l; nd++; }
+    }
+    double worst = INFINITY, one[15];
+    for (int e = 0; e < nd; e++) {
+        double hv[3] = {dirs[3 * e], dirs[3 * e + 1], dirs[3 * e + 2]}, len = hypot_n(hv, 3), unit[3], value = 0;
+        for (int i = 0; i < 3; i++) unit[i] = hv[i] / len;
+        for (int n = 0; n < oph->nb; n++) {
+            for (int i = 0; i < 15; i++) one[i] = 0;
+            one[oph->bkey[n]] = 1;
+            double c = or_stephens_sum(unit, one, oph->nb, oph->bkey, oph->bcoef);
+            double sv = js_finite(S[oph->bkey[n]]) ? S[oph->bkey[n]] : 0;
+            value = value + c * sv;
+        }
+        if (value < worst) worst = value;
+    }
+    free(dirs);
+    return worst;
+}
+
+/* gaussianVarianceGuards + varianceViolation (:1413-1496): every Gaussian polynomial
+ * (a phase's own, or the dataset's instrument one for a separated phase, its values
+ * defaulted), and the Stephens guards */
 static double violation_at_params(or_model *m, const double *p) {
     double worst = 0;
+    int stephens = 0;
+    for (int q = 0; q < m->P.n_phases; q++) stephens |= m->ph[q].nb > 0;
+    if (stephens) refresh_reflections(m, p);
     for (int q = 0; q < m->P.n_phases; q++) {
         const double *pr = p + phase_base(&m->P, q) + PT_PH_PROF0;
-        int type = m->phases[q].profile_type;
+        const or_phase *oph = &m->ph[q];
+        int type = oph->fam;
         int sets = type == PT_PROFILE_SPLIT ? 2 : 1;
         for (int s = 0; s < sets; s++) {
             double u, v, w;
-            if (type == PT_PROFILE_TCH) { u = pr[PT_TCH_U]; v = pr[PT_TCH_V]; w = pr[PT_TCH_W]; }
+            if (oph->sep) {
+                int o = type == PT_PROFILE_SPLIT ? (s ? PT_INST_U_R : PT_INST_U_L) : PT_INST_U;
+                double c[3];
+                for (int i = 0; i < 3; i++) { double x = p[PT_P_INST0 + o + i]; c[i] = js_finite(x) ? x : INST_DEFAULT[o + i]; }
+                u = c[0]; v = c[1]; w = c[2];
+            }
+            else if (type == PT_PROFILE_TCH) { u = pr[PT_TCH_U]; v = pr[PT_TCH_V]; w = pr[PT_TCH_W]; }
             else if (type == PT_PROFILE_SIMPLE) { double gp = pr[PT_SIM_GP]; u = pr[PT_SIM_GU] + gp; v = pr[PT_SIM_GV]; w = pr[PT_SIM_GW] + gp; }
             else { int o = s ? 4 : 0; u = pr[PT_SPL_GU_L + o]; v = pr[PT_SPL_GV_L + o]; w = pr[PT_SPL_GW_L + o]; }
             double mn = variance_min(u, v, w, m->guard_lo, m->guard_hi);
+            if (mn < 0 && -mn > worst) worst = -mn;
+        }
+        if (oph->nb > 0) {
+            double mn = stephens_guard_min(m, q, pr + PT_GS_S0);
             if (mn < 0 && -mn > worst) worst = -mn;
         }
     }

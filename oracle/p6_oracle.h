@@ -41,6 +41,32 @@ typedef struct {
 } or_voigt;
 int or_prepare_voigt(int type, const double *prof, double min_fwhm, double tth_peak, double x0,
                      or_voigt *out);
+
+/* A separated (GSAS) profile at one reflection (profile.js:384-422): the instrument
+ * values (defaults applied), the phase's PT_GS_* sample slots, the broadening models,
+ * the cell, the reflection's Miller indices and d, and the Stephens basis. */
+typedef struct {
+    double inst[PT_N_INST];
+    const double *sample;
+    int size_model, strain_model;
+    double size_axis[3], strain_axis[3];
+    double cell[6];
+    double h[3], d;
+    int n_basis;
+    const int32_t *basis_key;
+    const double *basis_coeff;
+} or_sep;
+/* prepareVoigt of a separated profile: `prof` holds the family's eta / SL / HL slots
+ * (PT_SIM_ETA, PT_SPL_ETA, PT_TCH_SL/HL), the widths come from sep at wavelength lambda */
+int or_prepare_voigt_sep(int type, const double *prof, const or_sep *sep, double lambda, double min_fwhm,
+                         double tth_peak, double x0, or_voigt *out);
+/* calculateProfileWidths of a separated profile (floored), side 0 left / 1 right */
+void or_gsas_widths(int type, const or_sep *sep, double lambda, double min_fwhm, double tth, int side,
+                    double *gamma_g, double *gamma_l);
+int or_stephens_basis(int n_ops, const int32_t *op_r, int32_t *keys, double *coeff);   /* profile.js:334-363 */
+double or_axis_angle(const double cell[6], const double n[3], const double axis[3]);  /* profile.js:390 */
+double or_stephens_sum(const double h[3], const double *s15, int n_basis, const int32_t *keys,
+                       const double *coeff);                                          /* profile.js:375-378 */
 double or_eval_voigt(double x, const or_voigt *p);               /* profile.js:733-759 */
 double or_asym_area(const or_voigt *p);                          /* profile.js:799-831 */
 

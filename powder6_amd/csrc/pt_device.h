@@ -77,9 +77,21 @@ __device__ __forceinline__ float lorentz_width(float g, float fmin) {
 }
 __device__ __forceinline__ float orz(float v) { return (v != 0.0f && !isnan(v)) ? v : 0.0f; }
 
-// profile.js:501-537 calculateProfileWidths (side 0 = left, 1 = right / centre)
+// One line of a separated (GSAS) profile: the dataset's instrument values (defaults
+// applied, PT_INST_* order), the reflection's sample terms, the line's wavelength.
+struct SepLine {
+    const float *inst;         // nullptr: not a separated profile
+    float inv_size, strain;    // 1/size along the reflection, microstrain of the reflection
+    float sm, em;              // sizeMix, strainMix
+    float lam;
+};
+
+// profile.js:501-537 calculateProfileWidths (side 0 = left, 1 = right / centre); with a
+// SepLine the separated profile's gsasCwComponents (profile.js:384-422): the family's
+// instrument Caglioti and Lorentzian terms plus size 1.8 lam / (pi D cos) and strain
+// 0.018 e tan / pi, split between the Gaussian and Lorentzian parts by the mixes
 __device__ inline void profile_widths(int type, const float *p, float fmin_in, float tth, int side,
-                                      float &gG, float &gL) {
+                                      float &gG, float &gL, const SepLine sl = SepLine{nullptr, 0.f, 0.f, 0.f, 0.f, 0.f}) {
     float fmin = fmin_in > 0.0f ? fmin_in : 1e-3f;
     float th = tth * (kPiF / 180.0f) * 0.5f;
     float safe = fminf_js(th, 1.5707953267948966f);
@@ -89,7 +101,22 @@ __device__ inline void profile_widths(int type, const float *p, float fmin_in, f
     float tn = sn / cs;
     float cs_safe = fmaxf_js(cs, 1e-9f), cs2_safe = fmaxf_js(cs * cs, 1e-9f);
     float g, l;
-    if (type == 1) {
+    if (sl.inst) {
+        const float *I = sl.inst;
+        float U, V, W, X, Y, Z;
+        if (type == 2) {
+            const int o = side == 0 ? 9 : 13;          // U_L.. or U_R..
+            U = I[o]; V = I[o + 1]; W = I[o + 2]; X = I[o + 3]; Y = 0.0f; Z = 0.0f;
+        } else {
+            U = I[0]; V = I[1]; W = I[2]; X = I[3]; Y = I[4]; Z = I[5];
+        }
+        const float sg = 1.8f * sl.lam * sl.inv_size / (kPiF * cs_safe);
+        const float eg = 0.018f * sl.strain * tn / kPiF;
+        const float a = sg * (1.0f - sl.sm), b = eg * (1.0f - sl.em);
+        const float s2 = U * tn * tn + V * tn + W + (a * a + b * b) * 0.18033688011112042f;   // 1 / (8 ln 2)
+        g = gauss_width(s2, fmin);
+        l = lorentz_width(X / cs_safe + Y * tn + Z + sg * sl.sm + eg * sl.em, fmin);
+    } else if (type == 1) {
         float s2 = orz(p[0]) * tn * tn + orz(p[1]) * tn + orz(p[2]) + orz(p[3]) / cs2_safe;
         g = gauss_width(s2, fmin);
         l = lorentz_width(orz(p[4]) / cs_safe, fmin);
@@ -182,7 +209,8 @@ __device__ inline float asym_area(const Prep &pk) {
 }
 
 // profile.js:603-718 prepareVoigt (+ preparedProfileArea): widths evaluated at tth_peak.
-__device__ inline void prepare_line(int type, const float *prof, float fmin, float tth_peak, Prep &o) {
+__device__ inline void prepare_line(int type, const float *prof, float fmin, float tth_peak, Prep &o,
+                                    const SepLine sl = SepLine{nullptr, 0.f, 0.f, 0.f, 0.f, 0.f}) {
     float asym = 0.0f;
     if (type == 3 && (orz(prof[5]) != 0.0f || orz(prof[6]) != 0.0f) && tth_peak >= 0.1f && tth_peak < 180.0f) {
         float th = tth_peak * (kPiF / 180.0f) * 0.5f;
@@ -194,7 +222,7 @@ __device__ inline void prepare_line(int type, const float *prof, float fmin, flo
     float eta, fwhm_total, analytic;
     if (type == 1) {
         float gG, gL;
-        profile_widths(1, prof, fmin, tth_peak, 1, gG, gL);
+        profile_widths(1, prof, fmin, tth_peak, 1, gG, gL, sl);
         float HG = fmaxf_js(1e-9f, gG), HL = fmaxf_js(1e-9f, gL);
         eta = fmaxf_js(0.0f, fminf_js(1.0f, isfinite(prof[5]) ? prof[5] : 0.5f));
         fwhm_total = tch_fwhm(HG, HL);
@@ -205,8 +233,8 @@ __device__ inline void prepare_line(int type, const float *prof, float fmin, flo
         analytic = cl * kLArea * HL + cg * kGArea * HG;
     } else if (type == 2) {
         float gGL, gLL, gGR, gLR;
-        profile_widths(2, prof, fmin, tth_peak, 0, gGL, gLL);
-        profile_widths(2, prof, fmin, tth_peak, 1, gGR, gLR);
+        profile_widths(2, prof, fmin, tth_peak, 0, gGL, gLL, sl);
+        profile_widths(2, prof, fmin, tth_peak, 1, gGR, gLR, sl);
         float HGL = fmaxf_js(1e-9f, gGL), HLL = fmaxf_js(1e-9f, gLL), HGR = fmaxf_js(1e-9f, gGR), HLR = fmaxf_js(1e-9f, gLR);
         eta = fmaxf_js(0.0f, fminf_js(1.0f, isfinite(prof[8]) ? prof[8] : 0.5f));
         fwhm_total = fmaxf_js(tch_fwhm(HGL, HLL), tch_fwhm(HGR, HLR));
@@ -218,7 +246,7 @@ __device__ inline void prepare_line(int type, const float *prof, float fmin, flo
         analytic = 0.5f * (clL * kLArea * HLL + cgL * kGArea * HGL) + 0.5f * (clR * kLArea * HLR + cgR * kGArea * HGR);
     } else {
         float gG, gL;
-        profile_widths(3, prof, fmin, tth_peak, 1, gG, gL);
+        profile_widths(3, prof, fmin, tth_peak, 1, gG, gL, sl);
         float HG = fmaxf_js(1e-9f, gG), HL = fmaxf_js(1e-9f, gL);
         float fw = fmaxf_js(1e-9f, tch_fwhm(HG, HL));
         eta = 0.0f;
@@ -368,7 +396,13 @@ __device__ inline bool profile_valid(const DevPlan &P, const double *p) {
         int type = P.phase[q].profile_type;
         for (int s = 0; s < (type == 2 ? 2 : 1); ++s) {
             double u, v, w;
-            if (type == 3) { u = pr[0]; v = pr[1]; w = pr[2]; }
+            if (P.phase[q].sep) {
+                // the dataset's instrument polynomial (U, V, W or a split flank's), a value
+                // that is not finite read as its default 2, -2, 5 (rietveld_core.js:1483-1488)
+                const double *in = p + PTK_P_INST0 + (type == 2 ? (s ? 13 : 9) : 0);
+                u = isfinite(in[0]) ? in[0] : 2.0; v = isfinite(in[1]) ? in[1] : -2.0; w = isfinite(in[2]) ? in[2] : 5.0;
+            }
+            else if (type == 3) { u = pr[0]; v = pr[1]; w = pr[2]; }
             else if (type == 1) { u = pr[0] + pr[3]; v = pr[1]; w = pr[2] + pr[3]; }
             else { int o = s ? 4 : 0; u = pr[o]; v = pr[o + 1]; w = pr[o + 2]; }
             if (variance_min(u, v, w, P.guard_lo, P.guard_hi) < 0.0) return false;

@@ -373,6 +373,7 @@ int validate(const pt_problem *pb) {
     if (pb->n_phases < 1 || pb->n_phases > PTK_MAX_PHASES) return fail(PT_ERR_ARG, "PT supports 1 to 4 phases per dataset.");
     if (pb->n_bkg < 0 || pb->n_bkg > PT_MAX_BKG) return fail(PT_ERR_ARG, "Choose 0 to 24 Chebyshev terms.");
     if (pb->bkg_type != PT_BKG_CHEBYSHEV && pb->bkg_type != PT_BKG_SPLINE) return fail(PT_ERR_ARG, "unknown background form");
+    if (pb->resolution_type < 0 || pb->resolution_type > 4) return fail(PT_ERR_ARG, "unknown instrument profile type");
     if (pb->bkg_type == PT_BKG_SPLINE) {
         if (pb->n_bkg > 0 && !pb->bkg_x) return fail(PT_ERR_ARG, "spline background anchors are missing");
         for (int j = 0; j < pb->n_bkg; ++j)
@@ -383,7 +384,11 @@ int validate(const pt_problem *pb) {
     for (int q = 0; q < pb->n_phases; ++q) {
         const pt_phase &ph = pb->phases[q];
         if (ph.n_ops < 1 || !ph.op_r || !ph.op_t) return fail(PT_ERR_ARG, "a phase has no symmetry operators");
-        if (ph.profile_type < 1 || ph.profile_type > 3) return fail(PT_ERR_ARG, "unknown profile type");
+        if (ph.profile_type < 1 || ph.profile_type > 4) return fail(PT_ERR_ARG, "unknown profile type");
+        if (ph.separated || ph.profile_type == PT_PROFILE_GSAS_CW) {
+            if (ph.size_model < 0 || ph.size_model > 2 || ph.strain_model < 0 || ph.strain_model > 2)
+                return fail(PT_ERR_ARG, "unknown size or strain broadening model");
+        }
         if (ph.n_atoms < 0 || (ph.n_atoms && !ph.atoms)) return fail(PT_ERR_ARG, "a phase has an invalid atom list");
         for (int j = 0; j < ph.n_atoms; ++j)
             if (ph.atoms[j].n_gauss < 0 || ph.atoms[j].n_gauss > PT_MAX_GAUSS) return fail(PT_ERR_ARG, "a scattering factor has more than 5 Gaussians");
@@ -511,6 +516,110 @@ static std::vector<double> start_background(const pt_plan *pl) {
     return bk;
 }
 
+// ---- separated profiles ------------------------------------------------------------
+// GSAS quartic monomials h^a k^b l^c, a + b + c = 4 (js/profile.js:332 order)
+const int kQuartic[15][3] = {{4, 0, 0}, {0, 4, 0}, {0, 0, 4}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}, {3, 1, 0}, {3, 0, 1},
+                             {1, 3, 0}, {1, 0, 3}, {0, 3, 1}, {0, 1, 3}, {2, 1, 1}, {1, 2, 1}, {1, 1, 2}};
+
+int quartic_index(int a, int b, int c) {
+    for (int i = 0; i < 15; ++i) if (kQuartic[i][0] == a && kQuartic[i][1] == b && kQuartic[i][2] == c) return i;
+    return -1;
+}
+double quartic_weight(int j) {     // 1, 2, 3 or 4 as the exponents hold a 4, a 3, a 0 or none
+    const int *e = kQuartic[j];
+    auto has = [&](int v) { return e[0] == v || e[1] == v || e[2] == v; };
+    return has(4) ? 1.0 : has(3) ? 2.0 : has(0) ? 3.0 : 4.0;
+}
+
+// The Stephens strain basis of a setting (js/profile.js:334-363 gsasStephensBasis):
+// each weighted monomial of h' = R^T h averaged over the distinct rotations, kept when
+// linearly independent of the kept ones, scaled to its key's weight at its leading
+// coefficient.  Row k of `basis` (15 x 15) holds the term of key k, or zeros.
+bool stephens_basis(const pt_phase &ph, std::vector<float> &basis) {
+    std::vector<std::array<int, 9>> rots;
+    for (int o = 0; o < ph.n_ops; ++o) {
+        std::array<int, 9> r;
+        for (int x = 0; x < 9; ++x) r[x] = ph.op_r[9 * o + x];
+        if (std::find(rots.begin(), rots.end(), r) == rots.end()) rots.push_back(r);
+    }
+    basis.assign(15 * 15, 0.0f);
+    std::vector<std::array<double, 15>> orth;
+    bool any = false;
+    for (int j = 0; j < 15; ++j) {
+        std::array<double, 15> c{};
+        for (const auto &R : rots) {
+            // expand prod_axis (sum_k R[3k + axis] h_k)^e_axis as a table over exponents
+            std::map<std::array<int, 3>, double> poly{{{0, 0, 0}, quartic_weight(j)}};
+            for (int axis = 0; axis < 3; ++axis)
+                for (int rep = 0; rep < kQuartic[j][axis]; ++rep) {
+                    std::map<std::array<int, 3>, double> next;
+                    for (const auto &t : poly)
+                        for (int k = 0; k < 3; ++k) {
+                            const int cc = R[3 * k + axis];
+                            if (!cc) continue;
+                            std::array<int, 3> e = t.first;
+                            ++e[k];
+                            next[e] += t.second * cc;
+                        }
+                    poly.swap(next);
+                }
+            for (const auto &t : poly) {
+                const int i = quartic_index(t.first[0], t.first[1], t.first[2]);
+                if (i >= 0) c[i] += t.second / (double)rots.size();
+            }
+        }
+        std::array<double, 15> v = c;
+        for (const auto &q : orth) {
+            double dot = 0;
+            for (int i = 0; i < 15; ++i) dot += v[i] * q[i];
+            for (int i = 0; i < 15; ++i) v[i] -= dot * q[i];
+        }
+        double norm = 0;
+        for (int i = 0; i < 15; ++i) norm += v[i] * v[i];
+        norm = std::sqrt(norm);
+        if (norm < 1e-8) continue;
+        for (int i = 0; i < 15; ++i) v[i] /= norm;
+        orth.push_back(v);
+        int lead = 0;
+        while (lead < 15 && !(std::fabs(c[lead]) > 1e-8)) ++lead;
+        const double factor = quartic_weight(lead) / c[lead];
+        for (int i = 0; i < 15; ++i) basis[15 * j + i] = std::fabs(c[i]) < 1e-9 ? 0.0f : (float)(c[i] * factor);
+        any = true;
+    }
+    return any;
+}
+
+// Stephens guard directions (rietveld_core.js:1456-1478, js/pt_gpu.js:201-205): every
+// h k l in [-2, 2] with the first non-zero index positive, then every envelope reflection
+// not yet listed; per direction the 15 key coefficients of the quartic at h / |h|.
+void stephens_guard(const std::vector<float> &basis, int q, const std::vector<int32_t> &hklm_rows,
+                    std::vector<double> &out, int &count) {
+    std::vector<std::array<int, 3>> dirs;
+    for (int h = -2; h <= 2; ++h)
+        for (int k = -2; k <= 2; ++k)
+            for (int l = -2; l <= 2; ++l)
+                if (h > 0 || (h == 0 && (k > 0 || (k == 0 && l > 0)))) dirs.push_back({h, k, l});
+    for (size_t r = 0; r + 4 <= hklm_rows.size(); r += 4) {
+        std::array<int, 3> d{hklm_rows[r], hklm_rows[r + 1], hklm_rows[r + 2]};
+        if (!d[0] && !d[1] && !d[2]) continue;
+        if (std::find(dirs.begin(), dirs.end(), d) == dirs.end()) dirs.push_back(d);
+    }
+    for (const auto &d : dirs) {
+        const double len = std::sqrt((double)d[0] * d[0] + (double)d[1] * d[1] + (double)d[2] * d[2]);
+        const double u[3] = {d[0] / len, d[1] / len, d[2] / len};
+        double mono[15];
+        for (int i = 0; i < 15; ++i)
+            mono[i] = std::pow(u[0], kQuartic[i][0]) * std::pow(u[1], kQuartic[i][1]) * std::pow(u[2], kQuartic[i][2]);
+        for (int k = 0; k < 15; ++k) {
+            double g = 0;
+            for (int i = 0; i < 15; ++i) g += (double)basis[15 * k + i] * mono[i];
+            out.push_back(g);
+        }
+    }
+    count = (int)dirs.size();
+    (void)q;
+}
+
 static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     if (!out) return fail(PT_ERR_ARG, "output handle is NULL");
     *out = nullptr;
@@ -546,7 +655,10 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     std::vector<int> rep_r;
     std::vector<double> rep_t;
     std::vector<ptk::DevAtom> atoms;
+    std::vector<float> sbasis((size_t)PTK_MAX_PHASES * 225, 0.0f);
+    std::vector<double> sguard;
     ptk::DevPlan &P = pl->P;
+    P.any_sep = 0; P.has_stephens = 0;
     for (int q = 0; q < pb->n_phases; ++q) {
         const pt_phase &ph = pb->phases[q];
         int pbase = phase_base_of(pb, q);
@@ -586,7 +698,20 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         dp.n_reps_full = (int)all_reps.size();
         dp.system = ph.system;
         dp.unique_axis = ph.unique_axis;
-        dp.profile_type = ph.profile_type;
+        // a separated profile takes the dataset's instrument family (rietveld_core.js:700-702)
+        dp.sep = (ph.separated || ph.profile_type == PT_PROFILE_GSAS_CW) ? 1 : 0;
+        dp.profile_type = dp.sep && pb->resolution_type ? pb->resolution_type : ph.profile_type;
+        if (dp.profile_type == PT_PROFILE_GSAS_CW) dp.profile_type = PT_PROFILE_TCH;
+        dp.size_model = dp.sep ? ph.size_model : 0;
+        dp.strain_model = dp.sep ? ph.strain_model : 0;
+        for (int w = 0; w < 2; ++w) {
+            // broadeningConfig axis(): three finite numbers, one non-zero, else 0 0 1
+            const double *a = w ? ph.strain_axis : ph.size_axis;
+            bool ok = std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]) && (a[0] != 0 || a[1] != 0 || a[2] != 0);
+            float *o = w ? dp.strain_axis : dp.size_axis;
+            for (int i = 0; i < 3; ++i) o[i] = ok ? (float)a[i] : (i == 2 ? 1.0f : 0.0f);
+        }
+        dp.stephens = 0; dp.n_sdir = 0; dp.sdir_off = 0;
         dp.n_reps = (int)reps.size();
         dp.rep_off = (int)(rep_r.size() / 9);
         dp.n_atoms = ph.n_atoms;
@@ -627,6 +752,17 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
             eq_off.push_back((int)(eq.size() / 3));
             rows.push_back(r.h); rows.push_back(r.k); rows.push_back(r.l); rows.push_back(r.m);
         }
+        if (dp.sep && dp.strain_model == PT_STRAIN_STEPHENS) {
+            std::vector<float> basis;
+            if (stephens_basis(ph, basis)) {
+                dp.stephens = 1;
+                std::copy(basis.begin(), basis.end(), sbasis.begin() + (size_t)q * 225);
+                dp.sdir_off = (int)(sguard.size() / 15);
+                stephens_guard(basis, q, rows, sguard, dp.n_sdir);
+            }
+        }
+        if (dp.sep) P.any_sep = 1;
+        if (dp.stephens) P.has_stephens = 1;
         pl->phase_hklm.push_back(std::move(rows));
     }
     pl->nr = (int)hklm.size();
@@ -720,6 +856,8 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     }
     std::vector<double> vlo(pb->lo, pb->lo + pb->n_vars), vhi(pb->hi, pb->hi + pb->n_vars), vsc(pb->scale, pb->scale + pb->n_vars);
     if (eq.empty()) eq.push_back(0);
+    if (sguard.empty()) sguard.assign(15, 0.0);
+    float *d_sbasis = nullptr; double *d_sguard = nullptr;
     double *d_tth, *d_bknots = nullptr; float2 *d_tff, *d_obs; float *d_zcheb; int4 *d_hklm; int *d_bseg = nullptr;
     float4 *d_bcoef = nullptr; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
     short *d_eq; double *d_rep_t, *d_base, *d_coeff, *d_lo, *d_hi, *d_sc, *d_state; ptk::DevAtom *d_atoms;
@@ -732,11 +870,13 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         (rc = upload(pl, &d_rep_r, rep_r)) || (rc = upload(pl, &d_rep_t, rep_t)) || (rc = upload(pl, &d_atoms, atoms)) ||
         (rc = upload(pl, &d_base, base)) || (rc = upload(pl, &d_term_off, term_off)) || (rc = upload(pl, &d_term_var, term_var)) ||
         (rc = upload(pl, &d_coeff, term_coeff)) || (rc = upload(pl, &d_lo, vlo)) || (rc = upload(pl, &d_hi, vhi)) ||
-        (rc = upload(pl, &d_sc, vsc)) || (rc = dalloc(pl, &d_state, (size_t)pl->chains * pl->stride))) {
+        (rc = upload(pl, &d_sc, vsc)) || (rc = upload(pl, &d_sbasis, sbasis)) || (rc = upload(pl, &d_sguard, sguard)) ||
+        (rc = dalloc(pl, &d_state, (size_t)pl->chains * pl->stride))) {
         pt_plan_destroy(pl);
         return rc;
     }
     P.bkg_spline = pb->bkg_type == PT_BKG_SPLINE ? 1 : 0; P.bseg = d_bseg; P.bcoef = d_bcoef; P.bknots = d_bknots;
+    P.sbasis = d_sbasis; P.sguard = d_sguard;
     P.tth = d_tth; P.tff = d_tff; P.zcheb = d_zcheb; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
     P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
     P.term_var = d_term_var; P.term_coeff = d_coeff; P.vlo = d_lo; P.vhi = d_hi; P.vscale = d_sc; P.state = d_state;

@@ -23,7 +23,9 @@ struct SetupShared {
     double metric[PTK_MAX_PHASES][6];         // reciprocal quadratic forms
     double apos[PTK_MAX_ATOMS][3];            // atom positions (fractional)
     float aocc[PTK_MAX_ATOMS], aB[PTK_MAX_ATOMS];
-    float prof[PTK_MAX_PHASES][16];           // profile slots (FP32)
+    float prof[PTK_MAX_PHASES][PTK_PROF_LDS]; // profile slots (FP32)
+    float inst[PTK_N_INST];                   // instrument values of separated profiles (defaults applied)
+    SepPhase sep[PTK_MAX_PHASES];             // separated profiles: per-trial sample broadening terms
     float bkg[PTK_MAX_BKG_F];                 // Chebyshev coefficients, or spline values [0, 24) and tangents [24, 48)
     float red_s[32], red_c[32];
     int phase_ok[PTK_MAX_PHASES];
@@ -88,6 +90,150 @@ __device__ inline void spline_setup(const DevPlan &P, const double *pv, float *o
     }
 }
 
+// the instrument defaults of rietveld_core.js:413, 454-459 (PT_INST_* order)
+__device__ __forceinline__ float inst_default(int i) {
+    const float d[PTK_N_INST] = {2.f, -2.f, 5.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.5f, 2.f, -2.f, 5.f, 1.f, 2.f, -2.f, 5.f, 1.f, 0.5f};
+    return d[i];
+}
+__device__ __forceinline__ double finite_or(double v, double d) { return isfinite(v) ? v : d; }
+
+// The per-trial terms of a separated profile (profile.js:384-422 gsasCwComponents with
+// crystal.js:263-275 cellBasis and profile.js:364-369 gsasReciprocalVector): the
+// reciprocal basis of the trial cell, the unit reciprocal vectors of the uniaxial axes,
+// the ellipsoid's inverse Euler rotation (about c by -rotC, then b by -rotB, then a by
+// -rotA) and inverse squared lengths, and the Stephens form e = sum_k S_k basis_k.
+// FP64 here, stored FP32.  One thread per phase.
+__device__ inline void sep_phase_setup(const DevPlan &P, const DevPhase &ph, const double *pp, int q, SepPhase &o) {
+    const double *S = pp + 8;
+    double a = pp[0], b = pp[1] > 0.0 ? pp[1] : a, c = pp[2] > 0.0 ? pp[2] : a;
+    double al = 90.0, be = 90.0, ga = 90.0;
+    switch (ph.system) {
+    case 6: b = c = a; break;
+    case 3: b = a; break;
+    case 4: case 5: b = a; ga = 120.0; break;
+    case 2: break;
+    case 1: {
+        const int ax = ph.unique_axis;
+        const double v = finite_or(pp[3 + (ax == 0 ? 0 : ax == 2 ? 2 : 1)], 90.0);
+        if (ax == 0) al = v; else if (ax == 2) ga = v; else be = v;
+        break;
+    }
+    default: al = finite_or(pp[3], 90.0); be = finite_or(pp[4], 90.0); ga = finite_or(pp[5], 90.0);
+    }
+    const double ca = cos(al * kD2R), cb = cos(be * kD2R), cg = cos(ga * kD2R), sg = sin(ga * kD2R);
+    const double vf = sqrt(fmax(0.0, 1.0 - ca * ca - cb * cb - cg * cg + 2.0 * ca * cb * cg));
+    const double av[3] = {a, 0.0, 0.0}, bv[3] = {b * cg, b * sg, 0.0};
+    const double cv[3] = {c * cb, c * (ca - cb * cg) / sg, c * vf / sg};
+    double rows[3][3] = {{bv[1] * cv[2] - bv[2] * cv[1], bv[2] * cv[0] - bv[0] * cv[2], bv[0] * cv[1] - bv[1] * cv[0]},
+                         {cv[1] * av[2] - cv[2] * av[1], cv[2] * av[0] - cv[0] * av[2], cv[0] * av[1] - cv[1] * av[0]},
+                         {av[1] * bv[2] - av[2] * bv[1], av[2] * bv[0] - av[0] * bv[2], av[0] * bv[1] - av[1] * bv[0]}};
+    const double V = av[0] * rows[0][0] + av[1] * rows[0][1] + av[2] * rows[0][2];
+    for (int r = 0; r < 3; ++r)
+        for (int i = 0; i < 3; ++i) { rows[r][i] /= V; o.rb[3 * r + i] = (float)rows[r][i]; }
+    for (int w = 0; w < 2; ++w) {
+        const float *ax = w ? ph.strain_axis : ph.size_axis;
+        double A[3];
+        for (int i = 0; i < 3; ++i) A[i] = ax[0] * rows[0][i] + ax[1] * rows[1][i] + ax[2] * rows[2][i];
+        double n = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+        if (n == 0.0) n = 1.0;
+        float *out = w ? o.axe : o.axs;
+        for (int i = 0; i < 3; ++i) out[i] = (float)(A[i] / n);
+    }
+    // rotation: the columns are the images of the unit vectors under the three plane
+    // rotations in their order
+    const double ang[3] = {finite_or(S[6], 0.0), finite_or(S[5], 0.0), finite_or(S[4], 0.0)};   // rotC, rotB, rotA
+    const int axis_of[3] = {2, 1, 0};
+    for (int col = 0; col < 3; ++col) {
+        double v[3] = {col == 0 ? 1.0 : 0.0, col == 1 ? 1.0 : 0.0, col == 2 ? 1.0 : 0.0};
+        for (int r = 0; r < 3; ++r) {
+            const double t = -ang[r] * kPi / 180.0, cs = cos(t), sn = sin(t);
+            const int i = (axis_of[r] + 1) % 3, j = (axis_of[r] + 2) % 3;
+            const double x = v[i], y = v[j];
+            v[i] = cs * x - sn * y; v[j] = sn * x + cs * y;
+        }
+        for (int i = 0; i < 3; ++i) o.rot[3 * i + col] = (float)v[i];
+    }
+    const double size = finite_or(S[0], 1.0), sizeAx = finite_or(S[1], 1.0);
+    const double L[3] = {size, finite_or(S[2], 1.0), finite_or(S[3], 1.0)};
+    for (int i = 0; i < 3; ++i) { const double m = fmax(0.001, L[i]); o.inv_l2[i] = (float)(1.0 / (m * m)); }
+    const double i1 = fmax(0.001, size), a1 = fmax(0.001, sizeAx);
+    o.inv_i2 = (float)(1.0 / (i1 * i1)); o.inv_a2 = (float)(1.0 / (a1 * a1));
+    o.inv_size = (float)(1.0 / i1);
+    o.si = (float)fmax(0.0, finite_or(S[7], 0.0));
+    o.sa = (float)fmax(0.0, finite_or(S[8], 0.0));
+    o.sm = (float)fmin(1.0, fmax(0.0, finite_or(S[9], 1.0)));
+    o.em = (float)fmin(1.0, fmax(0.0, finite_or(S[10], 1.0)));
+    for (int i = 0; i < 15; ++i) {
+        double e = 0.0;
+        if (ph.stephens)
+            for (int k = 0; k < 15; ++k) e += finite_or(S[11 + k], 0.0) * (double)P.sbasis[(q * 15 + k) * 15 + i];
+        o.e[i] = (float)e;
+    }
+}
+
+// the sample terms of one reflection (h k l, spacing d) of a separated phase:
+// 1/size along it and its microstrain (profile.js:384-411)
+__device__ __forceinline__ void sep_reflection(const DevPhase &ph, const SepPhase &sp, int h, int k, int l, float d,
+                                               float &inv_size, float &strain) {
+    const float fh = (float)h, fk = (float)k, fl = (float)l;
+    float n[3] = {0.f, 0.f, 0.f};
+    if (ph.size_model != 0 || ph.strain_model == 1) {
+        float qv[3];
+        for (int i = 0; i < 3; ++i) qv[i] = fh * sp.rb[i] + fk * sp.rb[3 + i] + fl * sp.rb[6 + i];
+        float nn = sqrtf(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2]);
+        if (nn == 0.0f) nn = 1.0f;
+        for (int i = 0; i < 3; ++i) n[i] = qv[i] / nn;
+    }
+    inv_size = sp.inv_size;
+    if (ph.size_model == 1) {
+        float c = n[0] * sp.axs[0] + n[1] * sp.axs[1] + n[2] * sp.axs[2];
+        const float c2 = fminf(1.0f, fmaxf(0.0f, c * c));
+        inv_size = sqrtf((1.0f - c2) * sp.inv_i2 + c2 * sp.inv_a2);
+    } else if (ph.size_model == 2) {
+        float acc = 0.0f;
+        for (int i = 0; i < 3; ++i) {
+            const float v = sp.rot[3 * i] * n[0] + sp.rot[3 * i + 1] * n[1] + sp.rot[3 * i + 2] * n[2];
+            acc += v * v * sp.inv_l2[i];
+        }
+        inv_size = sqrtf(acc);
+    }
+    strain = sp.si;
+    if (ph.strain_model == 1) {
+        float c = n[0] * sp.axe[0] + n[1] * sp.axe[1] + n[2] * sp.axe[2];
+        const float c2 = fminf(1.0f, fmaxf(0.0f, c * c));
+        const float i = sp.si, a = sp.sa;
+        strain = c2 <= 1e-14f ? i : c2 >= 1.0f ? a : i * a / fmaxf(1e-30f, sqrtf(i * i * c2 + a * a * (1.0f - c2)));
+    } else if (ph.strain_model == 2) {
+        const float h2 = fh * fh, k2 = fk * fk, l2 = fl * fl;
+        const float m[15] = {h2 * h2, k2 * k2, l2 * l2, h2 * k2, h2 * l2, k2 * l2, h2 * fh * fk, h2 * fh * fl, k2 * fk * fh,
+                             l2 * fl * fh, k2 * fk * fl, l2 * fl * fk, h2 * fk * fl, k2 * fh * fl, l2 * fh * fk};
+        float sum = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) sum = fmaf(sp.e[i], m[i], sum);
+        strain = d * d * sqrtf(fmaxf(0.0f, sum));
+    }
+}
+
+// the Stephens guards (rietveld_core.js:1456-1478, js/pt_gpu.js:201-205): the strain
+// quartic of every separated Stephens phase non-negative at every probe direction and
+// envelope reflection.  Every thread takes some directions; true when this thread saw
+// a negative value.
+__device__ inline bool stephens_bad(const DevPlan &P, const double *p) {
+    bool bad = false;
+    for (int q = 0; q < P.n_phases; ++q) {
+        const DevPhase &ph = P.phase[q];
+        if (!ph.sep || !ph.stephens) continue;
+        const double *S = p + ph.param_base + 8 + 11;
+        for (int dn = threadIdx.x; dn < ph.n_sdir; dn += blockDim.x) {
+            const double *g = P.sguard + (size_t)(ph.sdir_off + dn) * 15;
+            double v = 0.0;
+            for (int k = 0; k < 15; ++k) v += g[k] * finite_or(S[k], 0.0);
+            if (v < 0.0) bad = true;
+        }
+    }
+    return bad;
+}
+
 // p = base + M x  (rietveld_core.js:1855-1859 apply, affine links of atomic_constraints.js)
 __device__ inline void materialize(const DevPlan &P, const double *x, SetupShared &sh) {
     for (int k = threadIdx.x; k < P.n_params; k += blockDim.x) {
@@ -106,10 +252,25 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
         sh.phase_ok[tid] = phase_metric(ph, sh.p + ph.param_base, sh.metric[tid]) ? 1 : 0;
     }
     if (tid == 63 || (nt < 64 && tid == 0)) sh.prof_ok = profile_valid(P, sh.p) ? 1 : 0;
-    for (int i = tid; i < P.n_phases * 16; i += nt) {
-        int q = i >> 4, s = i & 15;
+    for (int i = tid; i < P.n_phases * PTK_PH_NPROF; i += nt) {
+        int q = i / PTK_PH_NPROF, s = i % PTK_PH_NPROF;
         sh.prof[q][s] = (float)sh.p[P.phase[q].param_base + 8 + s];
     }
+    for (int i = tid; i < PTK_N_INST; i += nt) sh.inst[i] = (float)finite_or(sh.p[PTK_P_INST0 + i], inst_default(i));
+    for (int q = 0; q < P.n_phases; ++q)
+        if (P.phase[q].sep && tid == (32 + q) % nt) {
+            sep_phase_setup(P, P.phase[q], sh.p + P.phase[q].param_base, q, sh.sep[q]);
+            // the family's eta / SL / HL slots, read from the instrument values
+            // (rietveld_core.js:699-701), after the model slots
+            float *fs = sh.prof[q] + PTK_PH_NPROF;
+            for (int i = 0; i < 9; ++i) fs[i] = 0.0f;
+            const int type = P.phase[q].profile_type;
+            const double *in = sh.p + PTK_P_INST0;
+            if (type == 3) { fs[5] = (float)finite_or(in[6], 0.0); fs[6] = (float)finite_or(in[7], 0.0); }
+            else if (type == 1) fs[5] = (float)finite_or(in[8], 0.5);
+            else fs[8] = (float)finite_or(in[17], 0.5);
+        }
+    const bool strain_bad = P.has_stephens ? stephens_bad(P, sh.p) : false;
     if (!P.bkg_spline) {
         for (int i = tid; i < P.n_bkg; i += nt) sh.bkg[i] = (float)sh.p[3 + i];
     } else if (tid == nt - 1) {
@@ -120,7 +281,7 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
         for (int j = tid; j < ph.n_atoms; j += nt) {
             int ga = ph.atom_off + j;
             const DevAtom &A = P.atoms[ga];
-            const double *ap = sh.p + ph.param_base + 24 + 5 * j;
+            const double *ap = sh.p + ph.param_base + PTK_PH_ATOM0 + 5 * j;
             double r[3] = {A.r0[0], A.r0[1], A.r0[2]};      // sitePosition, rietveld_core.js:372-378
             for (int f = 0; f < 3; ++f)
                 if (A.free_mask & (1 << f)) {
@@ -132,11 +293,12 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
             sh.aB[ga] = (float)ap[4];
         }
     }
-    __syncthreads();
+    const int any_bad = __syncthreads_or(strain_bad ? 1 : 0);
     if (tid == 0) {
         int ok = 1;
         for (int q = 0; q < P.n_phases; ++q) ok &= sh.phase_ok[q];
         sh.cell_ok = ok;
+        if (any_bad) sh.prof_ok = 0;
     }
     __syncthreads();
 }
@@ -272,7 +434,16 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
     if (!(tth > 0.0 && tth < 180.0)) return;                 // hasRenderablePosition
     double base1 = tth + zero;
     Prep pk1;
-    prepare_line(ph.profile_type, sh.prof[q], (float)P.min_fwhm, (float)base1, pk1);
+    // separated profile: the sample terms of this reflection, and the family's eta / SL /
+    // HL slots filled from the instrument values (rietveld_core.js:694-708)
+    SepLine sl{nullptr, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float *prof = sh.prof[q];
+    if (ph.sep) {
+        sep_reflection(ph, sh.sep[q], hk.x, hk.y, hk.z, (float)(1.0 / s), sl.inv_size, sl.strain);
+        sl.inst = sh.inst; sl.sm = sh.sep[q].sm; sl.em = sh.sep[q].em; sl.lam = (float)P.lambda1;
+        prof = sh.prof[q] + PTK_PH_NPROF;
+    }
+    prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base1, pk1, sl);
     if (!(pk1.area > 0.0f)) return;
     float lp1 = lp_factor((float)base1, (float)P.pol_k);
     const double *pp = sh.p + ph.param_base;
@@ -294,7 +465,8 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
             double b2 = 2.0 * asin(st2) * kR2D;
             double base2 = b2 + (double)peak_shift(b2, disp, transp) + zero;
             Prep pk2;
-            prepare_line(ph.profile_type, sh.prof[q], (float)P.min_fwhm, (float)base2, pk2);
+            sl.lam = (float)P.lambda2;
+            prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base2, pk2, sl);
             float w2 = (float)P.ratio * pk1.area / pk2.area * (lp_factor((float)base2, (float)P.pol_k) / lp1);
             emit_line(P, pk2, base2, height * w2, rec[1], span[1]);
         }
@@ -508,7 +680,9 @@ struct TrialImage {
     double metric[PTK_MAX_PHASES][6];
     double apos[PTK_MAX_ATOMS][3];
     float aocc[PTK_MAX_ATOMS], aB[PTK_MAX_ATOMS];
-    float prof[PTK_MAX_PHASES][16];
+    float prof[PTK_MAX_PHASES][PTK_PROF_LDS];
+    float inst[PTK_N_INST];
+    SepPhase sep[PTK_MAX_PHASES];
     float bkg[PTK_MAX_BKG_F];
     int phase_ok[PTK_MAX_PHASES];
     int cell_ok, prof_ok, go, boot;
@@ -528,7 +702,13 @@ __device__ inline void image_store(const DevPlan &P, const SetupShared &sh, Tria
     for (int i = tid; i < P.n_phases * 6; i += nt) img->metric[i / 6][i % 6] = sh.metric[i / 6][i % 6];
     for (int i = tid; i < na * 3; i += nt) img->apos[i / 3][i % 3] = sh.apos[i / 3][i % 3];
     for (int i = tid; i < na; i += nt) { img->aocc[i] = sh.aocc[i]; img->aB[i] = sh.aB[i]; }
-    for (int i = tid; i < P.n_phases * 16; i += nt) img->prof[i >> 4][i & 15] = sh.prof[i >> 4][i & 15];
+    for (int i = tid; i < P.n_phases * PTK_PROF_LDS; i += nt) img->prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS] = sh.prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS];
+    if (P.any_sep) {
+        for (int i = tid; i < PTK_N_INST; i += nt) img->inst[i] = sh.inst[i];
+        const int nw = (int)(sizeof(SepPhase) / sizeof(float));
+        for (int i = tid; i < P.n_phases * nw; i += nt)
+            reinterpret_cast<float *>(img->sep)[i] = reinterpret_cast<const float *>(sh.sep)[i];
+    }
     for (int i = tid; i < PTK_MAX_BKG_F; i += nt) img->bkg[i] = sh.bkg[i];
     if (tid < P.n_phases) img->phase_ok[tid] = sh.phase_ok[tid];
     if (tid == 0) { img->cell_ok = sh.cell_ok; img->prof_ok = sh.prof_ok; img->go = sh.go; img->boot = sh.accept; }
@@ -541,7 +721,13 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Setup
     for (int i = tid; i < P.n_phases * 6; i += nt) sh.metric[i / 6][i % 6] = img->metric[i / 6][i % 6];
     for (int i = tid; i < na * 3; i += nt) sh.apos[i / 3][i % 3] = img->apos[i / 3][i % 3];
     for (int i = tid; i < na; i += nt) { sh.aocc[i] = img->aocc[i]; sh.aB[i] = img->aB[i]; }
-    for (int i = tid; i < P.n_phases * 16; i += nt) sh.prof[i >> 4][i & 15] = img->prof[i >> 4][i & 15];
+    for (int i = tid; i < P.n_phases * PTK_PROF_LDS; i += nt) sh.prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS] = img->prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS];
+    if (P.any_sep) {
+        for (int i = tid; i < PTK_N_INST; i += nt) sh.inst[i] = img->inst[i];
+        const int nw = (int)(sizeof(SepPhase) / sizeof(float));
+        for (int i = tid; i < P.n_phases * nw; i += nt)
+            reinterpret_cast<float *>(sh.sep)[i] = reinterpret_cast<const float *>(img->sep)[i];
+    }
     for (int i = tid; i < PTK_MAX_BKG_F; i += nt) sh.bkg[i] = img->bkg[i];
     if (tid < P.n_phases) sh.phase_ok[tid] = img->phase_ok[tid];
     if (tid == 0) { sh.cell_ok = img->cell_ok; sh.prof_ok = img->prof_ok; sh.go = img->go; }

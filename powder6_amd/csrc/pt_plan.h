@@ -23,6 +23,12 @@
 #define PTK_MAX_VARS 256
 #define PTK_MAX_BKG 24       // background coefficients or spline anchors
 #define PTK_MAX_BKG_F 48     // a trial's background slots: values, then spline tangents
+#define PTK_P_INST0 27       // the dataset's instrument slots (separated profiles)
+#define PTK_N_INST 18
+#define PTK_PH_NPROF 32      // profile slots of a phase block (from slot 8)
+#define PTK_PH_ATOM0 40      // first atom slot of a phase block
+#define PTK_PROF_LDS 48      // a trial's profile slots of a phase in LDS: the 32 model slots, then
+                             // (separated profiles) the family's eta / SL / HL slots from the instrument
 #define PTK_S_COUNT_K 12
 #define PTK_MATRIX_MAX_REFL 4096
 #define PTK_MAX_GROUPS 64          // 512-point groups per pattern (32768 points)
@@ -52,6 +58,13 @@ struct DevPhase {
     double po_dir[3];
     double ang[6];             // alpha, beta, gamma (deg), cos alpha, cos beta, cos gamma
     double ang_sin[3];
+    // separated (GSAS) profile: instrument function from the inst slots, sample broadening
+    // from the phase's 26 sample slots; profile_type above is then the instrument family
+    int sep;
+    int size_model, strain_model;  // PT_SIZE_*, PT_STRAIN_*
+    int n_sdir, sdir_off;          // Stephens guard directions [sdir_off, sdir_off + n_sdir) of P.sguard
+    int stephens;                  // 1: P.sbasis rows of this phase hold the Stephens basis
+    float size_axis[3], strain_axis[3];  // uniaxial axes (Miller indices)
 };
 
 struct DevPlan {
@@ -60,6 +73,8 @@ struct DevPlan {
     int chains, stride, steps_per_sweep;
     int uniform, radiation, doublet;
     int max_lines_lds;         // line-record chunk held in LDS
+    int any_sep;               // some phase has a separated profile
+    int has_stephens;          // some separated phase has Stephens strain (guard directions in sguard)
     int pad0;
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
@@ -93,7 +108,23 @@ struct DevPlan {
     const double *term_coeff;
     const double *vlo, *vhi, *vscale;
     const double *temp;        // temperature of chain c: max_temp^(c / (chains - 1))
+    const float *sbasis;       // Stephens basis per phase, [phase][key 15][coefficient 15]
+    const double *sguard;      // Stephens guard: per direction the 15 key coefficients
     double *state;
+};
+
+// a separated-profile phase of one trial, formed once in the setup (FP32)
+struct SepPhase {
+    float rb[9];               // reciprocal basis rows (Cartesian, 1/A): h . rows = reciprocal vector
+    float axs[3], axe[3];      // unit reciprocal vectors of the size and strain axes
+    float rot[9];              // ellipsoid: inverse Euler rotation (rows)
+    float e[15];               // Stephens: sum_k S_k basis[k][i]
+    float inv_l2[3];           // ellipsoid: 1 / max(0.001, L)^2
+    float inv_i2, inv_a2;      // uniaxial size: 1 / size^2, 1 / sizeAxial^2 (floored at 0.001)
+    float inv_size;            // isotropic size: 1 / max(0.001, size)
+    float si, sa;              // strain, strainAxial (>= 0)
+    float sm, em;              // sizeMix, strainMix clamped to [0, 1]
+    float pad;
 };
 
 // line record (one Kalpha line of one reflection), as held in LDS / global memory:

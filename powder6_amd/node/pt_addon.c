@@ -140,6 +140,7 @@ static const char *convert(napi_env env, napi_value o, holder *h) {
     p->radiation = (int32_t)num(env, o, "radiation", 0);
     p->n_bkg = (int32_t)num(env, o, "nBkg", 0);
     p->bkg_type = (int32_t)num(env, o, "bkgType", PT_BKG_CHEBYSHEV);
+    p->resolution_type = (int32_t)num(env, o, "resolutionType", 0);
     if (p->bkg_type == PT_BKG_SPLINE) {
         if (field_doubles(env, o, "bkgX", &d, h) != p->n_bkg) return "problem.bkgX must hold nBkg anchors";
         p->bkg_x = d;
@@ -174,6 +175,14 @@ static const char *convert(napi_env env, napi_value o, holder *h) {
         double *pd;
         if (field_doubles(env, ph, "poDir", &pd, h) == 3) { P->po_dir[0] = pd[0]; P->po_dir[1] = pd[1]; P->po_dir[2] = pd[2]; }
         else { P->po_dir[2] = 1; }
+        P->separated = (int32_t)num(env, ph, "separated", 0);
+        P->size_model = (int32_t)num(env, ph, "sizeModel", PT_SIZE_ISOTROPIC);
+        P->strain_model = (int32_t)num(env, ph, "strainModel", PT_STRAIN_ISOTROPIC);
+        double *ax;
+        if (field_doubles(env, ph, "sizeAxis", &ax, h) == 3) { P->size_axis[0] = ax[0]; P->size_axis[1] = ax[1]; P->size_axis[2] = ax[2]; }
+        else { P->size_axis[2] = 1; }
+        if (field_doubles(env, ph, "strainAxis", &ax, h) == 3) { P->strain_axis[0] = ax[0]; P->strain_axis[1] = ax[1]; P->strain_axis[2] = ax[2]; }
+        else { P->strain_axis[2] = 1; }
         P->atoms = h->atoms + ao;
         if (get_prop(env, ph, "atoms", &atoms)) {
             uint32_t na = 0;

@@ -27,14 +27,23 @@ function native() {
 }
 
 var SYSTEM = { triclinic: 0, monoclinic: 1, orthorhombic: 2, tetragonal: 3, trigonal: 4, hexagonal: 5, rhombohedral: 5, cubic: 6 };
-var PROFILE = { simple_pvoigt: 1, split_pvoigt: 2, tch_aniso: 3 };
+var PROFILE = { simple_pvoigt: 1, split_pvoigt: 2, tch_aniso: 3, gsas_cw: 4 };
+/* the dataset's instrument slots of separated profiles and their defaults
+ * (js/rietveld_core.js:413, 454-459), and the sample slots of a separated phase
+ * (:414-415 SAMPLE_PROFILE order) */
+var INST = ['U', 'V', 'W', 'X', 'Y', 'Z', 'SL', 'HL', 'eta', 'U_L', 'V_L', 'W_L', 'X_L', 'U_R', 'V_R', 'W_R', 'X_R', 'eta_split'];
+var INST_DEFAULT = [2, -2, 5, 1, 0, 0, 0, 0, 0.5, 2, -2, 5, 1, 2, -2, 5, 1, 0.5];
+var SAMPLE = ['size', 'sizeAxial', 'sizeB', 'sizeC', 'rotA', 'rotB', 'rotC', 'strain', 'strainAxial', 'sizeMix', 'strainMix'].concat(
+    ['400', '040', '004', '220', '202', '022', '310', '301', '130', '103', '031', '013', '211', '121', '112'].map(function (k) { return 'S' + k; }));
+var SAMPLE_DEFAULT = { size: 1, sizeAxial: 1, sizeB: 1, sizeC: 1, sizeMix: 1, strainMix: 1 };
+var SIZE_MODEL = { isotropic: 0, uniaxial: 1, ellipsoid: 2 }, STRAIN_MODEL = { isotropic: 0, uniaxial: 1, stephens: 2 };
 var SLOTS = {
     tch_aniso: ['U', 'V', 'W', 'X', 'Y', 'SL', 'HL'],
     simple_pvoigt: ['GU', 'GV', 'GW', 'GP', 'LX', 'eta'],
     split_pvoigt: ['GU_L', 'GV_L', 'GW_L', 'LX_L', 'GU_R', 'GV_R', 'GW_R', 'LX_R', 'eta_split']
 };
 var CELL = ['a', 'b', 'c', 'alpha', 'beta', 'gamma'];
-var P_PHASE0 = 27, PH_ATOM0 = 24, AT_STRIDE = 5;
+var P_INST0 = 27, P_PHASE0 = 45, PH_PROF0 = 8, PH_ATOM0 = 40, AT_STRIDE = 5;
 
 /* Parameter index of a reference value name in the C-ABI layout, or -1.  The 24
  * background slots hold bkg<k> (Chebyshev) or bga<j> (spline anchor values,
@@ -45,6 +54,7 @@ function paramIndex(layout, name, spline) {
     if (name === 'transp') return 2;
     var m = (spline ? /^bga(\d+)$/ : /^bkg(\d+)$/).exec(name);
     if (m) return +m[1] < 24 ? 3 + (+m[1]) : -1;
+    if (name.indexOf('inst.') === 0) { var ii = INST.indexOf(name.slice(5)); return ii >= 0 ? P_INST0 + ii : -1; }
     for (var q = 0; q < layout.length; q++) {
         var L = layout[q];
         if (name.indexOf(L.prefix) !== 0) continue;
@@ -54,7 +64,7 @@ function paramIndex(layout, name, spline) {
         if (rest === 'scale') return L.base + 6;
         if (rest === 'po') return L.base + 7;
         var s = L.slots.indexOf(rest);
-        if (s >= 0) return L.base + 8 + s;
+        if (s >= 0) return L.base + PH_PROF0 + s;
         var am = /^(.+)\.(x|y|z|occ|B)$/.exec(rest);
         if (am) {
             var j = L.labels.indexOf(am[1]);
@@ -65,7 +75,7 @@ function paramIndex(layout, name, spline) {
 }
 
 /* Flatten a reference PT context (one dataset, Rietveld phases, Chebyshev or spline background,
- * non-separated TCH / simple / split pseudo-Voigt) into the C-ABI problem. */
+ * TCH / simple / split pseudo-Voigt, separated GSAS profiles) into the C-ABI problem. */
 function fromContext(ctx) {
     if (ctx.flat) return ctx.flat;
     var model = ctx.model, problem = ctx.problem;
@@ -78,10 +88,12 @@ function fromContext(ctx) {
         if (!D.has[q]) throw new Error('MI355X PT backend needs every phase in the dataset.');
         if (P.method !== 'rietveld') throw new Error('MI355X PT backend evaluates Rietveld phases only; using the CPU path.');
         var type = P.def.profileType;
-        if (P.def.profileType === 'gsas_cw' || P.def.separatedProfile) throw new Error('MI355X PT backend has no separated (GSAS) profile yet; using the CPU path.');
+        // rietveld_core.js:460 usesSeparatedProfile, :416-421 broadeningConfig
+        var sep = type === 'gsas_cw' || P.def.separatedProfile === true, b = P.def.broadening || {};
+        var axis = function (a) { return Array.isArray(a) && a.length === 3 && a.every(isFinite) && a.some(function (v) { return v !== 0; }) ? a.map(Number) : [0, 0, 1]; };
         if (!PROFILE[type]) throw new Error('unknown profile type ' + type);
         var atoms = P.def.atoms || [];
-        layout.push({ prefix: P.prefix, base: base, slots: SLOTS[type], labels: atoms.map(function (a) { return a.label; }) });
+        layout.push({ prefix: P.prefix, base: base, slots: sep ? SAMPLE : SLOTS[type], sep: sep, labels: atoms.map(function (a) { return a.label; }) });
         var opR = [], opT = [];
         P.def.ops.forEach(function (o) { for (var i = 0; i < 9; i++) opR.push(o.r[i]); for (var k = 0; k < 3; k++) opT.push(o.t[k]); });
         var anom = (D.anom && D.anom[q]) || [];
@@ -89,6 +101,8 @@ function fromContext(ctx) {
             system: SYSTEM[String(P.def.system || 'triclinic').toLowerCase()] || 0,
             uniqueAxis: { a: 0, b: 1, c: 2 }[P.def.uniqueAxis || 'b'],
             profileType: PROFILE[type], opR: Int32Array.from(opR), opT: Float64Array.from(opT),
+            separated: sep ? 1 : 0, sizeModel: SIZE_MODEL[b.size] || 0, strainModel: STRAIN_MODEL[b.strain] || 0,
+            sizeAxis: axis(b.sizeAxis), strainAxis: axis(b.strainAxis),
             poDir: P.def.poDir || [0, 0, 1],
             atoms: atoms.map(function (at, j) {
                 var site = at.site, basis = [0, 0, 0, 0, 0, 0, 0, 0, 0], mask = 0;
@@ -101,6 +115,12 @@ function fromContext(ctx) {
         base += PH_ATOM0 + AT_STRIDE * atoms.length;
     });
     var nParams = base, paramBase = new Float64Array(nParams);
+    // defaults where the reference reads a missing or non-finite value as one
+    // (rietveld_core.js:700-701 instrument values, profile.js:389 value(n, d) sample values)
+    for (var ii = 0; ii < INST.length; ii++) paramBase[P_INST0 + ii] = INST_DEFAULT[ii];
+    layout.forEach(function (L) {
+        if (L.sep) SAMPLE.forEach(function (n, j) { if (SAMPLE_DEFAULT[n]) paramBase[L.base + PH_PROF0 + j] = SAMPLE_DEFAULT[n]; });
+    });
     Object.keys(values).forEach(function (name) {
         var k = paramIndex(layout, name, spline);
         if (k >= 0 && Number.isFinite(values[name])) paramBase[k] = values[name];
@@ -124,6 +144,7 @@ function fromContext(ctx) {
         lambda1: def.lambda1, lambda2: def.lambda2 || 0, ratio: def.ratio || 0, polK: def.polK, minFwhm: def.minFwhm,
         radiation: def.radiation === 'neutron' ? 1 : 0, nBkg: spline ? D.bkgX.length : D.T.length,
         bkgType: spline ? 1 : 0, bkgX: Float64Array.from(spline ? D.bkgX : []), phases: phases, paramBase: paramBase,
+        resolutionType: PROFILE[def.resolutionType] || 0,
         x0: Float64Array.from(ctx.x), lo: Float64Array.from(ctx.info.map(function (i) { return i.lo; })),
         hi: Float64Array.from(ctx.info.map(function (i) { return i.hi; })), scale: Float64Array.from(ctx.setup.scales),
         termParam: Int32Array.from(termParam), termVar: Int32Array.from(termVar), termCoeff: Float64Array.from(termCoeff),

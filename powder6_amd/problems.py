@@ -32,12 +32,27 @@ from . import symmetry as S
 from .scatter_wk1995 import WK1995
 
 # model-parameter layout of include/powder6_pt.h
-P_ZERO, P_DISP, P_TRANSP, P_BKG0, MAX_BKG, P_PHASE0 = 0, 1, 2, 3, 24, 27
+P_ZERO, P_DISP, P_TRANSP, P_BKG0, MAX_BKG, P_INST0 = 0, 1, 2, 3, 24, 27
 BKG_CHEBYSHEV, BKG_SPLINE = 0, 1
+# the dataset's instrument function of a separated profile, `inst.<name>`, with the
+# defaults of js/rietveld_core.js:413, 454-459
+INST_NAMES = ['U', 'V', 'W', 'X', 'Y', 'Z', 'SL', 'HL', 'eta', 'U_L', 'V_L', 'W_L', 'X_L', 'U_R', 'V_R', 'W_R', 'X_R',
+              'eta_split']
+INST_DEFAULTS = [2, -2, 5, 1, 0, 0, 0, 0, 0.5, 2, -2, 5, 1, 2, -2, 5, 1, 0.5]
+P_PHASE0 = P_INST0 + len(INST_NAMES)
+# sample slots of a separated profile (js/rietveld_core.js:414-415 SAMPLE_PROFILE order)
+QUARTIC = ['400', '040', '004', '220', '202', '022', '310', '301', '130', '103', '031', '013', '211', '121', '112']
+SAMPLE_SLOTS = ['size', 'sizeAxial', 'sizeB', 'sizeC', 'rotA', 'rotB', 'rotC', 'strain', 'strainAxial', 'sizeMix',
+                'strainMix'] + ['S' + q for q in QUARTIC]
+SAMPLE_DEFAULTS = {'size': 1, 'sizeAxial': 1, 'sizeB': 1, 'sizeC': 1, 'rotA': 0, 'rotB': 0, 'rotC': 0, 'strain': 0,
+                   'strainAxial': 0, 'sizeMix': 1, 'strainMix': 1}
+SIZE_MODELS = {'isotropic': 0, 'uniaxial': 1, 'ellipsoid': 2}
+STRAIN_MODELS = {'isotropic': 0, 'uniaxial': 1, 'stephens': 2}
 PH = {'a': 0, 'b': 1, 'c': 2, 'alpha': 3, 'beta': 4, 'gamma': 5, 'scale': 6, 'po': 7}
-PH_PROF0, PH_ATOM0, AT_STRIDE = 8, 24, 5
+PH_PROF0, PH_NPROF, AT_STRIDE = 8, 32, 5
+PH_ATOM0 = PH_PROF0 + PH_NPROF
 AT = {'x': 0, 'y': 1, 'z': 2, 'occ': 3, 'B': 4}
-PROFILE = {'simple_pvoigt': 1, 'split_pvoigt': 2, 'tch_aniso': 3}
+PROFILE = {'simple_pvoigt': 1, 'split_pvoigt': 2, 'tch_aniso': 3, 'gsas_cw': 4}
 PROFILE_SLOTS = {
     'tch_aniso': ['U', 'V', 'W', 'X', 'Y', 'SL', 'HL'],
     'simple_pvoigt': ['GU', 'GV', 'GW', 'GP', 'LX', 'eta'],
@@ -65,8 +80,21 @@ class Atom:
 
 
 class Phase:
-    def __init__(self, key, group, cell, atoms, profile='tch_aniso', po_dir=(0, 0, 1), unique_axis=1, ops=None):
+    """A Rietveld phase.  profile 'gsas_cw' (or separated=True with another family) is a
+    separated profile: the dataset's instrument function plus sample size / strain
+    broadening (js/rietveld_core.js:460), with `broadening` as the reference's
+    phase.broadening: {'size': 'isotropic'|'uniaxial'|'ellipsoid', 'strain':
+    'isotropic'|'uniaxial'|'stephens', 'sizeAxis': [h, k, l], 'strainAxis': [h, k, l]}."""
+
+    def __init__(self, key, group, cell, atoms, profile='tch_aniso', po_dir=(0, 0, 1), unique_axis=1, ops=None,
+                 separated=False, broadening=None):
         self.key = key
+        self.separated = bool(separated) or profile == 'gsas_cw'
+        b = dict(broadening or {})
+        self.size_model = b.get('size', 'isotropic')
+        self.strain_model = b.get('strain', 'isotropic')
+        self.size_axis = list(b.get('sizeAxis', [0, 0, 1]))
+        self.strain_axis = list(b.get('strainAxis', [0, 0, 1]))
         if ops is None:
             ops = S.group_ops(group)
         self.op_r, self.op_t = ops
@@ -82,7 +110,7 @@ class Problem:
     """One dataset, its phases, model parameters p = base + M x, variables, PT settings."""
 
     def __init__(self, tth, phases, lambda1=CU_KA1, lambda2=0.0, ratio=0.0, pol_k=1.0, min_fwhm=None,
-                 n_bkg=6, radiation=0, bkg_x=None):
+                 n_bkg=6, radiation=0, bkg_x=None, resolution_type=None):
         self.tth = np.ascontiguousarray(tth, dtype=np.float64)
         self.y = np.zeros_like(self.tth)
         self.w = np.ones_like(self.tth)
@@ -99,6 +127,8 @@ class Problem:
             if len(xs) > MAX_BKG:
                 raise ValueError('at most %d spline anchors' % MAX_BKG)
             self.bkg_type, self.bkg_x, self.n_bkg = BKG_SPLINE, xs, len(xs)
+        # instrument family of the separated phases (the dataset's resolutionType), or None
+        self.resolution_type = resolution_type
         self.phases = phases
         self.names = []
         self.index = {}
@@ -112,11 +142,12 @@ class Problem:
     def _layout(self):
         stem = 'bga' if self.bkg_type == BKG_SPLINE else 'bkg'
         names = ['zero', 'disp', 'transp'] + [stem + '%d' % k for k in range(MAX_BKG)]
+        names += ['inst.' + n for n in INST_NAMES]
         for ph in self.phases:
             pre = ph.key + '.'
             names += [pre + n for n in ('a', 'b', 'c', 'alpha', 'beta', 'gamma', 'scale', 'po')]
-            slots = PROFILE_SLOTS[ph.profile]
-            names += [pre + (slots[i] if i < len(slots) else '_prof%d' % i) for i in range(16)]
+            slots = SAMPLE_SLOTS if ph.separated else PROFILE_SLOTS[ph.profile]
+            names += [pre + (slots[i] if i < len(slots) else '_prof%d' % i) for i in range(PH_NPROF)]
             for at in ph.atoms:
                 names += [pre + at.label + '.' + n for n in ('x', 'y', 'z', 'occ', 'B')]
         self.names = names
@@ -182,8 +213,12 @@ class Problem:
                 self.set(pre + n, v)
             self.set(pre + 'scale', 1.0)
             self.set(pre + 'po', 1.0)
-            for n, v in PROFILE_DEFAULTS[ph.profile].items():
+            defaults = SAMPLE_DEFAULTS if ph.separated else PROFILE_DEFAULTS[ph.profile]
+            for n, v in defaults.items():
                 self.set(pre + n, v)
+            if ph.separated:
+                for n, v in zip(INST_NAMES, INST_DEFAULTS):
+                    self.set('inst.' + n, v)
             for at in ph.atoms:
                 pos = at.site['pos']
                 for i, n in enumerate('xyz'):
@@ -248,6 +283,8 @@ def classify(name):
         return 'bkg', None
     if name.startswith('bga'):
         return 'bga', None
+    if name.startswith('inst.'):
+        return 'instrument', name[5:]
     parts = name.split('.')
     if len(parts) == 2:
         if parts[1] == 'scale':
@@ -277,9 +314,13 @@ def step_cap(name, v):
         return 0.01
     if kind == 'po':
         return 0.25
+    if kind == 'instrument':
+        return 0.01 if comp in ('SL', 'HL') else 0.5 * max(abs(v), 1)
     if kind == 'profile':
         if comp in ('sizeMix', 'strainMix'):
             return 0.2
+        if comp.startswith('size'):
+            return 0.2 * max(abs(v), 0.001)
         if comp.startswith('eta'):
             return 0.25
         if comp in ('SL', 'HL'):
@@ -724,6 +765,82 @@ def uneven_batch(simulate=None, seed=20260303, count=48, chains=32):
 
 
 TRICLINIC_FF = ((2.31, 1.02, 1.5886, .865), (20.8439, 10.2075, .5687, 51.6512), .2156)
+
+
+SEPARATED_INST = {'U': 8.0, 'V': -4.0, 'W': 4.0, 'X': 1.5, 'Y': 0.5, 'Z': 0.2, 'SL': 0.02, 'HL': 0.01, 'eta': 0.5,
+                  'U_L': 8.0, 'V_L': -4.0, 'W_L': 4.0, 'X_L': 1.5, 'U_R': 10.0, 'V_R': -5.0, 'W_R': 5.0, 'X_R': 2.0,
+                  'eta_split': 0.6}
+SEPARATED_SAMPLE = {'size': 0.12, 'sizeAxial': 0.06, 'sizeB': 0.09, 'sizeC': 0.15, 'rotA': 10.0, 'rotB': -20.0,
+                    'rotC': 30.0, 'strain': 900.0, 'strainAxial': 1500.0, 'sizeMix': 0.8, 'strainMix': 0.6,
+                    'S400': 190.0, 'S040': 1190.0, 'S004': 430.0, 'S220': 100.0, 'S202': 80.0, 'S022': 150.0}
+
+
+def separated_probe(size='isotropic', strain='isotropic', family='gsas_cw', simulate=None, seed=20260808, chains=16,
+                    size_axis=(0, 0, 1), strain_axis=(1, 0, 0)):
+    """The Pnma phase of pbso4_pnma with a separated (GSAS) profile: the dataset's
+    instrument function of `family` (gsas_cw = TCH, simple_pvoigt, split_pvoigt; inst
+    U 8, V -4, W 4, X 1.5, Y 0.5, Z 0.2, SL .02, HL .01, split flanks 8/-4/4/1.5 and
+    10/-5/5/2) plus sample broadening: size 0.12 um (uniaxial 0.06 along size_axis;
+    ellipsoid 0.12 x 0.09 x 0.15 rotated 10, -20, 30 deg), strain 900 (uniaxial 1500 along
+    strain_axis; Stephens S400 190, S040 1190, S004 430, S220 100, S202 80, S022 150),
+    mixes 0.8 / 0.6.  2theta 10-100 step 0.03 with the Cu Ka doublet, 3-term Chebyshev.
+    Refined: cell, zero, scale, inst U, W, X, the sample parameters of the models,
+    4 coordinates, 2 B, bkg0."""
+    rng = np.random.default_rng(seed)
+    r, t = S.group_ops(62)
+    cell = [8.48, 5.40, 6.96, 90, 90, 90]
+    atoms = []
+    for (lab, el, xyz, B) in PBSO4_SITES:
+        site = S.site_symmetry(r, t, xyz, cell)
+        pos = list(site['pos'])
+        for f in site['free']:
+            pos[f] += rng.uniform(-0.03, 0.03)
+        site['pos'] = pos
+        atoms.append(Atom(lab, el, site, B=float(rng.uniform(0.5, 2.0))))
+    ph = Phase('p1', 62, cell, atoms, 'gsas_cw' if family == 'gsas_cw' else family, separated=True,
+               broadening={'size': size, 'strain': strain, 'sizeAxis': list(size_axis), 'strainAxis': list(strain_axis)})
+    pb = Problem(_grid(10.0, 100.0, 0.03), [ph], lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0, n_bkg=3,
+                 resolution_type=family)
+    pb.default_values()
+    for n, v in SEPARATED_INST.items():
+        pb.set('inst.' + n, v)
+    for n, v in SEPARATED_SAMPLE.items():
+        pb.set('p1.' + n, v)
+    pb.set('zero', 0.01)
+    _cheb(pb, [150.0, -25.0, 8.0])
+    names = ['p1.a', 'p1.b', 'p1.c', 'zero', 'p1.scale']
+    names += ['inst.U_L', 'inst.W_R', 'inst.X_L'] if family == 'split_pvoigt' else ['inst.U', 'inst.W', 'inst.X']
+    names += ['p1.size', 'p1.sizeMix', 'p1.strainMix']
+    names += {'uniaxial': ['p1.sizeAxial'], 'ellipsoid': ['p1.sizeB', 'p1.sizeC', 'p1.rotC']}.get(size, [])
+    names += {'isotropic': ['p1.strain'], 'uniaxial': ['p1.strain', 'p1.strainAxial'],
+              'stephens': ['p1.S400', 'p1.S040', 'p1.S220']}[strain]
+    names += ['p1.%s.%s' % (at.label, 'xyz'[f]) for at in atoms[:2] for f in at.site['free']][:4]
+    names += ['p1.%s.B' % at.label for at in atoms[:2]] + ['bkg0']
+    for n in names:
+        kind, comp = classify(n)
+        v = pb.base[pb.index[n]]
+        if kind == 'cell':
+            pb.refine(n, 0.97 * v, 1.03 * v)
+        elif n == 'zero':
+            pb.refine(n, -1, 1)
+        elif n == 'p1.scale' or comp in ('size', 'sizeAxial', 'sizeB', 'sizeC'):
+            pb.refine(n, 0.002 if comp else 0.0, math.inf)
+        elif comp in ('sizeMix', 'strainMix'):
+            pb.refine(n, 0, 1)
+        elif comp in ('strain', 'strainAxial'):
+            pb.refine(n, 0, math.inf)
+        elif kind == 'atom' and comp == 'B':
+            pb.refine(n, -2, 30)
+        elif kind == 'atom':
+            pb.refine(n, v - 0.1, v + 0.1)
+        else:
+            pb.refine(n)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 20, 0, 100.0, 777
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed + 7)
+        pb.x0 = list(_perturb(pb, x_true, seed + 11, frac=0.15))
+    return pb
 
 
 def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, hl=0.0, po=0.87, n_points=160):

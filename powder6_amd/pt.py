@@ -38,7 +38,8 @@ class PtAtom(ctypes.Structure):
 class PtPhase(ctypes.Structure):
     _fields_ = [('system', c_int32), ('unique_axis', c_int32), ('profile_type', c_int32), ('n_ops', c_int32),
                 ('op_r', P_I), ('op_t', P_D), ('po_dir', c_double * 3), ('n_atoms', c_int32),
-                ('atoms', ctypes.POINTER(PtAtom))]
+                ('atoms', ctypes.POINTER(PtAtom)), ('separated', c_int32), ('size_model', c_int32),
+                ('strain_model', c_int32), ('size_axis', c_double * 3), ('strain_axis', c_double * 3)]
 
 
 class PtProblem(ctypes.Structure):
@@ -50,7 +51,8 @@ class PtProblem(ctypes.Structure):
                 ('n_vars', c_int32), ('x0', P_D), ('lo', P_D), ('hi', P_D), ('scale', P_D),
                 ('n_terms', c_int32), ('term_param', P_I), ('term_var', P_I), ('term_coeff', P_D),
                 ('chains', c_int32), ('sweeps', c_int32), ('steps_per_sweep', c_int32),
-                ('max_temp', c_double), ('seed', c_uint32), ('bkg_type', c_int32), ('bkg_x', P_D)]
+                ('max_temp', c_double), ('seed', c_uint32), ('bkg_type', c_int32), ('bkg_x', P_D),
+                ('resolution_type', c_int32)]
 
 
 class PtPlanInfo(ctypes.Structure):
@@ -111,6 +113,12 @@ class CProblem:
                 P.po_dir[x] = ph.po_dir[x]
             P.n_atoms = len(ph.atoms)
             P.atoms = ctypes.cast(atoms, ctypes.POINTER(PtAtom))
+            P.separated = 1 if getattr(ph, 'separated', False) else 0
+            P.size_model = PR.SIZE_MODELS[getattr(ph, 'size_model', 'isotropic')]
+            P.strain_model = PR.STRAIN_MODELS[getattr(ph, 'strain_model', 'isotropic')]
+            for x in range(3):
+                P.size_axis[x] = float(getattr(ph, 'size_axis', [0, 0, 1])[x])
+                P.strain_axis[x] = float(getattr(ph, 'strain_axis', [0, 0, 1])[x])
         keep.append(phases)
         s = self.struct = PtProblem()
         s.n_points = len(pb.tth)
@@ -132,6 +140,8 @@ class CProblem:
         s.bkg_type = getattr(pb, 'bkg_type', 0)
         bx = getattr(pb, 'bkg_x', None)
         s.bkg_x = d(np.asarray(bx if bx is not None else [0.0], dtype=np.float64))
+        rt = getattr(pb, 'resolution_type', None)
+        s.resolution_type = PR.PROFILE[rt] if rt else 0
 
     def ref(self):
         return ctypes.byref(self.struct)

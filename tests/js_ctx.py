@@ -31,13 +31,17 @@ def reference_ctx(pb, initial_cost=None):
                           'site': {'r0': list(map(float, at.site['r0'])), 'free': list(at.site['free']),
                                    'basis': basis, 'mult': at.site['mult']}})
         ops = [{'r': [int(v) for v in r], 't': [float(v) for v in t]} for r, t in zip(ph.op_r, ph.op_t)]
-        phases.append({'def': {'key': ph.key, 'system': ph.system, 'uniqueAxis': 'abc'[ph.unique_axis],
-                               'profileType': ph.profile, 'ops': ops, 'poDir': list(ph.po_dir), 'atoms': atoms},
-                       'method': 'rietveld', 'prefix': ph.key + '.'})
+        pdef = {'key': ph.key, 'system': ph.system, 'uniqueAxis': 'abc'[ph.unique_axis],
+                'profileType': ph.profile, 'ops': ops, 'poDir': list(ph.po_dir), 'atoms': atoms}
+        if getattr(ph, 'separated', False):
+            pdef['separatedProfile'] = True
+            pdef['broadening'] = {'size': ph.size_model, 'strain': ph.strain_model,
+                                  'sizeAxis': list(ph.size_axis), 'strainAxis': list(ph.strain_axis)}
+        phases.append({'def': pdef, 'method': 'rietveld', 'prefix': ph.key + '.'})
     anom = [[[at.fp, at.fpp] for at in ph.atoms] for ph in pb.phases]
     ds = {'def': {'tth': pb.tth.tolist(), 'y': pb.y.tolist(), 'w': pb.w.tolist(), 'lambda1': pb.lambda1,
                   'lambda2': pb.lambda2, 'ratio': pb.ratio, 'polK': pb.pol_k, 'minFwhm': pb.min_fwhm,
-                  'radiation': 'neutron' if pb.radiation else 'xray'},
+                  'radiation': 'neutron' if pb.radiation else 'xray', 'resolutionType': pb.resolution_type},
           'has': [True] * len(pb.phases), 'anom': anom}
     if getattr(pb, 'bkg_type', PR.BKG_CHEBYSHEV) == PR.BKG_SPLINE:
         ds.update(spline=True, T=[], bkgX=[float(v) for v in pb.bkg_x])

@@ -74,6 +74,15 @@ def lib():
         L.or_temper_parallel.argtypes = [ctypes.c_void_p, P_D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_uint32, ctypes.c_int, P_D]
         L.or_prepare_voigt.argtypes = [ctypes.c_int, P_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        L.or_stephens_basis.argtypes = [ctypes.c_int, P_I, P_I, P_D]
+        L.or_axis_angle.restype = ctypes.c_double
+        L.or_axis_angle.argtypes = [P_D, P_D, P_D]
+        L.or_stephens_sum.restype = ctypes.c_double
+        L.or_stephens_sum.argtypes = [P_D, P_D, ctypes.c_int, P_I, P_D]
+        L.or_gsas_widths.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, P_D, P_D]
+        L.or_prepare_voigt_sep.argtypes = [ctypes.c_int, P_D, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
         L.or_eval_voigt.restype = ctypes.c_double
         L.or_eval_voigt.argtypes = [ctypes.c_double, ctypes.c_void_p]
         L.or_asym_area.restype = ctypes.c_double
@@ -96,7 +105,7 @@ class OrVoigt(ctypes.Structure):
 
 
 def prepare_voigt(ptype, prof, min_fwhm, tth_peak, x0):
-    p = np.zeros(16)
+    p = np.zeros(32)
     p[:len(prof)] = prof
     v = OrVoigt()
     lib().or_prepare_voigt(ptype, p.ctypes.data_as(P_D), min_fwhm, tth_peak, x0, ctypes.byref(v))
@@ -207,6 +216,48 @@ def mlp_loss_grad(params, n_in, h, x, t, mask=None):
     loss = lib().om_loss_grad(p.ctypes.data_as(P_D), n_in, h, x.ctypes.data_as(P_D), t.ctypes.data_as(P_D),
                               None if m is None else m.ctypes.data_as(P_D), x.shape[0], g.ctypes.data_as(P_D))
     return loss, g
+
+
+class OrSep(ctypes.Structure):
+    _fields_ = [('inst', ctypes.c_double * 18), ('sample', P_D), ('size_model', ctypes.c_int), ('strain_model', ctypes.c_int),
+                ('size_axis', ctypes.c_double * 3), ('strain_axis', ctypes.c_double * 3), ('cell', ctypes.c_double * 6),
+                ('h', ctypes.c_double * 3), ('d', ctypes.c_double), ('n_basis', ctypes.c_int), ('basis_key', P_I),
+                ('basis_coeff', P_D)]
+
+
+def stephens_basis(op_r):
+    """The oracle's Stephens basis of a setting: [(key index, 15 coefficients)]."""
+    op_r = np.ascontiguousarray(np.asarray(op_r).reshape(-1), dtype=np.int32)
+    keys = np.zeros(15, dtype=np.int32)
+    coeff = np.zeros(15 * 15)
+    n = lib().or_stephens_basis(len(op_r) // 9, op_r.ctypes.data_as(P_I), keys.ctypes.data_as(P_I), coeff.ctypes.data_as(P_D))
+    return [(int(keys[t]), coeff[15 * t:15 * t + 15].copy()) for t in range(n)]
+
+
+def gsas_widths(ptype, cell, hkl, d, sample, inst=None, size_model=0, strain_model=0, size_axis=(0, 0, 1),
+                strain_axis=(0, 0, 1), basis=None, lam=1.540598, min_fwhm=0.01, tth=40.0, side=1):
+    """(Gaussian, Lorentzian) FWHM of a separated profile at one reflection."""
+    sp = OrSep()
+    inst = [2, -2, 5, 1, 0, 0, 0, 0, 0.5, 2, -2, 5, 1, 2, -2, 5, 1, 0.5] if inst is None else inst
+    for i in range(18):
+        sp.inst[i] = inst[i]
+    sample = np.ascontiguousarray(sample, dtype=np.float64)
+    sp.sample = sample.ctypes.data_as(P_D)
+    sp.size_model, sp.strain_model = size_model, strain_model
+    for i in range(3):
+        sp.size_axis[i], sp.strain_axis[i], sp.h[i] = size_axis[i], strain_axis[i], hkl[i]
+    for i in range(6):
+        sp.cell[i] = cell[i]
+    sp.d = d
+    keys = np.zeros(15, dtype=np.int32)
+    coeff = np.zeros(225)
+    for t, (k, c) in enumerate(basis or []):
+        keys[t], coeff[15 * t:15 * t + 15] = k, c
+    sp.n_basis = len(basis or [])
+    sp.basis_key, sp.basis_coeff = keys.ctypes.data_as(P_I), coeff.ctypes.data_as(P_D)
+    g, l = ctypes.c_double(), ctypes.c_double()
+    lib().or_gsas_widths(ptype, ctypes.byref(sp), lam, min_fwhm, tth, side, ctypes.byref(g), ctypes.byref(l))
+    return g.value, l.value
 
 
 def spline_values(tth, xs, ys):

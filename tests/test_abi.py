@@ -37,7 +37,7 @@ def test_library_exports(name):
 
 
 def test_abi_version():
-    assert pt.lib().pt_abi_version() == 2
+    assert pt.lib().pt_abi_version() == 3
 
 
 def test_phase_base_layout():
@@ -45,8 +45,10 @@ def test_phase_base_layout():
     pb.y = np.ones_like(pb.tth)
     cp = pt.CProblem(pb)
     lib = pt.lib()
-    assert lib.pt_phase_base(cp.ref(), 0) == 27
-    assert lib.pt_phase_base(cp.ref(), 1) == len(pb.base) == 27 + 24 + 5 * len(pb.phases[0].atoms)
+    # zero, disp, transp, 24 background slots, 18 instrument slots; a phase block is 8
+    # cell / scale / po slots, 32 profile slots and 5 per atom
+    assert lib.pt_phase_base(cp.ref(), 0) == 45
+    assert lib.pt_phase_base(cp.ref(), 1) == len(pb.base) == 45 + 40 + 5 * len(pb.phases[0].atoms)
 
 
 def test_compile_without_device_reports_counts():

@@ -59,6 +59,22 @@ def test_compile_spline_background_context(tmp_path):
     assert c['nv'] == info['nv'] == 16 and c['pointCount'] == info['point_count'] and c['paramCount'] == info['n_params']
 
 
+@needs_node
+@pytest.mark.parametrize('family', ['gsas_cw', 'split_pvoigt'])
+def test_compile_separated_profile_context(family, tmp_path):
+    """A phase with a separated (GSAS) profile and the dataset's instrument function
+    (js/rietveld_core.js:460, 694-708) compiles with every sample and inst.* name mapped."""
+    pb = PR.separated_probe('ellipsoid', 'stephens', family)
+    pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'compile')
+    assert 'compile' in out, r.stderr
+    with pt.Plan(pb, device=False) as pl:
+        info = pl.info
+    c = out['compile']
+    assert c['nv'] == info['nv'] and c['paramCount'] == info['n_params'] and c['reflectionCount'] == info['reflection_count']
+
+
 def _has_gpu():
     try:
         import torch
@@ -101,6 +117,20 @@ def test_node_costs_match_oracle(profile, tmp_path):
 @pytest.mark.gpu
 def test_node_costs_spline_background_match_oracle(tmp_path):
     pb = PR.nacl_spline(simulate=O.oracle_simulate)
+    m = O.Model(pb)
+    probes = _probes(pb)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])
+    out, r = _node(path, 'costs')
+    assert 'costs' in out, (out, r.stderr)
+    for c, g in enumerate(out['costs']):
+        cpu = m.cost(probes[c % len(probes)])
+        assert abs(g - cpu) / max(1.0, cpu) <= 1e-5
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_costs_separated_profile_match_oracle(tmp_path):
+    pb = PR.separated_probe('uniaxial', 'stephens', 'gsas_cw', simulate=O.oracle_simulate)
     m = O.Model(pb)
     probes = _probes(pb)
     path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])

@@ -390,13 +390,14 @@ __device__ __forceinline__ double variance_min(double u, double v, double w, dou
 
 // the profile-feasibility rule (rietveld_core.js:1413-1496): every Gaussian polynomial
 // of every phase non-negative over the fitted tan(theta) range
+template <bool SEP = true>
 __device__ inline bool profile_valid(const DevPlan &P, const double *p) {
     for (int q = 0; q < P.n_phases; ++q) {
         const double *pr = p + P.phase[q].param_base + 8;
         int type = P.phase[q].profile_type;
         for (int s = 0; s < (type == 2 ? 2 : 1); ++s) {
             double u, v, w;
-            if (P.phase[q].sep) {
+            if (SEP && P.sp->ph[q].sep) {
                 // the dataset's instrument polynomial (U, V, W or a split flank's), a value
                 // that is not finite read as its default 2, -2, 5 (rietveld_core.js:1483-1488)
                 const double *in = p + PTK_P_INST0 + (type == 2 ? (s ? 13 : 9) : 0);

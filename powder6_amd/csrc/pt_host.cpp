@@ -657,6 +657,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     std::vector<ptk::DevAtom> atoms;
     std::vector<float> sbasis((size_t)PTK_MAX_PHASES * 225, 0.0f);
     std::vector<double> sguard;
+    ptk::DevSepPlan sep_plan{};
     ptk::DevPlan &P = pl->P;
     P.any_sep = 0; P.has_stephens = 0;
     for (int q = 0; q < pb->n_phases; ++q) {
@@ -699,19 +700,20 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         dp.system = ph.system;
         dp.unique_axis = ph.unique_axis;
         // a separated profile takes the dataset's instrument family (rietveld_core.js:700-702)
-        dp.sep = (ph.separated || ph.profile_type == PT_PROFILE_GSAS_CW) ? 1 : 0;
-        dp.profile_type = dp.sep && pb->resolution_type ? pb->resolution_type : ph.profile_type;
+        ptk::DevSepPhase &sd = sep_plan.ph[q];
+        sd.sep = (ph.separated || ph.profile_type == PT_PROFILE_GSAS_CW) ? 1 : 0;
+        dp.profile_type = sd.sep && pb->resolution_type ? pb->resolution_type : ph.profile_type;
         if (dp.profile_type == PT_PROFILE_GSAS_CW) dp.profile_type = PT_PROFILE_TCH;
-        dp.size_model = dp.sep ? ph.size_model : 0;
-        dp.strain_model = dp.sep ? ph.strain_model : 0;
+        sd.size_model = sd.sep ? ph.size_model : 0;
+        sd.strain_model = sd.sep ? ph.strain_model : 0;
         for (int w = 0; w < 2; ++w) {
             // broadeningConfig axis(): three finite numbers, one non-zero, else 0 0 1
             const double *a = w ? ph.strain_axis : ph.size_axis;
             bool ok = std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]) && (a[0] != 0 || a[1] != 0 || a[2] != 0);
-            float *o = w ? dp.strain_axis : dp.size_axis;
+            float *o = w ? sd.strain_axis : sd.size_axis;
             for (int i = 0; i < 3; ++i) o[i] = ok ? (float)a[i] : (i == 2 ? 1.0f : 0.0f);
         }
-        dp.stephens = 0; dp.n_sdir = 0; dp.sdir_off = 0;
+        sd.stephens = 0; sd.n_sdir = 0; sd.sdir_off = 0;
         dp.n_reps = (int)reps.size();
         dp.rep_off = (int)(rep_r.size() / 9);
         dp.n_atoms = ph.n_atoms;
@@ -752,17 +754,17 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
             eq_off.push_back((int)(eq.size() / 3));
             rows.push_back(r.h); rows.push_back(r.k); rows.push_back(r.l); rows.push_back(r.m);
         }
-        if (dp.sep && dp.strain_model == PT_STRAIN_STEPHENS) {
+        if (sd.sep && sd.strain_model == PT_STRAIN_STEPHENS) {
             std::vector<float> basis;
             if (stephens_basis(ph, basis)) {
-                dp.stephens = 1;
+                sd.stephens = 1;
                 std::copy(basis.begin(), basis.end(), sbasis.begin() + (size_t)q * 225);
-                dp.sdir_off = (int)(sguard.size() / 15);
-                stephens_guard(basis, q, rows, sguard, dp.n_sdir);
+                sd.sdir_off = (int)(sguard.size() / 15);
+                stephens_guard(basis, q, rows, sguard, sd.n_sdir);
             }
         }
-        if (dp.sep) P.any_sep = 1;
-        if (dp.stephens) P.has_stephens = 1;
+        if (sd.sep) P.any_sep = 1;
+        if (sd.stephens) P.has_stephens = 1;
         pl->phase_hklm.push_back(std::move(rows));
     }
     pl->nr = (int)hklm.size();
@@ -876,7 +878,12 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         return rc;
     }
     P.bkg_spline = pb->bkg_type == PT_BKG_SPLINE ? 1 : 0; P.bseg = d_bseg; P.bcoef = d_bcoef; P.bknots = d_bknots;
-    P.sbasis = d_sbasis; P.sguard = d_sguard;
+    {
+        sep_plan.sbasis = d_sbasis; sep_plan.sguard = d_sguard;
+        ptk::DevSepPlan *d_sp = nullptr;
+        if ((rc = upload(pl, &d_sp, std::vector<ptk::DevSepPlan>{sep_plan}))) { pt_plan_destroy(pl); return rc; }
+        P.sp = d_sp;
+    }
     P.tth = d_tth; P.tff = d_tff; P.zcheb = d_zcheb; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
     P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
     P.term_var = d_term_var; P.term_coeff = d_coeff; P.vlo = d_lo; P.vhi = d_hi; P.vscale = d_sc; P.state = d_state;

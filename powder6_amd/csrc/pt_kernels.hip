@@ -104,6 +104,7 @@ __device__ __forceinline__ double finite_or(double v, double d) { return isfinit
 // -rotA) and inverse squared lengths, and the Stephens form e = sum_k S_k basis_k.
 // FP64 here, stored FP32.  One thread per phase.
 __device__ inline void sep_phase_setup(const DevPlan &P, const DevPhase &ph, const double *pp, int q, SepPhase &o) {
+    const DevSepPhase &sd = P.sp->ph[q];
     const double *S = pp + 8;
     double a = pp[0], b = pp[1] > 0.0 ? pp[1] : a, c = pp[2] > 0.0 ? pp[2] : a;
     double al = 90.0, be = 90.0, ga = 90.0;
@@ -131,7 +132,7 @@ __device__ inline void sep_phase_setup(const DevPlan &P, const DevPhase &ph, con
     for (int r = 0; r < 3; ++r)
         for (int i = 0; i < 3; ++i) { rows[r][i] /= V; o.rb[3 * r + i] = (float)rows[r][i]; }
     for (int w = 0; w < 2; ++w) {
-        const float *ax = w ? ph.strain_axis : ph.size_axis;
+        const float *ax = w ? sd.strain_axis : sd.size_axis;
         double A[3];
         for (int i = 0; i < 3; ++i) A[i] = ax[0] * rows[0][i] + ax[1] * rows[1][i] + ax[2] * rows[2][i];
         double n = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
@@ -165,15 +166,15 @@ __device__ inline void sep_phase_setup(const DevPlan &P, const DevPhase &ph, con
     o.em = (float)fmin(1.0, fmax(0.0, finite_or(S[10], 1.0)));
     for (int i = 0; i < 15; ++i) {
         double e = 0.0;
-        if (ph.stephens)
-            for (int k = 0; k < 15; ++k) e += finite_or(S[11 + k], 0.0) * (double)P.sbasis[(q * 15 + k) * 15 + i];
+        if (sd.stephens)
+            for (int k = 0; k < 15; ++k) e += finite_or(S[11 + k], 0.0) * (double)P.sp->sbasis[(q * 15 + k) * 15 + i];
         o.e[i] = (float)e;
     }
 }
 
 // the sample terms of one reflection (h k l, spacing d) of a separated phase:
 // 1/size along it and its microstrain (profile.js:384-411)
-__device__ __forceinline__ void sep_reflection(const DevPhase &ph, const SepPhase &sp, int h, int k, int l, float d,
+__device__ __forceinline__ void sep_reflection(const DevSepPhase &ph, const SepPhase &sp, int h, int k, int l, float d,
                                                float &inv_size, float &strain) {
     const float fh = (float)h, fk = (float)k, fl = (float)l;
     float n[3] = {0.f, 0.f, 0.f};
@@ -221,11 +222,11 @@ __device__ __forceinline__ void sep_reflection(const DevPhase &ph, const SepPhas
 __device__ inline bool stephens_bad(const DevPlan &P, const double *p) {
     bool bad = false;
     for (int q = 0; q < P.n_phases; ++q) {
-        const DevPhase &ph = P.phase[q];
+        const DevSepPhase &ph = P.sp->ph[q];
         if (!ph.sep || !ph.stephens) continue;
-        const double *S = p + ph.param_base + 8 + 11;
+        const double *S = p + P.phase[q].param_base + 8 + 11;
         for (int dn = threadIdx.x; dn < ph.n_sdir; dn += blockDim.x) {
-            const double *g = P.sguard + (size_t)(ph.sdir_off + dn) * 15;
+            const double *g = P.sp->sguard + (size_t)(ph.sdir_off + dn) * 15;
             double v = 0.0;
             for (int k = 0; k < 15; ++k) v += g[k] * finite_or(S[k], 0.0);
             if (v < 0.0) bad = true;
@@ -245,20 +246,30 @@ __device__ inline void materialize(const DevPlan &P, const double *x, SetupShare
 
 // per-trial setup after materialize (+barrier): metrics, atoms, profile slots, guards.
 // Ends with a barrier; sets sh.cell_ok and sh.prof_ok.
+// SEP = false: a plan without separated profiles (P.any_sep == 0); the separated code is
+// compiled out of that instance
+template <bool SEP = true>
 __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
     int tid = threadIdx.x, nt = blockDim.x;
     if (tid < P.n_phases) {
         const DevPhase &ph = P.phase[tid];
         sh.phase_ok[tid] = phase_metric(ph, sh.p + ph.param_base, sh.metric[tid]) ? 1 : 0;
     }
-    if (tid == 63 || (nt < 64 && tid == 0)) sh.prof_ok = profile_valid(P, sh.p) ? 1 : 0;
-    for (int i = tid; i < P.n_phases * PTK_PH_NPROF; i += nt) {
-        int q = i / PTK_PH_NPROF, s = i % PTK_PH_NPROF;
-        sh.prof[q][s] = (float)sh.p[P.phase[q].param_base + 8 + s];
+    if (tid == 63 || (nt < 64 && tid == 0)) sh.prof_ok = profile_valid<SEP>(P, sh.p) ? 1 : 0;
+    // profile slots: the families read nine, a separated phase all 32
+    if (SEP && P.any_sep) {
+        for (int q = 0; q < P.n_phases; ++q)
+            for (int s = tid; s < PTK_PH_NPROF; s += nt) sh.prof[q][s] = (float)sh.p[P.phase[q].param_base + 8 + s];
+    } else {
+        for (int i = tid; i < P.n_phases * 16; i += nt) {
+            int q = i >> 4, s = i & 15;
+            sh.prof[q][s] = (float)sh.p[P.phase[q].param_base + 8 + s];
+        }
     }
-    for (int i = tid; i < PTK_N_INST; i += nt) sh.inst[i] = (float)finite_or(sh.p[PTK_P_INST0 + i], inst_default(i));
+    if (SEP && P.any_sep)
+        for (int i = tid; i < PTK_N_INST; i += nt) sh.inst[i] = (float)finite_or(sh.p[PTK_P_INST0 + i], inst_default(i));
     for (int q = 0; q < P.n_phases; ++q)
-        if (P.phase[q].sep && tid == (32 + q) % nt) {
+        if (SEP && P.sp->ph[q].sep && tid == 32 + q) {           // every setup block has at least 64 threads
             sep_phase_setup(P, P.phase[q], sh.p + P.phase[q].param_base, q, sh.sep[q]);
             // the family's eta / SL / HL slots, read from the instrument values
             // (rietveld_core.js:699-701), after the model slots
@@ -270,7 +281,7 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
             else if (type == 1) fs[5] = (float)finite_or(in[8], 0.5);
             else fs[8] = (float)finite_or(in[17], 0.5);
         }
-    const bool strain_bad = P.has_stephens ? stephens_bad(P, sh.p) : false;
+    const bool strain_bad = SEP && P.has_stephens ? stephens_bad(P, sh.p) : false;
     if (!P.bkg_spline) {
         for (int i = tid; i < P.n_bkg; i += nt) sh.bkg[i] = (float)sh.p[3 + i];
     } else if (tid == nt - 1) {
@@ -293,7 +304,9 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
             sh.aB[ga] = (float)ap[4];
         }
     }
-    const int any_bad = __syncthreads_or(strain_bad ? 1 : 0);
+    int any_bad = 0;
+    if (SEP && P.has_stephens) any_bad = __syncthreads_or(strain_bad ? 1 : 0);   // uniform over the block
+    else __syncthreads();
     if (tid == 0) {
         int ok = 1;
         for (int q = 0; q < P.n_phases; ++q) ok &= sh.phase_ok[q];
@@ -411,6 +424,7 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
 // With unit_factor != null the lines are written at unit height (the Kalpha2 line at its
 // doublet weight) and the reflection's height factor m Lp PO / area goes to
 // unit_factor[k] instead: the columns of the shared profile matrix.
+template <bool SEP = true>
 __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
                                         float *unit_factor = nullptr, const int4 *hk_given = nullptr) {
     if (unit_factor) unit_factor[k] = 0.0f;
@@ -438,12 +452,13 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
     // HL slots filled from the instrument values (rietveld_core.js:694-708)
     SepLine sl{nullptr, 0.f, 0.f, 0.f, 0.f, 0.f};
     const float *prof = sh.prof[q];
-    if (ph.sep) {
-        sep_reflection(ph, sh.sep[q], hk.x, hk.y, hk.z, (float)(1.0 / s), sl.inv_size, sl.strain);
+    if (SEP && P.sp->ph[q].sep) {
+        sep_reflection(P.sp->ph[q], sh.sep[q], hk.x, hk.y, hk.z, (float)(1.0 / s), sl.inv_size, sl.strain);
         sl.inst = sh.inst; sl.sm = sh.sep[q].sm; sl.em = sh.sep[q].em; sl.lam = (float)P.lambda1;
         prof = sh.prof[q] + PTK_PH_NPROF;
     }
-    prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base1, pk1, sl);
+    if (SEP) prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base1, pk1, sl);
+    else prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base1, pk1);
     if (!(pk1.area > 0.0f)) return;
     float lp1 = lp_factor((float)base1, (float)P.pol_k);
     const double *pp = sh.p + ph.param_base;
@@ -466,7 +481,8 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
             double base2 = b2 + (double)peak_shift(b2, disp, transp) + zero;
             Prep pk2;
             sl.lam = (float)P.lambda2;
-            prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base2, pk2, sl);
+            if (SEP) prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base2, pk2, sl);
+            else prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base2, pk2);
             float w2 = (float)P.ratio * pk1.area / pk2.area * (lp_factor((float)base2, (float)P.pol_k) / lp1);
             emit_line(P, pk2, base2, height * w2, rec[1], span[1]);
         }
@@ -696,14 +712,20 @@ __device__ inline int total_atoms(const DevPlan &P) {
     return l.atom_off + l.n_atoms;
 }
 
+template <bool SEP = true>
 __device__ inline void image_store(const DevPlan &P, const SetupShared &sh, TrialImage *img) {
     const int tid = threadIdx.x, nt = blockDim.x, na = total_atoms(P);
     for (int i = tid; i < P.n_params; i += nt) img->p[i] = sh.p[i];
     for (int i = tid; i < P.n_phases * 6; i += nt) img->metric[i / 6][i % 6] = sh.metric[i / 6][i % 6];
     for (int i = tid; i < na * 3; i += nt) img->apos[i / 3][i % 3] = sh.apos[i / 3][i % 3];
     for (int i = tid; i < na; i += nt) { img->aocc[i] = sh.aocc[i]; img->aB[i] = sh.aB[i]; }
-    for (int i = tid; i < P.n_phases * PTK_PROF_LDS; i += nt) img->prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS] = sh.prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS];
-    if (P.any_sep) {
+    if (SEP && P.any_sep) {
+        for (int q = 0; q < P.n_phases; ++q)
+            for (int s = tid; s < PTK_PROF_LDS; s += nt) img->prof[q][s] = sh.prof[q][s];
+    } else {
+        for (int i = tid; i < P.n_phases * 16; i += nt) img->prof[i >> 4][i & 15] = sh.prof[i >> 4][i & 15];
+    }
+    if (SEP && P.any_sep) {
         for (int i = tid; i < PTK_N_INST; i += nt) img->inst[i] = sh.inst[i];
         const int nw = (int)(sizeof(SepPhase) / sizeof(float));
         for (int i = tid; i < P.n_phases * nw; i += nt)
@@ -715,14 +737,20 @@ __device__ inline void image_store(const DevPlan &P, const SetupShared &sh, Tria
     for (int g = tid; g < (P.np + 511) >> 9; g += nt) { img->grange[g][0] = 0x7fffffff; img->grange[g][1] = -1; }
 }
 
+template <bool SEP = true>
 __device__ inline void image_load(const DevPlan &P, const TrialImage *img, SetupShared &sh) {
     const int tid = threadIdx.x, nt = blockDim.x, na = total_atoms(P);
     for (int i = tid; i < P.n_params; i += nt) sh.p[i] = img->p[i];
     for (int i = tid; i < P.n_phases * 6; i += nt) sh.metric[i / 6][i % 6] = img->metric[i / 6][i % 6];
     for (int i = tid; i < na * 3; i += nt) sh.apos[i / 3][i % 3] = img->apos[i / 3][i % 3];
     for (int i = tid; i < na; i += nt) { sh.aocc[i] = img->aocc[i]; sh.aB[i] = img->aB[i]; }
-    for (int i = tid; i < P.n_phases * PTK_PROF_LDS; i += nt) sh.prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS] = img->prof[i / PTK_PROF_LDS][i % PTK_PROF_LDS];
-    if (P.any_sep) {
+    if (SEP && P.any_sep) {
+        for (int q = 0; q < P.n_phases; ++q)
+            for (int s = tid; s < PTK_PROF_LDS; s += nt) sh.prof[q][s] = img->prof[q][s];
+    } else {
+        for (int i = tid; i < P.n_phases * 16; i += nt) sh.prof[i >> 4][i & 15] = img->prof[i >> 4][i & 15];
+    }
+    if (SEP && P.any_sep) {
         for (int i = tid; i < PTK_N_INST; i += nt) sh.inst[i] = img->inst[i];
         const int nw = (int)(sizeof(SepPhase) / sizeof(float));
         for (int i = tid; i < P.n_phases * nw; i += nt)
@@ -737,6 +765,7 @@ __device__ inline void image_load(const DevPlan &P, const TrialImage *img, Setup
 // (js/pt_gpu.js:361 propose; rietveld_core.js:2196-2212): one coordinate j drawn
 // uniformly, x_j + z scale_j sqrt(T_c) reflected into its box, log u for the Metropolis
 // test.  Random numbers are keyed on (seed, chain, the chain's step counter).
+template <bool SEP = true>
 __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, TrialImage *img, int t, int pt_mode, int bootstrap,
                                   SetupShared &sh, double *prop, const double2 *pick = nullptr) {
     const int nv = P.nv;
@@ -775,7 +804,7 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
         materialize(P, X + (size_t)t * nv, sh);
     }
     __syncthreads();
-    trial_setup(P, sh);
+    trial_setup<SEP>(P, sh);
     if (threadIdx.x == 0) {
         bool go = sh.cell_ok;
         if (pt_mode) {
@@ -787,7 +816,7 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
         sh.accept = boot ? 1 : 0;
     }
     __syncthreads();
-    image_store(P, sh, img + t);
+    image_store<SEP>(P, sh, img + t);
 }
 
 // (t_off: the first trial of a launch over a contiguous part of the trials)
@@ -795,25 +824,27 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
                                                     int pt_mode, int bootstrap, const double2 *__restrict__ pick, int t_off) {
     __shared__ SetupShared sh;
     __shared__ double prop[PTK_MAX_VARS];
-    setup_body(P, X, img, t_off + blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
+    if (P.any_sep) setup_body<true>(P, X, img, t_off + blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
+    else setup_body<false>(P, X, img, t_off + blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
 }
 
 // line records of reflections [kb * blockDim.x, ...) of trial t; every line with a window
 // widens the line range of the 512-point groups it meets (reduced in LDS, then one global
 // atomic min / max per group and workgroup)
+template <bool SEP = true>
 __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, LineRec *rec, int4 *span, int t, int kb,
                                            Shared &sh, int (*gr)[2]) {
     const int k = kb * blockDim.x + threadIdx.x;
     const int4 hk = k < P.nr ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
     stage_reps(P, sh);
-    image_load(P, img + t, sh);
+    image_load<SEP>(P, img + t, sh);
     const int ng = (P.np + 511) >> 9;
     for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
     if (k < P.nr) {
         const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
         if (sh.go) {
-            build_reflection(P, sh, k, rec + base, span + base, nullptr, &hk);
+            build_reflection<SEP>(P, sh, k, rec + base, span + base, nullptr, &hk);
             for (int q = 0; q < 2; ++q) {
                 const int4 sp = span[base + q];
                 if (sp.y > sp.x)
@@ -852,7 +883,8 @@ __global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img,
     __shared__ int gr[PTK_MAX_GROUPS][2];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    lines_body(P, img, rec, span, t_off + t, u, sh, gr);
+    if (P.any_sep) lines_body<true>(P, img, rec, span, t_off + t, u, sh, gr);
+    else lines_body<false>(P, img, rec, span, t_off + t, u, sh, gr);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
@@ -1256,6 +1288,7 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
 
 // the Metropolis step of chain t, then its next proposal and setup: the finish of one PT
 // step fused with the setup of the next (one launch less per step)
+template <bool SEP = true>
 __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *img, const double2 *__restrict__ partial,
                                          int nsplit, int t, SetupShared &sh, double *prop, int &s_acc) {
     const int nv = P.nv;
@@ -1271,15 +1304,17 @@ __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *
         }
     }
     __syncthreads();
-    setup_body(P, nullptr, img, t, 1, 0, sh, prop);
+    setup_body<SEP>(P, nullptr, img, t, 1, 0, sh, prop);
 }
 
+// one instance per plan kind: a kernel holding both runs the plain one slower
+template <bool SEP>
 __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
                                                            int nsplit, int t_off) {
     __shared__ SetupShared sh;
     __shared__ double prop[PTK_MAX_VARS];
     __shared__ int s_acc;
-    finish_setup_body(P, img, partial, nsplit, t_off + blockIdx.x, sh, prop, s_acc);
+    finish_setup_body<SEP>(P, img, partial, nsplit, t_off + blockIdx.x, sh, prop, s_acc);
 }
 
 // ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
@@ -1321,7 +1356,8 @@ __global__ void __launch_bounds__(64) k_group_setup(const DevPlan *__restrict__ 
     __shared__ double prop[PTK_MAX_VARS];
     const int2 w = tab[blockIdx.x];
     const DevPlan &P = plans[w.x];
-    setup_body(P, nullptr, (TrialImage *)mem[w.x].img, w.y, 1, bootstrap, sh, prop);
+    if (P.any_sep) setup_body<true>(P, nullptr, (TrialImage *)mem[w.x].img, w.y, 1, bootstrap, sh, prop);
+    else setup_body<false>(P, nullptr, (TrialImage *)mem[w.x].img, w.y, 1, bootstrap, sh, prop);
 }
 
 __global__ void __launch_bounds__(256) k_group_lines(const DevPlan *__restrict__ plans, const DevMember *__restrict__ mem,
@@ -1332,7 +1368,8 @@ __global__ void __launch_bounds__(256) k_group_lines(const DevPlan *__restrict__
     if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    lines_body(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh, gr);
+    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh, gr);
+    else lines_body<false>(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh, gr);
 }
 
 template <int TPW>
@@ -1369,7 +1406,8 @@ __global__ void __launch_bounds__(64) k_group_finish_setup(const DevPlan *__rest
     const int2 w = tab[blockIdx.x];
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    finish_setup_body(P, (TrialImage *)M.img, M.partial, M.nsplit[0], w.y, sh, prop, s_acc);
+    if (P.any_sep) finish_setup_body<true>(P, (TrialImage *)M.img, M.partial, M.nsplit[0], w.y, sh, prop, s_acc);
+    else finish_setup_body<false>(P, (TrialImage *)M.img, M.partial, M.nsplit[0], w.y, sh, prop, s_acc);
 }
 
 // one thread per (member, pair)
@@ -1920,7 +1958,8 @@ hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 
 
 hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s, int ntrials,
                                      int t_off) {
-    hipLaunchKernelGGL(k_stage_finish_setup, dim3(ntrials), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit, t_off);
+    if (P.any_sep) hipLaunchKernelGGL(k_stage_finish_setup<true>, dim3(ntrials), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit, t_off);
+    else hipLaunchKernelGGL(k_stage_finish_setup<false>, dim3(ntrials), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit, t_off);
     return hipGetLastError();
 }
 

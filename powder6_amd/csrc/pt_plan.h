@@ -58,13 +58,22 @@ struct DevPhase {
     double po_dir[3];
     double ang[6];             // alpha, beta, gamma (deg), cos alpha, cos beta, cos gamma
     double ang_sin[3];
-    // separated (GSAS) profile: instrument function from the inst slots, sample broadening
-    // from the phase's 26 sample slots; profile_type above is then the instrument family
+};
+
+// A phase's separated (GSAS) profile: instrument function from the inst slots, sample
+// broadening from the phase's 26 sample slots (profile_type is then the instrument
+// family).  Kept in global memory, apart from the kernel arguments.
+struct DevSepPhase {
     int sep;
     int size_model, strain_model;  // PT_SIZE_*, PT_STRAIN_*
-    int n_sdir, sdir_off;          // Stephens guard directions [sdir_off, sdir_off + n_sdir) of P.sguard
-    int stephens;                  // 1: P.sbasis rows of this phase hold the Stephens basis
+    int stephens;                  // 1: the phase's rows of sbasis hold its Stephens basis
+    int n_sdir, sdir_off;          // Stephens guard directions [sdir_off, sdir_off + n_sdir) of sguard
     float size_axis[3], strain_axis[3];  // uniaxial axes (Miller indices)
+};
+struct DevSepPlan {
+    DevSepPhase ph[PTK_MAX_PHASES];
+    const float *sbasis;           // Stephens basis per phase, [phase][key 15][coefficient 15]
+    const double *sguard;          // Stephens guard: per direction the 15 key coefficients
 };
 
 struct DevPlan {
@@ -73,9 +82,8 @@ struct DevPlan {
     int chains, stride, steps_per_sweep;
     int uniform, radiation, doublet;
     int max_lines_lds;         // line-record chunk held in LDS
-    int any_sep;               // some phase has a separated profile
-    int has_stephens;          // some separated phase has Stephens strain (guard directions in sguard)
-    int pad0;
+    short any_sep;             // some phase has a separated profile
+    short has_stephens;        // some separated phase has Stephens strain
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
     double inv_step;           // 1 / step
@@ -108,8 +116,7 @@ struct DevPlan {
     const double *term_coeff;
     const double *vlo, *vhi, *vscale;
     const double *temp;        // temperature of chain c: max_temp^(c / (chains - 1))
-    const float *sbasis;       // Stephens basis per phase, [phase][key 15][coefficient 15]
-    const double *sguard;      // Stephens guard: per direction the 15 key coefficients
+    const DevSepPlan *sp;      // separated profiles (always set)
     double *state;
 };
 

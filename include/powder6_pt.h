@@ -205,6 +205,18 @@ void pt_plan_destroy(pt_plan *plan);                      /* close(), js/pt_gpu.
 int pt_plan_compile(const pt_problem *prob, pt_plan **out);
 int pt_plan_info_get(const pt_plan *plan, pt_plan_info *info);
 
+/* compile(ctx) + create(ctx) for a context with several datasets (js/pt_gpu.js:20-227
+ * loops problem.datasets; the per-chain cost is the sum of every dataset's chi^2,
+ * :259-297).  Each pt_problem is one dataset in its own model-parameter layout; all
+ * share the variables (n_vars, x0, lo, hi, scale identical; a variable a dataset does not
+ * use has no term there) and the PT settings of datasets[0].  pt_set_states, pt_batch,
+ * pt_costs and pt_eval (PT_FORM_BATCHED) then work on the sum; pt_pattern gives the
+ * first dataset's pattern and pt_plan_dataset the plan of dataset d (owned by `plan`)
+ * for the others'.  Surrogate screening, the plain and the shared-profile forms take one
+ * dataset per plan. */
+int pt_plan_create_multi(const pt_problem *datasets, int n_datasets, pt_plan **out);
+int pt_plan_dataset(pt_plan *plan, int dataset, pt_plan **sub);
+
 /* Reflection envelope of one phase as enumerated by the plan (h,k,l,m per row):
  * the list js/pt_gpu.js:110-131 builds.  Returns the row count, or <0 on error; writes
  * at most `cap` rows.                                                                */

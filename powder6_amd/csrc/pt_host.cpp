@@ -265,6 +265,10 @@ struct pt_plan {
     int n_bkg = 0;
     int bkg_type = PT_BKG_CHEBYSHEV;
     std::vector<double> bkg_x;         // spline anchors
+    // multi-dataset plans: the further datasets (owned), each with its own pipeline buffers;
+    // the first dataset's finish adds their chi^2 through d_follow
+    std::vector<pt_plan *> follow;
+    ptk::DevFollow *d_follow = nullptr;
     hipEvent_t ev_total[2] = {nullptr, nullptr};
     double last_total_ms = 0, last_kernel_ms = 0, last_matrix_ms[2] = {0, 0};
     int last_launches = 0;
@@ -421,6 +425,8 @@ int pt_phase_base(const pt_problem *prob, int phase) {
 
 void pt_plan_destroy(pt_plan *pl) {
     if (!pl) return;
+    for (pt_plan *f : pl->follow) pt_plan_destroy(f);
+    pl->follow.clear();
     if (pl->device >= 0) {
         (void)hipSetDevice(pl->device);
         if (pl->stream) (void)hipStreamSynchronize(pl->stream);
@@ -919,6 +925,33 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
 }
 
 int pt_plan_create(const pt_problem *pb, pt_plan **out) { return create_impl(pb, out, true); }
+
+int pt_plan_create_multi(const pt_problem *datasets, int n_datasets, pt_plan **out) {
+    if (!datasets || !out || n_datasets < 1 || n_datasets > 8) return fail(PT_ERR_ARG, "PT takes 1 to 8 datasets per plan.");
+    *out = nullptr;
+    for (int d = 1; d < n_datasets; ++d)
+        if (datasets[d].n_vars != datasets[0].n_vars) return fail(PT_ERR_ARG, "the datasets of a PT plan must share one variable list.");
+    pt_plan *pl = nullptr;
+    int rc = create_impl(&datasets[0], &pl, true);
+    if (rc) return rc;
+    for (int d = 1; d < n_datasets; ++d) {
+        pt_problem q = datasets[d];
+        // the PT settings are the first dataset's; the others only need room for its chains
+        q.chains = datasets[0].chains; q.sweeps = datasets[0].sweeps; q.steps_per_sweep = datasets[0].steps_per_sweep;
+        q.max_temp = datasets[0].max_temp; q.seed = datasets[0].seed;
+        pt_plan *f = nullptr;
+        if ((rc = create_impl(&q, &f, true))) { pt_plan_destroy(pl); return rc; }
+        pl->follow.push_back(f);
+    }
+    *out = pl;
+    return 0;
+}
+
+int pt_plan_dataset(pt_plan *pl, int dataset, pt_plan **sub) {
+    if (!pl || !sub || dataset < 0 || dataset > (int)pl->follow.size()) return fail(PT_ERR_ARG, "dataset index out of range");
+    *sub = dataset ? pl->follow[dataset - 1] : pl;
+    return 0;
+}
 int pt_plan_compile(const pt_problem *pb, pt_plan **out) { return create_impl(pb, out, false); }
 
 #define NEED_DEVICE(pl) \
@@ -970,6 +1003,8 @@ int pt_set_states(pt_plan *pl, const double *x, int nvec) {
     return 0;
 }
 
+static void drop_graph(hipGraphExec_t &ge, hipGraph_t &g);
+
 // staged-pipeline buffers for `n` trial slots
 static int ensure_slots(pt_plan *pl, int n) {
     if (n <= pl->slots) return 0;
@@ -980,6 +1015,19 @@ static int ensure_slots(pt_plan *pl, int n) {
         (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->ngroups)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
         return rc;
     pl->slots = n;
+    if (!pl->follow.empty()) {
+        // the further datasets get as many slots, and the first dataset's finish their buffers
+        std::vector<ptk::DevFollow> fv;
+        for (pt_plan *f : pl->follow) {
+            if ((rc = ensure_slots(f, n))) return rc;
+            fv.push_back(ptk::DevFollow{f->d_img, f->d_part, f->ngroups, 0});
+        }
+        if (!pl->d_follow && (rc = dalloc(pl, &pl->d_follow, fv.size()))) return rc;
+        HIPCHECK(hipMemcpy(pl->d_follow, fv.data(), sizeof(ptk::DevFollow) * fv.size(), hipMemcpyHostToDevice), "dataset links");
+        pl->P.n_follow = (int)fv.size();
+        pl->P.follow = pl->d_follow;
+        drop_graph(pl->gexec, pl->graph);
+    }
     return 0;
 }
 
@@ -987,10 +1035,18 @@ static int ensure_slots(pt_plan *pl, int n) {
 static int run_stages(pt_plan *pl, const double *dX, int ntrials, int pt_mode, int bootstrap, double *d_chi2,
                       hipEvent_t *ev_pair) {
     HIPCHECK(ptk::launch_stage_setup(pl->P, dX, pl->d_img, ntrials, pt_mode, bootstrap, pl->stream), "setup");
+    for (pt_plan *f : pl->follow) {
+        if (pt_mode) HIPCHECK(ptk::launch_stage_follow(f->P, pl->P.state, pl->stride, pl->d_img, f->d_img, ntrials, pl->stream), "setup");
+        else HIPCHECK(ptk::launch_stage_setup(f->P, dX, f->d_img, ntrials, 0, 0, pl->stream), "setup");
+    }
     HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, ntrials, pl->stream), "lines");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[0], pl->stream), "timing");
     HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[1], pl->stream), "timing");
+    for (pt_plan *f : pl->follow) {
+        HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_srec, f->d_sspan, ntrials, pl->stream), "lines");
+        HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_srec, f->d_sspan, f->d_part, f->ngroups, ntrials, f->tpw, pl->stream), "accumulate");
+    }
     HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, ntrials, d_chi2, pt_mode, pl->stream), "finish");
     return 0;
 }
@@ -1117,6 +1173,7 @@ static int tune_plan(pt_plan *pl) {
     if (rc) return rc;
     if (best != pl->tpw) drop_graph(pl->gexec, pl->graph);
     pl->tpw = best;
+    for (pt_plan *f : pl->follow) f->tpw = best;   // datasets of one plan share the tile shape
     pl->tuned = true;
     return 0;
 }
@@ -1173,17 +1230,32 @@ static int enqueue_screened_sweep(pt_plan *pl, hipEvent_t *ev) {
 // one sweep: steps_per_sweep x (setup/propose, lines, accumulate, finish/accept), then the
 // replica exchange; the finish of each step but the last is fused with the next setup.
 // Accumulate launches are bracketed by events when ev != null.
+// the further datasets' setup of chains [t0, t0 + n) from the first dataset's proposals
+static int enqueue_follow(pt_plan *pl, hipStream_t st, int t0, int n) {
+    for (pt_plan *f : pl->follow)
+        HIPCHECK(ptk::launch_stage_follow(f->P, pl->P.state, pl->stride, pl->d_img, f->d_img, n, st, t0), "setup");
+    return 0;
+}
+
 static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *ev) {
     HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, n, 1, 0, st, nullptr, t0), "setup");
+    int rc = enqueue_follow(pl, st, t0, n);
+    if (rc) return rc;
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
         HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, n, st, t0), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], st), "timing");
         HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, n, pl->tpw, st, t0),
                  "accumulate");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], st), "timing");
-        if (s + 1 < pl->steps_per_sweep)
+        for (pt_plan *f : pl->follow) {
+            HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_srec, f->d_sspan, n, st, t0), "lines");
+            HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_srec, f->d_sspan, f->d_part, f->ngroups, n, f->tpw, st, t0),
+                     "accumulate");
+        }
+        if (s + 1 < pl->steps_per_sweep) {
             HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, st, n, t0), "finish");
-        else
+            if ((rc = enqueue_follow(pl, st, t0, n))) return rc;
+        } else
             HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, n, pl->d_schi2, 1, st, t0), "finish");
     }
     return 0;
@@ -1218,6 +1290,7 @@ static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
 
 int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
     if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    if (!pl->follow.empty() && cfg) return fail(PT_ERR_STATE, "PT surrogate screening takes one dataset per plan.");
     NEED_DEVICE(pl);
     (void)hipSetDevice(pl->device);
     drop_graph(pl->gexec, pl->graph);
@@ -1376,6 +1449,7 @@ int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
     NEED_DEVICE(pl);
     (void)hipSetDevice(pl->device);
     if (form == PT_FORM_BATCHED) return eval_batched(pl, x, ntrials, chi2, true);
+    if (!pl->follow.empty()) return fail(PT_ERR_STATE, "PT evaluation form not available for a multi-dataset plan (use the batched form).");
     if (form == PT_FORM_PLAIN) {
         int rc = ensure_trials(pl, ntrials);
         if (rc) return rc;

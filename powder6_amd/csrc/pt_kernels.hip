@@ -828,6 +828,34 @@ __global__ void __launch_bounds__(64) k_stage_setup(DevPlan P, const double *__r
     else setup_body<false>(P, X, img, t_off + blockIdx.x, pt_mode, bootstrap, sh, prop, pick);
 }
 
+// Setup of trial t in a further dataset of a multi-dataset plan: the first dataset's PT
+// proposal of chain t (state slots [nv, 2 nv)) materialised in this dataset's parameter
+// layout; the trial goes when it goes in the first dataset and its cell and, for a
+// feasible chain, its profiles are valid here (js/pt_gpu.js:361 rule per dataset)
+template <bool SEP>
+__device__ __forceinline__ void follow_body(const DevPlan &P, const double *mstate, int mstride, const TrialImage *mimg,
+                                            TrialImage *img, int t, SetupShared &sh) {
+    const double *st = mstate + (size_t)t * mstride;
+    materialize(P, st + P.nv, sh);
+    __syncthreads();
+    trial_setup<SEP>(P, sh);
+    if (threadIdx.x == 0) {
+        const bool boot = mimg[t].boot != 0;
+        const bool go = mimg[t].go && sh.cell_ok && (boot || st[3 * P.nv + 11] < 0.5 || sh.prof_ok);
+        sh.go = go ? 1 : 0;
+        sh.accept = boot ? 1 : 0;
+    }
+    __syncthreads();
+    image_store<SEP>(P, sh, img + t);
+}
+
+__global__ void __launch_bounds__(64) k_stage_follow(DevPlan P, const double *__restrict__ mstate, int mstride,
+                                                     const TrialImage *__restrict__ mimg, TrialImage *img, int t_off) {
+    __shared__ SetupShared sh;
+    if (P.any_sep) follow_body<true>(P, mstate, mstride, mimg, img, t_off + blockIdx.x, sh);
+    else follow_body<false>(P, mstate, mstride, mimg, img, t_off + blockIdx.x, sh);
+}
+
 // line records of reflections [kb * blockDim.x, ...) of trial t; every line with a window
 // widens the line range of the 512-point groups it meets (reduced in LDS, then one global
 // atomic min / max per group and workgroup)
@@ -1235,9 +1263,27 @@ __device__ inline double trial_cost(const TrialImage *im, const double2 *partial
     return cost;
 }
 
+// the chi^2 of trial t over every dataset of the plan (js/pt_gpu.js:259-297: the targets
+// of all datasets add into one cost per chain), and whether every dataset's profile is
+// feasible
+__device__ inline double trial_cost_all(const DevPlan &P, const TrialImage *im, const double2 *partial, int nsplit, int t,
+                                        int &pok) {
+    double c = trial_cost(im, partial, nsplit, t);
+    pok = im->prof_ok;
+    for (int f = 0; f < P.n_follow; ++f) {
+        const DevFollow &F = P.follow[f];
+        const TrialImage *fi = (const TrialImage *)F.img + t;
+        const double cf = trial_cost(fi, F.partial, F.nsplit, t);
+        c = (c >= 3e38 || cf >= 3e38) ? 3e38 : c + cf;
+        pok = pok && fi->prof_ok;
+    }
+    return c;
+}
+
 // the Metropolis step of chain t in FP64 (js/pt_gpu.js:363-364, rietveld_core.js:2213-2222),
-// run by one thread; returns 0 (rejected), 1 (accepted) or 2 (accepted, new best)
-__device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, double cost, int t) {
+// run by one thread; returns 0 (rejected), 1 (accepted) or 2 (accepted, new best).
+// prof_ok: the trial's profiles are feasible in every dataset.
+__device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int t) {
     const int nv = P.nv;
     double *sc = P.state + (size_t)t * P.stride + 3 * nv;
     const bool boot = im->boot != 0;
@@ -1248,9 +1294,9 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
     const bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
     int r = 0;
     if (acc) {
-        const bool take_best = im->prof_ok && (boot || cost < sc[1]);
+        const bool take_best = prof_ok && (boot || cost < sc[1]);
         sc[0] = cost;
-        sc[11] = im->prof_ok ? 1.0 : 0.0;
+        sc[11] = prof_ok ? 1.0 : 0.0;
         if (!boot) sc[3] += 1.0;
         if (take_best) sc[1] = cost;
         r = take_best ? 2 : 1;
@@ -1262,8 +1308,8 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
 
 // the end of one PT step for chain t: Metropolis accept, and on acceptance the proposal
 // becomes the chain's state (and its best when it is a new feasible best)
-__device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, double cost, int t) {
-    const int r = accept_chain(P, im, cost, t);
+__device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int t) {
+    const int r = accept_chain(P, im, cost, prof_ok, t);
     if (r) {
         double *st = P.state + (size_t)t * P.stride;
         const int nv = P.nv;
@@ -1281,9 +1327,10 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
     const int tl = blockIdx.x * blockDim.x + threadIdx.x;
     if (tl >= ntrials) return;
     const int t = t_off + tl;
-    const double cost = trial_cost(img + t, partial, nsplit, t);
+    int pok;
+    const double cost = trial_cost_all(P, img + t, partial, nsplit, t, pok);
     if (!pt_mode) { chi2[t] = cost; return; }
-    finish_chain(P, img + t, cost, t);
+    finish_chain(P, img + t, cost, pok, t);
 }
 
 // the Metropolis step of chain t, then its next proposal and setup: the finish of one PT
@@ -1292,7 +1339,11 @@ template <bool SEP = true>
 __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *img, const double2 *__restrict__ partial,
                                          int nsplit, int t, SetupShared &sh, double *prop, int &s_acc) {
     const int nv = P.nv;
-    if (threadIdx.x == 0) s_acc = accept_chain(P, img + t, trial_cost(img + t, partial, nsplit, t), t);
+    if (threadIdx.x == 0) {
+        int pok;
+        const double cost = trial_cost_all(P, img + t, partial, nsplit, t, pok);
+        s_acc = accept_chain(P, img + t, cost, pok, t);
+    }
     __syncthreads();
     const int r = s_acc;
     if (r) {
@@ -1393,7 +1444,7 @@ __global__ void __launch_bounds__(64) k_group_finish(const DevPlan *__restrict__
     const DevMember &M = mem[w.x];
     const TrialImage *img = (const TrialImage *)M.img;
     const double cost = trial_cost(img + w.y, M.partial, M.nsplit[0], w.y);
-    if (pt_mode) finish_chain(P, img + w.y, cost, w.y);
+    if (pt_mode) finish_chain(P, img + w.y, cost, img[w.y].prof_ok, w.y);
     else M.chi2[w.y] = cost;
 }
 
@@ -1926,6 +1977,12 @@ hipError_t set_lds_limits(size_t bytes) {
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
                               hipStream_t s, const double2 *pick, int t_off) {
     hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick, t_off);
+    return hipGetLastError();
+}
+hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,
+                               hipStream_t s, int t_off) {
+    hipLaunchKernelGGL(k_stage_follow, dim3(ntrials), dim3(64), 0, s, P, mstate, mstride, (const TrialImage *)mimg,
+                       (TrialImage *)img, t_off);
     return hipGetLastError();
 }
 static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }

@@ -15,6 +15,8 @@ hipError_t set_lds_limits(size_t bytes);
 // setup draws the plain proposal itself
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
                               hipStream_t s, const double2 *pick = nullptr, int t_off = 0);
+hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,
+                               hipStream_t s, int t_off = 0);
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s,
                               int t_off = 0);
 // partial chi^2 per 512-point group: ngroups = ceil(np / 512) per trial, whatever tpw

@@ -70,6 +70,16 @@ struct DevSepPhase {
     int n_sdir, sdir_off;          // Stephens guard directions [sdir_off, sdir_off + n_sdir) of sguard
     float size_axis[3], strain_axis[3];  // uniaxial axes (Miller indices)
 };
+// another dataset of a multi-dataset plan, as the first dataset's finish reads it: the
+// trial images and chi^2 partials of its own pipeline (js/pt_gpu.js:259-297 sums the
+// datasets' chi^2 per chain)
+struct DevFollow {
+    const void *img;
+    const double2 *partial;
+    int nsplit;
+    int pad;
+};
+
 struct DevSepPlan {
     DevSepPhase ph[PTK_MAX_PHASES];
     const float *sbasis;           // Stephens basis per phase, [phase][key 15][coefficient 15]
@@ -84,6 +94,8 @@ struct DevPlan {
     int max_lines_lds;         // line-record chunk held in LDS
     short any_sep;             // some phase has a separated profile
     short has_stephens;        // some separated phase has Stephens strain
+    int n_follow;              // further datasets whose chi^2 the finish adds (first dataset only)
+    int pad1;
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
     double inv_step;           // 1 / step
@@ -117,6 +129,7 @@ struct DevPlan {
     const double *vlo, *vhi, *vscale;
     const double *temp;        // temperature of chain c: max_temp^(c / (chains - 1))
     const DevSepPlan *sp;      // separated profiles (always set)
+    const DevFollow *follow;   // [n_follow]
     double *state;
 };
 

@@ -268,7 +268,9 @@ static napi_value info_object(napi_env env, const pt_plan_info *in) {
     return o;
 }
 
-/* planCreate(problem, withDevice) -> {handle, info}; throws with the library message */
+/* planCreate(problem, withDevice) -> {handle, info}; throws with the library message.
+ * An array of problems (one per dataset, one variable list) makes a multi-dataset plan
+ * (pt_plan_create_multi; device only). */
 static napi_value plan_create(napi_env env, napi_callback_info cbi) {
     size_t argc = 2;
     napi_value argv[2];
@@ -276,12 +278,35 @@ static napi_value plan_create(napi_env env, napi_callback_info cbi) {
     if (argc < 1) { napi_throw_type_error(env, NULL, "planCreate(problem, withDevice)"); return NULL; }
     bool dev = true;
     if (argc > 1) napi_get_value_bool(env, argv[1], &dev);
-    holder h;
-    const char *err = convert(env, argv[0], &h);
-    if (err) { holder_free(&h); napi_throw_error(env, NULL, err); return NULL; }
+    bool is_array = false;
+    napi_is_array(env, argv[0], &is_array);
     pt_plan *plan = NULL;
-    int rc = dev ? pt_plan_create(&h.prob, &plan) : pt_plan_compile(&h.prob, &plan);
-    holder_free(&h);
+    int rc;
+    if (is_array) {
+        uint32_t n = 0;
+        napi_get_array_length(env, argv[0], &n);
+        if (n < 1 || n > 8) { napi_throw_error(env, NULL, "PT takes 1 to 8 datasets per plan."); return NULL; }
+        if (!dev) { napi_throw_error(env, NULL, "a multi-dataset plan needs the device"); return NULL; }
+        holder hs[8];
+        pt_problem probs[8];
+        const char *err = NULL;
+        uint32_t made = 0;
+        for (; made < n && !err; ++made) {
+            napi_value e;
+            napi_get_element(env, argv[0], made, &e);
+            err = convert(env, e, &hs[made]);
+            probs[made] = hs[made].prob;
+        }
+        rc = err ? 0 : pt_plan_create_multi(probs, (int)n, &plan);
+        for (uint32_t i = 0; i < made; ++i) holder_free(&hs[i]);
+        if (err) { napi_throw_error(env, NULL, err); return NULL; }
+    } else {
+        holder h;
+        const char *err = convert(env, argv[0], &h);
+        if (err) { holder_free(&h); napi_throw_error(env, NULL, err); return NULL; }
+        rc = dev ? pt_plan_create(&h.prob, &plan) : pt_plan_compile(&h.prob, &plan);
+        holder_free(&h);
+    }
     if (rc) { napi_throw_error(env, NULL, pt_last_error()); return NULL; }
     handle *hd = (handle *)calloc(1, sizeof(handle));
     hd->plan = plan;

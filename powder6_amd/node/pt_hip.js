@@ -74,18 +74,31 @@ function paramIndex(layout, name, spline) {
     return -1;
 }
 
-/* Flatten a reference PT context (one dataset, Rietveld phases, Chebyshev or spline background,
- * TCH / simple / split pseudo-Voigt, separated GSAS profiles) into the C-ABI problem. */
+/* Flatten a reference PT context (Rietveld phases, Chebyshev or spline background, TCH /
+ * simple / split pseudo-Voigt, separated GSAS profiles) into the C-ABI problem: one flat
+ * problem, or one per dataset (all with the context's variable list) when the context
+ * holds several datasets. */
 function fromContext(ctx) {
     if (ctx.flat) return ctx.flat;
+    var model = ctx.model;
+    if (!model || !model.datasets.length || model.datasets.length > 8) throw new Error('MI355X PT backend takes 1 to 8 datasets per fit; using the CPU path.');
+    if (model.datasets.length === 1) return flattenDataset(ctx, 0);
+    return model.datasets.map(function (_, d) { return flattenDataset(ctx, d); });
+}
+
+/* dataset d of the context: its phases (D.has), its value map problem.values[d], and the
+ * variables whose dataset list ctx.info[j].ds names d (js/pt_gpu.js:25 slots) */
+function flattenDataset(ctx, d) {
     var model = ctx.model, problem = ctx.problem;
-    if (!model || model.datasets.length !== 1) throw new Error('MI355X PT backend handles one dataset per fit; using the CPU path.');
-    var D = model.datasets[0], def = D.def;
+    var D = model.datasets[d], def = D.def, multi = model.datasets.length > 1;
     var spline = !!D.spline;
     if (spline && D.bkgX.length > 24) throw new Error('MI355X PT backend takes at most 24 spline anchors; using the CPU path.');
-    var values = problem.values[0], layout = [], base = P_PHASE0, phases = [];
+    var values = problem.values[d], layout = [], base = P_PHASE0, phases = [];
     model.phases.forEach(function (P, q) {
-        if (!D.has[q]) throw new Error('MI355X PT backend needs every phase in the dataset.');
+        if (!D.has[q]) {
+            if (!multi) throw new Error('MI355X PT backend needs every phase in the dataset.');
+            return;
+        }
         if (P.method !== 'rietveld') throw new Error('MI355X PT backend evaluates Rietveld phases only; using the CPU path.');
         var type = P.def.profileType;
         // rietveld_core.js:460 usesSeparatedProfile, :416-421 broadeningConfig
@@ -129,11 +142,14 @@ function fromContext(ctx) {
     layout.forEach(function (L) { for (var c = 3; c < 6; c++) if (!paramBase[L.base + c]) paramBase[L.base + c] = 90; });
     var termParam = [], termVar = [], termCoeff = [];
     problem.vars.forEach(function (vr, j) {
+        var ds = ctx.info[j] && ctx.info[j].ds;
+        if (multi && ds && ds.indexOf(d) < 0) return;          // acts in other datasets only
         var k = paramIndex(layout, vr.name, spline);
         if (k < 0) throw new Error('MI355X PT backend cannot refine ' + vr.name + '; using the CPU path.');
         paramBase[k] = 0; termParam.push(k); termVar.push(j); termCoeff.push(1);
     });
     if (ctx.constraints) ctx.constraints.links.forEach(function (link) {
+        if (multi && link.d !== undefined && link.d !== d) return;
         var k = paramIndex(layout, link.name, spline);
         if (k < 0) throw new Error('MI355X PT backend cannot link ' + link.name + '; using the CPU path.');
         paramBase[k] = link.offset;
@@ -152,12 +168,19 @@ function fromContext(ctx) {
     };
 }
 
-/* compile(ctx): the plan summary without a device (js/pt_gpu.js:226 fields) */
+/* compile(ctx): the plan summary without a device (js/pt_gpu.js:226 fields; counts summed
+ * over the datasets) */
 function compile(ctx) {
-    var r = native().planCreate(fromContext(ctx), false), i = r.info;
-    native().planDestroy(r.handle);
-    return { nv: i.nv, chains: i.chains, stride: i.stride, pointCount: i.pointCount, reflectionCount: i.reflectionCount,
-             columnCount: i.lineCount, workCount: i.pointCount, datasets: 1, paramCount: i.paramCount };
+    var flat = fromContext(ctx), list = Array.isArray(flat) ? flat : [flat], out = null;
+    list.forEach(function (f) {
+        var r = native().planCreate(f, false), i = r.info;
+        native().planDestroy(r.handle);
+        if (!out) out = { nv: i.nv, chains: i.chains, stride: i.stride, pointCount: 0, reflectionCount: 0, columnCount: 0,
+                          workCount: 0, datasets: list.length, paramCount: 0 };
+        out.pointCount += i.pointCount; out.reflectionCount += i.reflectionCount; out.columnCount += i.lineCount;
+        out.workCount += i.pointCount; out.paramCount += i.paramCount;
+    });
+    return out;
 }
 
 function flatten(vectors, nv) {

@@ -276,7 +276,9 @@ class Problem:
 
 
 def classify(name):
-    """Kind of a parameter name, js/rietveld_core.js:470-484 classify."""
+    """Kind of a parameter name, js/rietveld_core.js:470-484 classify (a variable label
+    name@d of a multi-dataset problem classifies as name)."""
+    name = name.split('@')[0]
     if name in ('zero', 'disp', 'transp'):
         return name, None
     if name.startswith('bkg'):
@@ -841,6 +843,89 @@ def separated_probe(size='isotropic', strain='isotropic', family='gsas_cw', simu
         pb.observe(simulate, x_true, seed + 7)
         pb.x0 = list(_perturb(pb, x_true, seed + 11, frac=0.15))
     return pb
+
+
+def link_datasets(problems, shared, own, limits=None):
+    """Give several datasets one variable list (a multi-dataset PT problem,
+    js/rietveld_core.js:2271-2273 with ctx.info[j].ds): first the `shared` parameter names,
+    one variable each acting in every dataset, then the names own[d] of dataset d alone,
+    labelled name@d.  Start values come from the first dataset holding the name; boxes
+    from `limits` (name -> (lo, hi); a cell length defaults to +-3 %), PT scales from the
+    reference's natural step."""
+    limits = dict(limits or {})
+    entries = [(n, None) for n in shared] + [(n, d) for d, names in enumerate(own) for n in names]
+    labels, x0, lo, hi, sc = [], [], [], [], []
+    for n, d in entries:
+        src = problems[0 if d is None else d]
+        v = float(src.base[src.index[n]])
+        kind, comp = classify(n)
+        l, h = limits.get(n, (0.97 * v, 1.03 * v) if kind == 'cell' and comp in ('a', 'b', 'c') else (-math.inf, math.inf))
+        labels.append(n if d is None else '%s@%d' % (n, d))
+        x0.append(min(h, max(l, v)))
+        lo.append(l)
+        hi.append(h)
+        sc.append(pt_scale(n, v))
+    for d, pb in enumerate(problems):
+        pb.vars, pb.terms = list(labels), []
+        for j, (n, dd) in enumerate(entries):
+            if dd is None or dd == d:
+                k = pb.index[n]
+                pb.base[k] = 0.0
+                pb.terms.append((k, j, 1.0))
+        pb.x0, pb.lo, pb.hi, pb.scale = list(x0), list(lo), list(hi), list(sc)
+    return problems
+
+
+def xray_neutron_pair(simulate=None, seed=20260909, chains=32):
+    """Two datasets of the Pnma phase of pbso4_pnma refined together: Cu Ka1/Ka2 X-ray
+    10-100 deg step 0.03 (TCH U..Y 18/-9/5.4/5/2) and constant-wavelength neutron
+    1.594 A 10-150 deg step 0.05 (TCH 30/-20/12/3/1, scattering lengths), each with its
+    own zero, scale, U, W and 3-term background.  Shared: a, b, c, 4 coordinates, 2 B."""
+    rng = np.random.default_rng(seed)
+    r, t = S.group_ops(62)
+    cell = [8.48, 5.40, 6.96, 90, 90, 90]
+    blen = {'Pb': 9.405, 'S': 2.847, 'O': 5.803}
+    atoms = []
+    for (lab, el, xyz, B) in PBSO4_SITES:
+        site = S.site_symmetry(r, t, xyz, cell)
+        pos = list(site['pos'])
+        for f in site['free']:
+            pos[f] += rng.uniform(-0.03, 0.03)
+        site['pos'] = pos
+        atoms.append(Atom(lab, el, site, B=float(rng.uniform(0.5, 2.0)), b_neutron=blen[el]))
+    xr = Problem(_grid(10.0, 100.0, 0.03), [Phase('p1', 62, cell, atoms, 'tch_aniso')], lambda1=CU_KA1, lambda2=CU_KA2,
+                 ratio=CU_RATIO, pol_k=1.0, n_bkg=3)
+    nt = Problem(_grid(10.0, 150.0, 0.05), [Phase('p1', 62, cell, atoms, 'tch_aniso')], lambda1=1.594, lambda2=0.0,
+                 ratio=0.0, pol_k=1.0, n_bkg=3, radiation=1)
+    for pb, prof, bk, zero in ((xr, (18.03, -9.02, 5.41, 5.0, 2.0), [150.0, -25.0, 8.0], 0.01),
+                               (nt, (30.0, -20.0, 12.0, 3.0, 1.0), [400.0, 60.0, -20.0], -0.02)):
+        pb.default_values()
+        for n, v in zip(('U', 'V', 'W', 'X', 'Y'), prof):
+            pb.set('p1.' + n, v)
+        pb.set('zero', zero)
+        _cheb(pb, bk)
+    coords = ['p1.%s.%s' % (at.label, 'xyz'[f]) for at in atoms[:2] for f in at.site['free']][:4]
+    shared = ['p1.a', 'p1.b', 'p1.c'] + coords + ['p1.%s.B' % at.label for at in atoms[:2]]
+    own = [['zero', 'p1.scale', 'p1.U', 'p1.W', 'bkg0'] for _ in range(2)]
+    limits = {'zero': (-1, 1), 'p1.scale': (0, math.inf)}
+    for n in coords:
+        v = float(xr.base[xr.index[n]])
+        limits[n] = (v - 0.1, v + 0.1)
+    for at in atoms[:2]:
+        limits['p1.%s.B' % at.label] = (-2, 30)
+    link_datasets([xr, nt], shared, own, limits)
+    for pb in (xr, nt):
+        pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 20, 0, 100.0, 9090
+    x_true = xr.start_vector()
+    if simulate is not None:
+        for d, pb in enumerate((xr, nt)):
+            pb.x0 = list(x_true)
+            pb.observe(simulate, x_true, seed + 7 + d)
+            # the scale rescaling touched this dataset's own scale variable in x_true
+        x_start = _perturb(xr, x_true, seed + 11, frac=0.15)
+        for pb in (xr, nt):
+            pb.x0 = list(x_start)
+    return [xr, nt]
 
 
 def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, hl=0.0, po=0.87, n_points=160):

@@ -160,6 +160,8 @@ def lib():
         L.pt_last_error.restype = ctypes.c_char_p
         L.pt_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
         L.pt_plan_compile.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.pt_plan_create_multi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.pt_plan_dataset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.pt_plan_destroy.argtypes = [ctypes.c_void_p]
         L.pt_plan_destroy.restype = None
         L.pt_plan_info_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(PtPlanInfo)]
@@ -212,14 +214,27 @@ def _check(rc):
 
 
 class Plan:
-    """A compiled PT plan (pt_plan_create, or pt_plan_compile with device=False)."""
+    """A compiled PT plan (pt_plan_create, or pt_plan_compile with device=False).  A list
+    of problems (datasets sharing one variable list) makes a multi-dataset plan
+    (pt_plan_create_multi): its costs and PT run on the sum of the datasets' chi^2."""
 
     def __init__(self, pb, device=True):
         self.problem = pb
-        self.cp = CProblem(pb)
         h = ctypes.c_void_p()
         L = lib()
-        _check((L.pt_plan_create if device else L.pt_plan_compile)(self.cp.ref(), ctypes.byref(h)))
+        if isinstance(pb, (list, tuple)):
+            if not device:
+                raise PTError('a multi-dataset plan needs the device')
+            self.cps = [CProblem(q) for q in pb]
+            arr = (PtProblem * len(pb))()
+            for i, cp in enumerate(self.cps):
+                arr[i] = cp.struct
+            self._arr = arr
+            _check(L.pt_plan_create_multi(ctypes.cast(arr, ctypes.c_void_p), len(pb), ctypes.byref(h)))
+            self.cp = self.cps[0]
+        else:
+            self.cp = CProblem(pb)
+            _check((L.pt_plan_create if device else L.pt_plan_compile)(self.cp.ref(), ctypes.byref(h)))
         self.h = h
         info = PtPlanInfo()
         _check(L.pt_plan_info_get(self.h, ctypes.byref(info)))
@@ -284,6 +299,17 @@ class Plan:
         x = self._vectors(x)
         out = np.zeros(self.info['point_count'])
         _check(lib().pt_pattern(self.h, _dptr(x[0]), _dptr(out)))
+        return out
+
+    def dataset_pattern(self, d, x):
+        """Calculated pattern of dataset d of a multi-dataset plan at one vector."""
+        x = self._vectors(x)
+        sub = ctypes.c_void_p()
+        _check(lib().pt_plan_dataset(self.h, int(d), ctypes.byref(sub)))
+        info = PtPlanInfo()
+        _check(lib().pt_plan_info_get(sub, ctypes.byref(info)))
+        out = np.zeros(info.point_count)
+        _check(lib().pt_pattern(sub, _dptr(x[0]), _dptr(out)))
         return out
 
     def workload(self, x):

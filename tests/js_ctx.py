@@ -14,7 +14,41 @@ def _num(v):
     return None if not math.isfinite(v) else v
 
 
+def _values(pb):
+    values = {}
+    p = pb.values()
+    for name, k in pb.index.items():
+        if '._prof' in name:
+            continue
+        values[name] = float(p[k])
+    return values
+
+
+def _dataset(pb):
+    anom = [[[at.fp, at.fpp] for at in ph.atoms] for ph in pb.phases]
+    ds = {'def': {'tth': pb.tth.tolist(), 'y': pb.y.tolist(), 'w': pb.w.tolist(), 'lambda1': pb.lambda1,
+                  'lambda2': pb.lambda2, 'ratio': pb.ratio, 'polK': pb.pol_k, 'minFwhm': pb.min_fwhm,
+                  'radiation': 'neutron' if pb.radiation else 'xray', 'resolutionType': pb.resolution_type},
+          'has': [True] * len(pb.phases), 'anom': anom}
+    if getattr(pb, 'bkg_type', PR.BKG_CHEBYSHEV) == PR.BKG_SPLINE:
+        ds.update(spline=True, T=[], bkgX=[float(v) for v in pb.bkg_x])
+    else:
+        ds.update(spline=False, T=[0] * pb.n_bkg, bkgX=[])
+    return ds
+
+
 def reference_ctx(pb, initial_cost=None):
+    if isinstance(pb, (list, tuple)):
+        # several datasets over the first one's phases: values per dataset; a variable
+        # label name@d is the reference's variable `name` acting in dataset d only
+        # (ctx.info[j].ds), a plain label acts in every dataset
+        ctx = reference_ctx(pb[0], initial_cost)
+        ctx['problem']['values'] = [_values(q) for q in pb]
+        ctx['problem']['vars'] = [{'name': n.split('@')[0]} for n in pb[0].vars]
+        ctx['model']['datasets'] = [_dataset(q) for q in pb]
+        for j, n in enumerate(pb[0].vars):
+            ctx['info'][j]['ds'] = [int(n.split('@')[1])] if '@' in n else list(range(len(pb)))
+        return ctx
     values = {}
     p = pb.values()
     for name, k in pb.index.items():

@@ -77,3 +77,17 @@ def test_device_calls_fail_loudly_without_gpu():
     with pt.Plan(pb, device=False) as pl:
         with pytest.raises(pt.PTError, match='without a device'):
             pl.costs(pb.start_vector())
+
+
+def test_multi_dataset_plan_needs_one_variable_list():
+    """pt_plan_create_multi refuses datasets whose variable counts differ (checked before
+    any device work)."""
+    a = PR.nacl_fm3m()
+    b = PR.nacl_fm3m()
+    b.refine('p1.po')
+    ca, cb = pt.CProblem(a), pt.CProblem(b)
+    arr = (pt.PtProblem * 2)()
+    arr[0], arr[1] = ca.struct, cb.struct
+    h = ctypes.c_void_p()
+    rc = pt.lib().pt_plan_create_multi(ctypes.cast(arr, ctypes.c_void_p), 2, ctypes.byref(h))
+    assert rc < 0 and b'variable list' in pt.lib().pt_last_error()

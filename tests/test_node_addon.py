@@ -143,6 +143,40 @@ def test_node_costs_separated_profile_match_oracle(tmp_path):
 
 @needs_node
 @pytest.mark.gpu
+def test_node_costs_two_datasets_match_oracle(tmp_path):
+    """A context with an X-ray and a neutron dataset (variables acting in one or both,
+    ctx.info[j].ds): the costs are the sums of the oracle's per-dataset costs."""
+    pbs = PR.xray_neutron_pair(simulate=O.oracle_simulate)
+    ms = [O.Model(pb) for pb in pbs]
+    probes = _probes(pbs[0])
+    path = js_ctx.write_ctx(pbs, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])
+    out, r = _node(path, 'costs')
+    assert 'costs' in out, (out, r.stderr)
+    for c, g in enumerate(out['costs']):
+        cpu = sum(m.cost(probes[c % len(probes)]) for m in ms)
+        assert abs(g - cpu) / max(1.0, cpu) <= 1e-5
+
+
+@needs_node
+def test_compile_two_dataset_context(tmp_path):
+    pbs = PR.xray_neutron_pair()
+    for pb in pbs:
+        pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pbs, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'compile')
+    assert 'compile' in out, r.stderr
+    c = out['compile']
+    infos = []
+    for pb in pbs:
+        with pt.Plan(pb, device=False) as pl:
+            infos.append(pl.info)
+    assert c['datasets'] == 2 and c['nv'] == infos[0]['nv']
+    assert c['pointCount'] == sum(i['point_count'] for i in infos)
+    assert c['reflectionCount'] == sum(i['reflection_count'] for i in infos)
+
+
+@needs_node
+@pytest.mark.gpu
 def test_node_run_pt(tmp_path):
     """tests/webgpu-pt.js:131-136 through the Node host: proposals = chains*sweeps*nv,
     swaps > 0, best <= initial, validation error reported."""

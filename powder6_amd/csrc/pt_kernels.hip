@@ -1476,13 +1476,14 @@ struct PlainScratch {
     Shared sh;
 };
 
+template <bool SEP>
 __global__ void __launch_bounds__(256) k_plain_setup(DevPlan P, const double *__restrict__ X, PlainScratch *scr) {
     __shared__ Shared sh;
     stage_reps(P, sh);
     __syncthreads();
     materialize(P, X, sh);
     __syncthreads();
-    trial_setup(P, sh);
+    trial_setup<SEP>(P, sh);
     // copy the shared image out (byte-wise, all threads)
     const int n = (int)(sizeof(Shared) / 4);
     const int *src = reinterpret_cast<const int *>(&sh);
@@ -1490,11 +1491,12 @@ __global__ void __launch_bounds__(256) k_plain_setup(DevPlan P, const double *__
     for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+template <bool SEP>
 __global__ void __launch_bounds__(256) k_plain_lines(DevPlan P, const PlainScratch *__restrict__ scr, LineRec *rec, int4 *span) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.nr) return;
     if (!scr->sh.cell_ok) { span[2 * k] = make_int4(0, 0, 0, 0); span[2 * k + 1] = make_int4(0, 0, 0, 0); return; }
-    build_reflection(P, scr->sh, k, rec + 2 * k, span + 2 * k);
+    build_reflection<SEP>(P, scr->sh, k, rec + 2 * k, span + 2 * k);
 }
 
 // each workgroup: 4 wavefronts = 4 tiles of 64 points; line records streamed through LDS
@@ -2112,8 +2114,13 @@ hipError_t launch_copy_probe(const void *a, void *b, size_t bytes, hipStream_t s
 hipError_t launch_plain(const DevPlan &P, const double *X, void *scratch, LineRec *rec, int4 *span, double *partial,
                         double *out, int ch, hipStream_t s) {
     PlainScratch *scr = reinterpret_cast<PlainScratch *>(scratch);
-    hipLaunchKernelGGL(k_plain_setup, dim3(1), dim3(256), 0, s, P, X, scr);
-    hipLaunchKernelGGL(k_plain_lines, dim3((P.nr + 255) / 256), dim3(256), 0, s, P, scr, rec, span);
+    if (P.any_sep) {
+        hipLaunchKernelGGL(k_plain_setup<true>, dim3(1), dim3(256), 0, s, P, X, scr);
+        hipLaunchKernelGGL(k_plain_lines<true>, dim3((P.nr + 255) / 256), dim3(256), 0, s, P, scr, rec, span);
+    } else {
+        hipLaunchKernelGGL(k_plain_setup<false>, dim3(1), dim3(256), 0, s, P, X, scr);
+        hipLaunchKernelGGL(k_plain_lines<false>, dim3((P.nr + 255) / 256), dim3(256), 0, s, P, scr, rec, span);
+    }
     int nblk = (P.np + 255) / 256;
     size_t dyn = (sizeof(LineRec) + sizeof(int4)) * (size_t)ch + 256 * sizeof(float);
     hipLaunchKernelGGL(k_plain_points, dim3(nblk), dim3(256), dyn, s, P, scr, rec, span, partial, ch);

@@ -119,6 +119,11 @@ typedef struct {
     int32_t size_model;    /* PT_SIZE_* */
     int32_t strain_model;  /* PT_STRAIN_* */
     double size_axis[3], strain_axis[3];
+    /* 1: a Le Bail or WPPF phase (P.method 'lebail' / 'wppf'): every reflection carries a
+     * free intensity >= 0 instead of scale |F|^2 Lp PO, and each trial's chi^2 is the
+     * least one over those intensities (js/pt_gpu.js:174-186, 259-297 per-trial NNLS;
+     * js/rietveld_core.js:1019-1090).  Atoms, scale and po of the phase are not used. */
+    int32_t free_intensities;
 } pt_phase;
 
 /* The whole PT problem of one dataset.  Model parameters are an affine function of

@@ -804,6 +804,11 @@ struct or_model {
     double **T;                      /* Chebyshev T_k(x_i) */
     double *bx;                      /* spline anchors (PT_BKG_SPLINE) */
     double guard_lo, guard_hi;
+    /* columns of the free-intensity reflections of the current evaluation */
+    int nf, fcap;
+    int *fstart, *flen;
+    double *fvals;
+    size_t fvals_n, fvals_cap;
     int has_key;
     double *key;                     /* cache key: zero, disp, transp, cells */
     /* scratch */
@@ -900,6 +905,7 @@ void or_destroy(or_model *m) {
     free(m->T); free(m->bx); free(m->atoms); free(m->op_r); free(m->op_t); free(m->ph); free(m->phases);
     free(m->tth); free(m->y); free(m->w); free(m->base); free(m->lo); free(m->hi); free(m->scale);
     free(m->x0); free(m->coeff); free(m->tparam); free(m->tvar); free(m->key); free(m->params);
+    free(m->fstart); free(m->flen); free(m->fvals);
     free(m->ybuf); free(m->col); free(m);
 }
 
@@ -1166,6 +1172,22 @@ static int eval_phase(or_model *m, int q, const double *p, double *y, refl_work 
             for (int i = R->cstart[pass]; i < R->cstop[pass]; i++)
                 col[i - R->start] += wgt * or_eval_voigt(m->tth[i], &R->v[pass]);
         }
+        if (ph->free_intensities) {
+            /* a free intensity: keep the column for the per-evaluation NNLS */
+            if (m->nf == m->fcap) {
+                m->fcap = m->fcap ? 2 * m->fcap : 64;
+                m->fstart = (int *)realloc(m->fstart, sizeof(int) * m->fcap);
+                m->flen = (int *)realloc(m->flen, sizeof(int) * m->fcap);
+            }
+            if (m->fvals_n + len > m->fvals_cap) {
+                m->fvals_cap = 2 * (m->fvals_n + len);
+                m->fvals = (double *)realloc(m->fvals, sizeof(double) * m->fvals_cap);
+            }
+            m->fstart[m->nf] = R->start; m->flen[m->nf] = len;
+            memcpy(m->fvals + m->fvals_n, col, sizeof(double) * len);
+            m->fvals_n += len; m->nf++;
+            continue;
+        }
         double hk = R->height;
         if (!(hk != 0)) continue;
         for (int i = 0; i < len; i++) y[R->start + i] += hk * col[i];
@@ -1226,6 +1248,118 @@ void or_spline_values(const double *tth, int n, const double *xs, const double *
     free(h); free(mm);
 }
 
+/* ---- free intensities (Le Bail / WPPF phases) ----------------------------------- */
+/* Cholesky solve of the passive block of a symmetric positive definite H (row major,
+ * m x m): H_PP z_P = b_P, z zero elsewhere.  Returns 0, or -1 when the block is not
+ * positive definite. */
+static int passive_solve(int m, const double *H, const double *b, const int *P, double *z, double *L, int *idx) {
+    int p = 0;
+    for (int j = 0; j < m; j++) { z[j] = 0; if (P[j]) idx[p++] = j; }
+    for (int i = 0; i < p; i++)
+        for (int j = 0; j <= i; j++) {
+            double a = H[idx[i] * m + idx[j]];
+            for (int k = 0; k < j; k++) a -= L[i * p + k] * L[j * p + k];
+            if (i == j) { if (!(a > 0)) return -1; L[i * p + i] = sqrt(a); }
+            else L[i * p + j] = a / L[j * p + j];
+        }
+    double *t = (double *)malloc(sizeof(double) * (p ? p : 1));
+    for (int i = 0; i < p; i++) {
+        double a = b[idx[i]];
+        for (int k = 0; k < i; k++) a -= L[i * p + k] * t[k];
+        t[i] = a / L[i * p + i];
+    }
+    for (int i = p - 1; i >= 0; i--) {
+        double a = t[i];
+        for (int k = i + 1; k < p; k++) a -= L[k * p + i] * t[k];
+        t[i] = a / L[i * p + i];
+    }
+    for (int i = 0; i < p; i++) z[idx[i]] = t[i];
+    free(t);
+    return 0;
+}
+
+/* min over x >= 0 of x'Hx - 2 b'x for a symmetric positive semidefinite H (the normal
+ * equations of a weighted least-squares fit): Lawson and Hanson's active-set algorithm
+ * (Solving Least Squares Problems, 1974, ch. 23), with the columns equilibrated to unit
+ * diagonal first as js/rietveld_core.js:1078-1082 does.  Columns with no weight
+ * (H_jj <= 0) stay at zero.  Returns the outer iterations, or -1. */
+int or_nnls_normal(int m, const double *H0, const double *b0, double *x) {
+    if (m <= 0) return 0;
+    double *H = (double *)malloc(sizeof(double) * m * m), *b = (double *)malloc(sizeof(double) * m);
+    double *nrm = (double *)malloc(sizeof(double) * m), *w = (double *)malloc(sizeof(double) * m);
+    double *z = (double *)malloc(sizeof(double) * m), *L = (double *)malloc(sizeof(double) * m * m);
+    int *P = (int *)calloc(m, sizeof(int)), *idx = (int *)malloc(sizeof(int) * m), *blocked = (int *)calloc(m, sizeof(int));
+    for (int j = 0; j < m; j++) { nrm[j] = H0[j * m + j] > 0 ? sqrt(H0[j * m + j]) : 0; if (!(nrm[j] > 0)) blocked[j] = 1; x[j] = 0; }
+    double bmax = 0;
+    for (int j = 0; j < m; j++) {
+        b[j] = blocked[j] ? 0 : b0[j] / nrm[j];
+        if (fabs(b[j]) > bmax) bmax = fabs(b[j]);
+        for (int k = 0; k < m; k++) H[j * m + k] = blocked[j] || blocked[k] ? (j == k ? 1 : 0) : H0[j * m + k] / (nrm[j] * nrm[k]);
+    }
+    const double tol = 1e-12 * (bmax > 1 ? bmax : 1);
+    int it = 0, rc = 0;
+    for (; it < 3 * m + 10; it++) {
+        for (int j = 0; j < m; j++) { double g = b[j]; for (int k = 0; k < m; k++) g -= H[j * m + k] * x[k]; w[j] = g; }
+        int jmax = -1;
+        for (int j = 0; j < m; j++) if (!P[j] && !blocked[j] && w[j] > tol && (jmax < 0 || w[j] > w[jmax])) jmax = j;
+        if (jmax < 0) break;
+        P[jmax] = 1;
+        int inner = 0;
+        for (; inner < 3 * m + 10; inner++) {
+            if (passive_solve(m, H, b, P, z, L, idx)) { rc = -1; break; }
+            int allpos = 1;
+            for (int j = 0; j < m; j++) if (P[j] && !(z[j] > 0)) { allpos = 0; break; }
+            if (allpos) { memcpy(x, z, sizeof(double) * m); break; }
+            double alpha = 2;
+            for (int j = 0; j < m; j++)
+                if (P[j] && !(z[j] > 0)) { double a = x[j] / (x[j] - z[j]); if (a < alpha) alpha = a; }
+            for (int j = 0; j < m; j++) if (P[j]) x[j] += alpha * (z[j] - x[j]);
+            for (int j = 0; j < m; j++) if (P[j] && !(x[j] > 0)) { P[j] = 0; x[j] = 0; }
+        }
+        if (rc) break;
+        /* a column that could not stay positive would be picked again: keep it out */
+        if (!P[jmax]) blocked[jmax] = 1;
+    }
+    for (int j = 0; j < m; j++) x[j] = P[j] && nrm[j] > 0 ? x[j] / nrm[j] : 0;
+    free(H); free(b); free(nrm); free(w); free(z); free(L); free(P); free(idx); free(blocked);
+    return rc ? -1 : it;
+}
+
+/* the free intensities of the evaluation (js/pt_gpu.js:259-297; rietveld_core.js:1057-1090
+ * wppf): the target y_obs - (background + Rietveld phases), the weighted normal
+ * equations of the free columns, their non-negative least-squares solution, and the
+ * columns rendered at it into y */
+static void free_intensities(or_model *m, double *y) {
+    const int nf = m->nf;
+    double *H = (double *)calloc((size_t)nf * nf, sizeof(double)), *b = (double *)calloc(nf, sizeof(double));
+    double *x = (double *)calloc(nf, sizeof(double));
+    size_t *off = (size_t *)malloc(sizeof(size_t) * nf);
+    size_t o = 0;
+    for (int j = 0; j < nf; j++) { off[j] = o; o += m->flen[j]; }
+    for (int j = 0; j < nf; j++) {
+        const double *cj = m->fvals + off[j];
+        const int sj = m->fstart[j], ej = sj + m->flen[j];
+        double r = 0;
+        for (int i = sj; i < ej; i++) r += cj[i - sj] * m->w[i] * (m->y[i] - y[i]);
+        b[j] = r;
+        for (int k = 0; k <= j; k++) {
+            const double *ck = m->fvals + off[k];
+            const int sk = m->fstart[k], ek = sk + m->flen[k];
+            const int lo = sj > sk ? sj : sk, hi = ej < ek ? ej : ek;
+            double a = 0;
+            for (int i = lo; i < hi; i++) a += cj[i - sj] * ck[i - sk] * m->w[i];
+            H[j * nf + k] = H[k * nf + j] = a;
+        }
+    }
+    or_nnls_normal(nf, H, b, x);
+    for (int j = 0; j < nf; j++) {
+        if (!(x[j] > 0)) continue;
+        const double *cj = m->fvals + off[j];
+        for (int i = 0; i < m->flen[j]; i++) y[m->fstart[j] + i] += x[j] * cj[i];
+    }
+    free(H); free(b); free(x); free(off);
+}
+
 /* Model.background (:920-931) + evalDataset (:1152-1164) */
 static int pattern_at_params(or_model *m, const double *p, double *y) {
     int n = m->P.n_points;
@@ -1245,12 +1379,14 @@ static int pattern_at_params(or_model *m, const double *p, double *y) {
     }
     /* each phase renders into its own array, then addTo(y, ph.y) */
     double *yp = (double *)malloc(sizeof(double) * n);
+    m->nf = 0; m->fvals_n = 0;
     for (int q = 0; q < m->P.n_phases; q++) {
         for (int i = 0; i < n; i++) yp[i] = 0;
         eval_phase(m, q, p, yp, NULL, NULL);
         for (int i = 0; i < n; i++) y[i] += yp[i];
     }
     free(yp);
+    if (m->nf) free_intensities(m, y);
     return 0;
 }
 

@@ -63,6 +63,9 @@ int or_prepare_voigt_sep(int type, const double *prof, const or_sep *sep, double
 /* calculateProfileWidths of a separated profile (floored), side 0 left / 1 right */
 void or_gsas_widths(int type, const or_sep *sep, double lambda, double min_fwhm, double tth, int side,
                     double *gamma_g, double *gamma_l);
+/* min over x >= 0 of x'Hx - 2b'x (H: m x m row major, symmetric positive semidefinite),
+ * Lawson-Hanson active set on equilibrated columns; returns outer iterations or -1 */
+int or_nnls_normal(int m, const double *H, const double *b, double *x);
 int or_stephens_basis(int n_ops, const int32_t *op_r, int32_t *keys, double *coeff);   /* profile.js:334-363 */
 double or_axis_angle(const double cell[6], const double n[3], const double axis[3]);  /* profile.js:390 */
 double or_stephens_sum(const double h[3], const double *s15, int n_basis, const int32_t *keys,

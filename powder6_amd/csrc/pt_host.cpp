@@ -389,6 +389,7 @@ int validate(const pt_problem *pb) {
         const pt_phase &ph = pb->phases[q];
         if (ph.n_ops < 1 || !ph.op_r || !ph.op_t) return fail(PT_ERR_ARG, "a phase has no symmetry operators");
         if (ph.profile_type < 1 || ph.profile_type > 4) return fail(PT_ERR_ARG, "unknown profile type");
+        if (ph.free_intensities != 0 && ph.free_intensities != 1) return fail(PT_ERR_ARG, "free_intensities must be 0 or 1");
         if (ph.separated || ph.profile_type == PT_PROFILE_GSAS_CW) {
             if (ph.size_model < 0 || ph.size_model > 2 || ph.strain_model < 0 || ph.strain_model > 2)
                 return fail(PT_ERR_ARG, "unknown size or strain broadening model");
@@ -664,6 +665,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     std::vector<float> sbasis((size_t)PTK_MAX_PHASES * 225, 0.0f);
     std::vector<double> sguard;
     ptk::DevSepPlan sep_plan{};
+    std::vector<int> free_k;             // envelope index of every free-intensity reflection
     ptk::DevPlan &P = pl->P;
     P.any_sep = 0; P.has_stephens = 0;
     for (int q = 0; q < pb->n_phases; ++q) {
@@ -720,6 +722,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
             for (int i = 0; i < 3; ++i) o[i] = ok ? (float)a[i] : (i == 2 ? 1.0f : 0.0f);
         }
         sd.stephens = 0; sd.n_sdir = 0; sd.sdir_off = 0;
+        sd.free_ph = ph.free_intensities ? 1 : 0;
         dp.n_reps = (int)reps.size();
         dp.rep_off = (int)(rep_r.size() / 9);
         dp.n_atoms = ph.n_atoms;
@@ -771,10 +774,18 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         }
         if (sd.sep) P.any_sep = 1;
         if (sd.stephens) P.has_stephens = 1;
+        if (sd.free_ph)
+            for (int j = 0; j < dp.refl_count; ++j) free_k.push_back(dp.refl_off + j);
         pl->phase_hklm.push_back(std::move(rows));
     }
     pl->nr = (int)hklm.size();
     if (!pl->nr) { delete pl; return fail(PT_ERR_ARG, "No reflections for PT."); }
+    if ((int)free_k.size() > PTK_MAX_FREE) {
+        delete pl;
+        return fail(PT_ERR_BUDGET, "PT has more than 120 potential free reflection intensities in one dataset; using the CPU solver.");
+    }
+    P.has_free = free_k.empty() ? 0 : 1;
+    P.n_free = (int)free_k.size();
 
     // grid
     double t0 = pb->tth[0], step = (pb->tth[n - 1] - pb->tth[0]) / (n - 1);
@@ -865,6 +876,8 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     std::vector<double> vlo(pb->lo, pb->lo + pb->n_vars), vhi(pb->hi, pb->hi + pb->n_vars), vsc(pb->scale, pb->scale + pb->n_vars);
     if (eq.empty()) eq.push_back(0);
     if (sguard.empty()) sguard.assign(15, 0.0);
+    if (free_k.empty()) free_k.push_back(0);
+    int *d_free_k = nullptr;
     float *d_sbasis = nullptr; double *d_sguard = nullptr;
     double *d_tth, *d_bknots = nullptr; float2 *d_tff, *d_obs; float *d_zcheb; int4 *d_hklm; int *d_bseg = nullptr;
     float4 *d_bcoef = nullptr; int *d_eq_off, *d_rep_r, *d_term_off, *d_term_var;
@@ -879,12 +892,15 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         (rc = upload(pl, &d_base, base)) || (rc = upload(pl, &d_term_off, term_off)) || (rc = upload(pl, &d_term_var, term_var)) ||
         (rc = upload(pl, &d_coeff, term_coeff)) || (rc = upload(pl, &d_lo, vlo)) || (rc = upload(pl, &d_hi, vhi)) ||
         (rc = upload(pl, &d_sc, vsc)) || (rc = upload(pl, &d_sbasis, sbasis)) || (rc = upload(pl, &d_sguard, sguard)) ||
+        (rc = upload(pl, &d_free_k, free_k)) ||
         (rc = dalloc(pl, &d_state, (size_t)pl->chains * pl->stride))) {
         pt_plan_destroy(pl);
         return rc;
     }
     P.bkg_spline = pb->bkg_type == PT_BKG_SPLINE ? 1 : 0; P.bseg = d_bseg; P.bcoef = d_bcoef; P.bknots = d_bknots;
     {
+        P.free_k = d_free_k;
+        if (P.has_free) HIPCHECK(ptk::nnls_lds_limit(ptk::nnls_shared_bytes(P.n_free)), "kernel setup");
         sep_plan.sbasis = d_sbasis; sep_plan.sguard = d_sguard;
         ptk::DevSepPlan *d_sp = nullptr;
         if ((rc = upload(pl, &d_sp, std::vector<ptk::DevSepPlan>{sep_plan}))) { pt_plan_destroy(pl); return rc; }
@@ -1014,6 +1030,14 @@ static int ensure_slots(pt_plan *pl, int n) {
         (rc = dalloc(pl, &pl->d_srec, (size_t)n * nl)) || (rc = dalloc(pl, &pl->d_sspan, (size_t)n * nl)) ||
         (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->ngroups)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
         return rc;
+    if (pl->P.has_free) {
+        // free intensities: their line spans and the residual they are fitted to, per slot
+        if ((rc = dalloc(pl, &pl->P.fspan, (size_t)n * nl)) || (rc = dalloc(pl, &pl->P.resid, (size_t)n * pl->np)) ||
+            (rc = dalloc(pl, &pl->P.fx, (size_t)n * pl->P.n_free)))
+            return rc;
+        HIPCHECK(hipMemset(pl->P.fx, 0, sizeof(double) * (size_t)n * pl->P.n_free), "intensity start");
+        drop_graph(pl->gexec, pl->graph);
+    }
     pl->slots = n;
     if (!pl->follow.empty()) {
         // the further datasets get as many slots, and the first dataset's finish their buffers
@@ -1043,9 +1067,11 @@ static int run_stages(pt_plan *pl, const double *dX, int ntrials, int pt_mode, i
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[0], pl->stream), "timing");
     HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[1], pl->stream), "timing");
+    if (pl->P.has_free) HIPCHECK(ptk::launch_stage_nnls(pl->P, pl->d_img, pl->d_srec, pl->d_part, pl->ngroups, ntrials, pl->stream), "intensities");
     for (pt_plan *f : pl->follow) {
         HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_srec, f->d_sspan, ntrials, pl->stream), "lines");
         HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_srec, f->d_sspan, f->d_part, f->ngroups, ntrials, f->tpw, pl->stream), "accumulate");
+        if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_srec, f->d_part, f->ngroups, ntrials, pl->stream), "intensities");
     }
     HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, ntrials, d_chi2, pt_mode, pl->stream), "finish");
     return 0;
@@ -1247,10 +1273,12 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
         HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, n, pl->tpw, st, t0),
                  "accumulate");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], st), "timing");
+        if (pl->P.has_free) HIPCHECK(ptk::launch_stage_nnls(pl->P, pl->d_img, pl->d_srec, pl->d_part, pl->ngroups, n, st, t0), "intensities");
         for (pt_plan *f : pl->follow) {
             HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_srec, f->d_sspan, n, st, t0), "lines");
             HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_srec, f->d_sspan, f->d_part, f->ngroups, n, f->tpw, st, t0),
                      "accumulate");
+            if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_srec, f->d_part, f->ngroups, n, st, t0), "intensities");
         }
         if (s + 1 < pl->steps_per_sweep) {
             HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, st, n, t0), "finish");
@@ -1291,6 +1319,7 @@ static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
 int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
     if (!pl) return fail(PT_ERR_ARG, "NULL plan");
     if (!pl->follow.empty() && cfg) return fail(PT_ERR_STATE, "PT surrogate screening takes one dataset per plan.");
+    if (pl->P.has_free && cfg) return fail(PT_ERR_STATE, "PT surrogate screening takes Rietveld phases only.");
     NEED_DEVICE(pl);
     (void)hipSetDevice(pl->device);
     drop_graph(pl->gexec, pl->graph);
@@ -1450,6 +1479,7 @@ int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
     (void)hipSetDevice(pl->device);
     if (form == PT_FORM_BATCHED) return eval_batched(pl, x, ntrials, chi2, true);
     if (!pl->follow.empty()) return fail(PT_ERR_STATE, "PT evaluation form not available for a multi-dataset plan (use the batched form).");
+    if (pl->P.has_free) return fail(PT_ERR_STATE, "PT evaluation form not available with free intensities (use the batched form).");
     if (form == PT_FORM_PLAIN) {
         int rc = ensure_trials(pl, ntrials);
         if (rc) return rc;
@@ -1635,6 +1665,7 @@ int pt_matrix_timing(const pt_plan *pl, double *intensity_ms, double *product_ms
 int pt_pattern(pt_plan *pl, const double *x, double *ycalc) {
     if (!pl || !x || !ycalc) return fail(PT_ERR_ARG, "NULL argument");
     NEED_DEVICE(pl);
+    if (pl->P.has_free) return fail(PT_ERR_STATE, "PT pattern not available with free intensities.");
     (void)hipSetDevice(pl->device);
     int rc = ensure_trials(pl, 1);
     if (rc) return rc;
@@ -1789,6 +1820,10 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
     for (int m = 0; m < count && !rc; ++m) {
         pt_plan *pl = nullptr;
         rc = create_impl(&problems[m], &pl, true);
+        if (!rc && pl->P.has_free) {
+            pt_plan_destroy(pl);
+            rc = fail(PT_ERR_ARG, "free intensities are not available in a group");
+        }
         if (rc) {
             g_err = "PT group member " + std::to_string(m) + ": " + g_err;
             break;

@@ -424,12 +424,17 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
 // With unit_factor != null the lines are written at unit height (the Kalpha2 line at its
 // doublet weight) and the reflection's height factor m Lp PO / area goes to
 // unit_factor[k] instead: the columns of the shared profile matrix.
+// fspan != null: the trial's span slots for free-intensity lines.  A reflection of a free
+// phase (Le Bail / WPPF) is written at unit height (its Kalpha2 line at the doublet
+// weight), its spans go to fspan and its own spans stay empty, so the accumulate stage
+// leaves it to the intensity solve.
 template <bool SEP = true>
 __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
-                                        float *unit_factor = nullptr, const int4 *hk_given = nullptr) {
+                                        float *unit_factor = nullptr, const int4 *hk_given = nullptr, int4 *fspan = nullptr) {
     if (unit_factor) unit_factor[k] = 0.0f;
     span[0] = make_int4(0, 0, 0, 0);
     span[1] = make_int4(0, 0, 0, 0);
+    if (fspan) { fspan[0] = make_int4(0, 0, 0, 0); fspan[1] = make_int4(0, 0, 0, 0); }
     int q = 0;
     while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
     const DevPhase &ph = P.phase[q];
@@ -464,7 +469,10 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
     const double *pp = sh.p + ph.param_base;
     float po = march_dollase(P, ph, f, pp[7], k);
     float height;
-    if (unit_factor) {
+    const bool free_ph = fspan && P.has_free && P.sp->ph[q].free_ph;
+    if (free_ph) {
+        height = 1.0f;
+    } else if (unit_factor) {
         unit_factor[k] = (float)hk.w * lp1 * po / pk1.area;
         height = 1.0f;
     } else {
@@ -486,6 +494,10 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
             float w2 = (float)P.ratio * pk1.area / pk2.area * (lp_factor((float)base2, (float)P.pol_k) / lp1);
             emit_line(P, pk2, base2, height * w2, rec[1], span[1]);
         }
+    }
+    if (free_ph) {
+        fspan[0] = span[0]; fspan[1] = span[1];
+        span[0] = make_int4(0, 0, 0, 0); span[1] = make_int4(0, 0, 0, 0);
     }
 }
 
@@ -872,7 +884,7 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, Li
     if (k < P.nr) {
         const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
         if (sh.go) {
-            build_reflection<SEP>(P, sh, k, rec + base, span + base, nullptr, &hk);
+            build_reflection<SEP>(P, sh, k, rec + base, span + base, nullptr, &hk, P.has_free ? P.fspan + base : nullptr);
             for (int q = 0; q < 2; ++q) {
                 const int4 sp = span[base + q];
                 if (sp.y > sp.x)
@@ -884,6 +896,7 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, Li
         } else {
             span[base] = make_int4(0, 0, 0, 0);
             span[base + 1] = make_int4(0, 0, 0, 0);
+            if (P.has_free) { P.fspan[base] = make_int4(0, 0, 0, 0); P.fspan[base + 1] = make_int4(0, 0, 0, 0); }
         }
     }
     __syncthreads();
@@ -1219,6 +1232,9 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                     const float yp = h ? (ta[j].two1 + ta[j].one1) + ta[j].gen1 : (ta[j].two0 + ta[j].one0) + ta[j].gen0;
                     const float r = ow.x - (yb + yp);
                     neumaier(s, c, ow.y * r * r);
+                    if (P.has_free) P.resid[(size_t)t * P.np + i] = r;      // the free intensities' target
+                } else if (P.has_free) {
+                    P.resid[(size_t)t * P.np + i] = 0.0f;
                 }
             }
         }
@@ -1261,6 +1277,244 @@ __device__ inline double trial_cost(const TrialImage *im, const double2 *partial
         if (!isfinite(cost) || cost > 3e38) cost = 3e38;
     }
     return cost;
+}
+
+// ---- free intensities (Le Bail / WPPF phases) -------------------------------------
+// One workgroup per trial: the weighted normal equations of the free reflections' columns
+// against the residual the accumulate stage left (y_obs - background - Rietveld phases),
+// then min over I >= 0 of I'HI - 2 b'I by Lawson and Hanson's active set (Solving Least
+// Squares Problems, 1974, ch. 23) on columns equilibrated to unit diagonal
+// (js/rietveld_core.js:1078-1082), all in FP64 in LDS.  chi^2 = c - 2 b'I + I'HI replaces
+// the trial's partial sums (js/pt_gpu.js:259-297 solves the same problem per chain).
+__host__ __device__ inline size_t nnls_lds_bytes(int m) {
+    const size_t tri = (size_t)m * (m + 1) / 2;
+    return 16 * (size_t)(2 * m) * 4 + ((tri * 2 + 6 * (size_t)m) * 8 + 15) / 16 * 16 + (size_t)m * 4 * 6 + 64;
+}
+__device__ __forceinline__ int tri_at(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+// column j (the free reflection's lines at their weights) at point i
+__device__ __forceinline__ float free_column(const DevPlan &P, const LineRec *R, const int4 *Sp, int j, int i) {
+    float v = 0.0f;
+    for (int q = 0; q < 2; ++q) {
+        const int4 sp = Sp[2 * j + q];
+        if (i >= sp.x && i < sp.y) {
+            const LineRec &L = R[2 * j + q];
+            const float2 ti = P.tff[i];
+            v += L.a.z * line_value(L, __int_as_float(sp.z), (ti.x - L.a.x) + (ti.y - L.a.y));
+        }
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const LineRec *__restrict__ rec,
+                                                    double2 *partial, int nsplit, int t_off) {
+    const int t = t_off + blockIdx.x;
+    if (!img[t].go) return;                              // not evaluated: the trial costs 3e38
+    extern __shared__ __align__(16) unsigned char dyn[];
+    __shared__ double red[256];
+    __shared__ int s_int[4];
+    __shared__ double s_tol;
+    const int m = P.n_free, tid = threadIdx.x, nt = blockDim.x, T = m * (m + 1) / 2;
+    LineRec *R = reinterpret_cast<LineRec *>(dyn);
+    int4 *Sp = reinterpret_cast<int4 *>(R + 2 * m);
+    double *H = reinterpret_cast<double *>(Sp + 2 * m);
+    double *Lf = H + T, *b = Lf + T, *x = b + m, *z = x + m, *g = z + m, *nrm = g + m, *tmp = nrm + m;
+    int *pas = reinterpret_cast<int *>(tmp + m), *blk = pas + m, *idx = blk + m, *cs = idx + m, *ce = cs + m;
+    const size_t base = (size_t)t * 2 * P.nr;
+    for (int j = tid; j < 2 * m; j += nt) {
+        const int k = P.free_k[j >> 1];
+        R[j] = rec[base + 2 * k + (j & 1)];
+        Sp[j] = P.fspan[base + 2 * k + (j & 1)];
+    }
+    __syncthreads();
+    for (int j = tid; j < m; j += nt) {
+        int lo = P.np, hi = 0;                           // a column without a window: [np, 0), empty
+        for (int q = 0; q < 2; ++q) {
+            const int4 sp = Sp[2 * j + q];
+            if (sp.y > sp.x) { lo = min(lo, sp.x); hi = max(hi, sp.y); }
+        }
+        cs[j] = lo; ce[j] = hi;
+    }
+    __syncthreads();
+    const float *r = P.resid + (size_t)t * P.np;
+    // normal matrix and right-hand side (FP64 sums of FP32 terms): one wavefront per entry,
+    // its lanes over the overlap of the two windows (coalesced point loads), then a
+    // wavefront sum; entries whose windows do not meet are zero
+    const int wv = tid >> 6, ln = tid & 63, nw = nt >> 6;
+    for (int e = wv; e < T + m; e += nw) {
+        int j, k, lo, hi;
+        if (e < T) {
+            j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+            while ((j + 1) * (j + 2) / 2 <= e) ++j;
+            while (j * (j + 1) / 2 > e) --j;
+            k = e - j * (j + 1) / 2;
+            lo = max(cs[j], cs[k]); hi = min(ce[j], ce[k]);
+        } else {
+            j = k = e - T;                               // right-hand side b_j
+            lo = cs[j]; hi = ce[j];
+        }
+        double a = 0.0;
+        if (lo < hi) {                                   // 0 <= lo, hi <= np
+            for (int i = lo + ln; i < hi; i += 64) {
+                const float w = P.obs[i].y;
+                if (w > 0.0f)
+                    a += (double)w * (double)free_column(P, R, Sp, j, i) * (double)(e < T ? free_column(P, R, Sp, k, i) : r[i]);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (ln == 0) {
+            if (e < T) H[e] = a;
+            else b[j] = a;
+        }
+    }
+    double cw = 0.0;
+    for (int i = tid; i < P.np; i += nt) {
+        const float w = P.obs[i].y;
+        if (w > 0.0f) cw += (double)w * (double)r[i] * (double)r[i];
+    }
+    red[tid] = cw;
+    __syncthreads();
+    for (int o = nt >> 1; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    const double csum = red[0];
+    // equilibrate; the slot's last solution is the warm start (any x >= 0 with its positive
+    // set passive is a valid start for the active set, and the optimum is the same)
+    double *fx = P.fx + (size_t)t * m;
+    for (int j = tid; j < m; j += nt) {
+        const double d = H[tri_at(j, j)];
+        nrm[j] = d > 0.0 ? sqrt(d) : 0.0;
+        blk[j] = d > 0.0 ? 0 : 1;
+        const double w0 = blk[j] ? 0.0 : fx[j] * nrm[j];
+        x[j] = w0 > 0.0 ? w0 : 0.0;
+        pas[j] = x[j] > 0.0 ? 1 : 0;
+    }
+    __syncthreads();
+    for (int e = tid; e < T; e += nt) {
+        int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while ((j + 1) * (j + 2) / 2 <= e) ++j;
+        while (j * (j + 1) / 2 > e) --j;
+        const int k = e - j * (j + 1) / 2;
+        H[e] = (blk[j] || blk[k]) ? (j == k ? 1.0 : 0.0) : H[e] / (nrm[j] * nrm[k]);
+    }
+    for (int j = tid; j < m; j += nt) b[j] = blk[j] ? 0.0 : b[j] / nrm[j];
+    __syncthreads();
+    if (tid == 0) {
+        double bm = 0.0;
+        for (int j = 0; j < m; ++j) bm = fmax(bm, fabs(b[j]));
+        s_tol = 1e-12 * fmax(bm, 1.0);
+        s_int[2] = 0;                                    // solve failure
+    }
+    __syncthreads();
+    const double tol = s_tol;
+    if (tid == 0) {
+        int any = 0;
+        for (int j = 0; j < m; ++j) any |= pas[j];
+        s_int[0] = any ? -2 : -1;                        // -2: first solve on the warm passive set
+    }
+    __syncthreads();
+    bool warm = s_int[0] == -2;
+    for (int it = 0; it < 3 * m + 10; ++it) {
+        if (!warm) {
+            for (int j = tid; j < m; j += nt) {
+                double a = b[j];
+                for (int k = 0; k < m; ++k) a -= H[tri_at(j, k)] * x[k];
+                g[j] = a;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int jm = -1;
+                for (int j = 0; j < m; ++j)
+                    if (!pas[j] && !blk[j] && g[j] > tol && (jm < 0 || g[j] > g[jm])) jm = j;
+                s_int[0] = jm;
+                if (jm >= 0) pas[jm] = 1;
+            }
+            __syncthreads();
+        }
+        const int jm = warm ? -2 : s_int[0];
+        warm = false;
+        if (jm == -1) break;
+        for (int inner = 0; inner < 3 * m + 10; ++inner) {
+            if (tid == 0) {
+                int p = 0;
+                for (int j = 0; j < m; ++j) if (pas[j]) idx[p++] = j;
+                s_int[1] = p;
+            }
+            __syncthreads();
+            const int p = s_int[1];
+            // Cholesky of the passive block, one column at a time (rows in parallel)
+            for (int kc = 0; kc < p; ++kc) {
+                if (tid == 0) {
+                    double a = H[tri_at(idx[kc], idx[kc])];
+                    for (int q = 0; q < kc; ++q) a -= Lf[tri_at(kc, q)] * Lf[tri_at(kc, q)];
+                    if (!(a > 0.0)) { s_int[2] = 1; a = 1.0; }
+                    Lf[tri_at(kc, kc)] = sqrt(a);
+                }
+                __syncthreads();
+                for (int i = kc + 1 + tid; i < p; i += nt) {
+                    double a = H[tri_at(idx[i], idx[kc])];
+                    for (int q = 0; q < kc; ++q) a -= Lf[tri_at(i, q)] * Lf[tri_at(kc, q)];
+                    Lf[tri_at(i, kc)] = a / Lf[tri_at(kc, kc)];
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                int stop = s_int[2];
+                if (!stop) {
+                    for (int i = 0; i < p; ++i) {
+                        double a = b[idx[i]];
+                        for (int q = 0; q < i; ++q) a -= Lf[tri_at(i, q)] * tmp[q];
+                        tmp[i] = a / Lf[tri_at(i, i)];
+                    }
+                    for (int i = p - 1; i >= 0; --i) {
+                        double a = tmp[i];
+                        for (int q = i + 1; q < p; ++q) a -= Lf[tri_at(q, i)] * tmp[q];
+                        tmp[i] = a / Lf[tri_at(i, i)];
+                    }
+                    for (int j = 0; j < m; ++j) z[j] = 0.0;
+                    for (int i = 0; i < p; ++i) z[idx[i]] = tmp[i];
+                    bool allpos = true;
+                    for (int j = 0; j < m; ++j) if (pas[j] && !(z[j] > 0.0)) { allpos = false; break; }
+                    if (allpos) {
+                        for (int j = 0; j < m; ++j) x[j] = z[j];
+                        stop = 1;
+                    } else {
+                        double alpha = 2.0;
+                        for (int j = 0; j < m; ++j)
+                            if (pas[j] && !(z[j] > 0.0)) alpha = fmin(alpha, x[j] / (x[j] - z[j]));
+                        for (int j = 0; j < m; ++j) if (pas[j]) x[j] += alpha * (z[j] - x[j]);
+                        for (int j = 0; j < m; ++j) if (pas[j] && !(x[j] > 0.0)) { pas[j] = 0; x[j] = 0.0; }
+                    }
+                }
+                s_int[3] = stop;
+            }
+            __syncthreads();
+            if (s_int[3]) break;
+        }
+        if (s_int[2]) break;                             // a block that is not positive definite
+        if (tid == 0 && jm >= 0 && !pas[jm]) blk[jm] = 1;   // could not stay positive: keep it out
+        __syncthreads();
+    }
+    for (int j = tid; j < m; j += nt) fx[j] = pas[j] && nrm[j] > 0.0 ? x[j] / nrm[j] : 0.0;
+    // chi^2 at the solution: c - sum_j x_j (2 b_j - (H x)_j), in the equilibrated frame
+    double q = 0.0;
+    for (int j = tid; j < m; j += nt) {
+        const double xj = pas[j] ? x[j] : 0.0;
+        if (xj == 0.0) continue;
+        double hx = 0.0;
+        for (int k = 0; k < m; ++k) hx += H[tri_at(j, k)] * (pas[k] ? x[k] : 0.0);
+        q += xj * (2.0 * b[j] - hx);
+    }
+    __syncthreads();
+    red[tid] = q;
+    __syncthreads();
+    for (int o = nt >> 1; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    for (int k = tid; k < nsplit; k += nt)
+        partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? csum - red[0] : 0.0, 0.0);
 }
 
 // the chi^2 of trial t over every dataset of the plan (js/pt_gpu.js:259-297: the targets
@@ -1981,6 +2235,17 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
     hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick, t_off);
     return hipGetLastError();
 }
+hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const LineRec *rec, double2 *partial, int nsplit, int ntrials,
+                             hipStream_t s, int t_off) {
+    hipLaunchKernelGGL(k_stage_nnls, dim3(ntrials), dim3(256), nnls_lds_bytes(P.n_free), s, P, (const TrialImage *)img, rec,
+                       partial, nsplit, t_off);
+    return hipGetLastError();
+}
+size_t nnls_shared_bytes(int m) { return nnls_lds_bytes(m); }
+hipError_t nnls_lds_limit(size_t bytes) {
+    return hipFuncSetAttribute((const void *)k_stage_nnls, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,
                                hipStream_t s, int t_off) {
     hipLaunchKernelGGL(k_stage_follow, dim3(ntrials), dim3(64), 0, s, P, mstate, mstride, (const TrialImage *)mimg,

@@ -27,6 +27,7 @@
 #define PTK_N_INST 18
 #define PTK_PH_NPROF 32      // profile slots of a phase block (from slot 8)
 #define PTK_PH_ATOM0 40      // first atom slot of a phase block
+#define PTK_MAX_FREE 120     // free reflection intensities per dataset (the NNLS stage holds them in LDS)
 #define PTK_PROF_LDS 48      // a trial's profile slots of a phase in LDS: the 32 model slots, then
                              // (separated profiles) the family's eta / SL / HL slots from the instrument
 #define PTK_S_COUNT_K 12
@@ -69,6 +70,7 @@ struct DevSepPhase {
     int stephens;                  // 1: the phase's rows of sbasis hold its Stephens basis
     int n_sdir, sdir_off;          // Stephens guard directions [sdir_off, sdir_off + n_sdir) of sguard
     float size_axis[3], strain_axis[3];  // uniaxial axes (Miller indices)
+    int free_ph;                   // 1: a Le Bail / WPPF phase (free reflection intensities)
 };
 // another dataset of a multi-dataset plan, as the first dataset's finish reads it: the
 // trial images and chi^2 partials of its own pipeline (js/pt_gpu.js:259-297 sums the
@@ -95,7 +97,8 @@ struct DevPlan {
     short any_sep;             // some phase has a separated profile
     short has_stephens;        // some separated phase has Stephens strain
     int n_follow;              // further datasets whose chi^2 the finish adds (first dataset only)
-    int pad1;
+    int has_free;              // some phase has free intensities (the NNLS stage runs)
+    int n_free, pad2;          // free reflections (all reflections of the free phases)
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
     double inv_step;           // 1 / step
@@ -130,6 +133,10 @@ struct DevPlan {
     const double *temp;        // temperature of chain c: max_temp^(c / (chains - 1))
     const DevSepPlan *sp;      // separated profiles (always set)
     const DevFollow *follow;   // [n_follow]
+    const int *free_k;         // [n_free] envelope index of every free reflection
+    int4 *fspan;               // per trial slot: the spans of the free reflections' lines (2 nr)
+    float *resid;              // per trial slot: y_obs - background - Rietveld phases (np)
+    double *fx;                // per trial slot: the last solve's free intensities (its warm start)
     double *state;
 };
 

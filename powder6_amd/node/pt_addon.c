@@ -176,6 +176,7 @@ static const char *convert(napi_env env, napi_value o, holder *h) {
         if (field_doubles(env, ph, "poDir", &pd, h) == 3) { P->po_dir[0] = pd[0]; P->po_dir[1] = pd[1]; P->po_dir[2] = pd[2]; }
         else { P->po_dir[2] = 1; }
         P->separated = (int32_t)num(env, ph, "separated", 0);
+        P->free_intensities = (int32_t)num(env, ph, "freeIntensities", 0);
         P->size_model = (int32_t)num(env, ph, "sizeModel", PT_SIZE_ISOTROPIC);
         P->strain_model = (int32_t)num(env, ph, "strainModel", PT_STRAIN_ISOTROPIC);
         double *ax;

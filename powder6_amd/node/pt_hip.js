@@ -99,7 +99,8 @@ function flattenDataset(ctx, d) {
             if (!multi) throw new Error('MI355X PT backend needs every phase in the dataset.');
             return;
         }
-        if (P.method !== 'rietveld') throw new Error('MI355X PT backend evaluates Rietveld phases only; using the CPU path.');
+        // Le Bail and WPPF phases carry free reflection intensities, solved per trial
+        if (P.method !== 'rietveld' && P.method !== 'lebail' && P.method !== 'wppf') throw new Error('MI355X PT backend cannot evaluate a ' + P.method + ' phase; using the CPU path.');
         var type = P.def.profileType;
         // rietveld_core.js:460 usesSeparatedProfile, :416-421 broadeningConfig
         var sep = type === 'gsas_cw' || P.def.separatedProfile === true, b = P.def.broadening || {};
@@ -115,6 +116,7 @@ function flattenDataset(ctx, d) {
             uniqueAxis: { a: 0, b: 1, c: 2 }[P.def.uniqueAxis || 'b'],
             profileType: PROFILE[type], opR: Int32Array.from(opR), opT: Float64Array.from(opT),
             separated: sep ? 1 : 0, sizeModel: SIZE_MODEL[b.size] || 0, strainModel: STRAIN_MODEL[b.strain] || 0,
+            freeIntensities: P.method === 'rietveld' ? 0 : 1,
             sizeAxis: axis(b.sizeAxis), strainAxis: axis(b.strainAxis),
             poDir: P.def.poDir || [0, 0, 1],
             atoms: atoms.map(function (at, j) {

@@ -87,8 +87,10 @@ class Phase:
     'isotropic'|'uniaxial'|'stephens', 'sizeAxis': [h, k, l], 'strainAxis': [h, k, l]}."""
 
     def __init__(self, key, group, cell, atoms, profile='tch_aniso', po_dir=(0, 0, 1), unique_axis=1, ops=None,
-                 separated=False, broadening=None):
+                 separated=False, broadening=None, method='rietveld'):
         self.key = key
+        # 'lebail' / 'wppf': free reflection intensities, fitted per trial (no atoms used)
+        self.method = method
         self.separated = bool(separated) or profile == 'gsas_cw'
         b = dict(broadening or {})
         self.size_model = b.get('size', 'isotropic')
@@ -926,6 +928,47 @@ def xray_neutron_pair(simulate=None, seed=20260909, chains=32):
         for pb in (xr, nt):
             pb.x0 = list(x_start)
     return [xr, nt]
+
+
+def lebail_probe(mixed=False, simulate=None, seed=20261010, chains=16):
+    """Free-intensity (Le Bail) refinement: NaCl Fm-3m a=5.64 over 10-120 deg step 0.02,
+    Cu Ka1/Ka2, TCH, 3-term Chebyshev; the observed pattern is the Rietveld model (Na 4a,
+    Cl 4b) with Poisson noise, the fitted phase a Le Bail phase whose intensities are free.
+    With mixed, a second phase ZnS F-43m a=5.41 (Zn 4a, S 4c) joins as a Rietveld phase.
+    Refined: a, zero, U, V, W, X, Y, bkg0, bkg1 (and the ZnS scale and a)."""
+    r, t = S.group_ops(225)
+    cell = [5.64, 5.64, 5.64, 90, 90, 90]
+    atoms = [Atom('Na1', 'Na', S.site_symmetry(r, t, [0, 0, 0], cell), B=1.1),
+             Atom('Cl1', 'Cl', S.site_symmetry(r, t, [0.5, 0.5, 0.5], cell), B=0.9)]
+    phases = [Phase('p1', 225, cell, atoms, 'tch_aniso')]
+    if mixed:
+        r2, t2 = S.group_ops(216)
+        c2 = [5.41, 5.41, 5.41, 90, 90, 90]
+        a2 = [Atom('Zn1', 'Zn', S.site_symmetry(r2, t2, [0, 0, 0], c2), B=0.8),
+              Atom('S1', 'S', S.site_symmetry(r2, t2, [0.25, 0.25, 0.25], c2), B=0.7)]
+        phases.append(Phase('p2', 216, c2, a2, 'tch_aniso'))
+    pb = Problem(_grid(10.0, 120.0, 0.02), phases, lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0, n_bkg=3)
+    pb.default_values()
+    for ph in phases:
+        for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
+            pb.set(ph.key + '.' + n, v)
+    if mixed:
+        pb.set('p2.scale', 0.6)
+    pb.set('zero', 0.012)
+    _cheb(pb, [150.0, -30.0, 10.0])
+    names = ['p1.a', 'zero', 'p1.U', 'p1.V', 'p1.W', 'p1.X', 'p1.Y', 'bkg0', 'bkg1'] + (['p2.a', 'p2.scale'] if mixed else [])
+    limits = {'p1.a': (5.64 * 0.98, 5.64 * 1.02), 'p2.a': (5.41 * 0.98, 5.41 * 1.02), 'zero': (-1, 1), 'p2.scale': (0, math.inf),
+              'p1.X': (0, math.inf), 'p1.Y': (0, math.inf)}
+    for n in names:
+        lo, hi = limits.get(n, (-math.inf, math.inf))
+        pb.refine(n, lo, hi)
+    pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = chains, 20, 0, 100.0, 1010
+    x_true = pb.start_vector()
+    if simulate is not None:
+        pb.observe(simulate, x_true, seed)
+        pb.x0 = list(_perturb(pb, x_true, seed + 1, frac=0.15))
+    phases[0].method = 'lebail'          # the observation came from its atoms; the fit frees them
+    return pb
 
 
 def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, hl=0.0, po=0.87, n_points=160):

@@ -39,7 +39,8 @@ class PtPhase(ctypes.Structure):
     _fields_ = [('system', c_int32), ('unique_axis', c_int32), ('profile_type', c_int32), ('n_ops', c_int32),
                 ('op_r', P_I), ('op_t', P_D), ('po_dir', c_double * 3), ('n_atoms', c_int32),
                 ('atoms', ctypes.POINTER(PtAtom)), ('separated', c_int32), ('size_model', c_int32),
-                ('strain_model', c_int32), ('size_axis', c_double * 3), ('strain_axis', c_double * 3)]
+                ('strain_model', c_int32), ('size_axis', c_double * 3), ('strain_axis', c_double * 3),
+                ('free_intensities', c_int32)]
 
 
 class PtProblem(ctypes.Structure):
@@ -114,6 +115,7 @@ class CProblem:
             P.n_atoms = len(ph.atoms)
             P.atoms = ctypes.cast(atoms, ctypes.POINTER(PtAtom))
             P.separated = 1 if getattr(ph, 'separated', False) else 0
+            P.free_intensities = 0 if getattr(ph, 'method', 'rietveld') == 'rietveld' else 1
             P.size_model = PR.SIZE_MODELS[getattr(ph, 'size_model', 'isotropic')]
             P.strain_model = PR.STRAIN_MODELS[getattr(ph, 'strain_model', 'isotropic')]
             for x in range(3):

@@ -71,7 +71,7 @@ def reference_ctx(pb, initial_cost=None):
             pdef['separatedProfile'] = True
             pdef['broadening'] = {'size': ph.size_model, 'strain': ph.strain_model,
                                   'sizeAxis': list(ph.size_axis), 'strainAxis': list(ph.strain_axis)}
-        phases.append({'def': pdef, 'method': 'rietveld', 'prefix': ph.key + '.'})
+        phases.append({'def': pdef, 'method': getattr(ph, 'method', 'rietveld'), 'prefix': ph.key + '.'})
     anom = [[[at.fp, at.fpp] for at in ph.atoms] for ph in pb.phases]
     ds = {'def': {'tth': pb.tth.tolist(), 'y': pb.y.tolist(), 'w': pb.w.tolist(), 'lambda1': pb.lambda1,
                   'lambda2': pb.lambda2, 'ratio': pb.ratio, 'polK': pb.pol_k, 'minFwhm': pb.min_fwhm,

@@ -91,3 +91,12 @@ def test_multi_dataset_plan_needs_one_variable_list():
     h = ctypes.c_void_p()
     rc = pt.lib().pt_plan_create_multi(ctypes.cast(arr, ctypes.c_void_p), 2, ctypes.byref(h))
     assert rc < 0 and b'variable list' in pt.lib().pt_last_error()
+
+
+def test_free_intensity_budget():
+    """More free reflection intensities than the per-trial solve holds is refused with the
+    reference's budget phrase (js/pt_gpu.js:175), before any device work."""
+    pb = PR.pbso4_pnma()
+    pb.phases[0].method = 'lebail'
+    with pytest.raises(pt.PTError, match='free reflection intensities'):
+        pt.Plan(pb, device=False)

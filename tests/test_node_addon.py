@@ -177,6 +177,33 @@ def test_compile_two_dataset_context(tmp_path):
 
 @needs_node
 @pytest.mark.gpu
+def test_node_costs_lebail_phase_match_oracle(tmp_path):
+    """A Le Bail phase next to a Rietveld phase: the free intensities are solved per trial."""
+    pb = PR.lebail_probe(mixed=True, simulate=O.oracle_simulate)
+    m = O.Model(pb)
+    probes = _probes(pb)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])
+    out, r = _node(path, 'costs')
+    assert 'costs' in out, (out, r.stderr)
+    for c, g in enumerate(out['costs']):
+        cpu = m.cost(probes[c % len(probes)])
+        assert abs(g - cpu) / max(1.0, cpu) <= 1e-5
+
+
+@needs_node
+def test_compile_lebail_context(tmp_path):
+    pb = PR.lebail_probe(mixed=True)
+    pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'compile')
+    assert 'compile' in out, r.stderr
+    with pt.Plan(pb, device=False) as pl:
+        info = pl.info
+    assert out['compile']['reflectionCount'] == info['reflection_count']
+
+
+@needs_node
+@pytest.mark.gpu
 def test_node_run_pt(tmp_path):
     """tests/webgpu-pt.js:131-136 through the Node host: proposals = chains*sweeps*nv,
     swaps > 0, best <= initial, validation error reported."""

@@ -930,7 +930,7 @@ def xray_neutron_pair(simulate=None, seed=20260909, chains=32):
     return [xr, nt]
 
 
-def lebail_probe(mixed=False, simulate=None, seed=20261010, chains=16):
+def lebail_probe(mixed=False, simulate=None, seed=20261010, chains=16, profile='tch_aniso', method='lebail'):
     """Free-intensity (Le Bail) refinement: NaCl Fm-3m a=5.64 over 10-120 deg step 0.02,
     Cu Ka1/Ka2, TCH, 3-term Chebyshev; the observed pattern is the Rietveld model (Na 4a,
     Cl 4b) with Poisson noise, the fitted phase a Le Bail phase whose intensities are free.
@@ -940,7 +940,7 @@ def lebail_probe(mixed=False, simulate=None, seed=20261010, chains=16):
     cell = [5.64, 5.64, 5.64, 90, 90, 90]
     atoms = [Atom('Na1', 'Na', S.site_symmetry(r, t, [0, 0, 0], cell), B=1.1),
              Atom('Cl1', 'Cl', S.site_symmetry(r, t, [0.5, 0.5, 0.5], cell), B=0.9)]
-    phases = [Phase('p1', 225, cell, atoms, 'tch_aniso')]
+    phases = [Phase('p1', 225, cell, atoms, profile)]
     if mixed:
         r2, t2 = S.group_ops(216)
         c2 = [5.41, 5.41, 5.41, 90, 90, 90]
@@ -949,16 +949,23 @@ def lebail_probe(mixed=False, simulate=None, seed=20261010, chains=16):
         phases.append(Phase('p2', 216, c2, a2, 'tch_aniso'))
     pb = Problem(_grid(10.0, 120.0, 0.02), phases, lambda1=CU_KA1, lambda2=CU_KA2, ratio=CU_RATIO, pol_k=1.0, n_bkg=3)
     pb.default_values()
+    widths = {'tch_aniso': ('U', 'V', 'W', 'X', 'Y'), 'simple_pvoigt': ('GU', 'GV', 'GW', 'GP', 'LX'),
+              'split_pvoigt': ('GU_L', 'GV_L', 'GW_L', 'GU_R', 'LX_L'), 'gsas_cw': ('inst.U', 'inst.V', 'inst.W', 'inst.X', 'inst.Y')}
     for ph in phases:
-        for n, v in zip(('U', 'V', 'W', 'X', 'Y'), (18.03, -9.02, 5.41, 5.0, 2.0)):
-            pb.set(ph.key + '.' + n, v)
+        names_w = widths[ph.profile]
+        for n, v in zip(names_w, (18.03, -9.02, 5.41, 5.0 if ph.profile != 'simple_pvoigt' else 0.0, 2.0)):
+            pb.set(n if n.startswith('inst.') else ph.key + '.' + n, v)
+    if profile == 'gsas_cw':
+        pb.set('p1.size', 0.5)
     if mixed:
         pb.set('p2.scale', 0.6)
     pb.set('zero', 0.012)
     _cheb(pb, [150.0, -30.0, 10.0])
-    names = ['p1.a', 'zero', 'p1.U', 'p1.V', 'p1.W', 'p1.X', 'p1.Y', 'bkg0', 'bkg1'] + (['p2.a', 'p2.scale'] if mixed else [])
+    w1 = [n if n.startswith('inst.') else 'p1.' + n for n in widths[profile]]
+    names = ['p1.a', 'zero'] + w1 + ['bkg0', 'bkg1'] + (['p2.a', 'p2.scale'] if mixed else [])
     limits = {'p1.a': (5.64 * 0.98, 5.64 * 1.02), 'p2.a': (5.41 * 0.98, 5.41 * 1.02), 'zero': (-1, 1), 'p2.scale': (0, math.inf),
-              'p1.X': (0, math.inf), 'p1.Y': (0, math.inf)}
+              'p1.X': (0, math.inf), 'p1.Y': (0, math.inf), 'p1.LX_L': (0, math.inf), 'p1.GP': (-1, math.inf),
+              'inst.X': (0, math.inf), 'inst.Y': (0, math.inf)}
     for n in names:
         lo, hi = limits.get(n, (-math.inf, math.inf))
         pb.refine(n, lo, hi)
@@ -967,11 +974,12 @@ def lebail_probe(mixed=False, simulate=None, seed=20261010, chains=16):
     if simulate is not None:
         pb.observe(simulate, x_true, seed)
         pb.x0 = list(_perturb(pb, x_true, seed + 1, frac=0.15))
-    phases[0].method = 'lebail'          # the observation came from its atoms; the fit frees them
+    phases[0].method = method            # the observation came from its atoms; the fit frees them
     return pb
 
 
-def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, hl=0.0, po=0.87, n_points=160):
+def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, hl=0.0, po=0.87, n_points=160,
+                    method='rietveld', broadening=None, lambda2=1.544, inst=None, span=None):
     """The one-atom P-1 problem of tests/webgpu-pt.js:11-36: a=3.7 b=4.1 c=4.3 alpha=87
     beta=94 gamma=102, C at (.13,.21,.32) with the 4-Gaussian f0 given there; 160 points
     8-63.65 deg (step 0.35), w 0.01 (point 5 excluded), lambda 1.540598/1.544 ratio .5,
@@ -982,11 +990,15 @@ def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, h
     cell = [3.7, 4.1, 4.3, 87, 94, 102]
     at = Atom('C1', 'C', S.site_symmetry(ops[0], ops[1], [.13, .21, .32], cell), B=0.5, ff=TRICLINIC_FF,
               b_neutron=6.646)
-    ph = Phase('p1', 2, cell, [at], profile, po_dir=(0, 0, 1))
-    tth = 8 + np.arange(n_points, dtype=np.float64) * .35
-    pb = Problem(tth, [ph], lambda1=1.540598, lambda2=1.544, ratio=.5, pol_k=.83, min_fwhm=.07, n_bkg=2,
+    sep = profile == 'gsas_cw' or broadening is not None
+    ph = Phase('p1', 2, cell, [at], profile, po_dir=(0, 0, 1), separated=sep, broadening=broadening)
+    step = .35 if span is None else span / (n_points - 1)
+    tth = 8 + np.arange(n_points, dtype=np.float64) * step
+    pb = Problem(tth, [ph], lambda1=1.540598, lambda2=lambda2, ratio=.5, pol_k=.83, min_fwhm=.07, n_bkg=2,
                  radiation=1 if neutron else 0)
     pb.default_values()
+    for n, v in (inst or {}).items():
+        pb.set('inst.' + n, v)
     pb.set('zero', .03)
     pb.set('disp', 1.5)
     pb.set('transp', .00001)
@@ -999,9 +1011,15 @@ def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, h
         pb.set('p1.HL', hl)
     names = ['p1.a', 'p1.b', 'p1.c', 'zero', 'disp', 'transp', 'bkg0', 'p1.scale', 'p1.po', 'p1.C1.x', 'p1.C1.y',
              'p1.C1.z', 'p1.C1.B', 'p1.C1.occ']
-    for n in ['GU', 'LX', 'eta', 'GU_L', 'GU_R', 'LX_L', 'LX_R', 'eta_split', 'U', 'W', 'X', 'Y', 'SL', 'HL']:
-        if 'p1.' + n in pb.index and n in PROFILE_SLOTS[profile]:
-            names.append('p1.' + n)
+    if sep:
+        names += ['inst.W', 'inst.X', 'p1.size']
+    else:
+        for n in ['GU', 'LX', 'eta', 'GU_L', 'GU_R', 'LX_L', 'LX_R', 'eta_split', 'U', 'W', 'X', 'Y', 'SL', 'HL']:
+            if 'p1.' + n in pb.index and n in PROFILE_SLOTS[profile]:
+                names.append('p1.' + n)
+    if method != 'rietveld':
+        # a Le Bail or WPPF phase has no scale, orientation or atoms (rietveld_core.js:540-545)
+        names = [n for n in names if n not in ('p1.scale', 'p1.po') and classify(n)[0] != 'atom']
     for n in names:
         kind, comp = classify(n)
         if kind == 'cell':
@@ -1021,6 +1039,8 @@ def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, h
             pb.refine(n, 0, 1)
         elif kind == 'profile' and comp in ('LX', 'LX_L', 'LX_R', 'X', 'Y', 'GP'):
             pb.refine(n, 0, math.inf)
+        elif n == 'p1.size':
+            pb.refine(n, 0.002, math.inf)
         else:
             pb.refine(n)
     pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = 4, 10, 0, 5.0, 137
@@ -1030,4 +1050,5 @@ def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, h
         pb.y = ycalc + 2 * np.sin(i * .13) + 1
         pb.w = np.full(n_points, .01)
         pb.w[5] = 0
+    ph.method = method                   # observed from the atoms; a free phase then frees them
     return pb

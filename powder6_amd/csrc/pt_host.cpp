@@ -782,7 +782,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     if (!pl->nr) { delete pl; return fail(PT_ERR_ARG, "No reflections for PT."); }
     if ((int)free_k.size() > PTK_MAX_FREE) {
         delete pl;
-        return fail(PT_ERR_BUDGET, "PT has more than 120 potential free reflection intensities in one dataset; using the CPU solver.");
+        return fail(PT_ERR_BUDGET, "PT has more than 2048 potential free reflection intensities in one dataset; using the CPU solver.");
     }
     P.has_free = free_k.empty() ? 0 : 1;
     P.n_free = (int)free_k.size();

@@ -1286,9 +1286,10 @@ __device__ inline double trial_cost(const TrialImage *im, const double2 *partial
 // Squares Problems, 1974, ch. 23) on columns equilibrated to unit diagonal
 // (js/rietveld_core.js:1078-1082), all in FP64 in LDS.  chi^2 = c - 2 b'I + I'HI replaces
 // the trial's partial sums (js/pt_gpu.js:259-297 solves the same problem per chain).
-__host__ __device__ inline size_t nnls_lds_bytes(int m) {
+// LDS for up to m live columns out of nf free reflections
+__host__ __device__ inline size_t nnls_lds_bytes(int m, int nf) {
     const size_t tri = (size_t)m * (m + 1) / 2;
-    return 16 * (size_t)(2 * m) * 4 + ((tri * 2 + 6 * (size_t)m) * 8 + 15) / 16 * 16 + (size_t)m * 4 * 6 + 64;
+    return 16 * (size_t)(2 * m) * 4 + ((tri * 2 + 6 * (size_t)m) * 8 + 15) / 16 * 16 + (size_t)m * 4 * 6 + (size_t)nf * 4 + 64;
 }
 __device__ __forceinline__ int tri_at(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
@@ -1314,15 +1315,37 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
     __shared__ double red[256];
     __shared__ int s_int[4];
     __shared__ double s_tol;
-    const int m = P.n_free, tid = threadIdx.x, nt = blockDim.x, T = m * (m + 1) / 2;
-    LineRec *R = reinterpret_cast<LineRec *>(dyn);
-    int4 *Sp = reinterpret_cast<int4 *>(R + 2 * m);
-    double *H = reinterpret_cast<double *>(Sp + 2 * m);
-    double *Lf = H + T, *b = Lf + T, *x = b + m, *z = x + m, *g = z + m, *nrm = g + m, *tmp = nrm + m;
-    int *pas = reinterpret_cast<int *>(tmp + m), *blk = pas + m, *idx = blk + m, *cs = idx + m, *ce = cs + m;
+    const int nf = P.n_free, cap = min(nf, PTK_MAX_FREE_LIVE), tid = threadIdx.x, nt = blockDim.x;
     const size_t base = (size_t)t * 2 * P.nr;
+    // the live columns of this trial: free reflections with a line window in the pattern,
+    // in envelope order (the others have no data and stay at zero)
+    LineRec *R = reinterpret_cast<LineRec *>(dyn);
+    int4 *Sp = reinterpret_cast<int4 *>(R + 2 * cap);
+    const int Tc = cap * (cap + 1) / 2;
+    double *H = reinterpret_cast<double *>(Sp + 2 * cap);
+    double *Lf = H + Tc, *b = Lf + Tc, *x = b + cap, *z = x + cap, *g = z + cap, *nrm = g + cap, *tmp = nrm + cap;
+    int *pas = reinterpret_cast<int *>(tmp + cap), *blk = pas + cap, *idx = blk + cap, *cs = idx + cap, *ce = cs + cap;
+    int *live = ce + cap, *flag = live + cap;
+    for (int gi = tid; gi < nf; gi += nt) {
+        const int k = P.free_k[gi];
+        const int4 s0 = P.fspan[base + 2 * k], s1 = P.fspan[base + 2 * k + 1];
+        flag[gi] = (s0.y > s0.x || s1.y > s1.x) ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int n = 0;
+        for (int gi = 0; gi < nf; ++gi)
+            if (flag[gi]) { if (n < cap) live[n] = gi; ++n; }
+        s_int[0] = n;
+    }
+    __syncthreads();
+    const int m = s_int[0], T = m * (m + 1) / 2;
+    if (m > cap) {                                       // more live intensities than LDS holds
+        for (int k = tid; k < nsplit; k += nt) partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? 3e38 : 0.0, 0.0);
+        return;
+    }
     for (int j = tid; j < 2 * m; j += nt) {
-        const int k = P.free_k[j >> 1];
+        const int k = P.free_k[live[j >> 1]];
         R[j] = rec[base + 2 * k + (j & 1)];
         Sp[j] = P.fspan[base + 2 * k + (j & 1)];
     }
@@ -1381,12 +1404,12 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
     const double csum = red[0];
     // equilibrate; the slot's last solution is the warm start (any x >= 0 with its positive
     // set passive is a valid start for the active set, and the optimum is the same)
-    double *fx = P.fx + (size_t)t * m;
+    double *fx = P.fx + (size_t)t * nf;
     for (int j = tid; j < m; j += nt) {
         const double d = H[tri_at(j, j)];
         nrm[j] = d > 0.0 ? sqrt(d) : 0.0;
         blk[j] = d > 0.0 ? 0 : 1;
-        const double w0 = blk[j] ? 0.0 : fx[j] * nrm[j];
+        const double w0 = blk[j] ? 0.0 : fx[live[j]] * nrm[j];
         x[j] = w0 > 0.0 ? w0 : 0.0;
         pas[j] = x[j] > 0.0 ? 1 : 0;
     }
@@ -1496,7 +1519,8 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
         if (tid == 0 && jm >= 0 && !pas[jm]) blk[jm] = 1;   // could not stay positive: keep it out
         __syncthreads();
     }
-    for (int j = tid; j < m; j += nt) fx[j] = pas[j] && nrm[j] > 0.0 ? x[j] / nrm[j] : 0.0;
+    for (int gi = tid; gi < nf; gi += nt) if (!flag[gi]) fx[gi] = 0.0;
+    for (int j = tid; j < m; j += nt) fx[live[j]] = pas[j] && nrm[j] > 0.0 ? x[j] / nrm[j] : 0.0;
     // chi^2 at the solution: c - sum_j x_j (2 b_j - (H x)_j), in the equilibrated frame
     double q = 0.0;
     for (int j = tid; j < m; j += nt) {
@@ -2237,11 +2261,11 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
 }
 hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const LineRec *rec, double2 *partial, int nsplit, int ntrials,
                              hipStream_t s, int t_off) {
-    hipLaunchKernelGGL(k_stage_nnls, dim3(ntrials), dim3(256), nnls_lds_bytes(P.n_free), s, P, (const TrialImage *)img, rec,
-                       partial, nsplit, t_off);
+    hipLaunchKernelGGL(k_stage_nnls, dim3(ntrials), dim3(256), nnls_lds_bytes(min(P.n_free, PTK_MAX_FREE_LIVE), P.n_free), s, P,
+                       (const TrialImage *)img, rec, partial, nsplit, t_off);
     return hipGetLastError();
 }
-size_t nnls_shared_bytes(int m) { return nnls_lds_bytes(m); }
+size_t nnls_shared_bytes(int nf) { return nnls_lds_bytes(std::min(nf, PTK_MAX_FREE_LIVE), nf); }
 hipError_t nnls_lds_limit(size_t bytes) {
     return hipFuncSetAttribute((const void *)k_stage_nnls, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }

@@ -27,7 +27,8 @@
 #define PTK_N_INST 18
 #define PTK_PH_NPROF 32      // profile slots of a phase block (from slot 8)
 #define PTK_PH_ATOM0 40      // first atom slot of a phase block
-#define PTK_MAX_FREE 120     // free reflection intensities per dataset (the NNLS stage holds them in LDS)
+#define PTK_MAX_FREE 2048    // free reflections per dataset (the envelope of the free phases)
+#define PTK_MAX_FREE_LIVE 120  // free reflections with a window in one trial (the NNLS stage holds them in LDS)
 #define PTK_PROF_LDS 48      // a trial's profile slots of a phase in LDS: the 32 model slots, then
                              // (separated profiles) the family's eta / SL / HL slots from the instrument
 #define PTK_S_COUNT_K 12

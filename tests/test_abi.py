@@ -94,9 +94,10 @@ def test_multi_dataset_plan_needs_one_variable_list():
 
 
 def test_free_intensity_budget():
-    """More free reflection intensities than the per-trial solve holds is refused with the
-    reference's budget phrase (js/pt_gpu.js:175), before any device work."""
-    pb = PR.pbso4_pnma()
+    """More than 2048 potential free reflection intensities in a dataset is refused with
+    the reference's budget phrase (js/pt_gpu.js:175), before any device work (the
+    molecular P2_1/c envelope holds 2,730 reflections)."""
+    pb = PR.molecular_p21c()
     pb.phases[0].method = 'lebail'
     with pytest.raises(pt.PTError, match='free reflection intensities'):
         pt.Plan(pb, device=False)

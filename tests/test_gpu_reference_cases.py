@@ -2,8 +2,8 @@
 HIP path, each against the FP64 oracle at 1e-5 instead of the page's 1e-3: every method
 and profile family, the separated broadening models, a distant second wavelength, the
 negative-variance continuation, the variance guard under PT and a 12,000-point pattern.
-Free-intensity (Le Bail / WPPF) phases run on the NaCl pattern: the page's P-1 probe has
-544 envelope reflections, beyond the 120 free intensities the per-trial solve holds."""
+Free-intensity (Le Bail / WPPF) phases run on the page's P-1 probe (544 envelope
+reflections, about 60 with a window in a trial) and on the NaCl pattern."""
 import numpy as np
 import pytest
 
@@ -40,6 +40,7 @@ def test_rietveld_each_family(family):
 @pytest.mark.parametrize('method', ['wppf', 'lebail'])
 @pytest.mark.parametrize('family', FAMILIES)
 def test_free_intensities_each_family(method, family):
+    assert _check(PR.triclinic_probe(family, simulate=O.oracle_simulate, method=method)) <= TOL
     assert _check(PR.lebail_probe(simulate=O.oracle_simulate, profile=family, method=method)) <= TOL
 
 

@@ -269,6 +269,9 @@ struct pt_plan {
     // the first dataset's finish adds their chi^2 through d_follow
     std::vector<pt_plan *> follow;
     ptk::DevFollow *d_follow = nullptr;
+    // the plan's separated-profile / free-intensity description (device copy and host image)
+    ptk::DevSepPlan sp_host{};
+    ptk::DevSepPlan *d_sp = nullptr;
     hipEvent_t ev_total[2] = {nullptr, nullptr};
     double last_total_ms = 0, last_kernel_ms = 0, last_matrix_ms[2] = {0, 0};
     int last_launches = 0;
@@ -899,12 +902,13 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     }
     P.bkg_spline = pb->bkg_type == PT_BKG_SPLINE ? 1 : 0; P.bseg = d_bseg; P.bcoef = d_bcoef; P.bknots = d_bknots;
     {
-        P.free_k = d_free_k;
         if (P.has_free) HIPCHECK(ptk::nnls_lds_limit(ptk::nnls_shared_bytes(P.n_free)), "kernel setup");
-        sep_plan.sbasis = d_sbasis; sep_plan.sguard = d_sguard;
+        sep_plan.sbasis = d_sbasis; sep_plan.sguard = d_sguard; sep_plan.free_k = d_free_k;
         ptk::DevSepPlan *d_sp = nullptr;
         if ((rc = upload(pl, &d_sp, std::vector<ptk::DevSepPlan>{sep_plan}))) { pt_plan_destroy(pl); return rc; }
         P.sp = d_sp;
+        pl->sp_host = sep_plan;
+        pl->d_sp = d_sp;
     }
     P.tth = d_tth; P.tff = d_tff; P.zcheb = d_zcheb; P.obs = d_obs; P.hklm = d_hklm; P.eq_off = d_eq_off; P.eq = d_eq;
     P.rep_r = d_rep_r; P.rep_t = d_rep_t; P.atoms = d_atoms; P.base = d_base; P.term_off = d_term_off;
@@ -1032,10 +1036,12 @@ static int ensure_slots(pt_plan *pl, int n) {
         return rc;
     if (pl->P.has_free) {
         // free intensities: their line spans and the residual they are fitted to, per slot
-        if ((rc = dalloc(pl, &pl->P.fspan, (size_t)n * nl)) || (rc = dalloc(pl, &pl->P.resid, (size_t)n * pl->np)) ||
-            (rc = dalloc(pl, &pl->P.fx, (size_t)n * pl->P.n_free)))
+        ptk::DevSepPlan &h = pl->sp_host;
+        if ((rc = dalloc(pl, &h.fspan, (size_t)n * nl)) || (rc = dalloc(pl, &h.resid, (size_t)n * pl->np)) ||
+            (rc = dalloc(pl, &h.fx, (size_t)n * pl->P.n_free)))
             return rc;
-        HIPCHECK(hipMemset(pl->P.fx, 0, sizeof(double) * (size_t)n * pl->P.n_free), "intensity start");
+        HIPCHECK(hipMemset(h.fx, 0, sizeof(double) * (size_t)n * pl->P.n_free), "intensity start");
+        HIPCHECK(hipMemcpy(pl->d_sp, &h, sizeof h, hipMemcpyHostToDevice), "intensity buffers");
         drop_graph(pl->gexec, pl->graph);
     }
     pl->slots = n;

@@ -469,7 +469,7 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
     const double *pp = sh.p + ph.param_base;
     float po = march_dollase(P, ph, f, pp[7], k);
     float height;
-    const bool free_ph = fspan && P.has_free && P.sp->ph[q].free_ph;
+    const bool free_ph = SEP && fspan && P.has_free && P.sp->ph[q].free_ph;
     if (free_ph) {
         height = 1.0f;
     } else if (unit_factor) {
@@ -884,7 +884,7 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, Li
     if (k < P.nr) {
         const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
         if (sh.go) {
-            build_reflection<SEP>(P, sh, k, rec + base, span + base, nullptr, &hk, P.has_free ? P.fspan + base : nullptr);
+            build_reflection<SEP>(P, sh, k, rec + base, span + base, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr);
             for (int q = 0; q < 2; ++q) {
                 const int4 sp = span[base + q];
                 if (sp.y > sp.x)
@@ -896,7 +896,7 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, Li
         } else {
             span[base] = make_int4(0, 0, 0, 0);
             span[base + 1] = make_int4(0, 0, 0, 0);
-            if (P.has_free) { P.fspan[base] = make_int4(0, 0, 0, 0); P.fspan[base + 1] = make_int4(0, 0, 0, 0); }
+            if (SEP && P.has_free) { P.sp->fspan[base] = make_int4(0, 0, 0, 0); P.sp->fspan[base + 1] = make_int4(0, 0, 0, 0); }
         }
     }
     __syncthreads();
@@ -924,7 +924,8 @@ __global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img,
     __shared__ int gr[PTK_MAX_GROUPS][2];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    if (P.any_sep) lines_body<true>(P, img, rec, span, t_off + t, u, sh, gr);
+    // the lean instance for plans with neither separated profiles nor free intensities
+    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, span, t_off + t, u, sh, gr);
     else lines_body<false>(P, img, rec, span, t_off + t, u, sh, gr);
 }
 
@@ -1142,7 +1143,8 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
 // order, into LDS; every wavefront then sums them for its own TPW 128-point tiles (two
 // points per lane) and the chi^2 of every 512-point group is reduced with two-sum into
 // partial[t * nsplit + group], nsplit = the trial's 512-point group count.
-template <int TPW>
+// FREE: the plan has free intensities, so the residual of every point is kept for their solve
+template <int TPW, bool FREE = false>
 __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict__ img, const LineRec *__restrict__ grec,
                                   const int4 *__restrict__ gspan, double2 *partial, int nsplit, int t, int split,
                                   AccShared &S) {
@@ -1232,9 +1234,9 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
                     const float yp = h ? (ta[j].two1 + ta[j].one1) + ta[j].gen1 : (ta[j].two0 + ta[j].one0) + ta[j].gen0;
                     const float r = ow.x - (yb + yp);
                     neumaier(s, c, ow.y * r * r);
-                    if (P.has_free) P.resid[(size_t)t * P.np + i] = r;      // the free intensities' target
-                } else if (P.has_free) {
-                    P.resid[(size_t)t * P.np + i] = 0.0f;
+                    if (FREE) P.sp->resid[(size_t)t * P.np + i] = r;      // the free intensities' target
+                } else if (FREE) {
+                    P.sp->resid[(size_t)t * P.np + i] = 0.0f;
                 }
             }
         }
@@ -1257,14 +1259,14 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     }
 }
 
-template <int TPW>
+template <int TPW, bool FREE>
 __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
                                                      const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
                                                      double2 *partial, int nsplit, int per, int ntrials, int t_off) {
     __shared__ AccShared S;
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    accum_body<TPW>(P, img, grec, gspan, partial, nsplit, t_off + t, u, S);
+    accum_body<TPW, FREE>(P, img, grec, gspan, partial, nsplit, t_off + t, u, S);
 }
 
 // chi^2 of trial t: the ordered sum of its partials (3e38 for an invalid trial)
@@ -1327,8 +1329,8 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
     int *pas = reinterpret_cast<int *>(tmp + cap), *blk = pas + cap, *idx = blk + cap, *cs = idx + cap, *ce = cs + cap;
     int *live = ce + cap, *flag = live + cap;
     for (int gi = tid; gi < nf; gi += nt) {
-        const int k = P.free_k[gi];
-        const int4 s0 = P.fspan[base + 2 * k], s1 = P.fspan[base + 2 * k + 1];
+        const int k = P.sp->free_k[gi];
+        const int4 s0 = P.sp->fspan[base + 2 * k], s1 = P.sp->fspan[base + 2 * k + 1];
         flag[gi] = (s0.y > s0.x || s1.y > s1.x) ? 1 : 0;
     }
     __syncthreads();
@@ -1345,9 +1347,9 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
         return;
     }
     for (int j = tid; j < 2 * m; j += nt) {
-        const int k = P.free_k[live[j >> 1]];
+        const int k = P.sp->free_k[live[j >> 1]];
         R[j] = rec[base + 2 * k + (j & 1)];
-        Sp[j] = P.fspan[base + 2 * k + (j & 1)];
+        Sp[j] = P.sp->fspan[base + 2 * k + (j & 1)];
     }
     __syncthreads();
     for (int j = tid; j < m; j += nt) {
@@ -1359,7 +1361,7 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
         cs[j] = lo; ce[j] = hi;
     }
     __syncthreads();
-    const float *r = P.resid + (size_t)t * P.np;
+    const float *r = P.sp->resid + (size_t)t * P.np;
     // normal matrix and right-hand side (FP64 sums of FP32 terms): one wavefront per entry,
     // its lanes over the overlap of the two windows (coalesced point loads), then a
     // wavefront sum; entries whose windows do not meet are zero
@@ -1404,7 +1406,7 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
     const double csum = red[0];
     // equilibrate; the slot's last solution is the warm start (any x >= 0 with its positive
     // set passive is a valid start for the active set, and the optimum is the same)
-    double *fx = P.fx + (size_t)t * nf;
+    double *fx = P.sp->fx + (size_t)t * nf;
     for (int j = tid; j < m; j += nt) {
         const double d = H[tri_at(j, j)];
         nrm[j] = d > 0.0 ? sqrt(d) : 0.0;
@@ -2289,12 +2291,13 @@ hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *
     const int per = (ngroups + tpw - 1) / tpw;
     const dim3 grid(xcd_grid(per, ntrials));
     const TrialImage *im = (const TrialImage *)img;
-    if (tpw == 1)
-        hipLaunchKernelGGL(k_stage_accum<1>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off);
-    else if (tpw == 4)
-        hipLaunchKernelGGL(k_stage_accum<4>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off);
-    else
-        hipLaunchKernelGGL(k_stage_accum<2>, grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off);
+#define PTK_ACCUM(T, F) hipLaunchKernelGGL((k_stage_accum<T, F>), grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off)
+    if (P.has_free) {
+        if (tpw == 1) PTK_ACCUM(1, true); else if (tpw == 4) PTK_ACCUM(4, true); else PTK_ACCUM(2, true);
+    } else {
+        if (tpw == 1) PTK_ACCUM(1, false); else if (tpw == 4) PTK_ACCUM(4, false); else PTK_ACCUM(2, false);
+    }
+#undef PTK_ACCUM
     return hipGetLastError();
 }
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,

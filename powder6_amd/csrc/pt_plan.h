@@ -87,6 +87,12 @@ struct DevSepPlan {
     DevSepPhase ph[PTK_MAX_PHASES];
     const float *sbasis;           // Stephens basis per phase, [phase][key 15][coefficient 15]
     const double *sguard;          // Stephens guard: per direction the 15 key coefficients
+    // free intensities (set when the plan has free phases; the per-slot arrays follow the
+    // trial slots)
+    const int *free_k;             // [n_free] envelope index of every free reflection
+    int4 *fspan;                   // per trial slot: the spans of the free reflections' lines (2 nr)
+    float *resid;                  // per trial slot: y_obs - background - Rietveld phases (np)
+    double *fx;                    // per trial slot: the last solve's free intensities (its warm start)
 };
 
 struct DevPlan {
@@ -132,12 +138,8 @@ struct DevPlan {
     const double *term_coeff;
     const double *vlo, *vhi, *vscale;
     const double *temp;        // temperature of chain c: max_temp^(c / (chains - 1))
-    const DevSepPlan *sp;      // separated profiles (always set)
+    const DevSepPlan *sp;      // separated profiles and free intensities (always set)
     const DevFollow *follow;   // [n_follow]
-    const int *free_k;         // [n_free] envelope index of every free reflection
-    int4 *fspan;               // per trial slot: the spans of the free reflections' lines (2 nr)
-    float *resid;              // per trial slot: y_obs - background - Rietveld phases (np)
-    double *fx;                // per trial slot: the last solve's free intensities (its warm start)
     double *state;
 };
 

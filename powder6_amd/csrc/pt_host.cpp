@@ -1040,6 +1040,11 @@ static int ensure_slots(pt_plan *pl, int n) {
         if ((rc = dalloc(pl, &h.fspan, (size_t)n * nl)) || (rc = dalloc(pl, &h.resid, (size_t)n * pl->np)) ||
             (rc = dalloc(pl, &h.fx, (size_t)n * pl->P.n_free)))
             return rc;
+        if (pl->P.n_free > PTK_MAX_FREE_LIVE) {
+            // a trial with more live intensities than LDS holds solves in its slot's workspace
+            h.nnls_stride = ptk::nnls_slot_bytes(pl->P.n_free);
+            if ((rc = dalloc(pl, &h.nnls_ws, (size_t)n * h.nnls_stride))) return rc;
+        }
         HIPCHECK(hipMemset(h.fx, 0, sizeof(double) * (size_t)n * pl->P.n_free), "intensity start");
         HIPCHECK(hipMemcpy(pl->d_sp, &h, sizeof h, hipMemcpyHostToDevice), "intensity buffers");
         drop_graph(pl->gexec, pl->graph);

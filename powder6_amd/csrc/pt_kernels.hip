@@ -1288,10 +1288,34 @@ __device__ inline double trial_cost(const TrialImage *im, const double2 *partial
 // Squares Problems, 1974, ch. 23) on columns equilibrated to unit diagonal
 // (js/rietveld_core.js:1078-1082), all in FP64 in LDS.  chi^2 = c - 2 b'I + I'HI replaces
 // the trial's partial sums (js/pt_gpu.js:259-297 solves the same problem per chain).
-// LDS for up to m live columns out of nf free reflections
-__host__ __device__ inline size_t nnls_lds_bytes(int m, int nf) {
-    const size_t tri = (size_t)m * (m + 1) / 2;
-    return 16 * (size_t)(2 * m) * 4 + ((tri * 2 + 6 * (size_t)m) * 8 + 15) / 16 * 16 + (size_t)m * 4 * 6 + (size_t)nf * 4 + 64;
+// the solve's arrays for up to c live columns: line records and spans, the packed normal
+// matrix and its factor, six FP64 and six int vectors
+__host__ __device__ inline size_t nnls_region_bytes(int c) {
+    const size_t tri = (size_t)c * (c + 1) / 2;
+    return 16 * (size_t)(2 * c) * 4 + ((tri * 2 + 6 * (size_t)c) * 8 + 15) / 16 * 16 + (size_t)c * 4 * 6;
+}
+// LDS for up to m live columns out of nf free reflections (the region, then nf flags)
+__host__ __device__ inline size_t nnls_lds_bytes(int m, int nf) { return nnls_region_bytes(m) + (size_t)nf * 4 + 64; }
+// a trial slot's global workspace for the solve when more columns are live than LDS holds
+size_t nnls_slot_bytes(int nf) { return (nnls_region_bytes(nf) + 255) / 256 * 256; }
+
+struct NnlsArrays {
+    LineRec *R;
+    int4 *Sp;
+    double *H, *Lf, *b, *x, *z, *g, *nrm, *tmp;
+    int *pas, *blk, *idx, *cs, *ce, *live;
+};
+// the arrays for capacity c carved from `base` (16-byte aligned), in nnls_region_bytes' order
+__device__ __forceinline__ NnlsArrays nnls_carve(unsigned char *base, int c) {
+    NnlsArrays A;
+    A.R = reinterpret_cast<LineRec *>(base);
+    A.Sp = reinterpret_cast<int4 *>(A.R + 2 * c);
+    const int Tc = c * (c + 1) / 2;
+    A.H = reinterpret_cast<double *>(A.Sp + 2 * c);
+    A.Lf = A.H + Tc; A.b = A.Lf + Tc; A.x = A.b + c; A.z = A.x + c; A.g = A.z + c; A.nrm = A.g + c; A.tmp = A.nrm + c;
+    A.pas = reinterpret_cast<int *>(A.tmp + c);
+    A.blk = A.pas + c; A.idx = A.blk + c; A.cs = A.idx + c; A.ce = A.cs + c; A.live = A.ce + c;
+    return A;
 }
 __device__ __forceinline__ int tri_at(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
@@ -1309,43 +1333,21 @@ __device__ __forceinline__ float free_column(const DevPlan &P, const LineRec *R,
     return v;
 }
 
-__global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const LineRec *__restrict__ rec,
-                                                    double2 *partial, int nsplit, int t_off) {
-    const int t = t_off + blockIdx.x;
-    if (!img[t].go) return;                              // not evaluated: the trial costs 3e38
-    extern __shared__ __align__(16) unsigned char dyn[];
-    __shared__ double red[256];
-    __shared__ int s_int[4];
-    __shared__ double s_tol;
-    const int nf = P.n_free, cap = min(nf, PTK_MAX_FREE_LIVE), tid = threadIdx.x, nt = blockDim.x;
+// the solve of trial t over its m live columns (flags per free reflection in `flag`)
+__device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__restrict__ rec, double2 *partial, int nsplit,
+                                          int t, const NnlsArrays &A, int m, const int *flag, double *red, int *s_int,
+                                          double &s_tol) {
+    const int nf = P.n_free, tid = threadIdx.x, nt = blockDim.x, T = m * (m + 1) / 2;
     const size_t base = (size_t)t * 2 * P.nr;
-    // the live columns of this trial: free reflections with a line window in the pattern,
-    // in envelope order (the others have no data and stay at zero)
-    LineRec *R = reinterpret_cast<LineRec *>(dyn);
-    int4 *Sp = reinterpret_cast<int4 *>(R + 2 * cap);
-    const int Tc = cap * (cap + 1) / 2;
-    double *H = reinterpret_cast<double *>(Sp + 2 * cap);
-    double *Lf = H + Tc, *b = Lf + Tc, *x = b + cap, *z = x + cap, *g = z + cap, *nrm = g + cap, *tmp = nrm + cap;
-    int *pas = reinterpret_cast<int *>(tmp + cap), *blk = pas + cap, *idx = blk + cap, *cs = idx + cap, *ce = cs + cap;
-    int *live = ce + cap, *flag = live + cap;
-    for (int gi = tid; gi < nf; gi += nt) {
-        const int k = P.sp->free_k[gi];
-        const int4 s0 = P.sp->fspan[base + 2 * k], s1 = P.sp->fspan[base + 2 * k + 1];
-        flag[gi] = (s0.y > s0.x || s1.y > s1.x) ? 1 : 0;
-    }
-    __syncthreads();
-    if (tid == 0) {
+    LineRec *R = A.R;
+    int4 *Sp = A.Sp;
+    double *H = A.H, *Lf = A.Lf, *b = A.b, *x = A.x, *z = A.z, *g = A.g, *nrm = A.nrm, *tmp = A.tmp;
+    int *pas = A.pas, *blk = A.blk, *idx = A.idx, *cs = A.cs, *ce = A.ce, *live = A.live;
+    if (tid == 0) {                                      // in envelope order
         int n = 0;
-        for (int gi = 0; gi < nf; ++gi)
-            if (flag[gi]) { if (n < cap) live[n] = gi; ++n; }
-        s_int[0] = n;
+        for (int gi = 0; gi < nf; ++gi) if (flag[gi]) live[n++] = gi;
     }
     __syncthreads();
-    const int m = s_int[0], T = m * (m + 1) / 2;
-    if (m > cap) {                                       // more live intensities than LDS holds
-        for (int k = tid; k < nsplit; k += nt) partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? 3e38 : 0.0, 0.0);
-        return;
-    }
     for (int j = tid; j < 2 * m; j += nt) {
         const int k = P.sp->free_k[live[j >> 1]];
         R[j] = rec[base + 2 * k + (j & 1)];
@@ -1541,6 +1543,42 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
     }
     for (int k = tid; k < nsplit; k += nt)
         partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? csum - red[0] : 0.0, 0.0);
+}
+
+__global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const LineRec *__restrict__ rec,
+                                                    double2 *partial, int nsplit, int t_off) {
+    const int t = t_off + blockIdx.x;
+    if (!img[t].go) return;                              // not evaluated: the trial costs 3e38
+    extern __shared__ __align__(16) unsigned char dyn[];
+    __shared__ double red[256];
+    __shared__ int s_int[4];
+    __shared__ double s_tol;
+    const int nf = P.n_free, cap = min(nf, PTK_MAX_FREE_LIVE), tid = threadIdx.x, nt = blockDim.x;
+    const size_t base = (size_t)t * 2 * P.nr;
+    // the live columns of this trial: free reflections with a line window in the pattern
+    // (the others have no data and stay at zero)
+    int *flag = reinterpret_cast<int *>(dyn + nnls_region_bytes(cap));
+    for (int gi = tid; gi < nf; gi += nt) {
+        const int k = P.sp->free_k[gi];
+        const int4 s0 = P.sp->fspan[base + 2 * k], s1 = P.sp->fspan[base + 2 * k + 1];
+        flag[gi] = (s0.y > s0.x || s1.y > s1.x) ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int n = 0;
+        for (int gi = 0; gi < nf; ++gi) n += flag[gi];
+        s_int[0] = n;
+    }
+    __syncthreads();
+    const int m = s_int[0];
+    if (m <= cap) {
+        nnls_core(P, rec, partial, nsplit, t, nnls_carve(dyn, cap), m, flag, red, s_int, s_tol);
+    } else if (P.sp->nnls_ws) {                          // more than LDS holds: the slot's workspace
+        nnls_core(P, rec, partial, nsplit, t, nnls_carve(P.sp->nnls_ws + (size_t)t * P.sp->nnls_stride, nf), m, flag, red,
+                  s_int, s_tol);
+    } else {
+        for (int k = tid; k < nsplit; k += nt) partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? 3e38 : 0.0, 0.0);
+    }
 }
 
 // the chi^2 of trial t over every dataset of the plan (js/pt_gpu.js:259-297: the targets

@@ -18,6 +18,7 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
 hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const LineRec *rec, double2 *partial, int nsplit, int ntrials,
                              hipStream_t s, int t_off = 0);
 size_t nnls_shared_bytes(int m);           // dynamic LDS of the free-intensity solve for m reflections
+size_t nnls_slot_bytes(int nf);             // global workspace per trial slot for a solve of up to nf reflections
 hipError_t nnls_lds_limit(size_t bytes);
 hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,
                                hipStream_t s, int t_off = 0);

@@ -93,6 +93,8 @@ struct DevSepPlan {
     int4 *fspan;                   // per trial slot: the spans of the free reflections' lines (2 nr)
     float *resid;                  // per trial slot: y_obs - background - Rietveld phases (np)
     double *fx;                    // per trial slot: the last solve's free intensities (its warm start)
+    unsigned char *nnls_ws;        // per trial slot: the solve's arrays when more intensities are live
+    size_t nnls_stride;            //   than LDS holds (set when n_free exceeds that), bytes per slot
 };
 
 struct DevPlan {

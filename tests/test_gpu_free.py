@@ -54,3 +54,26 @@ def test_free_intensity_pt_states_rescore():
     for c in range(S.shape[0]):
         assert _rel(S[c, 3 * nv + pt.S_COST], m.cost(S[c, :nv])) <= TOL
         assert _rel(S[c, 3 * nv + pt.S_BEST], m.cost(S[c, 2 * nv:3 * nv])) <= TOL
+
+
+@pytest.mark.parametrize('method', ['lebail', 'wppf'])
+def test_more_live_intensities_than_lds_holds(method):
+    """The P-1 probe over 8-120 deg: about 200 reflections have a window in every trial, more
+    than the 120 columns the LDS solve holds, so each trial solves in its slot's global
+    workspace. chi^2 of probes and of PT states against the oracle at 1e-5 (a trial over the
+    LDS limit without that workspace would cost 3e38)."""
+    pb = PR.triclinic_probe('tch_aniso', simulate=O.oracle_simulate, method=method, n_points=480, span=112.0)
+    m = O.Model(pb)
+    with pt.Plan(pb) as pl:
+        X = _probes(pb, pl.chains, 11, spread=0.5)   # pt_costs scores one vector per chain
+        g = pl.costs(X)
+        pl.initial([pb.start_vector()])
+        pl.batch(1, bootstrap=True)
+        st = pl.batch(1)
+        nv, stride = pl.nv, pl.stride
+    c = np.array([m.cost(x) for x in X])
+    assert np.all(g < 1e30)
+    assert _rel(g, c).max() <= TOL
+    S = st.reshape(-1, stride)
+    for k in range(S.shape[0]):
+        assert _rel(S[k, 3 * nv + pt.S_COST], m.cost(S[k, :nv])) <= TOL

@@ -126,6 +126,29 @@ MLP_FLOP_FWD = 2 * (64 * 256 + 2 * 256 * 256 + 256)   # one row through 64-256-2
 PEAK_BF16_MFMA = 2.5e15
 
 
+def free_intensities(pt, PR, sweeps=5):
+    """PT over the NaCl pattern (5,501 points, 256 replicas) with the NaCl phase as a Le Bail
+    phase (free intensities solved per trial) and as a Rietveld phase, alone and next to a
+    Rietveld ZnS phase: trial patterns/s of each, from `sweeps` timed sweeps."""
+    out = {'workload': 'NaCl Fm-3m 10-120 deg step 0.02 (5,501 pts), TCH, 256 replicas; '
+                       'with_zns adds a Rietveld ZnS phase'}
+    for mixed in (False, True):
+        for method in ('lebail', 'rietveld'):
+            pb = PR.lebail_probe(mixed=mixed, simulate=pt.gpu_simulate, chains=256)
+            pb.phases[0].method = method
+            with pt.Plan(pb) as pl:
+                pl.initial([pb.start_vector()])
+                pl.batch(1, bootstrap=True)
+                pl.sweeps(2)
+                t0 = time.perf_counter()
+                pl.sweeps(sweeps)
+                dt = (time.perf_counter() - t0) / sweeps
+                rate = pb.chains * pl.info['steps_per_sweep'] / dt
+            out[('with_zns_' if mixed else '') + method] = {'value': rate, 'unit': 'trial patterns/s',
+                                                            'ms_per_sweep': dt * 1e3}
+    return out
+
+
 def learned_screening(pt, PR, sweeps=2):
     """The P2_1/c molecular problem (17 atoms, 53 variables with one Biso), 512 replicas,
     7001 points: plain PT against surrogate-screened PT (4 candidates per chain and step,
@@ -248,6 +271,7 @@ def main():
     mat, mpb, mX, mchi, mchi_bf16 = matrix_form(pt, PR)
     uneven, upbs, ucosts = uneven_batch(pt, PR)
     learned = learned_screening(pt, PR)
+    free = free_intensities(pt, PR)
 
     S = state.reshape(chains, -1)
     nv = pb.nv
@@ -303,6 +327,7 @@ def main():
         'matrix_form': mat,
         'uneven_batch': uneven,
         'learned_screening': learned,
+        'free_intensities': free,
         'chosen_over_plain': value / plain,
         'roofline': roof,
         'cpu_baseline': cpu,

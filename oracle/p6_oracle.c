@@ -1431,6 +1431,31 @@ int or_reflection_table(or_model *m, const double *x, int q, int32_t *hklm, doub
     return nr;
 }
 
+/* The contributions buildPeakContributions (profile.js:1013-1080) makes for phase q at x:
+ * per reflection k and pass (0 Kalpha1, 1 Kalpha2), line[8 * k + 4 * pass + 0..3] =
+ * present (0/1), centre x0, doublet weight, prepared area (preparedProfileArea); and
+ * bragg[2 * k + pass] = the Bragg angle of the pass's wavelength (no shift, no zero). */
+int or_line_table(or_model *m, const double *x, int q, double *line, double *bragg, int cap) {
+    or_params(m, x, m->params);
+    if (!cell_ok(m, m->params)) return -1;
+    refresh_reflections(m, m->params);
+    double *yp = (double *)calloc(m->P.n_points, sizeof(double));
+    refl_work *rw = NULL; int nr = 0;
+    eval_phase(m, q, m->params, yp, &rw, &nr);
+    double l1 = m->P.lambda1 != 0 ? m->P.lambda1 : 1.54056, lr = m->P.lambda2 / l1;
+    for (int k = 0; k < nr && k < cap; k++) {
+        for (int pass = 0; pass < 2; pass++) {
+            double *o = line + 8 * k + 4 * pass;
+            o[0] = rw[k].present[pass]; o[1] = rw[k].v[pass].x0; o[2] = rw[k].weight[pass]; o[3] = rw[k].v[pass].area;
+        }
+        bragg[2 * k] = rw[k].bragg;
+        double st2 = sin(rw[k].bragg * (JS_PI / 180) / 2.0) * lr;
+        bragg[2 * k + 1] = fabs(st2) < 1 ? 2 * asin(st2) / (JS_PI / 180) : NAN;
+    }
+    free(rw); free(yp);
+    return nr;
+}
+
 /* gaussianVarianceMinimum (:1393-1398) */
 static double variance_min(double u, double v, double w, double lo, double hi) {
     double c[3] = {lo, hi, 0}; int nc = 2;

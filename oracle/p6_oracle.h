@@ -103,6 +103,9 @@ int or_pattern(or_model *m, const double *x, double *ycalc);
 /* per-reflection diagnostics of phase 0 at x: F^2, height, Kalpha1 centre (count out) */
 int or_reflection_table(or_model *m, const double *x, int phase, int32_t *hklm, double *f2,
                         double *height, double *centre, int cap);
+/* per-line Kalpha1 / Kalpha2 contributions of phase q at x (present, centre, weight, area;
+ * 8 values per reflection) and the Bragg angle of each wavelength (2 per reflection) */
+int or_line_table(or_model *m, const double *x, int phase, double *line, double *bragg, int cap);
 /* varianceViolation (rietveld_core.js:1489-1496) at x */
 double or_violation(or_model *m, const double *x);
 /* the profile-feasibility rule of pt_gpu.js propose (js/pt_gpu.js:361): 1 valid */

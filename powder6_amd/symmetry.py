@@ -89,6 +89,7 @@ CENTRING = {
     'C': ['0,0,0', '1/2,1/2,0'],
     'A': ['0,0,0', '0,1/2,1/2'],
     'B': ['0,0,0', '1/2,0,1/2'],
+    'R': ['0,0,0', '2/3,1/3,1/3', '1/3,2/3,2/3'],      # obverse, hexagonal axes
 }
 
 # ITA Vol. A generators (standard settings; origin choice 1 where there are two)
@@ -105,6 +106,9 @@ GROUPS = {
     123: ('P4/mmm', 'tetragonal', 'P', ['-y,x,z', 'x,-y,-z', '-x,-y,-z']),
     136: ('P4_2/mnm', 'tetragonal', 'P', ['-y+1/2,x+1/2,z+1/2', '-x+1/2,y+1/2,-z+1/2', '-x,-y,-z']),
     139: ('I4/mmm', 'tetragonal', 'I', ['-y,x,z', 'x,-y,-z', '-x,-y,-z']),
+    164: ('P-3m1', 'trigonal', 'P', ['-y,x-y,z', 'y,x,-z', '-x,-y,-z']),
+    166: ('R-3m', 'trigonal', 'R', ['-y,x-y,z', 'y,x,-z', '-x,-y,-z']),
+    191: ('P6/mmm', 'hexagonal', 'P', ['-y,x-y,z', '-x,-y,z', 'y,x,-z', '-x,-y,-z']),
     221: ('Pm-3m', 'cubic', 'P', ['-y,x,z', 'z,x,y', '-x,-y,-z']),
     216: ('F-43m', 'cubic', 'F', ['y,-x,-z', 'z,x,y']),
     225: ('Fm-3m', 'cubic', 'F', ['-y,x,z', 'z,x,y', '-x,-y,-z']),

@@ -67,6 +67,7 @@ def lib():
         L.or_violation.restype = ctypes.c_double
         L.or_violation.argtypes = [ctypes.c_void_p, P_D]
         L.or_reflection_table.argtypes = [ctypes.c_void_p, P_D, ctypes.c_int, P_I, P_D, P_D, P_D, ctypes.c_int]
+        L.or_line_table.argtypes = [ctypes.c_void_p, P_D, ctypes.c_int, P_D, P_D, ctypes.c_int]
         L.or_temper.restype = ctypes.c_double
         L.or_temper.argtypes = [ctypes.c_void_p, P_D, P_D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_uint32, P_D, P_D]
@@ -176,6 +177,15 @@ class Model:
                                       h.ctypes.data_as(P_D), c.ctypes.data_as(P_D), cap)
         n = min(n, cap)
         return hk[:n], f2[:n], h[:n], c[:n]
+
+    def lines(self, x, phase=0, cap=100000):
+        """Per reflection: lines[k, pass] = (present, centre, weight, area) of the Kalpha1
+        (pass 0) and Kalpha2 (pass 1) contributions, bragg[k, pass] the Bragg angles."""
+        a, p = d(x)
+        ln, br = np.zeros((cap, 2, 4)), np.zeros((cap, 2))
+        n = lib().or_line_table(self.h, p, phase, ln.ctypes.data_as(P_D), br.ctypes.data_as(P_D), cap)
+        n = min(n, cap)
+        return ln[:n], br[:n]
 
     def temper(self, x0, scales, sweeps, chains, max_temp, seed):
         a, p = d(x0)

@@ -178,7 +178,9 @@ typedef struct {
  *   [0,nv) current x, [nv,2nv) proposal, [2nv,3nv) best,
  *   3nv+0 cost, +1 best cost, +2 step counter (-1 = bootstrap), +3 accepted,
  *   +4 proposals, +5 swaps, +6 swap attempts, +7 trial valid, +8 log u,
- *   +9 unsolved flag (always 0: no free intensities), +10 trial cost, +11 feasible.
+ *   +9 unsolved flag (1 once an intensity solve of the chain's trial did not converge;
+ *   such a trial costs 3e38; the flag stays until pt_set_states), +10 trial cost,
+ *   +11 feasible.
  * Snapshots are returned in double precision.                                         */
 enum { PT_S_COST = 0, PT_S_BEST = 1, PT_S_STEP = 2, PT_S_ACCEPTED = 3, PT_S_PROPOSALS = 4,
        PT_S_SWAPS = 5, PT_S_ATTEMPTS = 6, PT_S_VALID = 7, PT_S_LOGU = 8, PT_S_UNSOLVED = 9,
@@ -232,11 +234,15 @@ int pt_plan_reflections(const pt_plan *plan, int phase, int32_t *hklm, int cap);
 int pt_set_states(pt_plan *plan, const double *x, int nvec);
 
 /* costs(vectors): js/pt_gpu.js:401.  chi^2 of every chain at the vectors (cycled),
- * written to chi2[chains]. */
+ * written to chi2[chains].  PT_ERR_NUMERIC with "... intensity solve did not converge"
+ * when a free-intensity solve of some chain failed (js/pt_gpu.js:402). */
 int pt_costs(pt_plan *plan, const double *x, int nvec, double *chi2);
 
 /* batch(count, bootstrap): js/pt_gpu.js:402.  `count` sweeps (or the bootstrap step),
- * then a snapshot of chains*stride doubles into state_out (may be NULL). */
+ * then a snapshot of chains*stride doubles into state_out (may be NULL).  With free
+ * intensities, a chain whose unsolved flag (+9) is set makes the call return
+ * PT_ERR_NUMERIC "... intensity solve did not converge" after the snapshot is written
+ * (the reference's run() throws on it, js/pt_gpu.js:418). */
 int pt_batch(pt_plan *plan, int count, int bootstrap, double *state_out);
 
 /* Evaluate ntrials arbitrary variable vectors (ntrials*nv doubles) with one of the
@@ -264,6 +270,13 @@ int pt_workload(pt_plan *plan, const double *x, double *out);
  * instead of replaying the captured sweep graph and brackets each launch of the dominant
  * kernel (the banded profile sum) with HIP events.  No counterpart in the reference. */
 int pt_set_kernel_timing(pt_plan *plan, int on);
+
+/* Active-set iterations one free-intensity solve may take per trial (0, the default:
+ * 3 m + 10 for m live intensities; applies to every dataset of the plan).  A trial whose
+ * solve needs more is unsolved: it costs 3e38, its chain's +9 flag is set and pt_costs /
+ * pt_batch report "did not converge" -- the reference bounds its solve the same way, at
+ * 8 solve / polish passes and a convergence flag (js/pt_gpu.js:398, 255, 363). */
+int pt_set_solve_limit(pt_plan *plan, int max_iterations);
 
 /* Timing of the last pt_batch / pt_eval call on the plan's stream (HIP events):
  * elapsed milliseconds of the whole call, summed milliseconds of the dominant kernel's

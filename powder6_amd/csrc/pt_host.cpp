@@ -288,6 +288,15 @@ int dalloc(Owner *pl, T **ptr, size_t count) {
     *ptr = reinterpret_cast<T *>(p);
     return 0;
 }
+// frees a buffer dalloc gave out (a null pointer is a no-op)
+template <class Owner, class T>
+void dfree(Owner *pl, T **ptr) {
+    void *p = reinterpret_cast<void *>(*ptr);
+    if (!p) return;
+    for (size_t i = 0; i < pl->allocs.size(); ++i)
+        if (pl->allocs[i] == p) { (void)hipFree(p); pl->allocs.erase(pl->allocs.begin() + i); break; }
+    *ptr = nullptr;
+}
 template <class Owner, class T>
 int upload(Owner *pl, T **ptr, const std::vector<T> &v) {
     int rc = dalloc(pl, ptr, v.size());
@@ -1030,6 +1039,16 @@ static int ensure_slots(pt_plan *pl, int n) {
     if (n <= pl->slots) return 0;
     int rc;
     size_t nl = 2 * (size_t)pl->nr;
+    // the smaller buffers go (nothing outlives a call in them; the captured sweep graph
+    // that points at them is dropped below)
+    drop_graph(pl->gexec, pl->graph);
+    (void)hipStreamSynchronize(pl->stream);
+    dfree(pl, (char **)&pl->d_img); dfree(pl, &pl->d_srec); dfree(pl, &pl->d_sspan); dfree(pl, &pl->d_part);
+    dfree(pl, &pl->d_schi2);
+    if (pl->P.has_free) {
+        ptk::DevSepPlan &h = pl->sp_host;
+        dfree(pl, &h.fspan); dfree(pl, &h.resid); dfree(pl, &h.fx); dfree(pl, &h.unsolved); dfree(pl, &h.nnls_ws);
+    }
     if ((rc = dalloc(pl, (char **)&pl->d_img, (size_t)n * ptk::trial_image_bytes())) ||
         (rc = dalloc(pl, &pl->d_srec, (size_t)n * nl)) || (rc = dalloc(pl, &pl->d_sspan, (size_t)n * nl)) ||
         (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->ngroups)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
@@ -1038,14 +1057,16 @@ static int ensure_slots(pt_plan *pl, int n) {
         // free intensities: their line spans and the residual they are fitted to, per slot
         ptk::DevSepPlan &h = pl->sp_host;
         if ((rc = dalloc(pl, &h.fspan, (size_t)n * nl)) || (rc = dalloc(pl, &h.resid, (size_t)n * pl->np)) ||
-            (rc = dalloc(pl, &h.fx, (size_t)n * pl->P.n_free)))
+            (rc = dalloc(pl, &h.fx, (size_t)n * pl->P.n_free)) || (rc = dalloc(pl, &h.unsolved, (size_t)n)))
             return rc;
         if (pl->P.n_free > PTK_MAX_FREE_LIVE) {
-            // a trial with more live intensities than LDS holds solves in its slot's workspace
+            // a trial with more live intensities than LDS holds (up to PTK_MAX_FREE_WS_LIVE)
+            // solves in its slot's workspace
             h.nnls_stride = ptk::nnls_slot_bytes(pl->P.n_free);
             if ((rc = dalloc(pl, &h.nnls_ws, (size_t)n * h.nnls_stride))) return rc;
         }
         HIPCHECK(hipMemset(h.fx, 0, sizeof(double) * (size_t)n * pl->P.n_free), "intensity start");
+        HIPCHECK(hipMemset(h.unsolved, 0, sizeof(int) * (size_t)n), "intensity flags");
         HIPCHECK(hipMemcpy(pl->d_sp, &h, sizeof h, hipMemcpyHostToDevice), "intensity buffers");
         drop_graph(pl->gexec, pl->graph);
     }
@@ -1055,7 +1076,7 @@ static int ensure_slots(pt_plan *pl, int n) {
         std::vector<ptk::DevFollow> fv;
         for (pt_plan *f : pl->follow) {
             if ((rc = ensure_slots(f, n))) return rc;
-            fv.push_back(ptk::DevFollow{f->d_img, f->d_part, f->ngroups, 0});
+            fv.push_back(ptk::DevFollow{f->d_img, f->d_part, f->P.has_free ? f->sp_host.unsolved : nullptr, f->ngroups, 0});
         }
         if (!pl->d_follow && (rc = dalloc(pl, &pl->d_follow, fv.size()))) return rc;
         HIPCHECK(hipMemcpy(pl->d_follow, fv.data(), sizeof(ptk::DevFollow) * fv.size(), hipMemcpyHostToDevice), "dataset links");
@@ -1107,13 +1128,22 @@ static double sum_pairs(pt_plan *pl, int pairs) {
 }
 
 static const int kEvalChunk = 2048;   // trial slots per pass of pt_eval
+static const int kEvalChunkFree = 512; // the same for plans with free intensities (per-slot solve buffers)
+
+// a plan or one of its further datasets has free intensities
+static bool any_free(const pt_plan *pl) {
+    if (pl->P.has_free) return true;
+    for (const pt_plan *f : pl->follow) if (f->P.has_free) return true;
+    return false;
+}
+static const char *kUnsolved = "MI355X PT intensity solve did not converge.";
 
 static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout, int bf16 = 0);
 
 static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2, bool time_it) {
     int rc = ensure_trials(pl, ntrials);
     if (rc) return rc;
-    int chunk = std::min(ntrials, kEvalChunk);
+    int chunk = std::min(ntrials, any_free(pl) ? kEvalChunkFree : kEvalChunk);
     if ((rc = ensure_slots(pl, std::max(chunk, pl->chains)))) return rc;
     int passes = (ntrials + chunk - 1) / chunk;
     if (time_it && (rc = ensure_events(pl, passes))) return rc;
@@ -1132,6 +1162,9 @@ static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2,
     pl->last_total_ms = ms;
     pl->last_kernel_ms = time_it ? sum_pairs(pl, passes) : 0;
     pl->last_launches = time_it ? passes : 0;
+    // an intensity solve that did not converge comes back as NaN (js/pt_gpu.js:402 throws)
+    for (int t = 0; t < ntrials; ++t)
+        if (std::isnan(chi2[t])) return fail(PT_ERR_NUMERIC, kUnsolved);
     return 0;
 }
 
@@ -1440,6 +1473,15 @@ int pt_set_kernel_timing(pt_plan *pl, int on) {
     return 0;
 }
 
+int pt_set_solve_limit(pt_plan *pl, int max_iterations) {
+    if (!pl) return fail(PT_ERR_ARG, "NULL plan");
+    if (max_iterations < 0) return fail(PT_ERR_ARG, "negative solve limit");
+    pl->P.nnls_cap = max_iterations;
+    for (pt_plan *f : pl->follow) f->P.nnls_cap = max_iterations;
+    drop_graph(pl->gexec, pl->graph);          // the sweep graph holds the plan by value
+    return 0;
+}
+
 int pt_batch(pt_plan *pl, int count, int bootstrap, double *state_out) {
     if (!pl) return fail(PT_ERR_ARG, "NULL plan");
     NEED_DEVICE(pl);
@@ -1473,14 +1515,23 @@ int pt_batch(pt_plan *pl, int count, int bootstrap, double *state_out) {
         for (int j = 0; j < count; ++j) HIPCHECK(hipGraphLaunch(pl->gexec, pl->stream), "sweep");
     }
     HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
-    if (state_out)
-        HIPCHECK(hipMemcpyAsync(state_out, pl->P.state, sizeof(double) * pl->chains * pl->stride, hipMemcpyDeviceToHost, pl->stream), "readback");
+    // with free intensities the snapshot is read anyway: a chain whose solve did not
+    // converge has its unsolved flag set (state +9), and the batch reports it as the
+    // reference's run() does (js/pt_gpu.js:418), after filling state_out
+    std::vector<double> own;
+    double *snap = state_out;
+    if (!snap && any_free(pl)) { own.resize((size_t)pl->chains * pl->stride); snap = own.data(); }
+    if (snap)
+        HIPCHECK(hipMemcpyAsync(snap, pl->P.state, sizeof(double) * pl->chains * pl->stride, hipMemcpyDeviceToHost, pl->stream), "readback");
     HIPCHECK(hipStreamSynchronize(pl->stream), "sweeps");
     float ms = 0;
     (void)hipEventElapsedTime(&ms, pl->ev_total[0], pl->ev_total[1]);
     pl->last_total_ms = ms;
     pl->last_kernel_ms = timed ? sum_pairs(pl, timed) : 0;
     pl->last_launches = timed;
+    if (snap && any_free(pl))
+        for (int c = 0; c < pl->chains; ++c)
+            if (snap[(size_t)c * pl->stride + 3 * pl->nv + PT_S_UNSOLVED] != 0.0) return fail(PT_ERR_NUMERIC, kUnsolved);
     return 0;
 }
 

@@ -1,18 +1,24 @@
 // pt_kernels.hip -- gfx950 kernels of the PT trial evaluator.
 //
-//   k_eval_batch   one workgroup per trial vector: parameters -> cell metric -> per-line
-//                  geometry/profile/|F|^2 (LDS line records) -> banded profile sum over
-//                  64-point tiles (one wavefront per tile, lanes = points) -> compensated
-//                  weighted chi^2 (wavefront shuffles + LDS).      [PT_FORM_BATCHED]
-//   k_pt_sweep     the same evaluator inside a persistent per-replica loop: propose,
-//                  evaluate, Metropolis accept in FP64, `steps` times per launch; one
-//                  workgroup per replica (js/pt_gpu.js propose/geometry/columns/targets/
-//                  energy/accept, :361-364, fused).
-//   k_exchange     replica exchange between neighbouring temperatures (js/pt_gpu.js:365).
-//   k_plain_*      the plain single-trial path: one trial spread over the whole GPU
-//                  (setup, lines, points, reduce), one launch sequence per trial.
-//   k_pattern      background + phases of one vector (no chi^2), for pt_pattern().
+// One PT step of every chain (js/pt_gpu.js propose/geometry/columns/targets/energy/accept,
+// :361-364) is a staged pipeline:
+//   k_stage_setup        one workgroup per trial: the proposal, the model parameters, the
+//                        cell metric, the atom positions, the variance guard -> trial image
+//   k_stage_lines        one thread per (reflection, trial): Bragg angle, widths, prepared
+//                        pseudo-Voigt, Lp, |F|^2, heights, the Ka2 companion -> line records
+//   k_stage_accum<T>     the banded profile sum over 128-point tiles and the compensated
+//                        weighted chi^2 of every 512-point group
+//   k_stage_nnls         free intensities (Le Bail / WPPF): per-trial normal equations and
+//                        a non-negative solve
+//   k_stage_finish(_setup)  the ordered chi^2 sum and the FP64 Metropolis step, fused with
+//                        the next step's setup
+//   k_exchange           replica exchange between neighbouring temperatures (:365)
+// k_group_*: the same stages over the members of a group of independent problems;
+// k_plain_*: one trial spread over the whole GPU (the plain form); k_matrix_*: the shared
+// profile matrix on MFMA; k_pattern / k_workload: one vector's pattern and work counts.
 #include "pt_device.h"
+#include <algorithm>
+#include <mutex>
 
 namespace ptk {
 
@@ -1297,7 +1303,7 @@ __host__ __device__ inline size_t nnls_region_bytes(int c) {
 // LDS for up to m live columns out of nf free reflections (the region, then nf flags)
 __host__ __device__ inline size_t nnls_lds_bytes(int m, int nf) { return nnls_region_bytes(m) + (size_t)nf * 4 + 64; }
 // a trial slot's global workspace for the solve when more columns are live than LDS holds
-size_t nnls_slot_bytes(int nf) { return (nnls_region_bytes(nf) + 255) / 256 * 256; }
+size_t nnls_slot_bytes(int nf) { return (nnls_region_bytes(std::min(nf, PTK_MAX_FREE_WS_LIVE)) + 255) / 256 * 256; }
 
 struct NnlsArrays {
     LineRec *R;
@@ -1442,7 +1448,12 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__res
     }
     __syncthreads();
     bool warm = s_int[0] == -2;
-    for (int it = 0; it < 3 * m + 10; ++it) {
+    // converged: the gradient has no positive entry off the passive set.  The iteration
+    // caps, or a passive block that is not positive definite, leave the trial unsolved
+    // (js/pt_gpu.js:363 solved(): its chi^2 is not used and the state's unsolved flag is set)
+    bool converged = false;
+    const int outer_cap = P.nnls_cap > 0 ? P.nnls_cap : 3 * m + 10;
+    for (int it = 0; it < outer_cap; ++it) {
         if (!warm) {
             for (int j = tid; j < m; j += nt) {
                 double a = b[j];
@@ -1461,7 +1472,8 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__res
         }
         const int jm = warm ? -2 : s_int[0];
         warm = false;
-        if (jm == -1) break;
+        if (jm == -1) { converged = true; break; }
+        bool settled = false;
         for (int inner = 0; inner < 3 * m + 10; ++inner) {
             if (tid == 0) {
                 int p = 0;
@@ -1517,9 +1529,9 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__res
                 s_int[3] = stop;
             }
             __syncthreads();
-            if (s_int[3]) break;
+            if (s_int[3]) { settled = true; break; }
         }
-        if (s_int[2]) break;                             // a block that is not positive definite
+        if (s_int[2] || !settled) break;                 // a block that is not positive definite, or the cap
         if (tid == 0 && jm >= 0 && !pas[jm]) blk[jm] = 1;   // could not stay positive: keep it out
         __syncthreads();
     }
@@ -1541,8 +1553,10 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__res
         if (tid < o) red[tid] += red[tid + o];
         __syncthreads();
     }
+    const bool ok = converged && !s_int[2];
+    if (tid == 0) P.sp->unsolved[t] = ok ? 0 : 1;
     for (int k = tid; k < nsplit; k += nt)
-        partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? csum - red[0] : 0.0, 0.0);
+        partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? (ok ? csum - red[0] : 3e38) : 0.0, 0.0);
 }
 
 __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const LineRec *__restrict__ rec,
@@ -1573,10 +1587,12 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
     const int m = s_int[0];
     if (m <= cap) {
         nnls_core(P, rec, partial, nsplit, t, nnls_carve(dyn, cap), m, flag, red, s_int, s_tol);
-    } else if (P.sp->nnls_ws) {                          // more than LDS holds: the slot's workspace
-        nnls_core(P, rec, partial, nsplit, t, nnls_carve(P.sp->nnls_ws + (size_t)t * P.sp->nnls_stride, nf), m, flag, red,
-                  s_int, s_tol);
+    } else if (P.sp->nnls_ws && m <= PTK_MAX_FREE_WS_LIVE) {   // more than LDS holds: the slot's workspace
+        nnls_core(P, rec, partial, nsplit, t, nnls_carve(P.sp->nnls_ws + (size_t)t * P.sp->nnls_stride, PTK_MAX_FREE_WS_LIVE), m,
+                  flag, red, s_int, s_tol);
     } else {
+        // more live intensities than one trial's solve takes: unsolved
+        if (tid == 0) P.sp->unsolved[t] = 1;
         for (int k = tid; k < nsplit; k += nt) partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? 3e38 : 0.0, 0.0);
     }
 }
@@ -1584,29 +1600,35 @@ __global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage 
 // the chi^2 of trial t over every dataset of the plan (js/pt_gpu.js:259-297: the targets
 // of all datasets add into one cost per chain), and whether every dataset's profile is
 // feasible
+// feasible, and (uns) whether an intensity solve of some dataset did not converge
 __device__ inline double trial_cost_all(const DevPlan &P, const TrialImage *im, const double2 *partial, int nsplit, int t,
-                                        int &pok) {
+                                        int &pok, int &uns) {
     double c = trial_cost(im, partial, nsplit, t);
     pok = im->prof_ok;
+    uns = (P.has_free && im->go) ? P.sp->unsolved[t] : 0;
     for (int f = 0; f < P.n_follow; ++f) {
         const DevFollow &F = P.follow[f];
         const TrialImage *fi = (const TrialImage *)F.img + t;
         const double cf = trial_cost(fi, F.partial, F.nsplit, t);
         c = (c >= 3e38 || cf >= 3e38) ? 3e38 : c + cf;
         pok = pok && fi->prof_ok;
+        if (F.unsolved && fi->go && F.unsolved[t]) uns = 1;
     }
     return c;
 }
 
 // the Metropolis step of chain t in FP64 (js/pt_gpu.js:363-364, rietveld_core.js:2213-2222),
 // run by one thread; returns 0 (rejected), 1 (accepted) or 2 (accepted, new best).
-// prof_ok: the trial's profiles are feasible in every dataset.
-__device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int t) {
+// prof_ok: the trial's profiles are feasible in every dataset; uns: an intensity solve of
+// the trial did not converge, which sets the chain's unsolved flag (js/pt_gpu.js:363,
+// state +9; it stays set until the states are initialised again) and costs 3e38.
+__device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int uns, int t) {
     const int nv = P.nv;
     double *sc = P.state + (size_t)t * P.stride + 3 * nv;
     const bool boot = im->boot != 0;
     const double step = sc[2];
     const double temp = P.temp[t];
+    if (uns) { sc[9] = 1.0; cost = 3e38; }
     sc[10] = cost;
     const double old = sc[0];
     const bool acc = boot || (cost < 3e38 && (cost <= old || sc[8] < (old - cost) / (2.0 * temp)));
@@ -1626,8 +1648,8 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
 
 // the end of one PT step for chain t: Metropolis accept, and on acceptance the proposal
 // becomes the chain's state (and its best when it is a new feasible best)
-__device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int t) {
-    const int r = accept_chain(P, im, cost, prof_ok, t);
+__device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int uns, int t) {
+    const int r = accept_chain(P, im, cost, prof_ok, uns, t);
     if (r) {
         double *st = P.state + (size_t)t * P.stride;
         const int nv = P.nv;
@@ -1645,10 +1667,11 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
     const int tl = blockIdx.x * blockDim.x + threadIdx.x;
     if (tl >= ntrials) return;
     const int t = t_off + tl;
-    int pok;
-    const double cost = trial_cost_all(P, img + t, partial, nsplit, t, pok);
-    if (!pt_mode) { chi2[t] = cost; return; }
-    finish_chain(P, img + t, cost, pok, t);
+    int pok, uns;
+    const double cost = trial_cost_all(P, img + t, partial, nsplit, t, pok, uns);
+    // batch mode: NaN marks an unsolved trial for the host (a cost is otherwise finite)
+    if (!pt_mode) { chi2[t] = uns ? __builtin_nan("") : cost; return; }
+    finish_chain(P, img + t, cost, pok, uns, t);
 }
 
 // the Metropolis step of chain t, then its next proposal and setup: the finish of one PT
@@ -1658,9 +1681,9 @@ __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *
                                          int nsplit, int t, SetupShared &sh, double *prop, int &s_acc) {
     const int nv = P.nv;
     if (threadIdx.x == 0) {
-        int pok;
-        const double cost = trial_cost_all(P, img + t, partial, nsplit, t, pok);
-        s_acc = accept_chain(P, img + t, cost, pok, t);
+        int pok, uns;
+        const double cost = trial_cost_all(P, img + t, partial, nsplit, t, pok, uns);
+        s_acc = accept_chain(P, img + t, cost, pok, uns, t);
     }
     __syncthreads();
     const int r = s_acc;
@@ -1762,7 +1785,7 @@ __global__ void __launch_bounds__(64) k_group_finish(const DevPlan *__restrict__
     const DevMember &M = mem[w.x];
     const TrialImage *img = (const TrialImage *)M.img;
     const double cost = trial_cost(img + w.y, M.partial, M.nsplit[0], w.y);
-    if (pt_mode) finish_chain(P, img + w.y, cost, img[w.y].prof_ok, w.y);
+    if (pt_mode) finish_chain(P, img + w.y, cost, img[w.y].prof_ok, 0, w.y);
     else M.chi2[w.y] = cost;
 }
 
@@ -2287,11 +2310,23 @@ size_t plain_scratch_bytes() { return sizeof(PlainScratch); }
 size_t trial_image_bytes() { return sizeof(TrialImage); }
 int accum_points_per_group(int tpw) { return acc_points(tpw); }
 
+// The dynamic-LDS limit of a kernel is one setting per process; every plan raises it to
+// what it launches with and none lowers it, so a plan created later with smaller needs
+// leaves the limit of an earlier one in place.
+static hipError_t raise_lds_limit(const void *fn, size_t bytes, size_t &current) {
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if (bytes <= current) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) current = bytes;
+    return e;
+}
 hipError_t set_lds_limits(size_t bytes) {
+    static size_t cur[3] = {0, 0, 0};
     hipError_t e;
-    if ((e = hipFuncSetAttribute((const void *)k_workload, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
-    if ((e = hipFuncSetAttribute((const void *)k_pattern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_plain_points, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if ((e = raise_lds_limit((const void *)k_workload, bytes, cur[0])) != hipSuccess) return e;
+    if ((e = raise_lds_limit((const void *)k_pattern, bytes, cur[1])) != hipSuccess) return e;
+    return raise_lds_limit((const void *)k_plain_points, bytes, cur[2]);
 }
 
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
@@ -2307,7 +2342,8 @@ hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const LineRec *r
 }
 size_t nnls_shared_bytes(int nf) { return nnls_lds_bytes(std::min(nf, PTK_MAX_FREE_LIVE), nf); }
 hipError_t nnls_lds_limit(size_t bytes) {
-    return hipFuncSetAttribute((const void *)k_stage_nnls, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    static size_t cur = 0;
+    return raise_lds_limit((const void *)k_stage_nnls, bytes, cur);
 }
 
 hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,

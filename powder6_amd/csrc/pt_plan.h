@@ -29,6 +29,8 @@
 #define PTK_PH_ATOM0 40      // first atom slot of a phase block
 #define PTK_MAX_FREE 2048    // free reflections per dataset (the envelope of the free phases)
 #define PTK_MAX_FREE_LIVE 120  // free reflections with a window in one trial (the NNLS stage holds them in LDS)
+#define PTK_MAX_FREE_WS_LIVE 256  // live free reflections one trial may solve in its slot's global workspace;
+                                  // a trial with more is reported unsolved (the caller takes the CPU path)
 #define PTK_PROF_LDS 48      // a trial's profile slots of a phase in LDS: the 32 model slots, then
                              // (separated profiles) the family's eta / SL / HL slots from the instrument
 #define PTK_S_COUNT_K 12
@@ -79,6 +81,7 @@ struct DevSepPhase {
 struct DevFollow {
     const void *img;
     const double2 *partial;
+    const int *unsolved;       // per trial slot: 1 when the dataset's intensity solve failed (free phases), else null
     int nsplit;
     int pad;
 };
@@ -93,6 +96,7 @@ struct DevSepPlan {
     int4 *fspan;                   // per trial slot: the spans of the free reflections' lines (2 nr)
     float *resid;                  // per trial slot: y_obs - background - Rietveld phases (np)
     double *fx;                    // per trial slot: the last solve's free intensities (its warm start)
+    int *unsolved;                 // per trial slot: 1 when the last solve did not converge (the trial costs 3e38)
     unsigned char *nnls_ws;        // per trial slot: the solve's arrays when more intensities are live
     size_t nnls_stride;            //   than LDS holds (set when n_free exceeds that), bytes per slot
 };
@@ -107,7 +111,8 @@ struct DevPlan {
     short has_stephens;        // some separated phase has Stephens strain
     int n_follow;              // further datasets whose chi^2 the finish adds (first dataset only)
     int has_free;              // some phase has free intensities (the NNLS stage runs)
-    int n_free, pad2;          // free reflections (all reflections of the free phases)
+    int n_free;                // free reflections (all reflections of the free phases)
+    int nnls_cap;              // active-set iterations of one solve (0: 3 m + 10 for m live columns)
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
     double inv_step;           // 1 / step

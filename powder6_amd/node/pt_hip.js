@@ -241,7 +241,10 @@ function run(ctx, options) {
             var stride = gpu.plan.stride;
             function inspect() {
                 for (var c = 0; c < chains; c++) {
-                    var b = c * stride, candidate = Array.from(state.subarray(b + 2 * nv, b + 3 * nv)), key = candidate.join(',');
+                    var b = c * stride;
+                    // js/pt_gpu.js:418: a chain whose intensity solve failed ends the GPU run
+                    if (state[b + 3 * nv + 9]) throw new Error('MI355X PT intensity solve did not converge; restarting CPU PT.');
+                    var candidate = Array.from(state.subarray(b + 2 * nv, b + 3 * nv)), key = candidate.join(',');
                     if (seen.has(key)) continue;
                     seen.add(key);
                     if (!candidate.every(function (v, j) { return Number.isFinite(v) && v >= ctx.info[j].lo && v <= ctx.info[j].hi; })) continue;
@@ -258,6 +261,9 @@ function run(ctx, options) {
                     state = s; completed += count; inspect();
                     if (ctx.onIter) ctx.onIter({ it: completed, chi2: bestCost, maxIter: ctx.maxIter, stage: 'pt', backend: 'mi355x' });
                     return loop();
+                }, function (e) {
+                    if (/did not converge/.test(e.message)) throw new Error('MI355X PT intensity solve did not converge; restarting CPU PT.');
+                    throw e;
                 });
             }
             return gpu.initial([ctx.x]).then(function () { return gpu.batch(1, true); }).then(function (s) { state = s; return loop(); }).then(function () {

@@ -177,6 +177,7 @@ def lib():
         L.pt_phase_base.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_workload.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pt_set_solve_limit.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_matrix_timing.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, P_I, P_D]
@@ -283,9 +284,19 @@ class Plan:
         return out
 
     def batch(self, count, bootstrap=False):
+        """batch(count, bootstrap): the state snapshot.  A PTError from an unsolved chain
+        carries the snapshot as its `state`."""
         out = np.zeros(self.chains * self.stride)
-        _check(lib().pt_batch(self.h, int(count), 1 if bootstrap else 0, _dptr(out)))
+        try:
+            _check(lib().pt_batch(self.h, int(count), 1 if bootstrap else 0, _dptr(out)))
+        except PTError as e:
+            e.state = out
+            raise
         return out
+
+    def solve_limit(self, max_iterations):
+        """Active-set iterations per free-intensity solve (0: the default 3 m + 10)."""
+        _check(lib().pt_set_solve_limit(self.h, int(max_iterations)))
 
     def sweeps(self, count):
         """`count` sweeps without the snapshot copy (timing)."""
@@ -642,7 +653,13 @@ def run(ctx, tolerance=1e-3):
                 stopped = True
                 break
             count = min(max(1, math.ceil(8 / nv)), sweeps - completed)
-            state = gpu.batch(count, False)
+            try:
+                state = gpu.batch(count, False)
+            except PTError as e:
+                # js/pt_gpu.js:418: a chain whose intensity solve failed ends the GPU run
+                if 'did not converge' in str(e):
+                    raise PTError('MI355X PT intensity solve did not converge; restarting CPU PT.') from None
+                raise
             completed += count
             inspect(state)
             if ctx.get('on_iter'):

@@ -2,6 +2,7 @@
 from a problems.Problem, for driving powder6_amd/node/pt_hip.js from tests.  JSON has no
 infinities: +-inf limits are written as null and restored by tests/js/pt_hip_probe.js."""
 import json
+import os
 import math
 
 import numpy as np
@@ -90,6 +91,44 @@ def reference_ctx(pb, initial_cost=None):
                   'maxTemp': pb.max_temp, 'seed': pb.seed, 'initialCost': initial_cost},
         'constraints': None, 'maxIter': pb.sweeps,
     }
+
+
+class OracleCostPipe:
+    """ctx.cost for tests/js/pt_hip_probe.js: two FIFOs in `directory` and a thread of the
+    test process that answers every vector the Node side writes (one JSON array per line)
+    with the FP64 oracle's cost (`model.cost`), one number per line.  Use as a context
+    manager; `paths` goes into the context as ctx.costPipe."""
+
+    def __init__(self, model, directory):
+        import threading
+        self.model = model
+        self.req = os.path.join(str(directory), 'cost_req')
+        self.resp = os.path.join(str(directory), 'cost_resp')
+        os.mkfifo(self.req)
+        os.mkfifo(self.resp)
+        self.calls = 0
+        self.thread = threading.Thread(target=self._serve, daemon=True)
+        self.thread.start()
+
+    @property
+    def paths(self):
+        return {'req': self.req, 'resp': self.resp}
+
+    def _serve(self):
+        # the same open order as the Node side: requests first, then answers
+        with open(self.req, 'r') as fr, open(self.resp, 'w') as fw:
+            for line in fr:
+                c = float(self.model.cost(np.asarray(json.loads(line), dtype=np.float64)))
+                self.calls += 1
+                fw.write(('Infinity' if c == math.inf else 'NaN' if c != c else repr(c)) + '\n')
+                fw.flush()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.thread.join(timeout=5)
+        return False
 
 
 def write_ctx(pb, path, **extra):

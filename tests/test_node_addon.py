@@ -203,24 +203,16 @@ def test_compile_lebail_context(tmp_path):
 
 
 @needs_node
-@pytest.mark.gpu
-def test_node_run_pt(tmp_path):
-    """tests/webgpu-pt.js:131-136 through the Node host: proposals = chains*sweeps*nv,
-    swaps > 0, best <= initial, validation error reported."""
+def test_cost_pipe_answers_with_the_oracle(tmp_path):
+    """The FIFO cost server the run() tests use: Node's synchronous ctx.cost gets the
+    oracle's FP64 cost of any vector, bit for bit."""
     pb = PR.triclinic_probe('tch_aniso', simulate=O.oracle_simulate)
     m = O.Model(pb)
-    x0 = pb.start_vector()
-    init = m.cost(x0)
-    probes = [x0, np.array([pt._reflected(v + pb.scale[j] * 0.25, pb.lo[j], pb.hi[j]) for j, v in enumerate(x0)])]
-    # run() re-scores candidate vectors with ctx.cost; the probe script looks costs up by
-    # key, so every candidate the GPU returns is scored here afterwards instead
-    costs = [list(map(float, p)) + [m.cost(p)] for p in probes]
-    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), initial_cost=init, expectedCosts=costs)
-    out, r = _node(path, 'run')
-    assert 'pt' in out, (out, r.stderr)
-    p = out['pt']
-    assert p['proposals'] == pb.chains * pb.sweeps * pb.nv
-    assert p['swaps'] > 0 and p['swapAttempts'] > 0
-    assert p['validationError'] <= 1e-5
-    assert p['bestChi2'] <= init
-    assert out['iters'] >= 1
+    probes = _probes(pb)
+    with js_ctx.OracleCostPipe(m, tmp_path) as pipe:
+        path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'), costPipe=pipe.paths,
+                                probes=[list(map(float, p)) for p in probes])
+        out, r = _node(path, 'cost')
+    assert 'cost' in out, (out, r.stderr)
+    want = [m.cost(pb.start_vector())] + [m.cost(p) for p in probes]
+    assert out['cost'] == want

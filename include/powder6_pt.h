@@ -250,6 +250,19 @@ int pt_batch(pt_plan *plan, int count, int bootstrap, double *state_out);
  * (atom) and scale variables differ between the trials and the plan's start vector. */
 int pt_eval(pt_plan *plan, int form, const double *x, int ntrials, double *chi2);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * chi^2 of ntrials vectors that are already in device memory (d_x: ntrials * nv doubles;
+ * d_chi2: ntrials doubles, both device pointers), by the kernels of pt_eval's
+ * PT_FORM_BATCHED, with every per-trial buffer taken from one fixed device scratch area
+ * of 32 MiB per GPU (held until the process ends) instead of the plan's buffers.  It calls
+ * no hipMalloc, hipFree or other allocating routine.  Runs on the plan's stream and
+ * synchronises it.  PT_ERR_BUDGET when the trials do not fit the scratch (the message
+ * states this plan's maximum, about 32 MiB / (10 KB + 128 B x reflections) trials);
+ * PT_ERR_STATE for multi-dataset plans and plans with free intensities.  All calls on one
+ * GPU share the scratch: calls that overlap in time on one GPU race. */
+int pt_eval_noalloc(pt_plan *plan, const double *d_x, int ntrials, double *d_chi2);
+
 /* Pattern of one structure-only vector through the shared profile matrix (PT_FORM_MATRIX
  * path): background of the start vector + P I.  No counterpart in the reference. */
 int pt_matrix_pattern(pt_plan *plan, const double *x, double *ycalc);

@@ -1473,6 +1473,40 @@ int pt_set_kernel_timing(pt_plan *pl, int on) {
     return 0;
 }
 
+// The chi^2 of ntrials device-resident vectors through the staged pipeline of the batched
+// form, staged in the library's fixed device scratch: no allocation of any kind, on the
+// plan's stream, synchronised before returning.  Calls that overlap on one GPU share the
+// scratch and race.
+int pt_eval_noalloc(pt_plan *pl, const double *d_x, int ntrials, double *d_chi2) {
+    if (!pl || !d_x || !d_chi2 || ntrials < 1) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    if (!pl->follow.empty() || pl->P.has_free)
+        return fail(PT_ERR_STATE, "PT allocation-free evaluation takes one dataset without free intensities (use pt_eval).");
+    (void)hipSetDevice(pl->device);
+    unsigned char *base = nullptr;
+    size_t cap = 0;
+    HIPCHECK(ptk::noalloc_scratch(pl->device, &base, &cap), "scratch lookup");
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t n = (size_t)ntrials, nl = 2 * (size_t)pl->nr;
+    const size_t img_b = up(n * ptk::trial_image_bytes()), rec_b = up(n * nl * sizeof(ptk::LineRec)),
+                 span_b = up(n * nl * sizeof(int4)), part_b = up(n * pl->ngroups * sizeof(double2));
+    if (img_b + rec_b + span_b + part_b > cap) {
+        const size_t per = ptk::trial_image_bytes() + nl * (sizeof(ptk::LineRec) + sizeof(int4)) + pl->ngroups * sizeof(double2);
+        return fail(PT_ERR_BUDGET, "PT allocation-free evaluation takes at most " + std::to_string((cap - 1024) / per) +
+                                       " trials of this plan per call (" + std::to_string(cap >> 20) + " MiB scratch).");
+    }
+    void *img = base;
+    ptk::LineRec *rec = reinterpret_cast<ptk::LineRec *>(base + img_b);
+    int4 *span = reinterpret_cast<int4 *>(base + img_b + rec_b);
+    double2 *part = reinterpret_cast<double2 *>(base + img_b + rec_b + span_b);
+    HIPCHECK(ptk::launch_stage_setup(pl->P, d_x, img, ntrials, 0, 0, pl->stream), "setup");
+    HIPCHECK(ptk::launch_stage_lines(pl->P, img, rec, span, ntrials, pl->stream), "lines");
+    HIPCHECK(ptk::launch_stage_accum(pl->P, img, rec, span, part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
+    HIPCHECK(ptk::launch_stage_finish(pl->P, img, part, pl->ngroups, ntrials, d_chi2, 0, pl->stream), "finish");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "allocation-free evaluation");
+    return 0;
+}
+
 int pt_set_solve_limit(pt_plan *pl, int max_iterations) {
     if (!pl) return fail(PT_ERR_ARG, "NULL plan");
     if (max_iterations < 0) return fail(PT_ERR_ARG, "negative solve limit");

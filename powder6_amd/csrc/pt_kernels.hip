@@ -17,6 +17,7 @@
 // k_plain_*: one trial spread over the whole GPU (the plain form); k_matrix_*: the shared
 // profile matrix on MFMA; k_pattern / k_workload: one vector's pattern and work counts.
 #include "pt_device.h"
+#include "pt_launch.h"
 #include <algorithm>
 #include <mutex>
 
@@ -2308,6 +2309,31 @@ namespace ptk {
 size_t shared_static_bytes() { return sizeof(Shared); }
 size_t plain_scratch_bytes() { return sizeof(PlainScratch); }
 size_t trial_image_bytes() { return sizeof(TrialImage); }
+
+// ---- the fixed scratch of the allocation-free evaluation ------------------------------
+// pt_eval_noalloc stages its trials (trial images, line records and spans, chi^2
+// partials) here instead of in the plan's buffers.  Every GPU the library runs on holds
+// one copy for the life of the process.
+static __device__ __align__(256) unsigned char scratch[PTK_NOALLOC_BYTES];
+
+// The device address of `scratch` on `device` (which must be current), looked up once per
+// device: on the host the array's name is not a device address.
+hipError_t noalloc_scratch(int device, unsigned char **ptr, size_t *bytes) {
+    static std::mutex mu;
+    static unsigned char *addr[PTK_NOALLOC_DEVICES] = {};
+    if (device < 0 || device >= PTK_NOALLOC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = static_cast<unsigned char *>(p);
+    }
+    *ptr = addr[device];
+    *bytes = PTK_NOALLOC_BYTES;
+    return hipSuccess;
+}
 int accum_points_per_group(int tpw) { return acc_points(tpw); }
 
 // The dynamic-LDS limit of a kernel is one setting per process; every plan raises it to

@@ -7,6 +7,10 @@ namespace ptk {
 size_t shared_static_bytes();
 size_t plain_scratch_bytes();
 size_t trial_image_bytes();
+// the allocation-free evaluation's fixed device scratch: its address on `device` (current)
+#define PTK_NOALLOC_BYTES (32u << 20)
+#define PTK_NOALLOC_DEVICES 64
+hipError_t noalloc_scratch(int device, unsigned char **ptr, size_t *bytes);
 int accum_points_per_group(int tpw);   // points per accumulate workgroup at tpw tiles per wavefront
 hipError_t set_lds_limits(size_t bytes);
 // Every stage launch covers trials [t_off, t_off + ntrials) (t_off a multiple of 8, so

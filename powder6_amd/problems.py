@@ -202,6 +202,19 @@ class Problem:
         self.scale.append(pt_scale(names[0], v) if scale is None else scale)
         return j
 
+    def constrain(self, name, offset, terms):
+        """Model parameter `name` = offset + sum_(j, c) c x_j: a dependent parameter of an
+        exact constraint (js/atomic_constraints.js:76-155 hold / equiv / sum, and the
+        cross-dataset parameterLinks :134-148), in the form PT contexts carry it
+        (ctx.constraints.links, {d, name, offset, terms}).  Recorded in `links` too."""
+        k = self.index[name]
+        self.base[k] = float(offset)
+        for j, c in terms:
+            self.terms.append((k, int(j), float(c)))
+        if not hasattr(self, 'links'):
+            self.links = []
+        self.links.append((name, float(offset), [(int(j), float(c)) for j, c in terms]))
+
     @property
     def nv(self):
         return len(self.vars)
@@ -1052,3 +1065,124 @@ def triclinic_probe(profile='tch_aniso', simulate=None, neutron=False, sl=0.0, h
         pb.w[5] = 0
     ph.method = method                   # observed from the atoms; a free phase then frees them
     return pb
+
+
+# the sample values the reference's WebGPU test page gives a separated phase
+# (tests/webgpu-pt.js:23)
+PAGE_SAMPLE = {'size': .2, 'sizeAxial': .4, 'sizeB': .3, 'sizeC': .45, 'rotA': 13, 'rotB': 21, 'rotC': 33,
+               'strain': 150, 'strainAxial': 230, 'S400': 4, 'S040': 5, 'S004': 6, 'sizeMix': .6, 'strainMix': .4}
+PAGE_CELL = [3.7, 4.1, 4.3, 87, 94, 102]
+
+
+def _page_dataset(phases, neutron=False, lambda2=1.544, ratio=.5, resolution_type=None):
+    """A dataset of the page (tests/webgpu-pt.js:25): 160 points 8-63.65 deg step 0.35,
+    lambda 1.540598, polK .83, minFwhm .07, 2 background terms, w = .01 with point 5 out,
+    and its values: zero .03, disp 1.5, transp 1e-5, bkg 13 + 2 T1, every phase's scale
+    .012, po .87 and the page's sample values."""
+    tth = 8 + np.arange(160, dtype=np.float64) * .35
+    pb = Problem(tth, phases, lambda1=1.540598, lambda2=lambda2, ratio=ratio, pol_k=.83, min_fwhm=.07, n_bkg=2,
+                 radiation=1 if neutron else 0, resolution_type=resolution_type)
+    pb.default_values()
+    for n, v in (('zero', .03), ('disp', 1.5), ('transp', 1e-5), ('bkg0', 13.0), ('bkg1', 2.0)):
+        pb.set(n, v)
+    for ph in phases:
+        pb.set(ph.key + '.scale', .012)
+        pb.set(ph.key + '.po', .87)
+        for n, v in PAGE_SAMPLE.items():
+            pb.set(ph.key + '.' + n, v)
+    pb.w = np.full(len(tth), .01)
+    pb.w[5] = 0
+    return pb
+
+
+def _page_atoms(two=False):
+    ops = S.group_ops(2)
+    atoms = [Atom('C1', 'C', S.site_symmetry(ops[0], ops[1], [.13, .21, .32], PAGE_CELL), B=.5, ff=TRICLINIC_FF,
+                  b_neutron=6.646, occ=.4 if two else 1.0)]
+    if two:
+        atoms.append(Atom('C2', 'C', S.site_symmetry(ops[0], ops[1], [.26, .33, .41], PAGE_CELL), B=.5, ff=TRICLINIC_FF,
+                          b_neutron=6.646, occ=.6))
+    return atoms
+
+
+def _page_observe(pbs, simulate, amp=2.0):
+    """y = model at the start + amp sin(0.13 j) + 1 (tests/webgpu-pt.js:95, 104)."""
+    for pb in pbs:
+        y = np.asarray(simulate(pb, pb.start_vector()), dtype=np.float64)
+        j = np.arange(len(y))
+        pb.y = y + amp * np.sin(j * .13) + 1.0
+
+
+def page_joint_case(simulate=None):
+    """The page's joint case (tests/webgpu-pt.js:83-99): the gsas_cw P-1 probe with a second
+    atom C2 at (.26, .33, .41), and
+      * atomic constraints (js/atomic_constraints.js:76-155): C1.B held; C2.x = 0.5 C1.x
+        (equivalence, mult 0.5); C1.occ + C2.occ = 1 (constraint equation, C2.occ the
+        dependent, as the last refined term);
+      * a WPPF phase p2 (a copy of p1, free intensities) beside the Rietveld phase p1 in
+        the X-ray dataset;
+      * a second dataset, neutron, without the doublet and without p2 (a phase absent from
+        one dataset), with its own bkg0 = 18 and zero = -0.02.
+    Variables: p1 cell, disp, transp, scale, po, C1 x y z occ, inst.W, inst.X, size shared by
+    both datasets; zero and bkg0 per dataset.  The observations are each dataset's model at
+    the start (p2 from its atoms) + 2 sin(0.13 j) + 1."""
+    p1 = Phase('p1', 2, PAGE_CELL, _page_atoms(True), 'gsas_cw', po_dir=(0, 0, 1), separated=True)
+    p2 = Phase('p2', 2, PAGE_CELL, _page_atoms(True), 'gsas_cw', po_dir=(0, 0, 1), separated=True)
+    xr = _page_dataset([p1, p2])
+    nt = _page_dataset([Phase('p1', 2, PAGE_CELL, _page_atoms(True), 'gsas_cw', po_dir=(0, 0, 1), separated=True)],
+                       neutron=True, lambda2=0.0, ratio=0.0)
+    for pb in (xr, nt):
+        pb.set('p1.C2.x', .065)
+        pb.set('p1.C1.occ', .4)
+        pb.set('p1.C2.occ', .6)
+    for n in ('C1.x', 'C1.y', 'C1.z', 'C1.occ', 'C1.B', 'C2.x', 'C2.y', 'C2.z', 'C2.occ', 'C2.B'):
+        xr.set('p2.' + n, xr.base[xr.index['p1.' + n]])
+    nt.set('bkg0', 18.0)
+    nt.set('zero', -.02)
+    shared = ['p1.a', 'p1.b', 'p1.c', 'disp', 'transp', 'p1.scale', 'p1.po', 'p1.C1.x', 'p1.C1.y', 'p1.C1.z',
+              'p1.C1.occ', 'inst.W', 'inst.X', 'p1.size']
+    own = [['zero', 'bkg0'], ['zero', 'bkg0']]
+    limits = {'zero': (-2, 2), 'p1.scale': (0, math.inf), 'p1.po': (.1, 5), 'p1.C1.occ': (0, 1), 'inst.X': (0, math.inf),
+              'p1.size': (.002, math.inf)}
+    for n in ('a', 'b', 'c'):
+        v = xr.base[xr.index['p1.' + n]]
+        limits['p1.' + n] = (.96 * v, 1.04 * v)
+    for n in ('x', 'y', 'z'):
+        v = xr.base[xr.index['p1.C1.' + n]]
+        limits['p1.C1.' + n] = (v - .1, v + .1)
+    link_datasets([xr, nt], shared, own, limits)
+    jx, jo = shared.index('p1.C1.x'), shared.index('p1.C1.occ')
+    for pb in (xr, nt):
+        pb.constrain('p1.C2.x', 0.0, [(jx, 0.5)])          # equivalence, mult [1, .5]
+        pb.constrain('p1.C2.occ', 1.0, [(jo, -1.0)])       # C1.occ + C2.occ = 1
+        pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = 4, 10, 0, 5.0, 137
+    if simulate is not None:
+        _page_observe([xr, nt], simulate)
+    p2.method = 'wppf'                       # observed from its atoms; the fit frees its intensities
+    return [xr, nt]
+
+
+def page_scoped_case(simulate=None):
+    """The page's scoped case (tests/webgpu-pt.js:100-106): two copies of the gsas_cw P-1
+    dataset, the first with the dataset resolution function simple_pvoigt and the second
+    split_pvoigt, every instrument value at its default (js/rietveld_core.js:459
+    ALL_INSTRUMENT_PROFILE); variables bkg0, zero and C1.B per dataset, and the
+    cross-dataset link C1.B(2) = 1.2 C1.B(1) + 0.1 (parameterLinks), so C1.B of the second
+    dataset is not a variable.  Observations: model + 1 + sin(0.13 j)."""
+    pbs = []
+    for res in ('simple_pvoigt', 'split_pvoigt'):
+        ph = Phase('p1', 2, PAGE_CELL, _page_atoms(False), 'gsas_cw', po_dir=(0, 0, 1), separated=True)
+        pb = _page_dataset([ph], resolution_type=res)
+        for n, v in zip(INST_NAMES, INST_DEFAULTS):
+            pb.set('inst.' + n, v)
+        pbs.append(pb)
+    pbs[1].set('p1.C1.B', .7)
+    link_datasets(pbs, [], [['bkg0', 'zero', 'p1.C1.B'], ['bkg0', 'zero']],
+                  {'zero': (-2, 2), 'p1.C1.B': (-2, 30)})
+    jb = pbs[0].vars.index('p1.C1.B@0')
+    pbs[1].constrain('p1.C1.B', 0.1, [(jb, 1.2)])
+    for pb in pbs:
+        pb.chains, pb.sweeps, pb.steps_per_sweep, pb.max_temp, pb.seed = 4, 10, 0, 5.0, 137
+    if simulate is not None:
+        _page_observe(pbs, simulate, amp=1.0)
+    return pbs

@@ -50,6 +50,14 @@ function out(o) {
 }
 if (mode === 'compile') {
     out({ compile: RVPTGPU.compile(ctx) });
+} else if (mode === 'flat') {
+    // the C-ABI problems the host flattens the context into (model-parameter terms)
+    var flat = RVPTGPU.fromContext(ctx), list = Array.isArray(flat) ? flat : [flat];
+    out({ flat: list.map(function (f) {
+        return { paramBase: Array.from(f.paramBase), termParam: Array.from(f.termParam), termVar: Array.from(f.termVar),
+                 termCoeff: Array.from(f.termCoeff), resolutionType: f.resolutionType, radiation: f.radiation,
+                 phases: f.phases.map(function (p) { return { free: p.freeIntensities, atoms: p.atoms.length }; }) };
+    }) });
 } else if (mode === 'cost') {
     // ctx.cost of the start vector and of the probes, without a device
     out({ cost: [ctx.x].concat(ctx.probes || []).map(function (v) { return ctx.cost(v); }) });

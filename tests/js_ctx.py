@@ -25,12 +25,16 @@ def _values(pb):
     return values
 
 
-def _dataset(pb):
-    anom = [[[at.fp, at.fpp] for at in ph.atoms] for ph in pb.phases]
+def _dataset(pb, model_phases=None):
+    """Model.datasets[d] of the context: `has` and `anom` follow the model's phase list
+    (a phase may be absent from a dataset)."""
+    model_phases = model_phases or pb.phases
+    own = {ph.key: ph for ph in pb.phases}
+    anom = [[[at.fp, at.fpp] for at in own[ph.key].atoms] if ph.key in own else [] for ph in model_phases]
     ds = {'def': {'tth': pb.tth.tolist(), 'y': pb.y.tolist(), 'w': pb.w.tolist(), 'lambda1': pb.lambda1,
                   'lambda2': pb.lambda2, 'ratio': pb.ratio, 'polK': pb.pol_k, 'minFwhm': pb.min_fwhm,
                   'radiation': 'neutron' if pb.radiation else 'xray', 'resolutionType': pb.resolution_type},
-          'has': [True] * len(pb.phases), 'anom': anom}
+          'has': [ph.key in own for ph in model_phases], 'anom': anom}
     if getattr(pb, 'bkg_type', PR.BKG_CHEBYSHEV) == PR.BKG_SPLINE:
         ds.update(spline=True, T=[], bkgX=[float(v) for v in pb.bkg_x])
     else:
@@ -46,9 +50,13 @@ def reference_ctx(pb, initial_cost=None):
         ctx = reference_ctx(pb[0], initial_cost)
         ctx['problem']['values'] = [_values(q) for q in pb]
         ctx['problem']['vars'] = [{'name': n.split('@')[0]} for n in pb[0].vars]
-        ctx['model']['datasets'] = [_dataset(q) for q in pb]
+        ctx['model']['datasets'] = [_dataset(q, pb[0].phases) for q in pb]
         for j, n in enumerate(pb[0].vars):
             ctx['info'][j]['ds'] = [int(n.split('@')[1])] if '@' in n else list(range(len(pb)))
+        links = [{'d': d, 'name': name, 'offset': off, 'terms': [[j, c] for j, c in terms]}
+                 for d, q in enumerate(pb) for (name, off, terms) in getattr(q, 'links', [])]
+        if links:
+            ctx['constraints'] = {'links': links, 'names': sorted({l['name'] for l in links})}
         return ctx
     values = {}
     p = pb.values()
@@ -89,7 +97,9 @@ def reference_ctx(pb, initial_cost=None):
         'info': [{'lo': _num(lo), 'hi': _num(hi)} for lo, hi in zip(pb.lo, pb.hi)],
         'setup': {'scales': [float(s) for s in pb.scale], 'chains': pb.chains, 'sweeps': pb.sweeps,
                   'maxTemp': pb.max_temp, 'seed': pb.seed, 'initialCost': initial_cost},
-        'constraints': None, 'maxIter': pb.sweeps,
+        'constraints': ({'links': [{'d': 0, 'name': n, 'offset': o, 'terms': [[j, c] for j, c in t]}
+                                   for (n, o, t) in pb.links]} if getattr(pb, 'links', None) else None),
+        'maxIter': pb.sweeps,
     }
 
 

@@ -99,3 +99,48 @@ def test_twelve_thousand_points():
     pb = PR.triclinic_probe('gsas_cw', n_points=12000, span=64.0, simulate=O.oracle_simulate)
     pb.w = np.full(len(pb.tth), .01)
     assert _check(pb) <= TOL
+
+
+def _multi_probes(pbs):
+    """x and x moved by 0.1 PT scales, clipped (webgpu-pt.js:97, 105), and three more
+    seeded moves across the boxes."""
+    pb = pbs[0]
+    x0 = pb.start_vector()
+    sc, lo, hi = np.array(pb.scale), np.array(pb.lo), np.array(pb.hi)
+    rng = np.random.default_rng(83)
+    X = [x0, np.clip(x0 + .1 * sc, lo, hi)] + [np.clip(x0 + rng.uniform(-.5, .5, len(x0)) * sc, lo, hi) for _ in range(3)]
+    return np.array(X)
+
+
+def _check_multi(pbs):
+    ms = [O.Model(pb) for pb in pbs]
+    X = _multi_probes(pbs)
+    with pt.Plan(pbs) as pl:
+        g = pl.costs(X)
+        # a few PT sweeps: every chain's state re-scored by the oracle
+        pl.initial([X[0]])
+        pl.batch(1, bootstrap=True)
+        S = pl.batch(3).reshape(pl.chains, pl.stride)
+        nv = pl.nv
+    worst = 0.0
+    for c, gc in enumerate(g):
+        cpu = sum(m.cost(X[c % len(X)]) for m in ms)
+        worst = max(worst, abs(gc - cpu) / max(1.0, cpu))
+    for c in range(S.shape[0]):
+        cpu = sum(m.cost(S[c, :nv]) for m in ms)
+        worst = max(worst, abs(S[c, 3 * nv] - cpu) / max(1.0, cpu))
+    assert S[:, 3 * nv + pt.S_PROPOSALS].sum() == S.shape[0] * 3 * nv
+    return worst
+
+
+def test_joint_constraints_mixed_phases_absent_phase():
+    """webgpu-pt.js:83-99: held B, equivalence C2.x = 0.5 C1.x, C1.occ + C2.occ = 1, a WPPF
+    phase beside the Rietveld phase in the X-ray dataset and absent from the neutron
+    dataset, per-dataset bkg0 and zero."""
+    assert _check_multi(PR.page_joint_case(simulate=O.oracle_simulate)) <= TOL
+
+
+def test_scoped_resolution_functions_and_cross_dataset_link():
+    """webgpu-pt.js:100-106: resolutionType simple_pvoigt in one dataset and split_pvoigt
+    in the other, and C1.B of the second dataset = 1.2 C1.B of the first + 0.1."""
+    assert _check_multi(PR.page_scoped_case(simulate=O.oracle_simulate)) <= TOL

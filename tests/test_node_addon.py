@@ -216,3 +216,46 @@ def test_cost_pipe_answers_with_the_oracle(tmp_path):
     assert 'cost' in out, (out, r.stderr)
     want = [m.cost(pb.start_vector())] + [m.cost(p) for p in probes]
     assert out['cost'] == want
+
+
+def _python_terms(pb):
+    return sorted((int(k), int(j), float(c)) for (k, j, c) in pb.terms)
+
+
+@needs_node
+@pytest.mark.parametrize('case', ['page_joint_case', 'page_scoped_case'])
+def test_context_constraints_flatten_like_the_problem(case, tmp_path):
+    """The page's joint and scoped contexts (tests/webgpu-pt.js:83-106) carry their
+    atomic constraints and cross-dataset link as ctx.constraints.links; the Node host
+    flattens them into exactly the model-parameter terms and bases of the Python problem."""
+    pbs = getattr(PR, case)()
+    for pb in pbs:
+        pb.y = np.ones_like(pb.tth)
+    path = js_ctx.write_ctx(pbs, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'flat')
+    assert 'flat' in out, r.stderr
+    assert len(out['flat']) == len(pbs)
+    for pb, f in zip(pbs, out['flat']):
+        js_terms = sorted(zip(f['termParam'], f['termVar'], f['termCoeff']))
+        assert js_terms == _python_terms(pb)
+        assert np.array_equal(np.array(f['paramBase']), pb.base)
+        assert f['resolutionType'] == (PR.PROFILE[pb.resolution_type] if pb.resolution_type else 0)
+        assert [p['free'] for p in f['phases']] == [0 if ph.method == 'rietveld' else 1 for ph in pb.phases]
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', ['page_joint_case', 'page_scoped_case'])
+def test_node_costs_page_joint_and_scoped_cases(case, tmp_path):
+    """The page's joint and scoped cases through the Node host: costs() of the probe
+    vectors equal the oracle's sums over the datasets to 1e-5."""
+    pbs = getattr(PR, case)(simulate=O.oracle_simulate)
+    ms = [O.Model(pb) for pb in pbs]
+    x0 = pbs[0].start_vector()
+    probes = [x0, np.clip(x0 + .1 * np.array(pbs[0].scale), pbs[0].lo, pbs[0].hi)]
+    path = js_ctx.write_ctx(pbs, str(tmp_path / 'ctx.json'), probes=[list(map(float, p)) for p in probes])
+    out, r = _node(path, 'costs')
+    assert 'costs' in out, (out, r.stderr)
+    for c, g in enumerate(out['costs']):
+        cpu = sum(m.cost(probes[c % len(probes)]) for m in ms)
+        assert abs(g - cpu) / max(1.0, cpu) <= 1e-5

@@ -263,6 +263,20 @@ int pt_eval(pt_plan *plan, int form, const double *x, int ntrials, double *chi2)
  * GPU share the scratch: calls that overlap in time on one GPU race. */
 int pt_eval_noalloc(pt_plan *plan, const double *d_x, int ntrials, double *d_chi2);
 
+/* The normal equations of one Levenberg-Marquardt step at the variables x, the step
+ * after PT (js/rietveld_core.js:1953-1983 normal): A = J'WJ (nv x nv, row major),
+ * g = J'W(y - ycalc) (nv), *chi2 = sum_{w>0} w (y - ycalc)^2, summed over the plan's
+ * datasets; with J not NULL also the derivative columns, J[j * N + i] = d ycalc_i / d x_j
+ * over the N points of all datasets in order.  The columns are the reference's
+ * (Model.derivatives, js/rietveld_core.js:1245-1298): symmetric finite differences with
+ * its fdStep and defaultLimits for cell, zero, displacement, transparency, profile and
+ * instrument parameters, the Chebyshev polynomials (a one-sided spline difference for
+ * spline anchors), the unit-scale pattern, the March-Dollase difference and the analytic
+ * |F|^2 gradients for atoms, through the plan's links.  Everything runs in FP64 on the GPU;
+ * the elapsed device time is reported by pt_last_timing.  PT_ERR_STATE for plans with
+ * free intensities or separated profiles. */
+int pt_normal_equations(pt_plan *plan, const double *x, double *A, double *g, double *chi2, double *J);
+
 /* Pattern of one structure-only vector through the shared profile matrix (PT_FORM_MATRIX
  * path): background of the start vector + P I.  No counterpart in the reference. */
 int pt_matrix_pattern(pt_plan *plan, const double *x, double *ycalc);

@@ -1660,3 +1660,372 @@ double or_temper_parallel(const pt_problem *prob, const double *scales, int swee
     free(th); free(jobs);
     return total / sec;
 }
+
+/* ------------------------------------------------------------------------------ */
+/*  the LM normal equations: Model.derivatives (rietveld_core.js:1245-1298) and      */
+/*  normal() (:1953-1983), FP64                                                      */
+/* ------------------------------------------------------------------------------ */
+static const char *const OR_INST_NAMES[PT_N_INST] = {"U", "V", "W", "X", "Y", "Z", "SL", "HL", "eta", "U_L", "V_L",
+                                                     "W_L", "X_L", "U_R", "V_R", "W_R", "X_R", "eta_split"};
+static const char *const OR_TCH_NAMES[7] = {"U", "V", "W", "X", "Y", "SL", "HL"};
+static const char *const OR_SIM_NAMES[6] = {"GU", "GV", "GW", "GP", "LX", "eta"};
+static const char *const OR_SPL_NAMES[9] = {"GU_L", "GV_L", "GW_L", "LX_L", "GU_R", "GV_R", "GW_R", "LX_R", "eta_split"};
+static const char *const OR_CELL_NAMES[6] = {"a", "b", "c", "alpha", "beta", "gamma"};
+static const char *const OR_ATOM_COMP[5] = {"x", "y", "z", "occ", "B"};
+
+/* The reference's name of model parameter k (the value-map names Model reads: zero,
+ * bkg<k> / bga<k>, inst.<n>, p<q>.<cell|scale|po|profile slot>, p<q>.A<j>.<comp>). */
+static void or_param_name(const or_model *m, int k, char *out, size_t cap) {
+    if (k == PT_P_ZERO) { snprintf(out, cap, "zero"); return; }
+    if (k == PT_P_DISP) { snprintf(out, cap, "disp"); return; }
+    if (k == PT_P_TRANSP) { snprintf(out, cap, "transp"); return; }
+    if (k < PT_P_INST0) { snprintf(out, cap, "%s%d", m->P.bkg_type == PT_BKG_SPLINE ? "bga" : "bkg", k - PT_P_BKG0); return; }
+    if (k < PT_P_PHASE0) { snprintf(out, cap, "inst.%s", OR_INST_NAMES[k - PT_P_INST0]); return; }
+    for (int q = 0; q < m->P.n_phases; q++) {
+        int rel = k - phase_base(&m->P, q), na = m->phases[q].n_atoms;
+        if (rel < 0 || rel >= PT_PH_ATOM0 + PT_AT_STRIDE * na) continue;
+        if (rel < 6) { snprintf(out, cap, "p%d.%s", q + 1, OR_CELL_NAMES[rel]); return; }
+        if (rel == PT_PH_SCALE) { snprintf(out, cap, "p%d.scale", q + 1); return; }
+        if (rel == PT_PH_PO) { snprintf(out, cap, "p%d.po", q + 1); return; }
+        if (rel < PT_PH_ATOM0) {
+            int s = rel - PT_PH_PROF0, fam = m->phases[q].profile_type;
+            const char *n = NULL;
+            if (fam == PT_PROFILE_SIMPLE && s < 6) n = OR_SIM_NAMES[s];
+            else if (fam == PT_PROFILE_SPLIT && s < 9) n = OR_SPL_NAMES[s];
+            else if ((fam == PT_PROFILE_TCH || fam == PT_PROFILE_GSAS_CW) && s < 7) n = OR_TCH_NAMES[s];
+            if (n) snprintf(out, cap, "p%d.%s", q + 1, n); else snprintf(out, cap, "p%d.slot%d", q + 1, s);
+            return;
+        }
+        rel -= PT_PH_ATOM0;
+        snprintf(out, cap, "p%d.A%d.%s", q + 1, rel / PT_AT_STRIDE, OR_ATOM_COMP[rel % PT_AT_STRIDE]);
+        return;
+    }
+    snprintf(out, cap, "?");
+}
+
+/* classify (:470-484) */
+enum { OK_UNKNOWN, OK_INSTRUMENT, OK_ZERO, OK_DISP, OK_TRANSP, OK_BKG, OK_BGA, OK_SCALE, OK_PO, OK_CELL, OK_PROFILE, OK_ATOM };
+typedef struct { int kind, phase, index, atom; char comp[64]; } or_class;
+static or_class or_classify(const char *name) {
+    or_class c;
+    memset(&c, 0, sizeof c);
+    c.kind = OK_UNKNOWN; c.phase = -1;
+    if (!strncmp(name, "inst.", 5)) { c.kind = OK_INSTRUMENT; snprintf(c.comp, sizeof c.comp, "%s", name + 5); return c; }
+    if (!strcmp(name, "zero")) { c.kind = OK_ZERO; return c; }
+    if (!strcmp(name, "disp")) { c.kind = OK_DISP; return c; }
+    if (!strcmp(name, "transp")) { c.kind = OK_TRANSP; return c; }
+    if (!strncmp(name, "bkg", 3)) { c.kind = OK_BKG; c.index = atoi(name + 3); return c; }
+    if (!strncmp(name, "bga", 3)) { c.kind = OK_BGA; c.index = atoi(name + 3); return c; }
+    if (name[0] == 'p') {
+        int q = atoi(name + 1) - 1;
+        const char *d1 = strchr(name, '.');
+        if (!d1) return c;
+        const char *d2 = strchr(d1 + 1, '.');
+        c.phase = q;
+        if (!d2) {
+            const char *r = d1 + 1;
+            if (!strcmp(r, "scale")) c.kind = OK_SCALE;
+            else if (!strcmp(r, "po")) c.kind = OK_PO;
+            else {
+                c.kind = OK_PROFILE;
+                for (int i = 0; i < 6; i++) if (!strcmp(r, OR_CELL_NAMES[i])) c.kind = OK_CELL;
+            }
+            snprintf(c.comp, sizeof c.comp, "%s", r);
+            return c;
+        }
+        c.kind = OK_ATOM; c.atom = atoi(d1 + 2);
+        snprintf(c.comp, sizeof c.comp, "%s", d2 + 1);
+    }
+    return c;
+}
+/* fdStep (:526-537) */
+static double or_fd_step(const or_class *c, double v) {
+    double a = fabs(v);
+    switch (c->kind) {
+    case OK_CELL: return (!strcmp(c->comp, "a") || !strcmp(c->comp, "b") || !strcmp(c->comp, "c")) ? dmax(1e-6 * a, 1e-6) : 1e-4;
+    case OK_ZERO: return 1e-4;
+    case OK_DISP: return 1e-2;
+    case OK_TRANSP: return 1e-6;
+    case OK_PO: return 1e-4;
+    case OK_BGA: return dmax(1e-4 * a, 1e-3);
+    default: return dmax(1e-4 * a, 1e-5);
+    }
+}
+static int or_is(const char *s, const char *const *list, int n) {
+    for (int i = 0; i < n; i++) if (!strcmp(s, list[i])) return 1;
+    return 0;
+}
+/* defaultLimits (:486-508) */
+static void or_default_limits(const or_class *c, double v, double *lo, double *hi) {
+    *lo = -INFINITY; *hi = INFINITY;
+    const char *s = c->comp;
+    switch (c->kind) {
+    case OK_SCALE: *lo = 0; break;
+    case OK_CELL:
+        if (!strcmp(s, "a") || !strcmp(s, "b") || !strcmp(s, "c")) { *lo = 0.5 * v; *hi = 2 * v; } else { *lo = 30; *hi = 150; }
+        break;
+    case OK_ATOM:
+        if (!strcmp(s, "occ")) { *lo = 0; *hi = 2; } else if (!strcmp(s, "B")) { *lo = -2; *hi = 30; }
+        break;
+    case OK_PO: *lo = 0.1; *hi = 5; break;
+    case OK_ZERO: *lo = -2; *hi = 2; break;
+    case OK_INSTRUMENT: {
+        static const char *const pos[] = {"X", "X_L", "X_R", "Y", "Z", "SL"};
+        if (!strncmp(s, "eta", 3)) { *lo = 0; *hi = 1; } else if (or_is(s, pos, 6)) *lo = 0;
+        break;
+    }
+    case OK_PROFILE: {
+        static const char *const sizes[] = {"size", "sizeAxial", "sizeB", "sizeC"}, *mixes[] = {"sizeMix", "strainMix"},
+                                 *quart[] = {"S400", "S040", "S004"}, *pos[] = {"LX", "LX_L", "LX_R", "X", "Y", "GP"};
+        if (or_is(s, sizes, 4)) { *lo = 0.001; *hi = 10; }
+        else if (or_is(s, mixes, 2)) { *lo = 0; *hi = 1; }
+        else if (!strncmp(s, "strain", 6)) { *lo = 0; *hi = 1e6; }
+        else if (or_is(s, quart, 3)) *lo = 0;
+        else if (!strncmp(s, "eta", 3)) { *lo = 0; *hi = 1; }
+        else if (or_is(s, pos, 6)) *lo = 0;
+        break;
+    }
+    default: break;
+    }
+}
+
+/* combine(ph.cols, dh, n) (:1192-1200): sum over the reflections of dh_k times the
+ * reflection's unit-height column (intensityDerivativeColumn, profile.js:1131-1150) */
+static void or_combine(or_model *m, const refl_work *rw, int nr, const double *dh, double *out) {
+    for (int k = 0; k < nr; k++) {
+        const refl_work *R = &rw[k];
+        if (!R->has_col || !(dh[k] != 0)) continue;
+        int len = R->stop - R->start;
+        double *col = m->col;
+        for (int i = 0; i < len; i++) col[i] = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            if (!R->present[pass]) continue;
+            for (int i = R->cstart[pass]; i < R->cstop[pass]; i++) col[i - R->start] += R->weight[pass] * or_eval_voigt(m->tth[i], &R->v[pass]);
+        }
+        for (int i = 0; i < len; i++) out[R->start + i] += dh[k] * col[i];
+    }
+}
+
+/* the gradient slot of Model.structure with grad (:749-821) for atom j, component comp
+ * (0-2 a free coordinate, 3 occ, 4 B): G[k] = d|F_k|^2 / d(slot) */
+static void or_structure_grad(or_model *m, int q, const double *pp, const refl_work *rw, int ja, int comp, double *G) {
+    const pt_phase *ph = &m->phases[q];
+    or_phase *op = &m->ph[q];
+    int nA = ph->n_atoms, nR = op->nrep, rad = m->P.radiation, neutron = rad == PT_NEUTRON;
+    double *pos = (double *)malloc(sizeof(double) * 3 * (nA > 0 ? nA : 1));
+    for (int j = 0; j < nA; j++) {
+        const pt_atom *a = &ph->atoms[j];
+        const double *ap = pp + PT_PH_ATOM0 + PT_AT_STRIDE * j;
+        double r[3] = {a->r0[0], a->r0[1], a->r0[2]};
+        for (int f = 0; f < 3; f++) if (a->free_mask & (1 << f)) for (int i = 0; i < 3; i++) r[i] += ap[f] * a->basis[f][i];
+        for (int i = 0; i < 3; i++) pos[3 * j + i] = r[i];
+    }
+    refl_list *L = &op->refl;
+    for (int k = 0; k < L->n; k++) {
+        G[k] = 0;
+        if (!js_finite(rw[k].tth)) continue;
+        double s = rw[k].s, s2 = s * s, Ar = 0, Br = 0, Cr = 0, Dr = 0, Aj = 0, Bj = 0, Cj = 0, Dj = 0;
+        double dA[3] = {0, 0, 0}, dB[3] = {0, 0, 0}, dC[3] = {0, 0, 0}, dD[3] = {0, 0, 0};
+        int h = L->r[k].h, kk = L->r[k].k, l = L->r[k].l;
+        for (int j = 0; j < nA; j++) {
+            const pt_atom *a = &ph->atoms[j];
+            const double *ap = pp + PT_PH_ATOM0 + PT_AT_STRIDE * j;
+            double occ = ap[PT_AT_OCC], B = ap[PT_AT_B], w0 = (a->mult != 0 ? a->mult : 1) / nR;
+            double dw = exp(-B * s2 / 4), f0 = form_factor(a, rad, s);
+            double wj = occ * (neutron ? f0 : f0 + a->fp) * dw * w0, wi = (neutron || !(a->fpp != 0)) ? 0 : occ * a->fpp * dw * w0;
+            double c = 0, sn = 0, dc[3] = {0, 0, 0}, ds[3] = {0, 0, 0};
+            for (int qq = 0; qq < nR; qq++) {
+                const int32_t *r = ph->op_r + 9 * op->rep[qq];
+                const double *t = ph->op_t + 3 * op->rep[qq];
+                double H[3] = {h * r[0] + kk * r[3] + l * r[6], h * r[1] + kk * r[4] + l * r[7], h * r[2] + kk * r[5] + l * r[8]};
+                double phs = TWO_PI * (H[0] * pos[3 * j] + H[1] * pos[3 * j + 1] + H[2] * pos[3 * j + 2] + h * t[0] + kk * t[1] + l * t[2]);
+                double cs = cos(phs), si = sin(phs);
+                c += cs; sn += si;
+                for (int i = 0; i < 3; i++) { dc[i] -= H[i] * si; ds[i] += H[i] * cs; }
+            }
+            Ar += wj * c; Br += wj * sn; Cr += wi * c; Dr += wi * sn;
+            if (j == ja) {
+                Aj = wj * c; Bj = wj * sn; Cj = wi * c; Dj = wi * sn;
+                for (int i = 0; i < 3; i++) {
+                    dA[i] = TWO_PI * wj * dc[i]; dB[i] = TWO_PI * wj * ds[i]; dC[i] = TWO_PI * wi * dc[i]; dD[i] = TWO_PI * wi * ds[i];
+                }
+            }
+        }
+        const pt_atom *a = &ph->atoms[ja];
+        const double *ap = pp + PT_PH_ATOM0 + PT_AT_STRIDE * ja;
+        if (comp < 3) {
+            const double *b = a->basis[comp];
+            double dAr = dA[0] * b[0] + dA[1] * b[1] + dA[2] * b[2], dBr = dB[0] * b[0] + dB[1] * b[1] + dB[2] * b[2];
+            double dCr = dC[0] * b[0] + dC[1] * b[1] + dC[2] * b[2], dDr = dD[0] * b[0] + dD[1] * b[1] + dD[2] * b[2];
+            G[k] = 2 * (Ar * dAr + Br * dBr + Cr * dCr + Dr * dDr);
+        } else if (comp == 3) {
+            double oc = ap[PT_AT_OCC];
+            if (fabs(oc) > 1e-12) G[k] = 2 * (Ar * Aj + Br * Bj + Cr * Cj + Dr * Dj) / oc;
+            else {                                   /* occ = 0: the contribution per unit occupancy */
+                double f1 = form_factor(a, rad, s), dw1 = exp(-ap[PT_AT_B] * s2 / 4), w0 = (a->mult != 0 ? a->mult : 1) / nR;
+                double w1 = (neutron ? f1 : f1 + a->fp) * dw1 * w0, w1i = neutron ? 0 : a->fpp * dw1 * w0, c1 = 0, s1 = 0;
+                for (int qq = 0; qq < nR; qq++) {
+                    const int32_t *r = ph->op_r + 9 * op->rep[qq];
+                    const double *t = ph->op_t + 3 * op->rep[qq];
+                    double p1 = TWO_PI * ((h * r[0] + kk * r[3] + l * r[6]) * pos[3 * ja] + (h * r[1] + kk * r[4] + l * r[7]) * pos[3 * ja + 1] +
+                                          (h * r[2] + kk * r[5] + l * r[8]) * pos[3 * ja + 2] + h * t[0] + kk * t[1] + l * t[2]);
+                    c1 += cos(p1); s1 += sin(p1);
+                }
+                G[k] = 2 * (Ar * w1 * c1 + Br * w1 * s1 + Cr * w1i * c1 + Dr * w1i * s1);
+            }
+        } else {
+            G[k] = 2 * (Ar * Aj + Br * Bj + Cr * Cj + Dr * Dj) * (-s2 / 4);
+        }
+    }
+    free(pos);
+}
+
+/* the background at parameters p (Model.background, :920-936) */
+static void or_background_at(or_model *m, const double *p, double *y) {
+    int n = m->P.n_points;
+    for (int i = 0; i < n; i++) y[i] = 0;
+    if (m->P.bkg_type == PT_BKG_SPLINE) {
+        double ys[PT_MAX_BKG];
+        for (int j = 0; j < m->P.n_bkg; j++) ys[j] = isnan(p[PT_P_BKG0 + j]) ? 0 : p[PT_P_BKG0 + j];
+        or_spline_values(m->tth, n, m->bx, ys, m->P.n_bkg, y);
+    } else {
+        for (int k = 0; k < m->P.n_bkg; k++) {
+            double b = p[PT_P_BKG0 + k];
+            if (!(b != 0) || isnan(b)) continue;
+            for (int i = 0; i < n; i++) y[i] += b * m->T[k][i];
+        }
+    }
+}
+
+int or_normal(or_model *m, const double *x, double *A, double *g, double *f, double *J) {
+    consts();
+    const int n = m->P.n_points, nv = m->P.n_vars, npar = m->P.n_params, nph = m->P.n_phases;
+    for (int q = 0; q < nph; q++) if (m->phases[q].free_intensities) return -2;   /* variable projection: not restated */
+    or_params(m, x, m->params);
+    if (!cell_ok(m, m->params)) return -1;
+    double *p = dup_d(m->params, npar), *pm = dup_d(m->params, npar);
+    refresh_reflections(m, p);                          /* evalDataset; derivatives keep this list */
+    /* the evaluation at p: every phase's pattern and working data, the background */
+    double **yph = (double **)calloc(nph, sizeof(double *));
+    refl_work **rw = (refl_work **)calloc(nph, sizeof(refl_work *));
+    int *nr = (int *)calloc(nph, sizeof(int));
+    double *bkg = (double *)calloc(n, sizeof(double)), *yb = (double *)calloc(n, sizeof(double));
+    double *tmp = (double *)calloc(n, sizeof(double)), *tmp2 = (double *)calloc(n, sizeof(double));
+    for (int q = 0; q < nph; q++) {
+        yph[q] = (double *)calloc(n, sizeof(double));
+        m->nf = 0; m->fvals_n = 0;
+        eval_phase(m, q, p, yph[q], &rw[q], &nr[q]);
+    }
+    or_background_at(m, p, bkg);
+    /* the columns of the model parameters the variables act on */
+    double **col = (double **)calloc(npar, sizeof(double *));
+    for (int t = 0; t < m->P.n_terms; t++) {
+        int k = m->tparam[t];
+        if (col[k]) continue;
+        col[k] = (double *)calloc(n, sizeof(double));
+        char name[64];
+        or_param_name(m, k, name, sizeof name);
+        or_class c = or_classify(name);
+        double v = p[k];
+        int q = c.phase;
+        if (c.kind == OK_BKG) {
+            if (m->P.bkg_type != PT_BKG_SPLINE && c.index < m->P.n_bkg) memcpy(col[k], m->T[c.index], sizeof(double) * n);
+        } else if (c.kind == OK_BGA) {
+            if (m->P.bkg_type == PT_BKG_SPLINE && c.index < m->P.n_bkg) {
+                double ys[PT_MAX_BKG];
+                for (int j = 0; j < m->P.n_bkg; j++) ys[j] = isnan(p[PT_P_BKG0 + j]) ? 0 : p[PT_P_BKG0 + j];
+                double hb = or_fd_step(&c, ys[c.index]);
+                ys[c.index] += hb;
+                or_spline_values(m->tth, n, m->bx, ys, m->P.n_bkg, tmp);
+                for (int i = 0; i < n; i++) col[k][i] = (tmp[i] - bkg[i]) / hb;
+            }
+        } else if (c.kind == OK_SCALE || c.kind == OK_ATOM || c.kind == OK_PO) {
+            const double *pp = p + phase_base(&m->P, q);
+            double S = pp[PT_PH_SCALE];
+            if (!(S != 0) || isnan(S)) S = 0;
+            double *dh = (double *)calloc(nr[q] > 0 ? nr[q] : 1, sizeof(double));
+            const refl_list *L = &m->ph[q].refl;
+            int ok = 1;
+            if (c.kind == OK_SCALE) {
+                for (int kk = 0; kk < nr[q]; kk++) if (rw[q][kk].has_col) dh[kk] = L->r[kk].m * rw[q][kk].lp * rw[q][kk].po / rw[q][kk].K * rw[q][kk].f2;
+            } else if (c.kind == OK_ATOM) {
+                int comp = !strcmp(c.comp, "x") ? 0 : !strcmp(c.comp, "y") ? 1 : !strcmp(c.comp, "z") ? 2 : !strcmp(c.comp, "occ") ? 3 : 4;
+                if (comp < 3 && !(m->phases[q].atoms[c.atom].free_mask & (1 << comp))) ok = 0;   /* no gradient slot */
+                if (ok) {
+                    double *G = (double *)calloc(nr[q] > 0 ? nr[q] : 1, sizeof(double));
+                    or_structure_grad(m, q, pp, rw[q], c.atom, comp, G);
+                    for (int kk = 0; kk < nr[q]; kk++) if (rw[q][kk].has_col) dh[kk] = S * (L->r[kk].m * rw[q][kk].lp * rw[q][kk].po / rw[q][kk].K) * G[kk];
+                    free(G);
+                }
+            } else {
+                double h = or_fd_step(&c, v), cell[6], ff[6];
+                memcpy(pm, p, sizeof(double) * npar);
+                pm[k] = v + h;
+                const double *ppm = pm + phase_base(&m->P, q);
+                int have_f = cell_of(&m->phases[q], ppm, cell) && or_inv_dsq(cell, ff);
+                refl_work *rw2 = (refl_work *)malloc(sizeof(refl_work) * (nr[q] > 0 ? nr[q] : 1));
+                memcpy(rw2, rw[q], sizeof(refl_work) * nr[q]);
+                preferred_orientation(m, q, ppm, ff, have_f, rw2);
+                for (int kk = 0; kk < nr[q]; kk++)
+                    dh[kk] = rw[q][kk].po > 0 ? rw[q][kk].height * (rw2[kk].po / rw[q][kk].po - 1) / h : 0;
+                free(rw2);
+            }
+            if (ok) or_combine(m, rw[q], nr[q], dh, col[k]);
+            free(dh);
+        } else if (c.kind != OK_UNKNOWN) {
+            /* symmetric finite difference of the phases the parameter moves (:1275-1296) */
+            double h0 = or_fd_step(&c, v), lo, hi;
+            or_default_limits(&c, v, &lo, &hi);
+            int plus = v + h0 <= hi, minus = v - h0 >= lo;
+            double vp = plus ? v + h0 : v, vm = minus ? v - h0 : v, d = vp - vm;
+            if (d != 0) {
+                for (int qq = 0; qq < nph; qq++) {
+                    if ((c.kind == OK_CELL || c.kind == OK_PROFILE) && qq != q) continue;
+                    const double *ep = yph[qq], *em = yph[qq];
+                    if (plus) {
+                        memcpy(pm, p, sizeof(double) * npar); pm[k] = vp;
+                        for (int i = 0; i < n; i++) tmp[i] = 0;
+                        eval_phase(m, qq, pm, tmp, NULL, NULL);
+                        ep = tmp;
+                    }
+                    if (minus) {
+                        memcpy(pm, p, sizeof(double) * npar); pm[k] = vm;
+                        for (int i = 0; i < n; i++) tmp2[i] = 0;
+                        eval_phase(m, qq, pm, tmp2, NULL, NULL);
+                        em = tmp2;
+                    }
+                    for (int i = 0; i < n; i++) col[k][i] += (ep[i] - em[i]) / d;
+                }
+            }
+        }
+    }
+    /* the variables' columns through the links, and normal() */
+    for (int i = 0; i < n; i++) { yb[i] = bkg[i]; for (int q = 0; q < nph; q++) yb[i] += yph[q][i]; }
+    double *Jl = J ? J : (double *)calloc((size_t)nv * n, sizeof(double));
+    for (size_t i = 0; i < (size_t)nv * n; i++) Jl[i] = 0;
+    for (int t = 0; t < m->P.n_terms; t++) {
+        int k = m->tparam[t], j = m->tvar[t];
+        for (int i = 0; i < n; i++) Jl[(size_t)j * n + i] += m->coeff[t] * col[k][i];
+    }
+    double fs = 0;
+    for (int i = 0; i < n; i++) { if (!(m->w[i] > 0)) { tmp[i] = 0; continue; } tmp[i] = m->y[i] - yb[i]; fs += m->w[i] * tmp[i] * tmp[i]; }
+    for (int a = 0; a < nv; a++) {
+        const double *Ja = Jl + (size_t)a * n;
+        double ga = 0;
+        for (int i = 0; i < n; i++) ga += m->w[i] * Ja[i] * tmp[i];
+        g[a] = ga;
+        for (int b = 0; b <= a; b++) {
+            const double *Jb = Jl + (size_t)b * n;
+            double s = 0;
+            for (int i = 0; i < n; i++) s += m->w[i] * Ja[i] * Jb[i];
+            A[(size_t)a * nv + b] = s;
+            A[(size_t)b * nv + a] = s;
+        }
+    }
+    *f = fs;
+    if (!J) free(Jl);
+    for (int k = 0; k < npar; k++) free(col[k]);
+    for (int q = 0; q < nph; q++) { free(yph[q]); free(rw[q]); }
+    free(col); free(yph); free(rw); free(nr); free(bkg); free(yb); free(tmp); free(tmp2); free(p); free(pm);
+    return 0;
+}

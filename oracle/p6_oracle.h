@@ -111,6 +111,12 @@ double or_violation(or_model *m, const double *x);
 /* the profile-feasibility rule of pt_gpu.js propose (js/pt_gpu.js:361): 1 valid */
 int or_cell_valid(or_model *m, const double *x);
 
+/* The normal equations of one LM step at x (rietveld_core.js:1953-1983 normal with the
+ * columns of Model.derivatives :1245-1298): A[nv*nv] = J'WJ, g[nv] = J'W(y - ycalc),
+ * *f = sum_{w>0} w (y - ycalc)^2, and J[nv*n] (may be NULL).  Returns 0, -1 for an
+ * invalid cell, -2 with free-intensity phases (their variable projection is not restated). */
+int or_normal(or_model *m, const double *x, double *A, double *g, double *f, double *J);
+
 /* CPU PT temper() (rietveld_core.js:2171-2240).  scales[nv]; best_x[nv] out;
  * stats[4] = accepted, proposals, swaps, swap attempts.  Returns best cost. */
 double or_temper(or_model *m, const double *x0, const double *scales, int sweeps, int chains,

@@ -32,7 +32,7 @@ def run(cmd):
 def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
-    for src in ('pt_kernels.hip', 'pt_mlp.hip'):
+    for src in ('pt_kernels.hip', 'pt_mlp.hip', 'pt_jacobian.hip'):
         obj = os.path.join(LIBDIR, src.replace('.hip', '.o'))
         run([HIPCC] + DEVICE_FLAGS + ['-c', os.path.join(CSRC, src), '-o', obj])
         objs.append(obj)

@@ -17,6 +17,7 @@
 
 #include "../../include/powder6_pt.h"
 #include "pt_common.h"
+#include "pt_jacobian.h"
 #include "pt_launch.h"
 #include "pt_mlp_host.h"
 
@@ -276,6 +277,18 @@ struct pt_plan {
     double last_total_ms = 0, last_kernel_ms = 0, last_matrix_ms[2] = {0, 0};
     int last_launches = 0;
     double qlim = 0;
+    // the LM normal equations (FP64): what the column specification needs on the host, and
+    // the device buffers (allocated at the first pt_normal_equations call)
+    std::vector<ptk::JacAtom> jac_atoms_h;
+    std::vector<double> yobs_h, w_h, param_base_h, term_coeff_h;
+    std::vector<int> term_off_h, term_var_h;
+    ptk::JacAtom *d_jatoms = nullptr;
+    double *d_jy = nullptr, *d_jw = nullptr, *d_jp = nullptr, *d_jye = nullptr, *d_jC = nullptr, *d_jr = nullptr,
+           *d_jext = nullptr;
+    ptk::JacEval *d_jev = nullptr;
+    ptk::JacLine *d_jlines = nullptr;
+    int jac_cap_evals = 0;
+    double last_normal_ms = 0;
 };
 
 namespace {
@@ -767,6 +780,14 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
             d.fp = (float)a.fp;
             d.fpp = (float)a.fpp;
             atoms.push_back(d);
+            ptk::JacAtom ja{};
+            for (int x = 0; x < 3; ++x) { ja.r0[x] = a.r0[x]; for (int y = 0; y < 3; ++y) ja.basis[x][y] = a.basis[x][y]; }
+            for (int g = 0; g < PT_MAX_GAUSS; ++g) { ja.ff_a[g] = a.ff_a[g]; ja.ff_b[g] = a.ff_b[g]; }
+            ja.ff_c = a.ff_c;
+            ja.w0 = (a.mult != 0 ? a.mult : 1) / (double)all_reps.size();
+            ja.b_neutron = a.b_neutron; ja.fp = a.fp; ja.fpp = a.fpp;
+            ja.free_mask = a.free_mask; ja.n_gauss = a.n_gauss;
+            pl->jac_atoms_h.push_back(ja);
         }
         std::vector<int32_t> rows;
         for (auto &r : list) {
@@ -826,6 +847,10 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         term_off[k + 1] = (int)term_var.size();
     }
 
+    pl->yobs_h.assign(pb->y_obs, pb->y_obs + n);
+    pl->w_h.assign(pb->weight, pb->weight + n);
+    pl->param_base_h.assign(pb->param_base, pb->param_base + pb->n_params);
+    pl->term_off_h = term_off; pl->term_var_h = term_var; pl->term_coeff_h = term_coeff;
     if (!with_device) { *out = pl; return 0; }
     rc = device_check(&pl->device);
     if (rc) { delete pl; return rc; }
@@ -1755,6 +1780,227 @@ int pt_matrix_timing(const pt_plan *pl, double *intensity_ms, double *product_ms
     if (!pl) return fail(PT_ERR_ARG, "NULL plan");
     if (intensity_ms) *intensity_ms = pl->last_matrix_ms[0];
     if (product_ms) *product_ms = pl->last_matrix_ms[1];
+    return 0;
+}
+
+// ---- the LM normal equations (FP64 derivative columns) ------------------------------
+namespace {
+
+// what the derivative column of one model parameter is (js/rietveld_core.js:1245-1298):
+// a symmetric finite difference of the phase patterns (all phases, or the parameter's
+// own), a Chebyshev polynomial, a one-sided spline difference, the unit-scale pattern,
+// the March-Dollase difference, an atom's analytic |F|^2 gradient, or nothing
+enum { COL_NONE, COL_FD_ALL, COL_FD_PHASE, COL_CHEB, COL_SPLINE, COL_SCALE, COL_PO, COL_ATOM };
+struct ColSpec {
+    int kind, phase, slot;
+    double h, lo, hi;          // step (fdStep, :526-537) and box (defaultLimits, :486-508)
+};
+
+const char *const kInstNames[PTK_N_INST] = {"U", "V", "W", "X", "Y", "Z", "SL", "HL", "eta", "U_L", "V_L", "W_L",
+                                            "X_L", "U_R", "V_R", "W_R", "X_R", "eta_split"};
+const char *const kTchNames[] = {"U", "V", "W", "X", "Y", "SL", "HL"};
+const char *const kSimNames[] = {"GU", "GV", "GW", "GP", "LX", "eta"};
+const char *const kSplNames[] = {"GU_L", "GV_L", "GW_L", "LX_L", "GU_R", "GV_R", "GW_R", "LX_R", "eta_split"};
+
+std::string profile_slot_name(int family, int s) {
+    if (family == PT_PROFILE_SIMPLE) return s < 6 ? kSimNames[s] : "";
+    if (family == PT_PROFILE_SPLIT) return s < 9 ? kSplNames[s] : "";
+    return s < 7 ? kTchNames[s] : "";
+}
+
+ColSpec column_spec(const pt_plan *pl, int k, double v) {
+    const double inf = INFINITY;
+    ColSpec c{COL_NONE, -1, 0, 0.0, -inf, inf};
+    const double h_default = std::max(1e-4 * std::fabs(v), 1e-5);      // FD_BASE_FRACTION, constants.js:411
+    if (k == PT_P_ZERO) { c.kind = COL_FD_ALL; c.h = 1e-4; c.lo = -2; c.hi = 2; return c; }
+    if (k == PT_P_DISP) { c.kind = COL_FD_ALL; c.h = 1e-2; return c; }
+    if (k == PT_P_TRANSP) { c.kind = COL_FD_ALL; c.h = 1e-6; return c; }
+    if (k >= PT_P_BKG0 && k < PT_P_BKG0 + PT_MAX_BKG) {
+        const int b = k - PT_P_BKG0;
+        if (b >= pl->n_bkg) return c;
+        if (pl->bkg_type == PT_BKG_SPLINE) { c.kind = COL_SPLINE; c.slot = b; c.h = std::max(1e-4 * std::fabs(v), 1e-3); }
+        else { c.kind = COL_CHEB; c.slot = b; }
+        return c;
+    }
+    if (k >= PT_P_INST0 && k < PT_P_INST0 + PT_N_INST) {
+        const std::string n = kInstNames[k - PT_P_INST0];
+        c.kind = COL_FD_ALL; c.h = h_default;
+        if (n.compare(0, 3, "eta") == 0) { c.lo = 0; c.hi = 1; }
+        else if (n == "X" || n == "X_L" || n == "X_R" || n == "Y" || n == "Z" || n == "SL") c.lo = 0;
+        return c;
+    }
+    for (int q = 0; q < pl->P.n_phases; ++q) {
+        const ptk::DevPhase &ph = pl->P.phase[q];
+        const int rel = k - ph.param_base;
+        if (rel < 0 || rel >= PTK_PH_ATOM0 + PT_AT_STRIDE * ph.n_atoms) continue;
+        c.phase = q;
+        if (rel < 6) {                                   // cell
+            c.kind = COL_FD_PHASE;
+            if (rel < 3) { c.h = std::max(1e-6 * std::fabs(v), 1e-6); c.lo = 0.5 * v; c.hi = 2 * v; }
+            else { c.h = 1e-4; c.lo = 30; c.hi = 150; }
+            return c;
+        }
+        if (rel == PT_PH_SCALE) { c.kind = COL_SCALE; return c; }
+        if (rel == PT_PH_PO) { c.kind = COL_PO; c.h = 1e-4; return c; }
+        if (rel < PTK_PH_ATOM0) {                        // profile slot
+            const std::string n = profile_slot_name(ph.profile_type, rel - PT_PH_PROF0);
+            c.kind = COL_FD_PHASE; c.h = h_default;
+            if (n.compare(0, 3, "eta") == 0) { c.lo = 0; c.hi = 1; }
+            else if (n == "LX" || n == "LX_L" || n == "LX_R" || n == "X" || n == "Y" || n == "GP") c.lo = 0;
+            return c;
+        }
+        const int a = (rel - PTK_PH_ATOM0) / PT_AT_STRIDE, comp = (rel - PTK_PH_ATOM0) % PT_AT_STRIDE;
+        // a coordinate the site fixes has no gradient slot, so no column (:1264-1266)
+        if (comp < 3 && !(pl->jac_atoms_h[ph.atom_off + a].free_mask & (1 << comp))) return c;
+        c.kind = COL_ATOM; c.slot = PT_AT_STRIDE * a + comp;
+        return c;
+    }
+    return c;
+}
+
+// one dataset: the packed upper triangle of [J; r] W [J; r]' into ext, and [J; r] into jr
+// when it is not null ((nv + 1) rows of np)
+int normal_dataset(pt_plan *pl, const double *x, std::vector<double> &ext, std::vector<double> *jr) {
+    const int nv = pl->nv, np = pl->np, npar = pl->n_params;
+    std::vector<double> p(pl->param_base_h);
+    for (int k = 0; k < npar; ++k)
+        for (int t = pl->term_off_h[k]; t < pl->term_off_h[k + 1]; ++t) p[k] += pl->term_coeff_h[t] * x[pl->term_var_h[t]];
+    const int all = (1 << pl->P.n_phases) - 1;
+    std::vector<double> pv(p);
+    std::vector<ptk::JacEval> ev;
+    ev.push_back(ptk::JacEval{0, all, ptk::JAC_NORMAL, 0, 0.0});   // 0: the phases at p
+    ev.push_back(ptk::JacEval{0, 0, ptk::JAC_BKG, 0, 0.0});        // 1: the background at p
+    std::vector<std::vector<std::pair<int, double>>> coef(npar);
+    for (int k = 0; k < npar; ++k) {
+        if (pl->term_off_h[k + 1] == pl->term_off_h[k]) continue;
+        const double v = p[k];
+        const ColSpec c = column_spec(pl, k, v);
+        auto add_eval = [&](int row, int mask, int mode, int slot, double h) {
+            ev.push_back(ptk::JacEval{row, mask, mode, slot, h});
+            return (int)ev.size() - 1;
+        };
+        auto add_row = [&](double value) {
+            const int row = (int)(pv.size() / npar);
+            pv.insert(pv.end(), p.begin(), p.end());
+            pv[(size_t)row * npar + k] = value;
+            return row;
+        };
+        switch (c.kind) {
+        case COL_FD_ALL:
+        case COL_FD_PHASE: {
+            // symmetric where the box allows (:1277-1295)
+            const bool plus = v + c.h <= c.hi, minus = v - c.h >= c.lo;
+            const double vp = plus ? v + c.h : v, vm = minus ? v - c.h : v, d = vp - vm;
+            if (!(d != 0.0)) break;
+            const int mask = c.kind == COL_FD_ALL ? all : (1 << c.phase);
+            const int ep = add_eval(add_row(vp), mask, ptk::JAC_NORMAL, 0, 0.0);
+            const int em = add_eval(add_row(vm), mask, ptk::JAC_NORMAL, 0, 0.0);
+            coef[k] = {{ep, 1.0 / d}, {em, -1.0 / d}};
+            break;
+        }
+        case COL_CHEB: coef[k] = {{add_eval(0, 0, ptk::JAC_CHEB, c.slot, 0.0), 1.0}}; break;
+        case COL_SPLINE: coef[k] = {{add_eval(0, 0, ptk::JAC_SPLINE, c.slot, c.h), 1.0}}; break;
+        case COL_SCALE: coef[k] = {{add_eval(0, 1 << c.phase, ptk::JAC_UNIT, 0, 0.0), 1.0}}; break;
+        case COL_PO: coef[k] = {{add_eval(0, 1 << c.phase, ptk::JAC_PO, 0, c.h), 1.0}}; break;
+        case COL_ATOM: coef[k] = {{add_eval(0, 1 << c.phase, ptk::JAC_GRAD, c.slot, 0.0), 1.0}}; break;
+        default: break;
+        }
+    }
+    const int ne = (int)ev.size();
+    // the variables' columns: J_j = sum over the terms (k, j, c) of c times column k
+    std::vector<double> C((size_t)nv * ne, 0.0);
+    for (int k = 0; k < npar; ++k)
+        for (int t = pl->term_off_h[k]; t < pl->term_off_h[k + 1]; ++t)
+            for (const auto &ec : coef[k]) C[(size_t)pl->term_var_h[t] * ne + ec.first] += pl->term_coeff_h[t] * ec.second;
+    int rc;
+    if (!pl->d_jatoms) {
+        if ((rc = upload(pl, &pl->d_jatoms, pl->jac_atoms_h.empty() ? std::vector<ptk::JacAtom>(1) : pl->jac_atoms_h)) ||
+            (rc = upload(pl, &pl->d_jy, pl->yobs_h)) || (rc = upload(pl, &pl->d_jw, pl->w_h)) ||
+            (rc = dalloc(pl, &pl->d_jr, (size_t)(nv + 1) * np)) || (rc = dalloc(pl, &pl->d_jext, (size_t)(nv + 1) * (nv + 2) / 2)))
+            return rc;
+    }
+    if (ne > pl->jac_cap_evals) {
+        dfree(pl, &pl->d_jp); dfree(pl, &pl->d_jev); dfree(pl, &pl->d_jlines); dfree(pl, &pl->d_jye); dfree(pl, &pl->d_jC);
+        const size_t cap = (size_t)ne;
+        if ((rc = dalloc(pl, &pl->d_jp, cap * npar)) || (rc = dalloc(pl, &pl->d_jev, cap)) ||
+            (rc = dalloc(pl, &pl->d_jlines, cap * 2 * pl->nr)) || (rc = dalloc(pl, &pl->d_jye, cap * np)) ||
+            (rc = dalloc(pl, &pl->d_jC, cap * nv)))
+            return rc;
+        pl->jac_cap_evals = ne;
+    }
+    HIPCHECK(hipMemcpyAsync(pl->d_jp, pv.data(), sizeof(double) * pv.size(), hipMemcpyHostToDevice, pl->stream), "normal equations");
+    HIPCHECK(hipMemcpyAsync(pl->d_jev, ev.data(), sizeof(ptk::JacEval) * ne, hipMemcpyHostToDevice, pl->stream), "normal equations");
+    HIPCHECK(hipMemcpyAsync(pl->d_jC, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice, pl->stream), "normal equations");
+    ptk::JacPlan J{};
+    J.atoms = pl->d_jatoms; J.yobs = pl->d_jy; J.w = pl->d_jw; J.pvecs = pl->d_jp; J.evals = pl->d_jev;
+    J.n_evals = ne; J.n_params = npar; J.lines = pl->d_jlines; J.ye = pl->d_jye; J.C = pl->d_jC; J.nv = nv;
+    J.jr = pl->d_jr; J.ext = pl->d_jext;
+    HIPCHECK(ptk::launch_jac_lines(pl->P, J, pl->stream), "normal equations (lines)");
+    HIPCHECK(ptk::launch_jac_patterns(pl->P, J, pl->stream), "normal equations (patterns)");
+    HIPCHECK(ptk::launch_jac_normal(pl->P, J, pl->stream), "normal equations (products)");
+    ext.assign((size_t)(nv + 1) * (nv + 2) / 2, 0.0);
+    HIPCHECK(hipMemcpyAsync(ext.data(), pl->d_jext, sizeof(double) * ext.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
+    if (jr) {
+        jr->assign((size_t)(nv + 1) * np, 0.0);
+        HIPCHECK(hipMemcpyAsync(jr->data(), pl->d_jr, sizeof(double) * jr->size(), hipMemcpyDeviceToHost, pl->stream), "readback");
+    }
+    HIPCHECK(hipStreamSynchronize(pl->stream), "normal equations");
+    return 0;
+}
+
+}  // namespace
+
+int pt_normal_equations(pt_plan *pl, const double *x, double *A, double *g, double *chi2, double *J) {
+    if (!pl || !x || !A || !g || !chi2) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    std::vector<pt_plan *> ds{pl};
+    for (pt_plan *f : pl->follow) ds.push_back(f);
+    for (pt_plan *d : ds)
+        if (d->P.has_free || d->P.any_sep)
+            return fail(PT_ERR_STATE, "MI355X LM normal equations take Rietveld phases without separated profiles; using the CPU path.");
+    (void)hipSetDevice(pl->device);
+    const int nv = pl->nv;
+    std::fill(A, A + (size_t)nv * nv, 0.0);
+    std::fill(g, g + nv, 0.0);
+    *chi2 = 0.0;
+    size_t ntot = 0;
+    for (pt_plan *d : ds) ntot += d->np;
+    size_t off = 0;
+    float ms_sum = 0;
+    for (pt_plan *d : ds) {
+        std::vector<double> ext, jr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHECK(hipEventCreate(&e0), "timing");
+        HIPCHECK(hipEventCreate(&e1), "timing");
+        (void)hipEventRecord(e0, d->stream);
+        int rc = normal_dataset(d, x, ext, J ? &jr : nullptr);
+        (void)hipEventRecord(e1, d->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms_sum += ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        // unpack: row a of the packed upper triangle starts at a (nv + 1) - a (a - 1) / 2
+        const int n = nv + 1;
+        auto at = [&](int a, int b) { return ext[(size_t)a * n - (size_t)a * (a - 1) / 2 + (b - a)]; };
+        for (int a = 0; a < nv; ++a) {
+            for (int b = a; b < nv; ++b) {
+                const double v = at(a, b);
+                A[(size_t)a * nv + b] += v;
+                if (b != a) A[(size_t)b * nv + a] += v;
+            }
+            g[a] += at(a, nv);
+        }
+        *chi2 += at(nv, nv);
+        if (J)
+            for (int j = 0; j < nv; ++j)
+                std::memcpy(J + (size_t)j * ntot + off, jr.data() + (size_t)j * d->np, sizeof(double) * d->np);
+        off += d->np;
+    }
+    pl->last_normal_ms = ms_sum;
+    pl->last_total_ms = ms_sum;
     return 0;
 }
 

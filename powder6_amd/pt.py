@@ -176,6 +176,7 @@ def lib():
         L.pt_last_timing.argtypes = [ctypes.c_void_p, P_D, P_D, P_I]
         L.pt_phase_base.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_workload.argtypes = [ctypes.c_void_p, P_D, P_D]
+        L.pt_normal_equations.argtypes = [ctypes.c_void_p, P_D, P_D, P_D, P_D, P_D]
         L.pt_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_set_solve_limit.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
@@ -324,6 +325,18 @@ class Plan:
         out = np.zeros(info.point_count)
         _check(lib().pt_pattern(sub, _dptr(x[0]), _dptr(out)))
         return out
+
+    def normal(self, x, columns=False):
+        """normal() of one LM step at x (js/rietveld_core.js:1953-1983): (A, g, chi2), and
+        with columns=True also J[j, i] = d ycalc_i / d x_j over the points of every dataset."""
+        x = self._vectors(x)
+        probs = self.problem if isinstance(self.problem, (list, tuple)) else [self.problem]
+        n = sum(len(q.tth) for q in probs)
+        A, g, f = np.zeros((self.nv, self.nv)), np.zeros(self.nv), c_double()
+        J = np.zeros((self.nv, n)) if columns else None
+        _check(lib().pt_normal_equations(self.h, _dptr(x[0]), _dptr(A), _dptr(g), ctypes.byref(f),
+                                         _dptr(J) if columns else None))
+        return (A, g, f.value, J) if columns else (A, g, f.value)
 
     def workload(self, x):
         """Per-trial work at x: lines, W_tot, N_hkl, S, NP, N_T."""

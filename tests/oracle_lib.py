@@ -68,6 +68,7 @@ def lib():
         L.or_violation.argtypes = [ctypes.c_void_p, P_D]
         L.or_reflection_table.argtypes = [ctypes.c_void_p, P_D, ctypes.c_int, P_I, P_D, P_D, P_D, ctypes.c_int]
         L.or_line_table.argtypes = [ctypes.c_void_p, P_D, ctypes.c_int, P_D, P_D, ctypes.c_int]
+        L.or_normal.argtypes = [ctypes.c_void_p, P_D, P_D, P_D, P_D, P_D]
         L.or_temper.restype = ctypes.c_double
         L.or_temper.argtypes = [ctypes.c_void_p, P_D, P_D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_uint32, P_D, P_D]
@@ -177,6 +178,17 @@ class Model:
                                       h.ctypes.data_as(P_D), c.ctypes.data_as(P_D), cap)
         n = min(n, cap)
         return hk[:n], f2[:n], h[:n], c[:n]
+
+    def normal(self, x):
+        """normal() of one LM step at x: (A, g, chi2, J) with J[j, i] = d ycalc_i / d x_j."""
+        a, p = d(x)
+        nv, n = self.pb.nv, len(self.pb.tth)
+        A, g, f, J = np.zeros((nv, nv)), np.zeros(nv), np.zeros(1), np.zeros((nv, n))
+        rc = lib().or_normal(self.h, p, A.ctypes.data_as(P_D), g.ctypes.data_as(P_D), f.ctypes.data_as(P_D),
+                             J.ctypes.data_as(P_D))
+        if rc != 0:
+            raise ValueError('oracle normal equations: status %d' % rc)
+        return A, g, float(f[0]), J
 
     def lines(self, x, phase=0, cap=100000):
         """Per reflection: lines[k, pass] = (present, centre, weight, area) of the Kalpha1

@@ -350,7 +350,7 @@ static napi_value reflections(napi_env env, napi_callback_info cbi) {
 }
 
 /* ---- asynchronous device calls ----------------------------------------------------- */
-enum { OP_COSTS = 1, OP_BATCH = 2, OP_EVAL = 3, OP_INITIAL = 4, OP_PATTERN = 5 };
+enum { OP_COSTS = 1, OP_BATCH = 2, OP_EVAL = 3, OP_INITIAL = 4, OP_PATTERN = 5, OP_NORMAL = 6 };
 typedef struct {
     napi_async_work work;
     napi_deferred deferred;
@@ -374,6 +374,11 @@ static void execute(napi_env env, void *data) {
     case OP_EVAL: j->rc = pt_eval(p, j->form, j->in, j->nvec, j->out); break;
     case OP_INITIAL: j->rc = pt_set_states(p, j->in, j->nvec); break;
     case OP_PATTERN: j->rc = pt_pattern(p, j->in, j->out); break;
+    case OP_NORMAL: {
+        const int nv = j->h->info.nv;
+        j->rc = pt_normal_equations(p, j->in, j->out, j->out + (size_t)nv * nv, j->out + (size_t)nv * nv + nv, NULL);
+        break;
+    }
     }
     if (j->rc) {
         strncpy(j->err, pt_last_error(), sizeof j->err - 1);
@@ -520,6 +525,22 @@ static napi_value pattern(napi_env env, napi_callback_info cbi) {
     return start(env, argv[0], j);
 }
 
+/* normal(handle, vector) -> Promise<Float64Array(nv*nv + nv + 1)>: A (row major), g, chi^2
+ * of one LM step (pt_normal_equations) */
+static napi_value normal(napi_env env, napi_callback_info cbi) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(env, napi_get_cb_info(env, cbi, &argc, argv, NULL, NULL));
+    job *j = (job *)calloc(1, sizeof(job));
+    j->h = get_handle(env, argv[0]);
+    j->op = OP_NORMAL;
+    if (j->h) j->in = copy_vectors(env, argv[1], j->h->info.nv, &j->nvec);
+    if (!j->h || !j->in || j->nvec != 1) { free(j->in); free(j); napi_throw_error(env, NULL, "normal(handle, vector)"); return NULL; }
+    j->n_out = (size_t)j->h->info.nv * j->h->info.nv + j->h->info.nv + 1;
+    j->out = (double *)calloc(j->n_out, sizeof(double));
+    return start(env, argv[0], j);
+}
+
 static napi_value version(napi_env env, napi_callback_info cbi) {
     (void)cbi;
     napi_value v;
@@ -537,6 +558,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"batch", NULL, batch, NULL, NULL, NULL, napi_enumerable, NULL},
         {"evalTrials", NULL, eval_trials, NULL, NULL, NULL, napi_enumerable, NULL},
         {"pattern", NULL, pattern, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"normal", NULL, normal, NULL, NULL, NULL, napi_enumerable, NULL},
         {"abiVersion", NULL, version, NULL, NULL, NULL, napi_enumerable, NULL},
     };
     napi_define_properties(env, exports, sizeof d / sizeof d[0], d);

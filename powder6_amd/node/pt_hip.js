@@ -202,6 +202,14 @@ function create(ctx) {
             batch: function (count, bootstrap) { return A.batch(h, count, !!bootstrap); },
             evalTrials: function (form, vectors) { return A.evalTrials(h, form, flatten(vectors, plan.nv)); },
             pattern: function (x) { return A.pattern(h, Float64Array.from(x)); },
+            // normal equations of one LM step at x (js/rietveld_core.js:1953-1983 normal):
+            // {A: nv*nv row major, g, f}, FP64 derivative columns on the GPU
+            normal: function (x) {
+                return A.normal(h, Float64Array.from(x)).then(function (o) {
+                    var nv = plan.nv;
+                    return { A: o.subarray(0, nv * nv), g: o.subarray(nv * nv, nv * nv + nv), f: o[nv * nv + nv] };
+                });
+            },
             reflections: function (phase) { return A.reflections(h, phase || 0); },
             close: function () { if (!closed) { closed = true; A.planDestroy(h); } }
         });

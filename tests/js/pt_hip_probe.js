@@ -66,6 +66,11 @@ if (mode === 'compile') {
         return g.costs(ctx.probes).then(function (c) { g.close(); out({ costs: c, plan: g.plan }); },
                                         function (e) { g.close(); throw e; });
     }).catch(function (e) { out({ error: e.message }); process.exitCode = 3; });
+} else if (mode === 'normal') {
+    RVPTGPU.create(ctx).then(function (g) {
+        return g.normal(ctx.x).then(function (r) { g.close(); out({ A: Array.from(r.A), g: Array.from(r.g), f: r.f }); },
+                                    function (e) { g.close(); throw e; });
+    }).catch(function (e) { out({ error: e.message }); process.exitCode = 3; });
 } else if (mode === 'run') {
     var iters = 0;
     ctx.onIter = function () { iters++; };

@@ -259,3 +259,21 @@ def test_node_costs_page_joint_and_scoped_cases(case, tmp_path):
     for c, g in enumerate(out['costs']):
         cpu = sum(m.cost(probes[c % len(probes)]) for m in ms)
         assert abs(g - cpu) / max(1.0, cpu) <= 1e-5
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_normal_equations_match_oracle(tmp_path):
+    """session.normal(x) through the Node host: A, g and chi^2 of one LM step against the
+    oracle's normal() (the columns the reference's Model.derivatives builds)."""
+    pb = PR.pbso4_pnma(simulate=O.oracle_simulate, chains=16)
+    m = O.Model(pb)
+    A0, g0, f0, _ = m.normal(pb.start_vector())
+    path = js_ctx.write_ctx(pb, str(tmp_path / 'ctx.json'))
+    out, r = _node(path, 'normal')
+    assert 'A' in out, (out, r.stderr)
+    A, g = np.array(out['A']).reshape(pb.nv, pb.nv), np.array(out['g'])
+    d = np.sqrt(np.diag(A0))
+    assert np.all(np.abs(A - A0) <= 1e-6 * np.outer(d, d))
+    assert np.all(np.abs(g - g0) <= 1e-6 * d * np.sqrt(f0))
+    assert abs(out['f'] - f0) <= 1e-12 * f0

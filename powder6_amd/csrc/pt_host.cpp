@@ -1705,10 +1705,12 @@ static int ensure_matrix_trials(pt_plan *pl, int tpad) {
 }
 
 static const int kMatrixChunk = 4096;
+static const int kMatrixYout = 64;      // patterns one eval_matrix call may return (d_myout rows)
 
 // chi^2 of ntrials structure-only vectors through Y = P I; yout (optional, ntrials <= 64)
 // receives Y (without background) per trial and point
 static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout, int bf16) {
+    if (yout && ntrials > kMatrixYout) return fail(PT_ERR_ARG, "PT shared-profile pattern: at most 64 vectors.");
     int rc = matrix_prepare(pl, bf16);
     if (rc) return rc;
     for (int t = 0; t < ntrials; ++t)
@@ -1718,9 +1720,9 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
                                         "may differ from the start vector.");
     if ((rc = ensure_trials(pl, ntrials))) return rc;
     int chunk = std::min(ntrials, kMatrixChunk);
-    int tpad_max = (chunk + 63) / 64 * 64;
+    int tpad_max = (chunk + 63) / 64 * 64;                 // whole 64-trial product tiles
     if ((rc = ensure_slots(pl, chunk)) || (rc = ensure_matrix_trials(pl, tpad_max))) return rc;
-    if (yout && !pl->d_myout && (rc = dalloc(pl, &pl->d_myout, (size_t)64 * pl->np))) return rc;
+    if (yout && !pl->d_myout && (rc = dalloc(pl, &pl->d_myout, (size_t)kMatrixYout * pl->np))) return rc;
     int passes = (ntrials + chunk - 1) / chunk;
     if ((rc = ensure_events(pl, 2 * passes))) return rc;
     HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * ntrials * pl->nv, hipMemcpyHostToDevice, pl->stream), "trial upload");
@@ -1736,7 +1738,8 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 1], pl->stream), "timing");
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 2], pl->stream), "timing");
         HIPCHECK(ptk::launch_matrix_chi2(pl->P, pl->d_phi, pl->d_plo, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->d_band, pl->d_yw,
-                                         pl->d_mpart, pl->kpad, tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, bf16, pl->stream),
+                                         pl->d_mpart, pl->kpad, tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, n, bf16,
+                                         pl->stream),
                  "matrix product");
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 3], pl->stream), "timing");
         HIPCHECK(ptk::launch_matrix_reduce(pl->d_mpart, pl->row_tiles, tpad, n, pl->d_img, pl->d_chi2 + t0, pl->stream), "reduce");

@@ -1886,13 +1886,14 @@ __global__ void k_plain_reduce(const double *partial, int n, double *out) {
 // P[i, k] the unit-height column of reflection k (Kalpha1 + weighted Kalpha2) and
 // I[k, trial] = S m Lp PO |F|^2 / area.  P and I are held as FP16 hi/lo pairs (P scaled
 // by 2^12, each trial's I by a power of two), the products hi*hi + hi*lo + lo*hi are
-// accumulated in FP32 by v_mfma_f32_16x16x16f16, and the weighted chi^2 is reduced in
-// the epilogue -- Y is never stored.  Only the band of reflections whose windows meet a
-// 64-point row tile is multiplied.  The same kernels also run the pairs as BF16
-// (v_mfma_f32_16x16x16_bf16), the precision comparison of PT_FORM_MATRIX_BF16.
+// accumulated in FP32 by gfx950's v_mfma_f32_16x16x32_f16 (K = 32: twice the work per
+// instruction of the 16x16x16 shape), and the weighted chi^2 is reduced in the epilogue
+// -- Y is never stored.  Only the band of reflections whose windows meet a 128-point row
+// tile is multiplied.  The same kernels also run the pairs as BF16
+// (v_mfma_f32_16x16x32_bf16), the precision comparison of PT_FORM_MATRIX_BF16.
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 constexpr float kPScale = 4096.0f;   // 2^12
 
@@ -2163,30 +2164,31 @@ template <bool BF16>
 struct Quad;
 template <>
 struct Quad<false> {
-    typedef half4_t T;
-    static __device__ __forceinline__ T from(uint2 u) { return __builtin_bit_cast(half4_t, u); }
+    typedef half8_t T;
+    static __device__ __forceinline__ T from(uint4 u) { return __builtin_bit_cast(half8_t, u); }
     static __device__ __forceinline__ floatx4_t mfma(T a, T b, floatx4_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
 };
 template <>
 struct Quad<true> {
-    typedef bf16x4_t T;
-    static __device__ __forceinline__ T from(uint2 u) { return __builtin_bit_cast(bf16x4_t, u); }
+    typedef bf16x8_t T;
+    static __device__ __forceinline__ T from(uint4 u) { return __builtin_bit_cast(bf16x8_t, u); }
     static __device__ __forceinline__ floatx4_t mfma(T a, T b, floatx4_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 
-constexpr int kMxK = 32;                 // reflections per staged slice
-constexpr int kMxLd = kMxK + 4;          // LDS row stride (16-bit elements), padded
+constexpr int kMxK = 32;                 // reflections per staged slice: one MFMA K step
+constexpr int kMxLd = kMxK + 8;          // LDS row stride (16-bit elements), padded; rows 16-byte aligned
 
 // chi^2 partials of a 128-point x 64-trial tile.  The trial operand (I hi/lo, 64 trials x
 // 32 reflections per slice) is staged through LDS, double buffered, and shared by the four
 // wavefronts; wavefront w multiplies its 32 rows [i0 + 32 w, i0 + 32 w + 32) (two 16-row
 // blocks, P read from global) by the four 16-trial column blocks over the tile's band.
-// MFMA operand layout (16x16x16): lane l supplies A[l % 16][4 (l / 16) + e] and
-// B[4 (l / 16) + e][l % 16] and holds D[4 (l / 16) + e][l % 16].
+// MFMA operand layout (16x16x32): lane l supplies A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16], e < 8 (16 bytes of consecutive reflections each), and holds
+// D[4 (l / 16) + e][l % 16], e < 4.
 template <bool BF16>
 __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned short *__restrict__ phi,
                                                      const unsigned short *__restrict__ plo,
@@ -2194,7 +2196,7 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
                                                      const unsigned short *__restrict__ ilo,
                                                      const float *__restrict__ iscale, const int2 *__restrict__ band,
                                                      const float2 *__restrict__ yw, float *partial, int kpad, int tpad,
-                                                     double *yout) {
+                                                     double *yout, int ny) {
     typedef Quad<BF16> Q;
     __shared__ __align__(16) unsigned short sB[2][2][64][kMxLd];   // [buffer][hi, lo][trial][k]
     __shared__ float red[4][64];
@@ -2202,7 +2204,7 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i0 = rt * 128 + wave * 32;
     const int2 b = band[rt];
-    const int li = lane & 15, kq = 4 * (lane >> 4);
+    const int li = lane & 15, kq = 8 * (lane >> 4), dq = 4 * (lane >> 4);
     floatx4_t acc[2][4];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
@@ -2220,29 +2222,26 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     int buf = 0;
     for (int k0 = b.x; k0 < b.y; k0 += kMxK) {
         // the slice k0 into LDS, then the next slice's loads in flight while computing
-        *reinterpret_cast<uint2 *>(&sB[buf][0][sc_col][sc_k]) = make_uint2(nh.x, nh.y);
-        *reinterpret_cast<uint2 *>(&sB[buf][0][sc_col][sc_k + 4]) = make_uint2(nh.z, nh.w);
-        *reinterpret_cast<uint2 *>(&sB[buf][1][sc_col][sc_k]) = make_uint2(nl.x, nl.y);
-        *reinterpret_cast<uint2 *>(&sB[buf][1][sc_col][sc_k + 4]) = make_uint2(nl.z, nl.w);
+        *reinterpret_cast<uint4 *>(&sB[buf][0][sc_col][sc_k]) = nh;
+        *reinterpret_cast<uint4 *>(&sB[buf][1][sc_col][sc_k]) = nl;
         __syncthreads();
         if (k0 + kMxK < b.y) {
             nh = *reinterpret_cast<const uint4 *>(ihi + src + k0 + kMxK);
             nl = *reinterpret_cast<const uint4 *>(ilo + src + k0 + kMxK);
         }
-#pragma unroll
-        for (int kk = 0; kk < kMxK; kk += 16) {
-            if (k0 + kk >= b.y) break;
+        {
+            // the band ends on a slice boundary (bands are whole 32-reflection slices)
             typename Q::T ah[2], al[2];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
-                const size_t a = (size_t)(i0 + 16 * rb + li) * kpad + k0 + kk + kq;
-                ah[rb] = Q::from(*reinterpret_cast<const uint2 *>(phi + a));
-                al[rb] = Q::from(*reinterpret_cast<const uint2 *>(plo + a));
+                const size_t a = (size_t)(i0 + 16 * rb + li) * kpad + k0 + kq;
+                ah[rb] = Q::from(*reinterpret_cast<const uint4 *>(phi + a));
+                al[rb] = Q::from(*reinterpret_cast<const uint4 *>(plo + a));
             }
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
-                const typename Q::T bh = Q::from(*reinterpret_cast<const uint2 *>(&sB[buf][0][16 * nb + li][kk + kq]));
-                const typename Q::T bl = Q::from(*reinterpret_cast<const uint2 *>(&sB[buf][1][16 * nb + li][kk + kq]));
+                const typename Q::T bh = Q::from(*reinterpret_cast<const uint4 *>(&sB[buf][0][16 * nb + li][kq]));
+                const typename Q::T bl = Q::from(*reinterpret_cast<const uint4 *>(&sB[buf][1][16 * nb + li][kq]));
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb) {
                     acc[rb][nb] = Q::mfma(ah[rb], bh, acc[rb][nb]);
@@ -2263,11 +2262,11 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int row = i0 + 16 * rb + kq + e;
+                const int row = i0 + 16 * rb + dq + e;
                 if (row < P.np) {
                     const float Y = acc[rb][nb][e] * sc;
                     const float2 o = yw[row];
-                    if (yout && col < tpad) yout[(size_t)col * P.np + row] = (double)Y;
+                    if (col < ny) yout[(size_t)col * P.np + row] = (double)Y;   // the first ny trials' Y
                     if (o.y > 0.0f) {
                         const float r = o.x - Y;
                         sum = fmaf(o.y * r, r, sum);
@@ -2435,14 +2434,14 @@ hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const floa
 }
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
-                              float *partial, int kpad, int tpad, int row_tiles, double *yout, int bf16, hipStream_t s) {
+                              float *partial, int kpad, int tpad, int row_tiles, double *yout, int ny, int bf16, hipStream_t s) {
     // row_tiles: 128-point tiles
     if (bf16)
         hipLaunchKernelGGL(k_matrix_chi2<true>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
-                           partial, kpad, tpad, yout);
+                           partial, kpad, tpad, yout, yout ? ny : 0);
     else
         hipLaunchKernelGGL(k_matrix_chi2<false>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
-                           partial, kpad, tpad, yout);
+                           partial, kpad, tpad, yout, yout ? ny : 0);
     return hipGetLastError();
 }
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,

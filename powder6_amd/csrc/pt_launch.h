@@ -41,9 +41,10 @@ hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *
                               int kpad, int bf16, hipStream_t s);
 hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
                                    unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16, hipStream_t s);
+// yout (when not null) receives Y of the first ny trials (ny x np doubles)
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
-                              float *partial, int kpad, int tpad, int row_tiles, double *yout, int bf16, hipStream_t s);
+                              float *partial, int kpad, int tpad, int row_tiles, double *yout, int ny, int bf16, hipStream_t s);
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,
                                 hipStream_t s);
 hipError_t launch_pattern(const DevPlan &P, const double *X, double *y, int block, int ch, size_t dyn, hipStream_t s);

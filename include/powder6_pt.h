@@ -343,6 +343,8 @@ typedef struct {
     float skip_margin;   /* log Rwp allowance taken off a prediction before a skip      */
     float skip_eps;      /* skip the exact evaluation of a screened chain's pick when its
                             predicted Metropolis acceptance is below skip_eps (0: never) */
+    int train_every;     /* train (train_iters Adam steps) every train_every PT steps;
+                            0 or 1: every step                                          */
 } pt_screen_config;
 int pt_screen(pt_plan *plan, const pt_screen_config *cfg);
 /* out[0] Adam steps taken, out[1] ring entries filled, out[2] running mean of the

@@ -1301,7 +1301,8 @@ static int enqueue_screened_sweep(pt_plan *pl, hipEvent_t *ev) {
     const pt_screen_config &c = pl->scfg;
     const double *step_ptr = pl->P.state + 3 * pl->nv + PT_S_STEP;
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
-        for (int it = 0; it < c.train_iters; ++it) {
+        const bool train = c.train_every <= 1 || s % c.train_every == 0;
+        for (int it = 0; train && it < c.train_iters; ++it) {
             int rc = mlpx::train_iter(pl->mlp, pl->S.ring_x, pl->S.ring_t, pl->screen_cap, step_ptr, pl->chains, 0, c.batch,
                                       c.seed, c.batch, pl->stream);
             if (rc) return rc;

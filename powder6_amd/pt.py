@@ -409,11 +409,11 @@ class PtScreenConfig(ctypes.Structure):
     _fields_ = [('oversample', ctypes.c_int), ('train_iters', ctypes.c_int), ('batch', ctypes.c_int),
                 ('warmup_iters', ctypes.c_int), ('capacity', ctypes.c_int), ('lr', ctypes.c_float),
                 ('seed', ctypes.c_uint32), ('margin', ctypes.c_float), ('cold_fraction', ctypes.c_float),
-                ('skip_margin', ctypes.c_float), ('skip_eps', ctypes.c_float)]
+                ('skip_margin', ctypes.c_float), ('skip_eps', ctypes.c_float), ('train_every', ctypes.c_int)]
 
 
 SCREEN_DEFAULTS = dict(oversample=4, train_iters=1, batch=512, warmup_iters=200, capacity=65536, lr=1e-3, seed=7,
-                       margin=0.0, cold_fraction=1.0, skip_margin=0.0, skip_eps=0.0)
+                       margin=0.0, cold_fraction=1.0, skip_margin=0.0, skip_eps=0.0, train_every=1)
 
 
 def _fptr(a):

@@ -6,14 +6,18 @@ Workload (the batched FP32 VALU configuration): Pnma PbSO4-type phase a=8.48 b=5
 A "step" is one PT sweep: 256 x 64 = 16384 trial patterns, each a full calculated pattern
 and its chi^2, proposed, evaluated and accepted on the GPU, then one replica exchange.
 
-Prints one JSON line (value = trial patterns/s of the chosen form, the staged PT
-step pipeline replayed as one HIP graph per sweep), with the plain single-trial form, the roofline of the dominant kernel
-(k_stage_accum, FP32 VALU) and the FP64 CPU PT baseline (the oracle, timed on this host) in
-the same line.
+Prints one JSON line.  `value` is the exactly evaluated trial patterns per second of the
+chosen form (the staged PT step pipeline replayed as one HIP graph per sweep; proposals
+rejected without an evaluation are not counted).  The same line carries, for every
+workload of the benchmark table (the NaCl CPU-path case, the Pnma pipeline, the shared-
+profile MFMA form, the uneven batch, the learned screen, free intensities, the LM normal
+equations): the GPU rate, the plain single-trial rate, the FP64 CPU rate where the
+restatement runs it, the chi^2 error against the FP64 oracle, and the roofline of the
+dominant kernel, k_stage_accum (FP32 VALU), with its HBM traffic.
 """
 import argparse
+import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -25,6 +29,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'PT trial patterns/s on 1 MI355X, plain vs chosen form, χ² rel err ≤1e-5'
 PEAK_FP32 = 157.3e12      # MI355X FP32 vector peak, FLOP/s
+PEAK_FP16_MFMA = 2.5e15   # MI355X dense FP16/BF16 MFMA, FLOP/s
+PEAK_FP64 = 78.6e12       # MI355X FP64 vector peak, FLOP/s
+PEAK_BF16_MFMA = 2.5e15
+MLP_FLOP_FWD = 2 * (64 * 256 + 2 * 256 * 256 + 256)   # one row through 64-256-256-256-1
 
 
 def flop_per_trial(w):
@@ -32,42 +40,106 @@ def flop_per_trial(w):
     return 8 * w['n_hkl'] * w['s_terms'] + 10 * w['w_tot'] + 4 * w['np'] + 2 * w['n_bkg'] * w['np']
 
 
-def cpu_baseline(pb, x_states, gpu_costs):
-    """The FP64 CPU PT path (oracle restatement of temper() + cost()) on this host, on a
-    bounded sample; also checks chi^2 of the GPU chain states against the FP64 oracle."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, 'tests'))
     import oracle_lib as O
+    return O
+
+
+def chi2_error(pb, xs, costs):
+    """Largest relative error of GPU chi^2 against the FP64 oracle over (x, cost) pairs."""
+    O = _oracle()
     m = O.Model(pb)
-    errs = []
-    for x, g in zip(x_states, gpu_costs):
-        c = m.cost(x)
-        errs.append(abs(g - c) / max(1.0, c))
-    # single thread: one temper() run of 4 replicas x 20 sweeps (about 10 s)
-    x0 = pb.start_vector()
-    t = time.perf_counter()
-    _, _, st = m.temper(x0, pb.scale, 20, 4, pb.max_temp, 99)
-    dt = time.perf_counter() - t
-    single = st[1] / dt
-    # all host threads: one independent temper() per thread (the reference runs one PT fit
-    # per worker, js/rietveld/rv_commands.js:11-100)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    err = 0.0
+    for x, g in zip(xs, costs):
+        c = m.cost(np.asarray(x))
+        if np.isfinite(c) and c < 3e38:
+            err = max(err, abs(float(g) - c) / max(1.0, c))
+    m.close()
+    return err
+
+
+def host_info():
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    hwc = os.cpu_count() or 1
+    return {'cpu_model': model, 'hardware_concurrency': hwc, 'threads': max(1, min(16, hwc))}
+
+
+def cpu_rates(pb, single_sweeps, single_chains, par_sweeps, par_chains):
+    """FP64 CPU PT (the oracle's temper() + cost()) on this host: one thread, and one
+    independent temper() per thread on every thread of the box (capped at 16)."""
+    O = _oracle()
     from powder6_amd.pt import CProblem
+    m = O.Model(pb)
+    t = time.perf_counter()
+    _, _, st = m.temper(pb.start_vector(), pb.scale, single_sweeps, single_chains, pb.max_temp, 99)
+    single = st[1] / (time.perf_counter() - t)
+    m.close()
+    hi = host_info()
     cp = CProblem(pb)
     sc = np.ascontiguousarray(pb.scale, dtype=np.float64)
     secs = O.ctypes.c_double()
-    rate = O.lib().or_temper_parallel(O.ctypes.addressof(cp.struct), sc.ctypes.data_as(O.P_D), 40, 2, pb.max_temp, 7,
-                                      threads, O.ctypes.byref(secs))
-    m.close()
-    return {
-        'value': rate, 'unit': 'trial patterns/s', 'cores': threads, 'kind': 'port',
-        'sample': ('oracle FP64 temper() on the same problem: %d threads x (2 replicas x 40 sweeps x %d trials), '
-                   '%.1f s wall; single thread %.1f trials/s (4 replicas x 20 sweeps)' % (threads, pb.nv, secs.value, single)),
-        'single_thread': single,
-        'chi2_rel_err_vs_fp64': max(errs) if errs else None,
-    }
+    rate = O.lib().or_temper_parallel(O.ctypes.addressof(cp.struct), sc.ctypes.data_as(O.P_D), par_sweeps, par_chains,
+                                      pb.max_temp, 7, hi['threads'], O.ctypes.byref(secs))
+    return single, rate, secs.value, hi
 
 
-PEAK_FP16_MFMA = 2.5e15   # MI355X dense FP16/BF16 MFMA, FLOP/s
+def plain_rate(pt, plan, pb, n, seed=1):
+    rng = np.random.default_rng(seed)
+    x0 = pb.start_vector()
+    X = x0 + rng.uniform(-1, 1, size=(n, pb.nv)) * np.array(pb.scale)
+    X = np.minimum(np.array(pb.hi), np.maximum(np.array(pb.lo), X))
+    plan.eval(X[:4], pt.FORM_PLAIN)
+    t = time.perf_counter()
+    chi = plan.eval(X, pt.FORM_PLAIN)
+    return len(X) / (time.perf_counter() - t), X, chi
+
+
+def pt_states(plan, pb, every):
+    S = plan.batch(0).reshape(plan.chains, -1)
+    pick = list(range(0, plan.chains, max(1, plan.chains // every)))
+    return [S[c, :pb.nv].copy() for c in pick], [S[c, 3 * pb.nv] for c in pick]
+
+
+def nacl_cpu_path(pt, PR, no_cpu):
+    """NaCl Fm-3m, 5501 points, 16 replicas x 1000 sweeps x 12 parameters
+    (192,000 trials), the reference's CPU-path case: the whole PT run on the GPU, the
+    plain form, and the FP64 CPU restatement on a bounded sample."""
+    pb = PR.nacl_fm3m(simulate=pt.gpu_simulate)
+    out = {'workload': 'NaCl Fm-3m, 5,501 pts Cu Ka1, TCH, 6 Chebyshev terms, 16 replicas x 1000 sweeps x 12 params'}
+    with pt.Plan(pb) as pl:
+        pl.initial([pb.start_vector()])
+        pl.batch(1, bootstrap=True)
+        pl.sweeps(5)                                       # graph capture, tile tuning
+        e0, s0 = pl.trial_counts()
+        t = time.perf_counter()
+        pl.sweeps(1000)
+        dt = time.perf_counter() - t
+        e1, s1 = pl.trial_counts()
+        ev, skipped = e1 - e0, s1 - s0
+        xs, gc = pt_states(pl, pb, 16)
+        plain, X, chi = plain_rate(pt, pl, pb, 256)
+    out['chosen'] = {'value': ev / dt, 'unit': 'trial patterns/s', 'seconds_for_192000_proposals': dt,
+                     'evaluated': ev, 'rejected_unevaluated': skipped}
+    out['plain'] = {'value': plain, 'unit': 'trial patterns/s'}
+    out['chi2_rel_err_vs_fp64'] = max(chi2_error(pb, xs, gc), chi2_error(pb, X[:16], chi[:16]))
+    if not no_cpu:
+        single, rate, secs, hi = cpu_rates(pb, 20, 16, 40, 4)
+        out['cpu_fp64'] = {'single_thread': single, 'all_threads': rate, 'threads': hi['threads'], 'unit': 'trial patterns/s',
+                           'sample': 'single thread: 16 replicas x 20 sweeps; %d threads x (4 replicas x 40 sweeps), %.1f s'
+                                     % (hi['threads'], secs),
+                           'seconds_for_192000_single_thread': 192000 / single}
+        out['chosen_over_cpu_single'] = out['chosen']['value'] / single
+    out['chosen_over_plain'] = out['chosen']['value'] / plain
+    return out
 
 
 def matrix_form(pt, PR, trials=32768):
@@ -87,16 +159,21 @@ def matrix_form(pt, PR, trials=32768):
         ti_ms, tp_ms = plan.matrix_timing()
         n_hkl = plan.workload(x0)['n_hkl']
         chi_bf16 = plan.eval(X[:8], pt.FORM_MATRIX_BF16)     # the BF16-pair precision comparison
+        plain, _, _ = plain_rate(pt, plan, pb, 256)
     dense = 2.0 * len(pb.tth) * n_hkl                      # FLOP per trial of the dense product
     return {'value': trials / (dev_ms / 1e3), 'unit': 'trial patterns/s', 'trials': trials, 'n_vars': pb.nv,
-            'value_with_host_transfers': trials / dt,
-            'form': 'shared profile matrix, Y = P I on MFMA (FP16 hi/lo, 3 products), fused chi^2',
+            'value_with_host_transfers': trials / dt, 'plain': plain,
+            'form': 'shared profile matrix, Y = P I on MFMA (v_mfma_f32_16x16x32_f16, hi/lo, 3 products), fused chi^2',
             'intensity_ms': ti_ms, 'product_ms': tp_ms,
             'product_dense_tflops': dense * trials / (tp_ms / 1e3) / 1e12,
-            'product_frac_of_fp16_peak': dense * trials / (tp_ms / 1e3) / PEAK_FP16_MFMA}, pb, X[:8], chi2[:8], chi_bf16
+            'product_frac_of_fp16_peak': dense * trials / (tp_ms / 1e3) / PEAK_FP16_MFMA,
+            'roofline': {'bound': 'mfma', 'kernel': 'k_matrix_chi2', 'achieved': dense * trials / (tp_ms / 1e3) / 1e12,
+                         'peak': PEAK_FP16_MFMA / 1e12, 'unit': 'TFLOP/s',
+                         'frac': dense * trials / (tp_ms / 1e3) / PEAK_FP16_MFMA,
+                         'flop_per_trial_dense': dense}}, pb, X[:8], chi2[:8], chi_bf16
 
 
-def uneven_batch(pt, PR, sweeps=2):
+def uneven_batch(pt, PR, no_cpu, sweeps=2):
     """48 independent patterns (cubic / tetragonal / monoclinic, 2000-12000 points,
     1-12 sites), 32 replicas each, every PT stage one launch over all of them, one graph
     launch per sweep of all 48 PT runs."""
@@ -115,15 +192,119 @@ def uneven_batch(pt, PR, sweeps=2):
         refl = [m['reflection_count'] for m in g.members]
         pts = [m['point_count'] for m in g.members]
         trials = g.total_chains * steps * sweeps
-    return {'value': trials / dt, 'unit': 'trial patterns/s', 'patterns': len(pbs), 'replicas_each': 32,
-            'trials_per_step': 32 * len(pbs), 'steps_per_sweep': steps, 'ms_per_sweep': 1e3 * dt / sweeps,
-            'reflections_min_max': [min(refl), max(refl)], 'points_min_max': [min(pts), max(pts)],
-            'tiles_per_wave': tiles, 'tuner_accum_ms': [round(v, 4) for v in tune_ms],
-            'form': 'one launch per stage over all 48 PT runs, one graph per sweep'}, pbs, costs
+    # the work of one trial of every member (at its start vector), the plain form on a
+    # sample of members (the reference fits them one after another)
+    flop_step, plain_t, plain_n = 0.0, 0.0, 0
+    for i, pb in enumerate(pbs):
+        with pt.Plan(pb) as pl:
+            flop_step += flop_per_trial(pl.workload(pb.start_vector())) * pb.chains
+            if i % 6 == 0:
+                r, _, _ = plain_rate(pt, pl, pb, 48, seed=i)
+                plain_t += 48 / r
+                plain_n += 48
+    sweep_s = dt / sweeps
+    out = {'value': trials / dt, 'unit': 'trial patterns/s', 'patterns': len(pbs), 'replicas_each': 32,
+           'trials_per_step': 32 * len(pbs), 'steps_per_sweep': steps, 'ms_per_sweep': 1e3 * sweep_s,
+           'reflections_min_max': [min(refl), max(refl)], 'points_min_max': [min(pts), max(pts)],
+           'tiles_per_wave': tiles, 'tuner_accum_ms': [round(v, 4) for v in tune_ms],
+           'plain': plain_n / plain_t,
+           'roofline_whole_step': {'bound': 'valu', 'achieved': flop_step * steps / sweep_s / 1e12, 'peak': PEAK_FP32 / 1e12,
+                                   'unit': 'TFLOP/s', 'frac': flop_step * steps / sweep_s / PEAK_FP32,
+                                   'note': 'all stages of a sweep (setup, lines, accumulate, finish, exchange) in the denominator'},
+           'form': 'one launch per stage over all 48 PT runs, one graph per sweep'}
+    out['chi2_rel_err_vs_fp64'] = max(chi2_error(pb, [pb.start_vector()], [c]) for pb, c in zip(pbs, costs))
+    if not no_cpu:
+        # FP64 CPU PT of three members of increasing size, one thread each
+        rates = []
+        for i in (0, 17, 34):
+            pb = pbs[i]
+            O = _oracle()
+            m = O.Model(pb)
+            t = time.perf_counter()
+            _, _, st = m.temper(pb.start_vector(), pb.scale, 2, 4, pb.max_temp, 5)
+            rates.append(st[1] / (time.perf_counter() - t))
+            m.close()
+        out['cpu_fp64_single_thread'] = {'value': float(np.mean(rates)), 'unit': 'trial patterns/s',
+                                         'sample': 'members 0, 17, 34: 4 replicas x 2 sweeps each, mean rate'}
+    out['chosen_over_plain'] = out['value'] / out['plain']
+    return out
 
 
-MLP_FLOP_FWD = 2 * (64 * 256 + 2 * 256 * 256 + 256)   # one row through 64-256-256-256-1
-PEAK_BF16_MFMA = 2.5e15
+def learned_screening(pt, PR, no_cpu, sweeps=2):
+    """The P2_1/c molecular problem (17 atoms, 53 variables with one Biso), 512 replicas,
+    7001 points: plain PT against surrogate-screened PT (4 candidates per chain and step),
+    against skipping PT (proposals predicted hopeless are rejected unevaluated; the network
+    trained every step, and once per sweep), and the network's training throughput."""
+    pb = PR.molecular_p21c(simulate=pt.gpu_simulate, chains=512, shared_b=True)
+    x0 = pb.start_vector()
+    out = {'workload': 'P2_1/c, 17 C/N/S atoms, 53 variables, 7001 pts Ka1/Ka2, 512 replicas',
+           'network': '64-256-256-256-1 SiLU, BF16 MFMA, FP32 accumulation, Adam'}
+    skip = dict(oversample=1, train_iters=1, batch=512, warmup_iters=100, cold_fraction=1.0, skip_eps=1e-2, skip_margin=0.02)
+    runs = (('plain', None),
+            ('screened', dict(oversample=4, train_iters=1, batch=512, warmup_iters=100, margin=0.002, cold_fraction=0.5)),
+            ('skipping', skip),
+            ('skipping_train_per_sweep', dict(skip, train_iters=2, warmup_iters=8, train_every=pb.nv)))
+    errs = []
+    for label, screen in runs:
+        with pt.Plan(pb) as pl:
+            if screen:
+                pl.screen(**screen)
+            pl.initial([x0])
+            pl.batch(1, bootstrap=True)
+            pl.sweeps(2)                                     # warm: graph capture, network warm-up
+            e0, s0 = pl.trial_counts()
+            t = time.perf_counter()
+            pl.sweeps(sweeps)
+            dt = time.perf_counter() - t
+            e1, s1 = pl.trial_counts()
+            spw = pl.info['steps_per_sweep']
+            out[label] = {'value': (e1 - e0) / dt, 'unit': 'exact trial patterns/s', 'ms_per_sweep': 1e3 * dt / sweeps,
+                          'proposals_per_s': pb.chains * spw * sweeps / dt,
+                          'exact_evaluation_fraction': (e1 - e0) / max(1, (e1 - e0) + (s1 - s0))}
+            if screen:
+                out[label]['proposals_screened_per_s'] = screen['oversample'] * pb.chains * spw * sweeps / dt
+                out[label]['stats'] = pl.screen_stats()
+            xs, gc = pt_states(pl, pb, 8)
+            errs.append(chi2_error(pb, xs, gc))
+            if label == 'plain':
+                roof_w = pl.workload(x0)
+                pl.set_kernel_timing(True)
+                pl.sweeps(1)
+                _, kms, nl = pl.timing()
+                pl.set_kernel_timing(False)
+                f = flop_per_trial(roof_w)
+                out['roofline'] = {'bound': 'valu', 'kernel': 'k_stage_accum', 'achieved': f * pb.chains / (kms / nl / 1e3) / 1e12,
+                                   'peak': PEAK_FP32 / 1e12, 'unit': 'TFLOP/s',
+                                   'frac': f * pb.chains / (kms / nl / 1e3) / PEAK_FP32, 'avg_launch_ms': kms / nl,
+                                   'flop_per_trial': f, 'workload': roof_w}
+                plain, X, chi = plain_rate(pt, pl, pb, 128)
+                out['plain_single_trial'] = {'value': plain, 'unit': 'trial patterns/s'}
+                errs.append(chi2_error(pb, X[:8], chi[:8]))
+    out['chi2_rel_err_vs_fp64'] = max(errs)
+    out['screen_overhead'] = out['screened']['ms_per_sweep'] / out['plain']['ms_per_sweep'] - 1.0
+    out['skipping_overhead'] = out['skipping_train_per_sweep']['ms_per_sweep'] / out['plain']['ms_per_sweep'] - 1.0
+    if not no_cpu:
+        O = _oracle()
+        m = O.Model(pb)
+        t = time.perf_counter()
+        _, _, st = m.temper(x0, pb.scale, 1, 2, pb.max_temp, 3)
+        out['cpu_fp64_single_thread'] = {'value': st[1] / (time.perf_counter() - t), 'unit': 'trial patterns/s',
+                                         'sample': '2 replicas x 1 sweep (106 trials)'}
+        m.close()
+    # training throughput of the network alone (minibatches of 512 rows)
+    rng = np.random.default_rng(4)
+    Xn = rng.uniform(-1, 1, size=(65536, 53)).astype(np.float32)
+    T = rng.normal(size=65536).astype(np.float32)
+    with pt.Mlp(seed=2) as net:
+        net.train(Xn, T, iters=20, batch=512)
+        t = time.perf_counter()
+        net.train(Xn, T, iters=400, batch=512)
+        dt = time.perf_counter() - t
+    flop = 3 * MLP_FLOP_FWD * 512 * 400
+    out['training'] = {'rows_per_s': 512 * 400 / dt, 'tflops': flop / dt / 1e12,
+                       'frac_of_bf16_peak': flop / dt / PEAK_BF16_MFMA, 'batch': 512,
+                       'note': 'wall time of 400 Adam steps (13 launches each) incl. launch gaps'}
+    return out
 
 
 def free_intensities(pt, PR, sweeps=5):
@@ -132,6 +313,7 @@ def free_intensities(pt, PR, sweeps=5):
     Rietveld ZnS phase: trial patterns/s of each, from `sweeps` timed sweeps."""
     out = {'workload': 'NaCl Fm-3m 10-120 deg step 0.02 (5,501 pts), TCH, 256 replicas; '
                        'with_zns adds a Rietveld ZnS phase'}
+    errs = []
     for mixed in (False, True):
         for method in ('lebail', 'rietveld'):
             pb = PR.lebail_probe(mixed=mixed, simulate=pt.gpu_simulate, chains=256)
@@ -140,62 +322,45 @@ def free_intensities(pt, PR, sweeps=5):
                 pl.initial([pb.start_vector()])
                 pl.batch(1, bootstrap=True)
                 pl.sweeps(2)
+                e0, _ = pl.trial_counts()
                 t0 = time.perf_counter()
                 pl.sweeps(sweeps)
                 dt = (time.perf_counter() - t0) / sweeps
-                rate = pb.chains * pl.info['steps_per_sweep'] / dt
-            out[('with_zns_' if mixed else '') + method] = {'value': rate, 'unit': 'trial patterns/s',
+                e1, _ = pl.trial_counts()
+                xs, gc = pt_states(pl, pb, 8)
+                errs.append(chi2_error(pb, xs, gc))
+            out[('with_zns_' if mixed else '') + method] = {'value': (e1 - e0) / (dt * sweeps), 'unit': 'trial patterns/s',
                                                             'ms_per_sweep': dt * 1e3}
+    out['chi2_rel_err_vs_fp64'] = max(errs)
     return out
 
 
-def learned_screening(pt, PR, sweeps=2):
-    """The P2_1/c molecular problem (17 atoms, 53 variables with one Biso), 512 replicas,
-    7001 points: plain PT against surrogate-screened PT (4 candidates per chain and step,
-    one Adam step of 512 rows per PT step), against skipping PT (proposals predicted
-    hopeless are rejected unevaluated), and the network's training throughput."""
-    pb = PR.molecular_p21c(simulate=pt.gpu_simulate, chains=512, shared_b=True)
-    x0 = pb.start_vector()
-    out = {'workload': 'P2_1/c, 17 C/N/S atoms, 53 variables, 7001 pts Ka1/Ka2, 512 replicas',
-           'network': '64-256-256-256-1 SiLU, BF16 MFMA, FP32 accumulation, Adam'}
-    runs = (('plain', None),
-            ('screened', dict(oversample=4, train_iters=1, batch=512, warmup_iters=100, margin=0.002, cold_fraction=0.5)),
-            ('skipping', dict(oversample=1, train_iters=1, batch=512, warmup_iters=100, cold_fraction=1.0,
-                              skip_eps=1e-2, skip_margin=0.02)))
-    for label, screen in runs:
-        with pt.Plan(pb) as pl:
-            if screen:
-                pl.screen(**screen)
-            pl.initial([x0])
-            pl.batch(1, bootstrap=True)
-            pl.sweeps(2)                                     # warm: graph capture, network warm-up
-            t = time.perf_counter()
-            pl.sweeps(sweeps)
-            dt = time.perf_counter() - t
-            spw = pl.info['steps_per_sweep']
-            out[label] = {'value': pb.chains * spw * sweeps / dt, 'unit': 'exact trial patterns/s',
-                          'ms_per_sweep': 1e3 * dt / sweeps}
-            if screen:
-                out[label]['proposals_screened_per_s'] = screen['oversample'] * pb.chains * spw * sweeps / dt
-                out[label]['stats'] = st = pl.screen_stats()
-                if screen.get('skip_eps'):
-                    proposals = float(np.sum(pl.batch(0).reshape(pb.chains, -1)[:, 3 * pb.nv + 4]))
-                    out[label]['value'] *= 1.0 - st['skipped'] / proposals
-                    out[label]['exact_evaluation_fraction'] = 1.0 - st['skipped'] / proposals
-    out['screen_overhead'] = out['screened']['ms_per_sweep'] / out['plain']['ms_per_sweep'] - 1.0
-    # training throughput of the network alone (minibatches of 512 rows)
-    rng = np.random.default_rng(4)
-    X = rng.uniform(-1, 1, size=(65536, 53)).astype(np.float32)
-    T = rng.normal(size=65536).astype(np.float32)
-    with pt.Mlp(seed=2) as net:
-        net.train(X, T, iters=20, batch=512)
+def normal_equations(pt, PR, no_cpu, reps=10):
+    """The FP64 normal equations of one LM step (js/rietveld_core.js:1953-1983) of the Pnma
+    workload on the GPU, against the oracle's FP64 restatement on one host thread."""
+    O = _oracle()
+    pb = PR.pbso4_pnma(simulate=pt.gpu_simulate, chains=16)
+    x = pb.start_vector()
+    with pt.Plan(pb) as pl:
+        pl.normal(x)
         t = time.perf_counter()
-        net.train(X, T, iters=400, batch=512)
-        dt = time.perf_counter() - t
-    flop = 3 * MLP_FLOP_FWD * 512 * 400
-    out['training'] = {'rows_per_s': 512 * 400 / dt, 'tflops': flop / dt / 1e12,
-                       'frac_of_bf16_peak': flop / dt / PEAK_BF16_MFMA, 'batch': 512,
-                       'note': 'wall time of 400 Adam steps (13 launches each) incl. launch gaps'}
+        for _ in range(reps):
+            A, g, f = pl.normal(x)
+        wall = (time.perf_counter() - t) / reps
+        dev_ms = pl.timing()[0]
+    m = O.Model(pb)
+    t = time.perf_counter()
+    A0, g0, f0, _ = m.normal(x)
+    cpu = time.perf_counter() - t
+    m.close()
+    d = np.sqrt(np.diag(A0))
+    out = {'workload': 'Pnma bench plan, 32 variables, 7001 points', 'ms_per_build': wall * 1e3, 'device_ms_per_build': dev_ms,
+           'a_rel_err_vs_fp64': float(np.max(np.abs(A - A0) / np.outer(d, d))),
+           'g_rel_err_vs_fp64': float(np.max(np.abs(g - g0) / (d * np.sqrt(f0)))),
+           'chi2_rel_err_vs_fp64': abs(f - f0) / f0}
+    if not no_cpu:
+        out['cpu_fp64_single_thread_ms'] = cpu * 1e3
+        out['gpu_over_cpu'] = cpu / wall
     return out
 
 
@@ -232,27 +397,24 @@ def main():
         plan.batch(1, bootstrap=True)
         for _ in range(args.warmup):
             plan.sweeps(1)
+        e0, s0 = plan.trial_counts()
         t0 = time.perf_counter()
         plan.sweeps(args.steps)                               # graph replays; synchronises its stream
         t1 = time.perf_counter()
-        state = plan.batch(0)
+        e1, s1 = plan.trial_counts()
+        xs, gc = pt_states(plan, pb, 16)
         tiles, tune_ms = plan.tiles()
         # dominant kernel (banded profile sum) timed with HIP events on the plan's stream
         plan.set_kernel_timing(True)
         plan.sweeps(2)
         total_ms, kernel_ms, launches = plan.timing()
         plan.set_kernel_timing(False)
-        trials = chains * spw * args.steps
-        value = trials / (t1 - t0)
+        proposals = chains * spw * args.steps
+        evaluated = e1 - e0
+        value = evaluated / (t1 - t0)
         ms_per_step = 1e3 * (t1 - t0) / args.steps
         # plain single-trial form on a bounded sample of trial vectors
-        rng = np.random.default_rng(1)
-        X = x0 + rng.uniform(-1, 1, size=(args.plain_trials, pb.nv)) * np.array(pb.scale)
-        X = np.minimum(np.array(pb.hi), np.maximum(np.array(pb.lo), X))
-        plan.eval(X[:8], pt.FORM_PLAIN)
-        t = time.perf_counter()
-        plan.eval(X, pt.FORM_PLAIN)
-        plain = len(X) / (time.perf_counter() - t)
+        plain, X, chi_plain = plain_rate(pt, plan, pb, args.plain_trials)
         # batched (one launch, one workgroup per trial) on 16384 independent vectors
         XB = np.repeat(X, (16384 + len(X) - 1) // len(X), axis=0)[:16384]
         plan.eval(XB, pt.FORM_BATCHED)
@@ -268,17 +430,6 @@ def main():
         print(json.dumps({'metric': METRIC, 'value': value, 'unit': 'trial patterns/s', 'main_only': True,
                           'plain': plain, 'batched': batched, 'hbm_copy_gbps': copy_gbps}))
         return
-    mat, mpb, mX, mchi, mchi_bf16 = matrix_form(pt, PR)
-    uneven, upbs, ucosts = uneven_batch(pt, PR)
-    learned = learned_screening(pt, PR)
-    free = free_intensities(pt, PR)
-
-    S = state.reshape(chains, -1)
-    nv = pb.nv
-    pick = list(range(0, chains, max(1, chains // 16)))
-    xs = [S[c, :nv].copy() for c in pick]
-    gc = [S[c, 3 * nv + pt.S_COST] for c in pick]
-
     f_trial = flop_per_trial(work)
     launch_s = kernel_ms / max(1, launches) / 1e3
     achieved = f_trial * chains / launch_s           # one launch = one PT step of every chain
@@ -295,45 +446,65 @@ def main():
     roof = {'bound': 'valu', 'kernel': 'k_stage_accum', 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
             'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': traffic, 'traffic_source': traffic_src,
             'hbm_gbps_at_traffic': (traffic / launch_s / 1e9) if traffic else None,
-            'flop_per_trial': f_trial, 'avg_launch_ms': launch_s * 1e3, 'workload': work}
+            'flop_per_trial': f_trial, 'avg_launch_ms': launch_s * 1e3, 'workload': work,
+            'whole_step_frac': f_trial * evaluated / (t1 - t0) / PEAK_FP32}
+
+    legs = {}
+    for name, fn in (('nacl', lambda: nacl_cpu_path(pt, PR, args.no_cpu)), ('matrix', lambda: matrix_form(pt, PR)),
+                     ('uneven', lambda: uneven_batch(pt, PR, args.no_cpu)),
+                     ('learned', lambda: learned_screening(pt, PR, args.no_cpu)),
+                     ('free', lambda: free_intensities(pt, PR)), ('normal', lambda: normal_equations(pt, PR, args.no_cpu))):
+        t = time.perf_counter()
+        legs[name] = fn()
+        print('bench leg %s done in %.1f s' % (name, time.perf_counter() - t), file=sys.stderr, flush=True)
+    nacl, uneven, learned, free, normal = legs['nacl'], legs['uneven'], legs['learned'], legs['free'], legs['normal']
+    mat, mpb, mX, mchi, mchi_bf16 = legs['matrix']
+
     cpu = None
+    err_chosen = chi2_error(pb, xs, gc)
     if not args.no_cpu:
-        cpu = cpu_baseline(pb, xs, gc)
-        # the matrix form's chi^2 against the FP64 oracle on a sample of its trials
-        sys.path.insert(0, os.path.join(ROOT, 'tests'))
-        import oracle_lib as O
+        single, rate, secs, hi = cpu_rates(pb, 20, 4, 40, 2)
+        cpu = {'value': rate, 'unit': 'trial patterns/s', 'cores': hi['threads'], 'kind': 'port',
+               'sample': ('oracle FP64 temper() + cost() on the same Pnma problem: %d threads x (2 replicas x 40 sweeps x %d '
+                          'trials), %.1f s wall; single thread 4 replicas x 20 sweeps' % (hi['threads'], pb.nv, secs)),
+               'single_thread': single, 'cpu_model': hi['cpu_model'], 'hardware_concurrency': hi['hardware_concurrency'],
+               'threads_used': hi['threads'],
+               'note': ('one independent PT run per thread; the reference pool runs at most min(8, hardware_concurrency - 1) '
+                        'fits at once (js/rietveld/rv_commands.js:14), so %d threads is at least its parallelism'
+                        % hi['threads'])}
+        O = _oracle()
         mm = O.Model(mpb)
         mat['chi2_rel_err_vs_fp64'] = max(abs(g - mm.cost(x)) / max(1.0, mm.cost(x)) for x, g in zip(mX, mchi))
         mat['chi2_rel_err_vs_fp64_with_bf16_pairs'] = max(abs(g - mm.cost(x)) / max(1.0, mm.cost(x))
                                                           for x, g in zip(mX, mchi_bf16))
         mm.close()
-        err = 0.0
-        for upb, g in zip(upbs, ucosts):
-            um = O.Model(upb)
-            ref = um.cost(upb.start_vector())
-            um.close()
-            err = max(err, abs(g - ref) / max(1.0, ref))
-        uneven['chi2_rel_err_vs_fp64'] = err
     out = {
         'metric': METRIC, 'value': value, 'unit': 'trial patterns/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'fp32', 'data': 'synthetic (seeded Poisson pattern of the model at seeded true parameters)',
         'config': {'workload': 'Pnma PbSO4-type, 24 atoms, 7001 pts Ka1/Ka2, TCH, 256 replicas x 64 trials/sweep',
-                   'replicas': chains, 'trials_per_sweep': spw, 'n_vars': nv, 'points': work['np'],
+                   'replicas': chains, 'trials_per_sweep': spw, 'n_vars': pb.nv, 'points': work['np'],
                    'reflections': work['n_hkl'], 'lines': work['lines'], 'form': 'staged PT step pipeline, one graph per sweep',
                    'tiles_per_wave': tiles, 'tuner_accum_ms': [round(v, 4) for v in tune_ms]},
-        'plain': {'value': plain, 'unit': 'trial patterns/s', 'form': 'one trial per launch sequence'},
-        'batched_eval': {'value': batched, 'unit': 'trial patterns/s', 'form': 'staged pipeline, 16384 independent vectors'},
+        'proposals_per_s': proposals / (t1 - t0), 'evaluated_fraction': evaluated / max(1, evaluated + (s1 - s0)),
+        'chi2_rel_err_vs_fp64': err_chosen,
+        'plain': {'value': plain, 'unit': 'trial patterns/s', 'form': 'one trial per launch sequence',
+                  'chi2_rel_err_vs_fp64': chi2_error(pb, X[:16], chi_plain[:16])},
+        'batched_eval': {'value': batched, 'unit': 'trial patterns/s', 'form': 'staged pipeline, 16384 independent vectors',
+                         'chi2_rel_err_vs_fp64': chi2_error(pb, XB[:16], gb[:16])},
+        'nacl_cpu_path_case': nacl,
         'matrix_form': mat,
         'uneven_batch': uneven,
         'learned_screening': learned,
         'free_intensities': free,
+        'lm_normal_equations': normal,
         'chosen_over_plain': value / plain,
         'roofline': roof,
         'cpu_baseline': cpu,
     }
     if cpu:
         out['chosen_over_cpu'] = value / cpu['value']
+        out['chosen_over_cpu_single_thread'] = value / cpu['single_thread']
     print(json.dumps(out))
 
 

@@ -305,6 +305,11 @@ int pt_set_kernel_timing(pt_plan *plan, int on);
  * 8 solve / polish passes and a convergence flag (js/pt_gpu.js:398, 255, 363). */
 int pt_set_solve_limit(pt_plan *plan, int max_iterations);
 
+/* PT proposals since pt_set_states over all chains: out[0] evaluated exactly, out[1]
+ * rejected without an evaluation (invalid cell, an infeasible move of a feasible chain,
+ * or skipped by the surrogate screen).  No counterpart in the reference (measurement). */
+int pt_trial_counts(pt_plan *plan, double *out);
+
 /* Timing of the last pt_batch / pt_eval call on the plan's stream (HIP events):
  * elapsed milliseconds of the whole call, summed milliseconds of the dominant kernel's
  * timed launches (kernel timing mode, or batched pt_eval), and that launch count. */

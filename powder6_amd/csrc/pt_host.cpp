@@ -930,7 +930,7 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
         (rc = upload(pl, &d_coeff, term_coeff)) || (rc = upload(pl, &d_lo, vlo)) || (rc = upload(pl, &d_hi, vhi)) ||
         (rc = upload(pl, &d_sc, vsc)) || (rc = upload(pl, &d_sbasis, sbasis)) || (rc = upload(pl, &d_sguard, sguard)) ||
         (rc = upload(pl, &d_free_k, free_k)) ||
-        (rc = dalloc(pl, &d_state, (size_t)pl->chains * pl->stride))) {
+        (rc = dalloc(pl, &d_state, (size_t)pl->chains * pl->stride)) || (rc = dalloc(pl, &P.noeval, (size_t)pl->chains))) {
         pt_plan_destroy(pl);
         return rc;
     }
@@ -1052,6 +1052,7 @@ int pt_set_states(pt_plan *pl, const double *x, int nvec) {
         b[3 * nv + PT_S_VALID] = 1;
     }
     HIPCHECK(hipMemcpyAsync(pl->P.state, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, pl->stream), "state upload");
+    HIPCHECK(hipMemsetAsync(pl->P.noeval, 0, sizeof(double) * pl->chains, pl->stream), "state upload");
     HIPCHECK(hipStreamSynchronize(pl->stream), "state upload");
     pl->states_set = true;
     return 0;
@@ -2040,6 +2041,24 @@ int pt_workload(pt_plan *pl, const double *x, double *out) {
     for (int q = 0; q < pl->P.n_phases; ++q) S += pl->P.phase[q].n_atoms * pl->P.phase[q].n_reps_full;
     out[0] = (double)h[0]; out[1] = (double)h[1]; out[2] = (double)h[2]; out[3] = (double)S;
     out[4] = (double)pl->np; out[5] = (double)pl->P.n_bkg;
+    return 0;
+}
+
+int pt_trial_counts(pt_plan *pl, double *out) {
+    if (!pl || !out) return fail(PT_ERR_ARG, "NULL argument");
+    NEED_DEVICE(pl);
+    (void)hipSetDevice(pl->device);
+    std::vector<double> st((size_t)pl->chains * pl->stride), ne(pl->chains);
+    HIPCHECK(hipMemcpyAsync(st.data(), pl->P.state, sizeof(double) * st.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipMemcpyAsync(ne.data(), pl->P.noeval, sizeof(double) * ne.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipStreamSynchronize(pl->stream), "trial counts");
+    double props = 0, skipped = 0;
+    for (int c = 0; c < pl->chains; ++c) {
+        props += st[(size_t)c * pl->stride + 3 * pl->nv + PT_S_PROPOSALS];
+        skipped += ne[c];
+    }
+    out[0] = props - skipped;
+    out[1] = skipped;
     return 0;
 }
 

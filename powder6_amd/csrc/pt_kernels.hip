@@ -1643,6 +1643,7 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
         r = take_best ? 2 : 1;
     }
     if (!boot) sc[4] += 1.0;
+    if (!boot && !im->go && P.noeval) P.noeval[t] += 1.0;
     sc[2] = step + 1.0;
     return r;
 }

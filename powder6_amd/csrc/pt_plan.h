@@ -148,6 +148,8 @@ struct DevPlan {
     const DevSepPlan *sp;      // separated profiles and free intensities (always set)
     const DevFollow *follow;   // [n_follow]
     double *state;
+    double *noeval;            // per chain: proposals rejected without an evaluation (invalid cell,
+                               // infeasible move of a feasible chain, or skipped by the screen)
 };
 
 // a separated-profile phase of one trial, formed once in the setup (FP32)

@@ -179,6 +179,7 @@ def lib():
         L.pt_normal_equations.argtypes = [ctypes.c_void_p, P_D, P_D, P_D, P_D, P_D]
         L.pt_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.pt_set_solve_limit.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pt_trial_counts.argtypes = [ctypes.c_void_p, P_D]
         L.pt_matrix_pattern.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_matrix_timing.argtypes = [ctypes.c_void_p, P_D, P_D]
         L.pt_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, P_I, P_D]
@@ -294,6 +295,12 @@ class Plan:
             e.state = out
             raise
         return out
+
+    def trial_counts(self):
+        """Proposals since initial(): (evaluated exactly, rejected without an evaluation)."""
+        out = np.zeros(2)
+        _check(lib().pt_trial_counts(self.h, _dptr(out)))
+        return int(out[0]), int(out[1])
 
     def solve_limit(self, max_iterations):
         """Active-set iterations per free-intensity solve (0: the default 3 m + 10)."""

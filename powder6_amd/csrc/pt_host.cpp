@@ -234,8 +234,9 @@ struct pt_plan {
     bool tuned = false;
     double tune_ms[3] = {0, 0, 0};        // accumulate time per step at 1, 2, 4 tiles per wavefront
     void *d_img = nullptr;
-    ptk::LineRec *d_srec = nullptr;
-    int4 *d_sspan = nullptr;
+    ptk::StagedLine *d_lrec = nullptr;   // staged line records, 2 nr per slot
+    ptk::LineRec *d_mrec = nullptr;      // unit-height lines of the shared profile matrix, 2 nr
+    int4 *d_mspan = nullptr;
     double2 *d_part = nullptr;
     double *d_schi2 = nullptr;
     // one PT sweep captured as a graph
@@ -1069,14 +1070,14 @@ static int ensure_slots(pt_plan *pl, int n) {
     // that points at them is dropped below)
     drop_graph(pl->gexec, pl->graph);
     (void)hipStreamSynchronize(pl->stream);
-    dfree(pl, (char **)&pl->d_img); dfree(pl, &pl->d_srec); dfree(pl, &pl->d_sspan); dfree(pl, &pl->d_part);
+    dfree(pl, (char **)&pl->d_img); dfree(pl, &pl->d_lrec); dfree(pl, &pl->d_part);
     dfree(pl, &pl->d_schi2);
     if (pl->P.has_free) {
         ptk::DevSepPlan &h = pl->sp_host;
         dfree(pl, &h.fspan); dfree(pl, &h.resid); dfree(pl, &h.fx); dfree(pl, &h.unsolved); dfree(pl, &h.nnls_ws);
     }
     if ((rc = dalloc(pl, (char **)&pl->d_img, (size_t)n * ptk::trial_image_bytes())) ||
-        (rc = dalloc(pl, &pl->d_srec, (size_t)n * nl)) || (rc = dalloc(pl, &pl->d_sspan, (size_t)n * nl)) ||
+        (rc = dalloc(pl, &pl->d_lrec, (size_t)n * nl)) ||
         (rc = dalloc(pl, &pl->d_part, (size_t)n * pl->ngroups)) || (rc = dalloc(pl, &pl->d_schi2, (size_t)n)))
         return rc;
     if (pl->P.has_free) {
@@ -1121,15 +1122,15 @@ static int run_stages(pt_plan *pl, const double *dX, int ntrials, int pt_mode, i
         if (pt_mode) HIPCHECK(ptk::launch_stage_follow(f->P, pl->P.state, pl->stride, pl->d_img, f->d_img, ntrials, pl->stream), "setup");
         else HIPCHECK(ptk::launch_stage_setup(f->P, dX, f->d_img, ntrials, 0, 0, pl->stream), "setup");
     }
-    HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, ntrials, pl->stream), "lines");
+    HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, ntrials, pl->stream), "lines");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[0], pl->stream), "timing");
-    HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
+    HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
     if (ev_pair) HIPCHECK(hipEventRecord(ev_pair[1], pl->stream), "timing");
-    if (pl->P.has_free) HIPCHECK(ptk::launch_stage_nnls(pl->P, pl->d_img, pl->d_srec, pl->d_part, pl->ngroups, ntrials, pl->stream), "intensities");
+    if (pl->P.has_free) HIPCHECK(ptk::launch_stage_nnls(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, ntrials, pl->stream), "intensities");
     for (pt_plan *f : pl->follow) {
-        HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_srec, f->d_sspan, ntrials, pl->stream), "lines");
-        HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_srec, f->d_sspan, f->d_part, f->ngroups, ntrials, f->tpw, pl->stream), "accumulate");
-        if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_srec, f->d_part, f->ngroups, ntrials, pl->stream), "intensities");
+        HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_lrec, ntrials, pl->stream), "lines");
+        HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_lrec, f->d_part, f->ngroups, ntrials, f->tpw, pl->stream), "accumulate");
+        if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_lrec, f->d_part, f->ngroups, ntrials, pl->stream), "intensities");
     }
     HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, ntrials, d_chi2, pt_mode, pl->stream), "finish");
     return 0;
@@ -1261,10 +1262,10 @@ static int tune_plan(pt_plan *pl) {
     HIPCHECK(hipMemcpy2DAsync(pl->d_x, nvb, pl->P.state, sizeof(double) * pl->stride, nvb, pl->chains, hipMemcpyDeviceToDevice,
                               pl->stream), "tile tuning");
     HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, pl->chains, 0, 0, pl->stream), "setup");
-    HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
+    HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, pl->chains, pl->stream), "lines");
     int best = 2;
     rc = time_tiles(pl->stream, [&](int tpw) {
-        return ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, pl->chains, tpw, pl->stream);
+        return ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, pl->chains, tpw, pl->stream);
     }, pl->tune_ms, &best);
     if (rc) return rc;
     if (best != pl->tpw) drop_graph(pl->gexec, pl->graph);
@@ -1313,9 +1314,9 @@ static int enqueue_screened_sweep(pt_plan *pl, hipEvent_t *ev) {
         if (rc) return rc;
         HIPCHECK(ptk::screen_pick(pl->P, pl->S, pl->mlp->iter, pl->stream), "screen");
         HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, pl->chains, 1, 0, pl->stream, pl->S.pick), "setup");
-        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->chains, pl->stream), "lines");
+        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, pl->chains, pl->stream), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], pl->stream), "timing");
-        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, pl->chains, pl->tpw, pl->stream), "accumulate");
+        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, pl->chains, pl->tpw, pl->stream), "accumulate");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], pl->stream), "timing");
         HIPCHECK(ptk::launch_stage_finish(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->chains, pl->d_schi2, 1, pl->stream), "finish");
         HIPCHECK(ptk::screen_record(pl->P, pl->S, pl->stream), "screen");
@@ -1339,17 +1340,17 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
     int rc = enqueue_follow(pl, st, t0, n);
     if (rc) return rc;
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
-        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, n, st, t0), "lines");
+        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, n, st, t0), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], st), "timing");
-        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_part, pl->ngroups, n, pl->tpw, st, t0),
+        HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, n, pl->tpw, st, t0),
                  "accumulate");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s + 1], st), "timing");
-        if (pl->P.has_free) HIPCHECK(ptk::launch_stage_nnls(pl->P, pl->d_img, pl->d_srec, pl->d_part, pl->ngroups, n, st, t0), "intensities");
+        if (pl->P.has_free) HIPCHECK(ptk::launch_stage_nnls(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, n, st, t0), "intensities");
         for (pt_plan *f : pl->follow) {
-            HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_srec, f->d_sspan, n, st, t0), "lines");
-            HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_srec, f->d_sspan, f->d_part, f->ngroups, n, f->tpw, st, t0),
+            HIPCHECK(ptk::launch_stage_lines(f->P, f->d_img, f->d_lrec, n, st, t0), "lines");
+            HIPCHECK(ptk::launch_stage_accum(f->P, f->d_img, f->d_lrec, f->d_part, f->ngroups, n, f->tpw, st, t0),
                      "accumulate");
-            if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_srec, f->d_part, f->ngroups, n, st, t0), "intensities");
+            if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_lrec, f->d_part, f->ngroups, n, st, t0), "intensities");
         }
         if (s + 1 < pl->steps_per_sweep) {
             HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, st, n, t0), "finish");
@@ -1515,20 +1516,19 @@ int pt_eval_noalloc(pt_plan *pl, const double *d_x, int ntrials, double *d_chi2)
     HIPCHECK(ptk::noalloc_scratch(pl->device, &base, &cap), "scratch lookup");
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t n = (size_t)ntrials, nl = 2 * (size_t)pl->nr;
-    const size_t img_b = up(n * ptk::trial_image_bytes()), rec_b = up(n * nl * sizeof(ptk::LineRec)),
-                 span_b = up(n * nl * sizeof(int4)), part_b = up(n * pl->ngroups * sizeof(double2));
-    if (img_b + rec_b + span_b + part_b > cap) {
-        const size_t per = ptk::trial_image_bytes() + nl * (sizeof(ptk::LineRec) + sizeof(int4)) + pl->ngroups * sizeof(double2);
+    const size_t img_b = up(n * ptk::trial_image_bytes()), rec_b = up(n * nl * sizeof(ptk::StagedLine)),
+                 part_b = up(n * pl->ngroups * sizeof(double2));
+    if (img_b + rec_b + part_b > cap) {
+        const size_t per = ptk::trial_image_bytes() + nl * sizeof(ptk::StagedLine) + pl->ngroups * sizeof(double2);
         return fail(PT_ERR_BUDGET, "PT allocation-free evaluation takes at most " + std::to_string((cap - 1024) / per) +
                                        " trials of this plan per call (" + std::to_string(cap >> 20) + " MiB scratch).");
     }
     void *img = base;
-    ptk::LineRec *rec = reinterpret_cast<ptk::LineRec *>(base + img_b);
-    int4 *span = reinterpret_cast<int4 *>(base + img_b + rec_b);
-    double2 *part = reinterpret_cast<double2 *>(base + img_b + rec_b + span_b);
+    ptk::StagedLine *rec = reinterpret_cast<ptk::StagedLine *>(base + img_b);
+    double2 *part = reinterpret_cast<double2 *>(base + img_b + rec_b);
     HIPCHECK(ptk::launch_stage_setup(pl->P, d_x, img, ntrials, 0, 0, pl->stream), "setup");
-    HIPCHECK(ptk::launch_stage_lines(pl->P, img, rec, span, ntrials, pl->stream), "lines");
-    HIPCHECK(ptk::launch_stage_accum(pl->P, img, rec, span, part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
+    HIPCHECK(ptk::launch_stage_lines(pl->P, img, rec, ntrials, pl->stream), "lines");
+    HIPCHECK(ptk::launch_stage_accum(pl->P, img, rec, part, pl->ngroups, ntrials, pl->tpw, pl->stream), "accumulate");
     HIPCHECK(ptk::launch_stage_finish(pl->P, img, part, pl->ngroups, ntrials, d_chi2, 0, pl->stream), "finish");
     HIPCHECK(hipStreamSynchronize(pl->stream), "allocation-free evaluation");
     return 0;
@@ -1636,7 +1636,7 @@ int pt_eval(pt_plan *pl, int form, const double *x, int ntrials, double *chi2) {
 static int matrix_fill(pt_plan *pl, int bf16) {
     HIPCHECK(hipMemsetAsync(pl->d_phi, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
     HIPCHECK(hipMemsetAsync(pl->d_plo, 0, sizeof(unsigned short) * (size_t)pl->np_pad * pl->kpad, pl->stream), "clear");
-    HIPCHECK(ptk::launch_matrix_fill(pl->P, pl->d_srec, pl->d_sspan, pl->d_phi, pl->d_plo, pl->kpad, bf16, pl->stream), "matrix fill");
+    HIPCHECK(ptk::launch_matrix_fill(pl->P, pl->d_mrec, pl->d_mspan, pl->d_phi, pl->d_plo, pl->kpad, bf16, pl->stream), "matrix fill");
     pl->matrix_bf16 = bf16;
     return 0;
 }
@@ -1647,7 +1647,7 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
         // same columns, other element type: rebuild the unit-height lines and refill P
         HIPCHECK(hipMemcpyAsync(pl->d_x, pl->x0.data(), sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
         HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, 1, 0, 0, pl->stream), "setup");
-        HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_factor, pl->stream), "matrix columns");
+        HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_mrec, pl->d_mspan, pl->d_factor, pl->stream), "matrix columns");
         int rc = matrix_fill(pl, bf16);
         if (rc) return rc;
         HIPCHECK(hipStreamSynchronize(pl->stream), "matrix build");
@@ -1659,16 +1659,17 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
     pl->kpad = (pl->nr + 31) / 32 * 32;           // 32-reflection slices
     pl->np_pad = (pl->np + 127) / 128 * 128;      // 128-point row tiles
     pl->row_tiles = pl->np_pad / 128;
-    if ((rc = dalloc(pl, &pl->d_factor, (size_t)pl->nr)) || (rc = dalloc(pl, &pl->d_phi, (size_t)pl->np_pad * pl->kpad)) ||
+    if ((rc = dalloc(pl, &pl->d_factor, (size_t)pl->nr)) || (rc = dalloc(pl, &pl->d_mrec, 2 * (size_t)pl->nr)) ||
+        (rc = dalloc(pl, &pl->d_mspan, 2 * (size_t)pl->nr)) || (rc = dalloc(pl, &pl->d_phi, (size_t)pl->np_pad * pl->kpad)) ||
         (rc = dalloc(pl, &pl->d_plo, (size_t)pl->np_pad * pl->kpad)) || (rc = dalloc(pl, &pl->d_band, (size_t)pl->row_tiles)) ||
         (rc = dalloc(pl, &pl->d_yw, (size_t)pl->np_pad)))
         return rc;
     // unit-height lines of the start vector
     HIPCHECK(hipMemcpyAsync(pl->d_x, pl->x0.data(), sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
     HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, 1, 0, 0, pl->stream), "setup");
-    HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_srec, pl->d_sspan, pl->d_factor, pl->stream), "matrix columns");
+    HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_mrec, pl->d_mspan, pl->d_factor, pl->stream), "matrix columns");
     std::vector<int4> spans(2 * (size_t)pl->nr);
-    HIPCHECK(hipMemcpyAsync(spans.data(), pl->d_sspan, sizeof(int4) * spans.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
+    HIPCHECK(hipMemcpyAsync(spans.data(), pl->d_mspan, sizeof(int4) * spans.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
     if ((rc = matrix_fill(pl, bf16))) return rc;
     HIPCHECK(hipStreamSynchronize(pl->stream), "matrix build");
     // band of reflections meeting each 128-point row tile, in whole 32-reflection slices
@@ -2222,7 +2223,7 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
         const size_t C = (size_t)pl->chains, nl = 2 * (size_t)pl->nr;
         for (int v = 0; v < 3; ++v) M.nsplit[v] = tiles_of(pl->np, 1 << v);
         if ((rc = dalloc(g, (char **)&M.img, C * ptk::trial_image_bytes())) || (rc = dalloc(g, &M.rec, C * nl)) ||
-            (rc = dalloc(g, &M.span, C * nl)) || (rc = dalloc(g, &M.partial, C * M.nsplit[0])) || (rc = dalloc(g, &M.chi2, C)))
+            (rc = dalloc(g, &M.partial, C * M.nsplit[0])) || (rc = dalloc(g, &M.chi2, C)))
             return bail(rc);
         g->hmem.push_back(M);
         g->offsets.push_back(off);

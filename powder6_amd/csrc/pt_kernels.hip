@@ -5,9 +5,11 @@
 //   k_stage_setup        one workgroup per trial: the proposal, the model parameters, the
 //                        cell metric, the atom positions, the variance guard -> trial image
 //   k_stage_lines        one thread per (reflection, trial): Bragg angle, widths, prepared
-//                        pseudo-Voigt, Lp, |F|^2, heights, the Ka2 companion -> line records
-//   k_stage_accum<T>     the banded profile sum over 128-point tiles and the compensated
-//                        weighted chi^2 of every 512-point group
+//                        pseudo-Voigt, Lp, |F|^2, heights, the Ka2 companion -> staged line
+//                        records (64 bytes each, in the form the sum evaluates)
+//   k_stage_accum<T>     the banded profile sum over 128-point tiles (line records by
+//                        scalar loads) and the compensated weighted chi^2 of every
+//                        512-point group
 //   k_stage_nnls         free intensities (Le Bail / WPPF): per-trial normal equations and
 //                        a non-negative solve
 //   k_stage_finish(_setup)  the ordered chi^2 sum and the FP64 Metropolis step, fused with
@@ -875,12 +877,47 @@ __global__ void __launch_bounds__(64) k_stage_follow(DevPlan P, const double *__
     else follow_body<false>(P, mstate, mstride, mimg, img, t_off + blockIdx.x, sh);
 }
 
-// line records of reflections [kb * blockDim.x, ...) of trial t; every line with a window
-// widens the line range of the 512-point groups it meets (reduced in LDS, then one global
-// atomic min / max per group and workgroup)
+// span.w bit of a staged line that must take the general path
+constexpr int kGeneralLine = 4;
+// |u| beyond which exp2(-u^2) < 2^-40: the Gaussian term of a plain line is dropped there
+constexpr float kGaussCutF = 6.3245553f;
+
+// a line record and its window in the staged form (StagedLine); uni: the grid is uniform
+__device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec &R, int4 v, bool uni) {
+    if (!uni) return StagedLine{R.a, R.r, R.l, v};
+    // k need not be the nearest grid point: any k within a few steps keeps c - t_k small
+    const double c = (double)R.a.x + (double)R.a.y;
+    const double k = (double)rintf((float)(c - P.t0) * (1.0f / P.step_hi));
+    const double off = c - (P.t0 + k * P.step);
+    StagedLine o{make_float4((float)k, (float)off, R.a.z, R.a.w), R.r, R.l, v};
+    if (v.w == 0) {
+        if (R.r.x < 0.0f) {
+            const double sc = (double)sqrtf(-R.r.x);
+            o.l = make_float4((float)(P.step * sc), (float)(off * sc), R.r.y / R.r.x, 0.0f);
+            const double sq = (double)sqrtf(fmaxf(R.r.y, 0.0f));
+            o.r.x = (float)(P.step * sq);
+            o.r.y = (float)(off * sq);
+            // the Gaussian's range in grid points, i - k in ((l.y - cut) / l.x, (l.y + cut) / l.x),
+            // widened by a point on each side for the rounding of u and of this estimate
+            const float il = __builtin_amdgcn_rcpf(o.l.x), kf = o.a.x;
+            const float lo = floorf((o.l.y - kGaussCutF) * il) - 2.0f + kf, hi = ceilf((o.l.y + kGaussCutF) * il) + 3.0f + kf;
+            const float sx = (float)v.x, sy = (float)v.y;
+            const unsigned lo_off = lo > sx ? (unsigned)fminf(lo - sx, 65535.0f) : 0u;
+            const unsigned hi_off = hi < sy ? (unsigned)fminf(sy - hi, 65535.0f) : 0u;
+            o.span.z = (int)(lo_off | (hi_off << 16));
+        } else {
+            o.span.w = kGeneralLine;
+        }
+    }
+    return o;
+}
+
+// staged line records of reflections [kb * blockDim.x, ...) of trial t; every line with a
+// window widens the line range of the 512-point groups it meets (reduced in LDS, then one
+// global atomic min / max per group and workgroup)
 template <bool SEP = true>
-__device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, LineRec *rec, int4 *span, int t, int kb,
-                                           Shared &sh, int (*gr)[2]) {
+__device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, StagedLine *rec, int t, int kb, Shared &sh,
+                                           int (*gr)[2]) {
     const int k = kb * blockDim.x + threadIdx.x;
     const int4 hk = k < P.nr ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
     stage_reps(P, sh);
@@ -890,19 +927,27 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, Li
     __syncthreads();
     if (k < P.nr) {
         const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (sh.go) {
-            build_reflection<SEP>(P, sh, k, rec + base, span + base, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr);
+            LineRec lr[2] = {{z4, z4, z4}, {z4, z4, z4}};
+            int4 ls[2];
+            build_reflection<SEP>(P, sh, k, lr, ls, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr);
+            const bool uni = P.uniform != 0;
             for (int q = 0; q < 2; ++q) {
-                const int4 sp = span[base + q];
-                if (sp.y > sp.x)
+                const int4 sp = ls[q];
+                if (sp.y > sp.x) {
+                    rec[base + q] = stage_line(P, lr[q], sp, uni);
                     for (int g = sp.x >> 9; g <= (sp.y - 1) >> 9; ++g) {
                         atomicMin(&gr[g][0], 2 * k + q);
                         atomicMax(&gr[g][1], 2 * k + q);
                     }
+                } else {
+                    rec[base + q] = StagedLine{lr[q].a, lr[q].r, lr[q].l, sp};   // no window here (a free line's is in fspan)
+                }
             }
         } else {
-            span[base] = make_int4(0, 0, 0, 0);
-            span[base + 1] = make_int4(0, 0, 0, 0);
+            rec[base] = StagedLine{z4, z4, z4, make_int4(0, 0, 0, 0)};
+            rec[base + 1] = StagedLine{z4, z4, z4, make_int4(0, 0, 0, 0)};
             if (SEP && P.has_free) { P.sp->fspan[base] = make_int4(0, 0, 0, 0); P.sp->fspan[base + 1] = make_int4(0, 0, 0, 0); }
         }
     }
@@ -925,19 +970,18 @@ __device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, in
     return t < ntrials;
 }
 
-__global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img, LineRec *rec,
-                                                     int4 *span, int per, int ntrials, int t_off) {
+__global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img, StagedLine *rec, int per, int ntrials,
+                                                     int t_off) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
     // the lean instance for plans with neither separated profiles nor free intensities
-    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, span, t_off + t, u, sh, gr);
-    else lines_body<false>(P, img, rec, span, t_off + t, u, sh, gr);
+    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, t_off + t, u, sh, gr);
+    else lines_body<false>(P, img, rec, t_off + t, u, sh, gr);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
-constexpr int kStage = 256;            // line records staged in LDS per pass (16 KB: 8 waves per SIMD)
 
 // points per accumulate workgroup for TPW 128-point tiles per wavefront (two points per lane
 // and tile); TPW is the tile shape the autotuner picks (1, 2 or 4)
@@ -955,37 +999,6 @@ __device__ __forceinline__ float line_point(const LineRec &R, bool split, float 
     const float l = __builtin_amdgcn_rcpf(fmaf(f.y, q, 1.0f));
     const float v = fmaxf(fmaf(f.z, l, fmaf(f.w, g, -R.a.w)), 0.0f);
     return inwin ? R.a.z * v : 0.0f;
-}
-
-// A line record staged in LDS together with its window.  On a uniform grid the staging
-// rewrites the centre relative to the nearest grid point k: a = (k, c - t_k, height,
-// pedestal), and a plain line (equal flanks, no asymmetry) gets its profile in scaled
-// form, l = (step s, (c - t_k) s, -B, 0) with s = sqrt(-r.x) and B = r.y / s^2, so that
-// u = (i - k) step s - (c - t_k) s is the Gaussian's argument and -u^2 serves both terms.
-struct StagedLine {
-    float4 a, r, l;
-    int4 span;
-};
-
-// span.w bit of a staged line that must take the general path
-constexpr int kGeneralLine = 4;
-
-__device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec &R, int4 v, bool uni) {
-    if (!uni) return StagedLine{R.a, R.r, R.l, v};
-    // k need not be the nearest grid point: any k within a few steps keeps c - t_k small
-    const double c = (double)R.a.x + (double)R.a.y;
-    const double k = (double)rintf((float)(c - P.t0) * (1.0f / P.step_hi));
-    const double off = c - (P.t0 + k * P.step);
-    StagedLine o{make_float4((float)k, (float)off, R.a.z, R.a.w), R.r, R.l, v};
-    if (v.w == 0) {
-        if (R.r.x < 0.0f) {
-            const double sc = (double)sqrtf(-R.r.x);
-            o.l = make_float4((float)(P.step * sc), (float)(off * sc), R.r.y / R.r.x, 0.0f);
-        } else {
-            o.span.w = kGeneralLine;
-        }
-    }
-    return o;
 }
 
 // One staged line of the general form (split flanks, asymmetry) at a lane's two points
@@ -1045,25 +1058,35 @@ __device__ __forceinline__ float plain_value(const float4 &a, const float4 &r, c
     return __builtin_amdgcn_fmed3f(fmaf(r.z, lz, fmaf(r.w, g, -a.w)), 0.0f, 1.0f);
 }
 
-// LDS of one accumulate workgroup
+constexpr int kStage = 256;            // line records staged in LDS per pass (16 KB: 8 waves per SIMD)
+
+// LDS of one accumulate workgroup: a chunk of the trial's staged line records, the chi^2
+// of every wavefront tile
 struct AccShared {
     StagedLine line[kStage];
-    float bkg[PTK_MAX_BKG_F];
-    int wcnt[kAccWaves];
-    int count;
     float red_s[4][kAccWaves], red_c[4][kAccWaves];
 };
 
+// A plain staged line on a uniform grid at a point where its Gaussian term is dropped
+// (index fi): clamp(cl / (1 + w^2) - pedestal, 0, 1), w in the Lorentzian's scaled form.
+__device__ __forceinline__ float lorentz_value(const float4 &a, const float4 &r, float fi) {
+    const float w = fmaf(fi - a.x, r.x, -r.y);
+    return __builtin_amdgcn_fmed3f(fmaf(r.z, __builtin_amdgcn_rcpf(fmaf(w, w, 1.0f)), -a.w), 0.0f, 1.0f);
+}
+
 // Per-point sums of one wavefront tile, one per class of (line, tile) pair: plain lines
-// that meet both 64-point halves of the tile, plain lines that meet one half, other lines.
-// Each class is summed in line order and the three are added at the end, so the order of
+// that meet both 64-point halves of the tile, with their Gaussian term (two) or without it
+// (twl: the Gaussian is negligible over the whole tile); plain lines that meet one half,
+// with (one) or without (onl) the Gaussian there; other lines (gen).  Each class is summed
+// in line order and the classes are added in a fixed order at the end, so the order of
 // the additions does not depend on how lines fall into LDS passes or tile shapes.
 struct TileAcc {
-    float two0, two1, one0, one1, gen0, gen1;
+    float two0, two1, twl0, twl1, one0, one1, onl0, onl1, gen0, gen1;
 };
 
-// The staged lines that meet one wavefront's 128-point tile [t_lo, t_lo + 128), summed
-// into the lane's two points (t_lo + lane, t_lo + lane + 64).
+// The staged lines in LDS that meet one wavefront's 128-point tile [t_lo, t_lo + 128),
+// summed into the lane's two points (t_lo + lane, t_lo + lane + 64).  Lines whose window
+// misses the tile take no part (their mask bits are clear).
 template <bool UNI>
 __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_lo, float2 t0, float2 t1, float step, TileAcc &A) {
     const int lane = threadIdx.x & 63, t_mid = t_lo + 64, t_hi = t_lo + 128;
@@ -1072,7 +1095,7 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
     for (int lb = 0; lb < count; lb += 64) {
         // per staged line, as bit masks over the 64 lines of this block (scalar registers)
         const int q = lb + lane;
-        bool oa = false, ob = false, pl = false, sf = false, af = false;
+        bool oa = false, ob = false, pl = false, sf = false, af = false, ga = false, gb = false;
         if (q < count) {
             const int4 v = S.line[q].span;
             oa = v.x < t_mid && v.y > t_lo;
@@ -1080,8 +1103,16 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
             pl = v.w == 0;
             sf = (v.w & 1) != 0;
             af = (v.w & 2) != 0;
+            // the Gaussian's range (the window itself off a uniform grid)
+            const int glo = UNI && pl ? v.x + (v.z & 0xffff) : v.x;
+            const int ghi = UNI && pl ? v.y - (int)((unsigned)v.z >> 16) : v.y;
+            ga = glo < t_mid && ghi > t_lo;
+            gb = glo < t_hi && ghi > t_mid;
         }
-        unsigned long long m_two = __ballot(pl && oa && ob), m_a = __ballot(pl && oa && !ob), m_b = __ballot(pl && ob && !oa);
+        const bool both = pl && oa && ob, only_a = pl && oa && !ob, only_b = pl && ob && !oa;
+        unsigned long long m_two = __ballot(both && (ga || gb)), m_a = __ballot(only_a && ga), m_b = __ballot(only_b && gb);
+        unsigned long long m_twl = UNI ? __ballot(both && !ga && !gb) : 0ull;
+        const unsigned long long m_al = UNI ? __ballot(only_a && !ga) : 0ull, m_bl = UNI ? __ballot(only_b && !gb) : 0ull;
         unsigned long long m_gen = __ballot(!pl && (oa || ob));
         const unsigned long long m_split = __ballot(sf), m_asym = __ballot(af);
         // plain lines over the whole tile, two per iteration (all their LDS reads first)
@@ -1106,6 +1137,30 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
             const float v10 = plain_value<UNI>(a1, r1, l1, t0, fi0), v11 = plain_value<UNI>(a1, r1, l1, t1, fi1);
             A.two0 = fmaf(a1.z, v10, A.two0);
             A.two1 = fmaf(a1.z, v11, A.two1);
+        }
+        if (UNI) {
+            // the same without the Gaussian term
+            n = __popcll(m_twl);
+            for (; n >= 2; n -= 2) {
+                const int b1 = __ffsll((long long)m_twl) - 1;
+                m_twl &= m_twl - 1ull;
+                const int b2 = __ffsll((long long)m_twl) - 1;
+                m_twl &= m_twl - 1ull;
+                const StagedLine &L1 = S.line[lb + b1], &L2 = S.line[lb + b2];
+                const float4 a1 = L1.a, r1 = L1.r, a2 = L2.a, r2 = L2.r;
+                const float v10 = lorentz_value(a1, r1, fi0), v11 = lorentz_value(a1, r1, fi1);
+                const float v20 = lorentz_value(a2, r2, fi0), v21 = lorentz_value(a2, r2, fi1);
+                A.twl0 = fmaf(a1.z, v10, A.twl0);
+                A.twl1 = fmaf(a1.z, v11, A.twl1);
+                A.twl0 = fmaf(a2.z, v20, A.twl0);
+                A.twl1 = fmaf(a2.z, v21, A.twl1);
+            }
+            if (n) {
+                const StagedLine &L1 = S.line[lb + __ffsll((long long)m_twl) - 1];
+                const float4 a1 = L1.a, r1 = L1.r;
+                A.twl0 = fmaf(a1.z, lorentz_value(a1, r1, fi0), A.twl0);
+                A.twl1 = fmaf(a1.z, lorentz_value(a1, r1, fi1), A.twl1);
+            }
         }
         // plain lines over one half: two lines at the lane's point of that half
 #pragma unroll
@@ -1132,6 +1187,28 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
                 acc = fmaf(a1.z, plain_value<UNI>(a1, L1.r, UNI ? L1.l : zero4, t, fi), acc);
             }
             if (h) A.one1 = acc; else A.one0 = acc;
+            if (UNI) {
+                m = h ? m_bl : m_al;
+                acc = h ? A.onl1 : A.onl0;
+                k = __popcll(m);
+                for (; k >= 2; k -= 2) {
+                    const int b1 = __ffsll((long long)m) - 1;
+                    m &= m - 1ull;
+                    const int b2 = __ffsll((long long)m) - 1;
+                    m &= m - 1ull;
+                    const StagedLine &L1 = S.line[lb + b1], &L2 = S.line[lb + b2];
+                    const float4 a1 = L1.a, r1 = L1.r, a2 = L2.a, r2 = L2.r;
+                    const float v1 = lorentz_value(a1, r1, fi), v2 = lorentz_value(a2, r2, fi);
+                    acc = fmaf(a1.z, v1, acc);
+                    acc = fmaf(a2.z, v2, acc);
+                }
+                if (k) {
+                    const StagedLine &L1 = S.line[lb + __ffsll((long long)m) - 1];
+                    const float4 a1 = L1.a;
+                    acc = fmaf(a1.z, lorentz_value(a1, L1.r, fi), acc);
+                }
+                if (h) A.onl1 = acc; else A.onl0 = acc;
+            }
         }
         // the others (split flanks, asymmetry, or a profile that needs the general form)
         while (m_gen) {
@@ -1146,15 +1223,15 @@ __device__ __forceinline__ void tile_lines(const AccShared &S, int count, int t_
 }
 
 // Banded profile sum and partial chi^2 of points [P s, P s + P) of trial t, P =
-// acc_points(TPW): the lines whose windows meet those points are compacted, in line
-// order, into LDS; every wavefront then sums them for its own TPW 128-point tiles (two
-// points per lane) and the chi^2 of every 512-point group is reduced with two-sum into
-// partial[t * nsplit + group], nsplit = the trial's 512-point group count.
+// acc_points(TPW): the staged line records of the workgroup's line range (the union of
+// its 512-point groups' ranges) are copied into LDS a chunk at a time; every wavefront
+// sums those that meet each of its TPW 128-point tiles (two points per lane), and the
+// chi^2 of every 512-point group (the tiles j of the four wavefronts) is reduced with
+// two-sum into partial[t * nsplit + group], nsplit = the trial's 512-point group count.
 // FREE: the plan has free intensities, so the residual of every point is kept for their solve
 template <int TPW, bool FREE = false>
-__device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict__ img, const LineRec *__restrict__ grec,
-                                  const int4 *__restrict__ gspan, double2 *partial, int nsplit, int t, int split,
-                                  AccShared &S) {
+__device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict__ img, const StagedLine *__restrict__ grec,
+                                  double2 *partial, int nsplit, int t, int split, AccShared &S) {
     // the wavefront index is uniform: keep it (and every tile bound) in scalar registers
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const TrialImage *im = img + t;
@@ -1162,11 +1239,9 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
         if (tid < TPW && split * TPW + tid < nsplit) partial[(size_t)t * nsplit + split * TPW + tid] = make_double2(0.0, 0.0);
         return;
     }
-    if (tid < PTK_MAX_BKG_F) S.bkg[tid] = im->bkg[tid];
     const int lo_pt = split * acc_points(TPW), hi_pt = min(P.np, lo_pt + acc_points(TPW));
     const int nl = 2 * P.nr;
-    const LineRec *rec = grec + (size_t)t * nl;
-    const int4 *sp = gspan + (size_t)t * nl;
+    const StagedLine *rec = grec + (size_t)t * nl;
     const bool uni = P.uniform != 0;
     // lines that can meet this workgroup's points: the union of its groups' ranges
     int first = 0x7fffffff, last = -1;
@@ -1176,58 +1251,33 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     }
     const int lend = min(nl, last + 1);
     TileAcc ta[TPW];
-    float2 tf[2 * TPW];
+    float2 tf[2 * TPW], ow[2 * TPW];
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
         const int base = lo_pt + (wave + kAccWaves * j) * 128 + lane;
-        ta[j] = TileAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        tf[2 * j] = !uni && base < P.np ? P.tff[base] : make_float2(0.f, 0.f);
-        tf[2 * j + 1] = !uni && base + 64 < P.np ? P.tff[base + 64] : make_float2(0.f, 0.f);
+        ta[j] = TileAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        tf[2 * j] = !uni && base < hi_pt ? P.tff[base] : make_float2(0.f, 0.f);
+        tf[2 * j + 1] = !uni && base + 64 < hi_pt ? P.tff[base + 64] : make_float2(0.f, 0.f);
+        ow[2 * j] = base < hi_pt ? P.obs[base] : make_float2(0.f, 0.f);
+        ow[2 * j + 1] = base + 64 < hi_pt ? P.obs[base + 64] : make_float2(0.f, 0.f);
     }
-    if (tid == 0) S.count = 0;
-    __syncthreads();
-    for (int c0 = first; c0 < lend; c0 += 256) {
-        // compact the overlapping lines of this chunk, in order, into the stage
-        const int L = c0 + tid;
-        // the record is loaded with its window (one round trip), used only if the window
-        // meets this workgroup's points
-        int4 sv = make_int4(0, 0, 0, 0);
-        LineRec R{};
-        bool ov = false;
-        if (L < lend) {
-            sv = sp[L];
-            R = rec[L];
-            ov = sv.x < hi_pt && sv.y > lo_pt;
-        }
-        const unsigned long long m = __ballot(ov);
-        if (lane == 0) S.wcnt[wave] = __popcll(m);
+    for (int c0 = first; c0 < lend; c0 += kStage) {
+        const int count = min(kStage, lend - c0);
+        if (c0 > first) __syncthreads();                 // the last chunk's readers are done
+        for (int i = tid; i < count; i += blockDim.x) S.line[i] = rec[c0 + i];
         __syncthreads();
-        int off = S.count;
-        for (int w = 0; w < wave; ++w) off += S.wcnt[w];
-        const int pre = __popcll(m & ((1ull << lane) - 1ull));
-        if (ov) S.line[off + pre] = stage_line(P, R, sv, uni);
-        __syncthreads();
-        if (tid == 0) { int n = S.count; for (int w = 0; w < kAccWaves; ++w) n += S.wcnt[w]; S.count = n; }
-        __syncthreads();
-        const int count = S.count;
-        if (count > kStage - 256 || (c0 + 256 >= lend && count > 0)) {
 #pragma unroll
-            for (int j = 0; j < TPW; ++j) {
-                const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
-                if (t_lo < hi_pt) {
-                    if (uni) tile_lines<true>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, ta[j]);
-                    else tile_lines<false>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, ta[j]);
-                }
+        for (int j = 0; j < TPW; ++j) {
+            const int t_lo = lo_pt + (wave + kAccWaves * j) * 128;
+            if (t_lo < hi_pt) {
+                if (uni) tile_lines<true>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, ta[j]);
+                else tile_lines<false>(S, count, t_lo, tf[2 * j], tf[2 * j + 1], P.step_hi, ta[j]);
             }
-            __syncthreads();
-            if (tid == 0) S.count = 0;
-            __syncthreads();
         }
     }
     // chi^2 of this workgroup's points, one partial per 512-point group (the 128-point
     // tiles j of the four wavefronts), so that the partials -- and the ordered sum the
     // finish takes over them -- do not depend on the tile shape
-    const int ngroups = (P.np + 511) >> 9;
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
         float s = 0.f, c = 0.f;
@@ -1235,12 +1285,13 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
         for (int h = 0; h < 2; ++h) {
             const int i = lo_pt + (wave + kAccWaves * j) * 128 + h * 64 + lane;
             if (i < hi_pt) {
-                const float2 ow = P.obs[i];
-                if (ow.y > 0.0f) {
-                    const float yb = bkg_value(P, S.bkg, i);
-                    const float yp = h ? (ta[j].two1 + ta[j].one1) + ta[j].gen1 : (ta[j].two0 + ta[j].one0) + ta[j].gen0;
-                    const float r = ow.x - (yb + yp);
-                    neumaier(s, c, ow.y * r * r);
+                const float2 o = ow[2 * j + h];
+                if (o.y > 0.0f) {
+                    const float yb = bkg_value(P, im->bkg, i);
+                    const float yp = h ? ((ta[j].two1 + ta[j].twl1) + (ta[j].one1 + ta[j].onl1)) + ta[j].gen1
+                                       : ((ta[j].two0 + ta[j].twl0) + (ta[j].one0 + ta[j].onl0)) + ta[j].gen0;
+                    const float r = o.x - (yb + yp);
+                    neumaier(s, c, o.y * r * r);
                     if (FREE) P.sp->resid[(size_t)t * P.np + i] = r;      // the free intensities' target
                 } else if (FREE) {
                     P.sp->resid[(size_t)t * P.np + i] = 0.0f;
@@ -1258,7 +1309,7 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
     __syncthreads();
     if (tid < TPW) {
         const int g = split * TPW + tid;
-        if (g < ngroups) {
+        if (g < nsplit) {
             double hi = 0.0, lo = 0.0;
             for (int w = 0; w < kAccWaves; ++w) { hi += (double)S.red_s[tid][w]; lo += (double)S.red_c[tid][w]; }
             partial[(size_t)t * nsplit + g] = make_double2(hi, lo);
@@ -1268,12 +1319,12 @@ __device__ inline void accum_body(const DevPlan &P, const TrialImage *__restrict
 
 template <int TPW, bool FREE>
 __global__ void __launch_bounds__(256) k_stage_accum(DevPlan P, const TrialImage *__restrict__ img,
-                                                     const LineRec *__restrict__ grec, const int4 *__restrict__ gspan,
-                                                     double2 *partial, int nsplit, int per, int ntrials, int t_off) {
+                                                     const StagedLine *__restrict__ grec, double2 *partial, int nsplit, int per,
+                                                     int ntrials, int t_off) {
     __shared__ AccShared S;
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
-    accum_body<TPW, FREE>(P, img, grec, gspan, partial, nsplit, t_off + t, u, S);
+    accum_body<TPW, FREE>(P, img, grec, partial, nsplit, t_off + t, u, S);
 }
 
 // chi^2 of trial t: the ordered sum of its partials (3e38 for an invalid trial)
@@ -1341,7 +1392,7 @@ __device__ __forceinline__ float free_column(const DevPlan &P, const LineRec *R,
 }
 
 // the solve of trial t over its m live columns (flags per free reflection in `flag`)
-__device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__restrict__ rec, double2 *partial, int nsplit,
+__device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__restrict__ rec, double2 *partial, int nsplit,
                                           int t, const NnlsArrays &A, int m, const int *flag, double *red, int *s_int,
                                           double &s_tol) {
     const int nf = P.n_free, tid = threadIdx.x, nt = blockDim.x, T = m * (m + 1) / 2;
@@ -1357,7 +1408,8 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__res
     __syncthreads();
     for (int j = tid; j < 2 * m; j += nt) {
         const int k = P.sp->free_k[live[j >> 1]];
-        R[j] = rec[base + 2 * k + (j & 1)];
+        const StagedLine &q = rec[base + 2 * k + (j & 1)];      // a line without a window: the LineRec form
+        R[j] = LineRec{q.a, q.r, q.l};
         Sp[j] = P.sp->fspan[base + 2 * k + (j & 1)];
     }
     __syncthreads();
@@ -1560,7 +1612,7 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const LineRec *__res
         partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? (ok ? csum - red[0] : 3e38) : 0.0, 0.0);
 }
 
-__global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const LineRec *__restrict__ rec,
+__global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const StagedLine *__restrict__ rec,
                                                     double2 *partial, int nsplit, int t_off) {
     const int t = t_off + blockIdx.x;
     if (!img[t].go) return;                              // not evaluated: the trial costs 3e38
@@ -1762,8 +1814,8 @@ __global__ void __launch_bounds__(256) k_group_lines(const DevPlan *__restrict__
     if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh, gr);
-    else lines_body<false>(P, (TrialImage *)M.img, M.rec, M.span, w.y, w.z, sh, gr);
+    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr);
+    else lines_body<false>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr);
 }
 
 template <int TPW>
@@ -1774,7 +1826,7 @@ __global__ void __launch_bounds__(256) k_group_accum(const DevPlan *__restrict__
     if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    accum_body<TPW>(P, (const TrialImage *)M.img, M.rec, M.span, M.partial, M.nsplit[0], w.y, w.z, S);
+    accum_body<TPW>(P, (const TrialImage *)M.img, M.rec, M.partial, M.nsplit[0], w.y, w.z, S);
 }
 
 // one thread per chain: the Metropolis step (pt_mode) or the chi^2 of the chain's vector
@@ -2360,7 +2412,7 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
     hipLaunchKernelGGL(k_stage_setup, dim3(ntrials), dim3(64), 0, s, P, X, (TrialImage *)img, pt_mode, bootstrap, pick, t_off);
     return hipGetLastError();
 }
-hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const LineRec *rec, double2 *partial, int nsplit, int ntrials,
+hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int nsplit, int ntrials,
                              hipStream_t s, int t_off) {
     hipLaunchKernelGGL(k_stage_nnls, dim3(ntrials), dim3(256), nnls_lds_bytes(min(P.n_free, PTK_MAX_FREE_LIVE), P.n_free), s, P,
                        (const TrialImage *)img, rec, partial, nsplit, t_off);
@@ -2380,18 +2432,18 @@ hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstri
 }
 static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }
 
-hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s, int t_off) {
+hipError_t launch_stage_lines(const DevPlan &P, const void *img, StagedLine *rec, int ntrials, hipStream_t s, int t_off) {
     const int per = (P.nr + 255) / 256;
-    hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, span, per,
-                       ntrials, t_off);
+    hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per, ntrials,
+                       t_off);
     return hipGetLastError();
 }
-hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
-                              int ngroups, int ntrials, int tpw, hipStream_t s, int t_off) {
+hipError_t launch_stage_accum(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int ngroups,
+                              int ntrials, int tpw, hipStream_t s, int t_off) {
     const int per = (ngroups + tpw - 1) / tpw;
     const dim3 grid(xcd_grid(per, ntrials));
     const TrialImage *im = (const TrialImage *)img;
-#define PTK_ACCUM(T, F) hipLaunchKernelGGL((k_stage_accum<T, F>), grid, dim3(256), 0, s, P, im, rec, span, partial, ngroups, per, ntrials, t_off)
+#define PTK_ACCUM(T, F) hipLaunchKernelGGL((k_stage_accum<T, F>), grid, dim3(256), 0, s, P, im, rec, partial, ngroups, per, ntrials, t_off)
     if (P.has_free) {
         if (tpw == 1) PTK_ACCUM(1, true); else if (tpw == 4) PTK_ACCUM(4, true); else PTK_ACCUM(2, true);
     } else {

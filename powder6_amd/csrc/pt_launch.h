@@ -19,18 +19,17 @@ hipError_t set_lds_limits(size_t bytes);
 // setup draws the plain proposal itself
 hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int ntrials, int pt_mode, int bootstrap,
                               hipStream_t s, const double2 *pick = nullptr, int t_off = 0);
-hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const LineRec *rec, double2 *partial, int nsplit, int ntrials,
+hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int nsplit, int ntrials,
                              hipStream_t s, int t_off = 0);
 size_t nnls_shared_bytes(int m);           // dynamic LDS of the free-intensity solve for m reflections
 size_t nnls_slot_bytes(int nf);             // global workspace per trial slot for a solve of up to nf reflections
 hipError_t nnls_lds_limit(size_t bytes);
 hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,
                                hipStream_t s, int t_off = 0);
-hipError_t launch_stage_lines(const DevPlan &P, const void *img, LineRec *rec, int4 *span, int ntrials, hipStream_t s,
-                              int t_off = 0);
+hipError_t launch_stage_lines(const DevPlan &P, const void *img, StagedLine *rec, int ntrials, hipStream_t s, int t_off = 0);
 // partial chi^2 per 512-point group: ngroups = ceil(np / 512) per trial, whatever tpw
-hipError_t launch_stage_accum(const DevPlan &P, const void *img, const LineRec *rec, const int4 *span, double2 *partial,
-                              int ngroups, int ntrials, int tpw, hipStream_t s, int t_off = 0);
+hipError_t launch_stage_accum(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int ngroups,
+                              int ntrials, int tpw, hipStream_t s, int t_off = 0);
 hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 *partial, int nsplit, int ntrials,
                                double *chi2, int pt_mode, hipStream_t s, int t_off = 0);
 hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s, int ntrials,

@@ -175,12 +175,28 @@ struct LineRec {
     float4 a, r, l;            // r first: a line with equal flanks needs only a and r
 };
 
-// staged-pipeline buffers of one member of a group (per chain: a trial image, 2 nr line
-// records and spans, one partial chi^2 per 512-point group, the chi^2)
+// The staged pipeline's line record (64 bytes, one scalar load): a line record with its
+// window, written by the lines stage in the form the accumulate stage sums.  On a uniform
+// grid the centre is taken relative to a grid point k near it: a = (k, c - t_k, height,
+// pedestal), and a plain line (equal flanks, no asymmetry) gets its profile in scaled
+// form, l = (step s, (c - t_k) s, -B, 0) with s = sqrt(-r.x) and B = r.y / s^2, so that
+// u = (i - k) step s - (c - t_k) s is the Gaussian's argument and -u^2 serves both terms;
+// r.xy = (step q, (c - t_k) q) with q = sqrt(r.y) scale the Lorentzian alone, whose
+// denominator is 1 + w^2, w = (i - k) step q - (c - t_k) q; and span.z packs the points
+// [span.x + lo, span.y - hi) outside which the Gaussian term is below 2^-40 of the line's
+// peak (lo, hi: 16 bits each, rounded so that the range only grows).  Other lines, lines
+// on other grids and lines without a window (among them the free-intensity lines, which
+// the intensity solve reads) keep the LineRec form, span.z the asymmetry's bits.
+struct StagedLine {
+    float4 a, r, l;
+    int4 span;
+};
+
+// staged-pipeline buffers of one member of a group (per chain: a trial image, 2 nr staged
+// line records, one partial chi^2 per 512-point group, the chi^2)
 struct DevMember {
     void *img;
-    LineRec *rec;
-    int4 *span;
+    StagedLine *rec;
     double2 *partial;
     double *chi2;
     int nsplit[3];             // accumulate workgroups per trial for 1, 2, 4 tiles per wavefront;

@@ -1352,10 +1352,23 @@ __host__ __device__ inline size_t nnls_region_bytes(int c) {
     const size_t tri = (size_t)c * (c + 1) / 2;
     return 16 * (size_t)(2 * c) * 4 + ((tri * 2 + 6 * (size_t)c) * 8 + 15) / 16 * 16 + (size_t)c * 4 * 6;
 }
-// LDS for up to m live columns out of nf free reflections (the region, then nf flags)
-__host__ __device__ inline size_t nnls_lds_bytes(int m, int nf) { return nnls_region_bytes(m) + (size_t)nf * 4 + 64; }
+// LDS for up to m live columns out of nf free reflections (the region, then nf flags and
+// nf envelope indices)
+__host__ __device__ inline size_t nnls_lds_bytes(int m, int nf) { return nnls_region_bytes(m) + (size_t)nf * 8 + 64; }
 // a trial slot's global workspace for the solve when more columns are live than LDS holds
 size_t nnls_slot_bytes(int nf) { return (nnls_region_bytes(std::min(nf, PTK_MAX_FREE_WS_LIVE)) + 255) / 256 * 256; }
+// dynamic LDS of the solve: the region and flags, then a cache of the live columns' values
+// over their windows in what is left of the CU's 160 KB (the static reductions aside)
+constexpr int kNnlsThreads = 1024;     // 16 wavefronts per trial: the CU's latency is hidden by them
+constexpr size_t kNnlsLdsMax = 160 * 1024 - 12 * 1024;
+__host__ __device__ inline size_t nnls_base_bytes(int nf) {
+    return (nnls_lds_bytes(nf < PTK_MAX_FREE_LIVE ? nf : PTK_MAX_FREE_LIVE, nf) + 15) / 16 * 16;
+}
+__host__ __device__ inline int nnls_cache_floats(int nf) {
+    const size_t b = nnls_base_bytes(nf), f = b >= kNnlsLdsMax ? 0 : (kNnlsLdsMax - b) / 4;
+    return (int)(f < 32768 ? f : 32768);
+}
+__host__ __device__ inline size_t nnls_total_bytes(int nf) { return nnls_base_bytes(nf) + 4 * (size_t)nnls_cache_floats(nf); }
 
 struct NnlsArrays {
     LineRec *R;
@@ -1377,37 +1390,240 @@ __device__ __forceinline__ NnlsArrays nnls_carve(unsigned char *base, int c) {
 }
 __device__ __forceinline__ int tri_at(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
+// float-float 2theta of point i: computed on a uniform grid (t0 + i step, within 1e-9 of
+// a step of the axis), loaded otherwise
+__device__ __forceinline__ float2 point_tff(const DevPlan &P, int i) {
+    if (!P.uniform) return P.tff[i];
+    const double t = P.t0 + (double)i * P.step;
+    const float hi = (float)t;
+    return make_float2(hi, (float)(t - (double)hi));
+}
+
 // column j (the free reflection's lines at their weights) at point i
 __device__ __forceinline__ float free_column(const DevPlan &P, const LineRec *R, const int4 *Sp, int j, int i) {
     float v = 0.0f;
+    const float2 ti = point_tff(P, i);
     for (int q = 0; q < 2; ++q) {
         const int4 sp = Sp[2 * j + q];
         if (i >= sp.x && i < sp.y) {
             const LineRec &L = R[2 * j + q];
-            const float2 ti = P.tff[i];
             v += L.a.z * line_value(L, __int_as_float(sp.z), (ti.x - L.a.x) + (ti.y - L.a.y));
         }
     }
     return v;
 }
 
-// the solve of trial t over its m live columns (flags per free reflection in `flag`)
+// ---- the active set of a small solve in one wavefront ----------------------------------
+// FP64 helpers: lane k's value (k uniform), lane src's value (src per lane), the value of
+// the first active lane (for a value every lane holds alike), the wavefront's sum
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)u, k), hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_fetch(double v, int src) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__shfl((int)u, src), hi = (unsigned)__shfl((int)(u >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double uniform_value(double v) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)u), hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+// LDS written by some lanes of the wavefront, then read by others: the writes complete
+// (s_waitcnt lgkmcnt(0)) before the wavefront goes on, and no access moves across
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int tri_row(int i) { return i * (i + 1) / 2; }
+
+// Lawson and Hanson's active set for m <= 64 equilibrated columns, run by the 64 lanes of
+// one wavefront (lane j: variable j's x, b, passive and blocked flags in registers) with
+// no workgroup barrier.  The passive set keeps its order of entry; its Cholesky factor
+// (packed rows in L, inverse diagonal in invd, plist[position] = variable, pos[variable]
+// = position) grows by one row when a column enters and is formed again when columns
+// leave.  The steps are those of the workgroup form: the largest gradient entry above
+// tol enters (the first on ties), the step back to the boundary when the passive solution
+// is not positive, a column that cannot stay is blocked.  In: x (warm start), pas, blk;
+// out: x, pas, s_int[2] = a block was not positive definite, s_int[3] = converged.  Every
+// array is in LDS (the wavefront's own accesses to it are in order).
+__device__ __forceinline__ void nnls_wave(const double *H, const double *b, double *x, int *pas, int *blk, int m, double tol,
+                                       int outer_cap, double *L, double *invd, int *plist, int *pos, int *s_int) {
+    const int ln = threadIdx.x & 63;
+    const bool act = ln < m;
+    double xj = act ? x[ln] : 0.0;
+    const double bj = act ? b[ln] : 0.0;
+    bool pj = act && pas[ln] != 0, kj = act && blk[ln] != 0;
+    int fail = 0, conv = 0;
+    // Cholesky of the passive block in list order, rows over the lanes (right looking)
+    auto factor = [&](int p) -> bool {
+        if (ln < p) {
+            const int vi = plist[ln];
+            for (int k = 0; k <= ln; ++k) L[tri_row(ln) + k] = H[tri_at(vi, plist[k])];
+        }
+        wave_lds_sync();
+        for (int c = 0; c < p; ++c) {
+            const double d = uniform_value(L[tri_row(c) + c]);
+            if (!(d > 0.0)) return false;
+            const double sd = sqrt(d), inv = 1.0 / sd;
+            if (ln == c) { L[tri_row(c) + c] = sd; invd[c] = inv; }
+            if (ln > c && ln < p) L[tri_row(ln) + c] *= inv;
+            wave_lds_sync();
+            if (ln > c && ln < p) {
+                const double lic = L[tri_row(ln) + c];
+                for (int k = c + 1; k <= ln; ++k) L[tri_row(ln) + k] -= lic * L[tri_row(k) + c];
+            }
+            wave_lds_sync();
+        }
+        return true;
+    };
+    // variable q enters at position p: the new row solves L y = H[passive, q]
+    auto append = [&](int q, int p) -> bool {
+        double h = ln < p ? H[tri_at(plist[ln], q)] : 0.0;
+        for (int c = 0; c < p; ++c) {
+            const double yc = lane_value(h, c) * invd[c];
+            if (ln == c) h = yc;
+            else if (ln > c && ln < p) h -= L[tri_row(ln) + c] * yc;
+        }
+        const double d = uniform_value(H[tri_at(q, q)] - wave_sum(ln < p ? h * h : 0.0));
+        if (!(d > 0.0)) return false;
+        if (ln < p) L[tri_row(p) + ln] = h;
+        if (ln == 0) { const double sd = sqrt(d); L[tri_row(p) + p] = sd; invd[p] = 1.0 / sd; plist[p] = q; pos[q] = p; }
+        wave_lds_sync();
+        return true;
+    };
+    // the passive solution L L' z = b_P, returned in variable space (z of lane j's variable)
+    auto solve = [&](int p) -> double {
+        double y = lane_fetch(bj, ln < p ? plist[ln] : 0);
+        for (int c = 0; c < p; ++c) {
+            const double yc = lane_value(y, c) * invd[c];
+            if (ln == c) y = yc;
+            else if (ln > c && ln < p) y -= L[tri_row(ln) + c] * yc;
+        }
+        for (int c = p - 1; c >= 0; --c) {
+            const double zc = lane_value(y, c) * invd[c];
+            if (ln == c) y = zc;
+            else if (ln < c) y -= L[tri_row(c) + ln] * zc;
+        }
+        return lane_fetch(y, pj ? pos[ln] : 0);
+    };
+    // drop the variables that left the passive set from the list, keeping the order
+    auto compact = [&](int p) -> int {
+        const int vi = ln < p ? plist[ln] : 0;
+        // every lane takes part in the shuffle: a lane outside it would read zero from it
+        const int pv = __shfl(pj ? 1 : 0, vi);
+        const bool keep = ln < p && pv != 0;
+        const unsigned long long km = __ballot(keep);
+        const int newpos = __popcll(km & ((1ull << ln) - 1ull));
+        wave_lds_sync();
+        if (keep) { plist[newpos] = vi; pos[vi] = newpos; }
+        wave_lds_sync();
+        return __popcll(km);
+    };
+    // the warm passive set in index order
+    const unsigned long long pm = __ballot(pj);
+    int p = __popcll(pm);
+    if (pj) { const int ps = __popcll(pm & ((1ull << ln) - 1ull)); plist[ps] = ln; pos[ln] = ps; }
+    wave_lds_sync();
+    bool warm = p > 0;
+    if (warm && !factor(p)) fail = 1;
+    for (int it = 0; !fail && it < outer_cap; ++it) {
+        int jm = -2;
+        if (!warm) {
+            double gj = bj;
+            for (int k = 0; k < m; ++k) {
+                const double xk = lane_value(xj, k);
+                if (act) gj -= H[tri_at(ln, k)] * xk;
+            }
+            const bool cand = act && !pj && !kj && gj > tol;
+            double bv = cand ? gj : -1.0;
+            int bi = cand ? ln : 64;
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(bv, o);
+                const int oi = __shfl_xor(bi, o);
+                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            }
+            jm = __builtin_amdgcn_readfirstlane(bi);
+            if (jm >= 64) { conv = 1; break; }
+            if (ln == jm) pj = true;
+            if (!append(jm, p)) { fail = 1; break; }
+            ++p;
+        }
+        warm = false;
+        bool settled = false;
+        for (int inner = 0; inner < 3 * m + 10; ++inner) {
+            const double z = solve(p);
+            const bool bad = pj && !(z > 0.0);
+            if (!__ballot(bad)) {
+                xj = pj ? z : 0.0;
+                settled = true;
+                break;
+            }
+            double al = bad ? xj / (xj - z) : 2.0;
+            for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o));
+            const double alpha = uniform_value(al);
+            if (pj) xj += alpha * (z - xj);
+            if (pj && !(xj > 0.0)) { pj = false; xj = 0.0; }
+            p = compact(p);
+            if (!factor(p)) { fail = 1; break; }
+        }
+        if (fail || !settled) break;
+        if (jm >= 0 && ln == jm && !pj) kj = true;       // could not stay positive: keep it out
+    }
+    if (act) { x[ln] = xj; pas[ln] = pj ? 1 : 0; blk[ln] = kj ? 1 : 0; }
+    if (ln == 0) { s_int[2] = fail; s_int[3] = conv; }
+}
+
+// the solve of trial t over its m live columns (flag[gi]: 1 + the column of free
+// reflection gi when it is live, else 0; kk[gi]: its envelope index)
 __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__restrict__ rec, double2 *partial, int nsplit,
-                                          int t, const NnlsArrays &A, int m, const int *flag, double *red, int *s_int,
-                                          double &s_tol) {
+                                          int t, const NnlsArrays &A, int m, const int *flag, const int *kk, double *red,
+                                          int *s_int, double &s_tol, float *colc, int ncache, bool in_lds) {
     const int nf = P.n_free, tid = threadIdx.x, nt = blockDim.x, T = m * (m + 1) / 2;
     const size_t base = (size_t)t * 2 * P.nr;
     LineRec *R = A.R;
     int4 *Sp = A.Sp;
     double *H = A.H, *Lf = A.Lf, *b = A.b, *x = A.x, *z = A.z, *g = A.g, *nrm = A.nrm, *tmp = A.tmp;
     int *pas = A.pas, *blk = A.blk, *idx = A.idx, *cs = A.cs, *ce = A.ce, *live = A.live;
-    if (tid == 0) {                                      // in envelope order
-        int n = 0;
-        for (int gi = 0; gi < nf; ++gi) if (flag[gi]) live[n++] = gi;
+    for (int gi = tid; gi < nf; gi += nt)               // in envelope order
+        if (flag[gi]) live[flag[gi] - 1] = gi;
+    const float *r = P.sp->resid + (size_t)t * P.np;
+    // the points' weights and residuals in LDS when they fit the cache (the columns' values
+    // after them), and c = sum w r^2 in the order of a strided loop (four loads in flight)
+    const int np = P.np;
+    const bool pts = 2 * np <= ncache;
+    float *wl = colc, *rl = colc + np;
+    double cw = 0.0;
+    for (int i0 = tid; i0 < np; i0 += 4 * nt) {
+        float2 o[4];
+        float rv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * nt;
+            o[u] = i < np ? P.obs[i] : make_float2(0.f, 0.f);
+            rv[u] = i < np ? r[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * nt;
+            if (i < np) {
+                if (pts) { wl[i] = o[u].y; rl[i] = rv[u]; }
+                if (o[u].y > 0.0f) cw += (double)o[u].y * (double)rv[u] * (double)rv[u];
+            }
+        }
     }
+    if (pts) { colc += 2 * np; ncache -= 2 * np; }
     __syncthreads();
     for (int j = tid; j < 2 * m; j += nt) {
-        const int k = P.sp->free_k[live[j >> 1]];
+        const int k = kk[live[j >> 1]];
         const StagedLine &q = rec[base + 2 * k + (j & 1)];      // a line without a window: the LineRec form
         R[j] = LineRec{q.a, q.r, q.l};
         Sp[j] = P.sp->fspan[base + 2 * k + (j & 1)];
@@ -1422,41 +1638,42 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
         cs[j] = lo; ce[j] = hi;
     }
     __syncthreads();
-    const float *r = P.sp->resid + (size_t)t * P.np;
-    // normal matrix and right-hand side (FP64 sums of FP32 terms): one wavefront per entry,
-    // its lanes over the overlap of the two windows (coalesced point loads), then a
-    // wavefront sum; entries whose windows do not meet are zero
+    // the columns' values over their windows, once, when they fit the LDS cache (offsets in idx)
+    if (tid == 0) {
+        int tot = 0;
+        for (int j = 0; j < m; ++j) { idx[j] = tot; tot += ce[j] > cs[j] ? ce[j] - cs[j] : 0; }
+        s_int[1] = tot;
+    }
+    __syncthreads();
+    const bool cached = s_int[1] <= ncache;
     const int wv = tid >> 6, ln = tid & 63, nw = nt >> 6;
-    for (int e = wv; e < T + m; e += nw) {
-        int j, k, lo, hi;
-        if (e < T) {
-            j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-            while ((j + 1) * (j + 2) / 2 <= e) ++j;
-            while (j * (j + 1) / 2 > e) --j;
-            k = e - j * (j + 1) / 2;
-            lo = max(cs[j], cs[k]); hi = min(ce[j], ce[k]);
-        } else {
-            j = k = e - T;                               // right-hand side b_j
-            lo = cs[j]; hi = ce[j];
-        }
-        double a = 0.0;
-        if (lo < hi) {                                   // 0 <= lo, hi <= np
-            for (int i = lo + ln; i < hi; i += 64) {
-                const float w = P.obs[i].y;
-                if (w > 0.0f)
-                    a += (double)w * (double)free_column(P, R, Sp, j, i) * (double)(e < T ? free_column(P, R, Sp, k, i) : r[i]);
+    if (cached) {                                        // one wavefront per column
+        for (int j = wv; j < m; j += nw)
+            for (int i = cs[j] + ln; i < ce[j]; i += 64) colc[idx[j] + i - cs[j]] = free_column(P, R, Sp, j, i);
+        __syncthreads();
+    }
+    auto column = [&](int j, int i) -> float { return cached ? colc[idx[j] + i - cs[j]] : free_column(P, R, Sp, j, i); };
+    // normal matrix and right-hand side (FP64 sums of FP32 terms): one wavefront per row j
+    // (k <= j, then b_j), its lanes over the overlap of the two windows (coalesced point
+    // loads), then a wavefront sum; entries whose windows do not meet are zero
+    for (int j = wv; j < m; j += nw) {
+        const int cj0 = cs[j], cj1 = ce[j];
+        for (int k = 0; k <= j + 1; ++k) {
+            const bool rhs = k == j + 1;                 // b_j
+            const int lo = rhs ? cj0 : max(cj0, cs[min(k, j)]), hi = rhs ? cj1 : min(cj1, ce[min(k, j)]);
+            double a = 0.0;
+            if (lo < hi) {                               // 0 <= lo, hi <= np; the same in every lane
+                for (int i = lo + ln; i < hi; i += 64) {
+                    const float w = pts ? wl[i] : P.obs[i].y;
+                    if (w > 0.0f) a += (double)w * (double)column(j, i) * (double)(rhs ? (pts ? rl[i] : r[i]) : column(k, i));
+                }
+                for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            }
+            if (ln == 0) {
+                if (rhs) b[j] = a;
+                else H[tri_at(j, k)] = a;
             }
         }
-        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
-        if (ln == 0) {
-            if (e < T) H[e] = a;
-            else b[j] = a;
-        }
-    }
-    double cw = 0.0;
-    for (int i = tid; i < P.np; i += nt) {
-        const float w = P.obs[i].y;
-        if (w > 0.0f) cw += (double)w * (double)r[i] * (double)r[i];
     }
     red[tid] = cw;
     __syncthreads();
@@ -1494,99 +1711,107 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
     }
     __syncthreads();
     const double tol = s_tol;
-    if (tid == 0) {
-        int any = 0;
-        for (int j = 0; j < m; ++j) any |= pas[j];
-        s_int[0] = any ? -2 : -1;                        // -2: first solve on the warm passive set
-    }
-    __syncthreads();
-    bool warm = s_int[0] == -2;
     // converged: the gradient has no positive entry off the passive set.  The iteration
     // caps, or a passive block that is not positive definite, leave the trial unsolved
     // (js/pt_gpu.js:363 solved(): its chi^2 is not used and the state's unsolved flag is set)
     bool converged = false;
     const int outer_cap = P.nnls_cap > 0 ? P.nnls_cap : 3 * m + 10;
-    for (int it = 0; it < outer_cap; ++it) {
-        if (!warm) {
-            for (int j = tid; j < m; j += nt) {
-                double a = b[j];
-                for (int k = 0; k < m; ++k) a -= H[tri_at(j, k)] * x[k];
-                g[j] = a;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int jm = -1;
-                for (int j = 0; j < m; ++j)
-                    if (!pas[j] && !blk[j] && g[j] > tol && (jm < 0 || g[j] > g[jm])) jm = j;
-                s_int[0] = jm;
-                if (jm >= 0) pas[jm] = 1;
-            }
-            __syncthreads();
-        }
-        const int jm = warm ? -2 : s_int[0];
-        warm = false;
-        if (jm == -1) { converged = true; break; }
-        bool settled = false;
-        for (int inner = 0; inner < 3 * m + 10; ++inner) {
-            if (tid == 0) {
-                int p = 0;
-                for (int j = 0; j < m; ++j) if (pas[j]) idx[p++] = j;
-                s_int[1] = p;
-            }
-            __syncthreads();
-            const int p = s_int[1];
-            // Cholesky of the passive block, one column at a time (rows in parallel)
-            for (int kc = 0; kc < p; ++kc) {
-                if (tid == 0) {
-                    double a = H[tri_at(idx[kc], idx[kc])];
-                    for (int q = 0; q < kc; ++q) a -= Lf[tri_at(kc, q)] * Lf[tri_at(kc, q)];
-                    if (!(a > 0.0)) { s_int[2] = 1; a = 1.0; }
-                    Lf[tri_at(kc, kc)] = sqrt(a);
-                }
-                __syncthreads();
-                for (int i = kc + 1 + tid; i < p; i += nt) {
-                    double a = H[tri_at(idx[i], idx[kc])];
-                    for (int q = 0; q < kc; ++q) a -= Lf[tri_at(i, q)] * Lf[tri_at(kc, q)];
-                    Lf[tri_at(i, kc)] = a / Lf[tri_at(kc, kc)];
-                }
-                __syncthreads();
-            }
-            if (tid == 0) {
-                int stop = s_int[2];
-                if (!stop) {
-                    for (int i = 0; i < p; ++i) {
-                        double a = b[idx[i]];
-                        for (int q = 0; q < i; ++q) a -= Lf[tri_at(i, q)] * tmp[q];
-                        tmp[i] = a / Lf[tri_at(i, i)];
-                    }
-                    for (int i = p - 1; i >= 0; --i) {
-                        double a = tmp[i];
-                        for (int q = i + 1; q < p; ++q) a -= Lf[tri_at(q, i)] * tmp[q];
-                        tmp[i] = a / Lf[tri_at(i, i)];
-                    }
-                    for (int j = 0; j < m; ++j) z[j] = 0.0;
-                    for (int i = 0; i < p; ++i) z[idx[i]] = tmp[i];
-                    bool allpos = true;
-                    for (int j = 0; j < m; ++j) if (pas[j] && !(z[j] > 0.0)) { allpos = false; break; }
-                    if (allpos) {
-                        for (int j = 0; j < m; ++j) x[j] = z[j];
-                        stop = 1;
-                    } else {
-                        double alpha = 2.0;
-                        for (int j = 0; j < m; ++j)
-                            if (pas[j] && !(z[j] > 0.0)) alpha = fmin(alpha, x[j] / (x[j] - z[j]));
-                        for (int j = 0; j < m; ++j) if (pas[j]) x[j] += alpha * (z[j] - x[j]);
-                        for (int j = 0; j < m; ++j) if (pas[j] && !(x[j] > 0.0)) { pas[j] = 0; x[j] = 0.0; }
-                    }
-                }
-                s_int[3] = stop;
-            }
-            __syncthreads();
-            if (s_int[3]) { settled = true; break; }
-        }
-        if (s_int[2] || !settled) break;                 // a block that is not positive definite, or the cap
-        if (tid == 0 && jm >= 0 && !pas[jm]) blk[jm] = 1;   // could not stay positive: keep it out
+    if (in_lds && m <= 64) {
+        // one wavefront, no workgroup barrier inside (plist in idx, pos in cs, the factor's
+        // inverse diagonal in tmp)
+        if (tid < 64) nnls_wave(H, b, x, pas, blk, m, tol, outer_cap, Lf, tmp, idx, cs, s_int);
         __syncthreads();
+        converged = s_int[3] != 0;
+    } else {
+        if (tid == 0) {
+            int any = 0;
+            for (int j = 0; j < m; ++j) any |= pas[j];
+            s_int[0] = any ? -2 : -1;                        // -2: first solve on the warm passive set
+        }
+        __syncthreads();
+        bool warm = s_int[0] == -2;
+        for (int it = 0; it < outer_cap; ++it) {
+            if (!warm) {
+                for (int j = tid; j < m; j += nt) {
+                    double a = b[j];
+                    for (int k = 0; k < m; ++k) a -= H[tri_at(j, k)] * x[k];
+                    g[j] = a;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    int jm = -1;
+                    for (int j = 0; j < m; ++j)
+                        if (!pas[j] && !blk[j] && g[j] > tol && (jm < 0 || g[j] > g[jm])) jm = j;
+                    s_int[0] = jm;
+                    if (jm >= 0) pas[jm] = 1;
+                }
+                __syncthreads();
+            }
+            const int jm = warm ? -2 : s_int[0];
+            warm = false;
+            if (jm == -1) { converged = true; break; }
+            bool settled = false;
+            for (int inner = 0; inner < 3 * m + 10; ++inner) {
+                if (tid == 0) {
+                    int p = 0;
+                    for (int j = 0; j < m; ++j) if (pas[j]) idx[p++] = j;
+                    s_int[1] = p;
+                }
+                __syncthreads();
+                const int p = s_int[1];
+                // Cholesky of the passive block, one column at a time (rows in parallel)
+                for (int kc = 0; kc < p; ++kc) {
+                    if (tid == 0) {
+                        double a = H[tri_at(idx[kc], idx[kc])];
+                        for (int q = 0; q < kc; ++q) a -= Lf[tri_at(kc, q)] * Lf[tri_at(kc, q)];
+                        if (!(a > 0.0)) { s_int[2] = 1; a = 1.0; }
+                        Lf[tri_at(kc, kc)] = sqrt(a);
+                    }
+                    __syncthreads();
+                    for (int i = kc + 1 + tid; i < p; i += nt) {
+                        double a = H[tri_at(idx[i], idx[kc])];
+                        for (int q = 0; q < kc; ++q) a -= Lf[tri_at(i, q)] * Lf[tri_at(kc, q)];
+                        Lf[tri_at(i, kc)] = a / Lf[tri_at(kc, kc)];
+                    }
+                    __syncthreads();
+                }
+                if (tid == 0) {
+                    int stop = s_int[2];
+                    if (!stop) {
+                        for (int i = 0; i < p; ++i) {
+                            double a = b[idx[i]];
+                            for (int q = 0; q < i; ++q) a -= Lf[tri_at(i, q)] * tmp[q];
+                            tmp[i] = a / Lf[tri_at(i, i)];
+                        }
+                        for (int i = p - 1; i >= 0; --i) {
+                            double a = tmp[i];
+                            for (int q = i + 1; q < p; ++q) a -= Lf[tri_at(q, i)] * tmp[q];
+                            tmp[i] = a / Lf[tri_at(i, i)];
+                        }
+                        for (int j = 0; j < m; ++j) z[j] = 0.0;
+                        for (int i = 0; i < p; ++i) z[idx[i]] = tmp[i];
+                        bool allpos = true;
+                        for (int j = 0; j < m; ++j) if (pas[j] && !(z[j] > 0.0)) { allpos = false; break; }
+                        if (allpos) {
+                            for (int j = 0; j < m; ++j) x[j] = z[j];
+                            stop = 1;
+                        } else {
+                            double alpha = 2.0;
+                            for (int j = 0; j < m; ++j)
+                                if (pas[j] && !(z[j] > 0.0)) alpha = fmin(alpha, x[j] / (x[j] - z[j]));
+                            for (int j = 0; j < m; ++j) if (pas[j]) x[j] += alpha * (z[j] - x[j]);
+                            for (int j = 0; j < m; ++j) if (pas[j] && !(x[j] > 0.0)) { pas[j] = 0; x[j] = 0.0; }
+                        }
+                    }
+                    s_int[3] = stop;
+                }
+                __syncthreads();
+                if (s_int[3]) { settled = true; break; }
+            }
+            if (s_int[2] || !settled) break;                 // a block that is not positive definite, or the cap
+            if (tid == 0 && jm >= 0 && !pas[jm]) blk[jm] = 1;   // could not stay positive: keep it out
+            __syncthreads();
+        }
     }
     for (int gi = tid; gi < nf; gi += nt) if (!flag[gi]) fx[gi] = 0.0;
     for (int j = tid; j < m; j += nt) fx[live[j]] = pas[j] && nrm[j] > 0.0 ? x[j] / nrm[j] : 0.0;
@@ -1612,37 +1837,52 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
         partial[(size_t)t * nsplit + k] = make_double2(k == 0 ? (ok ? csum - red[0] : 3e38) : 0.0, 0.0);
 }
 
-__global__ void __launch_bounds__(256) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const StagedLine *__restrict__ rec,
+__global__ void __launch_bounds__(kNnlsThreads) k_stage_nnls(DevPlan P, const TrialImage *__restrict__ img, const StagedLine *__restrict__ rec,
                                                     double2 *partial, int nsplit, int t_off) {
     const int t = t_off + blockIdx.x;
     if (!img[t].go) return;                              // not evaluated: the trial costs 3e38
     extern __shared__ __align__(16) unsigned char dyn[];
-    __shared__ double red[256];
-    __shared__ int s_int[4];
+    __shared__ double red[kNnlsThreads];
+    __shared__ int s_int[4], s_wc[kNnlsThreads / 64];
     __shared__ double s_tol;
     const int nf = P.n_free, cap = min(nf, PTK_MAX_FREE_LIVE), tid = threadIdx.x, nt = blockDim.x;
     const size_t base = (size_t)t * 2 * P.nr;
+    const int *free_k = P.sp->free_k;
+    const int4 *fspan = P.sp->fspan;
     // the live columns of this trial: free reflections with a line window in the pattern
-    // (the others have no data and stay at zero)
-    int *flag = reinterpret_cast<int *>(dyn + nnls_region_bytes(cap));
-    for (int gi = tid; gi < nf; gi += nt) {
-        const int k = P.sp->free_k[gi];
-        const int4 s0 = P.sp->fspan[base + 2 * k], s1 = P.sp->fspan[base + 2 * k + 1];
-        flag[gi] = (s0.y > s0.x || s1.y > s1.x) ? 1 : 0;
+    // (the others have no data and stay at zero), numbered in envelope order by a prefix
+    // count over the workgroup
+    int *flag = reinterpret_cast<int *>(dyn + nnls_region_bytes(cap)), *kk = flag + nf;
+    if (tid == 0) s_int[0] = 0;
+    for (int c0 = 0; c0 < nf; c0 += nt) {
+        const int gi = c0 + tid;
+        bool f = false;
+        if (gi < nf) {
+            const int k = free_k[gi];
+            const int4 s0 = fspan[base + 2 * k], s1 = fspan[base + 2 * k + 1];
+            kk[gi] = k;
+            f = s0.y > s0.x || s1.y > s1.x;
+        }
+        const unsigned long long bm = __ballot(f);
+        const int w = tid >> 6, l = tid & 63;
+        if (l == 0) s_wc[w] = __popcll(bm);
+        __syncthreads();
+        int off = s_int[0];
+        for (int q = 0; q < w; ++q) off += s_wc[q];
+        if (gi < nf) flag[gi] = f ? off + __popcll(bm & ((1ull << l) - 1ull)) + 1 : 0;
+        __syncthreads();
+        if (tid == 0) { int n = s_int[0]; for (int q = 0; q < (nt >> 6); ++q) n += s_wc[q]; s_int[0] = n; }
+        __syncthreads();
     }
-    __syncthreads();
-    if (tid == 0) {
-        int n = 0;
-        for (int gi = 0; gi < nf; ++gi) n += flag[gi];
-        s_int[0] = n;
-    }
-    __syncthreads();
+    if (nf <= 0) __syncthreads();                        // (no chunk: the count's reset)
     const int m = s_int[0];
+    float *colc = reinterpret_cast<float *>(dyn + nnls_base_bytes(nf));
+    const int ncache = nnls_cache_floats(nf);
     if (m <= cap) {
-        nnls_core(P, rec, partial, nsplit, t, nnls_carve(dyn, cap), m, flag, red, s_int, s_tol);
+        nnls_core(P, rec, partial, nsplit, t, nnls_carve(dyn, cap), m, flag, kk, red, s_int, s_tol, colc, ncache, true);
     } else if (P.sp->nnls_ws && m <= PTK_MAX_FREE_WS_LIVE) {   // more than LDS holds: the slot's workspace
         nnls_core(P, rec, partial, nsplit, t, nnls_carve(P.sp->nnls_ws + (size_t)t * P.sp->nnls_stride, PTK_MAX_FREE_WS_LIVE), m,
-                  flag, red, s_int, s_tol);
+                  flag, kk, red, s_int, s_tol, colc, ncache, false);
     } else {
         // more live intensities than one trial's solve takes: unsolved
         if (tid == 0) P.sp->unsolved[t] = 1;
@@ -2414,11 +2654,11 @@ hipError_t launch_stage_setup(const DevPlan &P, const double *X, void *img, int 
 }
 hipError_t launch_stage_nnls(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int nsplit, int ntrials,
                              hipStream_t s, int t_off) {
-    hipLaunchKernelGGL(k_stage_nnls, dim3(ntrials), dim3(256), nnls_lds_bytes(min(P.n_free, PTK_MAX_FREE_LIVE), P.n_free), s, P,
-                       (const TrialImage *)img, rec, partial, nsplit, t_off);
+    hipLaunchKernelGGL(k_stage_nnls, dim3(ntrials), dim3(kNnlsThreads), nnls_total_bytes(P.n_free), s, P, (const TrialImage *)img, rec,
+                       partial, nsplit, t_off);
     return hipGetLastError();
 }
-size_t nnls_shared_bytes(int nf) { return nnls_lds_bytes(std::min(nf, PTK_MAX_FREE_LIVE), nf); }
+size_t nnls_shared_bytes(int nf) { return nnls_total_bytes(nf); }
 hipError_t nnls_lds_limit(size_t bytes) {
     static size_t cur = 0;
     return raise_lds_limit((const void *)k_stage_nnls, bytes, cur);

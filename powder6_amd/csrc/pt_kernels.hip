@@ -1448,80 +1448,108 @@ __device__ __forceinline__ int tri_row(int i) { return i * (i + 1) / 2; }
 // Lawson and Hanson's active set for m <= 64 equilibrated columns, run by the 64 lanes of
 // one wavefront (lane j: variable j's x, b, passive and blocked flags in registers) with
 // no workgroup barrier.  The passive set keeps its order of entry; its Cholesky factor
-// (packed rows in L, inverse diagonal in invd, plist[position] = variable, pos[variable]
-// = position) grows by one row when a column enters and is formed again when columns
-// leave.  The steps are those of the workgroup form: the largest gradient entry above
-// tol enters (the first on ties), the step back to the boundary when the passive solution
-// is not positive, a column that cannot stay is blocked.  In: x (warm start), pas, blk;
-// out: x, pas, s_int[2] = a block was not positive definite, s_int[3] = converged.  Every
-// array is in LDS (the wavefront's own accesses to it are in order).
+// (packed rows in L, plist[position] = variable, pos[variable] = position, the inverse of
+// diagonal c in a register of lane c) grows by one row when a column enters and is formed
+// again, row by row, when columns leave.  The triangular solves take eight columns per
+// batch of LDS loads, so only the readlane / FMA chain is sequential.  The steps are those
+// of the workgroup form: the largest gradient entry above tol enters (the first on ties),
+// the step back to the boundary when the passive solution is not positive, a column that
+// cannot stay is blocked; a warm start more than a quarter of whose passive set comes out
+// not positive, or that needs a third step back, gives way to x = 0 (the optimum is the
+// same, reached in fewer steps).
+// In: x (warm start), pas, blk; out: x, pas, s_int[2] = a block
+// was not positive definite, s_int[3] = converged.  Every array is in LDS (the
+// wavefront's own accesses to it are in order).
 __device__ __forceinline__ void nnls_wave(const double *H, const double *b, double *x, int *pas, int *blk, int m, double tol,
-                                       int outer_cap, double *L, double *invd, int *plist, int *pos, int *s_int) {
+                                          int outer_cap, double *L, int *plist, int *pos, int *s_int) {
     const int ln = threadIdx.x & 63;
     const bool act = ln < m;
     double xj = act ? x[ln] : 0.0;
     const double bj = act ? b[ln] : 0.0;
     bool pj = act && pas[ln] != 0, kj = act && blk[ln] != 0;
+    double dinv = 0.0;                                   // lane c: 1 / L[c][c]
     int fail = 0, conv = 0;
-    // Cholesky of the passive block in list order, rows over the lanes (right looking)
-    auto factor = [&](int p) -> bool {
-        if (ln < p) {
-            const int vi = plist[ln];
-            for (int k = 0; k <= ln; ++k) L[tri_row(ln) + k] = H[tri_at(vi, plist[k])];
-        }
-        wave_lds_sync();
-        for (int c = 0; c < p; ++c) {
-            const double d = uniform_value(L[tri_row(c) + c]);
-            if (!(d > 0.0)) return false;
-            const double sd = sqrt(d), inv = 1.0 / sd;
-            if (ln == c) { L[tri_row(c) + c] = sd; invd[c] = inv; }
-            if (ln > c && ln < p) L[tri_row(ln) + c] *= inv;
-            wave_lds_sync();
-            if (ln > c && ln < p) {
-                const double lic = L[tri_row(ln) + c];
-                for (int k = c + 1; k <= ln; ++k) L[tri_row(ln) + k] -= lic * L[tri_row(k) + c];
+    // L y = v over positions < p (lane i holds v_i, gets y_i)
+    auto forward = [&](double v, int p) -> double {
+        for (int c0 = 0; c0 < p; c0 += 8) {
+            double lr[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = c0 + u;
+                const bool use = c < p && ln > c && ln < p;
+                lr[u] = L[use ? tri_row(ln) + c : 0];
+                if (!use) lr[u] = 0.0;
             }
-            wave_lds_sync();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = c0 + u;
+                if (c < p) {
+                    const double yc = lane_value(v, c) * lane_value(dinv, c);
+                    v = ln == c ? yc : v - lr[u] * yc;
+                }
+            }
         }
-        return true;
+        return v;
+    };
+    // L' z = v over positions < p
+    auto backward = [&](double v, int p) -> double {
+        for (int c1 = p - 1; c1 >= 0; c1 -= 8) {
+            double lr[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = c1 - u;
+                const bool use = c >= 0 && ln < c;
+                lr[u] = L[use ? tri_row(c) + ln : 0];
+                if (!use) lr[u] = 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = c1 - u;
+                if (c >= 0) {
+                    const double zc = lane_value(v, c) * lane_value(dinv, c);
+                    v = ln == c ? zc : v - lr[u] * zc;
+                }
+            }
+        }
+        return v;
     };
     // variable q enters at position p: the new row solves L y = H[passive, q]
     auto append = [&](int q, int p) -> bool {
-        double h = ln < p ? H[tri_at(plist[ln], q)] : 0.0;
-        for (int c = 0; c < p; ++c) {
-            const double yc = lane_value(h, c) * invd[c];
-            if (ln == c) h = yc;
-            else if (ln > c && ln < p) h -= L[tri_row(ln) + c] * yc;
-        }
+        const double h = forward(ln < p ? H[tri_at(plist[ln], q)] : 0.0, p);
         const double d = uniform_value(H[tri_at(q, q)] - wave_sum(ln < p ? h * h : 0.0));
         if (!(d > 0.0)) return false;
+        const double sd = sqrt(d);
         if (ln < p) L[tri_row(p) + ln] = h;
-        if (ln == 0) { const double sd = sqrt(d); L[tri_row(p) + p] = sd; invd[p] = 1.0 / sd; plist[p] = q; pos[q] = p; }
+        if (ln == p) dinv = 1.0 / sd;
+        if (ln == 0) { L[tri_row(p) + p] = sd; plist[p] = q; pos[q] = p; }
         wave_lds_sync();
+        return true;
+    };
+    // the factor of the passive list, formed again a row at a time from position r0 (the
+    // rows above r0 depend only on the passive columns above r0)
+    auto factor = [&](int p, int r0) -> bool {
+        for (int r = r0; r < p; ++r) {
+            const int q = __builtin_amdgcn_readfirstlane(plist[r]);
+            if (!append(q, r)) return false;
+        }
         return true;
     };
     // the passive solution L L' z = b_P, returned in variable space (z of lane j's variable)
     auto solve = [&](int p) -> double {
         double y = lane_fetch(bj, ln < p ? plist[ln] : 0);
-        for (int c = 0; c < p; ++c) {
-            const double yc = lane_value(y, c) * invd[c];
-            if (ln == c) y = yc;
-            else if (ln > c && ln < p) y -= L[tri_row(ln) + c] * yc;
-        }
-        for (int c = p - 1; c >= 0; --c) {
-            const double zc = lane_value(y, c) * invd[c];
-            if (ln == c) y = zc;
-            else if (ln < c) y -= L[tri_row(c) + ln] * zc;
-        }
+        y = backward(forward(y, p), p);
         return lane_fetch(y, pj ? pos[ln] : 0);
     };
-    // drop the variables that left the passive set from the list, keeping the order
-    auto compact = [&](int p) -> int {
+    // drop the variables that left the passive set from the list, keeping the order; r0:
+    // the first position that changed
+    auto compact = [&](int p, int &r0) -> int {
         const int vi = ln < p ? plist[ln] : 0;
         // every lane takes part in the shuffle: a lane outside it would read zero from it
         const int pv = __shfl(pj ? 1 : 0, vi);
         const bool keep = ln < p && pv != 0;
         const unsigned long long km = __ballot(keep);
+        const unsigned long long gone = __ballot(ln < p && !keep);
+        r0 = gone ? __ffsll((long long)gone) - 1 : p;
         const int newpos = __popcll(km & ((1ull << ln) - 1ull));
         wave_lds_sync();
         if (keep) { plist[newpos] = vi; pos[vi] = newpos; }
@@ -1534,14 +1562,23 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
     if (pj) { const int ps = __popcll(pm & ((1ull << ln) - 1ull)); plist[ps] = ln; pos[ln] = ps; }
     wave_lds_sync();
     bool warm = p > 0;
-    if (warm && !factor(p)) fail = 1;
+    if (warm && !factor(p, 0)) fail = 1;
     for (int it = 0; !fail && it < outer_cap; ++it) {
         int jm = -2;
         if (!warm) {
             double gj = bj;
-            for (int k = 0; k < m; ++k) {
-                const double xk = lane_value(xj, k);
-                if (act) gj -= H[tri_at(ln, k)] * xk;
+            for (int k0 = 0; k0 < m; k0 += 8) {
+                double hr[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool use = act && k0 + u < m;
+                    hr[u] = H[use ? tri_at(ln, k0 + u) : 0];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const double xk = lane_value(xj, k0 + u < m ? k0 + u : 0);
+                    if (act && k0 + u < m) gj -= hr[u] * xk;
+                }
             }
             const bool cand = act && !pj && !kj && gj > tol;
             double bv = cand ? gj : -1.0;
@@ -1562,8 +1599,19 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
         for (int inner = 0; inner < 3 * m + 10; ++inner) {
             const double z = solve(p);
             const bool bad = pj && !(z > 0.0);
-            if (!__ballot(bad)) {
+            const unsigned long long bm = __ballot(bad);
+            if (!bm) {
                 xj = pj ? z : 0.0;
+                settled = true;
+                break;
+            }
+            if (jm == -2 && (inner >= 2 || 4 * __popcll(bm) > p)) {
+                // a warm passive set far from this trial's (more than a quarter of it not
+                // positive, or a third step back): start from x = 0 instead of stepping back
+                // one column at a time
+                pj = false;
+                xj = 0.0;
+                p = 0;
                 settled = true;
                 break;
             }
@@ -1572,8 +1620,9 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
             const double alpha = uniform_value(al);
             if (pj) xj += alpha * (z - xj);
             if (pj && !(xj > 0.0)) { pj = false; xj = 0.0; }
-            p = compact(p);
-            if (!factor(p)) { fail = 1; break; }
+            int r0 = 0;
+            p = compact(p, r0);
+            if (!factor(p, r0)) { fail = 1; break; }
         }
         if (fail || !settled) break;
         if (jm >= 0 && ln == jm && !pj) kj = true;       // could not stay positive: keep it out
@@ -1717,9 +1766,8 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
     bool converged = false;
     const int outer_cap = P.nnls_cap > 0 ? P.nnls_cap : 3 * m + 10;
     if (in_lds && m <= 64) {
-        // one wavefront, no workgroup barrier inside (plist in idx, pos in cs, the factor's
-        // inverse diagonal in tmp)
-        if (tid < 64) nnls_wave(H, b, x, pas, blk, m, tol, outer_cap, Lf, tmp, idx, cs, s_int);
+        // one wavefront, no workgroup barrier inside (plist in idx, pos in cs)
+        if (tid < 64) nnls_wave(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
         __syncthreads();
         converged = s_int[3] != 0;
     } else {

@@ -1445,82 +1445,137 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 __device__ __forceinline__ int tri_row(int i) { return i * (i + 1) / 2; }
 
-// Lawson and Hanson's active set for m <= 64 equilibrated columns, run by the 64 lanes of
-// one wavefront (lane j: variable j's x, b, passive and blocked flags in registers) with
-// no workgroup barrier.  The passive set keeps its order of entry; its Cholesky factor
-// (packed rows in L, plist[position] = variable, pos[variable] = position, the inverse of
-// diagonal c in a register of lane c) grows by one row when a column enters and is formed
-// again, row by row, when columns leave.  The triangular solves take eight columns per
-// batch of LDS loads, so only the readlane / FMA chain is sequential.  The steps are those
-// of the workgroup form: the largest gradient entry above tol enters (the first on ties),
-// the step back to the boundary when the passive solution is not positive, a column that
-// cannot stay is blocked; a warm start more than a quarter of whose passive set comes out
-// not positive, or that needs a third step back, gives way to x = 0 (the optimum is the
-// same, reached in fewer steps).
-// In: x (warm start), pas, blk; out: x, pas, s_int[2] = a block
-// was not positive definite, s_int[3] = converged.  Every array is in LDS (the
-// wavefront's own accesses to it are in order).
+// Lawson and Hanson's active set for m <= 64 S equilibrated columns, run by the 64 lanes
+// of one wavefront (lane l, slot s: variable and passive position l + 64 s; x, b, the
+// passive and blocked flags and the factor's inverse diagonal in registers) with no
+// workgroup barrier.  The passive set keeps its order of entry; its Cholesky factor (packed
+// rows in L, plist[position] = variable, pos[variable] = position) grows by one row when a
+// column enters and is formed again from the first changed row when columns leave.  The
+// triangular solves take a batch of columns per round of loads, so only the readlane /
+// FMA chain is sequential.  The steps are those of the workgroup form: the largest
+// gradient entry above tol enters (the first on ties), the step back to the boundary when
+// the passive solution is not positive, a column that cannot stay is blocked; a warm start
+// more than a quarter of whose passive set comes out not positive, or that needs a third
+// step back, gives way to x = 0 (the optimum is the same, reached in fewer steps).
+// In: x (warm start), pas, blk; out: x, pas, s_int[2] = a block was not positive definite,
+// s_int[3] = converged.  The arrays are in LDS, or for more columns than it holds in the
+// trial's global workspace; either way only this wavefront touches them here, and its
+// memory accesses are in order.
+template <int S>
 __device__ __forceinline__ void nnls_wave(const double *H, const double *b, double *x, int *pas, int *blk, int m, double tol,
                                           int outer_cap, double *L, int *plist, int *pos, int *s_int) {
+    constexpr int B = S == 1 ? 8 : S == 2 ? 2 : 1;      // columns per batch of loads
     const int ln = threadIdx.x & 63;
-    const bool act = ln < m;
-    double xj = act ? x[ln] : 0.0;
-    const double bj = act ? b[ln] : 0.0;
-    bool pj = act && pas[ln] != 0, kj = act && blk[ln] != 0;
-    double dinv = 0.0;                                   // lane c: 1 / L[c][c]
+    const unsigned long long below = (1ull << ln) - 1ull;
+    double xj[S], bj[S], dinv[S];
+    bool pj[S], kj[S], act[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int j = ln + 64 * s;
+        act[s] = j < m;
+        xj[s] = act[s] ? x[j] : 0.0;
+        bj[s] = act[s] ? b[j] : 0.0;
+        pj[s] = act[s] && pas[j] != 0;
+        kj[s] = act[s] && blk[j] != 0;
+        dinv[s] = 0.0;
+    }
     int fail = 0, conv = 0;
-    // L y = v over positions < p (lane i holds v_i, gets y_i)
-    auto forward = [&](double v, int p) -> double {
-        for (int c0 = 0; c0 < p; c0 += 8) {
-            double lr[8];
+    // the value a register array holds for index c (uniform): lane c & 63, slot c >> 6
+    auto at = [&](const double (&v)[S], int c) -> double {
+        double r = lane_value(v[0], c & 63);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int c = c0 + u;
-                const bool use = c < p && ln > c && ln < p;
-                lr[u] = L[use ? tri_row(ln) + c : 0];
-                if (!use) lr[u] = 0.0;
-            }
+        for (int s = 1; s < S; ++s)
+            if ((c >> 6) == s) r = lane_value(v[s], c & 63);
+        return r;
+    };
+    // the value for index idx (per lane); every lane takes part in every shuffle
+    auto fetch = [&](const double (&v)[S], int idx) -> double {
+        double r = 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+        for (int s = 0; s < S; ++s) {
+            const double t = lane_fetch(v[s], idx & 63);
+            if ((idx >> 6) == s) r = t;
+        }
+        return r;
+    };
+    auto fetch_flag = [&](const bool (&f)[S], int idx) -> bool {
+        bool r = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int t = __shfl(f[s] ? 1 : 0, idx & 63);
+            if ((idx >> 6) == s) r = t != 0;
+        }
+        return r;
+    };
+    // L y = v over positions < p
+    auto forward = [&](double (&v)[S], int p) {
+        for (int c0 = 0; c0 < p; c0 += B) {
+            double lr[B][S];
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int c = c0 + u, ps = ln + 64 * s;
+                    const bool use = c < p && ps > c && ps < p;
+                    lr[u][s] = L[use ? tri_row(ps) + c : 0];
+                    if (!use) lr[u][s] = 0.0;
+                }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
                 const int c = c0 + u;
                 if (c < p) {
-                    const double yc = lane_value(v, c) * lane_value(dinv, c);
-                    v = ln == c ? yc : v - lr[u] * yc;
+                    const double yc = at(v, c) * at(dinv, c);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) v[s] = ln + 64 * s == c ? yc : v[s] - lr[u][s] * yc;
                 }
             }
         }
-        return v;
     };
     // L' z = v over positions < p
-    auto backward = [&](double v, int p) -> double {
-        for (int c1 = p - 1; c1 >= 0; c1 -= 8) {
-            double lr[8];
+    auto backward = [&](double (&v)[S], int p) {
+        for (int c1 = p - 1; c1 >= 0; c1 -= B) {
+            double lr[B][S];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int c = c1 - u;
-                const bool use = c >= 0 && ln < c;
-                lr[u] = L[use ? tri_row(c) + ln : 0];
-                if (!use) lr[u] = 0.0;
-            }
+            for (int u = 0; u < B; ++u)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+                for (int s = 0; s < S; ++s) {
+                    const int c = c1 - u, ps = ln + 64 * s;
+                    const bool use = c >= 0 && ps < c;
+                    lr[u][s] = L[use ? tri_row(c) + ps : 0];
+                    if (!use) lr[u][s] = 0.0;
+                }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
                 const int c = c1 - u;
                 if (c >= 0) {
-                    const double zc = lane_value(v, c) * lane_value(dinv, c);
-                    v = ln == c ? zc : v - lr[u] * zc;
+                    const double zc = at(v, c) * at(dinv, c);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) v[s] = ln + 64 * s == c ? zc : v[s] - lr[u][s] * zc;
                 }
             }
         }
-        return v;
     };
     // variable q enters at position p: the new row solves L y = H[passive, q]
     auto append = [&](int q, int p) -> bool {
-        const double h = forward(ln < p ? H[tri_at(plist[ln], q)] : 0.0, p);
-        const double d = uniform_value(H[tri_at(q, q)] - wave_sum(ln < p ? h * h : 0.0));
+        double h[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ps = ln + 64 * s;
+            h[s] = ps < p ? H[tri_at(plist[ps], q)] : 0.0;
+        }
+        forward(h, p);
+        double ss = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) if (ln + 64 * s < p) ss += h[s] * h[s];
+        const double d = uniform_value(H[tri_at(q, q)] - wave_sum(ss));
         if (!(d > 0.0)) return false;
         const double sd = sqrt(d);
-        if (ln < p) L[tri_row(p) + ln] = h;
-        if (ln == p) dinv = 1.0 / sd;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ps = ln + 64 * s;
+            if (ps < p) L[tri_row(p) + ps] = h[s];
+            if (ps == p) dinv[s] = 1.0 / sd;
+        }
         if (ln == 0) { L[tri_row(p) + p] = sd; plist[p] = q; pos[q] = p; }
         wave_lds_sync();
         return true;
@@ -1534,100 +1589,147 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
         }
         return true;
     };
-    // the passive solution L L' z = b_P, returned in variable space (z of lane j's variable)
-    auto solve = [&](int p) -> double {
-        double y = lane_fetch(bj, ln < p ? plist[ln] : 0);
-        y = backward(forward(y, p), p);
-        return lane_fetch(y, pj ? pos[ln] : 0);
+    // the passive solution L L' z = b_P, in variable space (z of each slot's variable)
+    auto solve = [&](int p, double (&z)[S]) {
+        double y[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ps = ln + 64 * s;
+            y[s] = fetch(bj, ps < p ? plist[ps] : 0);
+        }
+        forward(y, p);
+        backward(y, p);
+#pragma unroll
+        for (int s = 0; s < S; ++s) z[s] = fetch(y, pj[s] ? pos[ln + 64 * s] : 0);
     };
     // drop the variables that left the passive set from the list, keeping the order; r0:
     // the first position that changed
     auto compact = [&](int p, int &r0) -> int {
-        const int vi = ln < p ? plist[ln] : 0;
-        // every lane takes part in the shuffle: a lane outside it would read zero from it
-        const int pv = __shfl(pj ? 1 : 0, vi);
-        const bool keep = ln < p && pv != 0;
-        const unsigned long long km = __ballot(keep);
-        const unsigned long long gone = __ballot(ln < p && !keep);
-        r0 = gone ? __ffsll((long long)gone) - 1 : p;
-        const int newpos = __popcll(km & ((1ull << ln) - 1ull));
+        int vi[S];
+        bool keep[S];
+        unsigned long long km[S];
+        int nkeep = 0;
+        r0 = p;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ps = ln + 64 * s;
+            vi[s] = ps < p ? plist[ps] : 0;
+            const bool still = fetch_flag(pj, vi[s]);   // every lane takes part in its shuffles
+            keep[s] = ps < p && still;
+            km[s] = __ballot(keep[s]);
+            const unsigned long long gone = __ballot(ps < p && !keep[s]);
+            if (gone && r0 == p) r0 = 64 * s + __ffsll((long long)gone) - 1;
+        }
         wave_lds_sync();
-        if (keep) { plist[newpos] = vi; pos[vi] = newpos; }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int newpos = nkeep + __popcll(km[s] & below);
+            if (keep[s]) { plist[newpos] = vi[s]; pos[vi[s]] = newpos; }
+            nkeep += __popcll(km[s]);
+        }
         wave_lds_sync();
-        return __popcll(km);
+        return nkeep;
     };
     // the warm passive set in index order
-    const unsigned long long pm = __ballot(pj);
-    int p = __popcll(pm);
-    if (pj) { const int ps = __popcll(pm & ((1ull << ln) - 1ull)); plist[ps] = ln; pos[ln] = ps; }
+    int p = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const unsigned long long pm = __ballot(pj[s]);
+        if (pj[s]) { const int ps = p + __popcll(pm & below); plist[ps] = ln + 64 * s; pos[ln + 64 * s] = ps; }
+        p += __popcll(pm);
+    }
     wave_lds_sync();
     bool warm = p > 0;
     if (warm && !factor(p, 0)) fail = 1;
     for (int it = 0; !fail && it < outer_cap; ++it) {
         int jm = -2;
         if (!warm) {
-            double gj = bj;
-            for (int k0 = 0; k0 < m; k0 += 8) {
-                double hr[8];
+            double gj[S];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const bool use = act && k0 + u < m;
-                    hr[u] = H[use ? tri_at(ln, k0 + u) : 0];
-                }
+            for (int s = 0; s < S; ++s) gj[s] = bj[s];
+            for (int k0 = 0; k0 < m; k0 += B) {
+                double hr[B][S];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const double xk = lane_value(xj, k0 + u < m ? k0 + u : 0);
-                    if (act && k0 + u < m) gj -= hr[u] * xk;
+                for (int u = 0; u < B; ++u)
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const bool use = act[s] && k0 + u < m;
+                        hr[u][s] = H[use ? tri_at(ln + 64 * s, k0 + u) : 0];
+                    }
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const double xk = at(xj, k0 + u < m ? k0 + u : 0);
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                        if (act[s] && k0 + u < m) gj[s] -= hr[u][s] * xk;
                 }
             }
-            const bool cand = act && !pj && !kj && gj > tol;
-            double bv = cand ? gj : -1.0;
-            int bi = cand ? ln : 64;
+            // the largest candidate, the first on ties: each lane's slots in index order,
+            // then the wavefront
+            double bv = -1.0;
+            int bi = 1 << 20;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if (act[s] && !pj[s] && !kj[s] && gj[s] > tol && gj[s] > bv) { bv = gj[s]; bi = ln + 64 * s; }
             for (int o = 32; o > 0; o >>= 1) {
                 const double ov = __shfl_xor(bv, o);
                 const int oi = __shfl_xor(bi, o);
                 if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
             }
             jm = __builtin_amdgcn_readfirstlane(bi);
-            if (jm >= 64) { conv = 1; break; }
-            if (ln == jm) pj = true;
+            if (jm >= m) { conv = 1; break; }
+#pragma unroll
+            for (int s = 0; s < S; ++s) if (ln + 64 * s == jm) pj[s] = true;
             if (!append(jm, p)) { fail = 1; break; }
             ++p;
         }
         warm = false;
         bool settled = false;
         for (int inner = 0; inner < 3 * m + 10; ++inner) {
-            const double z = solve(p);
-            const bool bad = pj && !(z > 0.0);
-            const unsigned long long bm = __ballot(bad);
-            if (!bm) {
-                xj = pj ? z : 0.0;
+            double z[S];
+            solve(p, z);
+            bool bad[S];
+            int nbad = 0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                bad[s] = pj[s] && !(z[s] > 0.0);
+                nbad += __popcll(__ballot(bad[s]));
+            }
+            if (!nbad) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) xj[s] = pj[s] ? z[s] : 0.0;
                 settled = true;
                 break;
             }
-            if (jm == -2 && (inner >= 2 || 4 * __popcll(bm) > p)) {
-                // a warm passive set far from this trial's (more than a quarter of it not
-                // positive, or a third step back): start from x = 0 instead of stepping back
-                // one column at a time
-                pj = false;
-                xj = 0.0;
+            if (jm == -2 && (inner >= 2 || 4 * nbad > p)) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) { pj[s] = false; xj[s] = 0.0; }
                 p = 0;
                 settled = true;
                 break;
             }
-            double al = bad ? xj / (xj - z) : 2.0;
+            double al = 2.0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) if (bad[s]) al = fmin(al, xj[s] / (xj[s] - z[s]));
             for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o));
             const double alpha = uniform_value(al);
-            if (pj) xj += alpha * (z - xj);
-            if (pj && !(xj > 0.0)) { pj = false; xj = 0.0; }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (pj[s]) xj[s] += alpha * (z[s] - xj[s]);
+                if (pj[s] && !(xj[s] > 0.0)) { pj[s] = false; xj[s] = 0.0; }
+            }
             int r0 = 0;
             p = compact(p, r0);
             if (!factor(p, r0)) { fail = 1; break; }
         }
         if (fail || !settled) break;
-        if (jm >= 0 && ln == jm && !pj) kj = true;       // could not stay positive: keep it out
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (jm >= 0 && ln + 64 * s == jm && !pj[s]) kj[s] = true;   // could not stay positive: keep it out
     }
-    if (act) { x[ln] = xj; pas[ln] = pj ? 1 : 0; blk[ln] = kj ? 1 : 0; }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (act[s]) { const int j = ln + 64 * s; x[j] = xj[s]; pas[j] = pj[s] ? 1 : 0; blk[j] = kj[s] ? 1 : 0; }
     if (ln == 0) { s_int[2] = fail; s_int[3] = conv; }
 }
 
@@ -1765,9 +1867,18 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
     // (js/pt_gpu.js:363 solved(): its chi^2 is not used and the state's unsolved flag is set)
     bool converged = false;
     const int outer_cap = P.nnls_cap > 0 ? P.nnls_cap : 3 * m + 10;
-    if (in_lds && m <= 64) {
+    if (m <= 256) {
         // one wavefront, no workgroup barrier inside (plist in idx, pos in cs)
-        if (tid < 64) nnls_wave(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+        // (in LDS at most PTK_MAX_FREE_LIVE = 120 columns, beyond in the workspace)
+        if (tid < 64) {
+            if (in_lds) {
+                if (m <= 64) nnls_wave<1>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+                else nnls_wave<2>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+            } else {
+                if (m <= 128) nnls_wave<2>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+                else nnls_wave<4>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+            }
+        }
         __syncthreads();
         converged = s_int[3] != 0;
     } else {

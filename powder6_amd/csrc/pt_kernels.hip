@@ -1344,13 +1344,16 @@ __device__ inline double trial_cost(const TrialImage *im, const double2 *partial
 // against the residual the accumulate stage left (y_obs - background - Rietveld phases),
 // then min over I >= 0 of I'HI - 2 b'I by Lawson and Hanson's active set (Solving Least
 // Squares Problems, 1974, ch. 23) on columns equilibrated to unit diagonal
-// (js/rietveld_core.js:1078-1082), all in FP64 in LDS.  chi^2 = c - 2 b'I + I'HI replaces
-// the trial's partial sums (js/pt_gpu.js:259-297 solves the same problem per chain).
+// (js/rietveld_core.js:1078-1082), all in FP64: the normal equations by the whole
+// workgroup from columns cached in LDS, the active set by one wavefront (nnls_wave), in LDS
+// up to 120 live columns and in the trial's global workspace up to 256.  chi^2 =
+// c - 2 b'I + I'HI replaces the trial's partial sums (js/pt_gpu.js:259-297 solves the same
+// problem per chain).
 // the solve's arrays for up to c live columns: line records and spans, the packed normal
-// matrix and its factor, six FP64 and six int vectors
+// matrix and its factor, three FP64 and six int vectors
 __host__ __device__ inline size_t nnls_region_bytes(int c) {
     const size_t tri = (size_t)c * (c + 1) / 2;
-    return 16 * (size_t)(2 * c) * 4 + ((tri * 2 + 6 * (size_t)c) * 8 + 15) / 16 * 16 + (size_t)c * 4 * 6;
+    return 16 * (size_t)(2 * c) * 4 + ((tri * 2 + 3 * (size_t)c) * 8 + 15) / 16 * 16 + (size_t)c * 4 * 6;
 }
 // LDS for up to m live columns out of nf free reflections (the region, then nf flags and
 // nf envelope indices)
@@ -1373,7 +1376,7 @@ __host__ __device__ inline size_t nnls_total_bytes(int nf) { return nnls_base_by
 struct NnlsArrays {
     LineRec *R;
     int4 *Sp;
-    double *H, *Lf, *b, *x, *z, *g, *nrm, *tmp;
+    double *H, *Lf, *b, *x, *nrm;
     int *pas, *blk, *idx, *cs, *ce, *live;
 };
 // the arrays for capacity c carved from `base` (16-byte aligned), in nnls_region_bytes' order
@@ -1383,8 +1386,8 @@ __device__ __forceinline__ NnlsArrays nnls_carve(unsigned char *base, int c) {
     A.Sp = reinterpret_cast<int4 *>(A.R + 2 * c);
     const int Tc = c * (c + 1) / 2;
     A.H = reinterpret_cast<double *>(A.Sp + 2 * c);
-    A.Lf = A.H + Tc; A.b = A.Lf + Tc; A.x = A.b + c; A.z = A.x + c; A.g = A.z + c; A.nrm = A.g + c; A.tmp = A.nrm + c;
-    A.pas = reinterpret_cast<int *>(A.tmp + c);
+    A.Lf = A.H + Tc; A.b = A.Lf + Tc; A.x = A.b + c; A.nrm = A.x + c;
+    A.pas = reinterpret_cast<int *>(A.nrm + c);
     A.blk = A.pas + c; A.idx = A.blk + c; A.cs = A.idx + c; A.ce = A.cs + c; A.live = A.ce + c;
     return A;
 }
@@ -1742,7 +1745,7 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
     const size_t base = (size_t)t * 2 * P.nr;
     LineRec *R = A.R;
     int4 *Sp = A.Sp;
-    double *H = A.H, *Lf = A.Lf, *b = A.b, *x = A.x, *z = A.z, *g = A.g, *nrm = A.nrm, *tmp = A.tmp;
+    double *H = A.H, *Lf = A.Lf, *b = A.b, *x = A.x, *nrm = A.nrm;
     int *pas = A.pas, *blk = A.blk, *idx = A.idx, *cs = A.cs, *ce = A.ce, *live = A.live;
     for (int gi = tid; gi < nf; gi += nt)               // in envelope order
         if (flag[gi]) live[flag[gi] - 1] = gi;
@@ -1865,113 +1868,20 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
     // converged: the gradient has no positive entry off the passive set.  The iteration
     // caps, or a passive block that is not positive definite, leave the trial unsolved
     // (js/pt_gpu.js:363 solved(): its chi^2 is not used and the state's unsolved flag is set)
-    bool converged = false;
     const int outer_cap = P.nnls_cap > 0 ? P.nnls_cap : 3 * m + 10;
-    if (m <= 256) {
-        // one wavefront, no workgroup barrier inside (plist in idx, pos in cs)
-        // (in LDS at most PTK_MAX_FREE_LIVE = 120 columns, beyond in the workspace)
-        if (tid < 64) {
-            if (in_lds) {
-                if (m <= 64) nnls_wave<1>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
-                else nnls_wave<2>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
-            } else {
-                if (m <= 128) nnls_wave<2>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
-                else nnls_wave<4>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
-            }
-        }
-        __syncthreads();
-        converged = s_int[3] != 0;
-    } else {
-        if (tid == 0) {
-            int any = 0;
-            for (int j = 0; j < m; ++j) any |= pas[j];
-            s_int[0] = any ? -2 : -1;                        // -2: first solve on the warm passive set
-        }
-        __syncthreads();
-        bool warm = s_int[0] == -2;
-        for (int it = 0; it < outer_cap; ++it) {
-            if (!warm) {
-                for (int j = tid; j < m; j += nt) {
-                    double a = b[j];
-                    for (int k = 0; k < m; ++k) a -= H[tri_at(j, k)] * x[k];
-                    g[j] = a;
-                }
-                __syncthreads();
-                if (tid == 0) {
-                    int jm = -1;
-                    for (int j = 0; j < m; ++j)
-                        if (!pas[j] && !blk[j] && g[j] > tol && (jm < 0 || g[j] > g[jm])) jm = j;
-                    s_int[0] = jm;
-                    if (jm >= 0) pas[jm] = 1;
-                }
-                __syncthreads();
-            }
-            const int jm = warm ? -2 : s_int[0];
-            warm = false;
-            if (jm == -1) { converged = true; break; }
-            bool settled = false;
-            for (int inner = 0; inner < 3 * m + 10; ++inner) {
-                if (tid == 0) {
-                    int p = 0;
-                    for (int j = 0; j < m; ++j) if (pas[j]) idx[p++] = j;
-                    s_int[1] = p;
-                }
-                __syncthreads();
-                const int p = s_int[1];
-                // Cholesky of the passive block, one column at a time (rows in parallel)
-                for (int kc = 0; kc < p; ++kc) {
-                    if (tid == 0) {
-                        double a = H[tri_at(idx[kc], idx[kc])];
-                        for (int q = 0; q < kc; ++q) a -= Lf[tri_at(kc, q)] * Lf[tri_at(kc, q)];
-                        if (!(a > 0.0)) { s_int[2] = 1; a = 1.0; }
-                        Lf[tri_at(kc, kc)] = sqrt(a);
-                    }
-                    __syncthreads();
-                    for (int i = kc + 1 + tid; i < p; i += nt) {
-                        double a = H[tri_at(idx[i], idx[kc])];
-                        for (int q = 0; q < kc; ++q) a -= Lf[tri_at(i, q)] * Lf[tri_at(kc, q)];
-                        Lf[tri_at(i, kc)] = a / Lf[tri_at(kc, kc)];
-                    }
-                    __syncthreads();
-                }
-                if (tid == 0) {
-                    int stop = s_int[2];
-                    if (!stop) {
-                        for (int i = 0; i < p; ++i) {
-                            double a = b[idx[i]];
-                            for (int q = 0; q < i; ++q) a -= Lf[tri_at(i, q)] * tmp[q];
-                            tmp[i] = a / Lf[tri_at(i, i)];
-                        }
-                        for (int i = p - 1; i >= 0; --i) {
-                            double a = tmp[i];
-                            for (int q = i + 1; q < p; ++q) a -= Lf[tri_at(q, i)] * tmp[q];
-                            tmp[i] = a / Lf[tri_at(i, i)];
-                        }
-                        for (int j = 0; j < m; ++j) z[j] = 0.0;
-                        for (int i = 0; i < p; ++i) z[idx[i]] = tmp[i];
-                        bool allpos = true;
-                        for (int j = 0; j < m; ++j) if (pas[j] && !(z[j] > 0.0)) { allpos = false; break; }
-                        if (allpos) {
-                            for (int j = 0; j < m; ++j) x[j] = z[j];
-                            stop = 1;
-                        } else {
-                            double alpha = 2.0;
-                            for (int j = 0; j < m; ++j)
-                                if (pas[j] && !(z[j] > 0.0)) alpha = fmin(alpha, x[j] / (x[j] - z[j]));
-                            for (int j = 0; j < m; ++j) if (pas[j]) x[j] += alpha * (z[j] - x[j]);
-                            for (int j = 0; j < m; ++j) if (pas[j] && !(x[j] > 0.0)) { pas[j] = 0; x[j] = 0.0; }
-                        }
-                    }
-                    s_int[3] = stop;
-                }
-                __syncthreads();
-                if (s_int[3]) { settled = true; break; }
-            }
-            if (s_int[2] || !settled) break;                 // a block that is not positive definite, or the cap
-            if (tid == 0 && jm >= 0 && !pas[jm]) blk[jm] = 1;   // could not stay positive: keep it out
-            __syncthreads();
+    // one wavefront, no workgroup barrier inside (plist in idx, pos in cs); in LDS at most
+    // PTK_MAX_FREE_LIVE = 120 columns, beyond (up to PTK_MAX_FREE_WS_LIVE = 256) in the workspace
+    if (tid < 64) {
+        if (in_lds) {
+            if (m <= 64) nnls_wave<1>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+            else nnls_wave<2>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+        } else {
+            if (m <= 128) nnls_wave<2>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
+            else nnls_wave<4>(H, b, x, pas, blk, m, tol, outer_cap, Lf, idx, cs, s_int);
         }
     }
+    __syncthreads();
+    const bool converged = s_int[3] != 0;
     for (int gi = tid; gi < nf; gi += nt) if (!flag[gi]) fx[gi] = 0.0;
     for (int j = tid; j < m; j += nt) fx[live[j]] = pas[j] && nrm[j] > 0.0 ? x[j] / nrm[j] : 0.0;
     // chi^2 at the solution: c - sum_j x_j (2 b_j - (H x)_j), in the equilibrated frame

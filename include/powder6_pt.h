@@ -349,7 +349,11 @@ typedef struct {
     float skip_eps;      /* skip the exact evaluation of a screened chain's pick when its
                             predicted Metropolis acceptance is below skip_eps (0: never) */
     int train_every;     /* train (train_iters Adam steps) every train_every PT steps;
-                            0 or 1: every step                                          */
+                            0 or 1: every step.  With oversample 1 and train_every at
+                            least the steps of a sweep, the network trains once per sweep
+                            (train_iters steps, before its PT steps) and the skip test runs
+                            inside the PT step itself, whose chains keep running as
+                            independent parts on their own streams                       */
 } pt_screen_config;
 int pt_screen(pt_plan *plan, const pt_screen_config *cfg);
 /* out[0] Adam steps taken, out[1] ring entries filled, out[2] running mean of the

@@ -260,8 +260,10 @@ struct pt_plan {
     // surrogate-screened PT
     pt_mlp *mlp = nullptr;
     bool screen_on = false;
+    bool screen_fused = false;           // the skip test runs inside the PT setup (one proposal per chain)
     pt_screen_config scfg{};
     ptk::ScreenDev S{};
+    ptk::ScreenFused *d_sf = nullptr;
     int screen_cap = 0;
     std::vector<double> tth_host;
     int n_bkg = 0;
@@ -1362,7 +1364,17 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
 }
 
 static int enqueue_sweep(pt_plan *pl, hipEvent_t *ev) {
-    if (pl->screen_on) return enqueue_screened_sweep(pl, ev);
+    if (pl->screen_on && !pl->screen_fused) return enqueue_screened_sweep(pl, ev);
+    if (pl->screen_on) {
+        // the network's training steps of this sweep, before its PT steps read the weights
+        const pt_screen_config &c = pl->scfg;
+        const double *step_ptr = pl->P.state + 3 * pl->nv + PT_S_STEP;
+        for (int it = 0; it < c.train_iters; ++it) {
+            int rc = mlpx::train_iter(pl->mlp, pl->S.ring_x, pl->S.ring_t, pl->screen_cap, step_ptr, pl->chains, 0, c.batch,
+                                      c.seed, c.batch, pl->stream);
+            if (rc) return rc;
+        }
+    }
     // chains take no part in each other's steps until the exchange, so the parts run
     // independently between a fork and a join (the timed form runs as one part)
     const int parts = ev ? 1 : pl->parts;
@@ -1397,6 +1409,8 @@ int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
     drop_graph(pl->gexec, pl->graph);
     if (!cfg) {
         pl->screen_on = false;
+        pl->screen_fused = false;
+        pl->P.sf = nullptr;
         return 0;
     }
     const pt_screen_config c = *cfg;
@@ -1453,6 +1467,21 @@ int pt_screen(pt_plan *pl, const pt_screen_config *cfg) {
     pl->S.cold_chains = std::max(1, (int)std::lround(c.cold_fraction * pl->chains));
     pl->scfg = c;
     pl->screen_on = true;
+    // one proposal per chain and training between sweeps: the skip test runs inside the PT
+    // setup and the sweep keeps its parts on their streams
+    pl->screen_fused = c.oversample == 1 && c.train_every >= pl->steps_per_sweep;
+    pl->P.sf = nullptr;
+    if (pl->screen_fused) {
+        ptk::ScreenFused F{};
+        F.w = pl->mlp->wb; F.p = pl->mlp->p; F.iter = pl->mlp->iter;
+        F.center = pl->S.center; F.inv_half = pl->S.inv_half;
+        F.ring_x = pl->S.ring_x; F.ring_t = pl->S.ring_t; F.skipped = pl->S.skipped;
+        F.log_wss = pl->S.log_wss; F.skip_margin = pl->S.skip_margin; F.skip_log = pl->S.skip_log;
+        F.cold_chains = pl->S.cold_chains; F.cap = pl->S.cap; F.warmup_iters = pl->S.warmup_iters;
+        if (!pl->d_sf && (rc = dalloc(pl, &pl->d_sf, 1))) return rc;
+        HIPCHECK(hipMemcpy(pl->d_sf, &F, sizeof F, hipMemcpyHostToDevice), "screen setup");
+        pl->P.sf = pl->d_sf;
+    }
     return 0;
 }
 

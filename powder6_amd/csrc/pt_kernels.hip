@@ -20,6 +20,7 @@
 // profile matrix on MFMA; k_pattern / k_workload: one vector's pattern and work counts.
 #include "pt_device.h"
 #include "pt_launch.h"
+#include "pt_screen_fused.h"
 #include <algorithm>
 #include <mutex>
 
@@ -39,6 +40,7 @@ struct SetupShared {
     float red_s[32], red_c[32];
     int phase_ok[PTK_MAX_PHASES];
     int cell_ok, prof_ok, go, accept;
+    int skipped;                              // the surrogate screen marked the proposal not evaluated
     double chi2;
 };
 
@@ -722,7 +724,7 @@ struct TrialImage {
     SepPhase sep[PTK_MAX_PHASES];
     float bkg[PTK_MAX_BKG_F];
     int phase_ok[PTK_MAX_PHASES];
-    int cell_ok, prof_ok, go, boot;
+    int cell_ok, prof_ok, go, boot, skipped;
     // per 512-point group: the first and last line index whose window meets the group,
     // reset by the setup and gathered by the lines stage (atomic min / max)
     int grange[PTK_MAX_GROUPS][2];
@@ -754,7 +756,10 @@ __device__ inline void image_store(const DevPlan &P, const SetupShared &sh, Tria
     }
     for (int i = tid; i < PTK_MAX_BKG_F; i += nt) img->bkg[i] = sh.bkg[i];
     if (tid < P.n_phases) img->phase_ok[tid] = sh.phase_ok[tid];
-    if (tid == 0) { img->cell_ok = sh.cell_ok; img->prof_ok = sh.prof_ok; img->go = sh.go; img->boot = sh.accept; }
+    if (tid == 0) {
+        img->cell_ok = sh.cell_ok; img->prof_ok = sh.prof_ok; img->go = sh.go; img->boot = sh.accept;
+        img->skipped = sh.skipped;
+    }
     for (int g = tid; g < (P.np + 511) >> 9; g += nt) { img->grange[g][0] = 0x7fffffff; img->grange[g][1] = -1; }
 }
 
@@ -819,6 +824,26 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
             st[3 * nv + 8] = log(u3);
         }
         __syncthreads();
+        if (P.sf && !boot && !pick) {
+            // the surrogate's skip test (one proposal per chain): the network's log Rwp at the
+            // proposal and at the chain's state, and the predicted Metropolis acceptance
+            const ScreenFused &F = *P.sf;
+            __shared__ float sf_u[2][256], sf_h[2][2][256];
+            const int i = threadIdx.x;                   // 64 threads: one input each
+            sf_u[0][i] = i < nv ? sf_from_bf16(sf_input(F, i, prop[i])) : 0.0f;
+            sf_u[1][i] = i < nv ? sf_from_bf16(sf_input(F, i, st[i])) : 0.0f;
+            __syncthreads();
+            float y0, y1;
+            sf_predict2(F, sf_u, sf_h, y0, y1);
+            if (threadIdx.x == 0 && F.skip_log > 0.0f && *F.iter >= F.warmup_iters && t < F.cold_chains) {
+                const float d = y0 - y1 - F.skip_margin;
+                const double cur = st[3 * nv];
+                if (d > 0.0f && cur > 0.0 && cur < 3e38 && cur * expm1(2.0 * (double)d) > 2.0 * P.temp[t] * (double)F.skip_log) {
+                    skip = true;
+                    atomicAdd(F.skipped, 1u);
+                }
+            }
+        }
         for (int j = threadIdx.x; j < nv; j += blockDim.x) st[nv + j] = prop[j];
         materialize(P, prop, sh);
     } else {
@@ -835,6 +860,7 @@ __device__ __forceinline__ void setup_body(const DevPlan &P, const double *X, Tr
         }
         sh.go = go ? 1 : 0;
         sh.accept = boot ? 1 : 0;
+        sh.skipped = skip ? 1 : 0;
     }
     __syncthreads();
     image_store<SEP>(P, sh, img + t);
@@ -865,6 +891,7 @@ __device__ __forceinline__ void follow_body(const DevPlan &P, const double *msta
         const bool go = mimg[t].go && sh.cell_ok && (boot || st[3 * P.nv + 11] < 0.5 || sh.prof_ok);
         sh.go = go ? 1 : 0;
         sh.accept = boot ? 1 : 0;
+        sh.skipped = 0;
     }
     __syncthreads();
     image_store<SEP>(P, sh, img + t);
@@ -2009,6 +2036,27 @@ __device__ inline int accept_chain(const DevPlan &P, const TrialImage *im, doubl
     return r;
 }
 
+// after the Metropolis step of chain t (surrogate screen on): the trial just evaluated (the
+// proposal, its exact chi^2) goes into ring slot ((step - 1) chains + t) mod cap with target
+// log Rwp = (log chi^2 - log sum w y^2) / 2, NaN (left out of training) for an infeasible
+// trial or a chain outside the screened ones; a proposal not evaluated is not recorded.
+// Inputs i = i0, i0 + di, ... of the row are written by the caller's thread.
+__device__ inline void screen_record_chain(const DevPlan &P, const TrialImage *im, int t, int i0, int di) {
+    if (!P.sf || im->boot || im->skipped) return;
+    const ScreenFused &F = *P.sf;
+    const int nv = P.nv;
+    const double *st = P.state + (size_t)t * P.stride;
+    const long long step = (long long)st[3 * nv + 2];
+    if (step < 1) return;
+    const size_t pos = (size_t)(((step - 1) * P.chains + t) % F.cap);
+    for (int i = i0; i < MLP_IN; i += di) F.ring_x[pos * MLP_IN + i] = i < nv ? sf_input(F, i, st[nv + i]) : (uint16_t)0;
+    if (i0 == 0) {
+        const double cost = st[3 * nv + 10];
+        const bool ok = t < F.cold_chains && st[3 * nv + 7] > 0.5 && cost > 0.0 && cost < 3e38;
+        F.ring_t[pos] = ok ? (float)(0.5 * (log(cost) - F.log_wss)) : __int_as_float(0x7fc00000);
+    }
+}
+
 // the end of one PT step for chain t: Metropolis accept, and on acceptance the proposal
 // becomes the chain's state (and its best when it is a new feasible best)
 __device__ inline void finish_chain(const DevPlan &P, const TrialImage *im, double cost, int prof_ok, int uns, int t) {
@@ -2035,6 +2083,7 @@ __global__ void __launch_bounds__(64) k_stage_finish(DevPlan P, const TrialImage
     // batch mode: NaN marks an unsolved trial for the host (a cost is otherwise finite)
     if (!pt_mode) { chi2[t] = uns ? __builtin_nan("") : cost; return; }
     finish_chain(P, img + t, cost, pok, uns, t);
+    screen_record_chain(P, img + t, t, 0, 1);
 }
 
 // the Metropolis step of chain t, then its next proposal and setup: the finish of one PT
@@ -2058,6 +2107,7 @@ __device__ __forceinline__ void finish_setup_body(const DevPlan &P, TrialImage *
             if (r == 2) st[2 * nv + j] = v;
         }
     }
+    screen_record_chain(P, img + t, t, threadIdx.x, blockDim.x);
     __syncthreads();
     setup_body<SEP>(P, nullptr, img, t, 1, 0, sh, prop);
 }

@@ -150,6 +150,26 @@ struct DevPlan {
     double *state;
     double *noeval;            // per chain: proposals rejected without an evaluation (invalid cell,
                                // infeasible move of a feasible chain, or skipped by the screen)
+    const struct ScreenFused *sf;   // the surrogate's skip test inside the PT setup (null: off)
+};
+
+// The surrogate screen as the PT stages run it when each chain has one proposal per step:
+// the setup predicts log Rwp at the proposal and at the chain's state with the network
+// (BF16 weights, FP32 sums) and may mark the proposal not evaluated; the finish records
+// every evaluated trial into the replay ring.  The network trains between sweeps only.
+struct ScreenFused {
+    const uint16_t *w;         // BF16 weights in the network's flat layout
+    const float *p;            // FP32 parameters (the biases)
+    const int *iter;           // Adam steps taken
+    const double *center, *inv_half;   // network input u_i = (x_i - center_i) inv_half_i
+    uint16_t *ring_x;          // replay ring: cap rows of 64 BF16 inputs
+    float *ring_t;             //   and their targets (log Rwp, NaN = infeasible)
+    unsigned *skipped;         // count of proposals not evaluated
+    double log_wss;            // log sum w y_obs^2
+    float skip_margin;         // log Rwp allowance taken off a prediction before a skip
+    float skip_log;            // skip when the predicted acceptance < exp(-skip_log); 0 off
+    int cold_chains;           // chains [0, cold_chains) are screened and recorded
+    int cap, warmup_iters, pad;
 };
 
 // a separated-profile phase of one trial, formed once in the setup (FP32)

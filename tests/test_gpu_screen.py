@@ -143,3 +143,54 @@ def test_skip_screening_saves_exact_evaluations_at_equal_quality(mol):
         if best < 1e38:
             ref = m.cost(st[c, 2 * nv:3 * nv])
             assert abs(best - ref) <= 1e-5 * max(1.0, ref), (c, best, ref)
+
+
+def test_in_step_skip_test_without_skips_is_plain_pt(mol):
+    """One proposal per chain and training once per sweep: the skip test runs inside the
+    PT setup and the sweep keeps its parts.  With skipping off the network still trains
+    (train_iters Adam steps before every sweep once the ring holds a minibatch: from the
+    second sweep on) but takes no part in the PT steps, so the states are plain PT's bit for
+    bit."""
+    plain, _ = _run(mol, 2)
+    fused, stats = _run(mol, 2, oversample=1, train_every=mol.nv, train_iters=3, warmup_iters=0, batch=128, capacity=8192,
+                        skip_eps=0.0)
+    assert stats['iterations'] == 3 and stats['skipped'] == 0
+    assert stats['ring'] == min(8192, 2 * mol.nv * mol.chains)   # every evaluated trial recorded
+    assert np.array_equal(plain, fused)
+
+
+def test_in_step_skip_test_skips_and_keeps_exact_costs(mol):
+    """The in-step skip test from a far start: after the warm-up some proposals are
+    rejected unevaluated (counted by the plan's trial counts), every step still counts, the
+    best Rwp after four sweeps stays within 2% of plain PT's and every recorded best is the
+    exact chi^2 of its vector."""
+    x0 = _far_start(mol, 5)
+    wss = float(np.sum(mol.w * mol.y ** 2))
+    sweeps, nv, C = 4, mol.nv, mol.chains
+    cfg = dict(oversample=1, train_every=nv, train_iters=12, warmup_iters=24, batch=256, capacity=16384, cold_fraction=1.0,
+               skip_eps=0.1, skip_margin=0.005)
+    with pt.Plan(mol) as pl:
+        pl.screen(**cfg)
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        st = pl.batch(sweeps).reshape(C, -1)
+        stats = pl.screen_stats()
+        evaluated, skipped = pl.trial_counts()
+    with pt.Plan(mol) as pl:
+        pl.initial([x0])
+        pl.batch(1, bootstrap=True)
+        plain = pl.batch(sweeps).reshape(C, -1)
+    rwp = lambda c: 100 * np.sqrt(c / wss)
+    skip_best, plain_best = float(np.min(st[:, 3 * nv + 1])), float(np.min(plain[:, 3 * nv + 1]))
+    print('in-step skip stats %s; evaluated %d, not evaluated %d; best Rwp plain %.3f%%, skipping %.3f%%' %
+          (stats, evaluated, skipped, rwp(plain_best), rwp(skip_best)))
+    assert stats['iterations'] == 12 * (sweeps - 1) and stats['picked'] == 0 and stats['skipped'] > 0
+    assert evaluated + skipped == sweeps * nv * C and skipped >= stats['skipped']
+    assert np.all(st[:, 3 * nv + 4] == sweeps * nv)
+    assert rwp(skip_best) <= 1.02 * rwp(plain_best)
+    m = O.Model(mol)
+    for c in range(0, C, 8):
+        best = st[c, 3 * nv + 1]
+        if best < 1e38:
+            ref = m.cost(st[c, 2 * nv:3 * nv])
+            assert abs(best - ref) <= 1e-5 * max(1.0, ref), (c, best, ref)

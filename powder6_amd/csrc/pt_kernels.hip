@@ -2608,11 +2608,16 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
                                                      const unsigned short *__restrict__ ilo,
                                                      const float *__restrict__ iscale, const int2 *__restrict__ band,
                                                      const float2 *__restrict__ yw, float *partial, int kpad, int tpad,
-                                                     double *yout, int ny) {
+                                                     int row_tiles, double *yout, int ny) {
     typedef Quad<BF16> Q;
     __shared__ __align__(16) unsigned short sB[2][2][64][kMxLd];   // [buffer][hi, lo][trial][k]
     __shared__ float red[4][64];
-    const int rt = blockIdx.x, j0 = blockIdx.y * 64;
+    // XCD-aware tiles: workgroups go to the eight XCDs round robin, and XCD x takes the row
+    // tiles x, x + 8, ... for every trial tile, so each XCD's L2 holds an eighth of P
+    const int xcd = blockIdx.x & 7, kx = blockIdx.x >> 3, ntt = tpad / 64;
+    const int nrt = (row_tiles - xcd + 7) / 8;
+    if (kx >= nrt * ntt) return;
+    const int rt = xcd + 8 * (kx / ntt), j0 = (kx % ntt) * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i0 = rt * 128 + wave * 32;
     const int2 b = band[rt];
@@ -2847,13 +2852,15 @@ hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const floa
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
                               float *partial, int kpad, int tpad, int row_tiles, double *yout, int ny, int bf16, hipStream_t s) {
-    // row_tiles: 128-point tiles
+    // row_tiles: 128-point tiles; the grid is eight XCD slots of ceil(row_tiles / 8) row
+    // tiles x the trial tiles
+    const dim3 grid(8 * ((row_tiles + 7) / 8) * (tpad / 64));
     if (bf16)
-        hipLaunchKernelGGL(k_matrix_chi2<true>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
-                           partial, kpad, tpad, yout, yout ? ny : 0);
+        hipLaunchKernelGGL(k_matrix_chi2<true>, grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad, tpad,
+                           row_tiles, yout, yout ? ny : 0);
     else
-        hipLaunchKernelGGL(k_matrix_chi2<false>, dim3(row_tiles, tpad / 64), dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw,
-                           partial, kpad, tpad, yout, yout ? ny : 0);
+        hipLaunchKernelGGL(k_matrix_chi2<false>, grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad, tpad,
+                           row_tiles, yout, yout ? ny : 0);
     return hipGetLastError();
 }
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,

@@ -33,6 +33,7 @@ PEAK_FP16_MFMA = 2.5e15   # MI355X dense FP16/BF16 MFMA, FLOP/s
 PEAK_FP64 = 78.6e12       # MI355X FP64 vector peak, FLOP/s
 PEAK_BF16_MFMA = 2.5e15
 MLP_FLOP_FWD = 2 * (64 * 256 + 2 * 256 * 256 + 256)   # one row through 64-256-256-256-1
+LEGS = ('nacl', 'matrix', 'uneven', 'learned', 'free', 'normal')
 
 
 def flop_per_trial(w):
@@ -386,6 +387,8 @@ def main():
                     help='accumulate tile shape (128-point tiles per wavefront); 0 lets the autotuner pick')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run a plain 1 GiB device copy (HBM counter calibration)')
+    ap.add_argument('--leg', choices=LEGS, default=None,
+                    help='run only this workload of the table and print its JSON (for counter passes under a profiler)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one MI355X: this build runs on a single GPU (replica sharding across '
@@ -394,6 +397,18 @@ def main():
 
     from powder6_amd import problems as PR
     from powder6_amd import pt
+
+    legs_fn = {'nacl': lambda: nacl_cpu_path(pt, PR, args.no_cpu), 'matrix': lambda: matrix_form(pt, PR),
+               'uneven': lambda: uneven_batch(pt, PR, args.no_cpu), 'learned': lambda: learned_screening(pt, PR, args.no_cpu),
+               'free': lambda: free_intensities(pt, PR), 'normal': lambda: normal_equations(pt, PR, args.no_cpu)}
+    if args.leg:
+        r = legs_fn[args.leg]()
+        if args.leg == 'matrix':
+            r = r[0]
+        if args.copy_probe:
+            r['hbm_copy_gbps'] = pt.hbm_copy_probe(1 << 30, 3)
+        print(json.dumps({'metric': METRIC, 'leg': args.leg, 'result': r}))
+        return
 
     pb = PR.pbso4_pnma(simulate=pt.gpu_simulate)          # 256 replicas x 64 trials/sweep
     x0 = pb.start_vector()
@@ -459,12 +474,9 @@ def main():
             'whole_step_frac': f_trial * evaluated / (t1 - t0) / PEAK_FP32}
 
     legs = {}
-    for name, fn in (('nacl', lambda: nacl_cpu_path(pt, PR, args.no_cpu)), ('matrix', lambda: matrix_form(pt, PR)),
-                     ('uneven', lambda: uneven_batch(pt, PR, args.no_cpu)),
-                     ('learned', lambda: learned_screening(pt, PR, args.no_cpu)),
-                     ('free', lambda: free_intensities(pt, PR)), ('normal', lambda: normal_equations(pt, PR, args.no_cpu))):
+    for name in LEGS:
         t = time.perf_counter()
-        legs[name] = fn()
+        legs[name] = legs_fn[name]()
         print('bench leg %s done in %.1f s' % (name, time.perf_counter() - t), file=sys.stderr, flush=True)
     nacl, uneven, learned, free, normal = legs['nacl'], legs['uneven'], legs['learned'], legs['free'], legs['normal']
     mat, mpb, mX, mchi, mchi_bf16 = legs['matrix']

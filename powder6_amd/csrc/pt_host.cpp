@@ -1761,8 +1761,13 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
     HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
     for (int p = 0; p < passes; ++p) {
         int t0 = p * chunk, n = std::min(chunk, ntrials - t0), tpad = (n + 63) / 64 * 64;
-        HIPCHECK(hipMemsetAsync(pl->d_ihi, 0, sizeof(unsigned short) * (size_t)tpad * pl->kpad, pl->stream), "clear");
-        HIPCHECK(hipMemsetAsync(pl->d_ilo, 0, sizeof(unsigned short) * (size_t)tpad * pl->kpad, pl->stream), "clear");
+        if (tpad > n) {
+            // the intensity kernels write every reflection slot of the n trials; the padding
+            // trials of the last product tile stay zero
+            const size_t off = (size_t)n * pl->kpad, cnt = sizeof(unsigned short) * (size_t)(tpad - n) * pl->kpad;
+            HIPCHECK(hipMemsetAsync(pl->d_ihi + off, 0, cnt, pl->stream), "clear");
+            HIPCHECK(hipMemsetAsync(pl->d_ilo + off, 0, cnt, pl->stream), "clear");
+        }
         HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x + (size_t)t0 * pl->nv, pl->d_img, n, 0, 0, pl->stream), "setup");
         HIPCHECK(hipEventRecord(pl->ev[4 * p], pl->stream), "timing");
         HIPCHECK(ptk::launch_matrix_intensity(pl->P, pl->d_img, pl->d_factor, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->kpad, 0, n,

@@ -2766,6 +2766,11 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     const int i0 = rt * 128 + wave * 32;
     const int2 b = band[rt];
     const int li = lane & 15, kq = 8 * (lane >> 4), dq = 4 * (lane >> 4);
+    float2 ow[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ow[rb][e] = yw[i0 + 16 * rb + dq + e];
     floatx4_t acc[2][4];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
@@ -2823,17 +2828,18 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int row = i0 + 16 * rb + dq + e;
-                if (row < P.np) {
-                    const float Y = acc[rb][nb][e] * sc;
-                    const float2 o = yw[row];
-                    if (col < ny) yout[(size_t)col * P.np + row] = (double)Y;   // the first ny trials' Y
-                    if (o.y > 0.0f) {
-                        const float r = o.x - Y;
-                        sum = fmaf(o.y * r, r, sum);
-                    }
-                }
+                const float r = ow[rb][e].x - acc[rb][nb][e] * sc;
+                sum = fmaf(ow[rb][e].y * r, r, sum);
             }
+        if (col < ny) {
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int row = i0 + 16 * rb + dq + e;
+                    if (row < P.np) yout[(size_t)col * P.np + row] = (double)(acc[rb][nb][e] * sc);
+                }
+        }
         sum += __shfl_xor(sum, 16);
         sum += __shfl_xor(sum, 32);
         if (lane < 16) red[wave][16 * nb + lane] = sum;
@@ -2845,15 +2851,20 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     }
 }
 
-// chi^2 of each trial: ordered FP64 sum of its row-tile partials
-__global__ void k_matrix_reduce(const float *__restrict__ partial, int row_tiles, int tpad, int ntrials,
-                                const TrialImage *__restrict__ img, double *chi2) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntrials) return;
+// chi^2 of each trial: FP64 sum of its row-tile partials, one wavefront per trial (lane l
+// takes the row tiles l, l + 64, ... in order, then a fixed shuffle tree), so every load
+// of a trial is in flight at once and the sum is deterministic
+__global__ void __launch_bounds__(256) k_matrix_reduce(const float *__restrict__ partial, int row_tiles, int tpad, int ntrials,
+                                                       const TrialImage *__restrict__ img, double *chi2) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= ntrials) return;                   // uniform over the wavefront
     double s = 0.0;
-    for (int r = 0; r < row_tiles; ++r) s += (double)partial[(size_t)r * tpad + t];
-    if (!img[t].go || !isfinite(s) || s > 3e38) s = 3e38;
-    chi2[t] = s;
+    for (int r = lane; r < row_tiles; r += 64) s += (double)partial[(size_t)r * tpad + t];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) {
+        if (!img[t].go || !isfinite(s) || s > 3e38) s = 3e38;
+        chi2[t] = s;
+    }
 }
 
 // ---- HBM calibration: a plain streaming copy of known size ------------------------------
@@ -3010,7 +3021,7 @@ hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const
 }
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,
                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_matrix_reduce, dim3((ntrials + 63) / 64), dim3(64), 0, s, partial, row_tiles, tpad, ntrials,
+    hipLaunchKernelGGL(k_matrix_reduce, dim3((ntrials + 3) / 4), dim3(256), 0, s, partial, row_tiles, tpad, ntrials,
                        (const TrialImage *)img, chi2);
     return hipGetLastError();
 }

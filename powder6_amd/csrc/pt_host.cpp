@@ -2267,7 +2267,7 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
         for (int q = 0; q < (pl->chains + 1) / 2; ++q) pair_tab.push_back(make_int2(m, q));
     }
     const std::vector<GroupTrial> placed = place_trials(g);
-    lines_tab = xcd_table(placed, [&](int m) { return ptk::lines_units(g->members[m]->nr); });
+    lines_tab = xcd_table(placed, [&](int m) { return (g->members[m]->nr + 255) / 256; });
     g->total_chains = (int)chain_tab.size();
     g->total_pairs = (int)pair_tab.size();
     g->n_lines = (int)lines_tab.size();
@@ -2313,7 +2313,7 @@ int pt_group_create(const pt_problem *problems, int count, int steps_per_sweep, 
                 for (int q = 0; q < (g->members[m]->chains + 1) / 2; ++q) pt.push_back(make_int2(m, q));
             }
             const std::vector<GroupTrial> pl_p = place_trials(g, use);
-            std::vector<int4> lt = xcd_table(pl_p, [&](int m) { return ptk::lines_units(g->members[m]->nr); });
+            std::vector<int4> lt = xcd_table(pl_p, [&](int m) { return (g->members[m]->nr + 255) / 256; });
             P.n_chains = (int)ct.size();
             P.n_pairs = (int)pt.size();
             P.n_lines = (int)lt.size();

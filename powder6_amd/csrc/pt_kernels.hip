@@ -391,62 +391,6 @@ __device__ __forceinline__ float structure_f2(const DevPlan &P, const Shared &sh
     return ph.centro ? structure_f2_impl<true>(P, sh, ph, h, k, l, s2) : structure_f2_impl<false>(P, sh, ph, h, k, l, s2);
 }
 
-// The structure-factor sums of one reflection over the representatives [r0, r1) of its
-// phase (the loop of structure_f2_impl, split so that a lane pair shares a reflection):
-// s[0..3] = A, B, C, D partial sums (B = D = 0 for CENTRO); |F|^2 is formed from the sum of
-// the two halves, A^2 + B^2 + C^2 + D^2, or 4 (A^2 + C^2) for CENTRO.
-template <bool CENTRO>
-__device__ __forceinline__ void structure_part(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k, int l,
-                                               float s2, int r0, int r1, float s[4]) {
-    float Ar = 0.f, Br = 0.f, Cr = 0.f, Dr = 0.f;
-    const float q4 = 0.25f * s2;
-    for (int j0 = 0; j0 < ph.n_atoms; j0 += 4) {
-        const int nj = min(4, ph.n_atoms - j0);
-        float ca[4] = {0.f, 0.f, 0.f, 0.f}, sa[4] = {0.f, 0.f, 0.f, 0.f};
-        double px[4], py[4], pz[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int ga = ph.atom_off + j0 + (a < nj ? a : 0);
-            px[a] = sh.apos[ga][0]; py[a] = sh.apos[ga][1]; pz[a] = sh.apos[ga][2];
-        }
-        for (int r = r0; r < r1; ++r) {
-            const int *R = sh.rep_r[ph.rep_off + r];
-            const double *t = sh.rep_t[ph.rep_off + r];
-            const double H0 = h * R[0] + k * R[3] + l * R[6], H1 = h * R[1] + k * R[4] + l * R[7], H2 = h * R[2] + k * R[5] + l * R[8];
-            const double ht = (double)h * t[0] + (double)k * t[1] + (double)l * t[2];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                if (a < nj) {
-                    const double phs = H0 * px[a] + H1 * py[a] + H2 * pz[a] + ht;
-                    const float fr = (float)(phs - floor(phs));
-                    ca[a] += __builtin_amdgcn_cosf(fr);
-                    if (!CENTRO) sa[a] += __builtin_amdgcn_sinf(fr);
-                }
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            if (a < nj) {
-                const int ga = ph.atom_off + j0 + a;
-                const float *F = sh.aff[ga];
-                float f0;
-                if (P.radiation == 1) f0 = sh.afp[ga][0];
-                else {
-                    f0 = F[10];
-                    for (int g = 0; g < 5; ++g) f0 += F[g] * __expf(-F[5 + g] * q4);
-                    f0 += sh.afp[ga][0];
-                }
-                const float dw = __expf(-sh.aB[ga] * q4);
-                const float wj = sh.aocc[ga] * f0 * dw * F[11];
-                const float wi = (P.radiation == 1 || !(sh.afp[ga][1] != 0.f)) ? 0.f : sh.aocc[ga] * sh.afp[ga][1] * dw * F[11];
-                Ar += wj * ca[a]; Cr += wi * ca[a];
-                if (!CENTRO) { Br += wj * sa[a]; Dr += wi * sa[a]; }
-            }
-        }
-    }
-    s[0] = Ar; s[1] = Br; s[2] = Cr; s[3] = Dr;
-}
-
 // March-Dollase factor averaged over the equivalents (rietveld_core.js:824-843)
 __device__ inline float march_dollase(const DevPlan &P, const DevPhase &ph, const double *f, double r, int kref) {
     if (!(r > 0.0) || fabs(r - 1.0) < 1e-12) return 1.0f;
@@ -565,95 +509,6 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
     if (free_ph) {
         fspan[0] = span[0]; fspan[1] = span[1];
         span[0] = make_int4(0, 0, 0, 0); span[1] = make_int4(0, 0, 0, 0);
-    }
-}
-
-// One line of envelope reflection k on each lane of a lane pair (side = lane & 1): the
-// Kalpha1 line on side 0, its Kalpha2 companion on side 1 (Model.geometry :719-736,
-// buildPeakContributions profile.js:1013-1080, evalPhase heights :855-897), as
-// build_reflection forms them.  Each lane prepares its own line and sums half of the
-// structure factor's representatives; the pair trades the halves, and side 1 takes the
-// Kalpha1 line's validity, height, area and Lp from side 0.  Every lane of the wavefront
-// calls it (live = false for lanes past the reflection list): the shuffles sit outside
-// any branch.  fspan != null: the slot for this lane's line of a free phase (its span goes
-// there and the line's own span stays empty).
-template <bool SEP = true>
-__device__ __forceinline__ void build_pair(const DevPlan &P, const Shared &sh, int k, bool live, int side, const int4 &hk,
-                                           LineRec &rec, int4 &span, int4 *fspan) {
-    span = make_int4(0, 0, 0, 0);
-    if (fspan) *fspan = make_int4(0, 0, 0, 0);
-    int q = 0;
-    while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
-    const DevPhase &ph = P.phase[q];
-    const double *f = sh.metric[q];
-    const double h = hk.x, kk = hk.y, l = hk.z;
-    const double qv = h * h * f[0] + kk * kk * f[1] + l * l * f[2] + h * kk * f[3] + kk * l * f[4] + h * l * f[5];
-    bool ok = live && sh.phase_ok[q] && qv > 0.0 && qv <= P.qlim;
-    const double s = ok ? sqrt(qv) : 0.0, sinT = P.lambda1 * s * 0.5;
-    ok = ok && sinT < 1.0;
-    const float disp = (float)sh.p[1], transp = (float)sh.p[2];
-    const double zero = sh.p[0];
-    // this lane's line: Kalpha1 at sin(theta1), Kalpha2 at sin(theta1) lambda2 / lambda1
-    const double st = side ? sinT * (P.lambda2 / P.lambda1) : sinT;
-    const bool mine = ok && (side == 0 || (P.doublet && fabs(st) < 1.0));
-    Prep pk;
-    pk.area = 0.0f;
-    double base = 0.0;
-    float lp = 1.0f;
-    bool tth_ok = false;
-    const bool free_ph = SEP && fspan && P.has_free && P.sp->ph[q].free_ph;
-    if (mine) {
-        const double br = 2.0 * asin(st) * kR2D;
-        const double tth = br + (double)peak_shift(br, disp, transp);
-        tth_ok = tth > 0.0 && tth < 180.0;                   // hasRenderablePosition (of the Kalpha1 line)
-        base = tth + zero;
-        SepLine sl{nullptr, 0.f, 0.f, 0.f, 0.f, 0.f};
-        const float *prof = sh.prof[q];
-        if (SEP && P.sp->ph[q].sep) {
-            sep_reflection(P.sp->ph[q], sh.sep[q], hk.x, hk.y, hk.z, (float)(1.0 / s), sl.inv_size, sl.strain);
-            sl.inst = sh.inst; sl.sm = sh.sep[q].sm; sl.em = sh.sep[q].em;
-            sl.lam = side ? (float)P.lambda2 : (float)P.lambda1;
-            prof = sh.prof[q] + PTK_PH_NPROF;
-        }
-        if (SEP) prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base, pk, sl);
-        else prepare_line(ph.profile_type, prof, (float)P.min_fwhm, (float)base, pk);
-        lp = lp_factor((float)base, (float)P.pol_k);
-    }
-    // half of the structure factor on each side (both lanes of a pair agree on ok / free_ph)
-    float part[4] = {0.f, 0.f, 0.f, 0.f};
-    if (ok && !free_ph) {
-        const int half = ph.n_reps >> 1, r0 = side ? half : 0, r1 = side ? ph.n_reps : half;
-        if (ph.centro) structure_part<true>(P, sh, ph, hk.x, hk.y, hk.z, (float)qv, r0, r1, part);
-        else structure_part<false>(P, sh, ph, hk.x, hk.y, hk.z, (float)qv, r0, r1, part);
-    }
-    const float A = part[0] + __shfl_xor(part[0], 1), B = part[1] + __shfl_xor(part[1], 1);
-    const float C = part[2] + __shfl_xor(part[2], 1), D = part[3] + __shfl_xor(part[3], 1);
-    // side 0: the Kalpha1 line's validity and height
-    const bool v1 = side == 0 && mine && tth_ok && pk.area > 0.0f;
-    float height = 0.0f;
-    if (v1) {
-        const double *pp = sh.p + ph.param_base;
-        const float po = march_dollase(P, ph, f, pp[7], k);
-        if (free_ph) {
-            height = 1.0f;
-        } else {
-            const float f2 = ph.centro ? 4.0f * (A * A + C * C) : A * A + B * B + C * C + D * D;
-            double S = pp[6];
-            if (!(S != 0.0) || isnan(S)) S = 0.0;
-            height = (float)S * ((float)hk.w * lp * po / pk.area * f2);
-        }
-    }
-    const int v1o = __shfl_xor(v1 ? 1 : 0, 1);
-    const float area1 = __shfl_xor(pk.area, 1), lp1 = __shfl_xor(lp, 1), h1 = __shfl_xor(height, 1);
-    if (side == 0) {
-        if (v1) emit_line(P, pk, base, height, rec, span);
-    } else if (v1o && mine) {
-        const float w2 = (float)P.ratio * area1 / pk.area * (lp / lp1);
-        emit_line(P, pk, base, h1 * w2, rec, span);
-    }
-    if (free_ph) {
-        *fspan = span;
-        span = make_int4(0, 0, 0, 0);
     }
 }
 
@@ -1084,43 +939,44 @@ __device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec
     return o;
 }
 
-// staged line records of reflections [kb * R, kb * R + R) of trial t, R = blockDim.x / 2,
-// one lane pair per reflection (Kalpha1 line on the even lane, Kalpha2 on the odd); every
-// line with a window widens the line range of the 512-point groups it meets (reduced in
-// LDS, then one global atomic min / max per group and workgroup)
+// staged line records of reflections [kb * blockDim.x, ...) of trial t; every line with a
+// window widens the line range of the 512-point groups it meets (reduced in LDS, then one
+// global atomic min / max per group and workgroup)
 template <bool SEP = true>
 __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, StagedLine *rec, int t, int kb, Shared &sh,
                                            int (*gr)[2]) {
-    const int side = threadIdx.x & 1, k = kb * (int)(blockDim.x >> 1) + (int)(threadIdx.x >> 1);
-    const bool live = k < P.nr;
-    const int4 hk = live ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
+    const int k = kb * blockDim.x + threadIdx.x;
+    const int4 hk = k < P.nr ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
     stage_reps(P, sh);
     image_load<SEP>(P, img + t, sh);
     const int ng = (P.np + 511) >> 9;
     for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
-    const size_t slot = (size_t)t * 2 * P.nr + 2 * (size_t)k + side;
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (sh.go) {                        // uniform over the workgroup
-        LineRec lr = {z4, z4, z4};
-        int4 sp;
-        int4 *fsp = SEP && P.has_free && live ? P.sp->fspan + slot : nullptr;
-        build_pair<SEP>(P, sh, k, live, side, hk, lr, sp, fsp);
-        if (live) {
-            if (sp.y > sp.x) {
-                const int q = 2 * k + side;
-                rec[slot] = stage_line(P, lr, sp, P.uniform != 0);
-                for (int g = sp.x >> 9; g <= (sp.y - 1) >> 9; ++g) {
-                    atomicMin(&gr[g][0], q);
-                    atomicMax(&gr[g][1], q);
+    if (k < P.nr) {
+        const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (sh.go) {
+            LineRec lr[2] = {{z4, z4, z4}, {z4, z4, z4}};
+            int4 ls[2];
+            build_reflection<SEP>(P, sh, k, lr, ls, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr);
+            const bool uni = P.uniform != 0;
+            for (int q = 0; q < 2; ++q) {
+                const int4 sp = ls[q];
+                if (sp.y > sp.x) {
+                    rec[base + q] = stage_line(P, lr[q], sp, uni);
+                    for (int g = sp.x >> 9; g <= (sp.y - 1) >> 9; ++g) {
+                        atomicMin(&gr[g][0], 2 * k + q);
+                        atomicMax(&gr[g][1], 2 * k + q);
+                    }
+                } else {
+                    rec[base + q] = StagedLine{lr[q].a, lr[q].r, lr[q].l, sp};   // no window here (a free line's is in fspan)
                 }
-            } else {
-                rec[slot] = StagedLine{lr.a, lr.r, lr.l, sp};   // no window here (a free line's is in fspan)
             }
+        } else {
+            rec[base] = StagedLine{z4, z4, z4, make_int4(0, 0, 0, 0)};
+            rec[base + 1] = StagedLine{z4, z4, z4, make_int4(0, 0, 0, 0)};
+            if (SEP && P.has_free) { P.sp->fspan[base] = make_int4(0, 0, 0, 0); P.sp->fspan[base + 1] = make_int4(0, 0, 0, 0); }
         }
-    } else if (live) {
-        rec[slot] = StagedLine{z4, z4, z4, make_int4(0, 0, 0, 0)};
-        if (SEP && P.has_free) P.sp->fspan[slot] = make_int4(0, 0, 0, 0);
     }
     __syncthreads();
     for (int g = threadIdx.x; g < ng; g += blockDim.x)
@@ -2951,9 +2807,8 @@ hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstri
 }
 static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }
 
-int lines_units(int nr) { return (nr + 127) / 128; }
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, StagedLine *rec, int ntrials, hipStream_t s, int t_off) {
-    const int per = lines_units(P.nr);
+    const int per = (P.nr + 255) / 256;
     hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per, ntrials,
                        t_off);
     return hipGetLastError();

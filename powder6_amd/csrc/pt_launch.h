@@ -26,7 +26,6 @@ size_t nnls_slot_bytes(int nf);             // global workspace per trial slot f
 hipError_t nnls_lds_limit(size_t bytes);
 hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstride, const void *mimg, void *img, int ntrials,
                                hipStream_t s, int t_off = 0);
-int lines_units(int nr);                    // lines-stage workgroups per trial (128 reflections each)
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, StagedLine *rec, int ntrials, hipStream_t s, int t_off = 0);
 // partial chi^2 per 512-point group: ngroups = ceil(np / 512) per trial, whatever tpw
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int ngroups,

@@ -23,16 +23,30 @@ constexpr int MLP_OFF_W4 = MLP_OFF_B3 + MLP_H;
 constexpr int MLP_OFF_B4 = MLP_OFF_W4 + MLP_H;
 constexpr int MLP_PARAMS = MLP_OFF_B4 + 1;
 
-// GEMM epilogue operands (row stride ldc for every output)
-struct GemmEpi {
-    float *c32;          // FP32 output (store, or dz)
-    uint16_t *c16;       // BF16 output (silu(z), or dz)
-    const float *bias;   // bias per column (forward)
-    const float *z32;    // pre-activations (backward)
-    float *zout;         // pre-activations out (forward)
-    float *colpart;      // per row-block column sums of dz (backward), [gridDim.y][N]
-    int ldc;
-    int pad;
+// per-block partial row of the fused training kernel: every parameter's gradient, then
+// the block's sum of m e^2 (at MLP_PARAMS), padded to a multiple of four floats
+constexpr int kMlpPart = (MLP_PARAMS + 1 + 3) / 4 * 4;
+
+// one launch of k_mlp_fused: the rows (given in x16 / t / mask, or drawn from a replay
+// ring when x16 is null), the weights and the per-block areas
+struct MlpRun {
+    const uint16_t *x16;                 // rows x MLP_IN BF16 inputs (null: gather from the ring)
+    const float *t, *mask;               // targets and row weights of given rows (mask may be null: 0)
+    const uint16_t *ring_x;              // replay ring (rows of MLP_IN BF16) and its targets
+    const float *ring_t;
+    int cap;
+    const double *step_ptr;              // ring fill = step * chains (PT), or fixed_count when null
+    int chains, fixed_count, min_fill;
+    uint32_t seed;
+    const int *iter;                     // Adam step count (the minibatch draw's counter)
+    int *active;                         // out: whether the minibatch trains (gathered rows)
+    int rows;                            // rows of the minibatch: ceil(rows / 32) workgroups
+    float inv_n;                         // 1 / rows
+    const uint16_t *wb, *wbt;            // BF16 parameters, and with the weight blocks transposed
+    const float *p;                      // FP32 parameters (biases)
+    float *z;                            // 3 x 32 x MLP_H pre-activations per workgroup
+    float *part;                         // kMlpPart per workgroup (training)
+    float *y;                            // predictions (inference)
 };
 
 // device view of a plan's screening state
@@ -54,22 +68,18 @@ struct ScreenDev {
     int cap, K, warmup_iters, pad;
 };
 
-hipError_t mlp_gemm_forward(const uint16_t *A, const uint16_t *W, int M, int K, const float *bias, float *z, uint16_t *a,
-                            hipStream_t s);
-hipError_t mlp_gemm_dact(const uint16_t *dz, const uint16_t *W, int M, const float *z, float *d32, uint16_t *d16, float *colpart,
-                         hipStream_t s);
-hipError_t mlp_gemm_wgrad(const uint16_t *a, int a_cols, const uint16_t *dz, int M, float *gW, hipStream_t s);
-hipError_t mlp_head(const uint16_t *a3, const float *z3, const uint16_t *w4b, const float *b4, int M, float *y, const float *t,
-                    const float *mask, float inv_n, float *dz32, uint16_t *dz16, float *part, hipStream_t s);
-hipError_t mlp_bias_grads(const float *pb1, const float *pb2, const float *ph, int nblk, float inv_n, float *g, float *loss,
-                          hipStream_t s);
-hipError_t mlp_adam(float *p, const float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2, float eps,
-                    const int *iter, const int *active, hipStream_t s);
 hipError_t mlp_to_bf16(const float *p, uint16_t *wb, int n, hipStream_t s);
 hipError_t mlp_pack(const float *x, int n, int n_in, uint16_t *x16, hipStream_t s);
-hipError_t mlp_gather(const uint16_t *ring_x, const float *ring_t, int cap, const double *step_ptr, int chains, int fixed_count,
-                      int min_fill, uint32_t seed, const int *iter, int B, uint16_t *xb, float *tb, float *mb, int *active,
-                      hipStream_t s);
+// training (forward, backward, per-block gradients) or inference (forward, head) of
+// R.rows rows in ceil(rows / 32) workgroups
+hipError_t mlp_fused(const MlpRun &R, bool train, hipStream_t s);
+// the blocks' gradients summed into g (and the loss); with adam, one Adam step and the step count
+hipError_t mlp_reduce_adam(const float *part, int rows, float inv_n, float *g, float *loss, int adam, float *p, float *m, float *v,
+                           uint16_t *wb, uint16_t *wbt, float lr, float b1, float b2, float eps, const int *iter, const int *active,
+                           hipStream_t s);
+// BF16 copy of the parameters with the weight blocks transposed (out x in)
+hipError_t mlp_to_bf16_t(const float *p, uint16_t *wbt, int n, hipStream_t s);
+size_t mlp_fused_lds_bytes();
 hipError_t screen_candidates(const DevPlan &P, const ScreenDev &S, hipStream_t s);
 hipError_t screen_inputs(const double *x, int n, int nv, const ScreenDev &S, uint16_t *xs, hipStream_t s);
 hipError_t screen_pick(const DevPlan &P, const ScreenDev &S, const int *iter, hipStream_t s);

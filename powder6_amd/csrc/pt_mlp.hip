@@ -1,25 +1,24 @@
 // pt_mlp.hip -- gfx950 kernels of the surrogate network and of surrogate-screened PT.
 //
-//   k_mlp_gemm<TA, TB, EPI>  C = A B on BF16 MFMA (16x16x16, FP32 accumulation), 64x64
-//                            output tile per workgroup, K staged through LDS in 32-wide
-//                            slices; epilogues: plain FP32 store (weight gradients), bias
-//                            + SiLU (forward layers), times SiLU'(z) with per-tile column
-//                            sums for the bias gradients (backward layers).
-//   k_mlp_head               y = a3 . w4 + b4 (64 rows per workgroup); in training also
-//                            dz3 = dy w4 SiLU'(z3), dy = 2 m (y - t) / N, and partial
-//                            sums of the b3 / w4 / b4 gradients and the loss.
-//   k_mlp_bias_grads         ordered sums of the per-row-block partials (deterministic).
-//   k_mlp_adam / k_mlp_tick  Adam on the FP32 master weights, BF16 copies refreshed.
-//   k_mlp_gather / k_mlp_pack  minibatch rows from the replay ring / host rows to BF16.
+//   k_mlp_fused<TRAIN>       32 rows per workgroup through the network with every activation
+//                            in LDS (v_mfma_f32_16x16x32_bf16, FP32 accumulation): the
+//                            forward pass and head (inference), and with TRAIN the loss,
+//                            the backward pass and the block's gradient partials; training
+//                            rows are drawn from the replay ring inside the kernel.
+//   k_mlp_reduce_adam        ordered sums of the blocks' partials (deterministic), Adam on the
+//                            FP32 master weights, the BF16 copies (row-major and transposed).
+//   k_mlp_tick               the Adam step count.
+//   k_mlp_to_bf16(_t) / k_mlp_pack  BF16 copies of the parameters / host rows to BF16.
 //   k_screen_*               the screened PT step: K candidate moves per chain, their
 //                            predicted log Rwp, the pick, and the record of every exactly
 //                            evaluated trial into the replay ring.
+#include <mutex>
+
 #include "pt_device.h"
 #include "pt_mlp.h"
 
 namespace ptk {
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 // round to nearest even (finite inputs)
@@ -34,234 +33,6 @@ __device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
     const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
-}
-
-enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_DSILU = 2 };
-
-// four consecutive BF16 values of a row (8-byte load when in range and aligned)
-__device__ __forceinline__ uint2 load4(const uint16_t *__restrict__ p, int avail) {
-    if (avail >= 4) return *reinterpret_cast<const uint2 *>(p);
-    uint16_t v[4] = {0, 0, 0, 0};
-    for (int u = 0; u < avail; ++u) v[u] = p[u];
-    return make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
-}
-
-// C[M x N] = A[M x K] B[K x N].  A(m, k) = TA ? A[k lda + m] : A[m lda + k];
-// B(k, n) = TB ? B[n ldb + k] : B[k ldb + n]; leading dimensions are multiples of 4.
-// Four wavefronts, each a 32 x 32 quarter of the 64 x 64 tile as 2 x 2 MFMA blocks; the
-// operands are staged in LDS as sA[m][k] and sB[n][k] with 8-byte global loads.
-// Operand layout of v_mfma_f32_16x16x16_bf16 for lane l (i = l % 16, q = l / 16):
-// a[e] = A(i, 4q + e), b[e] = B(4q + e, i), and the result d[e] = C(4q + e, i).
-// With ep.colpart set (backward epilogue) the workgroup also writes the column sums of
-// its 64 rows of dz to colpart[blockIdx.y][n] (the bias gradient, reduced in order later).
-template <bool TA, bool TB, int EPI>
-__global__ void __launch_bounds__(256) k_mlp_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B,
-                                                  int ldb, int M, int N, int K, GemmEpi ep) {
-    __shared__ __align__(16) uint16_t sA[64][40];   // [m][k]
-    __shared__ __align__(16) uint16_t sB[64][40];   // [n][k]
-    __shared__ float csum[2][64];
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int li = lane & 15, lq = lane >> 4;
-    f32x4_t acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int e = tid + 256 * r;
-            if (!TA) {          // rows of A along k: 64 rows x 8 quads
-                const int mm = e >> 3, kq = (e & 7) * 4, gm = m0 + mm, gk = k0 + kq;
-                uint2 v = make_uint2(0u, 0u);
-                if (gm < M && gk < K) v = load4(A + (size_t)gm * lda + gk, K - gk);
-                *reinterpret_cast<uint2 *>(&sA[mm][kq]) = v;
-            } else {            // A stored k-major: 32 k x 16 quads of m, transposed into LDS
-                const int kk = e >> 4, mq = (e & 15) * 4, gm = m0 + mq, gk = k0 + kk;
-                uint2 v = make_uint2(0u, 0u);
-                if (gm < M && gk < K) v = load4(A + (size_t)gk * lda + gm, M - gm);
-                sA[mq][kk] = (uint16_t)(v.x & 0xffffu);
-                sA[mq + 1][kk] = (uint16_t)(v.x >> 16);
-                sA[mq + 2][kk] = (uint16_t)(v.y & 0xffffu);
-                sA[mq + 3][kk] = (uint16_t)(v.y >> 16);
-            }
-            if (TB) {           // B stored n-major (k contiguous)
-                const int nn = e >> 3, kq = (e & 7) * 4, gn = n0 + nn, gk = k0 + kq;
-                uint2 v = make_uint2(0u, 0u);
-                if (gn < N && gk < K) v = load4(B + (size_t)gn * ldb + gk, K - gk);
-                *reinterpret_cast<uint2 *>(&sB[nn][kq]) = v;
-            } else {            // B stored k-major (n contiguous), transposed into LDS
-                const int kk = e >> 4, nq = (e & 15) * 4, gn = n0 + nq, gk = k0 + kk;
-                uint2 v = make_uint2(0u, 0u);
-                if (gn < N && gk < K) v = load4(B + (size_t)gk * ldb + gn, N - gn);
-                sB[nq][kk] = (uint16_t)(v.x & 0xffffu);
-                sB[nq + 1][kk] = (uint16_t)(v.x >> 16);
-                sB[nq + 2][kk] = (uint16_t)(v.y & 0xffffu);
-                sB[nq + 3][kk] = (uint16_t)(v.y >> 16);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 32; kk += 16) {
-            bf16x4_t a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = *reinterpret_cast<const bf16x4_t *>(&sA[wm + 16 * i + li][kk + 4 * lq]);
-                b[i] = *reinterpret_cast<const bf16x4_t *>(&sB[wn + 16 * i + li][kk + 4 * lq]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    float cs[2] = {0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int m = m0 + wm + 16 * i + 4 * lq + e, n = n0 + wn + 16 * j + li;
-                if (m >= M || n >= N) continue;
-                const size_t o = (size_t)m * ep.ldc + n;
-                const float v = acc[i][j][e];
-                if (EPI == EPI_STORE) {
-                    ep.c32[o] = v;
-                } else if (EPI == EPI_BIAS_SILU) {
-                    const float z = v + ep.bias[n];
-                    ep.zout[o] = z;
-                    ep.c16[o] = to_bf16(silu(z));
-                } else {
-                    const float d = v * dsilu(ep.z32[o]);
-                    ep.c32[o] = d;
-                    ep.c16[o] = to_bf16(d);
-                    cs[j] += d;
-                }
-            }
-    if (EPI == EPI_DSILU && ep.colpart) {
-        // column sums: over e and i in the lane, over the four lane quads, then the two
-        // wavefronts that share the columns
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            cs[j] += __shfl_xor(cs[j], 16);
-            cs[j] += __shfl_xor(cs[j], 32);
-        }
-        if (lq == 0) {
-            csum[wave >> 1][wn + li] = cs[0];
-            csum[wave >> 1][wn + 16 + li] = cs[1];
-        }
-        __syncthreads();
-        if (tid < 64 && n0 + tid < N) ep.colpart[(size_t)blockIdx.y * N + n0 + tid] = csum[0][tid] + csum[1][tid];
-    }
-}
-
-// y = a3 . w4 + b4 for 64 rows per workgroup (16 per wavefront, lanes over the hidden
-// units); with t != null also the head's backward pass -- dy = 2 m (y - t) / N,
-// dz3 = dy w4 SiLU'(z3) (FP32 and BF16) -- and this workgroup's partial sums of the b3
-// and w4 gradients (part[blk][0..255], part[blk][256..511]) and of b4 and the loss
-// (part[blk][512], part[blk][513]), reduced in order by k_mlp_bias_grads
-__global__ void __launch_bounds__(256) k_mlp_head(const uint16_t *__restrict__ a3, const float *__restrict__ z3,
-                                                  const uint16_t *__restrict__ w4b, const float *__restrict__ b4, int M,
-                                                  float *y, const float *__restrict__ t, const float *__restrict__ mask,
-                                                  float inv_n, float *dz32, uint16_t *dz16, float *part) {
-    __shared__ float red[4][2 * MLP_H + 2];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    float w4[MLP_H / 64];
-#pragma unroll
-    for (int q = 0; q < MLP_H / 64; ++q) w4[q] = from_bf16(w4b[lane + 64 * q]);
-    float gb3[MLP_H / 64] = {0.f, 0.f, 0.f, 0.f}, gw4[MLP_H / 64] = {0.f, 0.f, 0.f, 0.f};
-    float gb4 = 0.f, gl = 0.f;
-    for (int rr = 0; rr < 16; ++rr) {
-        const int row = blockIdx.x * 64 + wave * 16 + rr;
-        if (row >= M) break;
-        float av[MLP_H / 64];
-        float s = 0.f;
-#pragma unroll
-        for (int q = 0; q < MLP_H / 64; ++q) {
-            av[q] = from_bf16(a3[(size_t)row * MLP_H + lane + 64 * q]);
-            s = fmaf(av[q], w4[q], s);
-        }
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        const float yv = s + b4[0];
-        if (lane == 0) y[row] = yv;
-        if (!t) continue;
-        const float m = mask[row], e = yv - t[row];
-        const float d = 2.0f * m * e * inv_n;
-        gb4 += d;
-        gl += m * e * e;
-#pragma unroll
-        for (int q = 0; q < MLP_H / 64; ++q) {
-            const size_t o = (size_t)row * MLP_H + lane + 64 * q;
-            const float g = d * w4[q] * dsilu(z3[o]);
-            dz32[o] = g;
-            dz16[o] = to_bf16(g);
-            gb3[q] += g;
-            gw4[q] = fmaf(d, av[q], gw4[q]);
-        }
-    }
-    if (!t) return;
-#pragma unroll
-    for (int q = 0; q < MLP_H / 64; ++q) {
-        red[wave][lane + 64 * q] = gb3[q];
-        red[wave][MLP_H + lane + 64 * q] = gw4[q];
-    }
-    if (lane == 0) { red[wave][2 * MLP_H] = gb4; red[wave][2 * MLP_H + 1] = gl; }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 2 * MLP_H + 2; i += blockDim.x)
-        part[(size_t)blockIdx.x * (2 * MLP_H + 2) + i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
-}
-
-// gradients of b1, b2 (column partials of the backward GEMMs), b3, w4, b4 and the loss
-// (head partials): ordered sums over the nblk row blocks, so the result is deterministic
-__global__ void __launch_bounds__(256) k_mlp_bias_grads(const float *__restrict__ pb1, const float *__restrict__ pb2,
-                                                        const float *__restrict__ ph, int nblk, float inv_n, float *g,
-                                                        float *loss) {
-    const int i = threadIdx.x;
-    float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
-    for (int b = 0; b < nblk; ++b) {
-        s1 += pb1[(size_t)b * MLP_H + i];
-        s2 += pb2[(size_t)b * MLP_H + i];
-        s3 += ph[(size_t)b * (2 * MLP_H + 2) + i];
-        s4 += ph[(size_t)b * (2 * MLP_H + 2) + MLP_H + i];
-    }
-    g[MLP_OFF_B1 + i] = s1;
-    g[MLP_OFF_B2 + i] = s2;
-    g[MLP_OFF_B3 + i] = s3;
-    g[MLP_OFF_W4 + i] = s4;
-    if (i == 0) {
-        float b = 0.f, l = 0.f;
-        for (int k = 0; k < nblk; ++k) {
-            b += ph[(size_t)k * (2 * MLP_H + 2) + 2 * MLP_H];
-            l += ph[(size_t)k * (2 * MLP_H + 2) + 2 * MLP_H + 1];
-        }
-        g[MLP_OFF_B4] = b;
-        loss[0] = l * inv_n;
-        // loss[1]: running mean over recent minibatches (weight 1/64), seeded by the first
-        loss[1] = loss[1] > 0.f ? loss[1] + (loss[0] - loss[1]) * (1.0f / 64.0f) : loss[0];
-    }
-}
-
-// Adam on the FP32 master parameters (step t = iter + 1), BF16 copy refreshed; nothing
-// when the minibatch was inactive (ring not yet filled)
-__global__ void __launch_bounds__(256) k_mlp_adam(float *p, const float *__restrict__ g, float *m, float *v, uint16_t *wb, int n,
-                                                  float lr, float b1, float b2, float eps, const int *iter, const int *active) {
-    if (!*active) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float t = (float)(*iter + 1);
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.0f - b1) * gi;
-    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    const float mh = mi / (1.0f - powf(b1, t)), vh = vi / (1.0f - powf(b2, t));
-    const float pi = p[i] - lr * mh / (sqrtf(vh) + eps);
-    p[i] = pi;
-    wb[i] = to_bf16(pi);
 }
 
 __global__ void k_mlp_tick(int *iter, const int *active) {
@@ -279,35 +50,321 @@ __global__ void __launch_bounds__(64) k_mlp_pack(const float *__restrict__ x, in
     x16[(size_t)r * MLP_IN + i] = i < n_in ? to_bf16(x[(size_t)r * n_in + i]) : (uint16_t)0;
 }
 
-// minibatch row r of B from the replay ring: index hash(seed, iteration, r) mod count,
-// where count = min(cap, step * chains) (ring fed by the PT steps) or fixed_count;
-// inactive (all rows masked) while count < min_fill.  Entries with a NaN target
-// (infeasible trials) are masked out.
-__global__ void __launch_bounds__(64) k_mlp_gather(const uint16_t *__restrict__ ring_x, const float *__restrict__ ring_t,
-                                                   int cap, const double *step_ptr, int chains, int fixed_count,
-                                                   int min_fill, uint32_t seed, const int *iter, uint16_t *xb, float *tb,
-                                                   float *mb, int *active) {
-    long long count = fixed_count;
-    if (step_ptr) count = (long long)fmax(0.0, *step_ptr) * chains;
-    if (count > cap) count = cap;
-    const bool act = count >= min_fill && count > 0;
-    const int r = blockIdx.x, i = threadIdx.x;
-    if (r == 0 && i == 0) *active = act ? 1 : 0;
-    if (!act) {
-        xb[(size_t)r * MLP_IN + i] = 0;
-        if (i == 0) { tb[r] = 0.f; mb[r] = 0.f; }
+// ---- one training step or inference in a few launches: 32 rows per workgroup ----------
+// k_mlp_fused<TRAIN> takes a block of 32 rows (given, or drawn from the replay ring: row r
+// takes entry floor(u count) of the ring's count filled entries, u uniform from (seed,
+// Adam step, r)) through the three hidden layers and the head with every
+// activation in LDS; with TRAIN it goes on through the backward pass and writes this
+// block's gradient of every parameter (the weight gradients as the 32-row products
+// A^T dZ, the bias gradients as column sums) and its loss into its own partial row.
+// k_mlp_reduce_adam sums the blocks' partials in block order (deterministic, no atomics)
+// and runs Adam; k_mlp_tick counts the step.  The layers run on v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation: a wavefront owns 64 output columns of the block's 32 rows (2 x 4
+// blocks of 16 x 16).  Operands are K-contiguous per lane, so each activation and
+// gradient is kept in LDS row-major (the forward's and backward's A operand) and
+// transposed (the weight gradients' operands), and the forward reads the transposed BF16
+// weights wbt (out x in).  Pre-activations z go to a per-block global area; each lane
+// reads back only the elements it wrote.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int kTR = 32;                  // rows per workgroup
+constexpr int kLdR = MLP_H + 8;          // row-major LDS stride (BF16), 16-byte aligned rows
+constexpr int kLdX = MLP_IN + 8;
+constexpr int kLdT = kTR + 8;            // transposed LDS stride (BF16)
+
+struct FusedLds {
+    uint16_t x[kTR][kLdX];               // input rows
+    uint16_t xt[MLP_IN][kLdT];           // the same, transposed
+    uint16_t r1[kTR][kLdR];              // a1, then dz2
+    uint16_t t1[MLP_H][kLdT];            // a1 transposed
+    uint16_t r2[kTR][kLdR];              // a2, then dz3
+    uint16_t t2[MLP_H][kLdT];            // a2 transposed, then dz1 transposed
+    uint16_t r3[kTR][kLdR];              // a3
+    uint16_t t3[MLP_H][kLdT];            // dz3 transposed
+    uint16_t t4[MLP_H][kLdT];            // dz2 transposed
+    float tgt[kTR], msk[kTR], dy[kTR];
+    float red[4][MLP_H + 2];             // head: per-wavefront w4 gradient, b4 gradient, loss
+};
+size_t mlp_fused_lds_bytes() { return sizeof(FusedLds); }
+
+__device__ __forceinline__ bf16x8_t frag(const uint16_t *p) { return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(p)); }
+__device__ __forceinline__ f32x4_t mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// z = a W + b for the block's rows: A row-major in LDS (K = KIN), W^T rows of the
+// wavefront's 64 outputs from global; z to zg, BF16 SiLU(z) to ar (and at, transposed)
+template <int KIN>
+__device__ __forceinline__ void fused_forward(const uint16_t *A, int lda, const uint16_t *__restrict__ wt,
+                                              const float *__restrict__ bias, float *zg, uint16_t *ar, uint16_t (*at)[kLdT],
+                                              bool store_z) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = 8 * (lane >> 4), dq = 4 * (lane >> 4);
+    f32x4_t acc[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[rb][c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < KIN; k0 += 32) {
+        bf16x8_t b[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = frag(wt + (size_t)(64 * wave + 16 * c + li) * KIN + k0 + kq);
+        const bf16x8_t a0 = frag(A + li * lda + k0 + kq), a1 = frag(A + (16 + li) * lda + k0 + kq);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            acc[0][c] = mfma32(a0, b[c], acc[0][c]);
+            acc[1][c] = mfma32(a1, b[c], acc[1][c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int o = 64 * wave + 16 * c + li;
+        const float bo = bias[o];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = 16 * rb + dq + e;
+                const float z = acc[rb][c][e] + bo;
+                if (store_z) zg[row * MLP_H + o] = z;
+                const uint16_t v = to_bf16(silu(z));
+                ar[row * kLdR + o] = v;
+                if (at) at[o][row] = v;
+            }
+    }
+}
+
+// dz_prev = (dZ W^T) SiLU'(z_prev): A = dZ row-major in LDS, W rows of the wavefront's 64
+// outputs from global (K = the layer's outputs); BF16 dz_prev to dr (if given) and dt
+// (transposed), its column sums (the bias gradient) to bgrad
+__device__ __forceinline__ void fused_backward(const uint16_t *D, const uint16_t *__restrict__ w, const float *zg, uint16_t *dr,
+                                               uint16_t (*dt)[kLdT], float *bgrad) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = 8 * (lane >> 4), dq = 4 * (lane >> 4);
+    f32x4_t acc[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[rb][c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < MLP_H; k0 += 32) {
+        bf16x8_t b[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = frag(w + (size_t)(64 * wave + 16 * c + li) * MLP_H + k0 + kq);
+        const bf16x8_t a0 = frag(D + li * kLdR + k0 + kq), a1 = frag(D + (16 + li) * kLdR + k0 + kq);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            acc[0][c] = mfma32(a0, b[c], acc[0][c]);
+            acc[1][c] = mfma32(a1, b[c], acc[1][c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int i = 64 * wave + 16 * c + li;
+        float cs = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = 16 * rb + dq + e;
+                const float d = acc[rb][c][e] * dsilu(zg[row * MLP_H + i]);
+                const uint16_t v = to_bf16(d);
+                if (dr) dr[row * kLdR + i] = v;
+                dt[i][row] = v;
+                cs += d;
+            }
+        cs += __shfl_xor(cs, 16);
+        cs += __shfl_xor(cs, 32);
+        if (lane < 16) bgrad[i] = cs;
+    }
+}
+
+// gW[i][o] = sum over the block's rows of A[row][i] dZ[row][o], from the transposed
+// copies (K = the 32 rows: one MFMA per 16 x 16 block); MI rows of gW, the wavefront's 64
+// columns
+template <int MI>
+__device__ __forceinline__ void fused_wgrad(const uint16_t (*at)[kLdT], const uint16_t (*dt)[kLdT], float *gw) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = 8 * (lane >> 4), dq = 4 * (lane >> 4);
+    bf16x8_t b[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[c] = frag(&dt[64 * wave + 16 * c + li][kq]);
+#pragma unroll 4
+    for (int ib = 0; ib < MI / 16; ++ib) {
+        const bf16x8_t a = frag(&at[16 * ib + li][kq]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const f32x4_t r = mfma32(a, b[c], f32x4_t{0.f, 0.f, 0.f, 0.f});
+            const int o = 64 * wave + 16 * c + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) gw[(size_t)(16 * ib + dq + e) * MLP_H + o] = r[e];
+        }
+    }
+}
+
+template <bool TRAIN>
+__global__ void __launch_bounds__(256) k_mlp_fused(MlpRun R) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    FusedLds &S = *reinterpret_cast<FusedLds *>(lds_raw);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, blk = blockIdx.x, r0 = blk * kTR;
+    // the block's rows: given, or drawn from the replay ring (inactive, all rows weightless,
+    // until the ring holds min_fill entries; entries with a NaN target weigh nothing)
+    bool act = true;
+    if (!R.x16) {
+        long long count = R.fixed_count;
+        if (R.step_ptr) count = (long long)fmax(0.0, *R.step_ptr) * R.chains;
+        if (count > R.cap) count = R.cap;
+        act = count >= R.min_fill && count > 0;
+        if (blk == 0 && tid == 0) *R.active = act ? 1 : 0;
+        for (int q = tid; q < kTR * (MLP_IN / 8); q += blockDim.x) {
+            const int rr = q / (MLP_IN / 8), c8 = 8 * (q % (MLP_IN / 8)), row = r0 + rr;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            float tv = 0.f, mv = 0.f;
+            if (act && row < R.rows) {
+                const double u = uniform01(R.seed, row, (unsigned long long)(*R.iter), 11);
+                long long idx = (long long)(u * (double)count);
+                if (idx >= count) idx = count - 1;
+                v = *reinterpret_cast<const uint4 *>(R.ring_x + (size_t)idx * MLP_IN + c8);
+                const float tr = R.ring_t[idx];
+                const bool ok = tr == tr;
+                tv = ok ? tr : 0.f;
+                mv = ok ? 1.f : 0.f;
+            }
+            *reinterpret_cast<uint4 *>(&S.x[rr][c8]) = v;
+            if (c8 == 0) { S.tgt[rr] = tv; S.msk[rr] = mv; }
+        }
+    } else {
+        for (int q = tid; q < kTR * (MLP_IN / 8); q += blockDim.x) {
+            const int rr = q / (MLP_IN / 8), c8 = 8 * (q % (MLP_IN / 8)), row = r0 + rr;
+            const bool in = row < R.rows;
+            *reinterpret_cast<uint4 *>(&S.x[rr][c8]) =
+                in ? *reinterpret_cast<const uint4 *>(R.x16 + (size_t)row * MLP_IN + c8) : make_uint4(0u, 0u, 0u, 0u);
+            if (c8 == 0) { S.tgt[rr] = in && R.t ? R.t[row] : 0.f; S.msk[rr] = in && R.mask ? R.mask[row] : 0.f; }
+        }
+    }
+    __syncthreads();
+    if (TRAIN)
+        for (int q = tid; q < kTR * MLP_IN; q += blockDim.x) S.xt[q % MLP_IN][q / MLP_IN] = S.x[q / MLP_IN][q % MLP_IN];
+    float *z = R.z + (size_t)blk * 3 * kTR * MLP_H;
+    fused_forward<MLP_IN>(&S.x[0][0], kLdX, R.wbt + MLP_OFF_W1, R.p + MLP_OFF_B1, z, &S.r1[0][0], TRAIN ? S.t1 : nullptr, TRAIN);
+    __syncthreads();
+    fused_forward<MLP_H>(&S.r1[0][0], kLdR, R.wbt + MLP_OFF_W2, R.p + MLP_OFF_B2, z + kTR * MLP_H, &S.r2[0][0],
+                         TRAIN ? S.t2 : nullptr, TRAIN);
+    __syncthreads();
+    fused_forward<MLP_H>(&S.r2[0][0], kLdR, R.wbt + MLP_OFF_W3, R.p + MLP_OFF_B3, z + 2 * kTR * MLP_H, &S.r3[0][0], nullptr, TRAIN);
+    __syncthreads();
+    // head (k_mlp_head's arithmetic): wavefront w takes rows 8w..8w+7, lanes the hidden units
+    float w4[MLP_H / 64], gw4[MLP_H / 64] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < MLP_H / 64; ++q) w4[q] = from_bf16(R.wb[MLP_OFF_W4 + lane + 64 * q]);
+    float gb4 = 0.f, gl = 0.f;
+    for (int rr = 8 * wave; rr < 8 * wave + 8; ++rr) {
+        float av[MLP_H / 64], s = 0.f;
+#pragma unroll
+        for (int q = 0; q < MLP_H / 64; ++q) {
+            av[q] = from_bf16(S.r3[rr][lane + 64 * q]);
+            s = fmaf(av[q], w4[q], s);
+        }
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        const float yv = s + R.p[MLP_OFF_B4];
+        if (!TRAIN) {
+            if (lane == 0 && r0 + rr < R.rows) R.y[r0 + rr] = yv;
+            continue;
+        }
+        const float m = S.msk[rr], e = yv - S.tgt[rr];
+        const float d = 2.0f * m * e * R.inv_n;
+        gb4 += d;
+        gl += m * e * e;
+#pragma unroll
+        for (int q = 0; q < MLP_H / 64; ++q) gw4[q] = fmaf(d, av[q], gw4[q]);
+        if (lane == 0) S.dy[rr] = d;
+    }
+    if (!TRAIN) return;
+#pragma unroll
+    for (int q = 0; q < MLP_H / 64; ++q) S.red[wave][lane + 64 * q] = gw4[q];
+    if (lane == 0) { S.red[wave][MLP_H] = gb4; S.red[wave][MLP_H + 1] = gl; }
+    __syncthreads();
+    float *part = R.part + (size_t)blk * kMlpPart;
+    for (int i = tid; i < MLP_H + 2; i += blockDim.x) {
+        const float v = S.red[0][i] + S.red[1][i] + S.red[2][i] + S.red[3][i];
+        if (i < MLP_H) part[MLP_OFF_W4 + i] = v;
+        else if (i == MLP_H) part[MLP_OFF_B4] = v;
+        else part[MLP_PARAMS] = v;                           // sum of m e^2
+    }
+    // dz3 = dy w4 SiLU'(z3), in the forward's element mapping (each lane's own z3)
+    {
+        const int li = lane & 15, dq = 4 * (lane >> 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int o = 64 * wave + 16 * c + li;
+            const float w4o = from_bf16(R.wb[MLP_OFF_W4 + o]);
+            float cs = 0.f;
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int row = 16 * rb + dq + e;
+                    const float g = S.dy[row] * w4o * dsilu(z[2 * kTR * MLP_H + row * MLP_H + o]);
+                    const uint16_t v = to_bf16(g);
+                    S.r2[row][o] = v;                        // a2 (row-major) is no longer needed
+                    S.t3[o][row] = v;
+                    cs += g;
+                }
+            cs += __shfl_xor(cs, 16);
+            cs += __shfl_xor(cs, 32);
+            if (lane < 16) part[MLP_OFF_B3 + o] = cs;
+        }
+    }
+    __syncthreads();
+    fused_wgrad<MLP_H>(S.t2, S.t3, part + MLP_OFF_W3);                                   // dW3 = a2^T dz3
+    fused_backward(&S.r2[0][0], R.wb + MLP_OFF_W3, z + kTR * MLP_H, &S.r1[0][0], S.t4, part + MLP_OFF_B2);   // dz2
+    __syncthreads();
+    fused_wgrad<MLP_H>(S.t1, S.t4, part + MLP_OFF_W2);                                   // dW2 = a1^T dz2
+    fused_backward(&S.r1[0][0], R.wb + MLP_OFF_W2, z, nullptr, S.t2, part + MLP_OFF_B1); // dz1 (a2^T is used up)
+    __syncthreads();
+    fused_wgrad<MLP_IN>(S.xt, S.t2, part + MLP_OFF_W1);                                  // dW1 = x^T dz1
+}
+
+// g = sum of the blocks' partials in block order; loss as k_mlp_bias_grads keeps it; with
+// adam, Adam on every parameter (as k_mlp_adam) and the BF16 copies (wbt: the weight
+// blocks transposed) refreshed
+__device__ __forceinline__ int mlp_transposed_index(int i) {
+    if (i < MLP_OFF_B1) return (i % MLP_H) * MLP_IN + i / MLP_H;
+    if (i >= MLP_OFF_W2 && i < MLP_OFF_B2) { const int j = i - MLP_OFF_W2; return MLP_OFF_W2 + (j % MLP_H) * MLP_H + j / MLP_H; }
+    if (i >= MLP_OFF_W3 && i < MLP_OFF_B3) { const int j = i - MLP_OFF_W3; return MLP_OFF_W3 + (j % MLP_H) * MLP_H + j / MLP_H; }
+    return i;
+}
+__global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float *__restrict__ part, int nblk, float inv_n, float *g,
+                                                         float *loss, int adam, float *p, float *m, float *v, uint16_t *wb,
+                                                         uint16_t *wbt, float lr, float b1, float b2, float eps,
+                                                         const int *iter, const int *active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == MLP_PARAMS) {
+        float l = 0.f;
+        for (int k = 0; k < nblk; ++k) l += part[(size_t)k * kMlpPart + MLP_PARAMS];
+        loss[0] = l * inv_n;
+        loss[1] = loss[1] > 0.f ? loss[1] + (loss[0] - loss[1]) * (1.0f / 64.0f) : loss[0];
         return;
     }
-    const double u = uniform01(seed, r, (unsigned long long)(*iter), 11);
-    long long idx = (long long)(u * (double)count);
-    if (idx >= count) idx = count - 1;
-    xb[(size_t)r * MLP_IN + i] = ring_x[(size_t)idx * MLP_IN + i];
-    if (i == 0) {
-        const float tv = ring_t[idx];
-        const bool ok = tv == tv;
-        tb[r] = ok ? tv : 0.f;
-        mb[r] = ok ? 1.f : 0.f;
-    }
+    if (i > MLP_PARAMS) return;
+    float s = 0.f;
+    for (int k = 0; k < nblk; ++k) s += part[(size_t)k * kMlpPart + i];
+    g[i] = s;
+    if (!adam || !*active) return;
+    const float t = (float)(*iter + 1);
+    const float mi = b1 * m[i] + (1.0f - b1) * s;
+    const float vi = b2 * v[i] + (1.0f - b2) * s * s;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.0f - powf(b1, t)), vh = vi / (1.0f - powf(b2, t));
+    const float pi = p[i] - lr * mh / (sqrtf(vh) + eps);
+    p[i] = pi;
+    const uint16_t h = to_bf16(pi);
+    wb[i] = h;
+    wbt[mlp_transposed_index(i)] = h;
+}
+
+__global__ void __launch_bounds__(256) k_mlp_to_bf16_t(const float *__restrict__ p, uint16_t *wbt, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wbt[mlp_transposed_index(i)] = to_bf16(p[i]);
 }
 
 // ---- surrogate-screened PT step ---------------------------------------------------------
@@ -411,46 +468,6 @@ __global__ void __launch_bounds__(64) k_screen_record(DevPlan P, ScreenDev S) {
 }
 
 // ---- launchers ------------------------------------------------------------------------
-template <bool TA, bool TB, int EPI>
-static hipError_t gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int M, int N, int K, const GemmEpi &ep,
-                       hipStream_t s) {
-    hipLaunchKernelGGL((k_mlp_gemm<TA, TB, EPI>), dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, s, A, lda, B, ldb, M, N, K, ep);
-    return hipGetLastError();
-}
-
-hipError_t mlp_gemm_forward(const uint16_t *A, const uint16_t *W, int M, int K, const float *bias, float *z, uint16_t *a,
-                            hipStream_t s) {
-    GemmEpi ep{};
-    ep.bias = bias; ep.zout = z; ep.c16 = a; ep.ldc = MLP_H;
-    return gemm<false, false, EPI_BIAS_SILU>(A, K, W, MLP_H, M, MLP_H, K, ep, s);
-}
-hipError_t mlp_gemm_dact(const uint16_t *dz, const uint16_t *W, int M, const float *z, float *d32, uint16_t *d16, float *colpart,
-                         hipStream_t s) {
-    GemmEpi ep{};
-    ep.z32 = z; ep.c32 = d32; ep.c16 = d16; ep.ldc = MLP_H; ep.colpart = colpart;
-    return gemm<false, true, EPI_DSILU>(dz, MLP_H, W, MLP_H, M, MLP_H, MLP_H, ep, s);
-}
-hipError_t mlp_gemm_wgrad(const uint16_t *a, int a_cols, const uint16_t *dz, int M, float *gW, hipStream_t s) {
-    GemmEpi ep{};
-    ep.c32 = gW; ep.ldc = MLP_H;
-    return gemm<true, false, EPI_STORE>(a, a_cols, dz, MLP_H, a_cols, MLP_H, M, ep, s);
-}
-hipError_t mlp_head(const uint16_t *a3, const float *z3, const uint16_t *w4b, const float *b4, int M, float *y, const float *t,
-                    const float *mask, float inv_n, float *dz32, uint16_t *dz16, float *part, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_head, dim3((M + 63) / 64), dim3(256), 0, s, a3, z3, w4b, b4, M, y, t, mask, inv_n, dz32, dz16, part);
-    return hipGetLastError();
-}
-hipError_t mlp_bias_grads(const float *pb1, const float *pb2, const float *ph, int nblk, float inv_n, float *g, float *loss,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_bias_grads, dim3(1), dim3(MLP_H), 0, s, pb1, pb2, ph, nblk, inv_n, g, loss);
-    return hipGetLastError();
-}
-hipError_t mlp_adam(float *p, const float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2, float eps,
-                    const int *iter, const int *active, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_adam, dim3((n + 255) / 256), dim3(256), 0, s, p, g, m, v, wb, n, lr, b1, b2, eps, iter, active);
-    hipLaunchKernelGGL(k_mlp_tick, dim3(1), dim3(64), 0, s, (int *)iter, active);
-    return hipGetLastError();
-}
 hipError_t mlp_to_bf16(const float *p, uint16_t *wb, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_mlp_to_bf16, dim3((n + 255) / 256), dim3(256), 0, s, p, wb, n);
     return hipGetLastError();
@@ -459,11 +476,36 @@ hipError_t mlp_pack(const float *x, int n, int n_in, uint16_t *x16, hipStream_t 
     if (n > 0) hipLaunchKernelGGL(k_mlp_pack, dim3(n), dim3(64), 0, s, x, n_in, x16);
     return hipGetLastError();
 }
-hipError_t mlp_gather(const uint16_t *ring_x, const float *ring_t, int cap, const double *step_ptr, int chains, int fixed_count,
-                      int min_fill, uint32_t seed, const int *iter, int B, uint16_t *xb, float *tb, float *mb, int *active,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_gather, dim3(B), dim3(64), 0, s, ring_x, ring_t, cap, step_ptr, chains, fixed_count, min_fill, seed,
-                       iter, xb, tb, mb, active);
+hipError_t mlp_fused(const MlpRun &R, bool train, hipStream_t s) {
+    static std::mutex mu;
+    static bool raised = false;
+    const size_t bytes = sizeof(FusedLds);
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!raised) {
+            hipError_t e = hipFuncSetAttribute((const void *)k_mlp_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void *)k_mlp_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    const dim3 grid((R.rows + kTR - 1) / kTR);
+    if (train) hipLaunchKernelGGL(k_mlp_fused<true>, grid, dim3(256), bytes, s, R);
+    else hipLaunchKernelGGL(k_mlp_fused<false>, grid, dim3(256), bytes, s, R);
+    return hipGetLastError();
+}
+hipError_t mlp_reduce_adam(const float *part, int rows, float inv_n, float *g, float *loss, int adam, float *p, float *m, float *v,
+                           uint16_t *wb, uint16_t *wbt, float lr, float b1, float b2, float eps, const int *iter, const int *active,
+                           hipStream_t s) {
+    const int nblk = (rows + kTR - 1) / kTR;
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_PARAMS + 1 + 255) / 256), dim3(256), 0, s, part, nblk, inv_n, g, loss, adam, p, m,
+                       v, wb, wbt, lr, b1, b2, eps, iter, active);
+    if (adam) hipLaunchKernelGGL(k_mlp_tick, dim3(1), dim3(64), 0, s, (int *)iter, active);
+    return hipGetLastError();
+}
+hipError_t mlp_to_bf16_t(const float *p, uint16_t *wbt, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_to_bf16_t, dim3((n + 255) / 256), dim3(256), 0, s, p, wbt, n);
     return hipGetLastError();
 }
 hipError_t screen_candidates(const DevPlan &P, const ScreenDev &S, hipStream_t s) {

@@ -33,24 +33,31 @@ int nalloc(pt_mlp *m, T **ptr, size_t count) {
     return 0;
 }
 
-// forward pass of `rows` rows already in m->x16 (activations kept), then -- when t is
-// given -- the loss, the backward pass and every gradient into m->g
-int fwd_bwd(pt_mlp *m, int rows, bool train, hipStream_t s) {
-    const uint16_t *W = m->wb;
-    MLPCHECK(mlp_gemm_forward(m->x16, W + MLP_OFF_W1, rows, MLP_IN, m->p + MLP_OFF_B1, m->z1, m->a1, s), "layer 1");
-    MLPCHECK(mlp_gemm_forward(m->a1, W + MLP_OFF_W2, rows, MLP_H, m->p + MLP_OFF_B2, m->z2, m->a2, s), "layer 2");
-    MLPCHECK(mlp_gemm_forward(m->a2, W + MLP_OFF_W3, rows, MLP_H, m->p + MLP_OFF_B3, m->z3, m->a3, s), "layer 3");
-    const float inv_n = 1.0f / (float)std::max(1, rows);
-    MLPCHECK(mlp_head(m->a3, m->z3, W + MLP_OFF_W4, m->p + MLP_OFF_B4, rows, m->y, train ? m->t : nullptr, m->mask, inv_n, m->d3,
-                      m->d3h, m->parth, s), "head");
-    if (!train) return 0;
-    float *g = m->g;
-    MLPCHECK(mlp_gemm_wgrad(m->a2, MLP_H, m->d3h, rows, g + MLP_OFF_W3, s), "W3 gradient");
-    MLPCHECK(mlp_gemm_dact(m->d3h, W + MLP_OFF_W3, rows, m->z2, m->d2, m->d2h, m->part2, s), "layer 2 backward");
-    MLPCHECK(mlp_gemm_wgrad(m->a1, MLP_H, m->d2h, rows, g + MLP_OFF_W2, s), "W2 gradient");
-    MLPCHECK(mlp_gemm_dact(m->d2h, W + MLP_OFF_W2, rows, m->z1, m->d1, m->d1h, m->part1, s), "layer 1 backward");
-    MLPCHECK(mlp_gemm_wgrad(m->x16, MLP_IN, m->d1h, rows, g + MLP_OFF_W1, s), "W1 gradient");
-    MLPCHECK(mlp_bias_grads(m->part1, m->part2, m->parth, (rows + 63) / 64, inv_n, g, m->loss, s), "bias gradients");
+// the fused kernel's launch for `rows` rows: given rows (x16, t, mask) or, with x16 null,
+// a minibatch drawn from a replay ring (filled in by the caller)
+MlpRun run_of(pt_mlp *m, int rows) {
+    MlpRun R{};
+    R.rows = rows;
+    R.inv_n = 1.0f / (float)std::max(1, rows);
+    R.wb = m->wb;
+    R.wbt = m->wbt;
+    R.p = m->p;
+    R.z = m->zf;
+    R.part = m->partf;
+    R.iter = m->iter;
+    R.active = m->active;
+    return R;
+}
+
+// loss and gradient (into m->g, m->loss) of `rows` given rows in m->x16, m->t, m->mask
+int gradient_rows(pt_mlp *m, int rows, hipStream_t s) {
+    MlpRun R = run_of(m, rows);
+    R.x16 = m->x16;
+    R.t = m->t;
+    R.mask = m->mask;
+    MLPCHECK(mlp_fused(R, true, s), "gradient");
+    MLPCHECK(mlp_reduce_adam(m->partf, rows, R.inv_n, m->g, m->loss, 0, m->p, m->m, m->v, m->wb, m->wbt, m->lr, m->beta1, m->beta2,
+                             m->eps, m->iter, m->active, s), "gradient sum");
     return 0;
 }
 
@@ -79,7 +86,8 @@ int create(uint32_t seed, pt_mlp **out) {
         return fail(PT_ERR_DEVICE, "PT surrogate stream creation failed.");
     }
     if ((rc = nalloc(m, &m->p, MLP_PARAMS)) || (rc = nalloc(m, &m->g, MLP_PARAMS)) || (rc = nalloc(m, &m->m, MLP_PARAMS)) ||
-        (rc = nalloc(m, &m->v, MLP_PARAMS)) || (rc = nalloc(m, &m->wb, MLP_PARAMS)) || (rc = nalloc(m, &m->iter, 1)) ||
+        (rc = nalloc(m, &m->v, MLP_PARAMS)) || (rc = nalloc(m, &m->wb, MLP_PARAMS)) || (rc = nalloc(m, &m->wbt, MLP_PARAMS)) ||
+        (rc = nalloc(m, &m->iter, 1)) ||
         (rc = nalloc(m, &m->active, 1)) || (rc = nalloc(m, &m->loss, 2))) {
         destroy(m);
         return rc;
@@ -100,6 +108,7 @@ int create(uint32_t seed, pt_mlp **out) {
         hipMemset(m->g, 0, sizeof(float) * MLP_PARAMS) != hipSuccess || hipMemset(m->iter, 0, sizeof(int)) != hipSuccess ||
         hipMemcpy(m->active, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(m->loss, 0, 2 * sizeof(float)) != hipSuccess || mlp_to_bf16(m->p, m->wb, MLP_PARAMS, m->stream) != hipSuccess ||
+        mlp_to_bf16_t(m->p, m->wbt, MLP_PARAMS, m->stream) != hipSuccess ||
         hipStreamSynchronize(m->stream) != hipSuccess) {
         destroy(m);
         return fail(PT_ERR_DEVICE, "PT surrogate initialisation failed.");
@@ -119,36 +128,40 @@ void destroy(pt_mlp *m) {
 int ensure_rows(pt_mlp *m, int rows) {
     if (rows <= m->rows_cap) return 0;
     const int cap = (std::max(rows, 256) + 63) / 64 * 64;
-    const size_t nh = (size_t)cap * MLP_H;
+    const size_t blocks = (size_t)cap / 32;                  // 32-row blocks of the fused kernel
     int rc;
-    if ((rc = nalloc(m, &m->x16, (size_t)cap * MLP_IN)) || (rc = nalloc(m, &m->a1, nh)) || (rc = nalloc(m, &m->a2, nh)) ||
-        (rc = nalloc(m, &m->a3, nh)) || (rc = nalloc(m, &m->d1h, nh)) || (rc = nalloc(m, &m->d2h, nh)) || (rc = nalloc(m, &m->d3h, nh)) ||
-        (rc = nalloc(m, &m->z1, nh)) || (rc = nalloc(m, &m->z2, nh)) || (rc = nalloc(m, &m->z3, nh)) || (rc = nalloc(m, &m->d1, nh)) ||
-        (rc = nalloc(m, &m->d2, nh)) || (rc = nalloc(m, &m->d3, nh)) || (rc = nalloc(m, &m->y, cap)) || (rc = nalloc(m, &m->t, cap)) ||
-        (rc = nalloc(m, &m->mask, cap)) || (rc = nalloc(m, &m->part1, (size_t)(cap / 64) * MLP_H)) ||
-        (rc = nalloc(m, &m->part2, (size_t)(cap / 64) * MLP_H)) || (rc = nalloc(m, &m->parth, (size_t)(cap / 64) * (2 * MLP_H + 2))))
+    if ((rc = nalloc(m, &m->x16, (size_t)cap * MLP_IN)) || (rc = nalloc(m, &m->y, cap)) || (rc = nalloc(m, &m->t, cap)) ||
+        (rc = nalloc(m, &m->mask, cap)) || (rc = nalloc(m, &m->zf, blocks * 3 * 32 * MLP_H)) ||
+        (rc = nalloc(m, &m->partf, blocks * kMlpPart)))
         return rc;
     m->rows_cap = cap;
     return 0;
 }
 
 int infer(pt_mlp *m, const uint16_t *x16, int rows, float *y, hipStream_t s) {
-    const uint16_t *W = m->wb;
-    MLPCHECK(mlp_gemm_forward(x16, W + MLP_OFF_W1, rows, MLP_IN, m->p + MLP_OFF_B1, m->z1, m->a1, s), "layer 1");
-    MLPCHECK(mlp_gemm_forward(m->a1, W + MLP_OFF_W2, rows, MLP_H, m->p + MLP_OFF_B2, m->z2, m->a2, s), "layer 2");
-    MLPCHECK(mlp_gemm_forward(m->a2, W + MLP_OFF_W3, rows, MLP_H, m->p + MLP_OFF_B3, m->z3, m->a3, s), "layer 3");
-    MLPCHECK(mlp_head(m->a3, m->z3, W + MLP_OFF_W4, m->p + MLP_OFF_B4, rows, y, nullptr, nullptr, 0.f, nullptr, nullptr, nullptr,
-                      s), "head");
+    MlpRun R = run_of(m, rows);
+    R.x16 = x16;
+    R.y = y;
+    MLPCHECK(mlp_fused(R, false, s), "inference");
     return 0;
 }
 
 int train_iter(pt_mlp *m, const uint16_t *ring_x, const float *ring_t, int cap, const double *step_ptr, int chains,
                int fixed_count, int min_fill, uint32_t seed, int batch, hipStream_t s) {
-    MLPCHECK(mlp_gather(ring_x, ring_t, cap, step_ptr, chains, fixed_count, min_fill, seed, m->iter, batch, m->x16, m->t, m->mask,
-                        m->active, s), "minibatch");
-    int rc = fwd_bwd(m, batch, true, s);
-    if (rc) return rc;
-    MLPCHECK(mlp_adam(m->p, m->g, m->m, m->v, m->wb, MLP_PARAMS, m->lr, m->beta1, m->beta2, m->eps, m->iter, m->active, s), "Adam");
+    // one fused launch (minibatch drawn from the ring, forward, backward, per-block
+    // gradients), one for the ordered gradient sum and Adam, one for the step count
+    MlpRun R = run_of(m, batch);
+    R.ring_x = ring_x;
+    R.ring_t = ring_t;
+    R.cap = cap;
+    R.step_ptr = step_ptr;
+    R.chains = chains;
+    R.fixed_count = fixed_count;
+    R.min_fill = min_fill;
+    R.seed = seed;
+    MLPCHECK(mlp_fused(R, true, s), "training");
+    MLPCHECK(mlp_reduce_adam(m->partf, batch, R.inv_n, m->g, m->loss, 1, m->p, m->m, m->v, m->wb, m->wbt, m->lr, m->beta1, m->beta2,
+                             m->eps, m->iter, m->active, s), "Adam");
     return 0;
 }
 
@@ -192,6 +205,7 @@ int pt_mlp_set_params(pt_mlp *net, const float *params) {
     MLPCHECK(hipMemsetAsync(net->v, 0, sizeof(float) * MLP_PARAMS, net->stream), "reset");
     MLPCHECK(hipMemsetAsync(net->iter, 0, sizeof(int), net->stream), "reset");
     MLPCHECK(mlp_to_bf16(net->p, net->wb, MLP_PARAMS, net->stream), "BF16 copy");
+    MLPCHECK(mlp_to_bf16_t(net->p, net->wbt, MLP_PARAMS, net->stream), "BF16 copy");
     MLPCHECK(hipStreamSynchronize(net->stream), "upload");
     return 0;
 }
@@ -228,7 +242,7 @@ int pt_mlp_gradient(pt_mlp *net, const float *x, const float *t, const float *ma
     if (!mask) { ones.assign(n, 1.0f); mask = ones.data(); }
     MLPCHECK(hipMemcpyAsync(net->t, t, sizeof(float) * n, hipMemcpyHostToDevice, net->stream), "upload");
     MLPCHECK(hipMemcpyAsync(net->mask, mask, sizeof(float) * n, hipMemcpyHostToDevice, net->stream), "upload");
-    if ((rc = fwd_bwd(net, n, true, net->stream))) return rc;
+    if ((rc = gradient_rows(net, n, net->stream))) return rc;
     MLPCHECK(hipMemcpyAsync(grad, net->g, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost, net->stream), "readback");
     float l = 0;
     MLPCHECK(hipMemcpyAsync(&l, net->loss, sizeof(float), hipMemcpyDeviceToHost, net->stream), "readback");

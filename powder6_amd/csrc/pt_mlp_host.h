@@ -15,16 +15,15 @@ struct pt_mlp {
     std::vector<void *> allocs;
     // parameters, gradients, Adam moments (FP32, MLP_PARAMS each), BF16 copy
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-    uint16_t *wb = nullptr;
+    uint16_t *wb = nullptr, *wbt = nullptr;   // BF16 copy; the same with the weight blocks transposed
     int *iter = nullptr, *active = nullptr;   // Adam step count; minibatch active flag
     float *loss = nullptr;                    // [0] last minibatch loss, [1] its running mean
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     // activations for up to rows_cap rows
     int rows_cap = 0;
-    uint16_t *x16 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *d1h = nullptr, *d2h = nullptr, *d3h = nullptr;
-    float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
+    uint16_t *x16 = nullptr;
     float *y = nullptr, *t = nullptr, *mask = nullptr;
-    float *part1 = nullptr, *part2 = nullptr, *parth = nullptr;   // per row-block gradient partials
+    float *zf = nullptr, *partf = nullptr;    // fused kernel: pre-activations and gradient partials per 32-row block
     // host-given training set of pt_mlp_train (rows of MLP_IN BF16, targets)
     int data_cap = 0;
     uint16_t *data_x = nullptr;

@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(256) k_mlp_fused(MlpRun R) {
     __syncthreads();
     fused_forward<MLP_H>(&S.r2[0][0], kLdR, R.wbt + MLP_OFF_W3, R.p + MLP_OFF_B3, z + 2 * kTR * MLP_H, &S.r3[0][0], nullptr, TRAIN);
     __syncthreads();
-    // head (k_mlp_head's arithmetic): wavefront w takes rows 8w..8w+7, lanes the hidden units
+    // head (256 -> 1, loss, and its backward): wavefront w takes rows 8w..8w+7, lanes the hidden units
     float w4[MLP_H / 64], gw4[MLP_H / 64] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < MLP_H / 64; ++q) w4[q] = from_bf16(R.wb[MLP_OFF_W4 + lane + 64 * q]);
@@ -323,9 +323,9 @@ __global__ void __launch_bounds__(256) k_mlp_fused(MlpRun R) {
     fused_wgrad<MLP_IN>(S.xt, S.t2, part + MLP_OFF_W1);                                  // dW1 = x^T dz1
 }
 
-// g = sum of the blocks' partials in block order; loss as k_mlp_bias_grads keeps it; with
-// adam, Adam on every parameter (as k_mlp_adam) and the BF16 copies (wbt: the weight
-// blocks transposed) refreshed
+// g = sum of the blocks' partials in block order; loss[0] = the partials' loss entries summed
+// and scaled by inv_n, loss[1] its running average; with adam, Adam on every parameter and the BF16 copies (wb row-major, wbt the
+// weight blocks transposed) refreshed
 __device__ __forceinline__ int mlp_transposed_index(int i) {
     if (i < MLP_OFF_B1) return (i % MLP_H) * MLP_IN + i / MLP_H;
     if (i >= MLP_OFF_W2 && i < MLP_OFF_B2) { const int j = i - MLP_OFF_W2; return MLP_OFF_W2 + (j % MLP_H) * MLP_H + j / MLP_H; }

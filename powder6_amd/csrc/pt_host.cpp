@@ -250,6 +250,16 @@ struct pt_plan {
     int kpad = 0, np_pad = 0, row_tiles = 0, cap_mtrials = 0;
     unsigned short *d_phi = nullptr, *d_plo = nullptr, *d_ihi = nullptr, *d_ilo = nullptr;
     float *d_factor = nullptr, *d_iscale = nullptr, *d_mpart = nullptr;
+    double4 *d_mhr = nullptr;            // per representative and reflection: h R, h.t
+    float *d_mf0w = nullptr;             // per atom and reflection: (f0 + f') w0
+    float2 *d_mqk = nullptr;             // per reflection: s^2 / 4, height factor
+    int *d_mgo = nullptr;                // the start vector's cell validity
+    // the matrix form's products (its intensities run on `stream`): the first side stream
+    // when the plan has one (streams beyond the hardware queues would share one with
+    // `stream` and run after it), else a stream of its own
+    hipStream_t mstream = nullptr;
+    bool mstream_own = false;
+    hipEvent_t mjoin = nullptr;
     int2 *d_band = nullptr;
     float2 *d_yw = nullptr;
     double *d_myout = nullptr;
@@ -459,6 +469,7 @@ void pt_plan_destroy(pt_plan *pl) {
     if (pl->device >= 0) {
         (void)hipSetDevice(pl->device);
         if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+        if (pl->mstream) (void)hipStreamSynchronize(pl->mstream);
         for (void *p : pl->allocs) (void)hipFree(p);
         for (auto e : pl->ev) (void)hipEventDestroy(e);
         if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
@@ -470,6 +481,8 @@ void pt_plan_destroy(pt_plan *pl) {
             if (pl->join[p]) (void)hipEventDestroy(pl->join[p]);
         }
         if (pl->fork) (void)hipEventDestroy(pl->fork);
+        if (pl->mjoin) (void)hipEventDestroy(pl->mjoin);
+        if (pl->mstream && pl->mstream_own) (void)hipStreamDestroy(pl->mstream);
         if (pl->stream) (void)hipStreamDestroy(pl->stream);
         mlpx::destroy(pl->mlp);
     }
@@ -1683,6 +1696,12 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
         return 0;
     }
     if (pl->nr > PTK_MATRIX_MAX_REFL) return fail(PT_ERR_BUDGET, "PT shared-profile form supports at most 4096 reflections.");
+    if (!pl->mstream) {
+        if (pl->side[0]) pl->mstream = pl->side[0];
+        else if (hipStreamCreateWithFlags(&pl->mstream, hipStreamNonBlocking) == hipSuccess) pl->mstream_own = true;
+        else return fail(PT_ERR_DEVICE, "PT stream creation failed.");
+        if (hipEventCreateWithFlags(&pl->mjoin, hipEventDisableTiming) != hipSuccess) return fail(PT_ERR_DEVICE, "PT event creation failed.");
+    }
     int rc = ensure_trials(pl, 1);
     if (rc || (rc = ensure_slots(pl, std::max(1, pl->chains)))) return rc;
     pl->kpad = (pl->nr + 31) / 32 * 32;           // 32-reflection slices
@@ -1697,6 +1716,20 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
     HIPCHECK(hipMemcpyAsync(pl->d_x, pl->x0.data(), sizeof(double) * pl->nv, hipMemcpyHostToDevice, pl->stream), "upload");
     HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x, pl->d_img, 1, 0, 0, pl->stream), "setup");
     HIPCHECK(ptk::launch_matrix_columns(pl->P, pl->d_img, pl->d_mrec, pl->d_mspan, pl->d_factor, pl->stream), "matrix columns");
+    {
+        int nrep_max = 1, na = 0;
+        for (int q = 0; q < pl->P.n_phases; ++q) {
+            nrep_max = std::max(nrep_max, pl->P.phase[q].n_reps);
+            na = std::max(na, pl->P.phase[q].atom_off + pl->P.phase[q].n_atoms);
+        }
+        if ((rc = dalloc(pl, &pl->d_mhr, (size_t)nrep_max * pl->kpad)) ||
+            (rc = dalloc(pl, &pl->d_mf0w, (size_t)std::max(1, na) * pl->kpad)) || (rc = dalloc(pl, &pl->d_mqk, (size_t)pl->kpad)) ||
+            (rc = dalloc(pl, &pl->d_mgo, 1)))
+            return rc;
+        HIPCHECK(ptk::launch_matrix_tables(pl->P, pl->d_img, pl->d_factor, pl->d_mhr, pl->d_mf0w, pl->d_mqk, pl->d_mgo, pl->kpad,
+                                           nrep_max, pl->stream),
+                 "matrix tables");
+    }
     std::vector<int4> spans(2 * (size_t)pl->nr);
     HIPCHECK(hipMemcpyAsync(spans.data(), pl->d_mspan, sizeof(int4) * spans.size(), hipMemcpyDeviceToHost, pl->stream), "readback");
     if ((rc = matrix_fill(pl, bf16))) return rc;
@@ -1729,14 +1762,19 @@ static int matrix_prepare(pt_plan *pl, int bf16) {
 static int ensure_matrix_trials(pt_plan *pl, int tpad) {
     if (tpad <= pl->cap_mtrials) return 0;
     int rc;
-    if ((rc = dalloc(pl, &pl->d_ihi, (size_t)tpad * pl->kpad)) || (rc = dalloc(pl, &pl->d_ilo, (size_t)tpad * pl->kpad)) ||
-        (rc = dalloc(pl, &pl->d_iscale, (size_t)tpad)) || (rc = dalloc(pl, &pl->d_mpart, (size_t)tpad * pl->row_tiles)))
+    // the pairs and scales twice: chunk p + 1's intensities are formed while chunk p's
+    // product reads the other buffer
+    if ((rc = dalloc(pl, &pl->d_ihi, 2 * (size_t)tpad * pl->kpad)) || (rc = dalloc(pl, &pl->d_ilo, 2 * (size_t)tpad * pl->kpad)) ||
+        (rc = dalloc(pl, &pl->d_iscale, 2 * (size_t)tpad)) || (rc = dalloc(pl, &pl->d_mpart, (size_t)tpad * pl->row_tiles)))
         return rc;
     pl->cap_mtrials = tpad;
     return 0;
 }
 
-static const int kMatrixChunk = 4096;
+// trials per pass of the matrix form: large grids amortise each launch's ramp and tail
+// (4096-trial passes ran 36% slower per trial); passes beyond one overlap the next pass's
+// intensities with the current product
+static const int kMatrixChunk = 32768;
 static const int kMatrixYout = 64;      // patterns one eval_matrix call may return (d_myout rows)
 
 // chi^2 of ntrials structure-only vectors through Y = P I; yout (optional, ntrials <= 64)
@@ -1753,34 +1791,44 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
     if ((rc = ensure_trials(pl, ntrials))) return rc;
     int chunk = std::min(ntrials, kMatrixChunk);
     int tpad_max = (chunk + 63) / 64 * 64;                 // whole 64-trial product tiles
-    if ((rc = ensure_slots(pl, chunk)) || (rc = ensure_matrix_trials(pl, tpad_max))) return rc;
+    if ((rc = ensure_matrix_trials(pl, tpad_max))) return rc;
     if (yout && !pl->d_myout && (rc = dalloc(pl, &pl->d_myout, (size_t)kMatrixYout * pl->np))) return rc;
     int passes = (ntrials + chunk - 1) / chunk;
     if ((rc = ensure_events(pl, 2 * passes))) return rc;
+    // two streams: the intensities of chunk p (on `stream`) overlap the product and chi^2
+    // of chunk p - 1 (on mstream); the pair buffers alternate between chunks
     HIPCHECK(hipMemcpyAsync(pl->d_x, x, sizeof(double) * ntrials * pl->nv, hipMemcpyHostToDevice, pl->stream), "trial upload");
     HIPCHECK(hipEventRecord(pl->ev_total[0], pl->stream), "timing");
+    HIPCHECK(hipStreamWaitEvent(pl->mstream, pl->ev_total[0], 0), "fork");
+    const size_t bstride = (size_t)pl->cap_mtrials * pl->kpad;
     for (int p = 0; p < passes; ++p) {
         int t0 = p * chunk, n = std::min(chunk, ntrials - t0), tpad = (n + 63) / 64 * 64;
+        const int bsel = p & 1;
+        unsigned short *ihi = pl->d_ihi + bsel * bstride, *ilo = pl->d_ilo + bsel * bstride;
+        float *isc = pl->d_iscale + bsel * (size_t)pl->cap_mtrials;
+        if (p >= 2) HIPCHECK(hipStreamWaitEvent(pl->stream, pl->ev[4 * (p - 2) + 3], 0), "buffer reuse");
         if (tpad > n) {
-            // the intensity kernels write every reflection slot of the n trials; the padding
+            // the intensity kernel writes every reflection slot of the n trials; the padding
             // trials of the last product tile stay zero
             const size_t off = (size_t)n * pl->kpad, cnt = sizeof(unsigned short) * (size_t)(tpad - n) * pl->kpad;
-            HIPCHECK(hipMemsetAsync(pl->d_ihi + off, 0, cnt, pl->stream), "clear");
-            HIPCHECK(hipMemsetAsync(pl->d_ilo + off, 0, cnt, pl->stream), "clear");
+            HIPCHECK(hipMemsetAsync(ihi + off, 0, cnt, pl->stream), "clear");
+            HIPCHECK(hipMemsetAsync(ilo + off, 0, cnt, pl->stream), "clear");
         }
-        HIPCHECK(ptk::launch_stage_setup(pl->P, pl->d_x + (size_t)t0 * pl->nv, pl->d_img, n, 0, 0, pl->stream), "setup");
         HIPCHECK(hipEventRecord(pl->ev[4 * p], pl->stream), "timing");
-        HIPCHECK(ptk::launch_matrix_intensity(pl->P, pl->d_img, pl->d_factor, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->kpad, 0, n,
-                                              bf16, pl->stream), "intensities");
+        HIPCHECK(ptk::launch_matrix_intensity(pl->P, pl->d_x + (size_t)t0 * pl->nv, pl->d_mhr, pl->d_mf0w, pl->d_mqk, ihi, ilo, isc,
+                                              pl->kpad, n, bf16, pl->stream),
+                 "intensities");
         HIPCHECK(hipEventRecord(pl->ev[4 * p + 1], pl->stream), "timing");
-        HIPCHECK(hipEventRecord(pl->ev[4 * p + 2], pl->stream), "timing");
-        HIPCHECK(ptk::launch_matrix_chi2(pl->P, pl->d_phi, pl->d_plo, pl->d_ihi, pl->d_ilo, pl->d_iscale, pl->d_band, pl->d_yw,
-                                         pl->d_mpart, pl->kpad, tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, n, bf16,
-                                         pl->stream),
+        HIPCHECK(hipStreamWaitEvent(pl->mstream, pl->ev[4 * p + 1], 0), "intensities ready");
+        HIPCHECK(hipEventRecord(pl->ev[4 * p + 2], pl->mstream), "timing");
+        HIPCHECK(ptk::launch_matrix_chi2(pl->P, pl->d_phi, pl->d_plo, ihi, ilo, isc, pl->d_band, pl->d_yw, pl->d_mpart, pl->kpad,
+                                         tpad, pl->row_tiles, yout ? pl->d_myout : nullptr, n, bf16, pl->mstream),
                  "matrix product");
-        HIPCHECK(hipEventRecord(pl->ev[4 * p + 3], pl->stream), "timing");
-        HIPCHECK(ptk::launch_matrix_reduce(pl->d_mpart, pl->row_tiles, tpad, n, pl->d_img, pl->d_chi2 + t0, pl->stream), "reduce");
+        HIPCHECK(hipEventRecord(pl->ev[4 * p + 3], pl->mstream), "timing");
+        HIPCHECK(ptk::launch_matrix_reduce(pl->d_mpart, pl->row_tiles, tpad, n, pl->d_mgo, pl->d_chi2 + t0, pl->mstream), "reduce");
     }
+    HIPCHECK(hipEventRecord(pl->mjoin, pl->mstream), "join");
+    HIPCHECK(hipStreamWaitEvent(pl->stream, pl->mjoin, 0), "join");
     HIPCHECK(hipEventRecord(pl->ev_total[1], pl->stream), "timing");
     HIPCHECK(hipMemcpyAsync(chi2, pl->d_chi2, sizeof(double) * ntrials, hipMemcpyDeviceToHost, pl->stream), "readback");
     if (yout)

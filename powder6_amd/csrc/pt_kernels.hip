@@ -2362,208 +2362,274 @@ __global__ void __launch_bounds__(256) k_matrix_fill(DevPlan P, const LineRec *_
     }
 }
 
-// I[k, t] = S m Lp PO |F|^2 / area of every reflection of trial t, as an FP16 hi/lo pair
-// scaled by 2^-e_t with max_k |I| 2^-e_t <= 2^14; iscale[t] = 2^e_t.
-__global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const TrialImage *__restrict__ img,
-                                                          const float *__restrict__ factor, unsigned short *ihi,
-                                                          unsigned short *ilo, float *iscale, int kpad, int t0, int bf16) {
-    __shared__ Shared sh;
-    __shared__ float ival[PTK_MATRIX_MAX_REFL];
-    __shared__ float wmax[16];
-    const int tl = blockIdx.x, t = t0 + tl;
-    stage_reps(P, sh);
-    image_load(P, img + tl, sh);
+// Per-reflection constants of the shared-profile intensities, formed once from the start
+// vector's image (every trial of the form has the start vector's cell, so 1/d^2 and the
+// form factors are shared):
+//   hr[r kpad + k]  = (h R_r, h . t_r) in FP64 for representative r of k's phase (zero past
+//                     the phase's n_reps),
+//   f0w[ga kpad + k] = (f0 + f') w0 of atom ga at k (the neutron length times w0), zero for
+//                     the atoms of other phases,
+//   qk[k]           = (s^2 / 4, the height factor m Lp PO / area, or 0 when k's phase cell
+//                     is invalid),
+//   *go             = the start vector's cell validity (a trial of the form goes when it does).
+__global__ void __launch_bounds__(256) k_matrix_tables(DevPlan P, const TrialImage *__restrict__ img,
+                                                       const float *__restrict__ factor, double4 *hr, float *f0w, float2 *qk,
+                                                       int *go, int kpad, int nrep_max) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *go = img->go;
+    if (k >= kpad) return;
+    const int na = total_atoms(P);
+    if (k >= P.nr) {
+        qk[k] = make_float2(0.f, 0.f);
+        for (int r = 0; r < nrep_max; ++r) hr[(size_t)r * kpad + k] = make_double4(0.0, 0.0, 0.0, 0.0);
+        for (int ga = 0; ga < na; ++ga) f0w[(size_t)ga * kpad + k] = 0.f;
+        return;
+    }
+    int q = 0;
+    while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
+    const DevPhase &ph = P.phase[q];
+    const int4 hk = P.hklm[k];
+    const double *f = img->metric[q];
+    const double h = hk.x, kx = hk.y, l = hk.z;
+    const double qv = h * h * f[0] + kx * kx * f[1] + l * l * f[2] + h * kx * f[3] + kx * l * f[4] + h * l * f[5];
+    const float q4 = 0.25f * (float)qv;
+    qk[k] = make_float2(q4, img->phase_ok[q] ? factor[k] : 0.0f);
+    for (int r = 0; r < nrep_max; ++r) {
+        double4 e = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (r < ph.n_reps) {
+            const int *R = P.rep_r + 9 * (ph.rep_off + r);
+            const double *t = P.rep_t + 3 * (ph.rep_off + r);
+            e.x = hk.x * R[0] + hk.y * R[3] + hk.z * R[6];
+            e.y = hk.x * R[1] + hk.y * R[4] + hk.z * R[7];
+            e.z = hk.x * R[2] + hk.y * R[5] + hk.z * R[8];
+            e.w = (double)hk.x * t[0] + (double)hk.y * t[1] + (double)hk.z * t[2];
+        }
+        hr[(size_t)r * kpad + k] = e;
+    }
+    for (int ga = 0; ga < na; ++ga) {
+        float v = 0.f;
+        if (ga >= ph.atom_off && ga < ph.atom_off + ph.n_atoms) {
+            const DevAtom &A = P.atoms[ga];
+            float f0;
+            if (P.radiation == 1) f0 = A.b_neutron;
+            else {
+                f0 = A.ff_c;
+                for (int g = 0; g < A.n_gauss; ++g) f0 += A.ff_a[g] * __expf(-A.ff_b[g] * q4);
+                f0 += A.fp;
+            }
+            v = f0 * A.w0;
+        }
+        f0w[(size_t)ga * kpad + k] = v;
+    }
+}
+
+// model parameter pi of the trial vector x: base + M x (the materialize sum)
+__device__ __forceinline__ double matrix_param(const DevPlan &P, const double *x, int pi) {
+    double v = P.base[pi];
+    for (int t = P.term_off[pi]; t < P.term_off[pi + 1]; ++t) v += P.term_coeff[t] * x[P.term_var[t]];
+    return v;
+}
+
+// I[k, t] = S m Lp PO |F|^2 / area of every reflection k of kMxT trials per workgroup,
+// formed straight from the trial vectors (only the structure parameters are materialised:
+// positions, occupancies, B, the phase scales), as FP16 (or BF16) hi/lo pairs scaled by
+// 2^-e_t with max_k |I| 2^-e_t <= 2^14; iscale[t] = 2^e_t.  A thread takes reflections k,
+// k + blockDim, ... for the workgroup's trials (blockDim is sized so every thread has the
+// same count and the whole grid is resident at once); per atom it sums the
+// representatives' cos (and sin) of the turn h R r + h.t, FP64 phases reduced to [0, 1),
+// with h R and h.t read from the table four representatives at a time (coalesced 32-byte
+// loads, in flight together, shared by the trials); the atoms' weights occ dw f0 w0 then
+// multiply the sums (structure_f2's order: atoms outer, representatives inner).  The
+// intensities wait in LDS for the trial's maximum, then go out as pairs.
+//
+// Dynamic LDS, na atoms, nv variables: xs [kMxT][nv] f64 (the trial vectors), apos
+// [kMxT][na][3] f64, aocc [kMxT][na], aB [kMxT][na], afi [na] (f'' w0),
+// S [kMxT][PTK_MAX_PHASES], wmax [kMxT][8], ival [kMxT][kpad] floats.
+constexpr int kMxT = 4;
+__host__ __device__ constexpr size_t matrix_intensity_lds(int na, int nv, int kpad) {
+    return sizeof(double) * ((size_t)kMxT * nv + 3 * (size_t)kMxT * na) +
+           sizeof(float) * ((size_t)2 * kMxT * na + na + (size_t)kMxT * PTK_MAX_PHASES + 8 * kMxT + (size_t)kMxT * kpad);
+}
+
+template <bool CENTRO>
+__device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos, const float *aocc, const float *aB,
+                                          const float *afi, int na, const double4 *__restrict__ hr, const float *__restrict__ f0w,
+                                          int k, int kpad, float q4, float *f2) {
+    float Ar[kMxT], Br[kMxT], Cr[kMxT], Dr[kMxT];
+#pragma unroll
+    for (int tt = 0; tt < kMxT; ++tt) Ar[tt] = Br[tt] = Cr[tt] = Dr[tt] = 0.f;
+    const int nrep = ph.n_reps;
+    // the first four representatives' h R, h.t are loaded once for all atoms
+    double4 e4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) e4[i] = 4 <= nrep ? hr[(size_t)i * kpad + k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int j = 0; j < ph.n_atoms; ++j) {
+        const int ga = ph.atom_off + j;
+        const float fw = f0w[(size_t)ga * kpad + k], fi = afi[ga];
+        double px[kMxT], py[kMxT], pz[kMxT];
+        float ca[kMxT], sa[kMxT];
+#pragma unroll
+        for (int tt = 0; tt < kMxT; ++tt) {
+            const double *a = apos + 3 * ((size_t)tt * na + ga);
+            px[tt] = a[0]; py[tt] = a[1]; pz[tt] = a[2];
+            ca[tt] = sa[tt] = 0.f;
+        }
+        // whole groups of four representatives: sixteen independent phase chains, summed
+        // afterwards in representative order
+        int r = 0;
+        for (; r + 4 <= nrep; r += 4) {
+            double4 e[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) e[i] = r == 0 ? e4[i] : hr[(size_t)(r + i) * kpad + k];
+            float cv[4][kMxT], sv[4][kMxT];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int tt = 0; tt < kMxT; ++tt) {
+                    const double phs = fma(e[i].x, px[tt], fma(e[i].y, py[tt], fma(e[i].z, pz[tt], e[i].w)));
+                    const float fr = (float)__builtin_amdgcn_fract(phs);
+                    cv[i][tt] = __builtin_amdgcn_cosf(fr);
+                    sv[i][tt] = CENTRO ? 0.f : __builtin_amdgcn_sinf(fr);
+                }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int tt = 0; tt < kMxT; ++tt) {
+                    ca[tt] += cv[i][tt];
+                    if (!CENTRO) sa[tt] += sv[i][tt];
+                }
+        }
+        for (; r < nrep; ++r) {
+            const double4 e = hr[(size_t)r * kpad + k];
+#pragma unroll
+            for (int tt = 0; tt < kMxT; ++tt) {
+                const double phs = fma(e.x, px[tt], fma(e.y, py[tt], fma(e.z, pz[tt], e.w)));
+                const float fr = (float)__builtin_amdgcn_fract(phs);
+                ca[tt] += __builtin_amdgcn_cosf(fr);
+                if (!CENTRO) sa[tt] += __builtin_amdgcn_sinf(fr);
+            }
+        }
+#pragma unroll
+        for (int tt = 0; tt < kMxT; ++tt) {
+            const float od = aocc[tt * na + ga] * __expf(-aB[tt * na + ga] * q4);
+            const float wj = od * fw, wi = od * fi;
+            Ar[tt] += wj * ca[tt]; Cr[tt] += wi * ca[tt];
+            if (!CENTRO) { Br[tt] += wj * sa[tt]; Dr[tt] += wi * sa[tt]; }
+        }
+    }
+#pragma unroll
+    for (int tt = 0; tt < kMxT; ++tt)
+        f2[tt] = CENTRO ? 4.0f * (Ar[tt] * Ar[tt] + Cr[tt] * Cr[tt])
+                        : Ar[tt] * Ar[tt] + Br[tt] * Br[tt] + Cr[tt] * Cr[tt] + Dr[tt] * Dr[tt];
+}
+
+// X: the chunk's n trial vectors (nv each)
+__global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const double *__restrict__ X,
+                                                          const double4 *__restrict__ hr, const float *__restrict__ f0w,
+                                                          const float2 *__restrict__ qk, unsigned short *ihi, unsigned short *ilo,
+                                                          float *iscale, int kpad, int n, int bf16) {
+    extern __shared__ __align__(16) unsigned char mx_raw[];
+    const int tid = threadIdx.x, nt = blockDim.x, tb = blockIdx.x * kMxT;
+    const int na = total_atoms(P);
+    double *xs = reinterpret_cast<double *>(mx_raw);
+    double *apos = xs + (size_t)kMxT * P.nv;
+    float *aocc = reinterpret_cast<float *>(apos + 3 * (size_t)kMxT * na);
+    float *aB = aocc + kMxT * na, *afi = aB + kMxT * na, *Sx = afi + na, *wmax = Sx + kMxT * PTK_MAX_PHASES;
+    float *ival = wmax + 8 * kMxT;
+    // the workgroup's trial vectors into LDS first (their loads do not wait on the
+    // parameter map's)
+    for (int i = tid; i < kMxT * P.nv; i += nt) {
+        const int tt = i / P.nv;
+        xs[i] = tb + tt < n ? X[(size_t)tb * P.nv + i] : 0.0;
+    }
     __syncthreads();
-    float mx = 0.0f;
-    for (int k = threadIdx.x; k < P.nr; k += blockDim.x) {
-        float v = 0.0f;
-        const float fk = factor[k];
-        if (sh.go && fk != 0.0f) {
+    // the trials' structure parameters, one (trial, atom, slot) per thread: the coordinate
+    // slots wait in apos, occupancy and B go to aocc / aB
+    for (int i = tid; i < kMxT * na * 5; i += nt) {
+        const int tt = i / (na * 5), ga = (i / 5) % na, f = i % 5;
+        double v = 0.0;
+        if (tb + tt < n) {
+            int q = 0;
+            while (q + 1 < P.n_phases && ga >= P.phase[q + 1].atom_off) ++q;
+            const DevPhase &ph = P.phase[q];
+            if (f >= 3 || (P.atoms[ga].free_mask & (1 << f)))
+                v = matrix_param(P, xs + (size_t)tt * P.nv, ph.param_base + PTK_PH_ATOM0 + 5 * (ga - ph.atom_off) + f);
+        }
+        if (f < 3) apos[3 * ((size_t)tt * na + ga) + f] = v;
+        else if (f == 3) aocc[tt * na + ga] = (float)v;
+        else aB[tt * na + ga] = (float)v;
+    }
+    for (int i = tid; i < kMxT * P.n_phases; i += nt) {
+        const int tt = i / P.n_phases, q = i % P.n_phases;
+        float S = 0.f;
+        if (tb + tt < n) {
+            S = (float)matrix_param(P, xs + (size_t)tt * P.nv, P.phase[q].param_base + 6);
+            if (!(S != 0.0f) || isnan(S)) S = 0.0f;
+        }
+        Sx[tt * PTK_MAX_PHASES + q] = S;
+    }
+    for (int ga = tid; ga < na; ga += nt) {
+        const DevAtom &A = P.atoms[ga];
+        afi[ga] = (P.radiation == 1 || !(A.fpp != 0.f)) ? 0.f : A.fpp * A.w0;
+    }
+    __syncthreads();
+    // positions r0 + sum over the free slots of v basis (trial_setup's arithmetic)
+    for (int i = tid; i < kMxT * na; i += nt) {
+        const int ga = i % na;
+        double *a = apos + 3 * (size_t)i;
+        const DevAtom &A = P.atoms[ga];
+        double r[3] = {A.r0[0], A.r0[1], A.r0[2]};
+        for (int f = 0; f < 3; ++f)
+            if (A.free_mask & (1 << f)) {
+                const double v = a[f];
+                r[0] += v * A.basis[f][0]; r[1] += v * A.basis[f][1]; r[2] += v * A.basis[f][2];
+            }
+        a[0] = r[0]; a[1] = r[1]; a[2] = r[2];
+    }
+    __syncthreads();
+    float mx[kMxT] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = tid; k < P.nr; k += nt) {
+        const float2 qf = qk[k];
+        float v[kMxT] = {0.f, 0.f, 0.f, 0.f};
+        if (qf.y != 0.0f) {
             int q = 0;
             while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
             const DevPhase &ph = P.phase[q];
-            const int4 hk = P.hklm[k];
-            const double *f = sh.metric[q];
-            const double h = hk.x, kk = hk.y, l = hk.z;
-            const double qv = h * h * f[0] + kk * kk * f[1] + l * l * f[2] + h * kk * f[3] + kk * l * f[4] + h * l * f[5];
-            double S = sh.p[ph.param_base + 6];
-            if (!(S != 0.0) || isnan(S)) S = 0.0;
-            v = (float)S * (fk * structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv));
-        }
-        ival[k] = v;
-        mx = fmaxf(mx, fabsf(v));
-    }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = 0.0f;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wmax[w]);
-        int e = 0;
-        if (m > 0.0f && isfinite(m)) { frexpf(m, &e); e -= 14; }
-        wmax[0] = ldexpf(1.0f, e);
-    }
-    __syncthreads();
-    const float sc = wmax[0], inv = 1.0f / sc;
-    for (int k = threadIdx.x; k < kpad; k += blockDim.x) {
-        unsigned short h, l;
-        split_pair(k < P.nr ? ival[k] * inv : 0.0f, bf16 != 0, h, l);
-        ihi[(size_t)t * kpad + k] = h;
-        ilo[(size_t)t * kpad + k] = l;
-    }
-    if (threadIdx.x == 0) iscale[t] = sc;
-}
-
-// Intensities of kMxTrials trials per workgroup (patterns with at most 4 x 256
-// reflections): only structure variables differ between the trials of the shared-profile
-// form, so the metric, the form factors and h R, h.t of every representative are formed
-// once per reflection and used for all the workgroup's trials.  Per trial and reflection
-// the sums run in the order structure_f2 takes (atoms in order, representatives inside).
-constexpr int kMxTrials = 4;
-constexpr int kMxRefPerThread = 4;
-struct MxShared {
-    int rep_r[PTK_MAX_REPS_TOTAL][9];
-    double rep_t[PTK_MAX_REPS_TOTAL][3];
-    float aff[PTK_MAX_ATOMS][12];
-    float afp[PTK_MAX_ATOMS][2];
-    double metric[PTK_MAX_PHASES][6];
-    int phase_ok[PTK_MAX_PHASES];
-    double apos[kMxTrials][PTK_MAX_ATOMS][3];
-    float aocc[kMxTrials][PTK_MAX_ATOMS], aB[kMxTrials][PTK_MAX_ATOMS];
-    float S[kMxTrials][PTK_MAX_PHASES];
-    int go[kMxTrials];
-    float wmax[kMxTrials][4];
-};
-
-__global__ void __launch_bounds__(256) k_matrix_intensity4(DevPlan P, const TrialImage *__restrict__ img,
-                                                           const float *__restrict__ factor, unsigned short *ihi,
-                                                           unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16) {
-    __shared__ MxShared M;
-    const int tid = threadIdx.x, tb = blockIdx.x * kMxTrials;
-    const int na = total_atoms(P);
-    int nrep = 0;
-    for (int q = 0; q < P.n_phases; ++q) nrep += P.phase[q].n_reps;
-    for (int i = tid; i < nrep * 9; i += blockDim.x) M.rep_r[i / 9][i % 9] = P.rep_r[i];
-    for (int i = tid; i < nrep * 3; i += blockDim.x) M.rep_t[i / 3][i % 3] = P.rep_t[i];
-    for (int j = tid; j < na; j += blockDim.x) {
-        const DevAtom &A = P.atoms[j];
-        for (int g = 0; g < 5; ++g) {
-            M.aff[j][g] = g < A.n_gauss ? A.ff_a[g] : 0.0f;
-            M.aff[j][5 + g] = g < A.n_gauss ? A.ff_b[g] : 0.0f;
-        }
-        M.aff[j][10] = A.ff_c;
-        M.aff[j][11] = A.w0;
-        M.afp[j][0] = P.radiation == 1 ? A.b_neutron : A.fp;
-        M.afp[j][1] = P.radiation == 1 ? 0.0f : A.fpp;
-    }
-    // the shared metric from the first trial (every trial has the start vector's cell)
-    for (int i = tid; i < P.n_phases * 6; i += blockDim.x) M.metric[i / 6][i % 6] = img[tb].metric[i / 6][i % 6];
-    if (tid < P.n_phases) M.phase_ok[tid] = img[tb].phase_ok[tid];
-    for (int tt = 0; tt < kMxTrials; ++tt) {
-        const bool live = tb + tt < n;
-        const TrialImage *im = img + (live ? tb + tt : tb);
-        for (int i = tid; i < na * 3; i += blockDim.x) M.apos[tt][i / 3][i % 3] = im->apos[i / 3][i % 3];
-        for (int i = tid; i < na; i += blockDim.x) { M.aocc[tt][i] = im->aocc[i]; M.aB[tt][i] = im->aB[i]; }
-        if (tid < P.n_phases) M.S[tt][tid] = (float)im->p[P.phase[tid].param_base + 6];
-        if (tid == 0) M.go[tt] = live && im->go;
-    }
-    __syncthreads();
-    float v[kMxRefPerThread][kMxTrials];
-    float mx[kMxTrials] = {0.f, 0.f, 0.f, 0.f};
+            float f2[kMxT];
+            if (ph.centro) matrix_f2<true>(ph, apos, aocc, aB, afi, na, hr, f0w, k, kpad, qf.x, f2);
+            else matrix_f2<false>(ph, apos, aocc, aB, afi, na, hr, f0w, k, kpad, qf.x, f2);
 #pragma unroll
-    for (int kk = 0; kk < kMxRefPerThread; ++kk) {
-        const int k = tid + kk * (int)blockDim.x;
-#pragma unroll
-        for (int tt = 0; tt < kMxTrials; ++tt) v[kk][tt] = 0.0f;
-        if (k >= P.nr) continue;
-        const float fk = factor[k];
-        int q = 0;
-        while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
-        const DevPhase &ph = P.phase[q];
-        if (fk == 0.0f || !M.phase_ok[q]) continue;
-        const int4 hk = P.hklm[k];
-        const double *f = M.metric[q];
-        const double h = hk.x, kx = hk.y, l = hk.z;
-        const double qv = h * h * f[0] + kx * kx * f[1] + l * l * f[2] + h * kx * f[3] + kx * l * f[4] + h * l * f[5];
-        const float q4 = 0.25f * (float)qv;
-        float Ar[kMxTrials], Br[kMxTrials], Cr[kMxTrials], Dr[kMxTrials];
-#pragma unroll
-        for (int tt = 0; tt < kMxTrials; ++tt) Ar[tt] = Br[tt] = Cr[tt] = Dr[tt] = 0.f;
-        const bool centro = ph.centro != 0;
-        for (int j = 0; j < ph.n_atoms; ++j) {
-            const int ga = ph.atom_off + j;
-            float ca[kMxTrials], sa[kMxTrials];
-#pragma unroll
-            for (int tt = 0; tt < kMxTrials; ++tt) ca[tt] = sa[tt] = 0.f;
-            for (int r = 0; r < ph.n_reps; ++r) {
-                const int *R = M.rep_r[ph.rep_off + r];
-                const double *t = M.rep_t[ph.rep_off + r];
-                const double H0 = hk.x * R[0] + hk.y * R[3] + hk.z * R[6], H1 = hk.x * R[1] + hk.y * R[4] + hk.z * R[7],
-                             H2 = hk.x * R[2] + hk.y * R[5] + hk.z * R[8];
-                const double ht = (double)hk.x * t[0] + (double)hk.y * t[1] + (double)hk.z * t[2];
-#pragma unroll
-                for (int tt = 0; tt < kMxTrials; ++tt) {
-                    const double phs = H0 * M.apos[tt][ga][0] + H1 * M.apos[tt][ga][1] + H2 * M.apos[tt][ga][2] + ht;
-                    const float fr = (float)(phs - floor(phs));
-                    ca[tt] += __builtin_amdgcn_cosf(fr);
-                    if (!centro) sa[tt] += __builtin_amdgcn_sinf(fr);
-                }
-            }
-            const float *F = M.aff[ga];
-            float f0;
-            if (P.radiation == 1) f0 = M.afp[ga][0];
-            else {
-                f0 = F[10];
-                for (int g = 0; g < 5; ++g) f0 += F[g] * __expf(-F[5 + g] * q4);
-                f0 += M.afp[ga][0];
-            }
-#pragma unroll
-            for (int tt = 0; tt < kMxTrials; ++tt) {
-                const float dw = __expf(-M.aB[tt][ga] * q4);
-                const float wj = M.aocc[tt][ga] * f0 * dw * F[11];
-                const float wi = (P.radiation == 1 || !(M.afp[ga][1] != 0.f)) ? 0.f : M.aocc[tt][ga] * M.afp[ga][1] * dw * F[11];
-                Ar[tt] += wj * ca[tt]; Cr[tt] += wi * ca[tt];
-                if (!centro) { Br[tt] += wj * sa[tt]; Dr[tt] += wi * sa[tt]; }
+            for (int tt = 0; tt < kMxT; ++tt) {
+                v[tt] = Sx[tt * PTK_MAX_PHASES + q] * (qf.y * f2[tt]);
+                mx[tt] = fmaxf(mx[tt], fabsf(v[tt]));
             }
         }
 #pragma unroll
-        for (int tt = 0; tt < kMxTrials; ++tt) {
-            if (!M.go[tt]) continue;
-            float S = M.S[tt][q];
-            if (!(S != 0.0f) || isnan(S)) S = 0.0f;
-            const float f2 = centro ? 4.0f * (Ar[tt] * Ar[tt] + Cr[tt] * Cr[tt])
-                                    : Ar[tt] * Ar[tt] + Br[tt] * Br[tt] + Cr[tt] * Cr[tt] + Dr[tt] * Dr[tt];
-            v[kk][tt] = S * (fk * f2);
-            mx[tt] = fmaxf(mx[tt], fabsf(v[kk][tt]));
-        }
+        for (int tt = 0; tt < kMxT; ++tt) ival[tt * kpad + k] = v[tt];
     }
     // per trial: a power-of-two scale that keeps the largest intensity near 2^14
 #pragma unroll
-    for (int tt = 0; tt < kMxTrials; ++tt) {
+    for (int tt = 0; tt < kMxT; ++tt) {
         float m = mx[tt];
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if ((tid & 63) == 0) M.wmax[tt][tid >> 6] = m;
+        if ((tid & 63) == 0) wmax[tt * 8 + (tid >> 6)] = m;
     }
     __syncthreads();
-    if (tid < kMxTrials) {
+    if (tid < kMxT) {
         float m = 0.0f;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, M.wmax[tid][w]);
+        for (int w = 0; w < (nt >> 6); ++w) m = fmaxf(m, wmax[tid * 8 + w]);
         int e = 0;
         if (m > 0.0f && isfinite(m)) { frexpf(m, &e); e -= 14; }
-        M.wmax[tid][0] = ldexpf(1.0f, e);
+        wmax[tid * 8] = ldexpf(1.0f, e);
     }
     __syncthreads();
-#pragma unroll
-    for (int tt = 0; tt < kMxTrials; ++tt) {
-        const int t = t0 + tb + tt;
-        if (tb + tt >= n) break;
-        const float sc = M.wmax[tt][0], inv = 1.0f / sc;
-#pragma unroll
-        for (int kk = 0; kk < kMxRefPerThread; ++kk) {
-            const int k = tid + kk * (int)blockDim.x;
-            if (k >= kpad) break;
+    for (int tt = 0; tt < kMxT; ++tt) {
+        const int t = tb + tt;
+        if (t >= n) break;
+        const float sc = wmax[tt * 8], inv = 1.0f / sc;
+        for (int k = tid; k < kpad; k += nt) {
             unsigned short hv, lv;
-            split_pair(k < P.nr ? v[kk][tt] * inv : 0.0f, bf16 != 0, hv, lv);
+            split_pair(k < P.nr ? ival[tt * kpad + k] * inv : 0.0f, bf16 != 0, hv, lv);
             ihi[(size_t)t * kpad + k] = hv;
             ilo[(size_t)t * kpad + k] = lv;
         }
@@ -2709,16 +2775,17 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
 
 // chi^2 of each trial: FP64 sum of its row-tile partials, one wavefront per trial (lane l
 // takes the row tiles l, l + 64, ... in order, then a fixed shuffle tree), so every load
-// of a trial is in flight at once and the sum is deterministic
+// of a trial is in flight at once and the sum is deterministic; go: the form's cell
+// validity (3e38 when the cell is invalid or the sum is not finite)
 __global__ void __launch_bounds__(256) k_matrix_reduce(const float *__restrict__ partial, int row_tiles, int tpad, int ntrials,
-                                                       const TrialImage *__restrict__ img, double *chi2) {
+                                                       const int *__restrict__ go, double *chi2) {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (t >= ntrials) return;                   // uniform over the wavefront
     double s = 0.0;
     for (int r = lane; r < row_tiles; r += 64) s += (double)partial[(size_t)r * tpad + t];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) {
-        if (!img[t].go || !isfinite(s) || s > 3e38) s = 3e38;
+        if (!*go || !isfinite(s) || s > 3e38) s = 3e38;
         chi2[t] = s;
     }
 }
@@ -2850,14 +2917,31 @@ hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *
     hipLaunchKernelGGL(k_matrix_fill, dim3(P.nr), dim3(256), 0, s, P, rec, span, phi, plo, kpad, bf16);
     return hipGetLastError();
 }
-hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
-                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16, hipStream_t s) {
-    if (kpad <= kMxRefPerThread * 256)
-        hipLaunchKernelGGL(k_matrix_intensity4, dim3((n + kMxTrials - 1) / kMxTrials), dim3(256), 0, s, P, (const TrialImage *)img,
-                           factor, ihi, ilo, iscale, kpad, t0, n, bf16);
-    else
-        hipLaunchKernelGGL(k_matrix_intensity, dim3(n), dim3(256), 0, s, P, (const TrialImage *)img, factor, ihi, ilo, iscale, kpad,
-                           t0, bf16);
+hipError_t launch_matrix_tables(const DevPlan &P, const void *img, const float *factor, double4 *hr, float *f0w, float2 *qk,
+                               int *go, int kpad, int nrep_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_matrix_tables, dim3((kpad + 255) / 256), dim3(256), 0, s, P, (const TrialImage *)img, factor, hr, f0w, qk,
+                       go, kpad, nrep_max);
+    return hipGetLastError();
+}
+hipError_t launch_matrix_intensity(const DevPlan &P, const double *X, const double4 *hr, const float *f0w, const float2 *qk,
+                                   unsigned short *ihi, unsigned short *ilo, float *iscale, int kpad, int n, int bf16,
+                                   hipStream_t s) {
+    // at most 256 threads, as few passes over the reflections as that allows and the same
+    // count for every thread: four-wavefront-or-fewer workgroups keep the chunk's grid
+    // resident at once at this kernel's register count
+    int na = 0;
+    for (int q = 0; q < P.n_phases; ++q) na = std::max(na, P.phase[q].atom_off + P.phase[q].n_atoms);
+    const int passes = std::max(1, (P.nr + 255) / 256);
+    const int threads = std::max(64, (P.nr + 64 * passes - 1) / (64 * passes) * 64);
+    static std::once_flag once;
+    static hipError_t attr = hipSuccess;
+    std::call_once(once, [] {
+        attr = hipFuncSetAttribute((const void *)k_matrix_intensity, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)matrix_intensity_lds(PTK_MAX_ATOMS, PTK_MAX_VARS, PTK_MATRIX_MAX_REFL));
+    });
+    if (attr != hipSuccess) return attr;
+    hipLaunchKernelGGL(k_matrix_intensity, dim3((n + kMxT - 1) / kMxT), dim3(threads), matrix_intensity_lds(na, P.nv, kpad), s, P, X, hr,
+                       f0w, qk, ihi, ilo, iscale, kpad, n, bf16);
     return hipGetLastError();
 }
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
@@ -2874,10 +2958,9 @@ hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const
                            row_tiles, yout, yout ? ny : 0);
     return hipGetLastError();
 }
-hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,
+hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const int *go, double *chi2,
                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_matrix_reduce, dim3((ntrials + 3) / 4), dim3(256), 0, s, partial, row_tiles, tpad, ntrials,
-                       (const TrialImage *)img, chi2);
+    hipLaunchKernelGGL(k_matrix_reduce, dim3((ntrials + 3) / 4), dim3(256), 0, s, partial, row_tiles, tpad, ntrials, go, chi2);
     return hipGetLastError();
 }
 

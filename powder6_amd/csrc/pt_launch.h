@@ -38,13 +38,19 @@ hipError_t launch_matrix_columns(const DevPlan &P, const void *img, LineRec *rec
 // bf16 != 0: the hi/lo pairs as BF16 instead of FP16
 hipError_t launch_matrix_fill(const DevPlan &P, const LineRec *rec, const int4 *span, unsigned short *phi, unsigned short *plo,
                               int kpad, int bf16, hipStream_t s);
-hipError_t launch_matrix_intensity(const DevPlan &P, const void *img, const float *factor, unsigned short *ihi,
-                                   unsigned short *ilo, float *iscale, int kpad, int t0, int n, int bf16, hipStream_t s);
+// per-reflection constants of the intensities (representative phases, f0 w0, s^2/4 with the
+// height factor) and the form's cell validity, from the start vector's image
+hipError_t launch_matrix_tables(const DevPlan &P, const void *img, const float *factor, double4 *hr, float *f0w, float2 *qk,
+                               int *go, int kpad, int nrep_max, hipStream_t s);
+// intensities of the n trial vectors X as hi/lo pairs (structure parameters materialised in the kernel)
+hipError_t launch_matrix_intensity(const DevPlan &P, const double *X, const double4 *hr, const float *f0w, const float2 *qk,
+                                   unsigned short *ihi, unsigned short *ilo, float *iscale, int kpad, int n, int bf16,
+                                   hipStream_t s);
 // yout (when not null) receives Y of the first ny trials (ny x np doubles)
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
                               float *partial, int kpad, int tpad, int row_tiles, double *yout, int ny, int bf16, hipStream_t s);
-hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const void *img, double *chi2,
+hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const int *go, double *chi2,
                                 hipStream_t s);
 hipError_t launch_pattern(const DevPlan &P, const double *X, double *y, int block, int ch, size_t dyn, hipStream_t s);
 hipError_t launch_workload(const DevPlan &P, const double *X, long long *stats, int block, int ch, size_t dyn,

@@ -53,7 +53,10 @@ def test_matrix_pattern_matches_oracle(atoms_only):
 
 
 def test_matrix_chunks_and_repeats(atoms_only):
-    X = _struct_probes(atoms_only, 4200, 11, spread=1.0)
+    """Repeats are identical; 33,000 vectors run as two passes (32,768 + 232, the second
+    one's pair buffers the other half of the double buffer) and agree with the same
+    vectors evaluated in other splits."""
+    X = _struct_probes(atoms_only, 33000, 11, spread=1.0)
     with pt.Plan(atoms_only) as pl:
         a = pl.eval(X, pt.FORM_MATRIX)
         b = pl.eval(X, pt.FORM_MATRIX)

@@ -2678,12 +2678,14 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     typedef Quad<BF16> Q;
     __shared__ __align__(16) unsigned short sB[2][2][64][kMxLd];   // [buffer][hi, lo][trial][k]
     __shared__ float red[4][64];
-    // XCD-aware tiles: workgroups go to the eight XCDs round robin, and XCD x takes the row
-    // tiles x, x + 8, ... for every trial tile, so each XCD's L2 holds an eighth of P
+    // XCD-aware tiles: workgroups go to the eight XCDs round robin, and XCD x takes the
+    // trial tiles x, x + 8, ..., each with all its row tiles in a row: a trial tile's
+    // operand is read from HBM once and stays in that XCD's L2 while its row tiles pass,
+    // and P's band (the only part of P any tile reads) stays resident in every XCD's L2
     const int xcd = blockIdx.x & 7, kx = blockIdx.x >> 3, ntt = tpad / 64;
-    const int nrt = (row_tiles - xcd + 7) / 8;
-    if (kx >= nrt * ntt) return;
-    const int rt = xcd + 8 * (kx / ntt), j0 = (kx % ntt) * 64;
+    const int ntx = (ntt - xcd + 7) / 8;
+    if (kx >= ntx * row_tiles) return;
+    const int rt = kx % row_tiles, j0 = (xcd + 8 * (kx / row_tiles)) * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i0 = rt * 128 + wave * 32;
     const int2 b = band[rt];
@@ -2947,9 +2949,9 @@ hipError_t launch_matrix_intensity(const DevPlan &P, const double *X, const doub
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
                               float *partial, int kpad, int tpad, int row_tiles, double *yout, int ny, int bf16, hipStream_t s) {
-    // row_tiles: 128-point tiles; the grid is eight XCD slots of ceil(row_tiles / 8) row
-    // tiles x the trial tiles
-    const dim3 grid(8 * ((row_tiles + 7) / 8) * (tpad / 64));
+    // row_tiles: 128-point tiles; the grid is eight XCD slots of ceil(trial tiles / 8) trial
+    // tiles x the row tiles
+    const dim3 grid(8 * ((tpad / 64 + 7) / 8) * row_tiles);
     if (bf16)
         hipLaunchKernelGGL(k_matrix_chi2<true>, grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad, tpad,
                            row_tiles, yout, yout ? ny : 0);

@@ -2442,35 +2442,38 @@ __device__ __forceinline__ double matrix_param(const DevPlan &P, const double *x
 // multiply the sums (structure_f2's order: atoms outer, representatives inner).  The
 // intensities wait in LDS for the trial's maximum, then go out as pairs.
 //
-// Dynamic LDS, na atoms, nv variables: xs [kMxT][nv] f64 (the trial vectors), apos
-// [kMxT][na][3] f64, aocc [kMxT][na], aB [kMxT][na], afi [na] (f'' w0),
-// S [kMxT][PTK_MAX_PHASES], wmax [kMxT][8], ival [kMxT][kpad] floats.
-constexpr int kMxT = 4;
-__host__ __device__ constexpr size_t matrix_intensity_lds(int na, int nv, int kpad) {
-    return sizeof(double) * ((size_t)kMxT * nv + 3 * (size_t)kMxT * na) +
-           sizeof(float) * ((size_t)2 * kMxT * na + na + (size_t)kMxT * PTK_MAX_PHASES + 8 * kMxT + (size_t)kMxT * kpad);
+// Dynamic LDS, na atoms, nv variables: xs [T][nv] f64 (the trial vectors), apos
+// [T][na][3] f64, aocc [T][na], aB [T][na], afi [na] (f'' w0),
+// S [T][PTK_MAX_PHASES], wmax [T][8], ival [T][kpad] floats.
+constexpr int kMxT = 4;                  // trials per workgroup
+__host__ __device__ constexpr size_t matrix_intensity_lds(int T, int na, int nv, int kpad) {
+    return sizeof(double) * ((size_t)T * nv + 3 * (size_t)T * na) +
+           sizeof(float) * ((size_t)2 * T * na + na + (size_t)T * PTK_MAX_PHASES + 8 * T + (size_t)T * kpad);
 }
 
-template <bool CENTRO>
+template <int T, bool CENTRO>
 __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos, const float *aocc, const float *aB,
                                           const float *afi, int na, const double4 *__restrict__ hr, const float *__restrict__ f0w,
                                           int k, int kpad, float q4, float *f2) {
-    float Ar[kMxT], Br[kMxT], Cr[kMxT], Dr[kMxT];
+    float Ar[T], Br[T], Cr[T], Dr[T];
 #pragma unroll
-    for (int tt = 0; tt < kMxT; ++tt) Ar[tt] = Br[tt] = Cr[tt] = Dr[tt] = 0.f;
+    for (int tt = 0; tt < T; ++tt) Ar[tt] = Br[tt] = Cr[tt] = Dr[tt] = 0.f;
     const int nrep = ph.n_reps;
-    // the first four representatives' h R, h.t are loaded once for all atoms
+    // with four trials, the first four representatives' h R, h.t are loaded once for all
+    // atoms (with more trials the registers go to the trials)
+    constexpr bool kHoist = T <= 4;
     double4 e4[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) e4[i] = 4 <= nrep ? hr[(size_t)i * kpad + k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int i = 0; i < 4; ++i)
+        e4[i] = kHoist && 4 <= nrep ? hr[(size_t)i * kpad + k] : make_double4(0.0, 0.0, 0.0, 0.0);
     for (int j = 0; j < ph.n_atoms; ++j) {
         const int ga = ph.atom_off + j;
         const float fw = f0w[(size_t)ga * kpad + k], fi = afi[ga];
-        double px[kMxT], py[kMxT], pz[kMxT];
-        float ca[kMxT], sa[kMxT];
+        double px[T], py[T], pz[T];
+        float ca[T], sa[T];
 #pragma unroll
-        for (int tt = 0; tt < kMxT; ++tt) {
-            const double *a = apos + 3 * ((size_t)tt * na + ga);
+        for (int tt = 0; tt < T; ++tt) {
+            const double *a = apos + 3 * (tt * na + ga);
             px[tt] = a[0]; py[tt] = a[1]; pz[tt] = a[2];
             ca[tt] = sa[tt] = 0.f;
         }
@@ -2480,12 +2483,12 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
         for (; r + 4 <= nrep; r += 4) {
             double4 e[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) e[i] = r == 0 ? e4[i] : hr[(size_t)(r + i) * kpad + k];
-            float cv[4][kMxT], sv[4][kMxT];
+            for (int i = 0; i < 4; ++i) e[i] = kHoist && r == 0 ? e4[i] : hr[(size_t)(r + i) * kpad + k];
+            float cv[4][T], sv[4][T];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int tt = 0; tt < kMxT; ++tt) {
+                for (int tt = 0; tt < T; ++tt) {
                     const double phs = fma(e[i].x, px[tt], fma(e[i].y, py[tt], fma(e[i].z, pz[tt], e[i].w)));
                     const float fr = (float)__builtin_amdgcn_fract(phs);
                     cv[i][tt] = __builtin_amdgcn_cosf(fr);
@@ -2494,7 +2497,7 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int tt = 0; tt < kMxT; ++tt) {
+                for (int tt = 0; tt < T; ++tt) {
                     ca[tt] += cv[i][tt];
                     if (!CENTRO) sa[tt] += sv[i][tt];
                 }
@@ -2502,7 +2505,7 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
         for (; r < nrep; ++r) {
             const double4 e = hr[(size_t)r * kpad + k];
 #pragma unroll
-            for (int tt = 0; tt < kMxT; ++tt) {
+            for (int tt = 0; tt < T; ++tt) {
                 const double phs = fma(e.x, px[tt], fma(e.y, py[tt], fma(e.z, pz[tt], e.w)));
                 const float fr = (float)__builtin_amdgcn_fract(phs);
                 ca[tt] += __builtin_amdgcn_cosf(fr);
@@ -2510,7 +2513,7 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
             }
         }
 #pragma unroll
-        for (int tt = 0; tt < kMxT; ++tt) {
+        for (int tt = 0; tt < T; ++tt) {
             const float od = aocc[tt * na + ga] * __expf(-aB[tt * na + ga] * q4);
             const float wj = od * fw, wi = od * fi;
             Ar[tt] += wj * ca[tt]; Cr[tt] += wi * ca[tt];
@@ -2518,34 +2521,35 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
         }
     }
 #pragma unroll
-    for (int tt = 0; tt < kMxT; ++tt)
+    for (int tt = 0; tt < T; ++tt)
         f2[tt] = CENTRO ? 4.0f * (Ar[tt] * Ar[tt] + Cr[tt] * Cr[tt])
                         : Ar[tt] * Ar[tt] + Br[tt] * Br[tt] + Cr[tt] * Cr[tt] + Dr[tt] * Dr[tt];
 }
 
 // X: the chunk's n trial vectors (nv each)
+template <int T>
 __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const double *__restrict__ X,
                                                           const double4 *__restrict__ hr, const float *__restrict__ f0w,
                                                           const float2 *__restrict__ qk, unsigned short *ihi, unsigned short *ilo,
                                                           float *iscale, int kpad, int n, int bf16) {
     extern __shared__ __align__(16) unsigned char mx_raw[];
-    const int tid = threadIdx.x, nt = blockDim.x, tb = blockIdx.x * kMxT;
+    const int tid = threadIdx.x, nt = blockDim.x, tb = blockIdx.x * T;
     const int na = total_atoms(P);
     double *xs = reinterpret_cast<double *>(mx_raw);
-    double *apos = xs + (size_t)kMxT * P.nv;
-    float *aocc = reinterpret_cast<float *>(apos + 3 * (size_t)kMxT * na);
-    float *aB = aocc + kMxT * na, *afi = aB + kMxT * na, *Sx = afi + na, *wmax = Sx + kMxT * PTK_MAX_PHASES;
-    float *ival = wmax + 8 * kMxT;
+    double *apos = xs + (size_t)T * P.nv;
+    float *aocc = reinterpret_cast<float *>(apos + 3 * (size_t)T * na);
+    float *aB = aocc + T * na, *afi = aB + T * na, *Sx = afi + na, *wmax = Sx + T * PTK_MAX_PHASES;
+    float *ival = wmax + 8 * T;
     // the workgroup's trial vectors into LDS first (their loads do not wait on the
     // parameter map's)
-    for (int i = tid; i < kMxT * P.nv; i += nt) {
+    for (int i = tid; i < T * P.nv; i += nt) {
         const int tt = i / P.nv;
         xs[i] = tb + tt < n ? X[(size_t)tb * P.nv + i] : 0.0;
     }
     __syncthreads();
     // the trials' structure parameters, one (trial, atom, slot) per thread: the coordinate
     // slots wait in apos, occupancy and B go to aocc / aB
-    for (int i = tid; i < kMxT * na * 5; i += nt) {
+    for (int i = tid; i < T * na * 5; i += nt) {
         const int tt = i / (na * 5), ga = (i / 5) % na, f = i % 5;
         double v = 0.0;
         if (tb + tt < n) {
@@ -2559,7 +2563,7 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const doubl
         else if (f == 3) aocc[tt * na + ga] = (float)v;
         else aB[tt * na + ga] = (float)v;
     }
-    for (int i = tid; i < kMxT * P.n_phases; i += nt) {
+    for (int i = tid; i < T * P.n_phases; i += nt) {
         const int tt = i / P.n_phases, q = i % P.n_phases;
         float S = 0.f;
         if (tb + tt < n) {
@@ -2574,7 +2578,7 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const doubl
     }
     __syncthreads();
     // positions r0 + sum over the free slots of v basis (trial_setup's arithmetic)
-    for (int i = tid; i < kMxT * na; i += nt) {
+    for (int i = tid; i < T * na; i += nt) {
         const int ga = i % na;
         double *a = apos + 3 * (size_t)i;
         const DevAtom &A = P.atoms[ga];
@@ -2587,35 +2591,39 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const doubl
         a[0] = r[0]; a[1] = r[1]; a[2] = r[2];
     }
     __syncthreads();
-    float mx[kMxT] = {0.f, 0.f, 0.f, 0.f};
+    float mx[T];
+#pragma unroll
+    for (int tt = 0; tt < T; ++tt) mx[tt] = 0.f;
     for (int k = tid; k < P.nr; k += nt) {
         const float2 qf = qk[k];
-        float v[kMxT] = {0.f, 0.f, 0.f, 0.f};
+        float v[T];
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt) v[tt] = 0.f;
         if (qf.y != 0.0f) {
             int q = 0;
             while (q + 1 < P.n_phases && k >= P.phase[q + 1].refl_off) ++q;
             const DevPhase &ph = P.phase[q];
-            float f2[kMxT];
-            if (ph.centro) matrix_f2<true>(ph, apos, aocc, aB, afi, na, hr, f0w, k, kpad, qf.x, f2);
-            else matrix_f2<false>(ph, apos, aocc, aB, afi, na, hr, f0w, k, kpad, qf.x, f2);
+            float f2[T];
+            if (ph.centro) matrix_f2<T, true>(ph, apos, aocc, aB, afi, na, hr, f0w, k, kpad, qf.x, f2);
+            else matrix_f2<T, false>(ph, apos, aocc, aB, afi, na, hr, f0w, k, kpad, qf.x, f2);
 #pragma unroll
-            for (int tt = 0; tt < kMxT; ++tt) {
+            for (int tt = 0; tt < T; ++tt) {
                 v[tt] = Sx[tt * PTK_MAX_PHASES + q] * (qf.y * f2[tt]);
                 mx[tt] = fmaxf(mx[tt], fabsf(v[tt]));
             }
         }
 #pragma unroll
-        for (int tt = 0; tt < kMxT; ++tt) ival[tt * kpad + k] = v[tt];
+        for (int tt = 0; tt < T; ++tt) ival[tt * kpad + k] = v[tt];
     }
     // per trial: a power-of-two scale that keeps the largest intensity near 2^14
 #pragma unroll
-    for (int tt = 0; tt < kMxT; ++tt) {
+    for (int tt = 0; tt < T; ++tt) {
         float m = mx[tt];
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
         if ((tid & 63) == 0) wmax[tt * 8 + (tid >> 6)] = m;
     }
     __syncthreads();
-    if (tid < kMxT) {
+    if (tid < T) {
         float m = 0.0f;
         for (int w = 0; w < (nt >> 6); ++w) m = fmaxf(m, wmax[tid * 8 + w]);
         int e = 0;
@@ -2623,7 +2631,7 @@ __global__ void __launch_bounds__(256) k_matrix_intensity(DevPlan P, const doubl
         wmax[tid * 8] = ldexpf(1.0f, e);
     }
     __syncthreads();
-    for (int tt = 0; tt < kMxT; ++tt) {
+    for (int tt = 0; tt < T; ++tt) {
         const int t = tb + tt;
         if (t >= n) break;
         const float sc = wmax[tt * 8], inv = 1.0f / sc;
@@ -2938,12 +2946,12 @@ hipError_t launch_matrix_intensity(const DevPlan &P, const double *X, const doub
     static std::once_flag once;
     static hipError_t attr = hipSuccess;
     std::call_once(once, [] {
-        attr = hipFuncSetAttribute((const void *)k_matrix_intensity, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)matrix_intensity_lds(PTK_MAX_ATOMS, PTK_MAX_VARS, PTK_MATRIX_MAX_REFL));
+        attr = hipFuncSetAttribute((const void *)k_matrix_intensity<kMxT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)matrix_intensity_lds(kMxT, PTK_MAX_ATOMS, PTK_MAX_VARS, PTK_MATRIX_MAX_REFL));
     });
     if (attr != hipSuccess) return attr;
-    hipLaunchKernelGGL(k_matrix_intensity, dim3((n + kMxT - 1) / kMxT), dim3(threads), matrix_intensity_lds(na, P.nv, kpad), s, P, X, hr,
-                       f0w, qk, ihi, ilo, iscale, kpad, n, bf16);
+    hipLaunchKernelGGL(k_matrix_intensity<kMxT>, dim3((n + kMxT - 1) / kMxT), dim3(threads), matrix_intensity_lds(kMxT, na, P.nv, kpad),
+                       s, P, X, hr, f0w, qk, ihi, ilo, iscale, kpad, n, bf16);
     return hipGetLastError();
 }
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,

@@ -451,12 +451,14 @@ def main():
         ms_per_step = 1e3 * (t1 - t0) / args.steps
         # plain single-trial form on a bounded sample of trial vectors
         plain, X, chi_plain = plain_rate(pt, plan, pb, args.plain_trials)
-        # batched (one launch, one workgroup per trial) on 16384 independent vectors
+        # the staged stages on 16384 independent vectors (one pass): device time between
+        # HIP events with the vectors in HBM, and wall time with the host copies
         XB = np.repeat(X, (16384 + len(X) - 1) // len(X), axis=0)[:16384]
         plan.eval(XB, pt.FORM_BATCHED)
         t = time.perf_counter()
         gb = plan.eval(XB, pt.FORM_BATCHED)
-        batched = len(XB) / (time.perf_counter() - t)
+        batched_wall = len(XB) / (time.perf_counter() - t)
+        batched = len(XB) / (plan.timing()[0] / 1e3)
 
     copy_gbps = None
     if args.copy_probe:
@@ -523,7 +525,8 @@ def main():
         'chi2_rel_err_vs_fp64': err_chosen,
         'plain': {'value': plain, 'unit': 'trial patterns/s', 'form': 'one trial per launch sequence',
                   'chi2_rel_err_vs_fp64': chi2_error(pb, X[:16], chi_plain[:16])},
-        'batched_eval': {'value': batched, 'unit': 'trial patterns/s', 'form': 'staged pipeline, 16384 independent vectors',
+        'batched_eval': {'value': batched, 'value_with_host_transfers': batched_wall, 'unit': 'trial patterns/s',
+                         'form': 'staged pipeline, 16384 independent vectors',
                          'chi2_rel_err_vs_fp64': chi2_error(pb, XB[:16], gb[:16])},
         'nacl_cpu_path_case': nacl,
         'matrix_form': mat,

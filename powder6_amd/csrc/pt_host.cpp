@@ -1169,7 +1169,11 @@ static double sum_pairs(pt_plan *pl, int pairs) {
     return k;
 }
 
-static const int kEvalChunk = 2048;   // trial slots per pass of pt_eval
+// trial slots per pass of pt_eval: up to 16384 (larger passes amortise each stage
+// launch's ramp and tail: 2048-trial passes ran 10% slower per trial on the Pnma bench
+// plan), halved while the slots would take more than 1 GiB, down to 2048
+static const int kEvalChunkMax = 16384, kEvalChunkMin = 2048;
+static const size_t kEvalSlotBytes = (size_t)1 << 30;
 static const int kEvalChunkFree = 512; // the same for plans with free intensities (per-slot solve buffers)
 
 // a plan or one of its further datasets has free intensities
@@ -1182,10 +1186,19 @@ static const char *kUnsolved = "MI355X PT intensity solve did not converge.";
 
 static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, double *yout, int bf16 = 0);
 
+static int eval_chunk(const pt_plan *pl) {
+    if (any_free(pl)) return kEvalChunkFree;
+    const size_t per = ptk::trial_image_bytes() + 2 * (size_t)pl->nr * sizeof(ptk::StagedLine) +
+                       (size_t)pl->ngroups * sizeof(double2) + sizeof(double);
+    int c = kEvalChunkMax;
+    while (c > kEvalChunkMin && (size_t)c * per > kEvalSlotBytes) c /= 2;
+    return c;
+}
+
 static int eval_batched(pt_plan *pl, const double *x, int ntrials, double *chi2, bool time_it) {
     int rc = ensure_trials(pl, ntrials);
     if (rc) return rc;
-    int chunk = std::min(ntrials, any_free(pl) ? kEvalChunkFree : kEvalChunk);
+    int chunk = std::min(ntrials, eval_chunk(pl));
     if ((rc = ensure_slots(pl, std::max(chunk, pl->chains)))) return rc;
     int passes = (ntrials + chunk - 1) / chunk;
     if (time_it && (rc = ensure_events(pl, passes))) return rc;

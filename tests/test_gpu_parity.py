@@ -173,9 +173,9 @@ def test_tile_shapes_give_identical_states(pbso4):
 
 
 def test_batched_eval_chunks_and_repeats(pbso4):
-    """Evaluating 2500 vectors (two passes of the slot buffer) equals evaluating them one
-    pass at a time, and a repeat is bit-identical."""
-    X = _probes(pbso4, 2500, 21, spread=1.0)
+    """Evaluating 17,000 vectors (two passes of the 16,384-slot buffer) equals evaluating
+    them in other splits, and a repeat is bit-identical."""
+    X = _probes(pbso4, 17000, 21, spread=1.0)
     with pt.Plan(pbso4) as pl:
         a = pl.eval(X, pt.FORM_BATCHED)
         b = pl.eval(X, pt.FORM_BATCHED)

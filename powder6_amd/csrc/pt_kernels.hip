@@ -2369,8 +2369,8 @@ __global__ void __launch_bounds__(256) k_matrix_fill(DevPlan P, const LineRec *_
 //                     the phase's n_reps),
 //   f0w[ga kpad + k] = (f0 + f') w0 of atom ga at k (the neutron length times w0), zero for
 //                     the atoms of other phases,
-//   qk[k]           = (s^2 / 4, the height factor m Lp PO / area, or 0 when k's phase cell
-//                     is invalid),
+//   qk[k]           = (log2(e) s^2 / 4, the Debye-Waller exponent per unit B in exp2 form;
+//                     the height factor m Lp PO / area, or 0 when k's phase cell is invalid),
 //   *go             = the start vector's cell validity (a trial of the form goes when it does).
 __global__ void __launch_bounds__(256) k_matrix_tables(DevPlan P, const TrialImage *__restrict__ img,
                                                        const float *__restrict__ factor, double4 *hr, float *f0w, float2 *qk,
@@ -2393,7 +2393,7 @@ __global__ void __launch_bounds__(256) k_matrix_tables(DevPlan P, const TrialIma
     const double h = hk.x, kx = hk.y, l = hk.z;
     const double qv = h * h * f[0] + kx * kx * f[1] + l * l * f[2] + h * kx * f[3] + kx * l * f[4] + h * l * f[5];
     const float q4 = 0.25f * (float)qv;
-    qk[k] = make_float2(q4, img->phase_ok[q] ? factor[k] : 0.0f);
+    qk[k] = make_float2(q4 * 1.44269504f, img->phase_ok[q] ? factor[k] : 0.0f);
     for (int r = 0; r < nrep_max; ++r) {
         double4 e = make_double4(0.0, 0.0, 0.0, 0.0);
         if (r < ph.n_reps) {
@@ -2451,10 +2451,11 @@ __host__ __device__ constexpr size_t matrix_intensity_lds(int T, int na, int nv,
            sizeof(float) * ((size_t)2 * T * na + na + (size_t)T * PTK_MAX_PHASES + 8 * T + (size_t)T * kpad);
 }
 
+// dwk: the Debye-Waller exponent per unit B in exp2 form (log2(e) s^2 / 4)
 template <int T, bool CENTRO>
 __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos, const float *aocc, const float *aB,
                                           const float *afi, int na, const double4 *__restrict__ hr, const float *__restrict__ f0w,
-                                          int k, int kpad, float q4, float *f2) {
+                                          int k, int kpad, float dwk, float *f2) {
     float Ar[T], Br[T], Cr[T], Dr[T];
 #pragma unroll
     for (int tt = 0; tt < T; ++tt) Ar[tt] = Br[tt] = Cr[tt] = Dr[tt] = 0.f;
@@ -2498,8 +2499,8 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int tt = 0; tt < T; ++tt) {
-                    ca[tt] += cv[i][tt];
-                    if (!CENTRO) sa[tt] += sv[i][tt];
+                    ca[tt] = r == 0 && i == 0 ? cv[i][tt] : ca[tt] + cv[i][tt];
+                    if (!CENTRO) sa[tt] = r == 0 && i == 0 ? sv[i][tt] : sa[tt] + sv[i][tt];
                 }
         }
         for (; r < nrep; ++r) {
@@ -2512,12 +2513,21 @@ __device__ __forceinline__ void matrix_f2(const DevPhase &ph, const double *apos
                 if (!CENTRO) sa[tt] += __builtin_amdgcn_sinf(fr);
             }
         }
+        float od[T];
 #pragma unroll
         for (int tt = 0; tt < T; ++tt) {
-            const float od = aocc[tt * na + ga] * __expf(-aB[tt * na + ga] * q4);
-            const float wj = od * fw, wi = od * fi;
-            Ar[tt] += wj * ca[tt]; Cr[tt] += wi * ca[tt];
-            if (!CENTRO) { Br[tt] += wj * sa[tt]; Dr[tt] += wi * sa[tt]; }
+            od[tt] = aocc[tt * na + ga] * __builtin_amdgcn_exp2f(-aB[tt * na + ga] * dwk);
+            const float wj = od[tt] * fw;
+            Ar[tt] += wj * ca[tt];
+            if (!CENTRO) Br[tt] += wj * sa[tt];
+        }
+        if (fi != 0.0f) {               // the f'' terms (anomalous scattering)
+#pragma unroll
+            for (int tt = 0; tt < T; ++tt) {
+                const float wi = od[tt] * fi;
+                Cr[tt] += wi * ca[tt];
+                if (!CENTRO) Dr[tt] += wi * sa[tt];
+            }
         }
     }
 #pragma unroll

@@ -350,7 +350,11 @@ __device__ __forceinline__ float structure_f2_impl(const DevPlan &P, const Share
         for (int r = 0; r < ph.n_reps; ++r) {
             const int *R = sh.rep_r[ph.rep_off + r];
             const double *t = sh.rep_t[ph.rep_off + r];
-            const double H0 = h * R[0] + k * R[3] + l * R[6], H1 = h * R[1] + k * R[4] + l * R[7], H2 = h * R[2] + k * R[5] + l * R[8];
+            // h R in 24-bit integer products (full rate; the operands are small integers, the
+            // sums exact)
+            const double H0 = __mul24(h, R[0]) + __mul24(k, R[3]) + __mul24(l, R[6]);
+            const double H1 = __mul24(h, R[1]) + __mul24(k, R[4]) + __mul24(l, R[7]);
+            const double H2 = __mul24(h, R[2]) + __mul24(k, R[5]) + __mul24(l, R[8]);
             const double ht = (double)h * t[0] + (double)k * t[1] + (double)l * t[2];
 #pragma unroll
             for (int a = 0; a < 4; ++a) {

@@ -898,6 +898,26 @@ static int create_impl(const pt_problem *pb, pt_plan **out, bool with_device) {
     P.guard_hi = std::tan(std::max(1e-6, std::min(179.999999, pb->tth[n - 1])) * kPi / 360);
     P.max_temp = pb->max_temp;
     P.seed = pb->seed;
+    {
+        // chunked representative sums in the lines stage: a workgroup of 256 threads holds at
+        // most 256 reflections; when it holds fewer and a phase has many representatives
+        // (a cubic phase of a small pattern), each reflection's representatives are split
+        // into chunks summed on otherwise idle threads (phases of at most
+        // PTK_REP_CHUNK_ATOMS atoms)
+        int maxrep = 1, maxat = 0;
+        for (int q = 0; q < P.n_phases; ++q) {
+            maxrep = std::max(maxrep, P.phase[q].n_reps);
+            maxat = std::max(maxat, P.phase[q].n_atoms);
+        }
+        const int nk = std::max(1, std::min(P.nr, 256));
+        int ch = 1;
+        if (maxat <= PTK_REP_CHUNK_ATOMS && maxrep >= 4) {
+            ch = std::max(1, std::min(maxrep, 256 / nk));
+            const int rpc = (maxrep + ch - 1) / ch;
+            ch = (maxrep + rpc - 1) / rpc;
+        }
+        P.rep_chunks = ch;
+    }
 
     std::vector<double> tth(pb->tth, pb->tth + n), base(pb->param_base, pb->param_base + pb->n_params);
     pl->x0.assign(pb->x0, pb->x0 + pb->n_vars);

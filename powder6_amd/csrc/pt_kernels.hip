@@ -333,38 +333,63 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
 // take revolutions).  Loop order: representatives outside, up to four atoms inside, so
 // h R and h.t are formed once per representative.  CENTRO: the representatives are one
 // of each (R, t) / (-R, -t) pair, so A = 2 sum cos, B = 0 (and C, D alike).
+// Chunked representative sums: with few reflections per workgroup and many representatives
+// (a cubic phase), the lines stage splits each reflection's representatives into
+// DevPlan::rep_chunks contiguous chunks, sums every (reflection, chunk) on its own thread,
+// and the reflection's thread adds the chunks in order.  kSfSlots floats per (reflection,
+// chunk): cos and sin sums of up to PTK_REP_CHUNK_ATOMS atoms.
+constexpr int kSfSlots = 2 * PTK_REP_CHUNK_ATOMS;
+
+// the representatives [r0, r1) of atoms j0 .. j0 + nj - 1 (nj <= 4): ca[a] += sum cos, sa[a] +=
+// sum sin of the turns
+template <bool CENTRO>
+__device__ __forceinline__ void sf_rep_sums(const Shared &sh, const DevPhase &ph, int h, int k, int l, int j0, int nj, int r0,
+                                            int r1, float *ca, float *sa) {
+    double px[4], py[4], pz[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int ga = ph.atom_off + j0 + (a < nj ? a : 0);
+        px[a] = sh.apos[ga][0]; py[a] = sh.apos[ga][1]; pz[a] = sh.apos[ga][2];
+    }
+    for (int r = r0; r < r1; ++r) {
+        const int *R = sh.rep_r[ph.rep_off + r];
+        const double *t = sh.rep_t[ph.rep_off + r];
+        const double H0 = __mul24(h, R[0]) + __mul24(k, R[3]) + __mul24(l, R[6]);
+        const double H1 = __mul24(h, R[1]) + __mul24(k, R[4]) + __mul24(l, R[7]);
+        const double H2 = __mul24(h, R[2]) + __mul24(k, R[5]) + __mul24(l, R[8]);
+        const double ht = (double)h * t[0] + (double)k * t[1] + (double)l * t[2];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            if (a < nj) {
+                const double phs = H0 * px[a] + H1 * py[a] + H2 * pz[a] + ht;
+                const float fr = (float)(phs - floor(phs));
+                ca[a] += __builtin_amdgcn_cosf(fr);
+                if (!CENTRO) sa[a] += __builtin_amdgcn_sinf(fr);
+            }
+        }
+    }
+}
+
+// parts (optional): the reflection's representative sums already formed in nch chunks of
+// representatives (sf_chunk_sums), added here chunk by chunk
 template <bool CENTRO>
 __device__ __forceinline__ float structure_f2_impl(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k,
-                                                   int l, float s2) {
+                                                   int l, float s2, const float *parts, int nch) {
     float Ar = 0.f, Br = 0.f, Cr = 0.f, Dr = 0.f;
     const float q4 = 0.25f * s2;
     for (int j0 = 0; j0 < ph.n_atoms; j0 += 4) {
         const int nj = min(4, ph.n_atoms - j0);
         float ca[4] = {0.f, 0.f, 0.f, 0.f}, sa[4] = {0.f, 0.f, 0.f, 0.f};
-        double px[4], py[4], pz[4];
+        if (parts) {
+            for (int c = 0; c < nch; ++c)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int ga = ph.atom_off + j0 + (a < nj ? a : 0);
-            px[a] = sh.apos[ga][0]; py[a] = sh.apos[ga][1]; pz[a] = sh.apos[ga][2];
-        }
-        for (int r = 0; r < ph.n_reps; ++r) {
-            const int *R = sh.rep_r[ph.rep_off + r];
-            const double *t = sh.rep_t[ph.rep_off + r];
-            // h R in 24-bit integer products (full rate; the operands are small integers, the
-            // sums exact)
-            const double H0 = __mul24(h, R[0]) + __mul24(k, R[3]) + __mul24(l, R[6]);
-            const double H1 = __mul24(h, R[1]) + __mul24(k, R[4]) + __mul24(l, R[7]);
-            const double H2 = __mul24(h, R[2]) + __mul24(k, R[5]) + __mul24(l, R[8]);
-            const double ht = (double)h * t[0] + (double)k * t[1] + (double)l * t[2];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                if (a < nj) {
-                    const double phs = H0 * px[a] + H1 * py[a] + H2 * pz[a] + ht;
-                    const float fr = (float)(phs - floor(phs));
-                    ca[a] += __builtin_amdgcn_cosf(fr);
-                    if (!CENTRO) sa[a] += __builtin_amdgcn_sinf(fr);
-                }
-            }
+                for (int a = 0; a < 4; ++a)
+                    if (a < nj) {
+                        ca[a] += parts[c * kSfSlots + 2 * (j0 + a)];
+                        if (!CENTRO) sa[a] += parts[c * kSfSlots + 2 * (j0 + a) + 1];
+                    }
+        } else {
+            sf_rep_sums<CENTRO>(sh, ph, h, k, l, j0, nj, 0, ph.n_reps, ca, sa);
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -391,8 +416,9 @@ __device__ __forceinline__ float structure_f2_impl(const DevPlan &P, const Share
 }
 
 __device__ __forceinline__ float structure_f2(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k, int l,
-                                              float s2) {
-    return ph.centro ? structure_f2_impl<true>(P, sh, ph, h, k, l, s2) : structure_f2_impl<false>(P, sh, ph, h, k, l, s2);
+                                              float s2, const float *parts = nullptr, int nch = 1) {
+    return ph.centro ? structure_f2_impl<true>(P, sh, ph, h, k, l, s2, parts, nch)
+                     : structure_f2_impl<false>(P, sh, ph, h, k, l, s2, parts, nch);
 }
 
 // March-Dollase factor averaged over the equivalents (rietveld_core.js:824-843)
@@ -445,7 +471,8 @@ __device__ inline void emit_line(const DevPlan &P, const Prep &pk, double x0, fl
 // leaves it to the intensity solve.
 template <bool SEP = true>
 __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared &sh, int k, LineRec *rec, int4 *span,
-                                        float *unit_factor = nullptr, const int4 *hk_given = nullptr, int4 *fspan = nullptr) {
+                                        float *unit_factor = nullptr, const int4 *hk_given = nullptr, int4 *fspan = nullptr,
+                                        const float *sf_parts = nullptr) {
     if (unit_factor) unit_factor[k] = 0.0f;
     span[0] = make_int4(0, 0, 0, 0);
     span[1] = make_int4(0, 0, 0, 0);
@@ -491,7 +518,7 @@ __device__ __forceinline__ void build_reflection(const DevPlan &P, const Shared 
         unit_factor[k] = (float)hk.w * lp1 * po / pk1.area;
         height = 1.0f;
     } else {
-        float f2 = structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv);
+        float f2 = structure_f2(P, sh, ph, hk.x, hk.y, hk.z, (float)qv, sf_parts, P.rep_chunks);
         double S = pp[6];
         if (!(S != 0.0) || isnan(S)) S = 0.0;
         height = (float)S * ((float)hk.w * lp1 * po / pk1.area * f2);
@@ -948,7 +975,7 @@ __device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec
 // global atomic min / max per group and workgroup)
 template <bool SEP = true>
 __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, StagedLine *rec, int t, int kb, Shared &sh,
-                                           int (*gr)[2]) {
+                                           int (*gr)[2], float *sfpart) {
     const int k = kb * blockDim.x + threadIdx.x;
     const int4 hk = k < P.nr ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
     stage_reps(P, sh);
@@ -956,13 +983,39 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, St
     const int ng = (P.np + 511) >> 9;
     for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
+    // chunked representative sums (a plan with few reflections and many representatives):
+    // thread p takes chunk p % nch of the workgroup's reflection p / nch
+    const int nch = P.rep_chunks;
+    if (nch > 1 && sh.go) {
+        const int k0 = kb * blockDim.x, nk = min((int)blockDim.x, P.nr - k0);
+        for (int p = threadIdx.x; p < nk * nch; p += blockDim.x) {
+            const int kc = k0 + p / nch, c = p % nch;
+            int q = 0;
+            while (q + 1 < P.n_phases && kc >= P.phase[q + 1].refl_off) ++q;
+            const DevPhase &ph = P.phase[q];
+            float *o = sfpart + (size_t)p * kSfSlots;
+            const int rpc = (ph.n_reps + nch - 1) / nch, r0 = min(ph.n_reps, c * rpc), r1 = min(ph.n_reps, r0 + rpc);
+            const int4 hc = P.hklm[kc];
+            for (int j0 = 0; j0 < ph.n_atoms; j0 += 4) {
+                const int nj = min(4, ph.n_atoms - j0);
+                float ca[4] = {0.f, 0.f, 0.f, 0.f}, sa[4] = {0.f, 0.f, 0.f, 0.f};
+                if (ph.centro) sf_rep_sums<true>(sh, ph, hc.x, hc.y, hc.z, j0, nj, r0, r1, ca, sa);
+                else sf_rep_sums<false>(sh, ph, hc.x, hc.y, hc.z, j0, nj, r0, r1, ca, sa);
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+                    if (a < nj) { o[2 * (j0 + a)] = ca[a]; o[2 * (j0 + a) + 1] = sa[a]; }
+            }
+        }
+        __syncthreads();
+    }
     if (k < P.nr) {
         const size_t base = (size_t)t * 2 * P.nr + 2 * (size_t)k;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (sh.go) {
             LineRec lr[2] = {{z4, z4, z4}, {z4, z4, z4}};
             int4 ls[2];
-            build_reflection<SEP>(P, sh, k, lr, ls, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr);
+            build_reflection<SEP>(P, sh, k, lr, ls, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr,
+                                  nch > 1 ? sfpart + (size_t)threadIdx.x * nch * kSfSlots : nullptr);
             const bool uni = P.uniform != 0;
             for (int q = 0; q < 2; ++q) {
                 const int4 sp = ls[q];
@@ -1005,11 +1058,12 @@ __global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img,
                                                      int t_off) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
+    __shared__ float sfpart[256 * kSfSlots];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
     // the lean instance for plans with neither separated profiles nor free intensities
-    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, t_off + t, u, sh, gr);
-    else lines_body<false>(P, img, rec, t_off + t, u, sh, gr);
+    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, t_off + t, u, sh, gr, sfpart);
+    else lines_body<false>(P, img, rec, t_off + t, u, sh, gr, sfpart);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
@@ -2173,12 +2227,13 @@ __global__ void __launch_bounds__(256) k_group_lines(const DevPlan *__restrict__
                                                      const int4 *__restrict__ tab) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
+    __shared__ float sfpart[256 * kSfSlots];
     const int4 w = tab[blockIdx.x];
     if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr);
-    else lines_body<false>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr);
+    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr, sfpart);
+    else lines_body<false>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr, sfpart);
 }
 
 template <int TPW>

@@ -36,6 +36,7 @@
 #define PTK_S_COUNT_K 12
 #define PTK_MATRIX_MAX_REFL 4096
 #define PTK_MAX_GROUPS 64          // 512-point groups per pattern (32768 points)
+#define PTK_REP_CHUNK_ATOMS 8      // atoms per phase up to which the lines stage may chunk the representative sums
 
 namespace ptk {
 
@@ -113,6 +114,8 @@ struct DevPlan {
     int has_free;              // some phase has free intensities (the NNLS stage runs)
     int n_free;                // free reflections (all reflections of the free phases)
     int nnls_cap;              // active-set iterations of one solve (0: 3 m + 10 for m live columns)
+    int rep_chunks;             // the lines stage sums each reflection's representatives in this many
+                               // chunks on separate threads (1: one thread per reflection)
     // dataset
     double t0, step;           // uniform grid: t_i = t0 + i*step
     double inv_step;           // 1 / step

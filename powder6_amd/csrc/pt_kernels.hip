@@ -334,10 +334,11 @@ __device__ inline void trial_setup(const DevPlan &P, SetupShared &sh) {
 // h R and h.t are formed once per representative.  CENTRO: the representatives are one
 // of each (R, t) / (-R, -t) pair, so A = 2 sum cos, B = 0 (and C, D alike).
 // Chunked representative sums: with few reflections per workgroup and many representatives
-// (a cubic phase), the lines stage splits each reflection's representatives into
-// DevPlan::rep_chunks contiguous chunks, sums every (reflection, chunk) on its own thread,
-// and the reflection's thread adds the chunks in order.  kSfSlots floats per (reflection,
-// chunk): cos and sin sums of up to PTK_REP_CHUNK_ATOMS atoms.
+// (a cubic phase), each reflection's representatives are split into DevPlan::rep_chunks
+// contiguous chunks, every chunk is summed from zero and the chunks are added in order.
+// The single-plan lines stage sums the chunks on otherwise idle threads (kSfSlots floats
+// per (reflection, chunk) in LDS: cos and sin sums of up to PTK_REP_CHUNK_ATOMS atoms);
+// every other caller sums them in the reflection's own thread, with the same arithmetic.
 constexpr int kSfSlots = 2 * PTK_REP_CHUNK_ATOMS;
 
 // the representatives [r0, r1) of atoms j0 .. j0 + nj - 1 (nj <= 4): ca[a] += sum cos, sa[a] +=
@@ -370,8 +371,8 @@ __device__ __forceinline__ void sf_rep_sums(const Shared &sh, const DevPhase &ph
     }
 }
 
-// parts (optional): the reflection's representative sums already formed in nch chunks of
-// representatives (sf_chunk_sums), added here chunk by chunk
+// nch > 1: the representative sums in nch chunks; parts (optional) holds them already
+// formed by other threads, else they are formed here
 template <bool CENTRO>
 __device__ __forceinline__ float structure_f2_impl(const DevPlan &P, const Shared &sh, const DevPhase &ph, int h, int k,
                                                    int l, float s2, const float *parts, int nch) {
@@ -388,6 +389,18 @@ __device__ __forceinline__ float structure_f2_impl(const DevPlan &P, const Share
                         ca[a] += parts[c * kSfSlots + 2 * (j0 + a)];
                         if (!CENTRO) sa[a] += parts[c * kSfSlots + 2 * (j0 + a) + 1];
                     }
+        } else if (nch > 1) {
+            const int rpc = (ph.n_reps + nch - 1) / nch;
+            for (int c = 0; c < nch; ++c) {
+                const int r0 = min(ph.n_reps, c * rpc), r1 = min(ph.n_reps, r0 + rpc);
+                float cc[4] = {0.f, 0.f, 0.f, 0.f}, sc[4] = {0.f, 0.f, 0.f, 0.f};
+                sf_rep_sums<CENTRO>(sh, ph, h, k, l, j0, nj, r0, r1, cc, sc);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    ca[a] += cc[a];
+                    if (!CENTRO) sa[a] += sc[a];
+                }
+            }
         } else {
             sf_rep_sums<CENTRO>(sh, ph, h, k, l, j0, nj, 0, ph.n_reps, ca, sa);
         }
@@ -984,9 +997,9 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, St
     for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
     // chunked representative sums (a plan with few reflections and many representatives):
-    // thread p takes chunk p % nch of the workgroup's reflection p / nch
+    // with sfpart, thread p takes chunk p % nch of the workgroup's reflection p / nch
     const int nch = P.rep_chunks;
-    if (nch > 1 && sh.go) {
+    if (nch > 1 && sfpart && sh.go) {
         const int k0 = kb * blockDim.x, nk = min((int)blockDim.x, P.nr - k0);
         for (int p = threadIdx.x; p < nk * nch; p += blockDim.x) {
             const int kc = k0 + p / nch, c = p % nch;
@@ -1015,7 +1028,7 @@ __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, St
             LineRec lr[2] = {{z4, z4, z4}, {z4, z4, z4}};
             int4 ls[2];
             build_reflection<SEP>(P, sh, k, lr, ls, nullptr, &hk, SEP && P.has_free ? P.sp->fspan + base : nullptr,
-                                  nch > 1 ? sfpart + (size_t)threadIdx.x * nch * kSfSlots : nullptr);
+                                  nch > 1 && sfpart ? sfpart + (size_t)threadIdx.x * nch * kSfSlots : nullptr);
             const bool uni = P.uniform != 0;
             for (int q = 0; q < 2; ++q) {
                 const int4 sp = ls[q];
@@ -1054,16 +1067,20 @@ __device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, in
     return t < ntrials;
 }
 
+// CHUNK: the plan chunks its representative sums (DevPlan::rep_chunks > 1); this instance
+// holds the chunk sums in LDS and forms them on otherwise idle threads
+template <bool CHUNK>
 __global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img, StagedLine *rec, int per, int ntrials,
                                                      int t_off) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
-    __shared__ float sfpart[256 * kSfSlots];
+    __shared__ float sfpart[CHUNK ? 256 * kSfSlots : 1];
     int t, u;
     if (!xcd_unit(blockIdx.x, per, ntrials, t, u)) return;
+    float *parts = CHUNK ? sfpart : nullptr;
     // the lean instance for plans with neither separated profiles nor free intensities
-    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, t_off + t, u, sh, gr, sfpart);
-    else lines_body<false>(P, img, rec, t_off + t, u, sh, gr, sfpart);
+    if (P.any_sep || P.has_free) lines_body<true>(P, img, rec, t_off + t, u, sh, gr, parts);
+    else lines_body<false>(P, img, rec, t_off + t, u, sh, gr, parts);
 }
 
 constexpr int kAccWaves = 4;           // wavefronts per accumulate workgroup
@@ -2227,13 +2244,13 @@ __global__ void __launch_bounds__(256) k_group_lines(const DevPlan *__restrict__
                                                      const int4 *__restrict__ tab) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
-    __shared__ float sfpart[256 * kSfSlots];
     const int4 w = tab[blockIdx.x];
     if (w.x < 0) return;
     const DevPlan &P = plans[w.x];
     const DevMember &M = mem[w.x];
-    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr, sfpart);
-    else lines_body<false>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr, sfpart);
+    // a group launch is throughput-bound: chunked sums stay in the reflection's thread
+    if (P.any_sep) lines_body<true>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr, nullptr);
+    else lines_body<false>(P, (TrialImage *)M.img, M.rec, w.y, w.z, sh, gr, nullptr);
 }
 
 template <int TPW>
@@ -2955,8 +2972,12 @@ static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((
 
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, StagedLine *rec, int ntrials, hipStream_t s, int t_off) {
     const int per = (P.nr + 255) / 256;
-    hipLaunchKernelGGL(k_stage_lines, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per, ntrials,
-                       t_off);
+    if (P.rep_chunks > 1)
+        hipLaunchKernelGGL(k_stage_lines<true>, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per,
+                           ntrials, t_off);
+    else
+        hipLaunchKernelGGL(k_stage_lines<false>, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per,
+                           ntrials, t_off);
     return hipGetLastError();
 }
 hipError_t launch_stage_accum(const DevPlan &P, const void *img, const StagedLine *rec, double2 *partial, int ngroups,

@@ -111,7 +111,7 @@ def pt_states(plan, pb, every):
     return [S[c, :pb.nv].copy() for c in pick], [S[c, 3 * pb.nv] for c in pick]
 
 
-def nacl_cpu_path(pt, PR, no_cpu):
+def nacl_cpu_path(pt, PR, no_cpu, sweeps=1000):
     """NaCl Fm-3m, 5501 points, 16 replicas x 1000 sweeps x 12 parameters
     (192,000 trials), the reference's CPU-path case: the whole PT run on the GPU, the
     plain form, and the FP64 CPU restatement on a bounded sample."""
@@ -123,13 +123,14 @@ def nacl_cpu_path(pt, PR, no_cpu):
         pl.sweeps(5)                                       # graph capture, tile tuning
         e0, s0 = pl.trial_counts()
         t = time.perf_counter()
-        pl.sweeps(1000)
+        pl.sweeps(sweeps)
         dt = time.perf_counter() - t
         e1, s1 = pl.trial_counts()
         ev, skipped = e1 - e0, s1 - s0
         xs, gc = pt_states(pl, pb, 16)
         plain, X, chi = plain_rate(pt, pl, pb, 256)
-    out['chosen'] = {'value': ev / dt, 'unit': 'trial patterns/s', 'seconds_for_192000_proposals': dt,
+    out['chosen'] = {'value': ev / dt, 'unit': 'trial patterns/s', 'sweeps': sweeps, 'us_per_step': dt / (sweeps * pb.nv) * 1e6,
+                     'seconds_for_192000_proposals': dt * 1000 / sweeps,
                      'evaluated': ev, 'rejected_unevaluated': skipped}
     out['plain'] = {'value': plain, 'unit': 'trial patterns/s'}
     out['chi2_rel_err_vs_fp64'] = max(chi2_error(pb, xs, gc), chi2_error(pb, X[:16], chi[:16]))
@@ -392,6 +393,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--plain-trials', type=int, default=512)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--nacl-sweeps', type=int, default=1000,
+                    help='sweeps of the NaCl case (fewer for a kernel trace of that leg alone)')
     ap.add_argument('--main-only', action='store_true',
                     help='only the chosen PT form (for counter passes under a profiler)')
     ap.add_argument('--tiles', type=int, default=0, choices=(0, 1, 2, 4),
@@ -409,7 +412,7 @@ def main():
     from powder6_amd import problems as PR
     from powder6_amd import pt
 
-    legs_fn = {'nacl': lambda: nacl_cpu_path(pt, PR, args.no_cpu), 'matrix': lambda: matrix_form(pt, PR),
+    legs_fn = {'nacl': lambda: nacl_cpu_path(pt, PR, args.no_cpu, args.nacl_sweeps), 'matrix': lambda: matrix_form(pt, PR),
                'uneven': lambda: uneven_batch(pt, PR, args.no_cpu), 'learned': lambda: learned_screening(pt, PR, args.no_cpu),
                'free': lambda: free_intensities(pt, PR), 'normal': lambda: normal_equations(pt, PR, args.no_cpu),
                'training': lambda: network_training(pt)}

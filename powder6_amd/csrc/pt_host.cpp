@@ -1823,7 +1823,8 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
                                         "may differ from the start vector.");
     if ((rc = ensure_trials(pl, ntrials))) return rc;
     int chunk = std::min(ntrials, kMatrixChunk);
-    int tpad_max = (chunk + 63) / 64 * 64;                 // whole 64-trial product tiles
+    const int tile = ptk::kMatrixTrialTile;
+    int tpad_max = (chunk + tile - 1) / tile * tile;       // whole product tiles
     if ((rc = ensure_matrix_trials(pl, tpad_max))) return rc;
     if (yout && !pl->d_myout && (rc = dalloc(pl, &pl->d_myout, (size_t)kMatrixYout * pl->np))) return rc;
     int passes = (ntrials + chunk - 1) / chunk;
@@ -1835,7 +1836,7 @@ static int eval_matrix(pt_plan *pl, const double *x, int ntrials, double *chi2, 
     HIPCHECK(hipStreamWaitEvent(pl->mstream, pl->ev_total[0], 0), "fork");
     const size_t bstride = (size_t)pl->cap_mtrials * pl->kpad;
     for (int p = 0; p < passes; ++p) {
-        int t0 = p * chunk, n = std::min(chunk, ntrials - t0), tpad = (n + 63) / 64 * 64;
+        int t0 = p * chunk, n = std::min(chunk, ntrials - t0), tpad = (n + tile - 1) / tile * tile;
         const int bsel = p & 1;
         unsigned short *ihi = pl->d_ihi + bsel * bstride, *ilo = pl->d_ilo + bsel * bstride;
         float *isc = pl->d_iscale + bsel * (size_t)pl->cap_mtrials;

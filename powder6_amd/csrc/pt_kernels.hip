@@ -2761,7 +2761,7 @@ constexpr int kMxLd = kMxK + 8;          // LDS row stride (16-bit elements), pa
 // MFMA operand layout (16x16x32): lane l supplies A[l % 16][8 (l / 16) + e] and
 // B[8 (l / 16) + e][l % 16], e < 8 (16 bytes of consecutive reflections each), and holds
 // D[4 (l / 16) + e][l % 16], e < 4.
-template <bool BF16>
+template <bool BF16, int TT>
 __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned short *__restrict__ phi,
                                                      const unsigned short *__restrict__ plo,
                                                      const unsigned short *__restrict__ ihi,
@@ -2770,16 +2770,17 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
                                                      const float2 *__restrict__ yw, float *partial, int kpad, int tpad,
                                                      int row_tiles, double *yout, int ny) {
     typedef Quad<BF16> Q;
-    __shared__ __align__(16) unsigned short sB[2][2][64][kMxLd];   // [buffer][hi, lo][trial][k]
-    __shared__ float red[4][64];
+    constexpr int NT = 64 * TT;          // trials per workgroup tile
+    __shared__ __align__(16) unsigned short sB[2][2][NT][kMxLd];   // [buffer][hi, lo][trial][k]
+    __shared__ float red[4][NT];
     // XCD-aware tiles: workgroups go to the eight XCDs round robin, and XCD x takes the
     // trial tiles x, x + 8, ..., each with all its row tiles in a row: a trial tile's
     // operand is read from HBM once and stays in that XCD's L2 while its row tiles pass,
     // and P's band (the only part of P any tile reads) stays resident in every XCD's L2
-    const int xcd = blockIdx.x & 7, kx = blockIdx.x >> 3, ntt = tpad / 64;
+    const int xcd = blockIdx.x & 7, kx = blockIdx.x >> 3, ntt = tpad / NT;
     const int ntx = (ntt - xcd + 7) / 8;
     if (kx >= ntx * row_tiles) return;
-    const int rt = kx % row_tiles, j0 = (xcd + 8 * (kx / row_tiles)) * 64;
+    const int rt = kx % row_tiles, j0 = (xcd + 8 * (kx / row_tiles)) * NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i0 = rt * 128 + wave * 32;
     const int2 b = band[rt];
@@ -2789,29 +2790,39 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int e = 0; e < 4; ++e) ow[rb][e] = yw[i0 + 16 * rb + dq + e];
-    floatx4_t acc[2][4];
+    floatx4_t acc[2][4 * TT];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) acc[rb][nb] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
-    // staging: thread t moves trial j0 + t / 4, reflections [8 (t % 4), 8 (t % 4) + 8) of
-    // the slice, hi and lo (16 bytes each)
+        for (int nb = 0; nb < 4 * TT; ++nb) acc[rb][nb] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+    // staging: thread t moves trials j0 + t / 4 (+ 64 u), reflections [8 (t % 4), 8 (t % 4)
+    // + 8) of the slice, hi and lo (16 bytes each)
     const int sc_col = tid >> 2, sc_k = 8 * (tid & 3);
     const size_t src = (size_t)(j0 + sc_col) * kpad + sc_k;
-    uint4 nh = make_uint4(0u, 0u, 0u, 0u), nl = nh;
-    if (b.x < b.y) {
-        nh = *reinterpret_cast<const uint4 *>(ihi + src + b.x);
-        nl = *reinterpret_cast<const uint4 *>(ilo + src + b.x);
+    uint4 nh[TT], nl[TT];
+#pragma unroll
+    for (int u = 0; u < TT; ++u) {
+        nh[u] = make_uint4(0u, 0u, 0u, 0u); nl[u] = nh[u];
+        if (b.x < b.y) {
+            nh[u] = *reinterpret_cast<const uint4 *>(ihi + src + (size_t)64 * u * kpad + b.x);
+            nl[u] = *reinterpret_cast<const uint4 *>(ilo + src + (size_t)64 * u * kpad + b.x);
+        }
     }
     int buf = 0;
     for (int k0 = b.x; k0 < b.y; k0 += kMxK) {
         // the slice k0 into LDS, then the next slice's loads in flight while computing
-        *reinterpret_cast<uint4 *>(&sB[buf][0][sc_col][sc_k]) = nh;
-        *reinterpret_cast<uint4 *>(&sB[buf][1][sc_col][sc_k]) = nl;
+#pragma unroll
+        for (int u = 0; u < TT; ++u) {
+            *reinterpret_cast<uint4 *>(&sB[buf][0][sc_col + 64 * u][sc_k]) = nh[u];
+            *reinterpret_cast<uint4 *>(&sB[buf][1][sc_col + 64 * u][sc_k]) = nl[u];
+        }
         __syncthreads();
         if (k0 + kMxK < b.y) {
-            nh = *reinterpret_cast<const uint4 *>(ihi + src + k0 + kMxK);
-            nl = *reinterpret_cast<const uint4 *>(ilo + src + k0 + kMxK);
+#pragma unroll
+            for (int u = 0; u < TT; ++u) {
+                nh[u] = *reinterpret_cast<const uint4 *>(ihi + src + (size_t)64 * u * kpad + k0 + kMxK);
+                nl[u] = *reinterpret_cast<const uint4 *>(ilo + src + (size_t)64 * u * kpad + k0 + kMxK);
+            }
         }
         {
             // the band ends on a slice boundary (bands are whole 32-reflection slices)
@@ -2823,7 +2834,7 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
                 al[rb] = Q::from(*reinterpret_cast<const uint4 *>(plo + a));
             }
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
+            for (int nb = 0; nb < 4 * TT; ++nb) {
                 const typename Q::T bh = Q::from(*reinterpret_cast<const uint4 *>(&sB[buf][0][16 * nb + li][kq]));
                 const typename Q::T bl = Q::from(*reinterpret_cast<const uint4 *>(&sB[buf][1][16 * nb + li][kq]));
 #pragma unroll
@@ -2838,7 +2849,7 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
     }
     // epilogue: weighted residuals, summed over the 32 rows of this wavefront
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
+    for (int nb = 0; nb < 4 * TT; ++nb) {
         const int col = j0 + 16 * nb + li;
         const float sc = iscale[col] * (1.0f / kPScale);
         float sum = 0.0f;
@@ -2863,9 +2874,9 @@ __global__ void __launch_bounds__(256) k_matrix_chi2(DevPlan P, const unsigned s
         if (lane < 16) red[wave][16 * nb + lane] = sum;
     }
     __syncthreads();
-    if (tid < 64) {
-        const float s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-        partial[(size_t)rt * tpad + j0 + tid] = s;
+    for (int c = tid; c < NT; c += blockDim.x) {
+        const float s = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        partial[(size_t)rt * tpad + j0 + c] = s;
     }
 }
 
@@ -3048,14 +3059,15 @@ hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,
                               float *partial, int kpad, int tpad, int row_tiles, double *yout, int ny, int bf16, hipStream_t s) {
     // row_tiles: 128-point tiles; the grid is eight XCD slots of ceil(trial tiles / 8) trial
-    // tiles x the row tiles
-    const dim3 grid(8 * ((tpad / 64 + 7) / 8) * row_tiles);
+    // tiles x the row tiles; tpad is a multiple of kMatrixTrialTile
+    constexpr int TT = kMatrixTrialTile / 64;
+    const dim3 grid(8 * ((tpad / kMatrixTrialTile + 7) / 8) * row_tiles);
     if (bf16)
-        hipLaunchKernelGGL(k_matrix_chi2<true>, grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad, tpad,
-                           row_tiles, yout, yout ? ny : 0);
+        hipLaunchKernelGGL((k_matrix_chi2<true, TT>), grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad,
+                           tpad, row_tiles, yout, yout ? ny : 0);
     else
-        hipLaunchKernelGGL(k_matrix_chi2<false>, grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad, tpad,
-                           row_tiles, yout, yout ? ny : 0);
+        hipLaunchKernelGGL((k_matrix_chi2<false, TT>), grid, dim3(256), 0, s, P, phi, plo, ihi, ilo, iscale, band, yw, partial, kpad,
+                           tpad, row_tiles, yout, yout ? ny : 0);
     return hipGetLastError();
 }
 hipError_t launch_matrix_reduce(const float *partial, int row_tiles, int tpad, int ntrials, const int *go, double *chi2,

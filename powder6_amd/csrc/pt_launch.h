@@ -46,6 +46,9 @@ hipError_t launch_matrix_tables(const DevPlan &P, const void *img, const float *
 hipError_t launch_matrix_intensity(const DevPlan &P, const double *X, const double4 *hr, const float *f0w, const float2 *qk,
                                    unsigned short *ihi, unsigned short *ilo, float *iscale, int kpad, int n, int bf16,
                                    hipStream_t s);
+// trials per workgroup tile of the matrix product (64 or 128); a pass pads its trials to
+// a multiple of it
+constexpr int kMatrixTrialTile = 128;
 // yout (when not null) receives Y of the first ny trials (ny x np doubles)
 hipError_t launch_matrix_chi2(const DevPlan &P, const unsigned short *phi, const unsigned short *plo, const unsigned short *ihi,
                               const unsigned short *ilo, const float *iscale, const int2 *band, const float2 *yw,

@@ -1387,8 +1387,11 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
     HIPCHECK(ptk::launch_stage_setup(pl->P, nullptr, pl->d_img, n, 1, 0, st, nullptr, t0), "setup");
     int rc = enqueue_follow(pl, st, t0, n);
     if (rc) return rc;
+    // a plan whose reflections fit one lines workgroup runs each step's finish, the next
+    // setup and its lines as one launch (no free intensities, further datasets or skip test)
+    const bool fuse = pl->P.nr <= 256 && !pl->P.has_free && pl->follow.empty() && !pl->P.sf;
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
-        HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, n, st, t0), "lines");
+        if (!fuse || s == 0) HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, n, st, t0), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], st), "timing");
         HIPCHECK(ptk::launch_stage_accum(pl->P, pl->d_img, pl->d_lrec, pl->d_part, pl->ngroups, n, pl->tpw, st, t0),
                  "accumulate");
@@ -1401,6 +1404,10 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
             if (f->P.has_free) HIPCHECK(ptk::launch_stage_nnls(f->P, f->d_img, f->d_lrec, f->d_part, f->ngroups, n, st, t0), "intensities");
         }
         if (s + 1 < pl->steps_per_sweep) {
+            if (fuse) {
+                HIPCHECK(ptk::launch_stage_finish_lines(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->d_lrec, n, st, t0), "finish");
+                continue;
+            }
             HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, st, n, t0), "finish");
             if ((rc = enqueue_follow(pl, st, t0, n))) return rc;
         } else

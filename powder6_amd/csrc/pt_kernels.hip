@@ -986,13 +986,15 @@ __device__ __forceinline__ StagedLine stage_line(const DevPlan &P, const LineRec
 // staged line records of reflections [kb * blockDim.x, ...) of trial t; every line with a
 // window widens the line range of the 512-point groups it meets (reduced in LDS, then one
 // global atomic min / max per group and workgroup)
+// set_up: the trial's setup values are already in sh (the same workgroup formed them), so
+// its image is not read back
 template <bool SEP = true>
 __device__ __forceinline__ void lines_body(const DevPlan &P, TrialImage *img, StagedLine *rec, int t, int kb, Shared &sh,
-                                           int (*gr)[2], float *sfpart) {
+                                           int (*gr)[2], float *sfpart, bool set_up = false) {
     const int k = kb * blockDim.x + threadIdx.x;
     const int4 hk = k < P.nr ? P.hklm[k] : make_int4(0, 0, 0, 0);     // issued before the staging waits
     stage_reps(P, sh);
-    image_load<SEP>(P, img + t, sh);
+    if (!set_up) image_load<SEP>(P, img + t, sh);
     const int ng = (P.np + 511) >> 9;
     for (int g = threadIdx.x; g < ng; g += blockDim.x) { gr[g][0] = 0x7fffffff; gr[g][1] = -1; }
     __syncthreads();
@@ -2197,6 +2199,27 @@ __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage
     finish_setup_body<SEP>(P, img, partial, nsplit, t_off + blockIdx.x, sh, prop, s_acc);
 }
 
+// The finish of one PT step, the setup of the next and its lines stage in one workgroup
+// per chain, for a plan whose reflections fit one lines workgroup (P.nr <= 256), without
+// free intensities, further datasets or the surrogate's skip test: one launch and one
+// round trip of the trial image less per step.  The lines see the setup's values in LDS;
+// the image (which the accumulate stage reads) is still written.  CHUNK as in
+// k_stage_lines.
+template <bool SEP, bool CHUNK>
+__global__ void __launch_bounds__(256) k_stage_finish_lines(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
+                                                            int nsplit, StagedLine *rec, int ntrials, int t_off) {
+    __shared__ Shared sh;
+    __shared__ double prop[PTK_MAX_VARS];
+    __shared__ int s_acc;
+    __shared__ int gr[PTK_MAX_GROUPS][2];
+    __shared__ float sfpart[CHUNK ? 256 * kSfSlots : 1];
+    int t, u;
+    if (!xcd_unit(blockIdx.x, 1, ntrials, t, u)) return;
+    finish_setup_body<SEP>(P, img, partial, nsplit, t_off + t, sh, prop, s_acc);
+    __syncthreads();
+    lines_body<SEP>(P, img, rec, t_off + t, 0, sh, gr, CHUNK ? sfpart : nullptr, true);
+}
+
 // ---- replica exchange (js/pt_gpu.js:365, rietveld_core.js:2229-2236) ------------------
 // pair `pair` of the sweep's parity: chains c = 2 pair + parity and c + 1
 __device__ inline void exchange_body(const DevPlan &P, int pair) {
@@ -3016,6 +3039,19 @@ hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 
                                      int t_off) {
     if (P.any_sep) hipLaunchKernelGGL(k_stage_finish_setup<true>, dim3(ntrials), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit, t_off);
     else hipLaunchKernelGGL(k_stage_finish_setup<false>, dim3(ntrials), dim3(64), 0, s, P, (TrialImage *)img, partial, nsplit, t_off);
+    return hipGetLastError();
+}
+hipError_t launch_stage_finish_lines(const DevPlan &P, void *img, const double2 *partial, int nsplit, StagedLine *rec, int ntrials,
+                                     hipStream_t s, int t_off) {
+    const dim3 grid(xcd_grid(1, ntrials));
+    TrialImage *im = (TrialImage *)img;
+#define PTK_FL(SEP, CH) hipLaunchKernelGGL((k_stage_finish_lines<SEP, CH>), grid, dim3(256), 0, s, P, im, partial, nsplit, rec, ntrials, t_off)
+    if (P.any_sep) {
+        if (P.rep_chunks > 1) PTK_FL(true, true); else PTK_FL(true, false);
+    } else {
+        if (P.rep_chunks > 1) PTK_FL(false, true); else PTK_FL(false, false);
+    }
+#undef PTK_FL
     return hipGetLastError();
 }
 

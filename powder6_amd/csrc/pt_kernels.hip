@@ -1072,7 +1072,7 @@ __device__ __forceinline__ bool xcd_unit(int b, int per, int ntrials, int &t, in
 // CHUNK: the plan chunks its representative sums (DevPlan::rep_chunks > 1); this instance
 // holds the chunk sums in LDS and forms them on otherwise idle threads
 template <bool CHUNK>
-__global__ void __launch_bounds__(256) k_stage_lines(DevPlan P, TrialImage *img, StagedLine *rec, int per, int ntrials,
+__global__ void __launch_bounds__(512) k_stage_lines(DevPlan P, TrialImage *img, StagedLine *rec, int per, int ntrials,
                                                      int t_off) {
     __shared__ Shared sh;
     __shared__ int gr[PTK_MAX_GROUPS][2];
@@ -2200,13 +2200,13 @@ __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage
 }
 
 // The finish of one PT step, the setup of the next and its lines stage in one workgroup
-// per chain, for a plan whose reflections fit one lines workgroup (P.nr <= 256), without
+// per chain, for a plan whose reflections fit one lines workgroup (P.nr <= 512), without
 // free intensities, further datasets or the surrogate's skip test: one launch and one
 // round trip of the trial image less per step.  The lines see the setup's values in LDS;
 // the image (which the accumulate stage reads) is still written.  CHUNK as in
 // k_stage_lines.
 template <bool SEP, bool CHUNK>
-__global__ void __launch_bounds__(256) k_stage_finish_lines(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
+__global__ void __launch_bounds__(512) k_stage_finish_lines(DevPlan P, TrialImage *img, const double2 *__restrict__ partial,
                                                             int nsplit, StagedLine *rec, int ntrials, int t_off) {
     __shared__ Shared sh;
     __shared__ double prop[PTK_MAX_VARS];
@@ -3004,13 +3004,18 @@ hipError_t launch_stage_follow(const DevPlan &P, const double *mstate, int mstri
 }
 static inline unsigned xcd_grid(int per, int ntrials) { return 8u * (unsigned)((ntrials + 7) / 8) * (unsigned)per; }
 
+// threads of a single-plan lines workgroup: 512 for 257-512 reflections (one workgroup per
+// trial, so the finish can join the lines stage), else 256 (larger patterns ran 10% slower
+// in 512-thread workgroups)
+static int lines_threads(const DevPlan &P) { return P.nr > 256 && P.nr <= 512 ? 512 : 256; }
 hipError_t launch_stage_lines(const DevPlan &P, const void *img, StagedLine *rec, int ntrials, hipStream_t s, int t_off) {
-    const int per = (P.nr + 255) / 256;
+    // one workgroup per trial up to 512 reflections, 256-reflection workgroups beyond
+    const int threads = lines_threads(P), per = (P.nr + threads - 1) / threads;
     if (P.rep_chunks > 1)
-        hipLaunchKernelGGL(k_stage_lines<true>, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per,
+        hipLaunchKernelGGL(k_stage_lines<true>, dim3(xcd_grid(per, ntrials)), dim3(threads), 0, s, P, (TrialImage *)img, rec, per,
                            ntrials, t_off);
     else
-        hipLaunchKernelGGL(k_stage_lines<false>, dim3(xcd_grid(per, ntrials)), dim3(256), 0, s, P, (TrialImage *)img, rec, per,
+        hipLaunchKernelGGL(k_stage_lines<false>, dim3(xcd_grid(per, ntrials)), dim3(threads), 0, s, P, (TrialImage *)img, rec, per,
                            ntrials, t_off);
     return hipGetLastError();
 }
@@ -3045,7 +3050,8 @@ hipError_t launch_stage_finish_lines(const DevPlan &P, void *img, const double2 
                                      hipStream_t s, int t_off) {
     const dim3 grid(xcd_grid(1, ntrials));
     TrialImage *im = (TrialImage *)img;
-#define PTK_FL(SEP, CH) hipLaunchKernelGGL((k_stage_finish_lines<SEP, CH>), grid, dim3(256), 0, s, P, im, partial, nsplit, rec, ntrials, t_off)
+    const int threads = lines_threads(P);
+#define PTK_FL(SEP, CH) hipLaunchKernelGGL((k_stage_finish_lines<SEP, CH>), grid, dim3(threads), 0, s, P, im, partial, nsplit, rec, ntrials, t_off)
     if (P.any_sep) {
         if (P.rep_chunks > 1) PTK_FL(true, true); else PTK_FL(true, false);
     } else {

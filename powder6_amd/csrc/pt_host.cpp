@@ -1388,8 +1388,8 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
     int rc = enqueue_follow(pl, st, t0, n);
     if (rc) return rc;
     // a plan whose reflections fit one lines workgroup runs each step's finish, the next
-    // setup and its lines as one launch (no free intensities, further datasets or skip test)
-    const bool fuse = pl->P.nr <= 512 && !pl->P.has_free && pl->follow.empty() && !pl->P.sf;
+    // setup and its lines as one launch (no further datasets or skip test)
+    const bool fuse = pl->P.nr <= 512 && pl->follow.empty() && !pl->P.sf;
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
         if (!fuse || s == 0) HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, n, st, t0), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], st), "timing");

@@ -2201,7 +2201,7 @@ __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage
 
 // The finish of one PT step, the setup of the next and its lines stage in one workgroup
 // per chain, for a plan whose reflections fit one lines workgroup (P.nr <= 512), without
-// free intensities, further datasets or the surrogate's skip test: one launch and one
+// further datasets or the surrogate's skip test: one launch and one
 // round trip of the trial image less per step.  The lines see the setup's values in LDS;
 // the image (which the accumulate stage reads) is still written.  CHUNK as in
 // k_stage_lines.
@@ -3052,7 +3052,9 @@ hipError_t launch_stage_finish_lines(const DevPlan &P, void *img, const double2 
     TrialImage *im = (TrialImage *)img;
     const int threads = lines_threads(P);
 #define PTK_FL(SEP, CH) hipLaunchKernelGGL((k_stage_finish_lines<SEP, CH>), grid, dim3(threads), 0, s, P, im, partial, nsplit, rec, ntrials, t_off)
-    if (P.any_sep) {
+    // the general instance for separated profiles or free intensities (its lines stage
+    // writes the free lines' spans), as k_stage_lines takes it
+    if (P.any_sep || P.has_free) {
         if (P.rep_chunks > 1) PTK_FL(true, true); else PTK_FL(true, false);
     } else {
         if (P.rep_chunks > 1) PTK_FL(false, true); else PTK_FL(false, false);

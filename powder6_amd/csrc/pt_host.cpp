@@ -1388,8 +1388,8 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
     int rc = enqueue_follow(pl, st, t0, n);
     if (rc) return rc;
     // a plan whose reflections fit one lines workgroup runs each step's finish, the next
-    // setup and its lines as one launch (no further datasets or skip test)
-    const bool fuse = pl->P.nr <= 512 && pl->follow.empty() && !pl->P.sf;
+    // setup and its lines as one launch (not with the surrogate's skip test)
+    const bool fuse = pl->P.nr <= 512 && !pl->P.sf;
     for (int s = 0; s < pl->steps_per_sweep; ++s) {
         if (!fuse || s == 0) HIPCHECK(ptk::launch_stage_lines(pl->P, pl->d_img, pl->d_lrec, n, st, t0), "lines");
         if (ev) HIPCHECK(hipEventRecord(ev[2 * s], st), "timing");
@@ -1406,6 +1406,7 @@ static int enqueue_part(pt_plan *pl, hipStream_t st, int t0, int n, hipEvent_t *
         if (s + 1 < pl->steps_per_sweep) {
             if (fuse) {
                 HIPCHECK(ptk::launch_stage_finish_lines(pl->P, pl->d_img, pl->d_part, pl->ngroups, pl->d_lrec, n, st, t0), "finish");
+                if ((rc = enqueue_follow(pl, st, t0, n))) return rc;
                 continue;
             }
             HIPCHECK(ptk::launch_stage_finish_setup(pl->P, pl->d_img, pl->d_part, pl->ngroups, st, n, t0), "finish");

@@ -2201,7 +2201,7 @@ __global__ void __launch_bounds__(64) k_stage_finish_setup(DevPlan P, TrialImage
 
 // The finish of one PT step, the setup of the next and its lines stage in one workgroup
 // per chain, for a plan whose reflections fit one lines workgroup (P.nr <= 512), without
-// further datasets or the surrogate's skip test: one launch and one
+// the surrogate's skip test: one launch and one
 // round trip of the trial image less per step.  The lines see the setup's values in LDS;
 // the image (which the accumulate stage reads) is still written.  CHUNK as in
 // k_stage_lines.

@@ -34,8 +34,8 @@ hipError_t launch_stage_finish(const DevPlan &P, const void *img, const double2 
                                double *chi2, int pt_mode, hipStream_t s, int t_off = 0);
 hipError_t launch_stage_finish_setup(const DevPlan &P, void *img, const double2 *partial, int nsplit, hipStream_t s, int ntrials,
                                      int t_off = 0);
-// finish, next setup and lines in one launch (plans with P.nr <= 512, no further datasets,
-// no surrogate skip test)
+// finish, next setup and lines in one launch (plans with P.nr <= 512, no surrogate skip
+// test)
 hipError_t launch_stage_finish_lines(const DevPlan &P, void *img, const double2 *partial, int nsplit, StagedLine *rec, int ntrials,
                                      hipStream_t s, int t_off = 0);
 hipError_t launch_matrix_columns(const DevPlan &P, const void *img, LineRec *rec, int4 *span, float *factor, hipStream_t s);

@@ -1538,9 +1538,50 @@ __device__ __forceinline__ double uniform_value(double v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)u), hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(u >> 32));
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+// Wavefront reductions without LDS round trips: four DPP row rotations (row_ror 8, 4, 2, 1)
+// leave every lane with a combination of all 16 lanes of its row, then lanes 0, 16, 32 and
+// 48 are combined in that order.  The result is the same in every lane and depends only on
+// the inputs.  All 64 lanes must be active.
+template <int N>
+__device__ __forceinline__ double row_ror(double v) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x120 + N, 0xf, 0xf, false),
+                   hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(u >> 32), 0x120 + N, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int N>
+__device__ __forceinline__ int row_ror(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x120 + N, 0xf, 0xf, false); }
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += row_ror<8>(v);
+    v += row_ror<4>(v);
+    v += row_ror<2>(v);
+    v += row_ror<1>(v);
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+__device__ __forceinline__ double wave_min(double v) {
+    v = fmin(v, row_ror<8>(v));
+    v = fmin(v, row_ror<4>(v));
+    v = fmin(v, row_ror<2>(v));
+    v = fmin(v, row_ror<1>(v));
+    return fmin(fmin(lane_value(v, 0), lane_value(v, 16)), fmin(lane_value(v, 32), lane_value(v, 48)));
+}
+// the index of the largest value, the smallest index on ties (an exact total order, so the
+// combination order does not matter)
+__device__ __forceinline__ int wave_argmax(double bv, int bi) {
+    auto take = [&](double ov, int oi) { if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; } };
+    take(row_ror<8>(bv), row_ror<8>(bi));
+    take(row_ror<4>(bv), row_ror<4>(bi));
+    take(row_ror<2>(bv), row_ror<2>(bi));
+    take(row_ror<1>(bv), row_ror<1>(bi));
+    double rv = lane_value(bv, 0);
+    int ri = __builtin_amdgcn_readlane(bi, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const double ov = lane_value(bv, r);
+        const int oi = __builtin_amdgcn_readlane(bi, r);
+        if (ov > rv || (ov == rv && oi < ri)) { rv = ov; ri = oi; }
+    }
+    return ri;
 }
 // LDS written by some lanes of the wavefront, then read by others: the writes complete
 // (s_waitcnt lgkmcnt(0)) before the wavefront goes on, and no access moves across
@@ -1778,12 +1819,7 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
 #pragma unroll
             for (int s = 0; s < S; ++s)
                 if (act[s] && !pj[s] && !kj[s] && gj[s] > tol && gj[s] > bv) { bv = gj[s]; bi = ln + 64 * s; }
-            for (int o = 32; o > 0; o >>= 1) {
-                const double ov = __shfl_xor(bv, o);
-                const int oi = __shfl_xor(bi, o);
-                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-            }
-            jm = __builtin_amdgcn_readfirstlane(bi);
+            jm = __builtin_amdgcn_readfirstlane(wave_argmax(bv, bi));
             if (jm >= m) { conv = 1; break; }
 #pragma unroll
             for (int s = 0; s < S; ++s) if (ln + 64 * s == jm) pj[s] = true;
@@ -1818,8 +1854,7 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
             double al = 2.0;
 #pragma unroll
             for (int s = 0; s < S; ++s) if (bad[s]) al = fmin(al, xj[s] / (xj[s] - z[s]));
-            for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o));
-            const double alpha = uniform_value(al);
+            const double alpha = uniform_value(wave_min(al));
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 if (pj[s]) xj[s] += alpha * (z[s] - xj[s]);
@@ -1925,7 +1960,7 @@ __device__ __forceinline__ void nnls_core(const DevPlan &P, const StagedLine *__
                     const float w = pts ? wl[i] : P.obs[i].y;
                     if (w > 0.0f) a += (double)w * (double)column(j, i) * (double)(rhs ? (pts ? rl[i] : r[i]) : column(k, i));
                 }
-                for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+                a = wave_sum(a);
             }
             if (ln == 0) {
                 if (rhs) b[j] = a;

@@ -1728,12 +1728,67 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
         wave_lds_sync();
         return true;
     };
-    // the factor of the passive list, formed again a row at a time from position r0 (the
-    // rows above r0 depend only on the passive columns above r0)
+    // the factor of the passive list, formed again from row r0 (the rows above r0 depend only
+    // on the passive columns above r0).  Column by column (Crout), every row below advancing
+    // at once: entry (i, k) = (A_ik - L_i0 L_k0 - L_i1 L_k1 - ... - L_i,k-1 L_k,k-1) / L_kk,
+    // subtracted in that order and multiplied by 1 / L_kk, and L_kk^2 = A_kk minus the
+    // wavefront sum of row k's squares: the operations of append's forward substitution for
+    // row i, in the same order, so the factor is the one appending row after row gives.  The
+    // first column whose pivot is not positive is the row at which append would stop.
     auto factor = [&](int p, int r0) -> bool {
-        for (int r = r0; r < p; ++r) {
-            const int q = __builtin_amdgcn_readfirstlane(plist[r]);
-            if (!append(q, r)) return false;
+        if (r0 >= p) return true;
+        int qv[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) { const int ps = ln + 64 * s; qv[s] = ps < p ? plist[ps] : 0; }
+        for (int k = 0; k < p; ++k) {
+            int qk = __builtin_amdgcn_readlane(qv[0], k & 63);
+#pragma unroll
+            for (int s = 1; s < S; ++s) if ((k >> 6) == s) qk = __builtin_amdgcn_readlane(qv[s], k & 63);
+            double acc[S];
+            bool row[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int ps = ln + 64 * s;
+                row[s] = ps > k && ps >= r0 && ps < p;
+                acc[s] = row[s] ? H[tri_at(qv[s], qk)] : 0.0;
+            }
+            double dk;
+            if (k >= r0) {                               // the pivot: row k is complete
+                double ss = 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int ps = ln + 64 * s;
+                    if (ps < k) { const double l = L[tri_row(k) + ps]; ss += l * l; }
+                }
+                const double d = uniform_value(H[tri_at(qk, qk)] - wave_sum(ss));
+                if (!(d > 0.0)) return false;
+                const double sd = sqrt(d);
+                dk = 1.0 / sd;
+#pragma unroll
+                for (int s = 0; s < S; ++s) if (ln + 64 * s == k) dinv[s] = dk;
+                if (ln == 0) L[tri_row(k) + k] = sd;
+            } else {
+                dk = at(dinv, k);
+            }
+            for (int c0 = 0; c0 < k; c0 += B) {
+                double lk[B], li[B][S];
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const int c = c0 + u;
+                    lk[u] = c < k ? L[tri_row(k) + c] : 0.0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) li[u][s] = row[s] && c < k ? L[tri_row(ln + 64 * s) + c] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (c0 + u < k) {
+#pragma unroll
+                        for (int s = 0; s < S; ++s) if (row[s]) acc[s] = acc[s] - li[u][s] * lk[u];
+                    }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) if (row[s]) L[tri_row(ln + 64 * s) + k] = acc[s] * dk;
+            wave_lds_sync();
         }
         return true;
     };

@@ -1603,8 +1603,9 @@ __device__ __forceinline__ int tri_row(int i) { return i * (i + 1) / 2; }
 // FMA chain is sequential.  The steps are those of the workgroup form: the largest
 // gradient entry above tol enters (the first on ties), the step back to the boundary when
 // the passive solution is not positive, a column that cannot stay is blocked; a warm start
-// more than a quarter of whose passive set comes out not positive, or that needs a third
-// step back, gives way to x = 0 (the optimum is the same, reached in fewer steps).
+// more than a quarter of whose passive set comes out not positive, or, with more than 64
+// passive variables, that needs a third step back, gives way to x = 0 (the optimum is the
+// same, reached in fewer steps; small passive sets step back faster than they start again).
 // In: x (warm start), pas, blk; out: x, pas, s_int[2] = a block was not positive definite,
 // s_int[3] = converged.  The arrays are in LDS, or for more columns than it holds in the
 // trial's global workspace; either way only this wavefront touches them here, and its
@@ -1899,7 +1900,7 @@ __device__ __forceinline__ void nnls_wave(const double *H, const double *b, doub
                 settled = true;
                 break;
             }
-            if (jm == -2 && (inner >= 2 || 4 * nbad > p)) {
+            if (jm == -2 && ((inner >= 2 && p > 64) || 4 * nbad > p)) {
 #pragma unroll
                 for (int s = 0; s < S; ++s) { pj[s] = false; xj[s] = 0.0; }
                 p = 0;
